@@ -1,0 +1,192 @@
+/*
+ * opaw_hip.h -- C ABI of the MI355X (gfx950) OPAW-TDDFT propagation hot path.
+ *
+ * The reference (Fortran 90 + MPI) keeps the hot path behind external
+ * subroutines that act on module state (main_mod, atom_mod, tddft_mod).  This
+ * ABI keeps that shape: an opaque context owns the device-resident copy of the
+ * module state (grid, kinetic table, projector boxes, potentials, D_ij, the
+ * propagated orbitals) and every entry point names the reference subroutine
+ * it replaces.  Plain C types only; arrays are passed in the reference's
+ * Fortran layouts (column-major, x fastest, complex interleaved).
+ *
+ * Conventions
+ *   - every function returns 0 (OPAW_OK) or a negative status; the message of
+ *     the last failure on the calling thread is in opaw_last_error().  The
+ *     reference aborts with `stop` instead (e.g. src/hphi.f90:38).
+ *   - atom indices `ia` and type indices `it` are 1-based, like the reference.
+ *   - "device" buffers are HIP device pointers holding nbatch complex fields of
+ *     nx*ny*nz points each: complex*16 in an OPAW_PREC_F64 context,
+ *     complex*8 in an OPAW_PREC_MIXED context.
+ *   - all work is enqueued on the context stream (opaw_set_stream); results in
+ *     caller buffers are ready after opaw_synchronize().
+ *   - host arrays passed to set/read functions are copied synchronously.
+ */
+#ifndef OPAW_HIP_H
+#define OPAW_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct opaw_ctx opaw_ctx;
+
+enum {
+  OPAW_OK = 0,
+  OPAW_ERR_ARG = -1,         /* bad argument / shape (reference: stop) */
+  OPAW_ERR_HIP = -2,         /* HIP runtime error */
+  OPAW_ERR_ALLOC = -3,       /* device allocation failed */
+  OPAW_ERR_STATE = -4,       /* tables / potentials not set yet */
+  OPAW_ERR_UNSUPPORTED = -5  /* grid size without a compiled FFT length */
+};
+
+enum { OPAW_PREC_F64 = 0, OPAW_PREC_MIXED = 1 };
+
+/* fields readable with opaw_read_field */
+enum { OPAW_FIELD_DENS = 0, OPAW_FIELD_NHAT = 1, OPAW_FIELD_VXC = 2, OPAW_FIELD_VH = 3, OPAW_FIELD_VKS = 4 };
+
+/* kernel families timed by the profiler (opaw_profile_read) */
+enum {
+  OPAW_K_FFT_X = 0,      /* kinetic: x forward line FFT                      */
+  OPAW_K_FFT_Y = 1,      /* kinetic: y forward / inverse line FFTs          */
+  OPAW_K_FFT_Z = 2,      /* kinetic: z line FFT, * ek, inverse (fused)      */
+  OPAW_K_FFT_XH = 3,     /* kinetic: x inverse + vks*psi epilogue           */
+  OPAW_K_HARTREE = 4,    /* Martyna-Tuckerman Hartree FFT passes            */
+  OPAW_K_BOX_PROJ = 5,   /* projector contractions P^T psi                 */
+  OPAW_K_BOX_ADD = 6,    /* projector add-backs P w                        */
+  OPAW_K_RK4 = 7,        /* RK4 stage updates                               */
+  OPAW_K_DENS = 8,       /* density, rho_ij, nhat, norms, dipoles           */
+  OPAW_K_POT = 9,        /* LDA, V_KS, D_ij, packing                        */
+  OPAW_K_OTHER = 10,     /* plain fft3d and everything else                 */
+  OPAW_K_COUNT = 11
+};
+
+const char* opaw_last_error(void);
+int opaw_version(void);
+int opaw_fft_length_supported(int n);
+
+/* ---- context: grid and orbital arrays ------------------------------------
+ * Replaces the grid part of main_mod (src/4_main.f90:5-50) and the orbital
+ * arrays phi_bar / phi_bar_pert of tddft_mod (src/5_tddft.f90, allocated in
+ * src/tddft_alloc.f90:15-16).  nsys systems (the reference propagates 2:
+ * unperturbed and kicked) of nb occupied orbitals each.  periodic != 0 is the
+ * reference `periodic` flag (Hartree scale_vh = 1, else 2: src/main.f90:351). */
+int opaw_create(opaw_ctx** out, int nx, int ny, int nz, double dx, double dy, double dz, int periodic,
+                int precision, int nsys, int nb);
+int opaw_destroy(opaw_ctx* ctx);
+int opaw_set_stream(opaw_ctx* ctx, void* hip_stream);
+int opaw_synchronize(opaw_ctx* ctx);
+long long opaw_device_bytes(opaw_ctx* ctx);
+
+/* ---- kinetic table -------------------------------------------------------
+ * ek_prep (src/ek.f90:1-51): ek = min(|k|^2/2, ekcut)/N with k folded into
+ * (-pi/d, pi/d]; computed on the device.  opaw_set_ek uploads a caller table
+ * ek(nx,ny,nz) already divided by N. */
+int opaw_ek_prep(opaw_ctx* ctx, double ekcut);
+int opaw_set_ek(opaw_ctx* ctx, const double* ek);
+
+/* ---- atoms and PAW tables (atom_mod / paw_mod, src/3_atom.f90, src/2_paw.f90)
+ * atom_map(natom) 1-based types; ir_start(3,natom) 1-based box origins as
+ * from get_irs (src/get_irs.f90:22-41; a box outside the grid is an error
+ * unless periodic, where it wraps); nrough(3,ntype); mstates(ntype);
+ * nl2 = (2*lmax+1)^2 compensation channels. */
+int opaw_set_atoms(opaw_ctx* ctx, int natom, const int* atom_map, const int* ir_start, int ntype,
+                   const int* nrough, const int* mstates, int nl2);
+/* local_p3d(nr1,nr2,nr3,ms) and local_p3d1(...) (real at the gamma point:
+ * src/p3d.f90:93-103 stores them as complex with zero phase), s(ms), sinv(ms)
+ * from transform_p3d (src/transform_p3d.f90:46-56). */
+int opaw_set_projectors(opaw_ctx* ctx, int ia, const double* local_p3d, const double* local_p3d1,
+                        const double* s, const double* sinv);
+/* compensation sphere: local_grid(ngrid,3) 1-based, local_g3d(ngrid,nl2)
+ * (src/3_atom.f90:24-26, src/g3d.f90). */
+int opaw_set_sphere(opaw_ctx* ctx, int ia, int ngrid, const int* local_grid, const double* local_g3d);
+/* per-type qijlm(nl2,ms,ms) (src/2_paw.f90), and the on-site D_ij model that
+ * stands in for libpaw pawdenpot/pawdij: dij0(ms,ms) + eijkl(ms,ms,ms,ms).rho */
+int opaw_set_type(opaw_ctx* ctx, int it, const double* qijlm, const double* dij0, const double* eijkl);
+/* builds the plane lists and uploads everything; call after the set_* above */
+int opaw_finalize_tables(opaw_ctx* ctx);
+
+/* ---- potentials -----------------------------------------------------------
+ * vloc_tot and ncoret (src/vloc.f90), rnel (src/rnel.f90); usexcnhat selects
+ * the D^hat potential (0: vks - vxc, as libpaw pawdij with usexcnhat=0). */
+int opaw_set_potentials(opaw_ctx* ctx, const double* vloc_tot, const double* ncoret, double rnel,
+                        int usexcnhat);
+/* Martyna-Tuckerman (or periodic 4pi/k^2) Hartree kernel on the
+ * scale_vh-times box: vk_prep (src/vk_scale.f90:1-48, src/vk_TuMa.f90:2-117),
+ * computed on the device. */
+int opaw_vk_prep(opaw_ctx* ctx);
+
+/* ---- per-system state (vks / vks_pert, dij / dij_pert, rhoij, ...) ------- */
+int opaw_write_vks(opaw_ctx* ctx, int sys, const double* vks);
+int opaw_read_vks(opaw_ctx* ctx, int sys, double* vks);
+int opaw_write_dij(opaw_ctx* ctx, int sys, int ia, const double* dij);
+int opaw_read_dij(opaw_ctx* ctx, int sys, int ia, double* dij);
+int opaw_read_rhoij(opaw_ctx* ctx, int sys, int ia, double* rhoij);
+int opaw_read_field(opaw_ctx* ctx, int which, int sys, double* out);
+
+/* ---- resident orbitals: phi_bar(:,:,:,1:nb) of system sys ---------------- */
+int opaw_write_orbitals(opaw_ctx* ctx, int sys, const void* host);
+int opaw_read_orbitals(opaw_ctx* ctx, int sys, void* host);
+void* opaw_orbitals_device(opaw_ctx* ctx, int sys);
+
+/* ---- operators on caller device buffers ----------------------------------
+ * h_phi / h_phi_pert (src/hphi.f90:1-66, src/hphi_pert.f90): hp = T pin +
+ * vks(sys) pin + sum_a p_a D_a(sys) <p_a|pin> dv, for nbatch fields. */
+int opaw_h_phi(opaw_ctx* ctx, int sys, const void* pin, void* hp, int nbatch);
+/* sn_phi (src/snphi.f90:1-49): sp = pin + sum_a p1_a ((1+s)^n - 1) <p1_a|pin> dv */
+int opaw_sn_phi(opaw_ctx* ctx, const void* pin, void* sp, int nbatch, double n);
+/* shs / shs_pert (src/shs.f90:3-70): pout = S^-1/2 H(sys) S^-1/2 pin */
+int opaw_shs(opaw_ctx* ctx, int sys, const void* pin, void* pout, int nbatch);
+/* proj / proj1 (src/proj.f90, src/proj1.f90) for all atoms: ca is a device
+ * complex*16 array (nbatch, sum_a ms_a), atom a at offset sum_{b<a} ms_b. */
+int opaw_proj(opaw_ctx* ctx, const void* pin, int nbatch, int which_p1, void* ca);
+/* fft3d_forward (dir=-1) / fft3d_backward (dir=+1), unnormalised
+ * (src/lib/library_fftw_v3.f90:56-83) */
+int opaw_fft3d(opaw_ctx* ctx, int dir, const void* in, void* out, int nbatch);
+
+/* ---- per-stage Hamiltonian rebuild (src/ncpaw_make_hamiltonian.f90) ------
+ * psi: device fields of system sys (nb of them), already S^-1/2 transformed
+ * where the reference transforms them.  Each call updates the resident
+ * per-system state exactly like the module-state routine it names. */
+int opaw_tddft_rhoij_2(opaw_ctx* ctx, int sys, const void* psi);  /* src/tddft_rhoij.f90:75-115 */
+int opaw_tddft_dens_2(opaw_ctx* ctx, int sys, const void* psi);   /* src/tddft_dens.f90:44-85 */
+int opaw_get_nhat(opaw_ctx* ctx, int sys);                        /* src/nhat.f90:1-43 */
+int opaw_get_pot_tddft_2(opaw_ctx* ctx, int sys);                 /* src/tddft_pot.f90:110-181 */
+int opaw_get_dij(opaw_ctx* ctx, int sys);                         /* src/get_dij.f90:1-156 */
+/* ncpaw_make_hamiltonian_tddft (all systems, :118-167) when sys < 0, else
+ * the _unpert / _pert variant (:213-281) for that system; acts on the
+ * resident orbitals. */
+int opaw_make_hamiltonian_tddft(opaw_ctx* ctx, int sys);
+
+/* ---- time propagation ------------------------------------------------------ */
+/* excite_psi_bar_pert (src/tddft_excite.f90:1-16): orbitals(to) =
+ * orbitals(from) * exp(-i sm r_ipol), ipol 1..3 */
+int opaw_excite_psi_bar_pert(opaw_ctx* ctx, int from_sys, int to_sys, double sm, int ipol);
+/* rk4_prop_shs (src/rk4.f90:1-103): one RK4 step of every system with the
+ * per-stage self-consistent rebuild and the per-orbital norm restore. */
+int opaw_rk4_prop_shs(opaw_ctx* ctx, double dt);
+/* plot_dip (src/tddft_plot.f90:1-155): dipoles of systems 0 and 1 and
+ * d_dip = (dip_pert - dip)/sm; any output pointer may be NULL. */
+int opaw_plot_dip(opaw_ctx* ctx, double sm, double* d_dip, double* dip, double* dip_pert);
+/* the time loop of opaw_tddft (src/tddft.f90:67-91, dyn=3): per step the
+ * rebuild of all systems, rk4_prop_shs, and the dipole; d_dip (nt,3) and
+ * dip (nt, 3*nsys) may be NULL.  No host synchronisation inside the loop. */
+int opaw_tddft_run(opaw_ctx* ctx, int nt, double dt, double sm, double* d_dip, double* dip);
+
+/* ---- orthogonalisation / subspace diagonalisation ------------------------- */
+/* orthog_phi (src/orthog_phi.f90:1-194): canonical phi U lambda^-1/2 of the
+ * resident orbitals of system sys; h_type 1: S = 1, h_type 0: S via sn_phi(1).
+ * eig (nb) receives the overlap eigenvalues (ascending). */
+int opaw_orthog_phi(opaw_ctx* ctx, int sys, int h_type, double* eig);
+/* diagh (src/diag.f90:1-180): Rayleigh-Ritz on shs (h_type 1) or h_phi
+ * (h_type 0); orbitals rotated in place, eig (nb) ascending. */
+int opaw_diagh(opaw_ctx* ctx, int sys, int h_type, double* eig);
+
+/* ---- profiling: HIP events around every launch of a kernel family ------- */
+int opaw_profile_enable(opaw_ctx* ctx, int on);
+int opaw_profile_reset(opaw_ctx* ctx);
+int opaw_profile_read(opaw_ctx* ctx, int kclass, double* total_ms, long long* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OPAW_HIP_H */

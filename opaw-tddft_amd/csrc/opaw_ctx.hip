@@ -1,0 +1,1420 @@
+// C ABI of the OPAW-TDDFT propagation hot path: context, resident state and
+// the orchestration of the kernels (one HIP stream, no host synchronisation
+// inside a time step).
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "opaw_fields.h"
+#include "opaw_hip.h"
+
+using namespace opaw;
+
+// ---------------------------------------------------------------------------
+// errors
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HCHK(expr)                                                                       \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) return fail(OPAW_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+#define RCHK(expr)            \
+  do {                        \
+    int r_ = (expr);          \
+    if (r_ != OPAW_OK) return r_; \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// profiler: HIP event pairs around launches of each kernel family
+struct Prof {
+  bool on = false;
+  struct Rec {
+    int cls;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  int open_cls = -1;
+  double total[OPAW_K_COUNT] = {0};
+  long long count[OPAW_K_COUNT] = {0};
+
+  hipEvent_t take() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+  void harvest() {
+    for (auto& r : recs) {
+      float ms = 0.f;
+      if (r.a && r.b && hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+        total[r.cls] += ms;
+        count[r.cls] += 1;
+      }
+      if (r.a) pool.push_back(r.a);
+      if (r.b) pool.push_back(r.b);
+    }
+    recs.clear();
+  }
+  void destroy() {
+    harvest();
+    for (auto e : pool) (void)hipEventDestroy(e);
+    pool.clear();
+  }
+};
+
+static thread_local Prof* g_prof = nullptr;
+
+namespace opaw {
+void prof_begin(hipStream_t st, int cls) {
+  Prof* p = g_prof;
+  if (!p || !p->on) return;
+  if (p->recs.size() > 20000) p->harvest();
+  Prof::Rec r{cls, p->take(), nullptr};
+  if (r.a) (void)hipEventRecord(r.a, st);
+  p->recs.push_back(r);
+}
+void prof_end(hipStream_t st, int cls) {
+  Prof* p = g_prof;
+  if (!p || !p->on || p->recs.empty()) return;
+  Prof::Rec& r = p->recs.back();
+  if (r.cls != cls || r.b) return;
+  r.b = p->take();
+  if (r.b) (void)hipEventRecord(r.b, st);
+}
+}  // namespace opaw
+
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct opaw_ctx {
+  int nx = 0, ny = 0, nz = 0;
+  double dx = 0, dy = 0, dz = 0;
+  int periodic = 0, prec = 0, nsys = 0, nb = 0;
+  long long N = 0;
+  int scale = 2;
+  GridGeom g{};
+  GridGeom gbig{};
+  hipStream_t st = nullptr;
+  Prof prof;
+  std::vector<void*> allocs;
+  long long bytes = 0;
+
+  std::map<int, c64*> tw64;
+  std::map<int, c32*> tw32;
+
+  double* ek = nullptr;
+  float* ekf = nullptr;
+  bool ek_ready = false;
+
+  // host staging of the tables
+  int natom = 0, ntype = 0, nl2 = 0, ctot = 0, mtot = 0, sphtot = 0;
+  std::vector<int> atom_type, type_ms, type_nr, atom_start;
+  std::vector<std::vector<double>> hp, hp1, hs, hsinv;
+  std::vector<std::vector<int>> hsph_idx;
+  std::vector<std::vector<double>> hsph_g;
+  std::vector<std::vector<double>> hq, hd0, he;
+  bool tables_ready = false;
+  std::vector<AtomDesc> hatoms;
+
+  // device tables
+  AtomDesc* datoms = nullptr;
+  double *P = nullptr, *P1 = nullptr;
+  double* sn[4] = {nullptr, nullptr, nullptr, nullptr};  // n = -1, -1/2, +1/2, +1
+  double* sn_custom = nullptr;
+  int* plane_ptr = nullptr;
+  PlaneEntry* plane = nullptr;
+  int* sph_idx = nullptr;
+  double* sph_g = nullptr;
+  double *qtab = nullptr, *d0tab = nullptr, *etab = nullptr;
+
+  // potentials
+  double *vloc = nullptr, *ncoret = nullptr;
+  double rnel = 0;
+  int usexcnhat = 0;
+  bool pot_ready = false;
+  double* vk = nullptr;
+  bool vk_ready = false;
+
+  // per-system state
+  double *vks = nullptr, *dij = nullptr, *rhoij = nullptr, *dens = nullptr, *nhat = nullptr, *vxc = nullptr,
+         *vh = nullptr;
+  float* vksf = nullptr;
+  double* fac = nullptr;
+  double* nrm0 = nullptr;
+  double* nrm1 = nullptr;
+  double* means = nullptr;
+  double* part = nullptr;
+  double* dipbuf = nullptr;
+  int dipcap = 0;
+
+  // orbitals + work
+  void* phi = nullptr;
+  DevBuf wy, wacc, wh, wu, wtmp, wca, wca2;
+  c64* pair = nullptr;
+  c64* hwork = nullptr;
+  c64* gpart = nullptr;
+  c64* gmat = nullptr;
+  c64* tmat = nullptr;
+
+  size_t esize() const { return prec == OPAW_PREC_F64 ? sizeof(c64) : sizeof(c32); }
+};
+
+static int dalloc(opaw_ctx* c, void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return fail(OPAW_ERR_ALLOC, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  c->allocs.push_back(*p);
+  c->bytes += (long long)bytes;
+  return OPAW_OK;
+}
+
+template <typename T>
+static int dalloc_t(opaw_ctx* c, T** p, size_t count) {
+  void* v = nullptr;
+  int r = dalloc(c, &v, count * sizeof(T));
+  *p = (T*)v;
+  return r;
+}
+
+static void dfree(opaw_ctx* c, void* p) {
+  if (!p) return;
+  auto it = std::find(c->allocs.begin(), c->allocs.end(), p);
+  if (it != c->allocs.end()) c->allocs.erase(it);
+  (void)hipFree(p);
+}
+
+static int ensure(opaw_ctx* c, DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes) return OPAW_OK;
+  dfree(c, b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+  RCHK(dalloc(c, &b.p, bytes));
+  b.bytes = bytes;
+  return OPAW_OK;
+}
+
+struct ProfScope {
+  Prof* prev;
+  explicit ProfScope(opaw_ctx* c) : prev(g_prof) { g_prof = &c->prof; }
+  ~ProfScope() { g_prof = prev; }
+};
+
+#define CTX_CHECK(c) \
+  if (!(c)) return fail(OPAW_ERR_ARG, "null context"); \
+  ProfScope prof_scope_(c)
+
+// ---------------------------------------------------------------------------
+// twiddles exp(-2 pi i m / n)
+static int twiddles(opaw_ctx* c, int n) {
+  if (c->tw64.count(n)) return OPAW_OK;
+  std::vector<c64> h64(n);
+  std::vector<c32> h32(n);
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int m = 0; m < n; ++m) {
+    long double a = -2.0L * pi * (long double)m / (long double)n;
+    h64[m] = mk<double>((double)cosl(a), (double)sinl(a));
+    h32[m] = mk<float>((float)cosl(a), (float)sinl(a));
+  }
+  c64* d64;
+  c32* d32;
+  RCHK(dalloc_t(c, &d64, n));
+  RCHK(dalloc_t(c, &d32, n));
+  HCHK(hipMemcpy(d64, h64.data(), n * sizeof(c64), hipMemcpyHostToDevice));
+  HCHK(hipMemcpy(d32, h32.data(), n * sizeof(c32), hipMemcpyHostToDevice));
+  c->tw64[n] = d64;
+  c->tw32[n] = d32;
+  return OPAW_OK;
+}
+
+template <typename R>
+struct TW;
+template <>
+struct TW<double> {
+  static const c64* get(opaw_ctx* c, int n) { return c->tw64[n]; }
+};
+template <>
+struct TW<float> {
+  static const c32* get(opaw_ctx* c, int n) { return c->tw32[n]; }
+};
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* opaw_last_error(void) { return g_err.c_str(); }
+int opaw_version(void) { return 10000; }
+int opaw_fft_length_supported(int n) { return fft_size_supported(n) ? 1 : 0; }
+
+int opaw_create(opaw_ctx** out, int nx, int ny, int nz, double dx, double dy, double dz, int periodic,
+                int precision, int nsys, int nb) {
+  if (!out) return fail(OPAW_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  if (nx <= 0 || ny <= 0 || nz <= 0 || dx <= 0 || dy <= 0 || dz <= 0)
+    return fail(OPAW_ERR_ARG, "invalid grid %d %d %d / %g %g %g", nx, ny, nz, dx, dy, dz);
+  if (nsys <= 0 || nb <= 0) return fail(OPAW_ERR_ARG, "nsys and nb must be positive");
+  if (precision != OPAW_PREC_F64 && precision != OPAW_PREC_MIXED) return fail(OPAW_ERR_ARG, "bad precision");
+  const int scale = periodic ? 1 : 2;
+  for (int n : {nx, ny, nz, scale * nx, scale * ny, scale * nz})
+    if (!fft_size_supported(n)) return fail(OPAW_ERR_UNSUPPORTED, "no compiled FFT length %d", n);
+  if (scale == 2 && (nx % 2 || ny % 2 || nz % 2))
+    return fail(OPAW_ERR_ARG, "Martyna-Tuckerman doubling needs even grid dimensions");
+  opaw_ctx* c = new opaw_ctx();
+  c->nx = nx;
+  c->ny = ny;
+  c->nz = nz;
+  c->dx = dx;
+  c->dy = dy;
+  c->dz = dz;
+  c->periodic = periodic ? 1 : 0;
+  c->prec = precision;
+  c->nsys = nsys;
+  c->nb = nb;
+  c->N = (long long)nx * ny * nz;
+  c->scale = scale;
+  c->g = GridGeom{nx, ny, nz};
+  c->gbig = GridGeom{scale * nx, scale * ny, scale * nz};
+  int r = OPAW_OK;
+  for (int n : {nx, ny, nz, scale * nx, scale * ny, scale * nz})
+    if (r == OPAW_OK) r = twiddles(c, n);
+  const long long N = c->N;
+  const size_t nf = (size_t)nsys * nb;
+  if (r == OPAW_OK) r = dalloc_t(c, &c->ek, N);
+  if (r == OPAW_OK) r = dalloc_t(c, &c->ekf, N);
+  if (r == OPAW_OK) r = dalloc_t(c, &c->vks, nsys * N);
+  if (r == OPAW_OK) r = dalloc_t(c, &c->vksf, nsys * N);
+  if (r == OPAW_OK) r = dalloc_t(c, &c->dens, nsys * N);
+  if (r == OPAW_OK) r = dalloc_t(c, &c->nhat, nsys * N);
+  if (r == OPAW_OK) r = dalloc_t(c, &c->vxc, nsys * N);
+  if (r == OPAW_OK) r = dalloc_t(c, &c->vh, nsys * N);
+  if (r == OPAW_OK) r = dalloc_t(c, &c->vloc, N);
+  if (r == OPAW_OK) r = dalloc_t(c, &c->ncoret, N);
+  if (r == OPAW_OK) r = dalloc_t(c, &c->fac, nsys);
+  if (r == OPAW_OK) r = dalloc_t(c, &c->nrm0, nf);
+  if (r == OPAW_OK) r = dalloc_t(c, &c->nrm1, nf);
+  if (r == OPAW_OK) r = dalloc_t(c, &c->means, nsys + 1);
+  if (r == OPAW_OK) r = dalloc_t(c, &c->part, (size_t)kChunks * std::max<size_t>(nf, 4 * (size_t)nsys + 4));
+  if (r == OPAW_OK) r = dalloc(c, &c->phi, nf * N * c->esize());
+  if (r == OPAW_OK) r = dalloc_t(c, &c->pair, (size_t)((nsys + 1) / 2) * N);
+  if (r == OPAW_OK) r = dalloc_t(c, &c->hwork, (size_t)c->gbig.nx * c->gbig.ny * c->gbig.nz);
+  if (r == OPAW_OK) r = dalloc_t(c, &c->vk, (size_t)c->gbig.nx * c->gbig.ny * c->gbig.nz);
+  if (r == OPAW_OK) r = dalloc_t(c, &c->gpart, (size_t)kGramChunks * nb * nb);
+  if (r == OPAW_OK) r = dalloc_t(c, &c->gmat, (size_t)nb * nb);
+  if (r == OPAW_OK) r = dalloc_t(c, &c->tmat, (size_t)nb * nb);
+  if (r == OPAW_OK) r = (hipMemset(c->phi, 0, nf * N * c->esize()) == hipSuccess) ? OPAW_OK : fail(OPAW_ERR_HIP, "memset");
+  if (r != OPAW_OK) {
+    opaw_destroy(c);
+    return r;
+  }
+  *out = c;
+  return OPAW_OK;
+}
+
+int opaw_destroy(opaw_ctx* c) {
+  if (!c) return OPAW_OK;
+  (void)hipDeviceSynchronize();
+  c->prof.destroy();
+  for (void* p : c->allocs) (void)hipFree(p);
+  c->allocs.clear();
+  delete c;
+  return OPAW_OK;
+}
+
+int opaw_set_stream(opaw_ctx* c, void* stream) {
+  if (!c) return fail(OPAW_ERR_ARG, "null context");
+  c->st = (hipStream_t)stream;
+  return OPAW_OK;
+}
+
+int opaw_synchronize(opaw_ctx* c) {
+  if (!c) return fail(OPAW_ERR_ARG, "null context");
+  HCHK(hipStreamSynchronize(c->st));
+  return OPAW_OK;
+}
+
+long long opaw_device_bytes(opaw_ctx* c) { return c ? c->bytes : 0; }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// small device kernels local to the context
+__global__ void k_ek_prep(double* ek, float* ekf, int nx, int ny, int nz, double dx, double dy, double dz,
+                          double ekcut) {
+  const long long N = (long long)nx * ny * nz;
+  const double pi = acos(-1.0);
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
+    const int ix = (int)(i % nx);
+    const int iy = (int)((i / nx) % ny);
+    const int iz = (int)(i / ((long long)nx * ny));
+    double kx = ix * (2.0 * pi / (nx * dx));
+    double ky = iy * (2.0 * pi / (ny * dy));
+    double kz = iz * (2.0 * pi / (nz * dz));
+    if (kx > pi / dx) kx = kx - 2.0 * pi / dx;
+    if (ky > pi / dy) ky = ky - 2.0 * pi / dy;
+    if (kz > pi / dz) kz = kz - 2.0 * pi / dz;
+    double e = (kx * kx + ky * ky + kz * kz) / 2.0;
+    if (e > ekcut) e = ekcut;
+    e = e / ((double)nx * (double)ny * (double)nz);
+    ek[i] = e;
+    ekf[i] = (float)e;
+  }
+}
+
+__global__ void k_vk_real(c64* cvk, int nx, int ny, int nz, double dx, double dy, double dz, double aa, int periodic) {
+  const long long N = (long long)nx * ny * nz;
+  const double dv = dx * dy * dz;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
+    double v = 0.0;
+    if (!periodic) {
+      const int ix = (int)(i % nx);
+      const int iy = (int)((i / nx) % ny);
+      const int iz = (int)(i / ((long long)nx * ny));
+      const double x = -nx / 2.0 * dx + ix * dx;
+      const double y = -ny / 2.0 * dy + iy * dy;
+      const double z = -nz / 2.0 * dz + iz * dz;
+      double r2 = x * x + y * y + z * z;
+      if (r2 < 1e-26) r2 = 1e-26;
+      const double rp = sqrt(r2);
+      v = erf(aa * rp) / rp * dv;
+    }
+    cvk[i] = mk(v, 0.0);
+  }
+}
+
+__global__ void k_vk_finish(const c64* cvk, double* vk, int nx, int ny, int nz, double dx, double dy, double dz,
+                            double aa, int periodic) {
+  const long long N = (long long)nx * ny * nz;
+  const double pi = acos(-1.0);
+  const double r0x = -nx / 2.0 * dx, r0y = -ny / 2.0 * dy, r0z = -nz / 2.0 * dz;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
+    const int ix = (int)(i % nx);
+    const int iy = (int)((i / nx) % ny);
+    const int iz = (int)(i / ((long long)nx * ny));
+    double kx = ix * (2.0 * pi / (nx * dx));
+    double ky = iy * (2.0 * pi / (ny * dy));
+    double kz = iz * (2.0 * pi / (nz * dz));
+    if (kx > pi / dx) kx = kx - 2.0 * pi / dx;
+    if (ky > pi / dy) ky = ky - 2.0 * pi / dy;
+    if (kz > pi / dz) kz = kz - 2.0 * pi / dz;
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    double re;
+    if (!periodic) {
+      const double ph = r0x * kx + r0y * ky + r0z * kz;
+      double sp, cp;
+      sincos(ph, &sp, &cp);
+      const c64 v = cvk[i];
+      re = v.x * cp - v.y * sp;
+      const double k2s = k2 / (4.0 * aa * aa);
+      double cterm;
+      if (k2s > 1e-8)
+        cterm = 4.0 * pi / k2 * (1.0 - exp(-k2s));
+      else
+        cterm = 4.0 * pi / (4.0 * aa * aa) * (1.0 - 0.5 * k2s + k2s * k2s / 6.0);
+      re += cterm;
+    } else {
+      re = (k2 > 1e-8) ? 4.0 * pi / k2 : 0.0;
+    }
+    vk[i] = re / ((double)nx * (double)ny * (double)nz);
+  }
+}
+
+__global__ void k_copy_d2f(const double* a, float* b, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+
+static unsigned blocks_for(long long n) {
+  long long b = (n + 255) / 256;
+  return (unsigned)std::max<long long>(1, std::min<long long>(b, 65536));
+}
+
+extern "C" {
+
+int opaw_ek_prep(opaw_ctx* c, double ekcut) {
+  CTX_CHECK(c);
+  hipLaunchKernelGGL(k_ek_prep, dim3(blocks_for(c->N)), dim3(256), 0, c->st, c->ek, c->ekf, c->nx, c->ny, c->nz,
+                     c->dx, c->dy, c->dz, ekcut);
+  HCHK(hipGetLastError());
+  c->ek_ready = true;
+  return OPAW_OK;
+}
+
+int opaw_set_ek(opaw_ctx* c, const double* ek) {
+  CTX_CHECK(c);
+  if (!ek) return fail(OPAW_ERR_ARG, "null ek");
+  HCHK(hipMemcpy(c->ek, ek, c->N * sizeof(double), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_copy_d2f, dim3(blocks_for(c->N)), dim3(256), 0, c->st, c->ek, c->ekf, c->N);
+  HCHK(hipGetLastError());
+  HCHK(hipStreamSynchronize(c->st));
+  c->ek_ready = true;
+  return OPAW_OK;
+}
+
+int opaw_set_atoms(opaw_ctx* c, int natom, const int* atom_map, const int* ir_start, int ntype, const int* nrough,
+                   const int* mstates, int nl2) {
+  CTX_CHECK(c);
+  if (natom < 0 || ntype < 0 || (natom > 0 && (!atom_map || !ir_start || !nrough || !mstates)))
+    return fail(OPAW_ERR_ARG, "bad atom arguments");
+  if (nl2 < 1 || nl2 > kMaxNl2) return fail(OPAW_ERR_ARG, "nl2=%d outside 1..%d", nl2, kMaxNl2);
+  c->natom = natom;
+  c->ntype = ntype;
+  c->nl2 = nl2;
+  c->atom_type.assign(natom, 0);
+  c->atom_start.assign(3 * natom, 0);
+  c->type_ms.assign(ntype, 0);
+  c->type_nr.assign(3 * ntype, 0);
+  const int dims[3] = {c->nx, c->ny, c->nz};
+  for (int t = 0; t < ntype; ++t) {
+    c->type_ms[t] = mstates[t];
+    if (mstates[t] < 1 || mstates[t] > kMaxMs) return fail(OPAW_ERR_ARG, "mstates(%d)=%d outside 1..%d", t + 1, mstates[t], kMaxMs);
+    for (int k = 0; k < 3; ++k) {
+      c->type_nr[3 * t + k] = nrough[3 * t + k];
+      // src/read_paw.f90:254-262: nrough may not exceed the grid
+      if (nrough[3 * t + k] < 1 || nrough[3 * t + k] > dims[k])
+        return fail(OPAW_ERR_ARG, "nrough(%d,%d)=%d outside 1..%d", k + 1, t + 1, nrough[3 * t + k], dims[k]);
+    }
+  }
+  for (int a = 0; a < natom; ++a) {
+    const int t = atom_map[a] - 1;
+    if (t < 0 || t >= ntype) return fail(OPAW_ERR_ARG, "atom_map(%d)=%d outside 1..%d", a + 1, atom_map[a], ntype);
+    c->atom_type[a] = t;
+    bool outside = false;
+    for (int k = 0; k < 3; ++k) {
+      const int s0 = ir_start[3 * a + k] - 1;
+      c->atom_start[3 * a + k] = s0;
+      const int nr = c->type_nr[3 * t + k];
+      if (s0 < 0 || s0 + nr > dims[k]) outside = true;
+      if (s0 < -dims[k] + 1 || s0 >= dims[k]) return fail(OPAW_ERR_ARG, "ir_start of atom %d wraps more than once", a + 1);
+    }
+    // src/get_irs.f90:30-39
+    if (outside && !c->periodic)
+      return fail(OPAW_ERR_ARG, "non periodic, local grid of atom %d cannot go outside the box", a + 1);
+  }
+  c->hp.assign(natom, {});
+  c->hp1.assign(natom, {});
+  c->hs.assign(natom, {});
+  c->hsinv.assign(natom, {});
+  c->hsph_idx.assign(natom, {});
+  c->hsph_g.assign(natom, {});
+  c->hq.assign(ntype, {});
+  c->hd0.assign(ntype, {});
+  c->he.assign(ntype, {});
+  c->tables_ready = false;
+  return OPAW_OK;
+}
+
+static int atom_ms(opaw_ctx* c, int a) { return c->type_ms[c->atom_type[a]]; }
+static int atom_npt(opaw_ctx* c, int a) {
+  const int t = c->atom_type[a];
+  return c->type_nr[3 * t] * c->type_nr[3 * t + 1] * c->type_nr[3 * t + 2];
+}
+
+int opaw_set_projectors(opaw_ctx* c, int ia, const double* p3d, const double* p3d1, const double* s,
+                        const double* sinv) {
+  CTX_CHECK(c);
+  const int a = ia - 1;
+  if (a < 0 || a >= c->natom) return fail(OPAW_ERR_ARG, "atom index %d outside 1..%d", ia, c->natom);
+  if (!p3d || !p3d1 || !s || !sinv) return fail(OPAW_ERR_ARG, "null projector table");
+  const int ms = atom_ms(c, a), npt = atom_npt(c, a);
+  c->hp[a].assign(p3d, p3d + (size_t)ms * npt);
+  c->hp1[a].assign(p3d1, p3d1 + (size_t)ms * npt);
+  c->hs[a].assign(s, s + ms);
+  c->hsinv[a].assign(sinv, sinv + ms);
+  c->tables_ready = false;
+  return OPAW_OK;
+}
+
+int opaw_set_sphere(opaw_ctx* c, int ia, int ngrid, const int* local_grid, const double* local_g3d) {
+  CTX_CHECK(c);
+  const int a = ia - 1;
+  if (a < 0 || a >= c->natom) return fail(OPAW_ERR_ARG, "atom index %d outside 1..%d", ia, c->natom);
+  if (ngrid < 0 || (ngrid > 0 && (!local_grid || !local_g3d))) return fail(OPAW_ERR_ARG, "bad sphere arguments");
+  std::vector<int> idx(ngrid);
+  for (int p = 0; p < ngrid; ++p) {
+    const int ix = local_grid[p] - 1, iy = local_grid[p + ngrid] - 1, iz = local_grid[p + 2 * ngrid] - 1;
+    if (ix < 0 || ix >= c->nx || iy < 0 || iy >= c->ny || iz < 0 || iz >= c->nz)
+      return fail(OPAW_ERR_ARG, "local_grid point %d of atom %d outside the grid", p + 1, ia);
+    idx[p] = ix + c->nx * (iy + c->ny * iz);
+  }
+  c->hsph_idx[a] = idx;
+  c->hsph_g[a].assign(local_g3d, local_g3d + (size_t)ngrid * c->nl2);
+  c->tables_ready = false;
+  return OPAW_OK;
+}
+
+int opaw_set_type(opaw_ctx* c, int it, const double* qijlm, const double* dij0, const double* eijkl) {
+  CTX_CHECK(c);
+  const int t = it - 1;
+  if (t < 0 || t >= c->ntype) return fail(OPAW_ERR_ARG, "type index %d outside 1..%d", it, c->ntype);
+  const int ms = c->type_ms[t];
+  const size_t m2 = (size_t)ms * ms;
+  c->hq[t].assign((size_t)c->nl2 * m2, 0.0);
+  c->hd0[t].assign(m2, 0.0);
+  c->he[t].assign(m2 * m2, 0.0);
+  if (qijlm) std::copy(qijlm, qijlm + (size_t)c->nl2 * m2, c->hq[t].begin());
+  if (dij0) std::copy(dij0, dij0 + m2, c->hd0[t].begin());
+  if (eijkl) std::copy(eijkl, eijkl + m2 * m2, c->he[t].begin());
+  c->tables_ready = false;
+  return OPAW_OK;
+}
+
+}  // extern "C"
+
+template <typename T>
+static int upload(opaw_ctx* c, T** dst, const std::vector<T>& v) {
+  RCHK(dalloc_t(c, dst, std::max<size_t>(v.size(), 1)));
+  if (!v.empty()) HCHK(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return OPAW_OK;
+}
+
+extern "C" {
+
+int opaw_finalize_tables(opaw_ctx* c) {
+  CTX_CHECK(c);
+  const int natom = c->natom;
+  for (int a = 0; a < natom; ++a)
+    if (c->hp[a].empty()) return fail(OPAW_ERR_STATE, "projectors of atom %d not set", a + 1);
+  // free previous device tables
+  for (void* p : {(void*)c->datoms, (void*)c->P, (void*)c->P1, (void*)c->sn[0], (void*)c->sn[1], (void*)c->sn[2],
+                  (void*)c->sn[3], (void*)c->sn_custom, (void*)c->plane_ptr, (void*)c->plane, (void*)c->sph_idx,
+                  (void*)c->sph_g, (void*)c->qtab, (void*)c->d0tab, (void*)c->etab, (void*)c->dij, (void*)c->rhoij})
+    dfree(c, p);
+  // per-type table offsets
+  std::vector<int> toff_q(c->ntype), toff_d0(c->ntype), toff_e(c->ntype);
+  std::vector<double> q, d0, e;
+  for (int t = 0; t < c->ntype; ++t) {
+    const int ms = c->type_ms[t];
+    const size_t m2 = (size_t)ms * ms;
+    if (c->hq[t].empty()) {
+      c->hq[t].assign((size_t)c->nl2 * m2, 0.0);
+      c->hd0[t].assign(m2, 0.0);
+      c->he[t].assign(m2 * m2, 0.0);
+    }
+    toff_q[t] = (int)q.size();
+    toff_d0[t] = (int)d0.size();
+    toff_e[t] = (int)e.size();
+    q.insert(q.end(), c->hq[t].begin(), c->hq[t].end());
+    d0.insert(d0.end(), c->hd0[t].begin(), c->hd0[t].end());
+    e.insert(e.end(), c->he[t].begin(), c->he[t].end());
+  }
+  c->hatoms.assign(natom, AtomDesc{});
+  std::vector<double> P, P1, s, sm1, smh, sph, sp1;
+  std::vector<int> sidx;
+  std::vector<double> sg;
+  long long tab = 0;
+  int coff = 0, moff = 0, soff = 0;
+  const int dims[3] = {c->nx, c->ny, c->nz};
+  for (int a = 0; a < natom; ++a) {
+    const int t = c->atom_type[a];
+    AtomDesc& A = c->hatoms[a];
+    A.ms = c->type_ms[t];
+    A.itype = t;
+    A.wrap = 0;
+    for (int k = 0; k < 3; ++k) {
+      A.start[k] = c->atom_start[3 * a + k];
+      A.nr[k] = c->type_nr[3 * t + k];
+      if (A.start[k] < 0 || A.start[k] + A.nr[k] > dims[k]) A.wrap = 1;
+    }
+    A.npt = A.nr[0] * A.nr[1] * A.nr[2];
+    A.tab = tab;
+    A.coff = coff;
+    A.moff = moff;
+    A.sph_off = soff;
+    A.sph_n = (int)c->hsph_idx[a].size();
+    A.toff_q = toff_q[t];
+    A.toff_d0 = toff_d0[t];
+    A.toff_e = toff_e[t];
+    P.insert(P.end(), c->hp[a].begin(), c->hp[a].end());
+    P1.insert(P1.end(), c->hp1[a].begin(), c->hp1[a].end());
+    for (int i = 0; i < A.ms; ++i) {
+      const double si = c->hs[a][i];
+      s.push_back(si);
+      sm1.push_back(c->hsinv[a][i]);
+      smh.push_back(std::pow(1.0 + si, -0.5) - 1.0);
+      sph.push_back(std::pow(1.0 + si, 0.5) - 1.0);
+      sp1.push_back(std::pow(1.0 + si, 1.0) - 1.0);
+    }
+    sidx.insert(sidx.end(), c->hsph_idx[a].begin(), c->hsph_idx[a].end());
+    sg.insert(sg.end(), c->hsph_g[a].begin(), c->hsph_g[a].end());
+    tab += (long long)A.ms * A.npt;
+    coff += A.ms;
+    moff += A.ms * A.ms;
+    soff += A.sph_n;
+  }
+  c->ctot = coff;
+  c->mtot = moff;
+  c->sphtot = soff;
+  // plane lists: atoms whose box covers grid plane z, in atom order
+  std::vector<std::vector<PlaneEntry>> pl(c->nz);
+  for (int a = 0; a < natom; ++a) {
+    const AtomDesc& A = c->hatoms[a];
+    for (int lz = 0; lz < A.nr[2]; ++lz) {
+      int z = A.start[2] + lz;
+      z = ((z % c->nz) + c->nz) % c->nz;
+      pl[z].push_back(PlaneEntry{a, lz});
+    }
+  }
+  std::vector<int> pptr(c->nz + 1, 0);
+  std::vector<PlaneEntry> plane;
+  for (int z = 0; z < c->nz; ++z) {
+    pptr[z] = (int)plane.size();
+    plane.insert(plane.end(), pl[z].begin(), pl[z].end());
+  }
+  pptr[c->nz] = (int)plane.size();
+  RCHK(upload(c, &c->datoms, c->hatoms));
+  RCHK(upload(c, &c->P, P));
+  RCHK(upload(c, &c->P1, P1));
+  RCHK(upload(c, &c->sn[0], sm1));
+  RCHK(upload(c, &c->sn[1], smh));
+  RCHK(upload(c, &c->sn[2], sph));
+  RCHK(upload(c, &c->sn[3], sp1));
+  RCHK(dalloc_t(c, &c->sn_custom, std::max(1, coff)));
+  RCHK(upload(c, &c->plane_ptr, pptr));
+  RCHK(upload(c, &c->plane, plane));
+  RCHK(upload(c, &c->sph_idx, sidx));
+  RCHK(upload(c, &c->sph_g, sg));
+  RCHK(upload(c, &c->qtab, q));
+  RCHK(upload(c, &c->d0tab, d0));
+  RCHK(upload(c, &c->etab, e));
+  RCHK(dalloc_t(c, &c->dij, (size_t)c->nsys * std::max(1, moff)));
+  RCHK(dalloc_t(c, &c->rhoij, (size_t)c->nsys * std::max(1, moff)));
+  HCHK(hipMemset(c->dij, 0, (size_t)c->nsys * std::max(1, moff) * sizeof(double)));
+  HCHK(hipMemset(c->rhoij, 0, (size_t)c->nsys * std::max(1, moff) * sizeof(double)));
+  c->tables_ready = true;
+  return OPAW_OK;
+}
+
+int opaw_set_potentials(opaw_ctx* c, const double* vloc_tot, const double* ncoret, double rnel, int usexcnhat) {
+  CTX_CHECK(c);
+  if (vloc_tot)
+    HCHK(hipMemcpy(c->vloc, vloc_tot, c->N * sizeof(double), hipMemcpyHostToDevice));
+  else
+    HCHK(hipMemset(c->vloc, 0, c->N * sizeof(double)));
+  if (ncoret)
+    HCHK(hipMemcpy(c->ncoret, ncoret, c->N * sizeof(double), hipMemcpyHostToDevice));
+  else
+    HCHK(hipMemset(c->ncoret, 0, c->N * sizeof(double)));
+  c->rnel = rnel;
+  c->usexcnhat = usexcnhat;
+  c->pot_ready = true;
+  return OPAW_OK;
+}
+
+int opaw_vk_prep(opaw_ctx* c) {
+  CTX_CHECK(c);
+  const GridGeom& B = c->gbig;
+  const long long NB = (long long)B.nx * B.ny * B.nz;
+  const double aa = 7.0 / std::min({B.nx * c->dx, B.ny * c->dy, B.nz * c->dz});
+  hipLaunchKernelGGL(k_vk_real, dim3(blocks_for(NB)), dim3(256), 0, c->st, c->hwork, B.nx, B.ny, B.nz, c->dx, c->dy,
+                     c->dz, aa, c->periodic);
+  HCHK(hipGetLastError());
+  if (!c->periodic) {
+    // fft3d_general(sign=1): unnormalised e^{+ikr} transform
+    HCHK(fft3d<double>(B, c->st, +1, c->hwork, c->hwork, 1, c->tw64[B.nx], c->tw64[B.ny], c->tw64[B.nz]));
+  }
+  hipLaunchKernelGGL(k_vk_finish, dim3(blocks_for(NB)), dim3(256), 0, c->st, c->hwork, c->vk, B.nx, B.ny, B.nz, c->dx,
+                     c->dy, c->dz, aa, c->periodic);
+  HCHK(hipGetLastError());
+  HCHK(hipStreamSynchronize(c->st));
+  c->vk_ready = true;
+  return OPAW_OK;
+}
+
+static int check_sys(opaw_ctx* c, int sys) {
+  if (sys < 0 || sys >= c->nsys) return fail(OPAW_ERR_ARG, "system %d outside 0..%d", sys, c->nsys - 1);
+  return OPAW_OK;
+}
+
+int opaw_write_vks(opaw_ctx* c, int sys, const double* vks) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  if (!vks) return fail(OPAW_ERR_ARG, "null vks");
+  HCHK(hipMemcpy(c->vks + sys * c->N, vks, c->N * sizeof(double), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_copy_d2f, dim3(blocks_for(c->N)), dim3(256), 0, c->st, c->vks + sys * c->N,
+                     c->vksf + sys * c->N, c->N);
+  HCHK(hipGetLastError());
+  HCHK(hipStreamSynchronize(c->st));
+  return OPAW_OK;
+}
+
+int opaw_read_vks(opaw_ctx* c, int sys, double* vks) { return opaw_read_field(c, OPAW_FIELD_VKS, sys, vks); }
+
+int opaw_read_field(opaw_ctx* c, int which, int sys, double* out) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  if (!out) return fail(OPAW_ERR_ARG, "null output");
+  const double* src = nullptr;
+  switch (which) {
+    case OPAW_FIELD_DENS: src = c->dens; break;
+    case OPAW_FIELD_NHAT: src = c->nhat; break;
+    case OPAW_FIELD_VXC: src = c->vxc; break;
+    case OPAW_FIELD_VH: src = c->vh; break;
+    case OPAW_FIELD_VKS: src = c->vks; break;
+    default: return fail(OPAW_ERR_ARG, "unknown field %d", which);
+  }
+  HCHK(hipStreamSynchronize(c->st));
+  HCHK(hipMemcpy(out, src + sys * c->N, c->N * sizeof(double), hipMemcpyDeviceToHost));
+  return OPAW_OK;
+}
+
+static int check_atom(opaw_ctx* c, int ia) {
+  if (!c->tables_ready) return fail(OPAW_ERR_STATE, "tables not finalised");
+  if (ia < 1 || ia > c->natom) return fail(OPAW_ERR_ARG, "atom index %d outside 1..%d", ia, c->natom);
+  return OPAW_OK;
+}
+
+int opaw_write_dij(opaw_ctx* c, int sys, int ia, const double* dij) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  RCHK(check_atom(c, ia));
+  if (!dij) return fail(OPAW_ERR_ARG, "null dij");
+  const AtomDesc& A = c->hatoms[ia - 1];
+  HCHK(hipMemcpy(c->dij + (size_t)sys * c->mtot + A.moff, dij, (size_t)A.ms * A.ms * sizeof(double),
+                 hipMemcpyHostToDevice));
+  return OPAW_OK;
+}
+
+int opaw_read_dij(opaw_ctx* c, int sys, int ia, double* dij) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  RCHK(check_atom(c, ia));
+  if (!dij) return fail(OPAW_ERR_ARG, "null dij");
+  const AtomDesc& A = c->hatoms[ia - 1];
+  HCHK(hipStreamSynchronize(c->st));
+  HCHK(hipMemcpy(dij, c->dij + (size_t)sys * c->mtot + A.moff, (size_t)A.ms * A.ms * sizeof(double),
+                 hipMemcpyDeviceToHost));
+  return OPAW_OK;
+}
+
+int opaw_read_rhoij(opaw_ctx* c, int sys, int ia, double* rho) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  RCHK(check_atom(c, ia));
+  if (!rho) return fail(OPAW_ERR_ARG, "null rhoij");
+  const AtomDesc& A = c->hatoms[ia - 1];
+  HCHK(hipStreamSynchronize(c->st));
+  HCHK(hipMemcpy(rho, c->rhoij + (size_t)sys * c->mtot + A.moff, (size_t)A.ms * A.ms * sizeof(double),
+                 hipMemcpyDeviceToHost));
+  return OPAW_OK;
+}
+
+int opaw_write_orbitals(opaw_ctx* c, int sys, const void* host) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  if (!host) return fail(OPAW_ERR_ARG, "null orbitals");
+  const size_t bytes = (size_t)c->nb * c->N * c->esize();
+  HCHK(hipStreamSynchronize(c->st));
+  HCHK(hipMemcpy((char*)c->phi + sys * bytes, host, bytes, hipMemcpyHostToDevice));
+  return OPAW_OK;
+}
+
+int opaw_read_orbitals(opaw_ctx* c, int sys, void* host) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  if (!host) return fail(OPAW_ERR_ARG, "null orbitals");
+  const size_t bytes = (size_t)c->nb * c->N * c->esize();
+  HCHK(hipStreamSynchronize(c->st));
+  HCHK(hipMemcpy(host, (char*)c->phi + sys * bytes, bytes, hipMemcpyDeviceToHost));
+  return OPAW_OK;
+}
+
+void* opaw_orbitals_device(opaw_ctx* c, int sys) {
+  if (!c || sys < 0 || sys >= c->nsys) return nullptr;
+  return (char*)c->phi + (size_t)sys * c->nb * c->N * c->esize();
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// operator flows (templated on the orbital precision)
+static BoxTables box_tables(opaw_ctx* c) {
+  BoxTables b{};
+  b.atoms = c->datoms;
+  b.natom = c->natom;
+  b.ctot = c->ctot;
+  b.p = c->P;
+  b.p1 = c->P1;
+  b.s = c->sn[3];
+  b.plane_ptr = c->plane_ptr;
+  b.plane = c->plane;
+  return b;
+}
+
+static int need_tables(opaw_ctx* c) {
+  if (!c->tables_ready) return fail(OPAW_ERR_STATE, "PAW tables not finalised (opaw_finalize_tables)");
+  if (!c->ek_ready) return fail(OPAW_ERR_STATE, "kinetic table not set (opaw_ek_prep / opaw_set_ek)");
+  return OPAW_OK;
+}
+
+static int sn_table(opaw_ctx* c, double n, const double** out) {
+  if (n == -1.0) *out = c->sn[0];
+  else if (n == -0.5) *out = c->sn[1];
+  else if (n == 0.5) *out = c->sn[2];
+  else if (n == 1.0) *out = c->sn[3];
+  else {
+    std::vector<double> v;
+    for (int a = 0; a < c->natom; ++a)
+      for (int i = 0; i < c->hatoms[a].ms; ++i) v.push_back(std::pow(1.0 + c->hs[a][i], n) - 1.0);
+    HCHK(hipStreamSynchronize(c->st));
+    if (!v.empty()) HCHK(hipMemcpy(c->sn_custom, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+    *out = c->sn_custom;
+  }
+  return OPAW_OK;
+}
+
+template <typename R>
+static int ensure_ca(opaw_ctx* c, int nf) {
+  RCHK(ensure(c, c->wca, (size_t)nf * std::max(1, c->ctot) * sizeof(c64)));
+  RCHK(ensure(c, c->wca2, (size_t)nf * std::max(1, c->ctot) * sizeof(c64)));
+  return OPAW_OK;
+}
+
+// out = S^n in  (in may equal out)
+template <typename R>
+static int flow_sn_phi(opaw_ctx* c, const cx<R>* in, cx<R>* out, int nf, double n) {
+  RCHK(ensure_ca<R>(c, nf));
+  const double* sn = nullptr;
+  RCHK(sn_table(c, n, &sn));
+  BoxTables bt = box_tables(c);
+  c64* ca = (c64*)c->wca.p;
+  HCHK(box_project<R>(c->g, bt, c->st, in, nf, 1, c->dx * c->dy * c->dz, ca));
+  if (in != out) HCHK(hipMemcpyAsync(out, in, (size_t)nf * c->N * sizeof(cx<R>), hipMemcpyDeviceToDevice, c->st));
+  HCHK(box_add<R>(c->g, bt, c->st, out, nf, 1, ca, 0, n, sn, nullptr, nf, c->mtot));
+  return OPAW_OK;
+}
+
+template <typename R>
+struct Fields;
+template <>
+struct Fields<double> {
+  static const double* vks(opaw_ctx* c) { return c->vks; }
+  static const double* ek(opaw_ctx* c) { return c->ek; }
+};
+template <>
+struct Fields<float> {
+  static const float* vks(opaw_ctx* c) { return c->vksf; }
+  static const float* ek(opaw_ctx* c) { return c->ekf; }
+};
+
+// out = H(sys) in for nf fields; field f uses system sys0 + f / nf_per_sys
+template <typename R>
+static int flow_h_phi(opaw_ctx* c, int sys0, const cx<R>* in, cx<R>* out, int nf, int nf_per_sys) {
+  RCHK(ensure_ca<R>(c, nf));
+  RCHK(ensure(c, c->wtmp, (size_t)nf * c->N * sizeof(cx<R>)));
+  BoxTables bt = box_tables(c);
+  c64* ca = (c64*)c->wca2.p;
+  HCHK(box_project<R>(c->g, bt, c->st, in, nf, 0, c->dx * c->dy * c->dz, ca));
+  HCHK(apply_t_plus_v<R>(c->g, c->st, in, out, (cx<R>*)c->wtmp.p, nf, nf_per_sys, Fields<R>::ek(c),
+                         Fields<R>::vks(c) + (size_t)sys0 * c->N, TW<R>::get(c, c->nx), TW<R>::get(c, c->ny),
+                         TW<R>::get(c, c->nz)));
+  HCHK(box_add<R>(c->g, bt, c->st, out, nf, 0, ca, 1, 0.0, nullptr, c->dij + (size_t)sys0 * c->mtot, nf_per_sys,
+                  c->mtot));
+  return OPAW_OK;
+}
+
+// out = S^-1/2 H S^-1/2 in; uses the wu work buffer
+template <typename R>
+static int flow_shs(opaw_ctx* c, int sys0, const cx<R>* in, cx<R>* out, int nf, int nf_per_sys) {
+  RCHK(ensure(c, c->wu, (size_t)nf * c->N * sizeof(cx<R>)));
+  cx<R>* u = (cx<R>*)c->wu.p;
+  RCHK(flow_sn_phi<R>(c, in, u, nf, -0.5));
+  RCHK(flow_h_phi<R>(c, sys0, u, out, nf, nf_per_sys));
+  RCHK(flow_sn_phi<R>(c, out, out, nf, -0.5));
+  return OPAW_OK;
+}
+
+// The per-stage rebuild for systems [s0, s0+ns): psi~ = S^-1/2 phi, rho_ij,
+// density, nhat + rnel renormalisation, V_H + V_xc -> V_KS, D_ij
+// (src/ncpaw_make_hamiltonian.f90:213-246).  phi holds ns*nb fields.
+template <typename R>
+static int flow_rebuild(opaw_ctx* c, int s0, int ns, const cx<R>* phi) {
+  if (!c->pot_ready) return fail(OPAW_ERR_STATE, "potentials not set (opaw_set_potentials)");
+  if (!c->vk_ready) return fail(OPAW_ERR_STATE, "Hartree kernel not set (opaw_vk_prep)");
+  const int nf = ns * c->nb;
+  const long long N = c->N;
+  const double dv = c->dx * c->dy * c->dz;
+  RCHK(ensure(c, c->wu, (size_t)nf * N * sizeof(cx<R>)));
+  cx<R>* psit = (cx<R>*)c->wu.p;
+  RCHK(flow_sn_phi<R>(c, phi, psit, nf, -0.5));
+  BoxTables bt = box_tables(c);
+  c64* ca = (c64*)c->wca2.p;
+  HCHK(box_project<R>(c->g, bt, c->st, psit, nf, 0, dv, ca));
+  prof_begin(c->st, KC_DENS);
+  HCHK(rhoij(c->st, c->datoms, c->natom, ca, c->ctot, ns, c->nb, c->mtot, c->rhoij + (size_t)s0 * c->mtot));
+  prof_end(c->st, KC_DENS);
+  prof_begin(c->st, KC_DENS);
+  HCHK(density<R>(c->st, psit, N, ns, c->nb, c->dens + s0 * N));
+  prof_end(c->st, KC_DENS);
+  HCHK(nhat(c->st, c->datoms, c->natom, c->rhoij + (size_t)s0 * c->mtot, c->mtot, c->qtab, c->nl2, c->sph_idx,
+            c->sph_g, N, ns, c->nhat + s0 * N));
+  HCHK(renorm_factor(c->st, c->nhat + s0 * N, c->dens + s0 * N, N, ns, dv, c->rnel, c->part, c->fac + s0));
+  prof_begin(c->st, KC_POT);
+  HCHK(scale_pack(c->st, c->dens + s0 * N, c->nhat + s0 * N, c->fac + s0, N, ns, c->pair));
+  prof_end(c->st, KC_POT);
+  const int npair = (ns + 1) / 2;
+  for (int q = 0; q < npair; ++q)
+    HCHK(hartree_conv(c->g, c->scale, c->st, c->pair + q * N, c->pair + q * N, c->hwork, c->vk,
+                      c->tw64[c->gbig.nx], c->tw64[c->gbig.ny], c->tw64[c->gbig.nz]));
+  if (c->periodic) HCHK(pair_means(c->st, c->pair, N, ns, c->part, c->means));
+  prof_begin(c->st, KC_POT);
+  HCHK(vks_build(c->st, c->dens + s0 * N, c->ncoret, c->pair, c->periodic ? c->means : nullptr, c->vloc, N, ns,
+                 c->vxc + s0 * N, c->vh + s0 * N, c->vks + s0 * N, c->vksf + s0 * N));
+  prof_end(c->st, KC_POT);
+  prof_begin(c->st, KC_POT);
+  HCHK(dij(c->st, c->datoms, c->natom, c->rhoij + (size_t)s0 * c->mtot, c->mtot, c->qtab, c->d0tab, c->etab, c->nl2,
+           c->sph_idx, c->sph_g, c->vks + s0 * N, c->vxc + s0 * N, c->usexcnhat == 0 ? 1 : 0, N, ns, dv,
+           c->dij + (size_t)s0 * c->mtot));
+  prof_end(c->st, KC_POT);
+  return OPAW_OK;
+}
+
+// one RK4 step of all systems (src/rk4.f90:1-103)
+template <typename R>
+static int flow_rk4(opaw_ctx* c, double dt) {
+  const int nf = c->nsys * c->nb;
+  const long long n = (long long)nf * c->N;
+  const double dv = c->dx * c->dy * c->dz;
+  RCHK(ensure(c, c->wy, n * sizeof(cx<R>)));
+  RCHK(ensure(c, c->wacc, n * sizeof(cx<R>)));
+  RCHK(ensure(c, c->wh, n * sizeof(cx<R>)));
+  cx<R>* p = (cx<R>*)c->phi;
+  cx<R>* y = (cx<R>*)c->wy.p;
+  cx<R>* acc = (cx<R>*)c->wacc.p;
+  cx<R>* h = (cx<R>*)c->wh.p;
+  prof_begin(c->st, KC_DENS);
+  HCHK(field_norms<R>(c->st, p, c->N, nf, dv, c->part, c->nrm0));
+  prof_end(c->st, KC_DENS);
+  const double cst[4] = {dt / 2.0, dt / 2.0, dt, dt};
+  for (int stage = 1; stage <= 4; ++stage) {
+    const cx<R>* yin = (stage == 1) ? p : y;
+    RCHK(flow_shs<R>(c, 0, yin, h, nf, c->nb));
+    prof_begin(c->st, KC_RK4);
+    HCHK(rk4_update<R>(c->st, stage, n, h, p, acc, y, cst[stage - 1], dt));
+    prof_end(c->st, KC_RK4);
+    if (stage < 4) RCHK(flow_rebuild<R>(c, 0, c->nsys, y));
+  }
+  prof_begin(c->st, KC_DENS);
+  HCHK(field_norms<R>(c->st, p, c->N, nf, dv, c->part, c->nrm1));
+  prof_end(c->st, KC_DENS);
+  prof_begin(c->st, KC_RK4);
+  HCHK(scale_fields<R>(c->st, p, c->N, nf, c->nrm0, c->nrm1));
+  prof_end(c->st, KC_RK4);
+  return OPAW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// host Hermitian eigensolver (cyclic complex Jacobi), FP64.  A is n x n
+// row-major A[i*n+j] = A(i,j); returns ascending eigenvalues and column
+// eigenvectors V[i*n+k] = V(i,k).
+static void herm_eig(int n, std::vector<std::complex<double>> A, std::vector<double>& w,
+                     std::vector<std::complex<double>>& V) {
+  using C = std::complex<double>;
+  V.assign((size_t)n * n, C(0, 0));
+  for (int i = 0; i < n; ++i) V[(size_t)i * n + i] = 1.0;
+  double scale = 0.0;
+  for (auto& v : A) scale += std::norm(v);
+  scale = std::sqrt(scale);
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) off += std::norm(A[(size_t)p * n + q]);
+    if (std::sqrt(off) <= 1e-17 * scale || off == 0.0) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const C apq = A[(size_t)p * n + q];
+        const double r = std::abs(apq);
+        if (r == 0.0) continue;
+        const C ph = apq / r;  // e^{i alpha}
+        const double app = A[(size_t)p * n + p].real(), aqq = A[(size_t)q * n + q].real();
+        const double theta = (aqq - app) / (2.0 * r);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
+        const C em = std::conj(ph);  // e^{-i alpha}
+        // A <- A W (columns), W = [[c, s], [-s e^{-ia}, c e^{-ia}]]
+        for (int k = 0; k < n; ++k) {
+          const C akp = A[(size_t)k * n + p], akq = A[(size_t)k * n + q];
+          A[(size_t)k * n + p] = cs * akp - sn * em * akq;
+          A[(size_t)k * n + q] = sn * akp + cs * em * akq;
+        }
+        // A <- W^H A (rows)
+        for (int k = 0; k < n; ++k) {
+          const C apk = A[(size_t)p * n + k], aqk = A[(size_t)q * n + k];
+          A[(size_t)p * n + k] = cs * apk - sn * ph * aqk;
+          A[(size_t)q * n + k] = sn * apk + cs * ph * aqk;
+        }
+        A[(size_t)p * n + q] = 0.0;
+        A[(size_t)q * n + p] = 0.0;
+        A[(size_t)p * n + p] = A[(size_t)p * n + p].real();
+        A[(size_t)q * n + q] = A[(size_t)q * n + q].real();
+        for (int k = 0; k < n; ++k) {
+          const C vkp = V[(size_t)k * n + p], vkq = V[(size_t)k * n + q];
+          V[(size_t)k * n + p] = cs * vkp - sn * em * vkq;
+          V[(size_t)k * n + q] = sn * vkp + cs * em * vkq;
+        }
+      }
+  }
+  std::vector<int> ord(n);
+  for (int i = 0; i < n; ++i) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return A[(size_t)a * n + a].real() < A[(size_t)b * n + b].real(); });
+  std::vector<C> V2((size_t)n * n);
+  w.resize(n);
+  for (int k = 0; k < n; ++k) {
+    w[k] = A[(size_t)ord[k] * n + ord[k]].real();
+    for (int i = 0; i < n; ++i) V2[(size_t)i * n + k] = V[(size_t)i * n + ord[k]];
+  }
+  V.swap(V2);
+}
+
+// hm = dv * <a_i | b_j> for nb fields; downloads the matrix (row-major)
+template <typename R>
+static int gram_host(opaw_ctx* c, const cx<R>* a, const cx<R>* b, std::vector<std::complex<double>>& hm) {
+  const int nb = c->nb;
+  HCHK(gram<R>(c->st, a, b, c->N, nb, c->dx * c->dy * c->dz, c->gpart, c->gmat));
+  hm.resize((size_t)nb * nb);
+  HCHK(hipStreamSynchronize(c->st));
+  HCHK(hipMemcpy(hm.data(), c->gmat, (size_t)nb * nb * sizeof(c64), hipMemcpyDeviceToHost));
+  return OPAW_OK;
+}
+
+// phi <- phi T with T(i,j) (column-major upload), through the wtmp buffer
+template <typename R>
+static int rotate_resident(opaw_ctx* c, int sys, const std::vector<std::complex<double>>& Tcol) {
+  const int nb = c->nb;
+  RCHK(ensure(c, c->wtmp, (size_t)nb * c->N * sizeof(cx<R>)));
+  HCHK(hipMemcpy(c->tmat, Tcol.data(), (size_t)nb * nb * sizeof(c64), hipMemcpyHostToDevice));
+  cx<R>* phi = (cx<R>*)c->phi + (size_t)sys * nb * c->N;
+  HCHK(rotate<R>(c->st, phi, (cx<R>*)c->wtmp.p, c->N, nb, c->tmat));
+  HCHK(hipMemcpyAsync(phi, c->wtmp.p, (size_t)nb * c->N * sizeof(cx<R>), hipMemcpyDeviceToDevice, c->st));
+  HCHK(hipStreamSynchronize(c->st));
+  return OPAW_OK;
+}
+
+template <typename R>
+static int flow_orthog(opaw_ctx* c, int sys, int h_type, double* eig) {
+  const int nb = c->nb;
+  cx<R>* phi = (cx<R>*)c->phi + (size_t)sys * nb * c->N;
+  const cx<R>* sp = phi;
+  if (h_type == 0) {
+    RCHK(ensure(c, c->wu, (size_t)nb * c->N * sizeof(cx<R>)));
+    RCHK(flow_sn_phi<R>(c, phi, (cx<R>*)c->wu.p, nb, 1.0));
+    sp = (const cx<R>*)c->wu.p;
+  }
+  std::vector<std::complex<double>> hm, V;
+  RCHK(gram_host<R>(c, sp, phi, hm));
+  std::vector<double> w;
+  herm_eig(nb, hm, w, V);
+  for (int k = 0; k < nb; ++k)
+    if (!(w[k] > 0.0)) return fail(OPAW_ERR_ARG, "overlap eigenvalue %d is %g (linearly dependent orbitals)", k + 1, w[k]);
+  std::vector<std::complex<double>> T((size_t)nb * nb);
+  for (int i = 0; i < nb; ++i)
+    for (int j = 0; j < nb; ++j) T[i + (size_t)nb * j] = V[(size_t)i * nb + j] / std::sqrt(w[j]);
+  RCHK(rotate_resident<R>(c, sys, T));
+  if (eig) std::copy(w.begin(), w.end(), eig);
+  return OPAW_OK;
+}
+
+template <typename R>
+static int flow_diagh(opaw_ctx* c, int sys, int h_type, double* eig) {
+  const int nb = c->nb;
+  RCHK(need_tables(c));
+  cx<R>* phi = (cx<R>*)c->phi + (size_t)sys * nb * c->N;
+  RCHK(ensure(c, c->wh, (size_t)nb * c->N * sizeof(cx<R>)));
+  cx<R>* hp = (cx<R>*)c->wh.p;
+  if (h_type == 1)
+    RCHK(flow_shs<R>(c, sys, phi, hp, nb, nb));
+  else
+    RCHK(flow_h_phi<R>(c, sys, phi, hp, nb, nb));
+  std::vector<std::complex<double>> hm, V;
+  RCHK(gram_host<R>(c, hp, phi, hm));
+  std::vector<double> w;
+  herm_eig(nb, hm, w, V);
+  std::vector<std::complex<double>> T((size_t)nb * nb);
+  for (int i = 0; i < nb; ++i)
+    for (int j = 0; j < nb; ++j) T[i + (size_t)nb * j] = V[(size_t)i * nb + j];
+  RCHK(rotate_resident<R>(c, sys, T));
+  if (eig) std::copy(w.begin(), w.end(), eig);
+  return OPAW_OK;
+}
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int opaw_h_phi(opaw_ctx* c, int sys, const void* pin, void* hp, int nbatch) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  RCHK(need_tables(c));
+  if (!pin || !hp || nbatch <= 0) return fail(OPAW_ERR_ARG, "bad h_phi arguments");
+  if (pin == hp) return fail(OPAW_ERR_ARG, "h_phi needs distinct input and output (src/hphi.f90:1-10)");
+  if (c->prec == OPAW_PREC_F64) return flow_h_phi<double>(c, sys, (const c64*)pin, (c64*)hp, nbatch, nbatch);
+  return flow_h_phi<float>(c, sys, (const c32*)pin, (c32*)hp, nbatch, nbatch);
+}
+
+int opaw_sn_phi(opaw_ctx* c, const void* pin, void* sp, int nbatch, double n) {
+  CTX_CHECK(c);
+  if (!c->tables_ready) return fail(OPAW_ERR_STATE, "tables not finalised");
+  if (!pin || !sp || nbatch <= 0) return fail(OPAW_ERR_ARG, "bad sn_phi arguments");
+  if (c->prec == OPAW_PREC_F64) return flow_sn_phi<double>(c, (const c64*)pin, (c64*)sp, nbatch, n);
+  return flow_sn_phi<float>(c, (const c32*)pin, (c32*)sp, nbatch, n);
+}
+
+int opaw_shs(opaw_ctx* c, int sys, const void* pin, void* pout, int nbatch) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  RCHK(need_tables(c));
+  if (!pin || !pout || nbatch <= 0) return fail(OPAW_ERR_ARG, "bad shs arguments");
+  if (pin == pout) return fail(OPAW_ERR_ARG, "shs needs distinct input and output (src/shs.f90:3-12)");
+  if (c->prec == OPAW_PREC_F64) return flow_shs<double>(c, sys, (const c64*)pin, (c64*)pout, nbatch, nbatch);
+  return flow_shs<float>(c, sys, (const c32*)pin, (c32*)pout, nbatch, nbatch);
+}
+
+int opaw_proj(opaw_ctx* c, const void* pin, int nbatch, int which_p1, void* ca) {
+  CTX_CHECK(c);
+  if (!c->tables_ready) return fail(OPAW_ERR_STATE, "tables not finalised");
+  if (!pin || !ca || nbatch <= 0) return fail(OPAW_ERR_ARG, "bad proj arguments");
+  BoxTables bt = box_tables(c);
+  const double dv = c->dx * c->dy * c->dz;
+  if (c->prec == OPAW_PREC_F64)
+    HCHK(box_project<double>(c->g, bt, c->st, (const c64*)pin, nbatch, which_p1 ? 1 : 0, dv, (c64*)ca));
+  else
+    HCHK(box_project<float>(c->g, bt, c->st, (const c32*)pin, nbatch, which_p1 ? 1 : 0, dv, (c64*)ca));
+  return OPAW_OK;
+}
+
+int opaw_fft3d(opaw_ctx* c, int dir, const void* in, void* out, int nbatch) {
+  CTX_CHECK(c);
+  if (!in || !out || nbatch <= 0 || (dir != 1 && dir != -1)) return fail(OPAW_ERR_ARG, "bad fft3d arguments");
+  if (c->prec == OPAW_PREC_F64)
+    HCHK(fft3d<double>(c->g, c->st, dir, (const c64*)in, (c64*)out, nbatch, c->tw64[c->nx], c->tw64[c->ny],
+                       c->tw64[c->nz]));
+  else
+    HCHK(fft3d<float>(c->g, c->st, dir, (const c32*)in, (c32*)out, nbatch, c->tw32[c->nx], c->tw32[c->ny],
+                      c->tw32[c->nz]));
+  return OPAW_OK;
+}
+
+int opaw_tddft_rhoij_2(opaw_ctx* c, int sys, const void* psi) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  if (!c->tables_ready) return fail(OPAW_ERR_STATE, "tables not finalised");
+  if (!psi) return fail(OPAW_ERR_ARG, "null psi");
+  const int nb = c->nb;
+  if (c->prec == OPAW_PREC_F64) RCHK(ensure_ca<double>(c, nb));
+  else RCHK(ensure_ca<float>(c, nb));
+  RCHK(opaw_proj(c, psi, nb, 0, c->wca2.p));
+  HCHK(rhoij(c->st, c->datoms, c->natom, (const c64*)c->wca2.p, c->ctot, 1, nb, c->mtot,
+             c->rhoij + (size_t)sys * c->mtot));
+  return OPAW_OK;
+}
+
+int opaw_tddft_dens_2(opaw_ctx* c, int sys, const void* psi) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  if (!psi) return fail(OPAW_ERR_ARG, "null psi");
+  if (c->prec == OPAW_PREC_F64)
+    HCHK(density<double>(c->st, (const c64*)psi, c->N, 1, c->nb, c->dens + sys * c->N));
+  else
+    HCHK(density<float>(c->st, (const c32*)psi, c->N, 1, c->nb, c->dens + sys * c->N));
+  return OPAW_OK;
+}
+
+int opaw_get_nhat(opaw_ctx* c, int sys) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  if (!c->tables_ready || !c->pot_ready) return fail(OPAW_ERR_STATE, "tables / potentials not set");
+  const long long N = c->N;
+  HCHK(nhat(c->st, c->datoms, c->natom, c->rhoij + (size_t)sys * c->mtot, c->mtot, c->qtab, c->nl2, c->sph_idx,
+            c->sph_g, N, 1, c->nhat + sys * N));
+  HCHK(renorm_factor(c->st, c->nhat + sys * N, c->dens + sys * N, N, 1, c->dx * c->dy * c->dz, c->rnel, c->part,
+                     c->fac + sys));
+  HCHK(scale_pack(c->st, c->dens + sys * N, c->nhat + sys * N, c->fac + sys, N, 1, c->pair));
+  return OPAW_OK;
+}
+
+int opaw_get_pot_tddft_2(opaw_ctx* c, int sys) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  if (!c->pot_ready || !c->vk_ready) return fail(OPAW_ERR_STATE, "potentials / Hartree kernel not set");
+  const long long N = c->N;
+  HCHK(scale_pack(c->st, c->dens + sys * N, c->nhat + sys * N, nullptr, N, 1, c->pair));
+  HCHK(hartree_conv(c->g, c->scale, c->st, c->pair, c->pair, c->hwork, c->vk, c->tw64[c->gbig.nx],
+                    c->tw64[c->gbig.ny], c->tw64[c->gbig.nz]));
+  if (c->periodic) HCHK(pair_means(c->st, c->pair, N, 1, c->part, c->means));
+  HCHK(vks_build(c->st, c->dens + sys * N, c->ncoret, c->pair, c->periodic ? c->means : nullptr, c->vloc, N, 1,
+                 c->vxc + sys * N, c->vh + sys * N, c->vks + sys * N, c->vksf + sys * N));
+  return OPAW_OK;
+}
+
+int opaw_get_dij(opaw_ctx* c, int sys) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  if (!c->tables_ready) return fail(OPAW_ERR_STATE, "tables not finalised");
+  const long long N = c->N;
+  HCHK(dij(c->st, c->datoms, c->natom, c->rhoij + (size_t)sys * c->mtot, c->mtot, c->qtab, c->d0tab, c->etab, c->nl2,
+           c->sph_idx, c->sph_g, c->vks + sys * N, c->vxc + sys * N, c->usexcnhat == 0 ? 1 : 0, N, 1,
+           c->dx * c->dy * c->dz, c->dij + (size_t)sys * c->mtot));
+  return OPAW_OK;
+}
+
+int opaw_make_hamiltonian_tddft(opaw_ctx* c, int sys) {
+  CTX_CHECK(c);
+  RCHK(need_tables(c));
+  int s0 = 0, ns = c->nsys;
+  if (sys >= 0) {
+    RCHK(check_sys(c, sys));
+    s0 = sys;
+    ns = 1;
+  }
+  if (c->prec == OPAW_PREC_F64)
+    return flow_rebuild<double>(c, s0, ns, (const c64*)c->phi + (size_t)s0 * c->nb * c->N);
+  return flow_rebuild<float>(c, s0, ns, (const c32*)c->phi + (size_t)s0 * c->nb * c->N);
+}
+
+int opaw_excite_psi_bar_pert(opaw_ctx* c, int from_sys, int to_sys, double sm, int ipol) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, from_sys));
+  RCHK(check_sys(c, to_sys));
+  if (ipol < 1 || ipol > 3) return fail(OPAW_ERR_ARG, "ipol %d outside 1..3", ipol);
+  const double d = ipol == 1 ? c->dx : ipol == 2 ? c->dy : c->dz;
+  const size_t fs = (size_t)c->nb * c->N;
+  if (c->prec == OPAW_PREC_F64)
+    HCHK(kick<double>(c->st, (const c64*)c->phi + from_sys * fs, (c64*)c->phi + to_sys * fs, c->nx, c->ny, c->nz, c->nb,
+                      ipol - 1, d, sm));
+  else
+    HCHK(kick<float>(c->st, (const c32*)c->phi + from_sys * fs, (c32*)c->phi + to_sys * fs, c->nx, c->ny, c->nz, c->nb,
+                     ipol - 1, d, sm));
+  return OPAW_OK;
+}
+
+int opaw_rk4_prop_shs(opaw_ctx* c, double dt) {
+  CTX_CHECK(c);
+  RCHK(need_tables(c));
+  if (c->prec == OPAW_PREC_F64) return flow_rk4<double>(c, dt);
+  return flow_rk4<float>(c, dt);
+}
+
+static int dipoles_to(opaw_ctx* c, double* dst) {
+  prof_begin(c->st, KC_DENS);
+  if (c->prec == OPAW_PREC_F64)
+    HCHK(dipole<double>(c->st, (const c64*)c->phi, c->nx, c->ny, c->nz, c->nsys, c->nb, c->dx, c->dy, c->dz,
+                        c->part, dst));
+  else
+    HCHK(dipole<float>(c->st, (const c32*)c->phi, c->nx, c->ny, c->nz, c->nsys, c->nb, c->dx, c->dy, c->dz, c->part,
+                       dst));
+  prof_end(c->st, KC_DENS);
+  return OPAW_OK;
+}
+
+int opaw_plot_dip(opaw_ctx* c, double sm, double* d_dip, double* dip, double* dip_pert) {
+  CTX_CHECK(c);
+  if (c->dipcap < 1) {
+    dfree(c, c->dipbuf);
+    RCHK(dalloc_t(c, &c->dipbuf, (size_t)3 * c->nsys));
+    c->dipcap = 1;
+  }
+  RCHK(dipoles_to(c, c->dipbuf));
+  std::vector<double> h((size_t)3 * c->nsys);
+  HCHK(hipStreamSynchronize(c->st));
+  HCHK(hipMemcpy(h.data(), c->dipbuf, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int k = 0; k < 3; ++k) {
+    if (dip) dip[k] = h[k];
+    if (dip_pert) dip_pert[k] = c->nsys > 1 ? h[3 + k] : 0.0;
+    if (d_dip) d_dip[k] = c->nsys > 1 ? (h[3 + k] - h[k]) / sm : 0.0;
+  }
+  return OPAW_OK;
+}
+
+int opaw_tddft_run(opaw_ctx* c, int nt, double dt, double sm, double* d_dip, double* dip) {
+  CTX_CHECK(c);
+  RCHK(need_tables(c));
+  if (nt <= 0) return fail(OPAW_ERR_ARG, "nt must be positive");
+  if (c->dipcap < nt) {
+    dfree(c, c->dipbuf);
+    RCHK(dalloc_t(c, &c->dipbuf, (size_t)3 * c->nsys * nt));
+    c->dipcap = nt;
+  }
+  for (int it = 0; it < nt; ++it) {
+    RCHK(opaw_make_hamiltonian_tddft(c, -1));
+    RCHK(opaw_rk4_prop_shs(c, dt));
+    RCHK(dipoles_to(c, c->dipbuf + (size_t)3 * c->nsys * it));
+  }
+  if (d_dip || dip) {
+    std::vector<double> h((size_t)3 * c->nsys * nt);
+    HCHK(hipStreamSynchronize(c->st));
+    HCHK(hipMemcpy(h.data(), c->dipbuf, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int it = 0; it < nt; ++it) {
+      const double* r = h.data() + (size_t)3 * c->nsys * it;
+      if (dip)
+        for (int k = 0; k < 3 * c->nsys; ++k) dip[(size_t)3 * c->nsys * it + k] = r[k];
+      if (d_dip)
+        for (int k = 0; k < 3; ++k) d_dip[(size_t)3 * it + k] = c->nsys > 1 ? (r[3 + k] - r[k]) / sm : 0.0;
+    }
+  }
+  return OPAW_OK;
+}
+
+int opaw_orthog_phi(opaw_ctx* c, int sys, int h_type, double* eig) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  if (h_type == 0 && !c->tables_ready) return fail(OPAW_ERR_STATE, "tables not finalised");
+  if (c->prec == OPAW_PREC_F64) return flow_orthog<double>(c, sys, h_type, eig);
+  return flow_orthog<float>(c, sys, h_type, eig);
+}
+
+int opaw_diagh(opaw_ctx* c, int sys, int h_type, double* eig) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  if (c->prec == OPAW_PREC_F64) return flow_diagh<double>(c, sys, h_type, eig);
+  return flow_diagh<float>(c, sys, h_type, eig);
+}
+
+int opaw_profile_enable(opaw_ctx* c, int on) {
+  if (!c) return fail(OPAW_ERR_ARG, "null context");
+  c->prof.on = on != 0;
+  return OPAW_OK;
+}
+
+int opaw_profile_reset(opaw_ctx* c) {
+  if (!c) return fail(OPAW_ERR_ARG, "null context");
+  c->prof.harvest();
+  for (int k = 0; k < OPAW_K_COUNT; ++k) {
+    c->prof.total[k] = 0;
+    c->prof.count[k] = 0;
+  }
+  return OPAW_OK;
+}
+
+int opaw_profile_read(opaw_ctx* c, int kclass, double* total_ms, long long* launches) {
+  if (!c) return fail(OPAW_ERR_ARG, "null context");
+  if (kclass < 0 || kclass >= OPAW_K_COUNT) return fail(OPAW_ERR_ARG, "bad kernel class");
+  c->prof.harvest();
+  if (total_ms) *total_ms = c->prof.total[kclass];
+  if (launches) *launches = c->prof.count[kclass];
+  return OPAW_OK;
+}
+
+}  // extern "C"

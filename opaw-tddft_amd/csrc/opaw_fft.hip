@@ -1,0 +1,332 @@
+// Batched 3D FFT passes for the OPAW hot path (gfx950).
+//
+// A 3D transform is three passes of 1D line FFTs held in LDS:
+//   * x pass: a workgroup owns LINES whole x-lines (contiguous in HBM);
+//   * y / z pass: a workgroup owns XT consecutive x-columns of one y- or
+//     z-line family, read as XT-wide contiguous rows (coalesced).
+// Every pass addresses its input and output through a "view" (offsets and
+// leading dimensions), and loads / stores only a window along the transform
+// axis; outside the load window the line is zero.  That lets the same kernels
+// run the kinetic operator on the orbital grid and the Martyna-Tuckerman
+// Hartree convolution on the doubled box with the zero padding pruned (the
+// embedding of the density and the crop of V_H are folded into the first and
+// last passes).
+//
+// Passes may run in place (in == out): every tile is read completely before
+// the barrier that precedes its stores, and tiles do not overlap.
+//
+// Fused epilogues:
+//   EPI_MULT  (y/z pass): forward FFT, multiply by a real table T(k) (ek for
+//             the kinetic term, vk for Hartree), inverse FFT, store.
+//   EPI_HPSI  (x pass):   out = IFFT result + vks(sys) * u   (h_phi's local part)
+#include "fft_core.h"
+#include "opaw_kernels.h"
+
+namespace opaw {
+
+struct View {
+  long long fs;   // field stride (elements)
+  int sx, sy, sz; // coordinate offsets added before addressing
+  int ldx, ldy;   // leading dimensions
+};
+
+__device__ __forceinline__ long long vaddr(const View& v, int f, int x, int y, int z) {
+  return (long long)f * v.fs + (long long)(x + v.sx) +
+         (long long)v.ldx * ((long long)(y + v.sy) + (long long)v.ldy * (long long)(z + v.sz));
+}
+
+struct XLines {
+  int y0, ly, z0, lz, nf;   // line set: y in [y0,y0+ly), z in [z0,z0+lz), f < nf
+  int la, lb;               // load window along x
+  int sa, sb;               // store window along x
+};
+
+struct SLines {
+  int nx;                   // x extent of the line set (x in [0,nx))
+  int o0, lo, nf;           // other coordinate in [o0,o0+lo), f < nf
+  int la, lb, sa, sb;       // windows along the transform axis
+};
+
+enum { EPI_NONE = 0, EPI_MULT = 1, EPI_HPSI = 2 };
+
+constexpr int kThreads = 256;
+
+template <int N>
+constexpr int x_lines() { return (2048 / N) > 0 ? (2048 / N) : 1; }
+
+// ---------------------------------------------------------------------------
+// x pass
+template <typename R, int N, int S, int EPI>
+__global__ __launch_bounds__(kThreads) void k_fft_x(const cx<R>* in, View vin,
+                                                     cx<R>* out, View vout, XLines L,
+                                                     const cx<R>* __restrict__ twg,
+                                                     const cx<R>* __restrict__ u,
+                                                     const R* __restrict__ vks, int nf_per_sys,
+                                                     long long nfield) {
+  constexpr int LINES = x_lines<N>();
+  __shared__ cx<R> tile[LINES * N];
+  __shared__ cx<R> tw[N];
+  fft::load_twiddles(tw, twg, N);
+  const long long nlines = (long long)L.ly * L.lz * L.nf;
+  const long long line0 = (long long)blockIdx.x * LINES;
+  for (int idx = threadIdx.x; idx < LINES * N; idx += kThreads) {
+    const int l = idx / N, x = idx % N;
+    const long long gl = line0 + l;
+    cx<R> val = mk<R>(0, 0);
+    if (gl < nlines && x >= L.la && x < L.lb) {
+      const int y = L.y0 + (int)(gl % L.ly);
+      const long long r = gl / L.ly;
+      const int z = L.z0 + (int)(r % L.lz);
+      const int f = (int)(r / L.lz);
+      val = in[vaddr(vin, f, x, y, z)];
+    }
+    tile[idx] = val;
+  }
+  __syncthreads();
+  fft::lines<R, N, LINES, kThreads, false, N, 1, S>(tile, tw);
+  for (int idx = threadIdx.x; idx < LINES * N; idx += kThreads) {
+    const int l = idx / N, x = idx % N;
+    const long long gl = line0 + l;
+    if (gl < nlines && x >= L.sa && x < L.sb) {
+      const int y = L.y0 + (int)(gl % L.ly);
+      const long long r = gl / L.ly;
+      const int z = L.z0 + (int)(r % L.lz);
+      const int f = (int)(r / L.lz);
+      const long long o = vaddr(vout, f, x, y, z);
+      cx<R> val = tile[idx];
+      if constexpr (EPI == EPI_HPSI) {
+        const int sys = f / nf_per_sys;
+        const long long pt = o - (long long)f * vout.fs;
+        const R v = vks[(long long)sys * vout.fs + pt];
+        const cx<R> uu = u[o];
+        val = mk(val.x + v * uu.x, val.y + v * uu.y);
+      }
+      out[o] = val;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// y (AXIS=1) or z (AXIS=2) pass
+template <typename R, int N, int AXIS, int S, int EPI>
+__global__ __launch_bounds__(kThreads) void k_fft_s(const cx<R>* in, View vin,
+                                                     cx<R>* out, View vout, SLines L,
+                                                     const cx<R>* __restrict__ twg,
+                                                     const R* __restrict__ tab, View vtab) {
+  constexpr int XT = 16;
+  __shared__ cx<R> tile[XT * N];
+  __shared__ cx<R> tw[N];
+  fft::load_twiddles(tw, twg, N);
+  const int xtiles = (L.nx + XT - 1) / XT;
+  long long b = blockIdx.x;
+  const int xt = (int)(b % xtiles);
+  b /= xtiles;
+  const int o = L.o0 + (int)(b % L.lo);
+  const int f = (int)(b / L.lo);
+  const int x0 = xt * XT;
+  for (int idx = threadIdx.x; idx < XT * N; idx += kThreads) {
+    const int c = idx % XT, j = idx / XT;
+    const int x = x0 + c;
+    cx<R> val = mk<R>(0, 0);
+    if (x < L.nx && j >= L.la && j < L.lb) {
+      const long long a = (AXIS == 1) ? vaddr(vin, f, x, j, o) : vaddr(vin, f, x, o, j);
+      val = in[a];
+    }
+    tile[idx] = val;
+  }
+  __syncthreads();
+  fft::lines<R, N, XT, kThreads, true, 1, XT, S>(tile, tw);
+  if constexpr (EPI == EPI_MULT) {
+    for (int idx = threadIdx.x; idx < XT * N; idx += kThreads) {
+      const int c = idx % XT, j = idx / XT;
+      const int x = x0 + c;
+      if (x < L.nx) {
+        const long long a = (AXIS == 1) ? vaddr(vtab, 0, x, j, o) : vaddr(vtab, 0, x, o, j);
+        const R t = tab[a];
+        tile[idx] = mk(tile[idx].x * t, tile[idx].y * t);
+      }
+    }
+    __syncthreads();
+    fft::lines<R, N, XT, kThreads, true, 1, XT, -S>(tile, tw);
+  }
+  for (int idx = threadIdx.x; idx < XT * N; idx += kThreads) {
+    const int c = idx % XT, j = idx / XT;
+    const int x = x0 + c;
+    if (x < L.nx && j >= L.sa && j < L.sb) {
+      const long long a = (AXIS == 1) ? vaddr(vout, f, x, j, o) : vaddr(vout, f, x, o, j);
+      out[a] = tile[idx];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host-side dispatch over the compiled line lengths
+#define OPAW_FFT_SIZES(X) \
+  X(4) X(5) X(6) X(8) X(10) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) X(80) X(96) \
+  X(128) X(160) X(192) X(256)
+
+bool fft_size_supported(int n) {
+  switch (n) {
+#define OPAW_CASE(n) case n: return true;
+    OPAW_FFT_SIZES(OPAW_CASE)
+#undef OPAW_CASE
+    default: return false;
+  }
+}
+
+template <typename R, int S, int EPI>
+static hipError_t launch_x_n(int n, dim3 grid, hipStream_t st, const cx<R>* in, View vin, cx<R>* out,
+                             View vout, XLines L, const cx<R>* tw, const cx<R>* u, const R* vks,
+                             int nfps, long long nfield) {
+  switch (n) {
+#define OPAW_CASE(nn)                                                                   \
+  case nn:                                                                              \
+    hipLaunchKernelGGL((k_fft_x<R, nn, S, EPI>), grid, dim3(kThreads), 0, st, in, vin, \
+                       out, vout, L, tw, u, vks, nfps, nfield);                         \
+    return hipGetLastError();
+    OPAW_FFT_SIZES(OPAW_CASE)
+#undef OPAW_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename R, int AXIS, int S, int EPI>
+static hipError_t launch_s_n(int n, dim3 grid, hipStream_t st, const cx<R>* in, View vin, cx<R>* out,
+                             View vout, SLines L, const cx<R>* tw, const R* tab, View vtab) {
+  switch (n) {
+#define OPAW_CASE(nn)                                                                        \
+  case nn:                                                                                   \
+    hipLaunchKernelGGL((k_fft_s<R, nn, AXIS, S, EPI>), grid, dim3(kThreads), 0, st, in, vin, \
+                       out, vout, L, tw, tab, vtab);                                         \
+    return hipGetLastError();
+    OPAW_FFT_SIZES(OPAW_CASE)
+#undef OPAW_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+static int x_lines_rt(int n) {
+  int l = 2048 / n;
+  return l > 0 ? l : 1;
+}
+
+template <typename R>
+hipError_t fft_x_pass(int cls, int n, int dir, int epi, hipStream_t st, const cx<R>* in, const View& vin,
+                      cx<R>* out, const View& vout, const XLines& L, const cx<R>* tw,
+                      const cx<R>* u, const R* vks, int nfps) {
+  const long long nlines = (long long)L.ly * L.lz * L.nf;
+  if (nlines <= 0) return hipSuccess;
+  const int lines = x_lines_rt(n);
+  dim3 grid((unsigned)((nlines + lines - 1) / lines));
+  hipError_t e;
+  prof_begin(st, cls);
+  if (epi == EPI_HPSI)
+    e = launch_x_n<R, 1, EPI_HPSI>(n, grid, st, in, vin, out, vout, L, tw, u, vks, nfps, 0);
+  else if (dir < 0)
+    e = launch_x_n<R, -1, EPI_NONE>(n, grid, st, in, vin, out, vout, L, tw, u, vks, nfps, 0);
+  else
+    e = launch_x_n<R, 1, EPI_NONE>(n, grid, st, in, vin, out, vout, L, tw, u, vks, nfps, 0);
+  prof_end(st, cls);
+  return e;
+}
+
+template <typename R>
+hipError_t fft_s_pass(int cls, int n, int axis, int dir, int epi, hipStream_t st, const cx<R>* in,
+                      const View& vin, cx<R>* out, const View& vout, const SLines& L,
+                      const cx<R>* tw, const R* tab, const View& vtab) {
+  const long long ntile = (long long)((L.nx + 15) / 16) * L.lo * L.nf;
+  if (ntile <= 0) return hipSuccess;
+  dim3 grid((unsigned)ntile);
+  hipError_t e;
+  prof_begin(st, cls);
+  if (axis == 1) {
+    if (epi == EPI_MULT) e = launch_s_n<R, 1, -1, EPI_MULT>(n, grid, st, in, vin, out, vout, L, tw, tab, vtab);
+    else if (dir < 0) e = launch_s_n<R, 1, -1, EPI_NONE>(n, grid, st, in, vin, out, vout, L, tw, tab, vtab);
+    else e = launch_s_n<R, 1, 1, EPI_NONE>(n, grid, st, in, vin, out, vout, L, tw, tab, vtab);
+  } else {
+    if (epi == EPI_MULT) e = launch_s_n<R, 2, -1, EPI_MULT>(n, grid, st, in, vin, out, vout, L, tw, tab, vtab);
+    else if (dir < 0) e = launch_s_n<R, 2, -1, EPI_NONE>(n, grid, st, in, vin, out, vout, L, tw, tab, vtab);
+    else e = launch_s_n<R, 2, 1, EPI_NONE>(n, grid, st, in, vin, out, vout, L, tw, tab, vtab);
+  }
+  prof_end(st, cls);
+  return e;
+}
+
+// ---------------------------------------------------------------------------
+// Kinetic + local potential: out_f = IFFT(ek * FFT(u_f)) + vks(sys(f)) * u_f
+// (src/hphi.f90:14-21).  tmp must hold nf fields.
+template <typename R>
+hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* u, cx<R>* out, cx<R>* tmp,
+                          int nf, int nf_per_sys, const R* ek, const R* vks,
+                          const cx<R>* twx, const cx<R>* twy, const cx<R>* twz) {
+  const long long N = (long long)g.nx * g.ny * g.nz;
+  View v{N, 0, 0, 0, g.nx, g.ny};
+  XLines Lx{0, g.ny, 0, g.nz, nf, 0, g.nx, 0, g.nx};
+  SLines Ly{g.nx, 0, g.nz, nf, 0, g.ny, 0, g.ny};
+  SLines Lz{g.nx, 0, g.ny, nf, 0, g.nz, 0, g.nz};
+  View vt{0, 0, 0, 0, g.nx, g.ny};
+  hipError_t e;
+  if ((e = fft_x_pass<R>(KC_FFT_X, g.nx, -1, EPI_NONE, st, u, v, tmp, v, Lx, twx, nullptr, nullptr, 1))) return e;
+  if ((e = fft_s_pass<R>(KC_FFT_Y, g.ny, 1, -1, EPI_NONE, st, tmp, v, tmp, v, Ly, twy, nullptr, vt))) return e;
+  if ((e = fft_s_pass<R>(KC_FFT_Z, g.nz, 2, -1, EPI_MULT, st, tmp, v, tmp, v, Lz, twz, ek, vt))) return e;
+  if ((e = fft_s_pass<R>(KC_FFT_Y, g.ny, 1, 1, EPI_NONE, st, tmp, v, tmp, v, Ly, twy, nullptr, vt))) return e;
+  return fft_x_pass<R>(KC_FFT_XH, g.nx, 1, EPI_HPSI, st, tmp, v, out, v, Lx, twx, u, vks, nf_per_sys);
+}
+
+// Hartree convolution on the (scale*nx, scale*ny, scale*nz) box:
+//   out = crop( IFFT( vk * FFT( embed(rho) ) ) )
+// for one complex field rho (two real densities packed as re + i im; the
+// kernel vk is real and even so the two results do not mix).  work must hold
+// the doubled box.  Zero lines of the padded box are never touched.
+hipError_t hartree_conv(const GridGeom& g, int scale, hipStream_t st, const c64* rho, c64* out,
+                        c64* work, const double* vk, const c64* twx, const c64* twy,
+                        const c64* twz) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int NX = scale * nx, NY = scale * ny, NZ = scale * nz;
+  const int ox = NX / 2 - nx / 2, oy = NY / 2 - ny / 2, oz = NZ / 2 - nz / 2;
+  View small{0, -ox, -oy, -oz, nx, ny};      // small-grid array addressed in big coordinates
+  View big{0, 0, 0, 0, NX, NY};
+  hipError_t e;
+  // x forward: lines (y,z) inside the density support, x loaded from the small grid
+  XLines Lx1{oy, ny, oz, nz, 1, ox, ox + nx, 0, NX};
+  if ((e = fft_x_pass<double>(KC_HARTREE, NX, -1, EPI_NONE, st, rho, small, work, big, Lx1, twx, nullptr, nullptr, 1))) return e;
+  // y forward: z inside the support, y loaded only where nonzero
+  SLines Ly1{NX, oz, nz, 1, oy, oy + ny, 0, NY};
+  if ((e = fft_s_pass<double>(KC_HARTREE, NY, 1, -1, EPI_NONE, st, work, big, work, big, Ly1, twy, nullptr, big))) return e;
+  // z: forward, * vk, inverse; only the cropped z range is stored
+  SLines Lz{NX, 0, NY, 1, oz, oz + nz, oz, oz + nz};
+  if ((e = fft_s_pass<double>(KC_HARTREE, NZ, 2, -1, EPI_MULT, st, work, big, work, big, Lz, twz, vk, big))) return e;
+  // y inverse: z in crop, store cropped y
+  SLines Ly2{NX, oz, nz, 1, 0, NY, oy, oy + ny};
+  if ((e = fft_s_pass<double>(KC_HARTREE, NY, 1, 1, EPI_NONE, st, work, big, work, big, Ly2, twy, nullptr, big))) return e;
+  // x inverse: (y,z) in crop, store cropped x into the small grid
+  XLines Lx2{oy, ny, oz, nz, 1, 0, NX, ox, ox + nx};
+  return fft_x_pass<double>(KC_HARTREE, NX, 1, EPI_NONE, st, work, big, out, small, Lx2, twx, nullptr, nullptr, 1);
+}
+
+template hipError_t apply_t_plus_v<double>(const GridGeom&, hipStream_t, const c64*, c64*, c64*, int, int,
+                                           const double*, const double*, const c64*, const c64*, const c64*);
+template hipError_t apply_t_plus_v<float>(const GridGeom&, hipStream_t, const c32*, c32*, c32*, int, int,
+                                          const float*, const float*, const c32*, const c32*, const c32*);
+
+// plain 3D transform of nf fields (used by tests and the Hartree-free paths)
+template <typename R>
+hipError_t fft3d(const GridGeom& g, hipStream_t st, int dir, const cx<R>* in, cx<R>* out, int nf,
+                 const cx<R>* twx, const cx<R>* twy, const cx<R>* twz) {
+  const long long N = (long long)g.nx * g.ny * g.nz;
+  View v{N, 0, 0, 0, g.nx, g.ny};
+  View vt{0, 0, 0, 0, g.nx, g.ny};
+  XLines Lx{0, g.ny, 0, g.nz, nf, 0, g.nx, 0, g.nx};
+  SLines Ly{g.nx, 0, g.nz, nf, 0, g.ny, 0, g.ny};
+  SLines Lz{g.nx, 0, g.ny, nf, 0, g.nz, 0, g.nz};
+  hipError_t e;
+  if ((e = fft_x_pass<R>(KC_OTHER, g.nx, dir, EPI_NONE, st, in, v, out, v, Lx, twx, nullptr, nullptr, 1))) return e;
+  if ((e = fft_s_pass<R>(KC_OTHER, g.ny, 1, dir, EPI_NONE, st, out, v, out, v, Ly, twy, nullptr, vt))) return e;
+  return fft_s_pass<R>(KC_OTHER, g.nz, 2, dir, EPI_NONE, st, out, v, out, v, Lz, twz, nullptr, vt);
+}
+template hipError_t fft3d<double>(const GridGeom&, hipStream_t, int, const c64*, c64*, int, const c64*,
+                                  const c64*, const c64*);
+template hipError_t fft3d<float>(const GridGeom&, hipStream_t, int, const c32*, c32*, int, const c32*,
+                                 const c32*, const c32*);
+
+}  // namespace opaw

@@ -1,0 +1,53 @@
+// Declarations of the grid-field / small-matrix kernels (opaw_fields.hip).
+#pragma once
+#include "opaw_kernels.h"
+
+namespace opaw {
+
+constexpr int kChunks = 256;          // fixed chunk count of deterministic reductions
+constexpr int kGramChunks = 64;
+constexpr double kXcDensThreshold = 1e-15;  // per-point LDA density cutoff
+constexpr double kRhoijSelect = 1e-10;      // src/get_dij.f90:82
+constexpr double kDijMax = 500.0;           // src/0_parameter.f90 param_dij_max
+
+template <typename R>
+hipError_t rk4_update(hipStream_t st, int stage, long long n, const cx<R>* h, cx<R>* p, cx<R>* acc, cx<R>* y,
+                      double c, double dt);
+template <typename R>
+hipError_t field_norms(hipStream_t st, const cx<R>* u, long long N, int nf, double dv, double* part, double* out);
+template <typename R>
+hipError_t scale_fields(hipStream_t st, cx<R>* u, long long N, int nf, const double* num, const double* den);
+template <typename R>
+hipError_t density(hipStream_t st, const cx<R>* u, long long N, int nsys, int nb, double* dens);
+hipError_t fill(hipStream_t st, double* p, long long n, double v);
+hipError_t nhat(hipStream_t st, const AtomDesc* atoms, int natom, const double* rhoij, int mtot,
+                const double* qtab, int nl2, const int* sph_idx, const double* sph_g, long long N, int nsys,
+                double* nh);
+hipError_t renorm_factor(hipStream_t st, const double* nh, const double* dens, long long N, int nsys, double dv,
+                         double rnel, double* part, double* fac);
+hipError_t scale_pack(hipStream_t st, double* dens, double* nh, const double* fac, long long N, int nsys,
+                      c64* pair);
+hipError_t pair_means(hipStream_t st, const c64* pair, long long N, int nsys, double* part, double* means);
+hipError_t vks_build(hipStream_t st, const double* dens, const double* ncoret, const c64* vhpair,
+                     const double* vhmean, const double* vloc, long long N, int nsys, double* vxc, double* vh,
+                     double* vks, float* vksf);
+hipError_t rhoij(hipStream_t st, const AtomDesc* atoms, int natom, const c64* ca, int ctot, int nsys, int nb,
+                 int mtot, double* rho);
+hipError_t dij(hipStream_t st, const AtomDesc* atoms, int natom, const double* rho, int mtot, const double* qtab,
+               const double* d0tab, const double* etab, int nl2, const int* sph_idx, const double* sph_g,
+               const double* vks, const double* vxc, int subtract_vxc, long long N, int nsys, double dv,
+               double* d);
+template <typename R>
+hipError_t dipole(hipStream_t st, const cx<R>* u, int nx, int ny, int nz, int nsys, int nb, double dx, double dy,
+                  double dz, double* part, double* out);
+template <typename R>
+hipError_t kick(hipStream_t st, const cx<R>* src, cx<R>* dst, int nx, int ny, int nz, int nb, int ipol, double d,
+                double sm);
+template <typename R>
+hipError_t gram(hipStream_t st, const cx<R>* a, const cx<R>* b, long long N, int nb, double dv, c64* part, c64* g);
+template <typename R>
+hipError_t rotate(hipStream_t st, const cx<R>* in, cx<R>* out, long long N, int nb, const c64* T);
+template <typename R>
+hipError_t convert_to_c64(hipStream_t st, const cx<R>* in, c64* out, long long n);
+
+}  // namespace opaw

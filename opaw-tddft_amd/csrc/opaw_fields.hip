@@ -1,0 +1,670 @@
+// Grid-field and small-matrix kernels of the OPAW TDDFT hot path (gfx950).
+//
+// All reductions are deterministic: a fixed number of chunks per row, each
+// reduced by one workgroup (wave64 shuffles + LDS), then summed in chunk
+// order by a finishing kernel.  Densities, rho_ij, D_ij, norms and dipoles
+// accumulate in FP64 for both precisions.
+#include "opaw_fields.h"
+
+namespace opaw {
+
+constexpr int kT = 256;
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// block-wide sum of NV values per thread (result valid in thread 0)
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+  __shared__ double red[kT / 64][NV];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    const double s = wsum(v[i]);
+    if (lane == 0) red[w][i] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      double s = 0.0;
+      for (int k = 0; k < kT / 64; ++k) s += red[k][i];
+      v[i] = s;
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void chunk_range(long long n, int nchunk, int c, long long& b, long long& e) {
+  const long long per = (n + nchunk - 1) / nchunk;
+  b = (long long)c * per;
+  e = b + per < n ? b + per : n;
+}
+
+// ---------------------------------------------------------------------------
+// RK4 stage update (src/rk4.f90:35-56 and :22): k = -i h
+//   stage 1: acc = k;        y = p + c k
+//   stage 2,3: acc += 2k;    y = p + c k
+//   stage 4: p = p + dt/6 (acc + k)
+template <typename R>
+__global__ __launch_bounds__(kT) void k_rk4_update(int stage, long long n, const cx<R>* __restrict__ h,
+                                                    cx<R>* __restrict__ p, cx<R>* __restrict__ acc,
+                                                    cx<R>* __restrict__ y, double c, double dt6) {
+  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n; i += (long long)gridDim.x * kT) {
+    const cx<R> hh = h[i];
+    const double kr = (double)hh.y, ki = -(double)hh.x;
+    const cx<R> pp = p[i];
+    if (stage == 4) {
+      const cx<R> a = acc[i];
+      p[i] = mk<R>((R)((double)pp.x + dt6 * ((double)a.x + kr)), (R)((double)pp.y + dt6 * ((double)a.y + ki)));
+    } else {
+      if (stage == 1) {
+        acc[i] = mk<R>((R)kr, (R)ki);
+      } else {
+        const cx<R> a = acc[i];
+        acc[i] = mk<R>((R)((double)a.x + 2.0 * kr), (R)((double)a.y + 2.0 * ki));
+      }
+      y[i] = mk<R>((R)((double)pp.x + c * kr), (R)((double)pp.y + c * ki));
+    }
+  }
+}
+
+// per-field partial sums of |psi|^2
+template <typename R>
+__global__ __launch_bounds__(kT) void k_norm_partial(const cx<R>* __restrict__ u, long long N, int nchunk,
+                                                      double* __restrict__ part) {
+  const int c = blockIdx.x, f = blockIdx.y;
+  long long b, e;
+  chunk_range(N, nchunk, c, b, e);
+  const cx<R>* uf = u + (long long)f * N;
+  double v[1] = {0.0};
+  for (long long i = b + threadIdx.x; i < e; i += kT) {
+    const double x = uf[i].x, y = uf[i].y;
+    v[0] += x * x + y * y;
+  }
+  block_sum<1>(v);
+  if (threadIdx.x == 0) part[(long long)f * nchunk + c] = v[0];
+}
+
+// out[r] = scale * sum_c part[r*nchunk + c]  (then sqrt if want_sqrt)
+__global__ void k_finish_rows(const double* __restrict__ part, int rows, int nchunk, double scale,
+                              int want_sqrt, double* __restrict__ out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  double s = 0.0;
+  for (int c = 0; c < nchunk; ++c) s += part[(long long)r * nchunk + c];
+  s *= scale;
+  out[r] = want_sqrt ? sqrt(s) : s;
+}
+
+// psi_f *= num[f] / den[f]   (src/rk4.f90:92-102, norm1)
+template <typename R>
+__global__ __launch_bounds__(kT) void k_scale_fields(cx<R>* __restrict__ u, long long N, int nf,
+                                                      const double* __restrict__ num,
+                                                      const double* __restrict__ den) {
+  const long long n = N * nf;
+  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n; i += (long long)gridDim.x * kT) {
+    const int f = (int)(i / N);
+    const double s = num[f] / den[f];
+    const cx<R> v = u[i];
+    u[i] = mk<R>((R)((double)v.x * s), (R)((double)v.y * s));
+  }
+}
+
+// dens[s][r] = 2 sum_{o<nb} |psi_{s,o}(r)|^2   (src/tddft_dens.f90:65-76)
+template <typename R>
+__global__ __launch_bounds__(kT) void k_density(const cx<R>* __restrict__ u, long long N, int nsys, int nb,
+                                                 double* __restrict__ dens) {
+  const long long n = N * nsys;
+  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n; i += (long long)gridDim.x * kT) {
+    const int s = (int)(i / N);
+    const long long r = i - (long long)s * N;
+    const cx<R>* us = u + (long long)s * nb * N + r;
+    double acc = 0.0;
+    for (int o = 0; o < nb; ++o) {
+      const cx<R> v = us[(long long)o * N];
+      acc += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+    }
+    dens[i] = 2.0 * acc;
+  }
+}
+
+// compensation charge, one workgroup per system (src/nhat.f90:13-28)
+__global__ __launch_bounds__(kT) void k_nhat(const AtomDesc* __restrict__ atoms, int natom,
+                                              const double* __restrict__ rhoij, int mtot,
+                                              const double* __restrict__ qtab, int nl2,
+                                              const int* __restrict__ sph_idx,
+                                              const double* __restrict__ sph_g, long long N,
+                                              double* __restrict__ nhat) {
+  const int s = blockIdx.x;
+  double* nh = nhat + (long long)s * N;
+  const double* rho = rhoij + (long long)s * mtot;
+  __shared__ double wl[kMaxNl2];
+  for (int a = 0; a < natom; ++a) {
+    const AtomDesc A = atoms[a];
+    if (threadIdx.x < nl2) {
+      const int L = threadIdx.x;
+      const double* q = qtab + A.toff_q;
+      double w = 0.0;
+      for (int j = 0; j < A.ms; ++j)
+        for (int i = 0; i < A.ms; ++i) w += rho[A.moff + i + A.ms * j] * q[L + nl2 * (i + A.ms * j)];
+      wl[L] = w;
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < A.sph_n; p += kT) {
+      double v = 0.0;
+      for (int L = 0; L < nl2; ++L) v += sph_g[(long long)A.sph_off * nl2 + (long long)L * A.sph_n + p] * wl[L];
+      nh[sph_idx[A.sph_off + p]] += v;
+    }
+    __syncthreads();
+  }
+}
+
+// partial sums of two real fields per system (nhat and dens) for the rnel
+// renormalisation (src/nhat.f90:31-41)
+__global__ __launch_bounds__(kT) void k_sum2_partial(const double* __restrict__ a, const double* __restrict__ b,
+                                                      long long N, int nchunk, double* __restrict__ part) {
+  const int c = blockIdx.x, s = blockIdx.y;
+  long long lo, hi;
+  chunk_range(N, nchunk, c, lo, hi);
+  double v[2] = {0.0, 0.0};
+  for (long long i = lo + threadIdx.x; i < hi; i += kT) {
+    v[0] += a[(long long)s * N + i];
+    v[1] += b[(long long)s * N + i];
+  }
+  block_sum<2>(v);
+  if (threadIdx.x == 0) {
+    part[((long long)s * 2 + 0) * nchunk + c] = v[0];
+    part[((long long)s * 2 + 1) * nchunk + c] = v[1];
+  }
+}
+
+// fac[s] = rnel / (dv*sum nhat + dv*sum dens)
+__global__ void k_renorm_factor(const double* __restrict__ part, int nsys, int nchunk, double dv,
+                                double rnel, double* __restrict__ fac) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsys) return;
+  double a = 0.0, b = 0.0;
+  for (int c = 0; c < nchunk; ++c) a += part[((long long)s * 2 + 0) * nchunk + c];
+  for (int c = 0; c < nchunk; ++c) b += part[((long long)s * 2 + 1) * nchunk + c];
+  fac[s] = rnel / (a * dv + b * dv);
+}
+
+// dens, nhat *= fac (when fac != null); rho pair for the Hartree solve:
+//   pair[q] = (dens[2q]+nhat[2q]) + i (dens[2q+1]+nhat[2q+1])
+__global__ __launch_bounds__(kT) void k_scale_pack(double* __restrict__ dens, double* __restrict__ nhat,
+                                                    const double* __restrict__ fac, long long N, int nsys,
+                                                    c64* __restrict__ pair) {
+  const int npair = (nsys + 1) / 2;
+  const long long n = N * npair;
+  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n; i += (long long)gridDim.x * kT) {
+    const int q = (int)(i / N);
+    const long long r = i - (long long)q * N;
+    double v[2] = {0.0, 0.0};
+    for (int k = 0; k < 2; ++k) {
+      const int s = 2 * q + k;
+      if (s < nsys) {
+        const long long j = (long long)s * N + r;
+        double d = dens[j], h = nhat[j];
+        if (fac) {
+          d *= fac[s];
+          h *= fac[s];
+          dens[j] = d;
+          nhat[j] = h;
+        }
+        v[k] = d + h;
+      }
+    }
+    pair[i] = mk(v[0], v[1]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LDA: Slater exchange + Perdew-Wang 1992 correlation, unpolarised
+// (libxc ids 1 and 12 as called from src/tddft_pot.f90:163-177)
+__device__ __forceinline__ double lda_vxc(double n) {
+  if (!(n > kXcDensThreshold)) return 0.0;
+  const double pi = 3.14159265358979323846;
+  const double cx = cbrt(3.0 / pi);
+  const double n13 = cbrt(n);
+  const double vx = -cx * n13;
+  const double rs = cbrt(3.0 / (4.0 * pi * n));
+  const double A = 0.031091, a1 = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382, b4 = 0.49294;
+  const double srs = sqrt(rs);
+  const double q0 = -2.0 * A * (1.0 + a1 * rs);
+  const double q1 = 2.0 * A * (b1 * srs + b2 * rs + b3 * rs * srs + b4 * rs * rs);
+  const double dq1 = A * (b1 / srs + 2.0 * b2 + 3.0 * b3 * srs + 4.0 * b4 * rs);
+  const double lg = log1p(1.0 / q1);
+  const double ec = q0 * lg;
+  const double dec = -2.0 * A * a1 * lg - q0 * dq1 / (q1 * q1 + q1);
+  return vx + ec - rs / 3.0 * dec;
+}
+
+// vxc[s] = LDA(dens[s] + ncoret); vh[s] from the packed Hartree output
+// (minus its mean when periodic); vks[s] = vh + vxc + vloc
+// (src/tddft_pot.f90:130-149)
+__global__ __launch_bounds__(kT) void k_vks(const double* __restrict__ dens, const double* __restrict__ ncoret,
+                                             const c64* __restrict__ vhpair, const double* __restrict__ vhmean,
+                                             const double* __restrict__ vloc, long long N, int nsys,
+                                             double* __restrict__ vxc, double* __restrict__ vh,
+                                             double* __restrict__ vks, float* __restrict__ vksf) {
+  const long long n = N * nsys;
+  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n; i += (long long)gridDim.x * kT) {
+    const int s = (int)(i / N);
+    const long long r = i - (long long)s * N;
+    const double x = lda_vxc(dens[i] + ncoret[r]);
+    const c64 p = vhpair[(long long)(s / 2) * N + r];
+    double h = (s & 1) ? p.y : p.x;
+    if (vhmean) h -= vhmean[s];
+    const double v = h + x + vloc[r];
+    vxc[i] = x;
+    vh[i] = h;
+    vks[i] = v;
+    if (vksf) vksf[i] = (float)v;
+  }
+}
+
+// partial sums of the packed Hartree output (re -> system 2q, im -> 2q+1)
+__global__ __launch_bounds__(kT) void k_pair_sum_partial(const c64* __restrict__ pair, long long N, int nchunk,
+                                                          double* __restrict__ part) {
+  const int c = blockIdx.x, q = blockIdx.y;
+  long long lo, hi;
+  chunk_range(N, nchunk, c, lo, hi);
+  double v[2] = {0.0, 0.0};
+  for (long long i = lo + threadIdx.x; i < hi; i += kT) {
+    const c64 p = pair[(long long)q * N + i];
+    v[0] += p.x;
+    v[1] += p.y;
+  }
+  block_sum<2>(v);
+  if (threadIdx.x == 0) {
+    part[((long long)q * 2 + 0) * nchunk + c] = v[0];
+    part[((long long)q * 2 + 1) * nchunk + c] = v[1];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// rho_ij = 2 sum_o Re(conj(c_i) c_j)  (src/tddft_rhoij.f90:88-114)
+__global__ void k_rhoij(const AtomDesc* __restrict__ atoms, const c64* __restrict__ ca, int ctot, int nb,
+                        int mtot, double* __restrict__ rhoij) {
+  const int a = blockIdx.x, s = blockIdx.y;
+  const AtomDesc A = atoms[a];
+  for (int q = threadIdx.x; q < A.ms * A.ms; q += blockDim.x) {
+    const int i = q % A.ms, j = q / A.ms;
+    double acc = 0.0;
+    for (int o = 0; o < nb; ++o) {
+      const c64* c = ca + (long long)(s * nb + o) * ctot + A.coff;
+      acc += 2.0 * (c[i].x * c[j].x + c[i].y * c[j].y);
+    }
+    rhoij[(long long)s * mtot + A.moff + q] = acc;
+  }
+}
+
+// D_ij (src/get_dij.f90:1-156): on-site model dij0 + eijkl . rho_sel plus the
+// grid term sum_L q_ij^L dv sum_sph (vks - vxc) g_L (pawdijhat,
+// src/libpaw/4_m_pawdij.F90:2227-2400), then transfer_dij's packed
+// upper-triangle symmetric fill and +-param_dij_max clip.
+__global__ __launch_bounds__(64) void k_dij(const AtomDesc* __restrict__ atoms, const double* __restrict__ rhoij,
+                                             int mtot, const double* __restrict__ qtab,
+                                             const double* __restrict__ d0tab, const double* __restrict__ etab,
+                                             int nl2, const int* __restrict__ sph_idx,
+                                             const double* __restrict__ sph_g, const double* __restrict__ vks,
+                                             const double* __restrict__ vxc, int subtract_vxc, long long N,
+                                             double dv, double* __restrict__ dij) {
+  const int a = blockIdx.x, s = blockIdx.y;
+  const AtomDesc A = atoms[a];
+  __shared__ double prod[kMaxNl2];
+  __shared__ double M[kMaxMs * kMaxMs];
+  double acc[kMaxNl2];
+#pragma unroll
+  for (int L = 0; L < kMaxNl2; ++L) acc[L] = 0.0;
+  for (int p = threadIdx.x; p < A.sph_n; p += 64) {
+    const long long r = sph_idx[A.sph_off + p];
+    double pot = vks[(long long)s * N + r];
+    if (subtract_vxc) pot -= vxc[(long long)s * N + r];
+#pragma unroll
+    for (int L = 0; L < kMaxNl2; ++L)
+      if (L < nl2) acc[L] += pot * sph_g[(long long)A.sph_off * nl2 + (long long)L * A.sph_n + p];
+  }
+#pragma unroll
+  for (int L = 0; L < kMaxNl2; ++L) {
+    if (L < nl2) {
+      const double t = wsum(acc[L]);
+      if (threadIdx.x == 0) prod[L] = t * dv;
+    }
+  }
+  __syncthreads();
+  const double* rho = rhoij + (long long)s * mtot + A.moff;
+  const int ms = A.ms, m2 = ms * ms;
+  for (int q = threadIdx.x; q < m2; q += 64) {
+    const int i = q % ms, j = q / ms;
+    double d = d0tab[A.toff_d0 + q];
+    const double* e = etab + A.toff_e + (long long)q;   // E(i,j,k,l) at i + ms(j + ms(k + ms l))
+    for (int kl = 0; kl < m2; ++kl) {
+      const double rv = rho[kl];
+      if (fabs(rv) > kRhoijSelect) d += e[(long long)kl * m2] * rv;
+    }
+    const double* qq = qtab + A.toff_q;
+    for (int L = 0; L < nl2; ++L) d += qq[L + nl2 * (i + ms * j)] * prod[L];
+    M[q] = d;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < m2; q += 64) {
+    const int i = q % ms, j = q / ms;
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    double d = M[lo + ms * hi];
+    if (fabs(d) > kDijMax) d = d / fabs(d) * kDijMax;
+    dij[(long long)s * mtot + A.moff + q] = d;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// dipole partial sums (src/tddft_plot.f90:40-55, h_type=1 density 2 sum|phi|^2)
+template <typename R>
+__global__ __launch_bounds__(kT) void k_dipole_partial(const cx<R>* __restrict__ u, int nx, int ny, int nz,
+                                                        int nb, int nchunk, double dx, double dy, double dz,
+                                                        double* __restrict__ part) {
+  const int c = blockIdx.x, s = blockIdx.y;
+  const long long N = (long long)nx * ny * nz;
+  long long lo, hi;
+  chunk_range(N, nchunk, c, lo, hi);
+  double v[3] = {0.0, 0.0, 0.0};
+  const cx<R>* us = u + (long long)s * nb * N;
+  for (long long i = lo + threadIdx.x; i < hi; i += kT) {
+    double rho = 0.0;
+    for (int o = 0; o < nb; ++o) {
+      const cx<R> w = us[(long long)o * N + i];
+      rho += (double)w.x * (double)w.x + (double)w.y * (double)w.y;
+    }
+    rho *= 2.0;
+    const int ix = (int)(i % nx);
+    const long long t = i / nx;
+    const int iy = (int)(t % ny);
+    const int iz = (int)(t / ny);
+    v[0] += rho * ((ix - nx / 2) * dx);
+    v[1] += rho * ((iy - ny / 2) * dy);
+    v[2] += rho * ((iz - nz / 2) * dz);
+  }
+  block_sum<3>(v);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) part[((long long)s * 3 + k) * nchunk + c] = v[k];
+}
+
+// phi_pert = phi * exp(-i sm r_pol)  (src/tddft_excite.f90:8-15)
+template <typename R>
+__global__ __launch_bounds__(kT) void k_kick(const cx<R>* __restrict__ src, cx<R>* __restrict__ dst, int nx,
+                                              int ny, int nz, int nb, int ipol, double d, double sm) {
+  const long long N = (long long)nx * ny * nz;
+  const long long n = N * nb;
+  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n; i += (long long)gridDim.x * kT) {
+    const long long r = i % N;
+    int k;
+    int nn;
+    if (ipol == 0) {
+      k = (int)(r % nx);
+      nn = nx;
+    } else if (ipol == 1) {
+      k = (int)((r / nx) % ny);
+      nn = ny;
+    } else {
+      k = (int)(r / ((long long)nx * ny));
+      nn = nz;
+    }
+    const double th = sm * ((k - nn / 2) * d);
+    double sn, cs;
+    sincos(th, &sn, &cs);
+    const cx<R> v = src[i];
+    const double vr = v.x, vi = v.y;
+    // (vr + i vi)(cs - i sn)
+    dst[i] = mk<R>((R)(vr * cs + vi * sn), (R)(vi * cs - vr * sn));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Gram partials: G[c][i][j] = sum_{r in chunk c} conj(a_i(r)) b_j(r)
+// 16x16 tile of (i, j) per workgroup; points staged through LDS.
+template <typename R>
+__global__ __launch_bounds__(256) void k_gram_partial(const cx<R>* __restrict__ a, const cx<R>* __restrict__ b,
+                                                       long long N, int nb, int nchunk, c64* __restrict__ part) {
+  const int ti = blockIdx.x, tj = blockIdx.y, c = blockIdx.z;
+  const int li = threadIdx.x % 16, lj = threadIdx.x / 16;
+  const int i = ti * 16 + li, j = tj * 16 + lj;
+  long long lo, hi;
+  chunk_range(N, nchunk, c, lo, hi);
+  __shared__ c64 sa[16][17], sb[16][17];
+  double gr = 0.0, gi = 0.0;
+  for (long long r0 = lo; r0 < hi; r0 += 16) {
+    // load 16 points x 16 orbitals of a and b (orbital = lj row, point = li col)
+    {
+      const int o = ti * 16 + lj;
+      const long long r = r0 + li;
+      c64 v = mk(0.0, 0.0);
+      if (o < nb && r < hi) {
+        const cx<R> w = a[(long long)o * N + r];
+        v = mk((double)w.x, (double)w.y);
+      }
+      sa[lj][li] = v;
+      const int o2 = tj * 16 + lj;
+      c64 v2 = mk(0.0, 0.0);
+      if (o2 < nb && r < hi) {
+        const cx<R> w = b[(long long)o2 * N + r];
+        v2 = mk((double)w.x, (double)w.y);
+      }
+      sb[lj][li] = v2;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const c64 x = sa[li][k], y = sb[lj][k];
+      gr += x.x * y.x + x.y * y.y;
+      gi += x.x * y.y - x.y * y.x;
+    }
+    __syncthreads();
+  }
+  if (i < nb && j < nb) part[((long long)c * nb + i) * nb + j] = mk(gr, gi);
+}
+
+__global__ void k_gram_finish(const c64* __restrict__ part, int nb, int nchunk, double dv, c64* __restrict__ g) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nb * nb) return;
+  double sr = 0.0, si = 0.0;
+  for (int c = 0; c < nchunk; ++c) {
+    const c64 v = part[(long long)c * nb * nb + q];
+    sr += v.x;
+    si += v.y;
+  }
+  g[q] = mk(sr * dv, si * dv);
+}
+
+// out_j = sum_i T[i + nb*j] in_i   (nb x nb complex FP64 T, column-major)
+template <typename R>
+__global__ __launch_bounds__(kT) void k_rotate(const cx<R>* __restrict__ in, cx<R>* __restrict__ out, long long N,
+                                                int nb, const c64* __restrict__ T) {
+  constexpr int JT = 8;
+  const int j0 = blockIdx.y * JT;
+  const long long r = (long long)blockIdx.x * kT + threadIdx.x;
+  if (r >= N) return;
+  double ar[JT], ai[JT];
+#pragma unroll
+  for (int q = 0; q < JT; ++q) ar[q] = ai[q] = 0.0;
+  for (int i = 0; i < nb; ++i) {
+    const cx<R> w = in[(long long)i * N + r];
+    const double wr = w.x, wi = w.y;
+#pragma unroll
+    for (int q = 0; q < JT; ++q) {
+      if (j0 + q < nb) {
+        const c64 t = T[i + (long long)nb * (j0 + q)];
+        ar[q] += wr * t.x - wi * t.y;
+        ai[q] += wr * t.y + wi * t.x;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < JT; ++q)
+    if (j0 + q < nb) out[(long long)(j0 + q) * N + r] = mk<R>((R)ar[q], (R)ai[q]);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kT) void k_convert(const cx<R>* __restrict__ in, c64* __restrict__ out, long long n) {
+  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n; i += (long long)gridDim.x * kT)
+    out[i] = mk((double)in[i].x, (double)in[i].y);
+}
+
+__global__ __launch_bounds__(kT) void k_fill(double* __restrict__ p, long long n, double v) {
+  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n; i += (long long)gridDim.x * kT) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// host wrappers
+static unsigned grid_for(long long n) {
+  long long b = (n + kT - 1) / kT;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+template <typename R>
+hipError_t rk4_update(hipStream_t st, int stage, long long n, const cx<R>* h, cx<R>* p, cx<R>* acc, cx<R>* y,
+                      double c, double dt) {
+  hipLaunchKernelGGL((k_rk4_update<R>), dim3(grid_for(n)), dim3(kT), 0, st, stage, n, h, p, acc, y, c, dt / 6.0);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t field_norms(hipStream_t st, const cx<R>* u, long long N, int nf, double dv, double* part, double* out) {
+  hipLaunchKernelGGL((k_norm_partial<R>), dim3(kChunks, nf), dim3(kT), 0, st, u, N, kChunks, part);
+  hipLaunchKernelGGL(k_finish_rows, dim3((nf + 63) / 64), dim3(64), 0, st, part, nf, kChunks, dv, 1, out);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t scale_fields(hipStream_t st, cx<R>* u, long long N, int nf, const double* num, const double* den) {
+  hipLaunchKernelGGL((k_scale_fields<R>), dim3(grid_for(N * nf)), dim3(kT), 0, st, u, N, nf, num, den);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t density(hipStream_t st, const cx<R>* u, long long N, int nsys, int nb, double* dens) {
+  hipLaunchKernelGGL((k_density<R>), dim3(grid_for(N * nsys)), dim3(kT), 0, st, u, N, nsys, nb, dens);
+  return hipGetLastError();
+}
+
+hipError_t fill(hipStream_t st, double* p, long long n, double v) {
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kT), 0, st, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t nhat(hipStream_t st, const AtomDesc* atoms, int natom, const double* rhoij, int mtot,
+                const double* qtab, int nl2, const int* sph_idx, const double* sph_g, long long N, int nsys,
+                double* nh) {
+  hipError_t e = fill(st, nh, N * nsys, 0.0);
+  if (e) return e;
+  hipLaunchKernelGGL(k_nhat, dim3(nsys), dim3(kT), 0, st, atoms, natom, rhoij, mtot, qtab, nl2, sph_idx, sph_g,
+                     N, nh);
+  return hipGetLastError();
+}
+
+hipError_t renorm_factor(hipStream_t st, const double* nh, const double* dens, long long N, int nsys, double dv,
+                         double rnel, double* part, double* fac) {
+  hipLaunchKernelGGL(k_sum2_partial, dim3(kChunks, nsys), dim3(kT), 0, st, nh, dens, N, kChunks, part);
+  hipLaunchKernelGGL(k_renorm_factor, dim3(1), dim3(64), 0, st, part, nsys, kChunks, dv, rnel, fac);
+  return hipGetLastError();
+}
+
+hipError_t scale_pack(hipStream_t st, double* dens, double* nh, const double* fac, long long N, int nsys,
+                      c64* pair) {
+  hipLaunchKernelGGL(k_scale_pack, dim3(grid_for(N * ((nsys + 1) / 2))), dim3(kT), 0, st, dens, nh, fac, N, nsys,
+                     pair);
+  return hipGetLastError();
+}
+
+hipError_t pair_means(hipStream_t st, const c64* pair, long long N, int nsys, double* part, double* means) {
+  const int npair = (nsys + 1) / 2;
+  hipLaunchKernelGGL(k_pair_sum_partial, dim3(kChunks, npair), dim3(kT), 0, st, pair, N, kChunks, part);
+  hipLaunchKernelGGL(k_finish_rows, dim3(1), dim3(64), 0, st, part, 2 * npair, kChunks, 1.0 / (double)N, 0, means);
+  return hipGetLastError();
+}
+
+hipError_t vks_build(hipStream_t st, const double* dens, const double* ncoret, const c64* vhpair,
+                     const double* vhmean, const double* vloc, long long N, int nsys, double* vxc, double* vh,
+                     double* vks, float* vksf) {
+  hipLaunchKernelGGL(k_vks, dim3(grid_for(N * nsys)), dim3(kT), 0, st, dens, ncoret, vhpair, vhmean, vloc, N, nsys,
+                     vxc, vh, vks, vksf);
+  return hipGetLastError();
+}
+
+hipError_t rhoij(hipStream_t st, const AtomDesc* atoms, int natom, const c64* ca, int ctot, int nsys, int nb,
+                 int mtot, double* rho) {
+  if (natom == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rhoij, dim3(natom, nsys), dim3(64), 0, st, atoms, ca, ctot, nb, mtot, rho);
+  return hipGetLastError();
+}
+
+hipError_t dij(hipStream_t st, const AtomDesc* atoms, int natom, const double* rho, int mtot, const double* qtab,
+               const double* d0tab, const double* etab, int nl2, const int* sph_idx, const double* sph_g,
+               const double* vks, const double* vxc, int subtract_vxc, long long N, int nsys, double dv,
+               double* d) {
+  if (natom == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dij, dim3(natom, nsys), dim3(64), 0, st, atoms, rho, mtot, qtab, d0tab, etab, nl2, sph_idx,
+                     sph_g, vks, vxc, subtract_vxc, N, dv, d);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t dipole(hipStream_t st, const cx<R>* u, int nx, int ny, int nz, int nsys, int nb, double dx, double dy,
+                  double dz, double* part, double* out) {
+  hipLaunchKernelGGL((k_dipole_partial<R>), dim3(kChunks, nsys), dim3(kT), 0, st, u, nx, ny, nz, nb, kChunks, dx, dy,
+                     dz, part);
+  hipLaunchKernelGGL(k_finish_rows, dim3(1), dim3(64), 0, st, part, 3 * nsys, kChunks, dx * dy * dz, 0, out);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t kick(hipStream_t st, const cx<R>* src, cx<R>* dst, int nx, int ny, int nz, int nb, int ipol, double d,
+                double sm) {
+  hipLaunchKernelGGL((k_kick<R>), dim3(grid_for((long long)nx * ny * nz * nb)), dim3(kT), 0, st, src, dst, nx, ny,
+                     nz, nb, ipol, d, sm);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t gram(hipStream_t st, const cx<R>* a, const cx<R>* b, long long N, int nb, double dv, c64* part, c64* g) {
+  const int t = (nb + 15) / 16;
+  hipLaunchKernelGGL((k_gram_partial<R>), dim3(t, t, kGramChunks), dim3(256), 0, st, a, b, N, nb, kGramChunks,
+                     part);
+  hipLaunchKernelGGL(k_gram_finish, dim3((nb * nb + 255) / 256), dim3(256), 0, st, part, nb, kGramChunks, dv, g);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t rotate(hipStream_t st, const cx<R>* in, cx<R>* out, long long N, int nb, const c64* T) {
+  hipLaunchKernelGGL((k_rotate<R>), dim3((unsigned)((N + kT - 1) / kT), (nb + 7) / 8), dim3(kT), 0, st, in, out, N,
+                     nb, T);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t convert_to_c64(hipStream_t st, const cx<R>* in, c64* out, long long n) {
+  hipLaunchKernelGGL((k_convert<R>), dim3(grid_for(n)), dim3(kT), 0, st, in, out, n);
+  return hipGetLastError();
+}
+
+#define OPAW_INST(R)                                                                                            \
+  template hipError_t rk4_update<R>(hipStream_t, int, long long, const cx<R>*, cx<R>*, cx<R>*, cx<R>*, double,  \
+                                    double);                                                                    \
+  template hipError_t field_norms<R>(hipStream_t, const cx<R>*, long long, int, double, double*, double*);      \
+  template hipError_t scale_fields<R>(hipStream_t, cx<R>*, long long, int, const double*, const double*);       \
+  template hipError_t density<R>(hipStream_t, const cx<R>*, long long, int, int, double*);                      \
+  template hipError_t dipole<R>(hipStream_t, const cx<R>*, int, int, int, int, int, double, double, double,     \
+                                double*, double*);                                                              \
+  template hipError_t kick<R>(hipStream_t, const cx<R>*, cx<R>*, int, int, int, int, int, double, double);      \
+  template hipError_t gram<R>(hipStream_t, const cx<R>*, const cx<R>*, long long, int, double, c64*, c64*);     \
+  template hipError_t rotate<R>(hipStream_t, const cx<R>*, cx<R>*, long long, int, const c64*);                 \
+  template hipError_t convert_to_c64<R>(hipStream_t, const cx<R>*, c64*, long long);
+OPAW_INST(double)
+OPAW_INST(float)
+
+}  // namespace opaw

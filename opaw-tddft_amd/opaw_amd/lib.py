@@ -1,0 +1,118 @@
+"""ctypes binding of libopaw_hip.so (the C ABI declared in include/opaw_hip.h).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libopaw_hip.so"
+
+OPAW_OK = 0
+OPAW_PREC_F64 = 0
+OPAW_PREC_MIXED = 1
+FIELD = dict(dens=0, nhat=1, vxc=2, vh=3, vks=4)
+KCLASS = dict(fft_x=0, fft_y=1, fft_z=2, fft_xh=3, hartree=4, box_proj=5, box_add=6, rk4=7,
+              dens=8, pot=9, other=10)
+
+P = C.c_void_p
+I = C.c_int
+D = C.c_double
+DP = C.POINTER(C.c_double)
+IP = C.POINTER(C.c_int)
+LLP = C.POINTER(C.c_longlong)
+
+# name: (restype, [argtypes])
+SIGNATURES = {
+    "opaw_last_error": (C.c_char_p, []),
+    "opaw_version": (I, []),
+    "opaw_fft_length_supported": (I, [I]),
+    "opaw_create": (I, [C.POINTER(P), I, I, I, D, D, D, I, I, I, I]),
+    "opaw_destroy": (I, [P]),
+    "opaw_set_stream": (I, [P, P]),
+    "opaw_synchronize": (I, [P]),
+    "opaw_device_bytes": (C.c_longlong, [P]),
+    "opaw_ek_prep": (I, [P, D]),
+    "opaw_set_ek": (I, [P, DP]),
+    "opaw_set_atoms": (I, [P, I, IP, IP, I, IP, IP, I]),
+    "opaw_set_projectors": (I, [P, I, DP, DP, DP, DP]),
+    "opaw_set_sphere": (I, [P, I, I, IP, DP]),
+    "opaw_set_type": (I, [P, I, DP, DP, DP]),
+    "opaw_finalize_tables": (I, [P]),
+    "opaw_set_potentials": (I, [P, DP, DP, D, I]),
+    "opaw_vk_prep": (I, [P]),
+    "opaw_write_vks": (I, [P, I, DP]),
+    "opaw_read_vks": (I, [P, I, DP]),
+    "opaw_write_dij": (I, [P, I, I, DP]),
+    "opaw_read_dij": (I, [P, I, I, DP]),
+    "opaw_read_rhoij": (I, [P, I, I, DP]),
+    "opaw_read_field": (I, [P, I, I, DP]),
+    "opaw_write_orbitals": (I, [P, I, P]),
+    "opaw_read_orbitals": (I, [P, I, P]),
+    "opaw_orbitals_device": (P, [P, I]),
+    "opaw_h_phi": (I, [P, I, P, P, I]),
+    "opaw_sn_phi": (I, [P, P, P, I, D]),
+    "opaw_shs": (I, [P, I, P, P, I]),
+    "opaw_proj": (I, [P, P, I, I, P]),
+    "opaw_fft3d": (I, [P, I, P, P, I]),
+    "opaw_tddft_rhoij_2": (I, [P, I, P]),
+    "opaw_tddft_dens_2": (I, [P, I, P]),
+    "opaw_get_nhat": (I, [P, I]),
+    "opaw_get_pot_tddft_2": (I, [P, I]),
+    "opaw_get_dij": (I, [P, I]),
+    "opaw_make_hamiltonian_tddft": (I, [P, I]),
+    "opaw_excite_psi_bar_pert": (I, [P, I, I, D, I]),
+    "opaw_rk4_prop_shs": (I, [P, D]),
+    "opaw_plot_dip": (I, [P, D, DP, DP, DP]),
+    "opaw_tddft_run": (I, [P, I, D, D, DP, DP]),
+    "opaw_orthog_phi": (I, [P, I, I, DP]),
+    "opaw_diagh": (I, [P, I, I, DP]),
+    "opaw_profile_enable": (I, [P, I]),
+    "opaw_profile_reset": (I, [P]),
+    "opaw_profile_read": (I, [P, I, DP, LLP]),
+}
+
+
+class OpawError(RuntimeError):
+    def __init__(self, fn: str, code: int, msg: str):
+        super().__init__(f"{fn} failed ({code}): {msg}")
+        self.code = code
+
+
+_LIB = None
+
+
+def load(path: Path | None = None):
+    """Load (once) and type the shared library; raises if it is missing."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise OpawError("load", -99, f"{p} not built: run __graft_entry__.build() "
+                        "(python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def last_error() -> str:
+    msg = load().opaw_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(fn: str, code: int):
+    if code != OPAW_OK:
+        raise OpawError(fn, code, last_error())
+    return code
+
+
+def call(name: str, *args):
+    """Call an int-returning entry point and raise on a nonzero status."""
+    return check(name, getattr(load(), name)(*args))

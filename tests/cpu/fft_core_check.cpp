@@ -1,0 +1,74 @@
+// Host-side check of the Stockham stage index logic in fft_core.h: runs the
+// gather/scatter halves for every emulated thread of a tile and prints the
+// transformed tile so the test can compare it with numpy.fft.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+#include "fft_core.h"
+
+using namespace opaw;
+using namespace opaw::fft;
+
+template <int N, int NS, int LINES, int THREADS, bool LFAST, int LSTR, int ESTR, int S>
+void host_stages(cx<double>* lds, const cx<double>* tw) {
+  if constexpr (NS < N) {
+    constexpr int RAD = pick_radix(N / NS);
+    using St = Stage<double, N, RAD, NS, LINES, THREADS, LFAST, LSTR, ESTR, S>;
+    std::vector<std::array<std::array<cx<double>, RAD>, St::K>> regs(THREADS);
+    for (int t = 0; t < THREADS; ++t) {
+      cx<double> v[St::K][RAD];
+      St::gather(lds, v, t);
+      for (int q = 0; q < St::K; ++q)
+        for (int r = 0; r < RAD; ++r) regs[t][q][r] = v[q][r];
+    }
+    for (int t = 0; t < THREADS; ++t) {
+      cx<double> v[St::K][RAD];
+      for (int q = 0; q < St::K; ++q)
+        for (int r = 0; r < RAD; ++r) v[q][r] = regs[t][q][r];
+      St::scatter(lds, v, tw, t);
+    }
+    host_stages<N, NS * RAD, LINES, THREADS, LFAST, LSTR, ESTR, S>(lds, tw);
+  }
+}
+
+template <int N, int LINES, bool LFAST, int S>
+void run() {
+  constexpr int THREADS = 64;
+  constexpr int LSTR = LFAST ? 1 : N;
+  constexpr int ESTR = LFAST ? LINES : 1;
+  std::vector<cx<double>> tw(N), lds(N * LINES);
+  for (int m = 0; m < N; ++m) {
+    long double a = -2.0L * 3.14159265358979323846264338327950288L * m / N;
+    tw[m] = mk<double>((double)std::cos(a), (double)std::sin(a));
+  }
+  // deterministic input: element (l, j)
+  for (int l = 0; l < LINES; ++l)
+    for (int j = 0; j < N; ++j)
+      lds[l * LSTR + j * ESTR] = mk<double>(std::sin(0.37 * (j + 1) + l), std::cos(0.11 * j * j + 0.5 * l));
+  host_stages<N, 1, LINES, THREADS, LFAST, LSTR, ESTR, S>(lds.data(), tw.data());
+  for (int l = 0; l < LINES; ++l)
+    for (int j = 0; j < N; ++j)
+      std::printf("%d %d %d %d %.17e %.17e\n", N, S, l, j, lds[l * LSTR + j * ESTR].x, lds[l * LSTR + j * ESTR].y);
+}
+
+#include <array>
+int main() {
+  run<8, 3, false, -1>();
+  run<12, 5, true, -1>();
+  run<16, 2, false, 1>();
+  run<20, 4, true, 1>();
+  run<48, 3, false, -1>();
+  run<64, 4, true, -1>();
+  run<96, 2, false, 1>();
+  run<10, 6, false, -1>();
+  run<5, 3, true, 1>();
+  run<6, 2, false, -1>();
+  run<128, 2, true, -1>();
+  run<192, 1, false, 1>();
+  run<256, 1, false, -1>();
+  run<40, 2, true, -1>();
+  run<80, 2, false, 1>();
+  run<160, 1, true, -1>();
+  return 0;
+}

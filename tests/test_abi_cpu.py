@@ -1,0 +1,60 @@
+"""CPU checks of the C ABI: the library loads and exports every entry point
+declared in include/opaw_hip.h, and the ctypes table matches the header.  No
+compute call is made (no GPU here)."""
+import os
+import re
+import subprocess
+
+import pytest
+
+from opaw_amd import lib
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, "include", "opaw_hip.h")
+
+
+def declared():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    out = {}
+    for m in re.finditer(r"^\s*([A-Za-z_][\w\s\*]*?)\b(opaw_\w+)\s*\(([^)]*)\)\s*;", src, flags=re.M):
+        args = [a for a in m.group(3).split(",") if a.strip() and a.strip() != "void"]
+        out[m.group(2)] = len(args)
+    return out
+
+
+def test_header_declares_the_api():
+    d = declared()
+    assert len(d) >= 40
+    for name in ("opaw_create", "opaw_h_phi", "opaw_shs", "opaw_sn_phi", "opaw_rk4_prop_shs",
+                 "opaw_tddft_run", "opaw_get_dij", "opaw_orthog_phi", "opaw_diagh"):
+        assert name in d
+
+
+def test_ctypes_table_matches_header():
+    d = declared()
+    assert set(d) == set(lib.SIGNATURES)
+    for name, nargs in d.items():
+        assert len(lib.SIGNATURES[name][1]) == nargs, name
+
+
+def test_library_loads_and_exports_every_symbol():
+    if not lib.LIB_PATH.exists():
+        pytest.fail(f"{lib.LIB_PATH} not built (run __graft_entry__.build())")
+    L = lib.load()
+    for name in declared():
+        assert hasattr(L, name), name
+    nm = subprocess.run(["nm", "-D", "--defined-only", str(lib.LIB_PATH)], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (opaw_\w+)", nm))
+    assert set(declared()) <= exported
+    assert L.opaw_version() >= 10000
+    # compiled FFT lengths cover every grid of the workloads (and their MT doubles)
+    for n in (48, 64, 96, 128, 192, 256, 32, 12, 10, 8):
+        assert L.opaw_fft_length_supported(n) == 1, n
+    assert L.opaw_fft_length_supported(7) == 0
+
+
+def test_null_context_is_an_error_not_a_crash():
+    L = lib.load()
+    assert L.opaw_synchronize(None) == -1
+    assert b"null" in L.opaw_last_error()

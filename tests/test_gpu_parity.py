@@ -1,0 +1,255 @@
+"""GPU parity of the HIP hot path against the CPU oracle, through the C ABI.
+
+Tolerances (FP64 path): H.psi, S^n psi, shs and projections <= 1e-10
+relative to the largest reference entry; rebuilt fields and D_ij <= 1e-10;
+one RK4 step <= 1e-10; dipoles over a few steps <= 1e-8 relative.  The mixed
+path (FP32 orbitals/FFTs, FP64 accumulations) is held to <= 2e-5 for a single
+operator application."""
+import numpy as np
+import pytest
+
+from tests.helpers import O, rel, state_from_spec
+from opaw_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _api():
+    import torch
+    from opaw_amd.api import OpawGPU
+    assert torch.cuda.is_available(), "GPU test without a GPU"
+    return torch, OpawGPU
+
+
+def _random_potentials(st, spec, seed):
+    rng = np.random.default_rng(seed)
+    for s, S in enumerate(st.systems):
+        S.vks = spec.vloc + rng.uniform(-0.3, 0.3, spec.n)
+        S.dij = []
+        for a in spec.atoms:
+            m = rng.uniform(-1, 1, (a.ms, a.ms))
+            S.dij.append(0.5 * (m + m.T))
+
+
+def _push_potentials(g, st):
+    for s, S in enumerate(st.systems):
+        g.write_vks(s, S.vks)
+        for ia, d in enumerate(S.dij, start=1):
+            g.write_dij(s, ia, d)
+
+
+@pytest.fixture(scope="module")
+def mol():
+    spec = synth.molecule_system(nring=1, n=(32, 32, 32), d=0.5, seed=11)
+    nb = 4
+    spec.rnel = 2.0 * nb
+    st = state_from_spec(spec, nsys=2)
+    _random_potentials(st, spec, 1)
+    phi = synth.lowdin_orthonormalise(synth.random_orbitals(spec, nb, seed=3, envelope=3.0), spec.dv)
+    return spec, st, phi
+
+
+@pytest.fixture(scope="module")
+def per():
+    spec = synth.hpsi_system(n=(32, 32, 40), d=0.4, natom=6, seed=5)
+    nb = 3
+    st = state_from_spec(spec, nsys=2)
+    _random_potentials(st, spec, 2)
+    phi = synth.lowdin_orthonormalise(synth.random_orbitals(spec, nb, seed=4), spec.dv)
+    return spec, st, phi
+
+
+def _gpu(spec, st, nb, prec="f64"):
+    torch, OpawGPU = _api()
+    g = OpawGPU(spec, prec, nsys=2, nb=nb)
+    _push_potentials(g, st)
+    return torch, g
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("shape", [(12, 10, 8), (32, 32, 32), (20, 24, 16), (48, 40, 64)])
+@pytest.mark.parametrize("prec", ["f64", "mixed"])
+def test_fft3d_matches_numpy(shape, prec):
+    torch, OpawGPU = _api()
+    nx, ny, nz = shape
+    spec = synth.hpsi_system(n=shape, d=0.4, natom=0, seed=1)
+    g = OpawGPU(spec, prec, nsys=1, nb=2)
+    rng = np.random.default_rng(0)
+    f = rng.standard_normal((2, spec.n)) + 1j * rng.standard_normal((2, spec.n))
+    x = g.to_device(f)
+    fw = g.fft3d(x, -1)
+    bw = g.fft3d(x, +1)
+    g.synchronize()
+    ref_f = np.fft.fftn(f.reshape(2, nz, ny, nx), axes=(1, 2, 3)).reshape(2, -1)
+    ref_b = (np.fft.ifftn(f.reshape(2, nz, ny, nx), axes=(1, 2, 3)) * spec.n).reshape(2, -1)
+    tol = 1e-13 if prec == "f64" else 2e-6
+    assert rel(fw.cpu().numpy(), ref_f) < tol
+    assert rel(bw.cpu().numpy(), ref_b) < tol
+    g.close()
+
+
+@pytest.mark.parametrize("case", ["mol", "per"])
+def test_h_phi_matches_oracle(case, request):
+    spec, st, phi = request.getfixturevalue(case)
+    torch, g = _gpu(spec, st, phi.shape[0])
+    x = g.to_device(phi)
+    for sys in (0, 1):
+        out = g.h_phi(x, sys).cpu().numpy()
+        ref = O.h_phi(st, phi, sys)
+        assert rel(out, ref) < 1e-10, (case, sys)
+    g.close()
+
+
+@pytest.mark.parametrize("case", ["mol", "per"])
+@pytest.mark.parametrize("n", [-1.0, -0.5, 0.5, 1.0, 0.3])
+def test_sn_phi_matches_oracle(case, n, request):
+    spec, st, phi = request.getfixturevalue(case)
+    torch, g = _gpu(spec, st, phi.shape[0])
+    out = g.sn_phi(g.to_device(phi), n).cpu().numpy()
+    assert rel(out, O.sn_phi(st, phi, n)) < 1e-12
+    g.close()
+
+
+@pytest.mark.parametrize("case", ["mol", "per"])
+def test_shs_and_proj_match_oracle(case, request):
+    spec, st, phi = request.getfixturevalue(case)
+    torch, g = _gpu(spec, st, phi.shape[0])
+    x = g.to_device(phi)
+    for sys in (0, 1):
+        assert rel(g.shs(x, sys).cpu().numpy(), O.shs(st, phi, sys)) < 1e-10
+    ca = g.proj(x).cpu().numpy()
+    ca1 = g.proj1(x).cpu().numpy()
+    for ia in range(len(spec.atoms)):
+        sl = slice(g.coff[ia], g.coff[ia] + spec.atoms[ia].ms)
+        assert rel(ca[:, sl], O.proj(st, ia, phi)) < 1e-11
+        assert rel(ca1[:, sl], O.proj1(st, ia, phi)) < 1e-11
+    g.close()
+
+
+def test_rebuild_matches_oracle(mol):
+    spec, st0, phi = mol
+    st = state_from_spec(spec, nsys=2)
+    _random_potentials(st, spec, 1)
+    phis = [phi, O.excite_psi_bar_pert(st, phi, spec.sm, spec.ipol)]
+    torch, g = _gpu(spec, st, phi.shape[0])
+    g.write_orbitals(0, phi)
+    g.excite_psi_bar_pert(spec.sm, spec.ipol + 1)
+    assert rel(g.read_orbitals(1), phis[1]) < 1e-14
+    g.ncpaw_make_hamiltonian_tddft(-1)
+    for s in (0, 1):
+        O.rebuild(st, phis[s], s)
+        S = st.systems[s]
+        assert rel(g.read_field("dens", s), S.dens) < 1e-12
+        assert rel(g.read_field("nhat", s), S.nhat) < 1e-12
+        assert rel(g.read_field("vxc", s), S.vxc) < 1e-12
+        assert rel(g.read_field("vh", s), S.vh) < 1e-11
+        assert rel(g.read_field("vks", s), S.vks) < 1e-11
+        for ia in range(1, len(spec.atoms) + 1):
+            assert rel(g.read_rhoij(s, ia), S.rhoij[ia - 1]) < 1e-11
+            assert rel(g.read_dij(s, ia), S.dij[ia - 1]) < 1e-10
+    g.close()
+
+
+def test_rebuild_pieces_match_oracle(mol):
+    spec, _, phi = mol
+    st = state_from_spec(spec, nsys=2)
+    torch, g = _gpu(spec, st, phi.shape[0])
+    x = g.to_device(phi)
+    g.tddft_rhoij_2(x, 1)
+    g.tddft_dens_2(x, 1)
+    g.get_nhat(1)
+    g.get_pot_tddft_2(1)
+    g.get_dij(1)
+    rho = O.tddft_rhoij_2(st, phi)
+    dens = O.tddft_dens_2(st, phi)
+    nh, dens = O.get_nhat(st, rho, dens)
+    vks, vxc, vh = O.get_pot_tddft_2(st, dens, nh)
+    dij = O.get_dij(st, rho, vks, vxc)
+    assert rel(g.read_field("dens", 1), dens) < 1e-12
+    assert rel(g.read_field("vks", 1), vks) < 1e-11
+    for ia in range(1, len(spec.atoms) + 1):
+        assert rel(g.read_rhoij(1, ia), rho[ia - 1]) < 1e-11
+        assert rel(g.read_dij(1, ia), dij[ia - 1]) < 1e-10
+    g.close()
+
+
+def test_rk4_step_matches_oracle(mol):
+    spec, _, phi = mol
+    st = state_from_spec(spec, nsys=2)
+    _random_potentials(st, spec, 1)
+    phis = [phi, O.excite_psi_bar_pert(st, phi, spec.sm, spec.ipol)]
+    torch, g = _gpu(spec, st, phi.shape[0])
+    g.write_orbitals(0, phis[0])
+    g.write_orbitals(1, phis[1])
+    g.rk4_prop_shs(spec.dt)
+    for s in (0, 1):
+        ref = O.rk4_prop_shs(st, phis[s], s)
+        assert rel(g.read_orbitals(s), ref) < 1e-10
+    g.close()
+
+
+def test_tddft_loop_dipole_matches_oracle(mol):
+    spec, _, phi = mol
+    st = state_from_spec(spec, nsys=2)
+    phis = [phi, O.excite_psi_bar_pert(st, phi, spec.sm, spec.ipol)]
+    torch, g = _gpu(spec, st, phi.shape[0])
+    g.write_orbitals(0, phi)
+    g.excite_psi_bar_pert(spec.sm, spec.ipol + 1)
+    nt = 3
+    d_gpu, dip_gpu = g.opaw_tddft(nt, spec.dt, spec.sm)
+    d_ref = []
+    for _ in range(nt):
+        phis, d, _, _ = O.tddft_step(st, phis, spec.sm)
+        d_ref.append(d)
+    d_ref = np.array(d_ref)
+    assert rel(d_gpu, d_ref) < 1e-8
+    assert rel(g.read_orbitals(1), phis[1]) < 1e-10
+    dd, a, b = g.plot_dip(spec.sm)
+    assert rel(dd, d_ref[-1]) < 1e-8
+    g.close()
+
+
+def test_orthog_and_diagh_match_oracle(per):
+    spec, st, phi = per
+    torch, g = _gpu(spec, st, phi.shape[0])
+    rng = np.random.default_rng(9)
+    raw = phi + 0.3 * (rng.standard_normal(phi.shape) + 1j * rng.standard_normal(phi.shape))
+    g.write_orbitals(0, raw)
+    eig = g.orthog_phi(0, 1)
+    ref, eig_ref, _ = O.orthog_phi(st, raw)
+    assert rel(eig, eig_ref) < 1e-12
+    out = g.read_orbitals(0)
+    G = (out.conj() @ out.T) * spec.dv
+    assert np.max(np.abs(G - np.eye(len(eig)))) < 1e-12
+    # same subspace as the oracle (eigenvector phases may differ): the overlap
+    # of the two orthonormal sets is unitary
+    M = (ref.conj() @ out.T) * spec.dv
+    assert np.max(np.abs(M.conj().T @ M - np.eye(len(eig)))) < 1e-10
+    ev = g.diagh(0, 1)
+    _, ev_ref, _ = O.diagh(st, ref, 1, 0)
+    assert np.max(np.abs(ev - ev_ref)) < 1e-10 * max(1.0, np.max(np.abs(ev_ref)))
+    g.close()
+
+
+def test_mixed_precision_operators(mol):
+    spec, st, phi = mol
+    torch, g = _gpu(spec, st, phi.shape[0], prec="mixed")
+    x = g.to_device(phi)
+    assert rel(g.h_phi(x, 0).cpu().numpy(), O.h_phi(st, phi, 0)) < 2e-5
+    assert rel(g.shs(x, 1).cpu().numpy(), O.shs(st, phi, 1)) < 2e-5
+    g.close()
+
+
+def test_error_paths():
+    torch, OpawGPU = _api()
+    from opaw_amd.lib import OpawError
+    spec = synth.molecule_system(nring=1, n=(32, 32, 32), d=0.5, seed=11)
+    # a nonperiodic box that leaves the grid is an error (src/get_irs.f90:30-35)
+    spec.atoms[0].start = spec.atoms[0].start.copy()
+    spec.atoms[0].start[0] = 25
+    with pytest.raises(OpawError, match="cannot go outside"):
+        OpawGPU(spec, "f64", nsys=1, nb=1)
+    spec = synth.hpsi_system(n=(12, 10, 8), d=0.4, natom=0, seed=1)
+    spec.nx = 14
+    with pytest.raises(OpawError, match="FFT length"):
+        OpawGPU(spec, "f64", nsys=1, nb=1)
