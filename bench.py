@@ -1,0 +1,210 @@
+"""Benchmark: RK4 TDDFT steps/s of the OPAW propagation hot path on one MI355X.
+
+Workload (one "step" = one iteration of the reference time loop,
+src/tddft.f90:67-91 with dyn=3): the Hamiltonian rebuild of both systems,
+rk4_prop_shs with a self-consistent rebuild after stages 1-3 and the norm
+restore, and the dipole of both systems.  The system is the naphthalene-shaped
+synthetic OPAW system: C10H8 D2h geometry (C-C 2.65, C-H 2.05 bohr), 64^3
+grid at 0.5 bohr, Martyna-Tuckerman Hartree on the 128^3 doubled box, 10 C
+atoms (14^3 projector boxes, 8 projectors) and 8 H atoms (10^3 boxes, 5
+projectors), 24 occupied orbitals, 2 systems (unperturbed + kick 1e-3 along
+x), dt = 0.05 a.u.  Projector/compensation tables, vloc, ncoret and the
+on-site D_ij model are seeded synthetic tables of the reference shapes.
+
+Both precisions run from the same initial orbitals: FP64, and the mixed path
+(FP32 orbitals and FFTs; FP64 density, rho_ij, D_ij, Hartree and dipoles).
+`value` is the mixed-path rate when its dipole trace stays within 1e-4
+(relative L2) of the FP64 trace, else the FP64 rate.  The CPU baseline is the
+numpy oracle timed on a bounded sample (one full step) on this host.
+
+Prints ONE JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(_HERE, "opaw-tddft_amd"))
+
+METRIC = "RK4 TDDFT steps/s, mixed vs FP64 on 1 GPU at ≤1e-4 dipole error; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--norb", type=int, default=24)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
+    ap.add_argument("--precision", choices=["both", "f64", "mixed"], default="both")
+    return ap.parse_args()
+
+
+def box_points(spec):
+    return sum(int(a.nr[0] * a.nr[1] * a.nr[2]) for a in spec.atoms)
+
+
+def table_bytes(spec):
+    return sum(int(a.nr[0] * a.nr[1] * a.nr[2]) * a.ms * 8 for a in spec.atoms)
+
+
+def class_bytes(spec, nsys, nb, sc):
+    """Algorithmic HBM bytes per launch for the kernel families of one
+    shs / rebuild application over all nsys*nb fields."""
+    N, nf, Nb, T = spec.n, nsys * nb, box_points(spec), table_bytes(spec)
+    sr = sc // 2
+    return {
+        "fft_x": 2 * nf * N * sc,
+        "fft_y": 2 * nf * N * sc,
+        "fft_z": 2 * nf * N * sc + N * sr,
+        "fft_xh": 3 * nf * N * sc + nsys * N * sr,
+        "box_proj": nf * Nb * sc + T,
+        "box_add": 2 * nf * Nb * sc + T,
+    }
+
+
+def step_bytes_model(spec, nsys, nb, sc, sr):
+    """Fused-minimal bytes per RK4 step: n_s*4*(n_o*(8 s_c N + 4 s_c N_b) +
+    n_o s_c N + s_r N + 4.5 s_r N2 + 6 s_r N) with N2 = 8N (MT)."""
+    N, Nb = spec.n, box_points(spec)
+    N2 = N if spec.periodic else 8 * N
+    return nsys * 4 * (nb * (8 * sc * N + 4 * sc * Nb) + nb * sc * N + sr * N + 4.5 * sr * N2 + 6 * sr * N)
+
+
+def run_gpu(spec, phi0, prec, steps, warmup, nb):
+    import numpy as np
+    import torch
+    from opaw_amd.api import OpawGPU
+    g = OpawGPU(spec, prec, nsys=2, nb=nb)
+    g.write_orbitals(0, phi0)
+    g.excite_psi_bar_pert(spec.sm, spec.ipol + 1)
+    # warmup, profiled for every family to find the dominant kernel
+    g.profile(True)
+    d_w, _ = g.opaw_tddft(warmup, spec.dt, spec.sm)
+    torch.cuda.synchronize()
+    tot = {k: g.profile_read(k) for k in ("fft_x", "fft_y", "fft_z", "fft_xh", "hartree", "box_proj",
+                                          "box_add", "rk4", "dens", "pot", "other")}
+    g.profile_reset()
+    sc = 16 if prec == "f64" else 8
+    model = class_bytes(spec, 2, nb, sc)
+    dom = max(model, key=lambda k: tot[k][0])
+    # timed region: events only around the dominant family's launches
+    g.profile([dom])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    d_t, _ = g.opaw_tddft(steps, spec.dt, spec.sm)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    dom_ms, dom_n = g.profile_read(dom)
+    g.profile(False)
+    g.close()
+    sec = (t1 - t0) / steps
+    avg = dom_ms / max(dom_n, 1) / 1e3
+    achieved = model[dom] / avg / 1e9
+    breakdown = {k: round(v[0] / max(warmup, 1), 4) for k, v in tot.items()}
+    return dict(steps_per_s=1.0 / sec, ms_per_step=sec * 1e3, d_dip=np.concatenate([d_w, d_t]),
+                dominant=dom, dom_bytes=model[dom], dom_avg_s=avg, dom_launches=dom_n,
+                achieved_gbs=achieved, breakdown_ms_per_step=breakdown)
+
+
+def cpu_baseline(spec, phi0):
+    """numpy oracle, one full time step (both systems), single thread."""
+    try:
+        from threadpoolctl import threadpool_limits
+    except Exception:          # pragma: no cover
+        threadpool_limits = None
+    sys.path.insert(0, _HERE)
+    from oracle.adapter import state_from_spec
+    import oracle.opaw_oracle as O
+    st = state_from_spec(spec, nsys=2)
+    phis = [phi0, O.excite_psi_bar_pert(st, phi0, spec.sm, spec.ipol)]
+    import contextlib
+    guard = threadpool_limits(limits=1) if threadpool_limits else contextlib.nullcontext()
+    with guard:
+        t0 = time.perf_counter()
+        O.tddft_step(st, phis, spec.sm)
+        t1 = time.perf_counter()
+    return 1.0 / (t1 - t0), t1 - t0
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
+              "the multi-GPU sharding is a design (orbital / system split), not a measured path",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from opaw_amd import synth
+    torch.cuda.init()
+    nb = a.norb
+    spec = synth.molecule_system(nring=2, n=(64, 64, 64), d=0.5, seed=1234)
+    spec.rnel = 2.0 * nb
+    phi0 = synth.lowdin_orthonormalise(synth.random_orbitals(spec, nb, seed=7, envelope=4.0), spec.dv)
+    print(f"[bench] workload ready: {spec.nx}x{spec.ny}x{spec.nz}, {len(spec.atoms)} atoms, {nb} orbitals",
+          file=sys.stderr, flush=True)
+
+    res = {}
+    for prec in (["f64", "mixed"] if a.precision == "both" else [a.precision]):
+        res[prec] = run_gpu(spec, phi0, prec, a.steps, a.warmup, nb)
+        print(f"[bench] {prec}: {res[prec]['steps_per_s']:.2f} steps/s, dominant {res[prec]['dominant']} "
+              f"{res[prec]['achieved_gbs']:.0f} GB/s", file=sys.stderr, flush=True)
+
+    dip_err = None
+    if "f64" in res and "mixed" in res:
+        d64, d32 = res["f64"]["d_dip"], res["mixed"]["d_dip"]
+        dip_err = float(np.linalg.norm(d32 - d64) / np.linalg.norm(d64))
+    use_mixed = "mixed" in res and (dip_err is None or dip_err <= 1e-4)
+    key = "mixed" if use_mixed else "f64"
+    r = res[key]
+    sc, sr = (16, 8) if key == "f64" else (8, 4)
+    model_gbs = step_bytes_model(spec, 2, nb, sc, sr) / (r["ms_per_step"] / 1e3) / 1e9
+
+    cpu = None
+    if not a.no_cpu:
+        v, t = cpu_baseline(spec, phi0)
+        cpu = {"value": v, "unit": "steps/s", "cores": 1, "kind": "port",
+               "sample": f"numpy oracle, 1 full time step (2 systems x {nb} orbitals, 4 RK4 stages, "
+                         f"4 rebuilds per system) = {t:.1f} s on 1 host thread"}
+
+    out = {
+        "metric": METRIC,
+        "value": r["steps_per_s"],
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if key == "f64" else "f32 orbitals/FFT + f64 rho/rhoij/Dij/Hartree/dipole",
+        "data": "synthetic",
+        "config": {"workload": "C10H8-shaped synthetic OPAW system, 64^3 MT grid @0.5 bohr, "
+                               "24 orbitals x 2 systems, RK4 dt=0.05 with per-stage rebuild",
+                   "grid": [spec.nx, spec.ny, spec.nz], "spacing_bohr": spec.dx, "boundary": "Martyna-Tuckerman",
+                   "atoms": {"C": spec.extra["natom_c"], "H": spec.extra["natom_h"]}, "orbitals": nb,
+                   "systems": 2, "dt": spec.dt, "kick": spec.sm, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": r["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": r["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": r["dominant"], "bytes_per_launch": r["dom_bytes"],
+                     "avg_launch_us": r["dom_avg_s"] * 1e6, "launches": r["dom_launches"]},
+        "cpu_baseline": cpu,
+        "hbm_model_gbs": model_gbs,
+        "fp64_steps_per_s": res["f64"]["steps_per_s"] if "f64" in res else None,
+        "mixed_steps_per_s": res["mixed"]["steps_per_s"] if "mixed" in res else None,
+        "mixed_over_fp64": (res["mixed"]["steps_per_s"] / res["f64"]["steps_per_s"])
+        if ("f64" in res and "mixed" in res) else None,
+        "dipole_rel_l2_mixed_vs_fp64": dip_err,
+        "breakdown_ms_per_step": r["breakdown_ms_per_step"],
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

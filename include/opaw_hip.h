@@ -181,8 +181,9 @@ int opaw_orthog_phi(opaw_ctx* ctx, int sys, int h_type, double* eig);
  * (h_type 0); orbitals rotated in place, eig (nb) ascending. */
 int opaw_diagh(opaw_ctx* ctx, int sys, int h_type, double* eig);
 
-/* ---- profiling: HIP events around every launch of a kernel family ------- */
-int opaw_profile_enable(opaw_ctx* ctx, int on);
+/* ---- profiling: HIP events around the launches of selected kernel families
+ * mask: bit k enables family k (OPAW_K_*); 0 disables, -1 enables all. */
+int opaw_profile_enable(opaw_ctx* ctx, int mask);
 int opaw_profile_reset(opaw_ctx* ctx);
 int opaw_profile_read(opaw_ctx* ctx, int kclass, double* total_ms, long long* launches);
 

@@ -46,6 +46,7 @@ static int fail(int code, const char* fmt, ...) {
 // profiler: HIP event pairs around launches of each kernel family
 struct Prof {
   bool on = false;
+  unsigned mask = 0;
   struct Rec {
     int cls;
     hipEvent_t a, b;
@@ -90,7 +91,7 @@ static thread_local Prof* g_prof = nullptr;
 namespace opaw {
 void prof_begin(hipStream_t st, int cls) {
   Prof* p = g_prof;
-  if (!p || !p->on) return;
+  if (!p || !p->on || !((p->mask >> cls) & 1u)) return;
   if (p->recs.size() > 20000) p->harvest();
   Prof::Rec r{cls, p->take(), nullptr};
   if (r.a) (void)hipEventRecord(r.a, st);
@@ -1392,9 +1393,10 @@ int opaw_diagh(opaw_ctx* c, int sys, int h_type, double* eig) {
   return flow_diagh<float>(c, sys, h_type, eig);
 }
 
-int opaw_profile_enable(opaw_ctx* c, int on) {
+int opaw_profile_enable(opaw_ctx* c, int mask) {
   if (!c) return fail(OPAW_ERR_ARG, "null context");
-  c->prof.on = on != 0;
+  c->prof.mask = (unsigned)mask;
+  c->prof.on = mask != 0;
   return OPAW_OK;
 }
 

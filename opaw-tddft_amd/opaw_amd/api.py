@@ -269,8 +269,18 @@ class OpawGPU:
         return eig
 
     # -------------------------------------------------------------- profile
-    def profile(self, on: bool = True):
-        lib.call("opaw_profile_enable", self.ctx, 1 if on else 0)
+    def profile(self, classes=None):
+        """classes: None / True = every kernel family, False / [] = off, or an
+        iterable of family names (lib.KCLASS keys)."""
+        if classes is None or classes is True:
+            mask = -1
+        elif classes is False:
+            mask = 0
+        else:
+            mask = 0
+            for name in classes:
+                mask |= 1 << lib.KCLASS[name]
+        lib.call("opaw_profile_enable", self.ctx, int(mask))
 
     def profile_reset(self):
         lib.call("opaw_profile_reset", self.ctx)
