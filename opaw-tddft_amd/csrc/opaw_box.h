@@ -1,0 +1,44 @@
+// Projector-box work: tables, projection, weights and gathered add-back.
+#pragma once
+#include "opaw_gather.h"
+#include "opaw_kernels.h"
+
+namespace opaw {
+
+constexpr int kProjChunk = 256;   // box points per projection workgroup
+
+struct BoxTables {
+  const AtomDesc* atoms;     // device
+  int natom;
+  int ctot;                  // sum of ms over atoms
+  int msp;                   // padded ms of the projection kernel (8 or 16)
+  int nchunks;               // projection chunks over all atoms
+  const int2* chunks;        // (atom, first point) per chunk
+  const int2* achunk;        // (first chunk, count) per atom
+  const int2* coef;          // (atom, i) per coefficient
+  const double* p;           // per-atom projector tables (reference layout)
+  const double* p1;          // per-atom orthonormal projector tables
+  const int* bpt;            // grid point of every box point (atoms concatenated, boff)
+  const int* bk;             // union index of every box point
+  UnionTabs U;               // union-of-boxes gather tables
+};
+
+// Projection of nf fields (of in + cpre when cpre, a compact correction, is
+// given), chunk partials in `part`, coefficients in ca (nf x ctot), and
+// weights w (mode 1: sn * ca, mode 2: D(sys) ca, mode 0: none).
+template <typename R>
+hipError_t box_coeffs(const GridGeom& g, const BoxTables& bt, hipStream_t st, const cx<R>* u, int nf, int which,
+                      const c64* cpre, double dv, double* part, c64* ca, int mode, const double* sn,
+                      const double* dij, int nf_per_sys, int mtot, c64* w);
+
+// compact correction corr (nf x U) = sum over atom entries of tab * w
+hipError_t union_corr(const BoxTables& bt, hipStream_t st, int nf, int which, const c64* w, c64* corr);
+
+hipError_t box_weights(const BoxTables& bt, hipStream_t st, const c64* ca, int nf, int mode, const double* sn,
+                       const double* dij, int nf_per_sys, int mtot, c64* w);
+
+// out_f(upt[k]) += corr(f, k), in place
+template <typename R>
+hipError_t union_apply(const GridGeom& g, const BoxTables& bt, hipStream_t st, cx<R>* out, int nf, const c64* corr);
+
+}  // namespace opaw

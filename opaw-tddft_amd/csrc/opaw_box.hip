@@ -1,198 +1,265 @@
 // Projector-box kernels of the OPAW hot path (gfx950).
 //
-//  box_project: ca(f, coff_a + i) = dv * sum_{box_a} u_f(r) p_{a,i}(r)
-//               (src/proj.f90:1-44 and src/proj1.f90:1-46; projectors are real
-//               at the gamma point so conj(p) = p).  One workgroup per
-//               (atom, field); FP64 accumulation; wave64 shuffles then LDS.
-//  box_add:     u_f(r) += sum_i p_{a,i}(r) w_{a,i} for every atom a whose box
-//               holds r, with w = sn(n) * ca (src/snphi.f90:15-48) or
-//               w = D ca (addvnl, src/hphi.f90:24-65).  One workgroup per
-//               (z-plane, field) walks the atoms whose boxes cut that plane in
-//               atom order, with a barrier between atoms, so overlapping boxes
-//               (C-C neighbours) add deterministically without atomics.
-// Both follow the periodic wrap mod(ix+nx-1,nx)+1 of the reference edge path.
-#include "opaw_kernels.h"
+// Projection  ca(f, coff_a + i) = dv * sum_{box_a} u_f(r) p_{a,i}(r)
+// (src/proj.f90:1-44, src/proj1.f90:1-46; the tables are real at the gamma
+// point so conj(p) = p).  A workgroup owns a 256-point chunk of one atom's box
+// and 16 fields: 16 lanes walk the chunk's points (consecutive points ->
+// consecutive lanes, coalesced box rows), the 16 lane groups are the fields,
+// each lane accumulates ms complex sums, and the 16 point lanes are reduced
+// with four xor-shuffles.  One partial per chunk is written; a finishing
+// kernel sums the chunks of an atom in chunk order (deterministic) and forms
+// the weights
+//     w = sn(n) * ca                      (sn_phi, src/snphi.f90:24-28)
+//     w = D(sys) ca                       (addvnl, src/hphi.f90:43-47)
+// With a compact correction cpre the input is u = in + cpre (S^-1/2 applied on
+// the fly), so a projection of S^-1/2 psi never needs S^-1/2 psi in memory.
+//
+// Add-back: union_corr turns a weight set into the compact correction
+// corr(f, k) over the union of boxes (opaw_gather.h); consumers add it.
+#include "opaw_box.h"
 
 namespace opaw {
 
 constexpr int kBoxThreads = 256;
+constexpr int kFieldsPerWG = 16;
+constexpr int kPointLanes = kBoxThreads / kFieldsPerWG;   // 16
 
-__device__ __forceinline__ int wrapi(int i, int n) {
-  i %= n;
-  return i < 0 ? i + n : i;
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
+template <typename R, int MSP, bool PRE>
+__global__ __launch_bounds__(kBoxThreads) void k_proj(const AtomDesc* __restrict__ atoms,
+                                                      const int2* __restrict__ chunks, const int* __restrict__ bpt,
+                                                      const int* __restrict__ bk, const double* __restrict__ tabs,
+                                                      const cx<R>* __restrict__ u, long long N, int nf,
+                                                      const c64* __restrict__ cpre, int U,
+                                                      double* __restrict__ part) {
+  const int ch = blockIdx.x;
+  const int kl = threadIdx.x % kPointLanes;
+  const int f = blockIdx.y * kFieldsPerWG + threadIdx.x / kPointLanes;
+  const int2 cd = chunks[ch];                     // (atom, first point)
+  const AtomDesc A = atoms[cd.x];
+  double accr[MSP], acci[MSP];
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
-}
-
-template <typename R>
-__global__ __launch_bounds__(kBoxThreads) void k_box_project(const AtomDesc* __restrict__ atoms,
-                                                             const double* __restrict__ tabs,
-                                                             const cx<R>* __restrict__ u, long long N,
-                                                             int nx, int ny, int nz, int ctot,
-                                                             double dv, c64* __restrict__ ca) {
-  const int a = blockIdx.x;
-  const int f = blockIdx.y;
-  const AtomDesc A = atoms[a];
-  const double* tab = tabs + A.tab;
-  const cx<R>* uf = u + (long long)f * N;
-  double accr[kMaxMs], acci[kMaxMs];
-#pragma unroll
-  for (int i = 0; i < kMaxMs; ++i) accr[i] = acci[i] = 0.0;
-  const int n1 = A.nr[0], n2 = A.nr[1];
-  for (int p = threadIdx.x; p < A.npt; p += kBoxThreads) {
-    const int lx = p % n1;
-    const int r = p / n1;
-    const int ly = r % n2;
-    const int lz = r / n2;
-    int gx = A.start[0] + lx, gy = A.start[1] + ly, gz = A.start[2] + lz;
-    if (A.wrap) {
-      gx = wrapi(gx, nx);
-      gy = wrapi(gy, ny);
-      gz = wrapi(gz, nz);
-    }
-    const cx<R> v = uf[(long long)gx + (long long)nx * ((long long)gy + (long long)ny * gz)];
-    const double vr = (double)v.x, vi = (double)v.y;
-#pragma unroll
-    for (int i = 0; i < kMaxMs; ++i) {
-      if (i < A.ms) {
-        const double t = tab[(long long)i * A.npt + p];
-        accr[i] += vr * t;
-        acci[i] += vi * t;
+  for (int i = 0; i < MSP; ++i) accr[i] = acci[i] = 0.0;
+  const int kend = min(A.npt, cd.y + kProjChunk);
+  if (f < nf) {
+    const cx<R>* uf = u + (long long)f * N;
+    for (int k = cd.y + kl; k < kend; k += kPointLanes) {
+      const int pt = bpt[A.boff + k];
+      const cx<R> v = uf[pt];
+      double vr = (double)v.x, vi = (double)v.y;
+      if constexpr (PRE) {
+        const c64 c = cpre[(long long)f * U + bk[A.boff + k]];
+        vr += c.x;
+        vi += c.y;
       }
-    }
-  }
-  __shared__ double red[kBoxThreads / 64][kMaxMs][2];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+      const double* t = tabs + A.tab + k;
 #pragma unroll
-  for (int i = 0; i < kMaxMs; ++i) {
-    if (i < A.ms) {
-      const double sr = wave_sum(accr[i]);
-      const double si = wave_sum(acci[i]);
-      if (lane == 0) {
-        red[wave][i][0] = sr;
-        red[wave][i][1] = si;
-      }
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x < A.ms) {
-    const int i = threadIdx.x;
-    double sr = 0.0, si = 0.0;
-#pragma unroll
-    for (int w = 0; w < kBoxThreads / 64; ++w) {
-      sr += red[w][i][0];
-      si += red[w][i][1];
-    }
-    ca[(long long)f * ctot + A.coff + i] = mk(sr * dv, si * dv);
-  }
-}
-
-template <typename R>
-__global__ __launch_bounds__(kBoxThreads) void k_box_add(const AtomDesc* __restrict__ atoms,
-                                                         const double* __restrict__ tabs,
-                                                         const int* __restrict__ plane_ptr,
-                                                         const PlaneEntry* __restrict__ plane,
-                                                         cx<R>* __restrict__ out, long long N, int nx,
-                                                         int ny, int nz, int ctot,
-                                                         const c64* __restrict__ ca, int mode,
-                                                         const double* __restrict__ sn,
-                                                         const double* __restrict__ dij, int nf_per_sys,
-                                                         int mtot) {
-  const int z = blockIdx.x;
-  const int f = blockIdx.y;
-  cx<R>* of = out + (long long)f * N;
-  __shared__ double wr[kMaxMs], wi[kMaxMs];
-  const int e0 = plane_ptr[z], e1 = plane_ptr[z + 1];
-  for (int e = e0; e < e1; ++e) {
-    const PlaneEntry pe = plane[e];
-    const AtomDesc A = atoms[pe.atom];
-    const c64* c = ca + (long long)f * ctot + A.coff;
-    if (threadIdx.x < A.ms) {
-      const int i = threadIdx.x;
-      double xr, xi;
-      if (mode == 0) {
-        const double s = sn[A.coff + i];
-        xr = s * c[i].x;
-        xi = s * c[i].y;
-      } else {
-        const int sys = f / nf_per_sys;
-        const double* D = dij + (long long)sys * mtot + A.moff;
-        xr = 0.0;
-        xi = 0.0;
-        for (int j = 0; j < A.ms; ++j) {
-          const double d = D[i + A.ms * j];
-          xr += d * c[j].x;
-          xi += d * c[j].y;
+      for (int i = 0; i < MSP; ++i) {
+        if (i < A.ms) {
+          const double tv = t[(long long)i * A.npt];
+          accr[i] += vr * tv;
+          acci[i] += vi * tv;
         }
       }
-      wr[i] = xr;
-      wi[i] = xi;
     }
-    __syncthreads();
-    const int n1 = A.nr[0], n2 = A.nr[1];
-    const int gz = A.wrap ? wrapi(A.start[2] + pe.lz, nz) : A.start[2] + pe.lz;
-    const double* tab = tabs + A.tab + (long long)pe.lz * n1 * n2;
-    for (int q = threadIdx.x; q < n1 * n2; q += kBoxThreads) {
-      const int lx = q % n1, ly = q / n1;
-      int gx = A.start[0] + lx, gy = A.start[1] + ly;
-      if (A.wrap) {
-        gx = wrapi(gx, nx);
-        gy = wrapi(gy, ny);
-      }
-      double vr = 0.0, vi = 0.0;
-      for (int i = 0; i < A.ms; ++i) {
-        const double t = tab[(long long)i * A.npt + q];
-        vr += t * wr[i];
-        vi += t * wi[i];
-      }
-      const long long g = (long long)gx + (long long)nx * ((long long)gy + (long long)ny * gz);
-      cx<R> o = of[g];
-      o.x = (R)((double)o.x + vr);
-      o.y = (R)((double)o.y + vi);
-      of[g] = o;
+  }
+#pragma unroll
+  for (int i = 0; i < MSP; ++i) {
+#pragma unroll
+    for (int off = kPointLanes / 2; off > 0; off >>= 1) {
+      accr[i] += __shfl_xor(accr[i], off, 64);
+      acci[i] += __shfl_xor(acci[i], off, 64);
     }
-    __syncthreads();
+  }
+  if (kl == 0 && f < nf) {
+    double* pp = part + ((long long)ch * nf + f) * MSP * 2;
+#pragma unroll
+    for (int i = 0; i < MSP; ++i) {
+      if (i < A.ms) {
+        pp[2 * i] = accr[i];
+        pp[2 * i + 1] = acci[i];
+      }
+    }
   }
 }
 
+// ca = dv * sum over the atom's chunks; then the weights (mode 0: none,
+// 1: sn * ca, 2: D(sys) ca).  One workgroup per field.
+template <int MSP>
+__global__ __launch_bounds__(128) void k_proj_finish(const AtomDesc* __restrict__ atoms,
+                                                     const int2* __restrict__ coef, const int2* __restrict__ achunk,
+                                                     const double* __restrict__ part, int nf, int ctot, double dv,
+                                                     c64* __restrict__ ca, int mode, const double* __restrict__ sn,
+                                                     const double* __restrict__ dij, int nf_per_sys, int mtot,
+                                                     c64* __restrict__ w) {
+  const int f = blockIdx.x;
+  extern __shared__ c64 row[];
+  for (int c = threadIdx.x; c < ctot; c += blockDim.x) {
+    const int2 ai = coef[c];                     // (atom, i)
+    const int2 cr = achunk[ai.x];                // (first chunk, chunk count)
+    double sr = 0.0, si = 0.0;
+    for (int ch = cr.x; ch < cr.x + cr.y; ++ch) {
+      const double* pp = part + (((long long)ch * nf + f) * MSP + ai.y) * 2;
+      sr += pp[0];
+      si += pp[1];
+    }
+    const c64 v = mk(sr * dv, si * dv);
+    ca[(long long)f * ctot + c] = v;
+    row[c] = v;
+  }
+  if (mode == 0) return;
+  __syncthreads();
+  for (int c = threadIdx.x; c < ctot; c += blockDim.x) {
+    const int2 ai = coef[c];
+    c64 out;
+    if (mode == 1) {
+      const double s = sn[c];
+      out = mk(s * row[c].x, s * row[c].y);
+    } else {
+      const AtomDesc A = atoms[ai.x];
+      const double* D = dij + (long long)(f / nf_per_sys) * mtot + A.moff;
+      double xr = 0.0, xi = 0.0;
+      for (int j = 0; j < A.ms; ++j) {
+        const double d = D[ai.y + A.ms * j];
+        xr += d * row[A.coff + j].x;
+        xi += d * row[A.coff + j].y;
+      }
+      out = mk(xr, xi);
+    }
+    w[(long long)f * ctot + c] = out;
+  }
+}
+
+// w = D(sys) ca or sn * ca from existing coefficients, one workgroup per field
+__global__ __launch_bounds__(128) void k_weights(const AtomDesc* __restrict__ atoms, const int2* __restrict__ coef,
+                                                 const c64* __restrict__ ca, int ctot, int mode,
+                                                 const double* __restrict__ sn, const double* __restrict__ dij,
+                                                 int nf_per_sys, int mtot, c64* __restrict__ w) {
+  const int f = blockIdx.x;
+  const c64* row = ca + (long long)f * ctot;
+  for (int c = threadIdx.x; c < ctot; c += blockDim.x) {
+    const int2 ai = coef[c];
+    c64 out;
+    if (mode == 1) {
+      const double s = sn[c];
+      out = mk(s * row[c].x, s * row[c].y);
+    } else {
+      const AtomDesc A = atoms[ai.x];
+      const double* D = dij + (long long)(f / nf_per_sys) * mtot + A.moff;
+      double xr = 0.0, xi = 0.0;
+      for (int j = 0; j < A.ms; ++j) {
+        const double d = D[ai.y + A.ms * j];
+        xr += d * row[A.coff + j].x;
+        xi += d * row[A.coff + j].y;
+      }
+      out = mk(xr, xi);
+    }
+    w[(long long)f * ctot + c] = out;
+  }
+}
+
+// corr(f, k) for every union point k (x) and field f (y)
+__global__ __launch_bounds__(kBoxThreads) void k_union_corr(UnionTabs U, const double* __restrict__ tab,
+                                                            const c64* __restrict__ w, int ctot,
+                                                            c64* __restrict__ corr) {
+  const int k = blockIdx.x * kBoxThreads + threadIdx.x;
+  const int f = blockIdx.y;
+  if (k >= U.U) return;
+  corr[(long long)f * U.U + k] = gather_corr(U, tab, k, w + (long long)f * ctot);
+}
+
+// out_f(upt[k]) += corr(f, k)
 template <typename R>
-hipError_t box_project(const GridGeom& g, const BoxTables& bt, hipStream_t st, const cx<R>* u, int nf,
-                       int which, double dv, c64* ca) {
-  if (bt.natom == 0 || nf == 0) return hipSuccess;
+__global__ __launch_bounds__(kBoxThreads) void k_union_apply(const int* __restrict__ upt, int U,
+                                                             cx<R>* __restrict__ out, long long N,
+                                                             const c64* __restrict__ corr) {
+  const int k = blockIdx.x * kBoxThreads + threadIdx.x;
+  const int f = blockIdx.y;
+  if (k >= U) return;
+  const c64 c = corr[(long long)f * U + k];
+  const long long o = (long long)f * N + upt[k];
+  cx<R> v = out[o];
+  v.x = (R)((double)v.x + c.x);
+  v.y = (R)((double)v.y + c.y);
+  out[o] = v;
+}
+
+// ---------------------------------------------------------------------------
+template <typename R, int MSP>
+static void launch_proj(const GridGeom& g, const BoxTables& bt, hipStream_t st, const cx<R>* u, int nf,
+                        const double* tab, const c64* cpre, double* part) {
   const long long N = (long long)g.nx * g.ny * g.nz;
-  prof_begin(st, KC_BOX_PROJ);
-  hipLaunchKernelGGL((k_box_project<R>), dim3(bt.natom, nf), dim3(kBoxThreads), 0, st, bt.atoms,
-                     which ? bt.p1 : bt.p, u, N, g.nx, g.ny, g.nz, bt.ctot, dv, ca);
-  prof_end(st, KC_BOX_PROJ);
-  return hipGetLastError();
+  dim3 grid(bt.nchunks, (nf + kFieldsPerWG - 1) / kFieldsPerWG);
+  if (cpre)
+    hipLaunchKernelGGL((k_proj<R, MSP, true>), grid, dim3(kBoxThreads), 0, st, bt.atoms, bt.chunks, bt.bpt, bt.bk,
+                       tab, u, N, nf, cpre, bt.U.U, part);
+  else
+    hipLaunchKernelGGL((k_proj<R, MSP, false>), grid, dim3(kBoxThreads), 0, st, bt.atoms, bt.chunks, bt.bpt, bt.bk,
+                       tab, u, N, nf, cpre, bt.U.U, part);
 }
 
 template <typename R>
-hipError_t box_add(const GridGeom& g, const BoxTables& bt, hipStream_t st, cx<R>* out, int nf,
-                   int which, const c64* ca, int mode, double n, const double* sn, const double* dij,
-                   int nf_per_sys, int mtot) {
+hipError_t box_coeffs(const GridGeom& g, const BoxTables& bt, hipStream_t st, const cx<R>* u, int nf, int which,
+                      const c64* cpre, double dv, double* part, c64* ca, int mode, const double* sn,
+                      const double* dij, int nf_per_sys, int mtot, c64* w) {
   if (bt.natom == 0 || nf == 0) return hipSuccess;
-  const long long N = (long long)g.nx * g.ny * g.nz;
-  (void)n;
+  const double* tab = which ? bt.p1 : bt.p;
+  prof_begin(st, KC_BOX_PROJ);
+  if (bt.msp == 8)
+    launch_proj<R, 8>(g, bt, st, u, nf, tab, cpre, part);
+  else
+    launch_proj<R, 16>(g, bt, st, u, nf, tab, cpre, part);
+  prof_end(st, KC_BOX_PROJ);
+  hipError_t e = hipGetLastError();
+  if (e) return e;
   prof_begin(st, KC_BOX_ADD);
-  hipLaunchKernelGGL((k_box_add<R>), dim3(g.nz, nf), dim3(kBoxThreads), 0, st, bt.atoms,
-                     which ? bt.p1 : bt.p, bt.plane_ptr, bt.plane, out, N, g.nx, g.ny, g.nz, bt.ctot, ca,
-                     mode, sn, dij, nf_per_sys, mtot);
+  const size_t shm = (size_t)bt.ctot * sizeof(c64);
+  if (bt.msp == 8)
+    hipLaunchKernelGGL(k_proj_finish<8>, dim3(nf), dim3(128), shm, st, bt.atoms, bt.coef, bt.achunk, part, nf,
+                       bt.ctot, dv, ca, mode, sn, dij, nf_per_sys, mtot, w);
+  else
+    hipLaunchKernelGGL(k_proj_finish<16>, dim3(nf), dim3(128), shm, st, bt.atoms, bt.coef, bt.achunk, part, nf,
+                       bt.ctot, dv, ca, mode, sn, dij, nf_per_sys, mtot, w);
   prof_end(st, KC_BOX_ADD);
   return hipGetLastError();
 }
 
-template hipError_t box_project<double>(const GridGeom&, const BoxTables&, hipStream_t, const c64*, int, int,
-                                        double, c64*);
-template hipError_t box_project<float>(const GridGeom&, const BoxTables&, hipStream_t, const c32*, int, int,
-                                       double, c64*);
-template hipError_t box_add<double>(const GridGeom&, const BoxTables&, hipStream_t, c64*, int, int,
-                                    const c64*, int, double, const double*, const double*, int, int);
-template hipError_t box_add<float>(const GridGeom&, const BoxTables&, hipStream_t, c32*, int, int,
-                                   const c64*, int, double, const double*, const double*, int, int);
+hipError_t box_weights(const BoxTables& bt, hipStream_t st, const c64* ca, int nf, int mode, const double* sn,
+                       const double* dij, int nf_per_sys, int mtot, c64* w) {
+  if (bt.natom == 0 || nf == 0) return hipSuccess;
+  prof_begin(st, KC_BOX_ADD);
+  hipLaunchKernelGGL(k_weights, dim3(nf), dim3(128), 0, st, bt.atoms, bt.coef, ca, bt.ctot, mode, sn, dij,
+                     nf_per_sys, mtot, w);
+  prof_end(st, KC_BOX_ADD);
+  return hipGetLastError();
+}
+
+hipError_t union_corr(const BoxTables& bt, hipStream_t st, int nf, int which, const c64* w, c64* corr) {
+  if (bt.natom == 0 || nf == 0 || bt.U.U == 0) return hipSuccess;
+  prof_begin(st, KC_BOX_ADD);
+  hipLaunchKernelGGL(k_union_corr, dim3((bt.U.U + kBoxThreads - 1) / kBoxThreads, nf), dim3(kBoxThreads), 0, st,
+                     bt.U, which ? bt.U.p1 : bt.U.p, w, bt.ctot, corr);
+  prof_end(st, KC_BOX_ADD);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t union_apply(const GridGeom& g, const BoxTables& bt, hipStream_t st, cx<R>* out, int nf, const c64* corr) {
+  if (bt.natom == 0 || nf == 0 || bt.U.U == 0) return hipSuccess;
+  const long long N = (long long)g.nx * g.ny * g.nz;
+  prof_begin(st, KC_BOX_ADD);
+  hipLaunchKernelGGL((k_union_apply<R>), dim3((bt.U.U + kBoxThreads - 1) / kBoxThreads, nf), dim3(kBoxThreads), 0,
+                     st, bt.U.upt, bt.U.U, out, N, corr);
+  prof_end(st, KC_BOX_ADD);
+  return hipGetLastError();
+}
+
+template hipError_t box_coeffs<double>(const GridGeom&, const BoxTables&, hipStream_t, const c64*, int, int,
+                                       const c64*, double, double*, c64*, int, const double*, const double*, int, int,
+                                       c64*);
+template hipError_t box_coeffs<float>(const GridGeom&, const BoxTables&, hipStream_t, const c32*, int, int,
+                                      const c64*, double, double*, c64*, int, const double*, const double*, int, int,
+                                      c64*);
+template hipError_t union_apply<double>(const GridGeom&, const BoxTables&, hipStream_t, c64*, int, const c64*);
+template hipError_t union_apply<float>(const GridGeom&, const BoxTables&, hipStream_t, c32*, int, const c64*);
 
 }  // namespace opaw

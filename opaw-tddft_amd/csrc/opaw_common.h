@@ -72,13 +72,7 @@ struct AtomDesc {
   int toff_q;      // offset of the type's qijlm table (nl2*ms*ms)
   int toff_d0;     // offset of the type's dij0 table (ms*ms)
   int toff_e;      // offset of the type's eijkl table (ms^4)
-  int pad;
-};
-
-// Plane -> atom list entry (which atom boxes intersect grid plane z).
-struct PlaneEntry {
-  int atom;
-  int lz;          // local z index inside the box
+  int boff;        // offset of this atom's box points in the per-box-point tables
 };
 
 constexpr int kMaxMs = 16;

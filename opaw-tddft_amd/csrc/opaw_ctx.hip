@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "opaw_box.h"
 #include "opaw_fields.h"
 #include "opaw_hip.h"
 
@@ -145,11 +146,18 @@ struct opaw_ctx {
 
   // device tables
   AtomDesc* datoms = nullptr;
+  int msp = 8, nchunks = 0, U = 0, S = 0;
+  int2 *chunks = nullptr, *achunk = nullptr, *coef = nullptr;
+  int *umap = nullptr, *uptr = nullptr, *upt = nullptr, *bpt = nullptr, *bk = nullptr;
+  UEntry* uent = nullptr;
+  double *utp = nullptr, *utp1 = nullptr;
+  int *smap = nullptr, *sptr = nullptr, *sphn = nullptr;
+  int2* sent = nullptr;
+  double* wl = nullptr;
+  bool prep_valid = false;   // cA/wA/cB hold the coefficients of the resident orbitals
   double *P = nullptr, *P1 = nullptr;
   double* sn[4] = {nullptr, nullptr, nullptr, nullptr};  // n = -1, -1/2, +1/2, +1
   double* sn_custom = nullptr;
-  int* plane_ptr = nullptr;
-  PlaneEntry* plane = nullptr;
   int* sph_idx = nullptr;
   double* sph_g = nullptr;
   double *qtab = nullptr, *d0tab = nullptr, *etab = nullptr;
@@ -176,7 +184,14 @@ struct opaw_ctx {
 
   // orbitals + work
   void* phi = nullptr;
-  DevBuf wy, wacc, wh, wu, wtmp, wca, wca2;
+  DevBuf wy, wacc, wh, wu, wtmp;
+  // projection coefficients / weights of the three projection roles:
+  // A = p1 of the input, B = p of S^-1/2 input, C = p1 of H S^-1/2 input
+  DevBuf cA, wA, cB, wB, cC, wC, ppart;
+  // compact corrections over the union of boxes (nf x U): S^-1/2 of the
+  // input (A), nonlocal add-back (NL), S^-1/2 of H S^-1/2 input (C)
+  DevBuf corrA, corrNL, corrC;
+  int coef_cap = 0;
   c64* pair = nullptr;
   c64* hwork = nullptr;
   c64* gpart = nullptr;
@@ -602,9 +617,14 @@ int opaw_finalize_tables(opaw_ctx* c) {
     if (c->hp[a].empty()) return fail(OPAW_ERR_STATE, "projectors of atom %d not set", a + 1);
   // free previous device tables
   for (void* p : {(void*)c->datoms, (void*)c->P, (void*)c->P1, (void*)c->sn[0], (void*)c->sn[1], (void*)c->sn[2],
-                  (void*)c->sn[3], (void*)c->sn_custom, (void*)c->plane_ptr, (void*)c->plane, (void*)c->sph_idx,
-                  (void*)c->sph_g, (void*)c->qtab, (void*)c->d0tab, (void*)c->etab, (void*)c->dij, (void*)c->rhoij})
+                  (void*)c->sn[3], (void*)c->sn_custom, (void*)c->sph_idx,
+                  (void*)c->sph_g, (void*)c->qtab, (void*)c->d0tab, (void*)c->etab, (void*)c->dij, (void*)c->rhoij,
+                  (void*)c->chunks, (void*)c->achunk, (void*)c->coef, (void*)c->umap, (void*)c->uptr, (void*)c->upt,
+                  (void*)c->uent, (void*)c->utp, (void*)c->utp1, (void*)c->smap, (void*)c->sptr, (void*)c->sphn,
+                  (void*)c->sent, (void*)c->wl, (void*)c->bpt, (void*)c->bk})
     dfree(c, p);
+  c->prep_valid = false;
+  c->coef_cap = 0;   // projection buffers depend on msp / chunk count
   // per-type table offsets
   std::vector<int> toff_q(c->ntype), toff_d0(c->ntype), toff_e(c->ntype);
   std::vector<double> q, d0, e;
@@ -670,24 +690,6 @@ int opaw_finalize_tables(opaw_ctx* c) {
   c->ctot = coff;
   c->mtot = moff;
   c->sphtot = soff;
-  // plane lists: atoms whose box covers grid plane z, in atom order
-  std::vector<std::vector<PlaneEntry>> pl(c->nz);
-  for (int a = 0; a < natom; ++a) {
-    const AtomDesc& A = c->hatoms[a];
-    for (int lz = 0; lz < A.nr[2]; ++lz) {
-      int z = A.start[2] + lz;
-      z = ((z % c->nz) + c->nz) % c->nz;
-      pl[z].push_back(PlaneEntry{a, lz});
-    }
-  }
-  std::vector<int> pptr(c->nz + 1, 0);
-  std::vector<PlaneEntry> plane;
-  for (int z = 0; z < c->nz; ++z) {
-    pptr[z] = (int)plane.size();
-    plane.insert(plane.end(), pl[z].begin(), pl[z].end());
-  }
-  pptr[c->nz] = (int)plane.size();
-  RCHK(upload(c, &c->datoms, c->hatoms));
   RCHK(upload(c, &c->P, P));
   RCHK(upload(c, &c->P1, P1));
   RCHK(upload(c, &c->sn[0], sm1));
@@ -695,13 +697,131 @@ int opaw_finalize_tables(opaw_ctx* c) {
   RCHK(upload(c, &c->sn[2], sph));
   RCHK(upload(c, &c->sn[3], sp1));
   RCHK(dalloc_t(c, &c->sn_custom, std::max(1, coff)));
-  RCHK(upload(c, &c->plane_ptr, pptr));
-  RCHK(upload(c, &c->plane, plane));
   RCHK(upload(c, &c->sph_idx, sidx));
   RCHK(upload(c, &c->sph_g, sg));
   RCHK(upload(c, &c->qtab, q));
   RCHK(upload(c, &c->d0tab, d0));
   RCHK(upload(c, &c->etab, e));
+  // projection chunks and coefficient map
+  {
+    std::vector<int2> chunks, achunk(std::max(1, natom)), coef;
+    int maxms = 1;
+    for (int a = 0; a < natom; ++a) {
+      const AtomDesc& A = c->hatoms[a];
+      maxms = std::max(maxms, A.ms);
+      achunk[a] = make_int2((int)chunks.size(), 0);
+      for (int p0 = 0; p0 < A.npt; p0 += kProjChunk) {
+        chunks.push_back(make_int2(a, p0));
+        achunk[a].y++;
+      }
+      for (int i = 0; i < A.ms; ++i) coef.push_back(make_int2(a, i));
+    }
+    c->msp = maxms <= 8 ? 8 : 16;
+    c->nchunks = (int)chunks.size();
+    RCHK(upload(c, &c->chunks, chunks));
+    RCHK(upload(c, &c->achunk, achunk));
+    RCHK(upload(c, &c->coef, coef));
+  }
+  // union of the projector boxes: grid point -> (atom, table values) entries
+  {
+    const long long N = c->N;
+    std::vector<std::vector<std::pair<int, int>>> at;   // per union point: (atom, local point)
+    std::vector<int> umap(N, -1), upt;
+    for (int a = 0; a < natom; ++a) {
+      const AtomDesc& A = c->hatoms[a];
+      for (int p = 0; p < A.npt; ++p) {
+        const int lx = p % A.nr[0], ly = (p / A.nr[0]) % A.nr[1], lz = p / (A.nr[0] * A.nr[1]);
+        const int gx = ((A.start[0] + lx) % c->nx + c->nx) % c->nx;
+        const int gy = ((A.start[1] + ly) % c->ny + c->ny) % c->ny;
+        const int gz = ((A.start[2] + lz) % c->nz + c->nz) % c->nz;
+        const long long pt = gx + (long long)c->nx * (gy + (long long)c->ny * gz);
+        if (umap[pt] < 0) {
+          umap[pt] = (int)upt.size();
+          upt.push_back((int)pt);
+          at.emplace_back();
+        }
+        at[umap[pt]].push_back({a, p});
+      }
+    }
+    // renumber union points in grid order
+    std::vector<int> order(upt.size());
+    for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
+    std::sort(order.begin(), order.end(), [&](int x, int y) { return upt[x] < upt[y]; });
+    std::vector<int> uptr(1, 0), upt2;
+    std::vector<UEntry> ue;
+    std::vector<double> tp, tp1;
+    for (int k : order) {
+      const int pt = upt[k];
+      umap[pt] = (int)upt2.size();
+      upt2.push_back(pt);
+      for (auto& ap : at[k]) {   // atoms were appended in atom order
+        const AtomDesc& A = c->hatoms[ap.first];
+        ue.push_back(UEntry{A.coff, A.ms, (int)tp.size(), ap.first});
+        for (int i = 0; i < A.ms; ++i) {
+          tp.push_back(c->hp[ap.first][(size_t)i * A.npt + ap.second]);
+          tp1.push_back(c->hp1[ap.first][(size_t)i * A.npt + ap.second]);
+        }
+      }
+      uptr.push_back((int)ue.size());
+    }
+    c->U = (int)upt2.size();
+    // per box point: grid point and union index (atoms concatenated)
+    std::vector<int> bpt, bk;
+    for (int a = 0; a < natom; ++a) {
+      AtomDesc& A = c->hatoms[a];
+      A.boff = (int)bpt.size();
+      for (int p = 0; p < A.npt; ++p) {
+        const int lx = p % A.nr[0], ly = (p / A.nr[0]) % A.nr[1], lz = p / (A.nr[0] * A.nr[1]);
+        const int gx = ((A.start[0] + lx) % c->nx + c->nx) % c->nx;
+        const int gy = ((A.start[1] + ly) % c->ny + c->ny) % c->ny;
+        const int gz = ((A.start[2] + lz) % c->nz + c->nz) % c->nz;
+        const int pt = gx + c->nx * (gy + c->ny * gz);
+        bpt.push_back(pt);
+        bk.push_back(umap[pt]);
+      }
+    }
+    RCHK(upload(c, &c->bpt, bpt));
+    RCHK(upload(c, &c->bk, bk));
+    RCHK(upload(c, &c->umap, umap));
+    RCHK(upload(c, &c->uptr, uptr));
+    RCHK(upload(c, &c->upt, upt2));
+    RCHK(upload(c, &c->uent, ue));
+    RCHK(upload(c, &c->utp, tp));
+    RCHK(upload(c, &c->utp1, tp1));
+  }
+  // union of the compensation spheres
+  {
+    std::vector<std::vector<int2>> at;
+    std::vector<int> smap(c->N, -1), spt;
+    std::vector<int> sphn(std::max(1, natom), 0);
+    for (int a = 0; a < natom; ++a) {
+      const AtomDesc& A = c->hatoms[a];
+      sphn[a] = A.sph_n;
+      for (int p = 0; p < A.sph_n; ++p) {
+        const int pt = c->hsph_idx[a][p];
+        if (smap[pt] < 0) {
+          smap[pt] = (int)spt.size();
+          spt.push_back(pt);
+          at.emplace_back();
+        }
+        at[smap[pt]].push_back(make_int2(a, A.sph_off * c->nl2 + p));
+      }
+    }
+    std::vector<int> sptr(1, 0);
+    std::vector<int2> se;
+    for (size_t k = 0; k < spt.size(); ++k) {
+      std::stable_sort(at[k].begin(), at[k].end(), [](int2 x, int2 y) { return x.x < y.x; });
+      se.insert(se.end(), at[k].begin(), at[k].end());
+      sptr.push_back((int)se.size());
+    }
+    c->S = (int)spt.size();
+    RCHK(upload(c, &c->smap, smap));
+    RCHK(upload(c, &c->sptr, sptr));
+    RCHK(upload(c, &c->sent, se));
+    RCHK(upload(c, &c->sphn, sphn));
+    RCHK(dalloc_t(c, &c->wl, (size_t)c->nsys * std::max(1, natom) * c->nl2));
+  }
+  RCHK(upload(c, &c->datoms, c->hatoms));
   RCHK(dalloc_t(c, &c->dij, (size_t)c->nsys * std::max(1, moff)));
   RCHK(dalloc_t(c, &c->rhoij, (size_t)c->nsys * std::max(1, moff)));
   HCHK(hipMemset(c->dij, 0, (size_t)c->nsys * std::max(1, moff) * sizeof(double)));
@@ -829,6 +949,7 @@ int opaw_write_orbitals(opaw_ctx* c, int sys, const void* host) {
   RCHK(check_sys(c, sys));
   if (!host) return fail(OPAW_ERR_ARG, "null orbitals");
   const size_t bytes = (size_t)c->nb * c->N * c->esize();
+  c->prep_valid = false;
   HCHK(hipStreamSynchronize(c->st));
   HCHK(hipMemcpy((char*)c->phi + sys * bytes, host, bytes, hipMemcpyHostToDevice));
   return OPAW_OK;
@@ -858,13 +979,20 @@ static BoxTables box_tables(opaw_ctx* c) {
   b.atoms = c->datoms;
   b.natom = c->natom;
   b.ctot = c->ctot;
+  b.msp = c->msp;
+  b.nchunks = c->nchunks;
+  b.chunks = c->chunks;
+  b.achunk = c->achunk;
+  b.coef = c->coef;
   b.p = c->P;
   b.p1 = c->P1;
-  b.s = c->sn[3];
-  b.plane_ptr = c->plane_ptr;
-  b.plane = c->plane;
+  b.bpt = c->bpt;
+  b.bk = c->bk;
+  b.U = UnionTabs{c->umap, c->uptr, c->uent, c->upt, c->utp, c->utp1, c->U};
   return b;
 }
+
+static SphereTabs sphere_tabs(opaw_ctx* c) { return SphereTabs{c->smap, c->sptr, c->sent, c->sphn}; }
 
 static int need_tables(opaw_ctx* c) {
   if (!c->tables_ready) return fail(OPAW_ERR_STATE, "PAW tables not finalised (opaw_finalize_tables)");
@@ -888,24 +1016,14 @@ static int sn_table(opaw_ctx* c, double n, const double** out) {
   return OPAW_OK;
 }
 
-template <typename R>
-static int ensure_ca(opaw_ctx* c, int nf) {
-  RCHK(ensure(c, c->wca, (size_t)nf * std::max(1, c->ctot) * sizeof(c64)));
-  RCHK(ensure(c, c->wca2, (size_t)nf * std::max(1, c->ctot) * sizeof(c64)));
-  return OPAW_OK;
-}
-
-// out = S^n in  (in may equal out)
-template <typename R>
-static int flow_sn_phi(opaw_ctx* c, const cx<R>* in, cx<R>* out, int nf, double n) {
-  RCHK(ensure_ca<R>(c, nf));
-  const double* sn = nullptr;
-  RCHK(sn_table(c, n, &sn));
-  BoxTables bt = box_tables(c);
-  c64* ca = (c64*)c->wca.p;
-  HCHK(box_project<R>(c->g, bt, c->st, in, nf, 1, c->dx * c->dy * c->dz, ca));
-  if (in != out) HCHK(hipMemcpyAsync(out, in, (size_t)nf * c->N * sizeof(cx<R>), hipMemcpyDeviceToDevice, c->st));
-  HCHK(box_add<R>(c->g, bt, c->st, out, nf, 1, ca, 0, n, sn, nullptr, nf, c->mtot));
+static int ensure_coef(opaw_ctx* c, int nf) {
+  if (nf <= c->coef_cap) return OPAW_OK;
+  const size_t cb = (size_t)nf * std::max(1, c->ctot) * sizeof(c64);
+  for (DevBuf* b : {&c->cA, &c->wA, &c->cB, &c->wB, &c->cC, &c->wC}) RCHK(ensure(c, *b, cb));
+  RCHK(ensure(c, c->ppart, (size_t)std::max(1, c->nchunks) * nf * c->msp * 2 * sizeof(double)));
+  const size_t ub = (size_t)nf * std::max(1, c->U) * sizeof(c64);
+  for (DevBuf* b : {&c->corrA, &c->corrNL, &c->corrC}) RCHK(ensure(c, *b, ub));
+  c->coef_cap = nf;
   return OPAW_OK;
 }
 
@@ -922,58 +1040,112 @@ struct Fields<float> {
   static const float* ek(opaw_ctx* c) { return c->ekf; }
 };
 
+static double dvol(opaw_ctx* c) { return c->dx * c->dy * c->dz; }
+
+// coefficients A (p1 of in, weights sn(-1/2), compact correction corrA) and
+// B (p of in + corrA = p of S^-1/2 in); the B weights are formed later with
+// the then-current D_ij
+template <typename R>
+static int flow_prep(opaw_ctx* c, const cx<R>* in, int nf) {
+  RCHK(ensure_coef(c, nf));
+  const BoxTables bt = box_tables(c);
+  HCHK(box_coeffs<R>(c->g, bt, c->st, in, nf, 1, nullptr, dvol(c), (double*)c->ppart.p, (c64*)c->cA.p, 1, c->sn[1],
+                     nullptr, nf, c->mtot, (c64*)c->wA.p));
+  HCHK(union_corr(bt, c->st, nf, 1, (const c64*)c->wA.p, (c64*)c->corrA.p));
+  HCHK(box_coeffs<R>(c->g, bt, c->st, in, nf, 0, (const c64*)c->corrA.p, dvol(c), (double*)c->ppart.p,
+                     (c64*)c->cB.p, 0, nullptr, nullptr, nf, c->mtot, nullptr));
+  return OPAW_OK;
+}
+
+// out = S^n in  (in may equal out)
+template <typename R>
+static int flow_sn_phi(opaw_ctx* c, const cx<R>* in, cx<R>* out, int nf, double n) {
+  RCHK(ensure_coef(c, nf));
+  c->prep_valid = false;
+  const double* sn = nullptr;
+  RCHK(sn_table(c, n, &sn));
+  const BoxTables bt = box_tables(c);
+  HCHK(box_coeffs<R>(c->g, bt, c->st, in, nf, 1, nullptr, dvol(c), (double*)c->ppart.p, (c64*)c->cA.p, 1, sn,
+                     nullptr, nf, c->mtot, (c64*)c->wA.p));
+  HCHK(union_corr(bt, c->st, nf, 1, (const c64*)c->wA.p, (c64*)c->corrA.p));
+  if (in != out) HCHK(hipMemcpyAsync(out, in, (size_t)nf * c->N * sizeof(cx<R>), hipMemcpyDeviceToDevice, c->st));
+  HCHK(union_apply<R>(c->g, bt, c->st, out, nf, (const c64*)c->corrA.p));
+  return OPAW_OK;
+}
+
 // out = H(sys) in for nf fields; field f uses system sys0 + f / nf_per_sys
 template <typename R>
 static int flow_h_phi(opaw_ctx* c, int sys0, const cx<R>* in, cx<R>* out, int nf, int nf_per_sys) {
-  RCHK(ensure_ca<R>(c, nf));
+  RCHK(ensure_coef(c, nf));
   RCHK(ensure(c, c->wtmp, (size_t)nf * c->N * sizeof(cx<R>)));
-  BoxTables bt = box_tables(c);
-  c64* ca = (c64*)c->wca2.p;
-  HCHK(box_project<R>(c->g, bt, c->st, in, nf, 0, c->dx * c->dy * c->dz, ca));
+  c->prep_valid = false;
+  const BoxTables bt = box_tables(c);
+  HCHK(box_coeffs<R>(c->g, bt, c->st, in, nf, 0, nullptr, dvol(c), (double*)c->ppart.p, (c64*)c->cB.p, 2, nullptr,
+                     c->dij + (size_t)sys0 * c->mtot, nf_per_sys, c->mtot, (c64*)c->wB.p));
+  HCHK(union_corr(bt, c->st, nf, 0, (const c64*)c->wB.p, (c64*)c->corrNL.p));
   HCHK(apply_t_plus_v<R>(c->g, c->st, in, out, (cx<R>*)c->wtmp.p, nf, nf_per_sys, Fields<R>::ek(c),
                          Fields<R>::vks(c) + (size_t)sys0 * c->N, TW<R>::get(c, c->nx), TW<R>::get(c, c->ny),
-                         TW<R>::get(c, c->nz)));
-  HCHK(box_add<R>(c->g, bt, c->st, out, nf, 0, ca, 1, 0.0, nullptr, c->dij + (size_t)sys0 * c->mtot, nf_per_sys,
-                  c->mtot));
+                         TW<R>::get(c, c->nz), c->umap, nullptr, (const c64*)c->corrNL.p, c->U));
   return OPAW_OK;
 }
 
-// out = S^-1/2 H S^-1/2 in; uses the wu work buffer
+// h = H S^-1/2 in from the A/B coefficients of `in` (weights B from D_ij now),
+// then the C coefficients of h (p1, weights sn(-1/2), compact corrC); the
+// outer S^-1/2 is left to the consumer (adds corrC)
+template <typename R>
+static int flow_hs(opaw_ctx* c, int sys0, const cx<R>* in, cx<R>* h, int nf, int nf_per_sys) {
+  RCHK(ensure(c, c->wtmp, (size_t)nf * c->N * sizeof(cx<R>)));
+  const BoxTables bt = box_tables(c);
+  HCHK(box_weights(bt, c->st, (const c64*)c->cB.p, nf, 2, nullptr, c->dij + (size_t)sys0 * c->mtot, nf_per_sys,
+                   c->mtot, (c64*)c->wB.p));
+  HCHK(union_corr(bt, c->st, nf, 0, (const c64*)c->wB.p, (c64*)c->corrNL.p));
+  HCHK(apply_t_plus_v<R>(c->g, c->st, in, h, (cx<R>*)c->wtmp.p, nf, nf_per_sys, Fields<R>::ek(c),
+                         Fields<R>::vks(c) + (size_t)sys0 * c->N, TW<R>::get(c, c->nx), TW<R>::get(c, c->ny),
+                         TW<R>::get(c, c->nz), c->umap, (const c64*)c->corrA.p, (const c64*)c->corrNL.p, c->U));
+  HCHK(box_coeffs<R>(c->g, bt, c->st, h, nf, 1, nullptr, dvol(c), (double*)c->ppart.p, (c64*)c->cC.p, 1, c->sn[1],
+                     nullptr, nf, c->mtot, (c64*)c->wC.p));
+  HCHK(union_corr(bt, c->st, nf, 1, (const c64*)c->wC.p, (c64*)c->corrC.p));
+  return OPAW_OK;
+}
+
+// out = S^-1/2 H S^-1/2 in
 template <typename R>
 static int flow_shs(opaw_ctx* c, int sys0, const cx<R>* in, cx<R>* out, int nf, int nf_per_sys) {
-  RCHK(ensure(c, c->wu, (size_t)nf * c->N * sizeof(cx<R>)));
-  cx<R>* u = (cx<R>*)c->wu.p;
-  RCHK(flow_sn_phi<R>(c, in, u, nf, -0.5));
-  RCHK(flow_h_phi<R>(c, sys0, u, out, nf, nf_per_sys));
-  RCHK(flow_sn_phi<R>(c, out, out, nf, -0.5));
+  RCHK(flow_prep<R>(c, in, nf));
+  c->prep_valid = false;
+  RCHK(flow_hs<R>(c, sys0, in, out, nf, nf_per_sys));
+  const BoxTables bt = box_tables(c);
+  HCHK(union_apply<R>(c->g, bt, c->st, out, nf, (const c64*)c->corrC.p));
   return OPAW_OK;
 }
 
-// The per-stage rebuild for systems [s0, s0+ns): psi~ = S^-1/2 phi, rho_ij,
-// density, nhat + rnel renormalisation, V_H + V_xc -> V_KS, D_ij
-// (src/ncpaw_make_hamiltonian.f90:213-246).  phi holds ns*nb fields.
+// The per-stage rebuild for systems [s0, s0+ns) from their orbitals phi:
+// psi~ = S^-1/2 phi (never stored: its coefficients A and B and its density are
+// gathered), rho_ij, density, nhat + rnel renormalisation, V_H + V_xc -> V_KS,
+// D_ij (src/ncpaw_make_hamiltonian.f90:213-246).
 template <typename R>
 static int flow_rebuild(opaw_ctx* c, int s0, int ns, const cx<R>* phi) {
   if (!c->pot_ready) return fail(OPAW_ERR_STATE, "potentials not set (opaw_set_potentials)");
   if (!c->vk_ready) return fail(OPAW_ERR_STATE, "Hartree kernel not set (opaw_vk_prep)");
   const int nf = ns * c->nb;
   const long long N = c->N;
-  const double dv = c->dx * c->dy * c->dz;
-  RCHK(ensure(c, c->wu, (size_t)nf * N * sizeof(cx<R>)));
-  cx<R>* psit = (cx<R>*)c->wu.p;
-  RCHK(flow_sn_phi<R>(c, phi, psit, nf, -0.5));
-  BoxTables bt = box_tables(c);
-  c64* ca = (c64*)c->wca2.p;
-  HCHK(box_project<R>(c->g, bt, c->st, psit, nf, 0, dv, ca));
+  const double dv = dvol(c);
+  RCHK(flow_prep<R>(c, phi, nf));
+  const BoxTables bt = box_tables(c);
   prof_begin(c->st, KC_DENS);
-  HCHK(rhoij(c->st, c->datoms, c->natom, ca, c->ctot, ns, c->nb, c->mtot, c->rhoij + (size_t)s0 * c->mtot));
+  HCHK(rhoij(c->st, c->datoms, c->natom, (const c64*)c->cB.p, c->ctot, ns, c->nb, c->mtot,
+             c->rhoij + (size_t)s0 * c->mtot));
   prof_end(c->st, KC_DENS);
   prof_begin(c->st, KC_DENS);
-  HCHK(density<R>(c->st, psit, N, ns, c->nb, c->dens + s0 * N));
+  HCHK(density<R>(c->st, phi, N, ns, c->nb, c->dens + s0 * N, c->umap, (const c64*)c->corrA.p, c->U));
   prof_end(c->st, KC_DENS);
-  HCHK(nhat(c->st, c->datoms, c->natom, c->rhoij + (size_t)s0 * c->mtot, c->mtot, c->qtab, c->nl2, c->sph_idx,
-            c->sph_g, N, ns, c->nhat + s0 * N));
+  prof_begin(c->st, KC_DENS);
+  HCHK(nhat(c->st, c->datoms, c->natom, c->rhoij + (size_t)s0 * c->mtot, c->mtot, c->qtab, c->nl2, sphere_tabs(c),
+            c->sph_g, N, ns, c->wl, c->nhat + s0 * N));
+  prof_end(c->st, KC_DENS);
+  prof_begin(c->st, KC_DENS);
   HCHK(renorm_factor(c->st, c->nhat + s0 * N, c->dens + s0 * N, N, ns, dv, c->rnel, c->part, c->fac + s0));
+  prof_end(c->st, KC_DENS);
   prof_begin(c->st, KC_POT);
   HCHK(scale_pack(c->st, c->dens + s0 * N, c->nhat + s0 * N, c->fac + s0, N, ns, c->pair));
   prof_end(c->st, KC_POT);
@@ -991,15 +1163,19 @@ static int flow_rebuild(opaw_ctx* c, int s0, int ns, const cx<R>* phi) {
            c->sph_idx, c->sph_g, c->vks + s0 * N, c->vxc + s0 * N, c->usexcnhat == 0 ? 1 : 0, N, ns, dv,
            c->dij + (size_t)s0 * c->mtot));
   prof_end(c->st, KC_POT);
+  c->prep_valid = (s0 == 0 && ns == c->nsys && (const void*)phi == c->phi);
   return OPAW_OK;
 }
 
-// one RK4 step of all systems (src/rk4.f90:1-103)
+// one RK4 step of all systems (src/rk4.f90:1-103).  Stage s applies
+// shs to y_s from the A/B coefficients of y_s (computed by the previous
+// rebuild), updates acc and y with the outer S^-1/2 gathered into the update,
+// and rebuilds H from the new y.
 template <typename R>
 static int flow_rk4(opaw_ctx* c, double dt) {
   const int nf = c->nsys * c->nb;
   const long long n = (long long)nf * c->N;
-  const double dv = c->dx * c->dy * c->dz;
+  const double dv = dvol(c);
   RCHK(ensure(c, c->wy, n * sizeof(cx<R>)));
   RCHK(ensure(c, c->wacc, n * sizeof(cx<R>)));
   RCHK(ensure(c, c->wh, n * sizeof(cx<R>)));
@@ -1007,15 +1183,18 @@ static int flow_rk4(opaw_ctx* c, double dt) {
   cx<R>* y = (cx<R>*)c->wy.p;
   cx<R>* acc = (cx<R>*)c->wacc.p;
   cx<R>* h = (cx<R>*)c->wh.p;
+  if (!c->prep_valid) RCHK(flow_prep<R>(c, p, nf));
+  const BoxTables bt = box_tables(c);
   prof_begin(c->st, KC_DENS);
   HCHK(field_norms<R>(c->st, p, c->N, nf, dv, c->part, c->nrm0));
   prof_end(c->st, KC_DENS);
   const double cst[4] = {dt / 2.0, dt / 2.0, dt, dt};
   for (int stage = 1; stage <= 4; ++stage) {
     const cx<R>* yin = (stage == 1) ? p : y;
-    RCHK(flow_shs<R>(c, 0, yin, h, nf, c->nb));
+    RCHK(flow_hs<R>(c, 0, yin, h, nf, c->nb));
     prof_begin(c->st, KC_RK4);
-    HCHK(rk4_update<R>(c->st, stage, n, h, p, acc, y, cst[stage - 1], dt));
+    HCHK(rk4_update<R>(c->st, stage, c->N, nf, h, p, acc, y, cst[stage - 1], dt, c->umap, (const c64*)c->corrC.p,
+                       c->U));
     prof_end(c->st, KC_RK4);
     if (stage < 4) RCHK(flow_rebuild<R>(c, 0, c->nsys, y));
   }
@@ -1025,6 +1204,7 @@ static int flow_rk4(opaw_ctx* c, double dt) {
   prof_begin(c->st, KC_RK4);
   HCHK(scale_fields<R>(c->st, p, c->N, nf, c->nrm0, c->nrm1));
   prof_end(c->st, KC_RK4);
+  c->prep_valid = false;
   return OPAW_OK;
 }
 
@@ -1112,6 +1292,7 @@ static int rotate_resident(opaw_ctx* c, int sys, const std::vector<std::complex<
   HCHK(rotate<R>(c->st, phi, (cx<R>*)c->wtmp.p, c->N, nb, c->tmat));
   HCHK(hipMemcpyAsync(phi, c->wtmp.p, (size_t)nb * c->N * sizeof(cx<R>), hipMemcpyDeviceToDevice, c->st));
   HCHK(hipStreamSynchronize(c->st));
+  c->prep_valid = false;
   return OPAW_OK;
 }
 
@@ -1197,12 +1378,14 @@ int opaw_proj(opaw_ctx* c, const void* pin, int nbatch, int which_p1, void* ca) 
   CTX_CHECK(c);
   if (!c->tables_ready) return fail(OPAW_ERR_STATE, "tables not finalised");
   if (!pin || !ca || nbatch <= 0) return fail(OPAW_ERR_ARG, "bad proj arguments");
-  BoxTables bt = box_tables(c);
-  const double dv = c->dx * c->dy * c->dz;
+  RCHK(ensure_coef(c, nbatch));
+  const BoxTables bt = box_tables(c);
   if (c->prec == OPAW_PREC_F64)
-    HCHK(box_project<double>(c->g, bt, c->st, (const c64*)pin, nbatch, which_p1 ? 1 : 0, dv, (c64*)ca));
+    HCHK(box_coeffs<double>(c->g, bt, c->st, (const c64*)pin, nbatch, which_p1 ? 1 : 0, nullptr, dvol(c),
+                            (double*)c->ppart.p, (c64*)ca, 0, nullptr, nullptr, nbatch, c->mtot, nullptr));
   else
-    HCHK(box_project<float>(c->g, bt, c->st, (const c32*)pin, nbatch, which_p1 ? 1 : 0, dv, (c64*)ca));
+    HCHK(box_coeffs<float>(c->g, bt, c->st, (const c32*)pin, nbatch, which_p1 ? 1 : 0, nullptr, dvol(c),
+                           (double*)c->ppart.p, (c64*)ca, 0, nullptr, nullptr, nbatch, c->mtot, nullptr));
   return OPAW_OK;
 }
 
@@ -1224,10 +1407,10 @@ int opaw_tddft_rhoij_2(opaw_ctx* c, int sys, const void* psi) {
   if (!c->tables_ready) return fail(OPAW_ERR_STATE, "tables not finalised");
   if (!psi) return fail(OPAW_ERR_ARG, "null psi");
   const int nb = c->nb;
-  if (c->prec == OPAW_PREC_F64) RCHK(ensure_ca<double>(c, nb));
-  else RCHK(ensure_ca<float>(c, nb));
-  RCHK(opaw_proj(c, psi, nb, 0, c->wca2.p));
-  HCHK(rhoij(c->st, c->datoms, c->natom, (const c64*)c->wca2.p, c->ctot, 1, nb, c->mtot,
+  RCHK(ensure_coef(c, nb));
+  c->prep_valid = false;
+  RCHK(opaw_proj(c, psi, nb, 0, c->cB.p));
+  HCHK(rhoij(c->st, c->datoms, c->natom, (const c64*)c->cB.p, c->ctot, 1, nb, c->mtot,
              c->rhoij + (size_t)sys * c->mtot));
   return OPAW_OK;
 }
@@ -1237,9 +1420,9 @@ int opaw_tddft_dens_2(opaw_ctx* c, int sys, const void* psi) {
   RCHK(check_sys(c, sys));
   if (!psi) return fail(OPAW_ERR_ARG, "null psi");
   if (c->prec == OPAW_PREC_F64)
-    HCHK(density<double>(c->st, (const c64*)psi, c->N, 1, c->nb, c->dens + sys * c->N));
+    HCHK(density<double>(c->st, (const c64*)psi, c->N, 1, c->nb, c->dens + sys * c->N, nullptr, nullptr, 0));
   else
-    HCHK(density<float>(c->st, (const c32*)psi, c->N, 1, c->nb, c->dens + sys * c->N));
+    HCHK(density<float>(c->st, (const c32*)psi, c->N, 1, c->nb, c->dens + sys * c->N, nullptr, nullptr, 0));
   return OPAW_OK;
 }
 
@@ -1248,10 +1431,9 @@ int opaw_get_nhat(opaw_ctx* c, int sys) {
   RCHK(check_sys(c, sys));
   if (!c->tables_ready || !c->pot_ready) return fail(OPAW_ERR_STATE, "tables / potentials not set");
   const long long N = c->N;
-  HCHK(nhat(c->st, c->datoms, c->natom, c->rhoij + (size_t)sys * c->mtot, c->mtot, c->qtab, c->nl2, c->sph_idx,
-            c->sph_g, N, 1, c->nhat + sys * N));
-  HCHK(renorm_factor(c->st, c->nhat + sys * N, c->dens + sys * N, N, 1, c->dx * c->dy * c->dz, c->rnel, c->part,
-                     c->fac + sys));
+  HCHK(nhat(c->st, c->datoms, c->natom, c->rhoij + (size_t)sys * c->mtot, c->mtot, c->qtab, c->nl2, sphere_tabs(c),
+            c->sph_g, N, 1, c->wl, c->nhat + sys * N));
+  HCHK(renorm_factor(c->st, c->nhat + sys * N, c->dens + sys * N, N, 1, dvol(c), c->rnel, c->part, c->fac + sys));
   HCHK(scale_pack(c->st, c->dens + sys * N, c->nhat + sys * N, c->fac + sys, N, 1, c->pair));
   return OPAW_OK;
 }
@@ -1302,6 +1484,7 @@ int opaw_excite_psi_bar_pert(opaw_ctx* c, int from_sys, int to_sys, double sm, i
   if (ipol < 1 || ipol > 3) return fail(OPAW_ERR_ARG, "ipol %d outside 1..3", ipol);
   const double d = ipol == 1 ? c->dx : ipol == 2 ? c->dy : c->dz;
   const size_t fs = (size_t)c->nb * c->N;
+  c->prep_valid = false;
   if (c->prec == OPAW_PREC_F64)
     HCHK(kick<double>(c->st, (const c64*)c->phi + from_sys * fs, (c64*)c->phi + to_sys * fs, c->nx, c->ny, c->nz, c->nb,
                       ipol - 1, d, sm));

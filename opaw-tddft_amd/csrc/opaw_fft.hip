@@ -18,7 +18,12 @@
 // Fused epilogues:
 //   EPI_MULT  (y/z pass): forward FFT, multiply by a real table T(k) (ek for
 //             the kinetic term, vk for Hartree), inverse FFT, store.
-//   EPI_HPSI  (x pass):   out = IFFT result + vks(sys) * u   (h_phi's local part)
+//   EPI_HPSI  (x pass):   out = IFFT result + vks(sys) * u (+ sum_a p_a wnl_a)
+//             (h_phi, src/hphi.f90:14-21 with addvnl's add-back gathered)
+// and a fused input correction: when cpre is given, the first pass loads
+// u = in + cpre (the compact S^-1/2 correction of the union of boxes,
+// src/snphi.f90), and the H.psi epilogue rebuilds the same u for its vks * u
+// term and adds the compact nonlocal correction cnl.
 #include "fft_core.h"
 #include "opaw_kernels.h"
 
@@ -62,7 +67,8 @@ __global__ __launch_bounds__(kThreads) void k_fft_x(const cx<R>* in, View vin,
                                                      const cx<R>* __restrict__ twg,
                                                      const cx<R>* __restrict__ u,
                                                      const R* __restrict__ vks, int nf_per_sys,
-                                                     long long nfield) {
+                                                     const int* __restrict__ umap, const c64* __restrict__ cpre,
+                                                     const c64* __restrict__ cnl, int U) {
   constexpr int LINES = x_lines<N>();
   __shared__ cx<R> tile[LINES * N];
   __shared__ cx<R> tw[N];
@@ -79,6 +85,13 @@ __global__ __launch_bounds__(kThreads) void k_fft_x(const cx<R>* in, View vin,
       const int z = L.z0 + (int)(r % L.lz);
       const int f = (int)(r / L.lz);
       val = in[vaddr(vin, f, x, y, z)];
+      if (EPI != EPI_HPSI && cpre) {
+        const int k = umap[vaddr(vin, 0, x, y, z)];
+        if (k >= 0) {
+          const c64 c = cpre[(long long)f * U + k];
+          val = mk<R>((R)((double)val.x + c.x), (R)((double)val.y + c.y));
+        }
+      }
     }
     tile[idx] = val;
   }
@@ -97,9 +110,26 @@ __global__ __launch_bounds__(kThreads) void k_fft_x(const cx<R>* in, View vin,
       if constexpr (EPI == EPI_HPSI) {
         const int sys = f / nf_per_sys;
         const long long pt = o - (long long)f * vout.fs;
-        const R v = vks[(long long)sys * vout.fs + pt];
+        const double v = (double)vks[(long long)sys * vout.fs + pt];
         const cx<R> uu = u[o];
-        val = mk(val.x + v * uu.x, val.y + v * uu.y);
+        double ur = (double)uu.x, ui = (double)uu.y;
+        double ar = (double)val.x, ai = (double)val.y;
+        if (cpre || cnl) {
+          const int k = umap[pt];
+          if (k >= 0) {
+            if (cpre) {
+              const c64 c = cpre[(long long)f * U + k];
+              ur += c.x;
+              ui += c.y;
+            }
+            if (cnl) {
+              const c64 c = cnl[(long long)f * U + k];
+              ar += c.x;
+              ai += c.y;
+            }
+          }
+        }
+        val = mk<R>((R)(ar + v * ur), (R)(ai + v * ui));
       }
       out[o] = val;
     }
@@ -177,12 +207,12 @@ bool fft_size_supported(int n) {
 template <typename R, int S, int EPI>
 static hipError_t launch_x_n(int n, dim3 grid, hipStream_t st, const cx<R>* in, View vin, cx<R>* out,
                              View vout, XLines L, const cx<R>* tw, const cx<R>* u, const R* vks,
-                             int nfps, long long nfield) {
+                             int nfps, const int* umap, const c64* cpre, const c64* cnl, int U) {
   switch (n) {
 #define OPAW_CASE(nn)                                                                   \
   case nn:                                                                              \
     hipLaunchKernelGGL((k_fft_x<R, nn, S, EPI>), grid, dim3(kThreads), 0, st, in, vin, \
-                       out, vout, L, tw, u, vks, nfps, nfield);                         \
+                       out, vout, L, tw, u, vks, nfps, umap, cpre, cnl, U);                         \
     return hipGetLastError();
     OPAW_FFT_SIZES(OPAW_CASE)
 #undef OPAW_CASE
@@ -213,7 +243,8 @@ static int x_lines_rt(int n) {
 template <typename R>
 hipError_t fft_x_pass(int cls, int n, int dir, int epi, hipStream_t st, const cx<R>* in, const View& vin,
                       cx<R>* out, const View& vout, const XLines& L, const cx<R>* tw,
-                      const cx<R>* u, const R* vks, int nfps) {
+                      const cx<R>* u, const R* vks, int nfps, const int* umap = nullptr,
+                      const c64* cpre = nullptr, const c64* cnl = nullptr, int U = 0) {
   const long long nlines = (long long)L.ly * L.lz * L.nf;
   if (nlines <= 0) return hipSuccess;
   const int lines = x_lines_rt(n);
@@ -221,11 +252,11 @@ hipError_t fft_x_pass(int cls, int n, int dir, int epi, hipStream_t st, const cx
   hipError_t e;
   prof_begin(st, cls);
   if (epi == EPI_HPSI)
-    e = launch_x_n<R, 1, EPI_HPSI>(n, grid, st, in, vin, out, vout, L, tw, u, vks, nfps, 0);
+    e = launch_x_n<R, 1, EPI_HPSI>(n, grid, st, in, vin, out, vout, L, tw, u, vks, nfps, umap, cpre, cnl, U);
   else if (dir < 0)
-    e = launch_x_n<R, -1, EPI_NONE>(n, grid, st, in, vin, out, vout, L, tw, u, vks, nfps, 0);
+    e = launch_x_n<R, -1, EPI_NONE>(n, grid, st, in, vin, out, vout, L, tw, u, vks, nfps, umap, cpre, cnl, U);
   else
-    e = launch_x_n<R, 1, EPI_NONE>(n, grid, st, in, vin, out, vout, L, tw, u, vks, nfps, 0);
+    e = launch_x_n<R, 1, EPI_NONE>(n, grid, st, in, vin, out, vout, L, tw, u, vks, nfps, umap, cpre, cnl, U);
   prof_end(st, cls);
   return e;
 }
@@ -253,12 +284,15 @@ hipError_t fft_s_pass(int cls, int n, int axis, int dir, int epi, hipStream_t st
 }
 
 // ---------------------------------------------------------------------------
-// Kinetic + local potential: out_f = IFFT(ek * FFT(u_f)) + vks(sys(f)) * u_f
-// (src/hphi.f90:14-21).  tmp must hold nf fields.
+// Kinetic + local potential (+ gathered projector terms):
+//   out_f = IFFT(ek * FFT(u_f)) + vks(sys(f)) * u_f [+ cnl_f],
+//   u_f = in_f [+ cpre_f]   (compact corrections on the union of boxes)
+// (src/hphi.f90:14-21 and the add-back of :42-60).  tmp must hold nf fields.
 template <typename R>
-hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* u, cx<R>* out, cx<R>* tmp,
+hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* in, cx<R>* out, cx<R>* tmp,
                           int nf, int nf_per_sys, const R* ek, const R* vks,
-                          const cx<R>* twx, const cx<R>* twy, const cx<R>* twz) {
+                          const cx<R>* twx, const cx<R>* twy, const cx<R>* twz,
+                          const int* umap, const c64* cpre, const c64* cnl, int U) {
   const long long N = (long long)g.nx * g.ny * g.nz;
   View v{N, 0, 0, 0, g.nx, g.ny};
   XLines Lx{0, g.ny, 0, g.nz, nf, 0, g.nx, 0, g.nx};
@@ -266,11 +300,14 @@ hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* u, cx<
   SLines Lz{g.nx, 0, g.ny, nf, 0, g.nz, 0, g.nz};
   View vt{0, 0, 0, 0, g.nx, g.ny};
   hipError_t e;
-  if ((e = fft_x_pass<R>(KC_FFT_X, g.nx, -1, EPI_NONE, st, u, v, tmp, v, Lx, twx, nullptr, nullptr, 1))) return e;
+  if ((e = fft_x_pass<R>(KC_FFT_X, g.nx, -1, EPI_NONE, st, in, v, tmp, v, Lx, twx, nullptr, nullptr, 1, umap, cpre,
+                         nullptr, U)))
+    return e;
   if ((e = fft_s_pass<R>(KC_FFT_Y, g.ny, 1, -1, EPI_NONE, st, tmp, v, tmp, v, Ly, twy, nullptr, vt))) return e;
   if ((e = fft_s_pass<R>(KC_FFT_Z, g.nz, 2, -1, EPI_MULT, st, tmp, v, tmp, v, Lz, twz, ek, vt))) return e;
   if ((e = fft_s_pass<R>(KC_FFT_Y, g.ny, 1, 1, EPI_NONE, st, tmp, v, tmp, v, Ly, twy, nullptr, vt))) return e;
-  return fft_x_pass<R>(KC_FFT_XH, g.nx, 1, EPI_HPSI, st, tmp, v, out, v, Lx, twx, u, vks, nf_per_sys);
+  return fft_x_pass<R>(KC_FFT_XH, g.nx, 1, EPI_HPSI, st, tmp, v, out, v, Lx, twx, in, vks, nf_per_sys, umap, cpre, cnl,
+                       U);
 }
 
 // Hartree convolution on the (scale*nx, scale*ny, scale*nz) box:
@@ -305,9 +342,11 @@ hipError_t hartree_conv(const GridGeom& g, int scale, hipStream_t st, const c64*
 }
 
 template hipError_t apply_t_plus_v<double>(const GridGeom&, hipStream_t, const c64*, c64*, c64*, int, int,
-                                           const double*, const double*, const c64*, const c64*, const c64*);
+                                           const double*, const double*, const c64*, const c64*, const c64*,
+                                           const int*, const c64*, const c64*, int);
 template hipError_t apply_t_plus_v<float>(const GridGeom&, hipStream_t, const c32*, c32*, c32*, int, int,
-                                          const float*, const float*, const c32*, const c32*, const c32*);
+                                          const float*, const float*, const c32*, const c32*, const c32*,
+                                          const int*, const c64*, const c64*, int);
 
 // plain 3D transform of nf fields (used by tests and the Hartree-free paths)
 template <typename R>

@@ -1,5 +1,6 @@
 // Declarations of the grid-field / small-matrix kernels (opaw_fields.hip).
 #pragma once
+#include "opaw_gather.h"
 #include "opaw_kernels.h"
 
 namespace opaw {
@@ -10,19 +11,29 @@ constexpr double kXcDensThreshold = 1e-15;  // per-point LDA density cutoff
 constexpr double kRhoijSelect = 1e-10;      // src/get_dij.f90:82
 constexpr double kDijMax = 500.0;           // src/0_parameter.f90 param_dij_max
 
+// union of the compensation spheres: point k owns entries [sptr[k],sptr[k+1])
+// = (atom, offset of its g value for L=0); g stride of atom a is sn[a]
+struct SphereTabs {
+  const int* smap;   // N: grid point -> sphere-union index or -1
+  const int* sptr;   // S+1
+  const int2* se;    // entries
+  const int* sn;     // per-atom sphere point count (g stride)
+};
+
 template <typename R>
-hipError_t rk4_update(hipStream_t st, int stage, long long n, const cx<R>* h, cx<R>* p, cx<R>* acc, cx<R>* y,
-                      double c, double dt);
+hipError_t rk4_update(hipStream_t st, int stage, long long N, int nf, const cx<R>* h, cx<R>* p, cx<R>* acc,
+                      cx<R>* y, double c, double dt, const int* umap, const c64* cc, int U);
 template <typename R>
 hipError_t field_norms(hipStream_t st, const cx<R>* u, long long N, int nf, double dv, double* part, double* out);
 template <typename R>
 hipError_t scale_fields(hipStream_t st, cx<R>* u, long long N, int nf, const double* num, const double* den);
 template <typename R>
-hipError_t density(hipStream_t st, const cx<R>* u, long long N, int nsys, int nb, double* dens);
+hipError_t density(hipStream_t st, const cx<R>* u, long long N, int nsys, int nb, double* dens, const int* umap,
+                   const c64* cpre, int U);
 hipError_t fill(hipStream_t st, double* p, long long n, double v);
 hipError_t nhat(hipStream_t st, const AtomDesc* atoms, int natom, const double* rhoij, int mtot,
-                const double* qtab, int nl2, const int* sph_idx, const double* sph_g, long long N, int nsys,
-                double* nh);
+                const double* qtab, int nl2, const SphereTabs& S, const double* sph_g, long long N, int nsys,
+                double* wl, double* nh);
 hipError_t renorm_factor(hipStream_t st, const double* nh, const double* dens, long long N, int nsys, double dv,
                          double rnel, double* part, double* fac);
 hipError_t scale_pack(hipStream_t st, double* dens, double* nh, const double* fac, long long N, int nsys,

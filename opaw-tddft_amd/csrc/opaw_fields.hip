@@ -45,17 +45,32 @@ __device__ __forceinline__ void chunk_range(long long n, int nchunk, int c, long
 }
 
 // ---------------------------------------------------------------------------
-// RK4 stage update (src/rk4.f90:35-56 and :22): k = -i h
+// RK4 stage update (src/rk4.f90:35-56 and :22): k = -i hs, where
+// hs = h + cc (the outer S^-1/2 of shs as a compact union correction) when cc
+// is set
 //   stage 1: acc = k;        y = p + c k
 //   stage 2,3: acc += 2k;    y = p + c k
 //   stage 4: p = p + dt/6 (acc + k)
 template <typename R>
-__global__ __launch_bounds__(kT) void k_rk4_update(int stage, long long n, const cx<R>* __restrict__ h,
+__global__ __launch_bounds__(kT) void k_rk4_update(int stage, long long N, int nf, const cx<R>* __restrict__ h,
                                                     cx<R>* __restrict__ p, cx<R>* __restrict__ acc,
-                                                    cx<R>* __restrict__ y, double c, double dt6) {
+                                                    cx<R>* __restrict__ y, double c, double dt6,
+                                                    const int* __restrict__ umap, const c64* __restrict__ cc, int U) {
+  const long long n = N * nf;
   for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n; i += (long long)gridDim.x * kT) {
     const cx<R> hh = h[i];
-    const double kr = (double)hh.y, ki = -(double)hh.x;
+    double hr = (double)hh.x, hi = (double)hh.y;
+    if (cc) {
+      const int f = (int)(i / N);
+      const long long pt = i - (long long)f * N;
+      const int k = umap[pt];
+      if (k >= 0) {
+        const c64 v = cc[(long long)f * U + k];
+        hr += v.x;
+        hi += v.y;
+      }
+    }
+    const double kr = hi, ki = -hr;
     const cx<R> pp = p[i];
     if (stage == 4) {
       const cx<R> a = acc[i];
@@ -89,15 +104,18 @@ __global__ __launch_bounds__(kT) void k_norm_partial(const cx<R>* __restrict__ u
   if (threadIdx.x == 0) part[(long long)f * nchunk + c] = v[0];
 }
 
-// out[r] = scale * sum_c part[r*nchunk + c]  (then sqrt if want_sqrt)
-__global__ void k_finish_rows(const double* __restrict__ part, int rows, int nchunk, double scale,
-                              int want_sqrt, double* __restrict__ out) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= rows) return;
-  double s = 0.0;
-  for (int c = 0; c < nchunk; ++c) s += part[(long long)r * nchunk + c];
-  s *= scale;
-  out[r] = want_sqrt ? sqrt(s) : s;
+// out[r] = scale * sum_c part[r*nchunk + c]  (then sqrt if want_sqrt); one
+// workgroup per row, fixed reduction tree
+__global__ __launch_bounds__(kT) void k_finish_rows(const double* __restrict__ part, int rows, int nchunk,
+                                                     double scale, int want_sqrt, double* __restrict__ out) {
+  const int r = blockIdx.x;
+  double v[1] = {0.0};
+  for (int c = threadIdx.x; c < nchunk; c += kT) v[0] += part[(long long)r * nchunk + c];
+  block_sum<1>(v);
+  if (threadIdx.x == 0) {
+    const double s = v[0] * scale;
+    out[r] = want_sqrt ? sqrt(s) : s;
+  }
 }
 
 // psi_f *= num[f] / den[f]   (src/rk4.f90:92-102, norm1)
@@ -114,52 +132,73 @@ __global__ __launch_bounds__(kT) void k_scale_fields(cx<R>* __restrict__ u, long
   }
 }
 
-// dens[s][r] = 2 sum_{o<nb} |psi_{s,o}(r)|^2   (src/tddft_dens.f90:65-76)
+// dens[s][r] = 2 sum_{o<nb} |psi~_{s,o}(r)|^2 (src/tddft_dens.f90:65-76) with
+// psi~ = psi + cpre (compact S^-1/2 correction) when cpre is set (on the fly,
+// src/ncpaw_make_hamiltonian.f90:228-235)
 template <typename R>
 __global__ __launch_bounds__(kT) void k_density(const cx<R>* __restrict__ u, long long N, int nsys, int nb,
-                                                 double* __restrict__ dens) {
+                                                 double* __restrict__ dens, const int* __restrict__ umap,
+                                                 const c64* __restrict__ cpre, int U) {
   const long long n = N * nsys;
   for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n; i += (long long)gridDim.x * kT) {
     const int s = (int)(i / N);
     const long long r = i - (long long)s * N;
     const cx<R>* us = u + (long long)s * nb * N + r;
+    const int k = cpre ? umap[r] : -1;
     double acc = 0.0;
     for (int o = 0; o < nb; ++o) {
       const cx<R> v = us[(long long)o * N];
-      acc += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+      double vr = (double)v.x, vi = (double)v.y;
+      if (k >= 0) {
+        const c64 c = cpre[(long long)(s * nb + o) * U + k];
+        vr += c.x;
+        vi += c.y;
+      }
+      acc += vr * vr + vi * vi;
     }
     dens[i] = 2.0 * acc;
   }
 }
 
-// compensation charge, one workgroup per system (src/nhat.f90:13-28)
-__global__ __launch_bounds__(kT) void k_nhat(const AtomDesc* __restrict__ atoms, int natom,
-                                              const double* __restrict__ rhoij, int mtot,
-                                              const double* __restrict__ qtab, int nl2,
-                                              const int* __restrict__ sph_idx,
-                                              const double* __restrict__ sph_g, long long N,
-                                              double* __restrict__ nhat) {
-  const int s = blockIdx.x;
-  double* nh = nhat + (long long)s * N;
-  const double* rho = rhoij + (long long)s * mtot;
-  __shared__ double wl[kMaxNl2];
-  for (int a = 0; a < natom; ++a) {
-    const AtomDesc A = atoms[a];
-    if (threadIdx.x < nl2) {
-      const int L = threadIdx.x;
-      const double* q = qtab + A.toff_q;
-      double w = 0.0;
-      for (int j = 0; j < A.ms; ++j)
-        for (int i = 0; i < A.ms; ++i) w += rho[A.moff + i + A.ms * j] * q[L + nl2 * (i + A.ms * j)];
-      wl[L] = w;
+// compensation charge (src/nhat.f90:13-28):
+//   nhat(r) = sum_a sum_L g_a(r, L) wl_a(L),  wl_a(L) = sum_ij rho_ij qijlm(L,i,j)
+// wl per (system, atom, L) first, then one thread per grid point gathers the
+// sphere entries of that point in atom order (no atomics).
+__global__ void k_nhat_w(const AtomDesc* __restrict__ atoms, int natom, const double* __restrict__ rhoij, int mtot,
+                         const double* __restrict__ qtab, int nl2, int nsys, double* __restrict__ wl) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nsys * natom * nl2) return;
+  const int L = q % nl2;
+  const int a = (q / nl2) % natom;
+  const int s = q / (nl2 * natom);
+  const AtomDesc A = atoms[a];
+  const double* rho = rhoij + (long long)s * mtot + A.moff;
+  const double* qq = qtab + A.toff_q;
+  double w = 0.0;
+  for (int j = 0; j < A.ms; ++j)
+    for (int i = 0; i < A.ms; ++i) w += rho[i + A.ms * j] * qq[L + nl2 * (i + A.ms * j)];
+  wl[q] = w;
+}
+
+__global__ __launch_bounds__(kT) void k_nhat_grid(SphereTabs S, const double* __restrict__ sph_g,
+                                                   const double* __restrict__ wl, int natom, int nl2, long long N,
+                                                   int nsys, double* __restrict__ nhat) {
+  const long long n = N * nsys;
+  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n; i += (long long)gridDim.x * kT) {
+    const int s = (int)(i / N);
+    const long long r = i - (long long)s * N;
+    const int k = S.smap[r];
+    double v = 0.0;
+    if (k >= 0) {
+      for (int e = S.sptr[k]; e < S.sptr[k + 1]; ++e) {
+        const int2 en = S.se[e];                  // (atom, g offset of L = 0)
+        const int a = en.x;
+        const int stride = S.sn[a];
+        const double* w = wl + ((long long)s * natom + a) * nl2;
+        for (int L = 0; L < nl2; ++L) v += sph_g[en.y + (long long)L * stride] * w[L];
+      }
     }
-    __syncthreads();
-    for (int p = threadIdx.x; p < A.sph_n; p += kT) {
-      double v = 0.0;
-      for (int L = 0; L < nl2; ++L) v += sph_g[(long long)A.sph_off * nl2 + (long long)L * A.sph_n + p] * wl[L];
-      nh[sph_idx[A.sph_off + p]] += v;
-    }
-    __syncthreads();
+    nhat[i] = v;
   }
 }
 
@@ -182,15 +221,17 @@ __global__ __launch_bounds__(kT) void k_sum2_partial(const double* __restrict__ 
   }
 }
 
-// fac[s] = rnel / (dv*sum nhat + dv*sum dens)
-__global__ void k_renorm_factor(const double* __restrict__ part, int nsys, int nchunk, double dv,
-                                double rnel, double* __restrict__ fac) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nsys) return;
-  double a = 0.0, b = 0.0;
-  for (int c = 0; c < nchunk; ++c) a += part[((long long)s * 2 + 0) * nchunk + c];
-  for (int c = 0; c < nchunk; ++c) b += part[((long long)s * 2 + 1) * nchunk + c];
-  fac[s] = rnel / (a * dv + b * dv);
+// fac[s] = rnel / (dv*sum nhat + dv*sum dens); one workgroup per system
+__global__ __launch_bounds__(kT) void k_renorm_factor(const double* __restrict__ part, int nsys, int nchunk,
+                                                       double dv, double rnel, double* __restrict__ fac) {
+  const int s = blockIdx.x;
+  double v[2] = {0.0, 0.0};
+  for (int c = threadIdx.x; c < nchunk; c += kT) {
+    v[0] += part[((long long)s * 2 + 0) * nchunk + c];
+    v[1] += part[((long long)s * 2 + 1) * nchunk + c];
+  }
+  block_sum<2>(v);
+  if (threadIdx.x == 0) fac[s] = rnel / (v[0] * dv + v[1] * dv);
 }
 
 // dens, nhat *= fac (when fac != null); rho pair for the Hartree solve:
@@ -527,16 +568,17 @@ static unsigned grid_for(long long n) {
 }
 
 template <typename R>
-hipError_t rk4_update(hipStream_t st, int stage, long long n, const cx<R>* h, cx<R>* p, cx<R>* acc, cx<R>* y,
-                      double c, double dt) {
-  hipLaunchKernelGGL((k_rk4_update<R>), dim3(grid_for(n)), dim3(kT), 0, st, stage, n, h, p, acc, y, c, dt / 6.0);
+hipError_t rk4_update(hipStream_t st, int stage, long long N, int nf, const cx<R>* h, cx<R>* p, cx<R>* acc,
+                      cx<R>* y, double c, double dt, const int* umap, const c64* cc, int U) {
+  hipLaunchKernelGGL((k_rk4_update<R>), dim3(grid_for(N * nf)), dim3(kT), 0, st, stage, N, nf, h, p, acc, y, c,
+                     dt / 6.0, umap, cc, U);
   return hipGetLastError();
 }
 
 template <typename R>
 hipError_t field_norms(hipStream_t st, const cx<R>* u, long long N, int nf, double dv, double* part, double* out) {
   hipLaunchKernelGGL((k_norm_partial<R>), dim3(kChunks, nf), dim3(kT), 0, st, u, N, kChunks, part);
-  hipLaunchKernelGGL(k_finish_rows, dim3((nf + 63) / 64), dim3(64), 0, st, part, nf, kChunks, dv, 1, out);
+  hipLaunchKernelGGL(k_finish_rows, dim3(nf), dim3(kT), 0, st, part, nf, kChunks, dv, 1, out);
   return hipGetLastError();
 }
 
@@ -547,8 +589,9 @@ hipError_t scale_fields(hipStream_t st, cx<R>* u, long long N, int nf, const dou
 }
 
 template <typename R>
-hipError_t density(hipStream_t st, const cx<R>* u, long long N, int nsys, int nb, double* dens) {
-  hipLaunchKernelGGL((k_density<R>), dim3(grid_for(N * nsys)), dim3(kT), 0, st, u, N, nsys, nb, dens);
+hipError_t density(hipStream_t st, const cx<R>* u, long long N, int nsys, int nb, double* dens, const int* umap,
+                   const c64* cpre, int U) {
+  hipLaunchKernelGGL((k_density<R>), dim3(grid_for(N * nsys)), dim3(kT), 0, st, u, N, nsys, nb, dens, umap, cpre, U);
   return hipGetLastError();
 }
 
@@ -558,19 +601,21 @@ hipError_t fill(hipStream_t st, double* p, long long n, double v) {
 }
 
 hipError_t nhat(hipStream_t st, const AtomDesc* atoms, int natom, const double* rhoij, int mtot,
-                const double* qtab, int nl2, const int* sph_idx, const double* sph_g, long long N, int nsys,
-                double* nh) {
-  hipError_t e = fill(st, nh, N * nsys, 0.0);
-  if (e) return e;
-  hipLaunchKernelGGL(k_nhat, dim3(nsys), dim3(kT), 0, st, atoms, natom, rhoij, mtot, qtab, nl2, sph_idx, sph_g,
-                     N, nh);
+                const double* qtab, int nl2, const SphereTabs& S, const double* sph_g, long long N, int nsys,
+                double* wl, double* nh) {
+  if (natom > 0) {
+    const int q = nsys * natom * nl2;
+    hipLaunchKernelGGL(k_nhat_w, dim3((q + 127) / 128), dim3(128), 0, st, atoms, natom, rhoij, mtot, qtab, nl2, nsys,
+                       wl);
+  }
+  hipLaunchKernelGGL(k_nhat_grid, dim3(grid_for(N * nsys)), dim3(kT), 0, st, S, sph_g, wl, natom, nl2, N, nsys, nh);
   return hipGetLastError();
 }
 
 hipError_t renorm_factor(hipStream_t st, const double* nh, const double* dens, long long N, int nsys, double dv,
                          double rnel, double* part, double* fac) {
   hipLaunchKernelGGL(k_sum2_partial, dim3(kChunks, nsys), dim3(kT), 0, st, nh, dens, N, kChunks, part);
-  hipLaunchKernelGGL(k_renorm_factor, dim3(1), dim3(64), 0, st, part, nsys, kChunks, dv, rnel, fac);
+  hipLaunchKernelGGL(k_renorm_factor, dim3(nsys), dim3(kT), 0, st, part, nsys, kChunks, dv, rnel, fac);
   return hipGetLastError();
 }
 
@@ -584,7 +629,8 @@ hipError_t scale_pack(hipStream_t st, double* dens, double* nh, const double* fa
 hipError_t pair_means(hipStream_t st, const c64* pair, long long N, int nsys, double* part, double* means) {
   const int npair = (nsys + 1) / 2;
   hipLaunchKernelGGL(k_pair_sum_partial, dim3(kChunks, npair), dim3(kT), 0, st, pair, N, kChunks, part);
-  hipLaunchKernelGGL(k_finish_rows, dim3(1), dim3(64), 0, st, part, 2 * npair, kChunks, 1.0 / (double)N, 0, means);
+  hipLaunchKernelGGL(k_finish_rows, dim3(2 * npair), dim3(kT), 0, st, part, 2 * npair, kChunks, 1.0 / (double)N, 0,
+                     means);
   return hipGetLastError();
 }
 
@@ -618,7 +664,7 @@ hipError_t dipole(hipStream_t st, const cx<R>* u, int nx, int ny, int nz, int ns
                   double dz, double* part, double* out) {
   hipLaunchKernelGGL((k_dipole_partial<R>), dim3(kChunks, nsys), dim3(kT), 0, st, u, nx, ny, nz, nb, kChunks, dx, dy,
                      dz, part);
-  hipLaunchKernelGGL(k_finish_rows, dim3(1), dim3(64), 0, st, part, 3 * nsys, kChunks, dx * dy * dz, 0, out);
+  hipLaunchKernelGGL(k_finish_rows, dim3(3 * nsys), dim3(kT), 0, st, part, 3 * nsys, kChunks, dx * dy * dz, 0, out);
   return hipGetLastError();
 }
 
@@ -653,11 +699,12 @@ hipError_t convert_to_c64(hipStream_t st, const cx<R>* in, c64* out, long long n
 }
 
 #define OPAW_INST(R)                                                                                            \
-  template hipError_t rk4_update<R>(hipStream_t, int, long long, const cx<R>*, cx<R>*, cx<R>*, cx<R>*, double,  \
-                                    double);                                                                    \
+  template hipError_t rk4_update<R>(hipStream_t, int, long long, int, const cx<R>*, cx<R>*, cx<R>*, cx<R>*,     \
+                                    double, double, const int*, const c64*, int);                               \
   template hipError_t field_norms<R>(hipStream_t, const cx<R>*, long long, int, double, double*, double*);      \
   template hipError_t scale_fields<R>(hipStream_t, cx<R>*, long long, int, const double*, const double*);       \
-  template hipError_t density<R>(hipStream_t, const cx<R>*, long long, int, int, double*);                      \
+  template hipError_t density<R>(hipStream_t, const cx<R>*, long long, int, int, double*, const int*,          \
+                                 const c64*, int);                                                              \
   template hipError_t dipole<R>(hipStream_t, const cx<R>*, int, int, int, int, int, double, double, double,     \
                                 double*, double*);                                                              \
   template hipError_t kick<R>(hipStream_t, const cx<R>*, cx<R>*, int, int, int, int, int, double, double);      \

@@ -1,0 +1,160 @@
+! iso_c_binding interfaces to libopaw_hip.so (include/opaw_hip.h) and the
+! drop-in wrappers a Fortran host would call in place of the reference's
+! external subroutines.  The module keeps one context (the device copy of
+! main_mod / atom_mod / tddft_mod state); arrays cross the boundary in the
+! reference layouts (column-major, complex*16 interleaved).
+module opaw_hip_mod
+  use iso_c_binding
+  implicit none
+  private
+
+  type(c_ptr), save, public :: opaw_ctx = c_null_ptr
+
+  interface
+    integer(c_int) function opaw_create(ctx, nx, ny, nz, dx, dy, dz, periodic, precision, nsys, nb) &
+        bind(C, name="opaw_create")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr) :: ctx
+      integer(c_int), value :: nx, ny, nz, periodic, precision, nsys, nb
+      real(c_double), value :: dx, dy, dz
+    end function
+    integer(c_int) function opaw_destroy(ctx) bind(C, name="opaw_destroy")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function opaw_ek_prep(ctx, ekcut) bind(C, name="opaw_ek_prep")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: ekcut
+    end function
+    integer(c_int) function opaw_set_atoms(ctx, natom, atom_map, ir_start, ntype, nrough, mstates, nl2) &
+        bind(C, name="opaw_set_atoms")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: natom, ntype, nl2
+      integer(c_int) :: atom_map(*), ir_start(*), nrough(*), mstates(*)
+    end function
+    integer(c_int) function opaw_set_projectors(ctx, ia, p3d, p3d1, s, sinv) bind(C, name="opaw_set_projectors")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ia
+      real(c_double) :: p3d(*), p3d1(*), s(*), sinv(*)
+    end function
+    integer(c_int) function opaw_set_sphere(ctx, ia, ngrid, local_grid, local_g3d) bind(C, name="opaw_set_sphere")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ia, ngrid
+      integer(c_int) :: local_grid(*)
+      real(c_double) :: local_g3d(*)
+    end function
+    integer(c_int) function opaw_set_type(ctx, it, qijlm, dij0, eijkl) bind(C, name="opaw_set_type")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: it
+      real(c_double) :: qijlm(*), dij0(*), eijkl(*)
+    end function
+    integer(c_int) function opaw_finalize_tables(ctx) bind(C, name="opaw_finalize_tables")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function opaw_set_potentials(ctx, vloc_tot, ncoret, rnel, usexcnhat) &
+        bind(C, name="opaw_set_potentials")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: vloc_tot(*), ncoret(*)
+      real(c_double), value :: rnel
+      integer(c_int), value :: usexcnhat
+    end function
+    integer(c_int) function opaw_vk_prep(ctx) bind(C, name="opaw_vk_prep")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function opaw_write_orbitals(ctx, sys, host) bind(C, name="opaw_write_orbitals")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, host
+      integer(c_int), value :: sys
+    end function
+    integer(c_int) function opaw_read_orbitals(ctx, sys, host) bind(C, name="opaw_read_orbitals")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, host
+      integer(c_int), value :: sys
+    end function
+    integer(c_int) function opaw_make_hamiltonian_tddft(ctx, sys) bind(C, name="opaw_make_hamiltonian_tddft")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: sys
+    end function
+    integer(c_int) function opaw_rk4_prop_shs(ctx, dt) bind(C, name="opaw_rk4_prop_shs")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dt
+    end function
+    integer(c_int) function opaw_plot_dip(ctx, sm, d_dip, dip, dip_pert) bind(C, name="opaw_plot_dip")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: sm
+      real(c_double) :: d_dip(3), dip(3), dip_pert(3)
+    end function
+    integer(c_int) function opaw_excite_psi_bar_pert(ctx, from_sys, to_sys, sm, ipol) &
+        bind(C, name="opaw_excite_psi_bar_pert")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: from_sys, to_sys, ipol
+      real(c_double), value :: sm
+    end function
+    function opaw_last_error() bind(C, name="opaw_last_error")
+      import :: c_ptr
+      type(c_ptr) :: opaw_last_error
+    end function
+  end interface
+
+  public :: opaw_create, opaw_destroy, opaw_ek_prep, opaw_set_atoms, opaw_set_projectors, opaw_set_sphere, &
+            opaw_set_type, opaw_finalize_tables, opaw_set_potentials, opaw_vk_prep, opaw_write_orbitals, &
+            opaw_read_orbitals, opaw_make_hamiltonian_tddft, opaw_rk4_prop_shs, opaw_plot_dip, &
+            opaw_excite_psi_bar_pert, opaw_check
+  public :: gpu_rk4_prop_shs, gpu_ncpaw_make_hamiltonian_tddft, gpu_plot_dip
+
+contains
+
+  ! reference convention: any failure stops the rank (src/lib/library_lib2.f:338)
+  subroutine opaw_check(stat, what)
+    integer(c_int), intent(in) :: stat
+    character(len=*), intent(in) :: what
+    character(kind=c_char), pointer :: msg(:)
+    integer :: n
+    if (stat /= 0) then
+      call c_f_pointer(opaw_last_error(), msg, [512])
+      n = 0
+      do while (n < 512)
+        if (msg(n + 1) == c_null_char) exit
+        n = n + 1
+      end do
+      write(6, *) 'opaw_hip: ', what, ' failed: ', msg(1:n)
+      error stop 'opaw_hip failure'
+    end if
+  end subroutine
+
+  ! drop-in for rk4_prop_shs(p, pp, n, nb) (src/rk4.f90:1): the orbitals are
+  ! resident on the device; p / pp are refreshed from the device afterwards
+  subroutine gpu_rk4_prop_shs(p, pp, n, nb, dt)
+    integer, intent(in) :: n, nb
+    complex(c_double_complex), target :: p(n, nb), pp(n, nb)
+    real(c_double), intent(in) :: dt
+    call opaw_check(opaw_rk4_prop_shs(opaw_ctx, dt), 'rk4_prop_shs')
+    call opaw_check(opaw_read_orbitals(opaw_ctx, 0_c_int, c_loc(p)), 'read phi_bar')
+    call opaw_check(opaw_read_orbitals(opaw_ctx, 1_c_int, c_loc(pp)), 'read phi_bar_pert')
+  end subroutine
+
+  ! drop-in for ncpaw_make_hamiltonian_tddft (src/ncpaw_make_hamiltonian.f90:118)
+  subroutine gpu_ncpaw_make_hamiltonian_tddft()
+    call opaw_check(opaw_make_hamiltonian_tddft(opaw_ctx, -1_c_int), 'make_hamiltonian_tddft')
+  end subroutine
+
+  ! drop-in for the dipole part of plot_dip (src/tddft_plot.f90:1-57)
+  subroutine gpu_plot_dip(sm, d_dip, dip, dip_pert)
+    real(c_double), intent(in) :: sm
+    real(c_double) :: d_dip(3), dip(3), dip_pert(3)
+    call opaw_check(opaw_plot_dip(opaw_ctx, sm, d_dip, dip, dip_pert), 'plot_dip')
+  end subroutine
+
+end module opaw_hip_mod
