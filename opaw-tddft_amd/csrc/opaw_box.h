@@ -5,7 +5,7 @@
 
 namespace opaw {
 
-constexpr int kProjChunk = 256;   // box points per projection workgroup
+constexpr int kProjChunk = 64;    // box points per projection workgroup
 
 struct BoxTables {
   const AtomDesc* atoms;     // device
@@ -31,13 +31,15 @@ hipError_t box_coeffs(const GridGeom& g, const BoxTables& bt, hipStream_t st, co
                       const c64* cpre, double dv, double* part, c64* ca, int mode, const double* sn,
                       const double* dij, int nf_per_sys, int mtot, c64* w);
 
-// compact correction corr (nf x U) = sum over atom entries of tab * w
+// compact correction corr(k, f) (row k of U rows, field stride 1, row stride
+// nf) = sum over the atom entries of union point k of tab * w, where the
+// weights are coefficient-major, w[c*nf + f]
 hipError_t union_corr(const BoxTables& bt, hipStream_t st, int nf, int which, const c64* w, c64* corr);
 
 hipError_t box_weights(const BoxTables& bt, hipStream_t st, const c64* ca, int nf, int mode, const double* sn,
                        const double* dij, int nf_per_sys, int mtot, c64* w);
 
-// out_f(upt[k]) += corr(f, k), in place
+// out_f(upt[k]) += corr(k, f), in place
 template <typename R>
 hipError_t union_apply(const GridGeom& g, const BoxTables& bt, hipStream_t st, cx<R>* out, int nf, const c64* corr);
 

@@ -2,7 +2,7 @@
 //
 // Projection  ca(f, coff_a + i) = dv * sum_{box_a} u_f(r) p_{a,i}(r)
 // (src/proj.f90:1-44, src/proj1.f90:1-46; the tables are real at the gamma
-// point so conj(p) = p).  A workgroup owns a 256-point chunk of one atom's box
+// point so conj(p) = p).  A workgroup owns a 64-point chunk of one atom's box
 // and 16 fields: 16 lanes walk the chunk's points (consecutive points ->
 // consecutive lanes, coalesced box rows), the 16 lane groups are the fields,
 // each lane accumulates ms complex sums, and the 16 point lanes are reduced
@@ -15,7 +15,7 @@
 // the fly), so a projection of S^-1/2 psi never needs S^-1/2 psi in memory.
 //
 // Add-back: union_corr turns a weight set into the compact correction
-// corr(f, k) over the union of boxes (opaw_gather.h); consumers add it.
+// corr(k, f) over the union of boxes (opaw_gather.h); consumers add it.
 #include "opaw_box.h"
 
 namespace opaw {
@@ -42,12 +42,13 @@ __global__ __launch_bounds__(kBoxThreads) void k_proj(const AtomDesc* __restrict
   const int kend = min(A.npt, cd.y + kProjChunk);
   if (f < nf) {
     const cx<R>* uf = u + (long long)f * N;
+#pragma unroll 4
     for (int k = cd.y + kl; k < kend; k += kPointLanes) {
       const int pt = bpt[A.boff + k];
       const cx<R> v = uf[pt];
       double vr = (double)v.x, vi = (double)v.y;
       if constexpr (PRE) {
-        const c64 c = cpre[(long long)f * U + bk[A.boff + k]];
+        const c64 c = cpre[(long long)bk[A.boff + k] * nf + f];
         vr += c.x;
         vi += c.y;
       }
@@ -125,13 +126,13 @@ __global__ __launch_bounds__(128) void k_proj_finish(const AtomDesc* __restrict_
       }
       out = mk(xr, xi);
     }
-    w[(long long)f * ctot + c] = out;
+    w[(long long)c * nf + f] = out;
   }
 }
 
 // w = D(sys) ca or sn * ca from existing coefficients, one workgroup per field
 __global__ __launch_bounds__(128) void k_weights(const AtomDesc* __restrict__ atoms, const int2* __restrict__ coef,
-                                                 const c64* __restrict__ ca, int ctot, int mode,
+                                                 const c64* __restrict__ ca, int nf, int ctot, int mode,
                                                  const double* __restrict__ sn, const double* __restrict__ dij,
                                                  int nf_per_sys, int mtot, c64* __restrict__ w) {
   const int f = blockIdx.x;
@@ -153,29 +154,49 @@ __global__ __launch_bounds__(128) void k_weights(const AtomDesc* __restrict__ at
       }
       out = mk(xr, xi);
     }
-    w[(long long)f * ctot + c] = out;
+    w[(long long)c * nf + f] = out;
   }
 }
 
-// corr(f, k) for every union point k (x) and field f (y)
+// corr(k, f) for every union point k: one wave per point, lanes = fields
+// (weights are coefficient-major, w[c*nf + f], so lane loads coalesce and the
+// table value is shared by the wave)
 __global__ __launch_bounds__(kBoxThreads) void k_union_corr(UnionTabs U, const double* __restrict__ tab,
-                                                            const c64* __restrict__ w, int ctot,
+                                                            const c64* __restrict__ w, int nf,
                                                             c64* __restrict__ corr) {
-  const int k = blockIdx.x * kBoxThreads + threadIdx.x;
-  const int f = blockIdx.y;
+  const int k = blockIdx.x * (kBoxThreads / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (k >= U.U) return;
-  corr[(long long)f * U.U + k] = gather_corr(U, tab, k, w + (long long)f * ctot);
+  const int e0 = U.uptr[k], e1 = U.uptr[k + 1];
+  for (int f0 = 0; f0 < nf; f0 += 64) {
+    const int f = f0 + lane;
+    double xr = 0.0, xi = 0.0;
+    for (int e = e0; e < e1; ++e) {
+      const UEntry E = U.ue[e];
+      const double* t = tab + E.toff;
+      const c64* ww = w + (long long)E.coff * nf + f;
+      for (int i = 0; i < E.ms; ++i) {
+        const double tv = t[i];
+        if (f < nf) {
+          const c64 wv = ww[(long long)i * nf];
+          xr += tv * wv.x;
+          xi += tv * wv.y;
+        }
+      }
+    }
+    if (f < nf) corr[(long long)k * nf + f] = mk(xr, xi);
+  }
 }
 
-// out_f(upt[k]) += corr(f, k)
+// out_f(upt[k]) += corr(k, f)
 template <typename R>
 __global__ __launch_bounds__(kBoxThreads) void k_union_apply(const int* __restrict__ upt, int U,
-                                                             cx<R>* __restrict__ out, long long N,
+                                                             cx<R>* __restrict__ out, long long N, int nf,
                                                              const c64* __restrict__ corr) {
   const int k = blockIdx.x * kBoxThreads + threadIdx.x;
   const int f = blockIdx.y;
   if (k >= U) return;
-  const c64 c = corr[(long long)f * U + k];
+  const c64 c = corr[(long long)k * nf + f];
   const long long o = (long long)f * N + upt[k];
   cx<R> v = out[o];
   v.x = (R)((double)v.x + c.x);
@@ -227,7 +248,7 @@ hipError_t box_weights(const BoxTables& bt, hipStream_t st, const c64* ca, int n
                        const double* dij, int nf_per_sys, int mtot, c64* w) {
   if (bt.natom == 0 || nf == 0) return hipSuccess;
   prof_begin(st, KC_BOX_ADD);
-  hipLaunchKernelGGL(k_weights, dim3(nf), dim3(128), 0, st, bt.atoms, bt.coef, ca, bt.ctot, mode, sn, dij,
+  hipLaunchKernelGGL(k_weights, dim3(nf), dim3(128), 0, st, bt.atoms, bt.coef, ca, nf, bt.ctot, mode, sn, dij,
                      nf_per_sys, mtot, w);
   prof_end(st, KC_BOX_ADD);
   return hipGetLastError();
@@ -236,8 +257,9 @@ hipError_t box_weights(const BoxTables& bt, hipStream_t st, const c64* ca, int n
 hipError_t union_corr(const BoxTables& bt, hipStream_t st, int nf, int which, const c64* w, c64* corr) {
   if (bt.natom == 0 || nf == 0 || bt.U.U == 0) return hipSuccess;
   prof_begin(st, KC_BOX_ADD);
-  hipLaunchKernelGGL(k_union_corr, dim3((bt.U.U + kBoxThreads - 1) / kBoxThreads, nf), dim3(kBoxThreads), 0, st,
-                     bt.U, which ? bt.U.p1 : bt.U.p, w, bt.ctot, corr);
+  const int per = kBoxThreads / 64;
+  hipLaunchKernelGGL(k_union_corr, dim3((bt.U.U + per - 1) / per), dim3(kBoxThreads), 0, st, bt.U,
+                     which ? bt.U.p1 : bt.U.p, w, nf, corr);
   prof_end(st, KC_BOX_ADD);
   return hipGetLastError();
 }
@@ -248,7 +270,7 @@ hipError_t union_apply(const GridGeom& g, const BoxTables& bt, hipStream_t st, c
   const long long N = (long long)g.nx * g.ny * g.nz;
   prof_begin(st, KC_BOX_ADD);
   hipLaunchKernelGGL((k_union_apply<R>), dim3((bt.U.U + kBoxThreads - 1) / kBoxThreads, nf), dim3(kBoxThreads), 0,
-                     st, bt.U.upt, bt.U.U, out, N, corr);
+                     st, bt.U.upt, bt.U.U, out, N, nf, corr);
   prof_end(st, KC_BOX_ADD);
   return hipGetLastError();
 }

@@ -1085,7 +1085,7 @@ static int flow_h_phi(opaw_ctx* c, int sys0, const cx<R>* in, cx<R>* out, int nf
   HCHK(union_corr(bt, c->st, nf, 0, (const c64*)c->wB.p, (c64*)c->corrNL.p));
   HCHK(apply_t_plus_v<R>(c->g, c->st, in, out, (cx<R>*)c->wtmp.p, nf, nf_per_sys, Fields<R>::ek(c),
                          Fields<R>::vks(c) + (size_t)sys0 * c->N, TW<R>::get(c, c->nx), TW<R>::get(c, c->ny),
-                         TW<R>::get(c, c->nz), c->umap, nullptr, (const c64*)c->corrNL.p, c->U));
+                         TW<R>::get(c, c->nz), c->umap, nullptr, (const c64*)c->corrNL.p, nf));
   return OPAW_OK;
 }
 
@@ -1101,7 +1101,7 @@ static int flow_hs(opaw_ctx* c, int sys0, const cx<R>* in, cx<R>* h, int nf, int
   HCHK(union_corr(bt, c->st, nf, 0, (const c64*)c->wB.p, (c64*)c->corrNL.p));
   HCHK(apply_t_plus_v<R>(c->g, c->st, in, h, (cx<R>*)c->wtmp.p, nf, nf_per_sys, Fields<R>::ek(c),
                          Fields<R>::vks(c) + (size_t)sys0 * c->N, TW<R>::get(c, c->nx), TW<R>::get(c, c->ny),
-                         TW<R>::get(c, c->nz), c->umap, (const c64*)c->corrA.p, (const c64*)c->corrNL.p, c->U));
+                         TW<R>::get(c, c->nz), c->umap, (const c64*)c->corrA.p, (const c64*)c->corrNL.p, nf));
   HCHK(box_coeffs<R>(c->g, bt, c->st, h, nf, 1, nullptr, dvol(c), (double*)c->ppart.p, (c64*)c->cC.p, 1, c->sn[1],
                      nullptr, nf, c->mtot, (c64*)c->wC.p));
   HCHK(union_corr(bt, c->st, nf, 1, (const c64*)c->wC.p, (c64*)c->corrC.p));
@@ -1137,7 +1137,7 @@ static int flow_rebuild(opaw_ctx* c, int s0, int ns, const cx<R>* phi) {
              c->rhoij + (size_t)s0 * c->mtot));
   prof_end(c->st, KC_DENS);
   prof_begin(c->st, KC_DENS);
-  HCHK(density<R>(c->st, phi, N, ns, c->nb, c->dens + s0 * N, c->umap, (const c64*)c->corrA.p, c->U));
+  HCHK(density<R>(c->st, phi, N, ns, c->nb, c->dens + s0 * N, c->umap, (const c64*)c->corrA.p, nf));
   prof_end(c->st, KC_DENS);
   prof_begin(c->st, KC_DENS);
   HCHK(nhat(c->st, c->datoms, c->natom, c->rhoij + (size_t)s0 * c->mtot, c->mtot, c->qtab, c->nl2, sphere_tabs(c),
@@ -1194,7 +1194,7 @@ static int flow_rk4(opaw_ctx* c, double dt) {
     RCHK(flow_hs<R>(c, 0, yin, h, nf, c->nb));
     prof_begin(c->st, KC_RK4);
     HCHK(rk4_update<R>(c->st, stage, c->N, nf, h, p, acc, y, cst[stage - 1], dt, c->umap, (const c64*)c->corrC.p,
-                       c->U));
+                       nf));
     prof_end(c->st, KC_RK4);
     if (stage < 4) RCHK(flow_rebuild<R>(c, 0, c->nsys, y));
   }

@@ -57,7 +57,13 @@ enum { EPI_NONE = 0, EPI_MULT = 1, EPI_HPSI = 2 };
 constexpr int kThreads = 256;
 
 template <int N>
-constexpr int x_lines() { return (2048 / N) > 0 ? (2048 / N) : 1; }
+constexpr int x_lines() { return (1024 / N) > 0 ? (1024 / N) : 1; }
+
+// x-tile line stride: lines of a multiple-of-16 length would all start on the
+// same LDS bank group; a stride of N + 8 elements spreads a wave's lines over
+// both halves of the 16 bank groups that one 16-byte element spans.
+template <int N>
+constexpr int x_stride() { return (N % 16 == 0) ? N + 8 : N; }
 
 // ---------------------------------------------------------------------------
 // x pass
@@ -70,7 +76,8 @@ __global__ __launch_bounds__(kThreads) void k_fft_x(const cx<R>* in, View vin,
                                                      const int* __restrict__ umap, const c64* __restrict__ cpre,
                                                      const c64* __restrict__ cnl, int U) {
   constexpr int LINES = x_lines<N>();
-  __shared__ cx<R> tile[LINES * N];
+  constexpr int LS = x_stride<N>();
+  __shared__ cx<R> tile[LINES * LS];
   __shared__ cx<R> tw[N];
   fft::load_twiddles(tw, twg, N);
   const long long nlines = (long long)L.ly * L.lz * L.nf;
@@ -88,15 +95,15 @@ __global__ __launch_bounds__(kThreads) void k_fft_x(const cx<R>* in, View vin,
       if (EPI != EPI_HPSI && cpre) {
         const int k = umap[vaddr(vin, 0, x, y, z)];
         if (k >= 0) {
-          const c64 c = cpre[(long long)f * U + k];
+          const c64 c = cpre[(long long)k * U + f];
           val = mk<R>((R)((double)val.x + c.x), (R)((double)val.y + c.y));
         }
       }
     }
-    tile[idx] = val;
+    tile[l * LS + x] = val;
   }
   __syncthreads();
-  fft::lines<R, N, LINES, kThreads, false, N, 1, S>(tile, tw);
+  fft::lines<R, N, LINES, kThreads, false, LS, 1, S>(tile, tw);
   for (int idx = threadIdx.x; idx < LINES * N; idx += kThreads) {
     const int l = idx / N, x = idx % N;
     const long long gl = line0 + l;
@@ -106,7 +113,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_x(const cx<R>* in, View vin,
       const int z = L.z0 + (int)(r % L.lz);
       const int f = (int)(r / L.lz);
       const long long o = vaddr(vout, f, x, y, z);
-      cx<R> val = tile[idx];
+      cx<R> val = tile[l * LS + x];
       if constexpr (EPI == EPI_HPSI) {
         const int sys = f / nf_per_sys;
         const long long pt = o - (long long)f * vout.fs;
@@ -118,12 +125,12 @@ __global__ __launch_bounds__(kThreads) void k_fft_x(const cx<R>* in, View vin,
           const int k = umap[pt];
           if (k >= 0) {
             if (cpre) {
-              const c64 c = cpre[(long long)f * U + k];
+              const c64 c = cpre[(long long)k * U + f];
               ur += c.x;
               ui += c.y;
             }
             if (cnl) {
-              const c64 c = cnl[(long long)f * U + k];
+              const c64 c = cnl[(long long)k * U + f];
               ar += c.x;
               ai += c.y;
             }
@@ -190,6 +197,133 @@ __global__ __launch_bounds__(kThreads) void k_fft_s(const cx<R>* in, View vin,
 }
 
 // ---------------------------------------------------------------------------
+// xy-plane pass: a workgroup holds one (field, z) plane in LDS and transforms
+// its rows (x) and columns (y) in one visit to HBM, so the kinetic operator is
+// plane-forward, z (fwd * ek inv), plane-inverse: three passes instead of five.
+//   EPI_NONE, S=-1: u = in (+ cpre) -> FFT_xy -> out
+//   EPI_HPSI, S=+1: in -> IFFT_xy -> out = result + vks * u + cnl,
+//                   u = uin (+ cpre)
+constexpr int kPlaneThreads = 512;
+
+template <typename R, int NX, int NY>
+constexpr int plane_stride() {
+  // pad rows by 8 elements (spreads a wave's rows over the LDS bank groups)
+  // when the padded plane still fits the 160 KB of LDS
+  return ((NX % 16 == 0) && (long)NY * (NX + 8) * (long)sizeof(cx<R>) + (NX + NY) * (long)sizeof(cx<R>) <= 163840)
+             ? NX + 8
+             : NX;
+}
+
+template <typename R, int NX, int NY, int S, int EPI>
+__global__ __launch_bounds__(kPlaneThreads) void k_fft_plane(const cx<R>* in, cx<R>* out, int nz, long long N,
+                                                              const cx<R>* __restrict__ twxg,
+                                                              const cx<R>* __restrict__ twyg,
+                                                              const cx<R>* __restrict__ uin,
+                                                              const R* __restrict__ vks, int nf_per_sys,
+                                                              const int* __restrict__ umap,
+                                                              const c64* __restrict__ cpre,
+                                                              const c64* __restrict__ cnl, int U) {
+  constexpr int LS = plane_stride<R, NX, NY>();
+  __shared__ cx<R> tile[NY * LS];
+  __shared__ cx<R> twx[NX];
+  __shared__ cx<R> twy[NY];
+  fft::load_twiddles(twx, twxg, NX);
+  fft::load_twiddles(twy, twyg, NY);
+  const int f = blockIdx.x / nz;
+  const int z = blockIdx.x % nz;
+  const long long plane0 = (long long)z * NX * NY;           // offset of the plane in a field
+  const cx<R>* src = in + (long long)f * N + plane0;
+  for (int idx = threadIdx.x; idx < NX * NY; idx += kPlaneThreads) {
+    const int x = idx % NX, y = idx / NX;
+    cx<R> v = src[idx];
+    if (EPI == EPI_NONE && cpre) {
+      const int k = umap[plane0 + idx];
+      if (k >= 0) {
+        const c64 c = cpre[(long long)k * U + f];
+        v = mk<R>((R)((double)v.x + c.x), (R)((double)v.y + c.y));
+      }
+    }
+    tile[y * LS + x] = v;
+  }
+  __syncthreads();
+  if constexpr (S < 0) {
+    fft::lines<R, NX, NY, kPlaneThreads, false, LS, 1, S>(tile, twx);   // rows (x)
+    fft::lines<R, NY, NX, kPlaneThreads, true, 1, LS, S>(tile, twy);    // columns (y)
+  } else {
+    fft::lines<R, NY, NX, kPlaneThreads, true, 1, LS, S>(tile, twy);
+    fft::lines<R, NX, NY, kPlaneThreads, false, LS, 1, S>(tile, twx);
+  }
+  cx<R>* dst = out + (long long)f * N + plane0;
+  for (int idx = threadIdx.x; idx < NX * NY; idx += kPlaneThreads) {
+    const int x = idx % NX, y = idx / NX;
+    cx<R> v = tile[y * LS + x];
+    if constexpr (EPI == EPI_HPSI) {
+      const long long pt = plane0 + idx;
+      const double vv = (double)vks[(long long)(f / nf_per_sys) * N + pt];
+      const cx<R> uu = uin[(long long)f * N + pt];
+      double ur = (double)uu.x, ui = (double)uu.y;
+      double ar = (double)v.x, ai = (double)v.y;
+      if (cpre || cnl) {
+        const int k = umap[pt];
+        if (k >= 0) {
+          if (cpre) {
+            const c64 c = cpre[(long long)k * U + f];
+            ur += c.x;
+            ui += c.y;
+          }
+          if (cnl) {
+            const c64 c = cnl[(long long)k * U + f];
+            ar += c.x;
+            ai += c.y;
+          }
+        }
+      }
+      v = mk<R>((R)(ar + vv * ur), (R)(ai + vv * ui));
+    }
+    dst[idx] = v;
+  }
+}
+
+// square planes with a compiled plane kernel (FP32 also 128 x 128)
+#define OPAW_PLANE_SIZES(X) X(16) X(24) X(32) X(48) X(64) X(80) X(96)
+
+template <typename R>
+static bool plane_supported(int nx, int ny) {
+  if (nx != ny) return false;
+  switch (nx) {
+#define OPAW_CASE(n) case n: return true;
+    OPAW_PLANE_SIZES(OPAW_CASE)
+#undef OPAW_CASE
+    case 128: return sizeof(R) == 4;
+    default: return false;
+  }
+}
+
+template <typename R, int S, int EPI>
+static hipError_t launch_plane(int n, hipStream_t st, const cx<R>* in, cx<R>* out, int nz, int nf, long long N,
+                               const cx<R>* tw, const cx<R>* uin, const R* vks, int nfps, const int* umap,
+                               const c64* cpre, const c64* cnl, int U) {
+  dim3 grid((unsigned)((long long)nf * nz));
+  switch (n) {
+#define OPAW_CASE(nn)                                                                                          \
+  case nn:                                                                                                     \
+    hipLaunchKernelGGL((k_fft_plane<R, nn, nn, S, EPI>), grid, dim3(kPlaneThreads), 0, st, in, out, nz, N, tw, tw, \
+                       uin, vks, nfps, umap, cpre, cnl, U);                                                     \
+    return hipGetLastError();
+    OPAW_PLANE_SIZES(OPAW_CASE)
+#undef OPAW_CASE
+    case 128:
+      if constexpr (sizeof(R) == 4) {
+        hipLaunchKernelGGL((k_fft_plane<R, 128, 128, S, EPI>), grid, dim3(kPlaneThreads), 0, st, in, out, nz, N, tw,
+                           tw, uin, vks, nfps, umap, cpre, cnl, U);
+        return hipGetLastError();
+      }
+      return hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host-side dispatch over the compiled line lengths
 #define OPAW_FFT_SIZES(X) \
   X(4) X(5) X(6) X(8) X(10) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) X(80) X(96) \
@@ -236,7 +370,7 @@ static hipError_t launch_s_n(int n, dim3 grid, hipStream_t st, const cx<R>* in, 
 }
 
 static int x_lines_rt(int n) {
-  int l = 2048 / n;
+  int l = 1024 / n;
   return l > 0 ? l : 1;
 }
 
@@ -300,6 +434,18 @@ hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* in, cx
   SLines Lz{g.nx, 0, g.ny, nf, 0, g.nz, 0, g.nz};
   View vt{0, 0, 0, 0, g.nx, g.ny};
   hipError_t e;
+  if (plane_supported<R>(g.nx, g.ny)) {
+    prof_begin(st, KC_FFT_X);
+    e = launch_plane<R, -1, EPI_NONE>(g.nx, st, in, tmp, g.nz, nf, N, twx, nullptr, nullptr, nf_per_sys, umap, cpre,
+                                      nullptr, U);
+    prof_end(st, KC_FFT_X);
+    if (e) return e;
+    if ((e = fft_s_pass<R>(KC_FFT_Z, g.nz, 2, -1, EPI_MULT, st, tmp, v, tmp, v, Lz, twz, ek, vt))) return e;
+    prof_begin(st, KC_FFT_XH);
+    e = launch_plane<R, 1, EPI_HPSI>(g.nx, st, tmp, out, g.nz, nf, N, twx, in, vks, nf_per_sys, umap, cpre, cnl, U);
+    prof_end(st, KC_FFT_XH);
+    return e;
+  }
   if ((e = fft_x_pass<R>(KC_FFT_X, g.nx, -1, EPI_NONE, st, in, v, tmp, v, Lx, twx, nullptr, nullptr, 1, umap, cpre,
                          nullptr, U)))
     return e;

@@ -65,7 +65,7 @@ __global__ __launch_bounds__(kT) void k_rk4_update(int stage, long long N, int n
       const long long pt = i - (long long)f * N;
       const int k = umap[pt];
       if (k >= 0) {
-        const c64 v = cc[(long long)f * U + k];
+        const c64 v = cc[(long long)k * U + f];
         hr += v.x;
         hi += v.y;
       }
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kT) void k_density(const cx<R>* __restrict__ u, lon
       const cx<R> v = us[(long long)o * N];
       double vr = (double)v.x, vi = (double)v.y;
       if (k >= 0) {
-        const c64 c = cpre[(long long)(s * nb + o) * U + k];
+        const c64 c = cpre[(long long)k * U + (s * nb + o)];
         vr += c.x;
         vi += c.y;
       }

@@ -9,8 +9,8 @@
 //     corr(f, k) = sum_e sum_{i<ms_e} tab[toff_e + i] * w_f[coff_e + i]
 // -- the reference's per-atom add-back (src/snphi.f90:29-47,
 // src/hphi.f90:42-60) summed in atom order, without atomics.  It is computed
-// once per weight set into a compact (nf x U) array; streaming kernels add
-// corr(f, umap[r]) at grid points inside the union.
+// once per weight set into a compact array corr[k*nf + f]; streaming kernels
+// add corr(umap[r], f) at grid points inside the union.
 #pragma once
 #include "opaw_common.h"
 
@@ -36,7 +36,7 @@ struct UnionTabs {
 // compact corrections read by the streaming kernels
 struct Corr {
   const int* umap;      // N: grid point -> union index or -1
-  const c64* c;         // (nf x U) correction, field stride U; null = none
+  const c64* c;         // corr[k*nf + f]; null = none
   int U;
 };
 

@@ -18,8 +18,9 @@ void prof_end(hipStream_t st, int cls);
 bool fft_size_supported(int n);
 
 // out_f = IFFT(ek FFT(u_f)) + vks(sys) u_f [+ cnl_f], where u_f = in_f [+ cpre_f]
-// (compact corrections over the union of boxes, field stride U; umap maps a
-// grid point to its union index or -1)
+// (compact corrections over the union of boxes stored as corr[k*U + f]: U is
+// the row stride, i.e. the field count of the batch; umap maps a grid point to
+// its union index k or -1)
 template <typename R>
 hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* in, cx<R>* out, cx<R>* tmp,
                           int nf, int nf_per_sys, const R* ek, const R* vks,
