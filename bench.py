@@ -53,17 +53,18 @@ def table_bytes(spec):
 
 
 def class_bytes(spec, nsys, nb, sc):
-    """Algorithmic HBM bytes per launch for the kernel families of one
-    shs / rebuild application over all nsys*nb fields."""
+    """Algorithmic HBM bytes per launch of the single-kernel families of one
+    shs / rebuild application over all nsys*nb fields (fft_x / fft_xh are the
+    xy-plane passes on these grids; rk4 averages its four stage variants,
+    which move 4, 5, 5 and 4 field arrays)."""
     N, nf, Nb, T = spec.n, nsys * nb, box_points(spec), table_bytes(spec)
     sr = sc // 2
     return {
         "fft_x": 2 * nf * N * sc,
-        "fft_y": 2 * nf * N * sc,
         "fft_z": 2 * nf * N * sc + N * sr,
         "fft_xh": 3 * nf * N * sc + nsys * N * sr,
         "box_proj": nf * Nb * sc + T,
-        "box_add": 2 * nf * Nb * sc + T,
+        "rk4": 4.5 * nf * N * sc,
     }
 
 

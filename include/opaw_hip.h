@@ -46,15 +46,15 @@ enum { OPAW_FIELD_DENS = 0, OPAW_FIELD_NHAT = 1, OPAW_FIELD_VXC = 2, OPAW_FIELD_
 
 /* kernel families timed by the profiler (opaw_profile_read) */
 enum {
-  OPAW_K_FFT_X = 0,      /* kinetic: x forward line FFT                      */
-  OPAW_K_FFT_Y = 1,      /* kinetic: y forward / inverse line FFTs          */
+  OPAW_K_FFT_X = 0,      /* kinetic: xy-plane (or x-line) forward FFT        */
+  OPAW_K_FFT_Y = 1,      /* kinetic: y line FFTs (non-plane grids only)     */
   OPAW_K_FFT_Z = 2,      /* kinetic: z line FFT, * ek, inverse (fused)      */
-  OPAW_K_FFT_XH = 3,     /* kinetic: x inverse + vks*psi epilogue           */
+  OPAW_K_FFT_XH = 3,     /* kinetic: xy inverse + vks*psi + nonlocal epilogue */
   OPAW_K_HARTREE = 4,    /* Martyna-Tuckerman Hartree FFT passes            */
   OPAW_K_BOX_PROJ = 5,   /* projector contractions P^T psi                 */
-  OPAW_K_BOX_ADD = 6,    /* projector add-backs P w                        */
-  OPAW_K_RK4 = 7,        /* RK4 stage updates                               */
-  OPAW_K_DENS = 8,       /* density, rho_ij, nhat, norms, dipoles           */
+  OPAW_K_BOX_ADD = 6,    /* projection finish, weights, union corrections  */
+  OPAW_K_RK4 = 7,        /* RK4 stage update kernel                         */
+  OPAW_K_DENS = 8,       /* density, rho_ij, nhat, norms + rescale, dipoles */
   OPAW_K_POT = 9,        /* LDA, V_KS, D_ij, packing                        */
   OPAW_K_OTHER = 10,     /* plain fft3d and everything else                 */
   OPAW_K_COUNT = 11

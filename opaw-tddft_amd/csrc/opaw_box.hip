@@ -7,8 +7,8 @@
 // consecutive lanes, coalesced box rows), the 16 lane groups are the fields,
 // each lane accumulates ms complex sums, and the 16 point lanes are reduced
 // with four xor-shuffles.  One partial per chunk is written; a finishing
-// kernel sums the chunks of an atom in chunk order (deterministic) and forms
-// the weights
+// kernel sums the chunks of an atom with a fixed shuffle tree (deterministic)
+// and a weights kernel forms
 //     w = sn(n) * ca                      (sn_phi, src/snphi.f90:24-28)
 //     w = D(sys) ca                       (addvnl, src/hphi.f90:43-47)
 // With a compact correction cpre the input is u = in + cpre (S^-1/2 applied on
@@ -83,51 +83,30 @@ __global__ __launch_bounds__(kBoxThreads) void k_proj(const AtomDesc* __restrict
   }
 }
 
-// ca = dv * sum over the atom's chunks; then the weights (mode 0: none,
-// 1: sn * ca, 2: D(sys) ca).  One workgroup per field.
+// ca(f, c) = dv * sum over the chunks of c's atom: one wave per (field,
+// coefficient), lanes over chunks, fixed shuffle tree (deterministic)
 template <int MSP>
-__global__ __launch_bounds__(128) void k_proj_finish(const AtomDesc* __restrict__ atoms,
-                                                     const int2* __restrict__ coef, const int2* __restrict__ achunk,
+__global__ __launch_bounds__(256) void k_proj_finish(const int2* __restrict__ coef, const int2* __restrict__ achunk,
                                                      const double* __restrict__ part, int nf, int ctot, double dv,
-                                                     c64* __restrict__ ca, int mode, const double* __restrict__ sn,
-                                                     const double* __restrict__ dij, int nf_per_sys, int mtot,
-                                                     c64* __restrict__ w) {
-  const int f = blockIdx.x;
-  extern __shared__ c64 row[];
-  for (int c = threadIdx.x; c < ctot; c += blockDim.x) {
-    const int2 ai = coef[c];                     // (atom, i)
-    const int2 cr = achunk[ai.x];                // (first chunk, chunk count)
-    double sr = 0.0, si = 0.0;
-    for (int ch = cr.x; ch < cr.x + cr.y; ++ch) {
-      const double* pp = part + (((long long)ch * nf + f) * MSP + ai.y) * 2;
-      sr += pp[0];
-      si += pp[1];
-    }
-    const c64 v = mk(sr * dv, si * dv);
-    ca[(long long)f * ctot + c] = v;
-    row[c] = v;
+                                                     c64* __restrict__ ca) {
+  const int f = blockIdx.y;
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (c >= ctot) return;
+  const int2 ai = coef[c];                       // (atom, i)
+  const int2 cr = achunk[ai.x];                  // (first chunk, chunk count)
+  double sr = 0.0, si = 0.0;
+  for (int q = lane; q < cr.y; q += 64) {
+    const double* pp = part + (((long long)(cr.x + q) * nf + f) * MSP + ai.y) * 2;
+    sr += pp[0];
+    si += pp[1];
   }
-  if (mode == 0) return;
-  __syncthreads();
-  for (int c = threadIdx.x; c < ctot; c += blockDim.x) {
-    const int2 ai = coef[c];
-    c64 out;
-    if (mode == 1) {
-      const double s = sn[c];
-      out = mk(s * row[c].x, s * row[c].y);
-    } else {
-      const AtomDesc A = atoms[ai.x];
-      const double* D = dij + (long long)(f / nf_per_sys) * mtot + A.moff;
-      double xr = 0.0, xi = 0.0;
-      for (int j = 0; j < A.ms; ++j) {
-        const double d = D[ai.y + A.ms * j];
-        xr += d * row[A.coff + j].x;
-        xi += d * row[A.coff + j].y;
-      }
-      out = mk(xr, xi);
-    }
-    w[(long long)c * nf + f] = out;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    sr += __shfl_xor(sr, off, 64);
+    si += __shfl_xor(si, off, 64);
   }
+  if (lane == 0) ca[(long long)f * ctot + c] = mk(sr * dv, si * dv);
 }
 
 // w = D(sys) ca or sn * ca from existing coefficients, one workgroup per field
@@ -160,7 +139,9 @@ __global__ __launch_bounds__(128) void k_weights(const AtomDesc* __restrict__ at
 
 // corr(k, f) for every union point k: one wave per point, lanes = fields
 // (weights are coefficient-major, w[c*nf + f], so lane loads coalesce and the
-// table value is shared by the wave)
+// table value is shared by the wave); the ms loads of an entry are issued
+// together (unrolled to MSP with a guard)
+template <int MSP>
 __global__ __launch_bounds__(kBoxThreads) void k_union_corr(UnionTabs U, const double* __restrict__ tab,
                                                             const c64* __restrict__ w, int nf,
                                                             c64* __restrict__ corr) {
@@ -170,21 +151,27 @@ __global__ __launch_bounds__(kBoxThreads) void k_union_corr(UnionTabs U, const d
   const int e0 = U.uptr[k], e1 = U.uptr[k + 1];
   for (int f0 = 0; f0 < nf; f0 += 64) {
     const int f = f0 + lane;
+    const bool on = f < nf;
     double xr = 0.0, xi = 0.0;
     for (int e = e0; e < e1; ++e) {
       const UEntry E = U.ue[e];
       const double* t = tab + E.toff;
-      const c64* ww = w + (long long)E.coff * nf + f;
-      for (int i = 0; i < E.ms; ++i) {
-        const double tv = t[i];
-        if (f < nf) {
-          const c64 wv = ww[(long long)i * nf];
-          xr += tv * wv.x;
-          xi += tv * wv.y;
-        }
+      const c64* ww = w + (long long)E.coff * nf + (on ? f : 0);
+      double tv[MSP];
+      c64 wv[MSP];
+#pragma unroll
+      for (int i = 0; i < MSP; ++i) {
+        const bool ok = i < E.ms;
+        tv[i] = ok ? t[i] : 0.0;
+        wv[i] = ok ? ww[(long long)i * nf] : mk(0.0, 0.0);
+      }
+#pragma unroll
+      for (int i = 0; i < MSP; ++i) {
+        xr += tv[i] * wv[i].x;
+        xi += tv[i] * wv[i].y;
       }
     }
-    if (f < nf) corr[(long long)k * nf + f] = mk(xr, xi);
+    if (on) corr[(long long)k * nf + f] = mk(xr, xi);
   }
 }
 
@@ -233,15 +220,15 @@ hipError_t box_coeffs(const GridGeom& g, const BoxTables& bt, hipStream_t st, co
   hipError_t e = hipGetLastError();
   if (e) return e;
   prof_begin(st, KC_BOX_ADD);
-  const size_t shm = (size_t)bt.ctot * sizeof(c64);
+  const dim3 fg((bt.ctot + 3) / 4, nf);
   if (bt.msp == 8)
-    hipLaunchKernelGGL(k_proj_finish<8>, dim3(nf), dim3(128), shm, st, bt.atoms, bt.coef, bt.achunk, part, nf,
-                       bt.ctot, dv, ca, mode, sn, dij, nf_per_sys, mtot, w);
+    hipLaunchKernelGGL(k_proj_finish<8>, fg, dim3(256), 0, st, bt.coef, bt.achunk, part, nf, bt.ctot, dv, ca);
   else
-    hipLaunchKernelGGL(k_proj_finish<16>, dim3(nf), dim3(128), shm, st, bt.atoms, bt.coef, bt.achunk, part, nf,
-                       bt.ctot, dv, ca, mode, sn, dij, nf_per_sys, mtot, w);
+    hipLaunchKernelGGL(k_proj_finish<16>, fg, dim3(256), 0, st, bt.coef, bt.achunk, part, nf, bt.ctot, dv, ca);
   prof_end(st, KC_BOX_ADD);
-  return hipGetLastError();
+  if ((e = hipGetLastError())) return e;
+  if (mode == 0) return hipSuccess;
+  return box_weights(bt, st, ca, nf, mode, sn, dij, nf_per_sys, mtot, w);
 }
 
 hipError_t box_weights(const BoxTables& bt, hipStream_t st, const c64* ca, int nf, int mode, const double* sn,
@@ -258,8 +245,12 @@ hipError_t union_corr(const BoxTables& bt, hipStream_t st, int nf, int which, co
   if (bt.natom == 0 || nf == 0 || bt.U.U == 0) return hipSuccess;
   prof_begin(st, KC_BOX_ADD);
   const int per = kBoxThreads / 64;
-  hipLaunchKernelGGL(k_union_corr, dim3((bt.U.U + per - 1) / per), dim3(kBoxThreads), 0, st, bt.U,
-                     which ? bt.U.p1 : bt.U.p, w, nf, corr);
+  if (bt.msp == 8)
+    hipLaunchKernelGGL(k_union_corr<8>, dim3((bt.U.U + per - 1) / per), dim3(kBoxThreads), 0, st, bt.U,
+                       which ? bt.U.p1 : bt.U.p, w, nf, corr);
+  else
+    hipLaunchKernelGGL(k_union_corr<16>, dim3((bt.U.U + per - 1) / per), dim3(kBoxThreads), 0, st, bt.U,
+                       which ? bt.U.p1 : bt.U.p, w, nf, corr);
   prof_end(st, KC_BOX_ADD);
   return hipGetLastError();
 }

@@ -1201,9 +1201,9 @@ static int flow_rk4(opaw_ctx* c, double dt) {
   prof_begin(c->st, KC_DENS);
   HCHK(field_norms<R>(c->st, p, c->N, nf, dv, c->part, c->nrm1));
   prof_end(c->st, KC_DENS);
-  prof_begin(c->st, KC_RK4);
+  prof_begin(c->st, KC_DENS);
   HCHK(scale_fields<R>(c->st, p, c->N, nf, c->nrm0, c->nrm1));
-  prof_end(c->st, KC_RK4);
+  prof_end(c->st, KC_DENS);
   c->prep_valid = false;
   return OPAW_OK;
 }

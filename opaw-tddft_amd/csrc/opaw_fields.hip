@@ -164,20 +164,24 @@ __global__ __launch_bounds__(kT) void k_density(const cx<R>* __restrict__ u, lon
 //   nhat(r) = sum_a sum_L g_a(r, L) wl_a(L),  wl_a(L) = sum_ij rho_ij qijlm(L,i,j)
 // wl per (system, atom, L) first, then one thread per grid point gathers the
 // sphere entries of that point in atom order (no atomics).
-__global__ void k_nhat_w(const AtomDesc* __restrict__ atoms, int natom, const double* __restrict__ rhoij, int mtot,
-                         const double* __restrict__ qtab, int nl2, int nsys, double* __restrict__ wl) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nsys * natom * nl2) return;
-  const int L = q % nl2;
-  const int a = (q / nl2) % natom;
-  const int s = q / (nl2 * natom);
+__global__ __launch_bounds__(64) void k_nhat_w(const AtomDesc* __restrict__ atoms, int natom,
+                                                const double* __restrict__ rhoij, int mtot,
+                                                const double* __restrict__ qtab, int nl2, int nsys,
+                                                double* __restrict__ wl) {
+  const int a = blockIdx.x, s = blockIdx.y;
   const AtomDesc A = atoms[a];
-  const double* rho = rhoij + (long long)s * mtot + A.moff;
+  const int m2 = A.ms * A.ms;
+  __shared__ double rho[kMaxMs * kMaxMs];
+  const double* rg = rhoij + (long long)s * mtot + A.moff;
+  for (int q = threadIdx.x; q < m2; q += 64) rho[q] = rg[q];
+  __syncthreads();
   const double* qq = qtab + A.toff_q;
-  double w = 0.0;
-  for (int j = 0; j < A.ms; ++j)
-    for (int i = 0; i < A.ms; ++i) w += rho[i + A.ms * j] * qq[L + nl2 * (i + A.ms * j)];
-  wl[q] = w;
+  for (int L = threadIdx.x; L < nl2; L += 64) {
+    double w = 0.0;
+#pragma unroll 8
+    for (int q = 0; q < m2; ++q) w += rho[q] * qq[L + nl2 * q];
+    wl[((long long)s * natom + a) * nl2 + L] = w;
+  }
 }
 
 __global__ __launch_bounds__(kT) void k_nhat_grid(SphereTabs S, const double* __restrict__ sph_g,
@@ -347,8 +351,10 @@ __global__ void k_rhoij(const AtomDesc* __restrict__ atoms, const c64* __restric
 // D_ij (src/get_dij.f90:1-156): on-site model dij0 + eijkl . rho_sel plus the
 // grid term sum_L q_ij^L dv sum_sph (vks - vxc) g_L (pawdijhat,
 // src/libpaw/4_m_pawdij.F90:2227-2400), then transfer_dij's packed
-// upper-triangle symmetric fill and +-param_dij_max clip.
-__global__ __launch_bounds__(64) void k_dij(const AtomDesc* __restrict__ atoms, const double* __restrict__ rhoij,
+// upper-triangle symmetric fill and +-param_dij_max clip.  One workgroup per
+// (atom, system); rho_sel, qijlm and the sphere moments are staged in LDS so
+// the contractions run from LDS.
+__global__ __launch_bounds__(kT) void k_dij(const AtomDesc* __restrict__ atoms, const double* __restrict__ rhoij,
                                              int mtot, const double* __restrict__ qtab,
                                              const double* __restrict__ d0tab, const double* __restrict__ etab,
                                              int nl2, const int* __restrict__ sph_idx,
@@ -357,12 +363,22 @@ __global__ __launch_bounds__(64) void k_dij(const AtomDesc* __restrict__ atoms, 
                                              double dv, double* __restrict__ dij) {
   const int a = blockIdx.x, s = blockIdx.y;
   const AtomDesc A = atoms[a];
+  const int ms = A.ms, m2 = ms * ms;
+  __shared__ double red[kT / 64][kMaxNl2];
   __shared__ double prod[kMaxNl2];
+  __shared__ double rho[kMaxMs * kMaxMs];
+  __shared__ double qq[kMaxNl2 * kMaxMs * kMaxMs];
   __shared__ double M[kMaxMs * kMaxMs];
+  const double* rg = rhoij + (long long)s * mtot + A.moff;
+  for (int q = threadIdx.x; q < m2; q += kT) {
+    const double rv = rg[q];
+    rho[q] = fabs(rv) > kRhoijSelect ? rv : 0.0;   // src/get_dij.f90:80-88 selection
+  }
+  for (int q = threadIdx.x; q < nl2 * m2; q += kT) qq[q] = qtab[A.toff_q + q];
   double acc[kMaxNl2];
 #pragma unroll
   for (int L = 0; L < kMaxNl2; ++L) acc[L] = 0.0;
-  for (int p = threadIdx.x; p < A.sph_n; p += 64) {
+  for (int p = threadIdx.x; p < A.sph_n; p += kT) {
     const long long r = sph_idx[A.sph_off + p];
     double pot = vks[(long long)s * N + r];
     if (subtract_vxc) pot -= vxc[(long long)s * N + r];
@@ -370,30 +386,32 @@ __global__ __launch_bounds__(64) void k_dij(const AtomDesc* __restrict__ atoms, 
     for (int L = 0; L < kMaxNl2; ++L)
       if (L < nl2) acc[L] += pot * sph_g[(long long)A.sph_off * nl2 + (long long)L * A.sph_n + p];
   }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int L = 0; L < kMaxNl2; ++L) {
     if (L < nl2) {
       const double t = wsum(acc[L]);
-      if (threadIdx.x == 0) prod[L] = t * dv;
+      if (lane == 0) red[w][L] = t;
     }
   }
   __syncthreads();
-  const double* rho = rhoij + (long long)s * mtot + A.moff;
-  const int ms = A.ms, m2 = ms * ms;
-  for (int q = threadIdx.x; q < m2; q += 64) {
+  if (threadIdx.x < nl2) {
+    double t = 0.0;
+    for (int k = 0; k < kT / 64; ++k) t += red[k][threadIdx.x];
+    prod[threadIdx.x] = t * dv;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < m2; q += kT) {
     const int i = q % ms, j = q / ms;
     double d = d0tab[A.toff_d0 + q];
     const double* e = etab + A.toff_e + (long long)q;   // E(i,j,k,l) at i + ms(j + ms(k + ms l))
-    for (int kl = 0; kl < m2; ++kl) {
-      const double rv = rho[kl];
-      if (fabs(rv) > kRhoijSelect) d += e[(long long)kl * m2] * rv;
-    }
-    const double* qq = qtab + A.toff_q;
+#pragma unroll 8
+    for (int kl = 0; kl < m2; ++kl) d += e[(long long)kl * m2] * rho[kl];
     for (int L = 0; L < nl2; ++L) d += qq[L + nl2 * (i + ms * j)] * prod[L];
     M[q] = d;
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < m2; q += 64) {
+  for (int q = threadIdx.x; q < m2; q += kT) {
     const int i = q % ms, j = q / ms;
     const int lo = i < j ? i : j, hi = i < j ? j : i;
     double d = M[lo + ms * hi];
@@ -604,8 +622,7 @@ hipError_t nhat(hipStream_t st, const AtomDesc* atoms, int natom, const double* 
                 const double* qtab, int nl2, const SphereTabs& S, const double* sph_g, long long N, int nsys,
                 double* wl, double* nh) {
   if (natom > 0) {
-    const int q = nsys * natom * nl2;
-    hipLaunchKernelGGL(k_nhat_w, dim3((q + 127) / 128), dim3(128), 0, st, atoms, natom, rhoij, mtot, qtab, nl2, nsys,
+    hipLaunchKernelGGL(k_nhat_w, dim3(natom, nsys), dim3(64), 0, st, atoms, natom, rhoij, mtot, qtab, nl2, nsys,
                        wl);
   }
   hipLaunchKernelGGL(k_nhat_grid, dim3(grid_for(N * nsys)), dim3(kT), 0, st, S, sph_g, wl, natom, nl2, N, nsys, nh);
@@ -654,7 +671,7 @@ hipError_t dij(hipStream_t st, const AtomDesc* atoms, int natom, const double* r
                const double* vks, const double* vxc, int subtract_vxc, long long N, int nsys, double dv,
                double* d) {
   if (natom == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_dij, dim3(natom, nsys), dim3(64), 0, st, atoms, rho, mtot, qtab, d0tab, etab, nl2, sph_idx,
+  hipLaunchKernelGGL(k_dij, dim3(natom, nsys), dim3(kT), 0, st, atoms, rho, mtot, qtab, d0tab, etab, nl2, sph_idx,
                      sph_g, vks, vxc, subtract_vxc, N, dv, d);
   return hipGetLastError();
 }
