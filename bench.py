@@ -41,7 +41,37 @@ def parse():
     ap.add_argument("--norb", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--precision", choices=["both", "f64", "mixed"], default="both")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="first run a plain 1-read/1-write kernel over MIB MiB (counter calibration)")
     return ap.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(_HERE, "profiles", "hbm_traffic_c10h8_64.json")
+
+
+def measured_traffic(prec, family):
+    """HBM bytes per launch of a kernel family from the committed FETCH_SIZE /
+    WRITE_SIZE counter runs (already scaled by the copy-kernel calibration),
+    or None when no such measurement is committed."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        e = t[prec][family]
+        return float(e["fetch_bytes_per_launch"]) + float(e["write_bytes_per_launch"])
+    except (OSError, KeyError, ValueError, TypeError):
+        return None
+
+
+def copy_calibration(mib):
+    """out = in * 1 over mib MiB of float32: reads and writes exactly mib MiB."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(f"[bench] copy calibration: 3 x {mib} MiB read + {mib} MiB write", file=sys.stderr, flush=True)
 
 
 def box_points(spec):
@@ -143,6 +173,8 @@ def main():
     import torch
     from opaw_amd import synth
     torch.cuda.init()
+    if a.copy_calib:
+        copy_calibration(a.copy_calib)
     nb = a.norb
     spec = synth.molecule_system(nring=2, n=(64, 64, 64), d=0.5, seed=1234)
     spec.rnel = 2.0 * nb
@@ -192,7 +224,7 @@ def main():
                    "atoms": {"C": spec.extra["natom_c"], "H": spec.extra["natom_h"]}, "orbitals": nb,
                    "systems": 2, "dt": spec.dt, "kick": spec.sm, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": r["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": r["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None,
+                     "frac": r["achieved_gbs"] / HBM_PEAK_GBS, "traffic": measured_traffic(key, r["dominant"]),
                      "kernel": r["dominant"], "bytes_per_launch": r["dom_bytes"],
                      "avg_launch_us": r["dom_avg_s"] * 1e6, "launches": r["dom_launches"]},
         "cpu_baseline": cpu,

@@ -5,7 +5,12 @@
 
 namespace opaw {
 
-constexpr int kProjChunk = 64;    // box points per projection workgroup
+constexpr int kProjChunk = 64;
+// union-correction blocks: at most this many points and (coefficient, value)
+// pairs per block, and the LDS budget of the staged weights
+constexpr int kUnionBlockRows = 64;
+constexpr int kUnionBlockPairs = 1024;
+constexpr int kUnionLdsWeightBytes = 48 * 1024;    // box points per projection workgroup
 
 struct BoxTables {
   const AtomDesc* atoms;     // device
@@ -20,6 +25,12 @@ struct BoxTables {
   const double* p1;          // per-atom orthonormal projector tables
   const int* bpt;            // grid point of every box point (atoms concatenated, boff)
   const int* bk;             // union index of every box point
+  const int* pptr;           // U+1: (coefficient, value) pair range of union point k
+  const int* pc;             // pair coefficient index
+  const double* pv;          // pair value, projector table p
+  const double* pv1;         // pair value, projector table p1
+  const int2* ublk;          // union point blocks (first point, count) for the LDS kernel
+  int nublk;                 // 0: no blocks (global kernel)
   UnionTabs U;               // union-of-boxes gather tables
 };
 

@@ -40,25 +40,53 @@ __global__ __launch_bounds__(kBoxThreads) void k_proj(const AtomDesc* __restrict
 #pragma unroll
   for (int i = 0; i < MSP; ++i) accr[i] = acci[i] = 0.0;
   const int kend = min(A.npt, cd.y + kProjChunk);
+  constexpr int PPT = kProjChunk / kPointLanes;   // points per lane
   if (f < nf) {
     const cx<R>* uf = u + (long long)f * N;
-#pragma unroll 4
-    for (int k = cd.y + kl; k < kend; k += kPointLanes) {
-      const int pt = bpt[A.boff + k];
-      const cx<R> v = uf[pt];
-      double vr = (double)v.x, vi = (double)v.y;
-      if constexpr (PRE) {
-        const c64 c = cpre[(long long)bk[A.boff + k] * nf + f];
-        vr += c.x;
-        vi += c.y;
-      }
-      const double* t = tabs + A.tab + k;
+    // issue every load of the lane first: grid indices, field values,
+    // corrections, table values; then accumulate
+    int pt[PPT];
 #pragma unroll
-      for (int i = 0; i < MSP; ++i) {
-        if (i < A.ms) {
-          const double tv = t[(long long)i * A.npt];
-          accr[i] += vr * tv;
-          acci[i] += vi * tv;
+    for (int q = 0; q < PPT; ++q) {
+      const int k = cd.y + kl + q * kPointLanes;
+      pt[q] = (k < kend) ? bpt[A.boff + k] : -1;
+    }
+    double vr[PPT], vi[PPT];
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+      cx<R> v = mk<R>(0, 0);
+      if (pt[q] >= 0) v = uf[pt[q]];
+      vr[q] = (double)v.x;
+      vi[q] = (double)v.y;
+    }
+    if constexpr (PRE) {
+      int kk[PPT];
+#pragma unroll
+      for (int q = 0; q < PPT; ++q) {
+        const int k = cd.y + kl + q * kPointLanes;
+        kk[q] = (k < kend) ? bk[A.boff + k] : -1;
+      }
+#pragma unroll
+      for (int q = 0; q < PPT; ++q) {
+        if (kk[q] >= 0) {
+          const c64 c = cpre[(long long)kk[q] * nf + f];
+          vr[q] += c.x;
+          vi[q] += c.y;
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+      const int k = cd.y + kl + q * kPointLanes;
+      if (k < kend) {
+        const double* t = tabs + A.tab + k;
+#pragma unroll
+        for (int i = 0; i < MSP; ++i) {
+          if (i < A.ms) {
+            const double tv = t[(long long)i * A.npt];
+            accr[i] += vr[q] * tv;
+            acci[i] += vi[q] * tv;
+          }
         }
       }
     }
@@ -83,30 +111,60 @@ __global__ __launch_bounds__(kBoxThreads) void k_proj(const AtomDesc* __restrict
   }
 }
 
-// ca(f, c) = dv * sum over the chunks of c's atom: one wave per (field,
-// coefficient), lanes over chunks, fixed shuffle tree (deterministic)
+// ca(f, coff_a + i) = dv * sum over the atom's chunks (one wave per
+// coefficient, lanes over chunks, fixed shuffle tree: deterministic), then the
+// weights of the atom (mode 1: sn * ca, mode 2: D(sys) ca, 0: none).  One
+// workgroup per (atom, field).
 template <int MSP>
-__global__ __launch_bounds__(256) void k_proj_finish(const int2* __restrict__ coef, const int2* __restrict__ achunk,
+__global__ __launch_bounds__(256) void k_proj_finish(const AtomDesc* __restrict__ atoms,
+                                                     const int2* __restrict__ achunk,
                                                      const double* __restrict__ part, int nf, int ctot, double dv,
-                                                     c64* __restrict__ ca) {
-  const int f = blockIdx.y;
-  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (c >= ctot) return;
-  const int2 ai = coef[c];                       // (atom, i)
-  const int2 cr = achunk[ai.x];                  // (first chunk, chunk count)
-  double sr = 0.0, si = 0.0;
-  for (int q = lane; q < cr.y; q += 64) {
-    const double* pp = part + (((long long)(cr.x + q) * nf + f) * MSP + ai.y) * 2;
-    sr += pp[0];
-    si += pp[1];
-  }
+                                                     c64* __restrict__ ca, int mode, const double* __restrict__ sn,
+                                                     const double* __restrict__ dij, int nf_per_sys, int mtot,
+                                                     c64* __restrict__ w) {
+  const int a = blockIdx.x, f = blockIdx.y;
+  const AtomDesc A = atoms[a];
+  const int2 cr = achunk[a];                     // (first chunk, chunk count)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __shared__ c64 row[kMaxMs];
+  for (int i = wave; i < A.ms; i += 4) {
+    double sr = 0.0, si = 0.0;
+    for (int q = lane; q < cr.y; q += 64) {
+      const double* pp = part + (((long long)(cr.x + q) * nf + f) * MSP + i) * 2;
+      sr += pp[0];
+      si += pp[1];
+    }
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    sr += __shfl_xor(sr, off, 64);
-    si += __shfl_xor(si, off, 64);
+    for (int off = 32; off > 0; off >>= 1) {
+      sr += __shfl_xor(sr, off, 64);
+      si += __shfl_xor(si, off, 64);
+    }
+    if (lane == 0) {
+      const c64 v = mk(sr * dv, si * dv);
+      ca[(long long)f * ctot + A.coff + i] = v;
+      row[i] = v;
+    }
   }
-  if (lane == 0) ca[(long long)f * ctot + c] = mk(sr * dv, si * dv);
+  if (mode == 0) return;
+  __syncthreads();
+  const int i = threadIdx.x;
+  if (i < A.ms) {
+    c64 out;
+    if (mode == 1) {
+      const double s = sn[A.coff + i];
+      out = mk(s * row[i].x, s * row[i].y);
+    } else {
+      const double* D = dij + (long long)(f / nf_per_sys) * mtot + A.moff;
+      double xr = 0.0, xi = 0.0;
+      for (int j = 0; j < A.ms; ++j) {
+        const double d = D[i + A.ms * j];
+        xr += d * row[j].x;
+        xi += d * row[j].y;
+      }
+      out = mk(xr, xi);
+    }
+    w[(long long)(A.coff + i) * nf + f] = out;
+  }
 }
 
 // w = D(sys) ca or sn * ca from existing coefficients, one workgroup per field
@@ -137,38 +195,41 @@ __global__ __launch_bounds__(128) void k_weights(const AtomDesc* __restrict__ at
   }
 }
 
-// corr(k, f) for every union point k: one wave per point, lanes = fields
-// (weights are coefficient-major, w[c*nf + f], so lane loads coalesce and the
-// table value is shared by the wave); the ms loads of an entry are issued
-// together (unrolled to MSP with a guard)
-template <int MSP>
-__global__ __launch_bounds__(kBoxThreads) void k_union_corr(UnionTabs U, const double* __restrict__ tab,
+// corr(k, f) for every union point k: one wave per point, lanes = fields.
+// The point's atom entries are flattened into (coefficient, table value)
+// pairs in atom order, read in batches of 8 so their loads and the weight
+// loads (coefficient-major, w[c*nf + f]: coalesced over lanes) are in flight
+// together.
+__global__ __launch_bounds__(kBoxThreads) void k_union_corr(const int* __restrict__ pptr,
+                                                            const int* __restrict__ pc,
+                                                            const double* __restrict__ pv, int U,
                                                             const c64* __restrict__ w, int nf,
                                                             c64* __restrict__ corr) {
   const int k = blockIdx.x * (kBoxThreads / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (k >= U.U) return;
-  const int e0 = U.uptr[k], e1 = U.uptr[k + 1];
+  if (k >= U) return;
+  const int p0 = pptr[k], p1 = pptr[k + 1];
   for (int f0 = 0; f0 < nf; f0 += 64) {
     const int f = f0 + lane;
     const bool on = f < nf;
+    const c64* wf = w + (on ? f : 0);
     double xr = 0.0, xi = 0.0;
-    for (int e = e0; e < e1; ++e) {
-      const UEntry E = U.ue[e];
-      const double* t = tab + E.toff;
-      const c64* ww = w + (long long)E.coff * nf + (on ? f : 0);
-      double tv[MSP];
-      c64 wv[MSP];
+    for (int p = p0; p < p1; p += 8) {
+      int c[8];
+      double t[8];
 #pragma unroll
-      for (int i = 0; i < MSP; ++i) {
-        const bool ok = i < E.ms;
-        tv[i] = ok ? t[i] : 0.0;
-        wv[i] = ok ? ww[(long long)i * nf] : mk(0.0, 0.0);
+      for (int q = 0; q < 8; ++q) {
+        const bool ok = p + q < p1;
+        c[q] = ok ? pc[p + q] : 0;
+        t[q] = ok ? pv[p + q] : 0.0;
       }
+      c64 wv[8];
 #pragma unroll
-      for (int i = 0; i < MSP; ++i) {
-        xr += tv[i] * wv[i].x;
-        xi += tv[i] * wv[i].y;
+      for (int q = 0; q < 8; ++q) wv[q] = wf[(long long)c[q] * nf];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        xr += t[q] * wv[q].x;
+        xi += t[q] * wv[q].y;
       }
     }
     if (on) corr[(long long)k * nf + f] = mk(xr, xi);
@@ -220,15 +281,15 @@ hipError_t box_coeffs(const GridGeom& g, const BoxTables& bt, hipStream_t st, co
   hipError_t e = hipGetLastError();
   if (e) return e;
   prof_begin(st, KC_BOX_ADD);
-  const dim3 fg((bt.ctot + 3) / 4, nf);
+  const dim3 fg(bt.natom, nf);
   if (bt.msp == 8)
-    hipLaunchKernelGGL(k_proj_finish<8>, fg, dim3(256), 0, st, bt.coef, bt.achunk, part, nf, bt.ctot, dv, ca);
+    hipLaunchKernelGGL(k_proj_finish<8>, fg, dim3(256), 0, st, bt.atoms, bt.achunk, part, nf, bt.ctot, dv, ca, mode,
+                       sn, dij, nf_per_sys, mtot, w);
   else
-    hipLaunchKernelGGL(k_proj_finish<16>, fg, dim3(256), 0, st, bt.coef, bt.achunk, part, nf, bt.ctot, dv, ca);
+    hipLaunchKernelGGL(k_proj_finish<16>, fg, dim3(256), 0, st, bt.atoms, bt.achunk, part, nf, bt.ctot, dv, ca, mode,
+                       sn, dij, nf_per_sys, mtot, w);
   prof_end(st, KC_BOX_ADD);
-  if ((e = hipGetLastError())) return e;
-  if (mode == 0) return hipSuccess;
-  return box_weights(bt, st, ca, nf, mode, sn, dij, nf_per_sys, mtot, w);
+  return hipGetLastError();
 }
 
 hipError_t box_weights(const BoxTables& bt, hipStream_t st, const c64* ca, int nf, int mode, const double* sn,
@@ -241,16 +302,86 @@ hipError_t box_weights(const BoxTables& bt, hipStream_t st, const c64* ca, int n
   return hipGetLastError();
 }
 
+// The same corrections with the weights staged in LDS.  Every union point
+// reads ~20 weight rows, so the global kernel above re-reads the (small)
+// weight matrix from L2 about 20 times per launch.  Here a workgroup owns a
+// block of consecutive union points (host-built: at most kUnionBlockRows
+// points and kUnionBlockPairs pairs) and FG fields: it copies the weights of
+// those fields for every coefficient (ctot x FG) and the block's pairs into
+// LDS with coalesced loads, then every sum reads LDS only.  Lanes are
+// (point, field); the pair order per point is the one of k_union_corr.
+template <int FG>
+__global__ __launch_bounds__(kBoxThreads) void k_union_corr_lds(const int2* __restrict__ ublk,
+                                                                const int* __restrict__ pptr,
+                                                                const int* __restrict__ pc,
+                                                                const double* __restrict__ pv,
+                                                                const c64* __restrict__ w, int nf, int ctot,
+                                                                c64* __restrict__ corr) {
+  extern __shared__ __align__(16) double smem[];
+  c64* wl = reinterpret_cast<c64*>(smem);                 // [ctot][FG]
+  double* tl = smem + 2 * (long long)ctot * FG;           // pair values
+  int* cl = reinterpret_cast<int*>(tl + kUnionBlockPairs);  // pair coefficients
+  int* rl = cl + kUnionBlockPairs;                        // row starts (block-relative)
+  const int2 blk = ublk[blockIdx.x];                      // (first point, point count)
+  const int f0 = blockIdx.y * FG;
+  const int q0 = pptr[blk.x];
+  for (int q = threadIdx.x; q < ctot * FG; q += kBoxThreads) {
+    const int c = q / FG, j = q % FG;
+    wl[q] = (f0 + j < nf) ? w[(long long)c * nf + f0 + j] : mk(0.0, 0.0);
+  }
+  for (int r = threadIdx.x; r <= blk.y; r += kBoxThreads) rl[r] = pptr[blk.x + r] - q0;
+  const int q1 = pptr[blk.x + blk.y];
+  for (int q = q0 + threadIdx.x; q < q1; q += kBoxThreads) {
+    tl[q - q0] = pv[q];
+    cl[q - q0] = pc[q];
+  }
+  __syncthreads();
+  constexpr int PL = kBoxThreads / FG;
+  const int j = threadIdx.x % FG;
+  const int f = f0 + j;
+  for (int r = threadIdx.x / FG; r < blk.y; r += PL) {
+    const int a = rl[r], e = rl[r + 1];
+    double xr = 0.0, xi = 0.0;
+    for (int p = a; p < e; ++p) {
+      const c64 wv = wl[cl[p] * FG + j];
+      const double t = tl[p];
+      xr += t * wv.x;
+      xi += t * wv.y;
+    }
+    if (f < nf) corr[(long long)(blk.x + r) * nf + f] = mk(xr, xi);
+  }
+}
+
+// fields per workgroup of the LDS kernel for ctot coefficients (0: use the
+// global kernel)
+static int union_lds_fields(int ctot) {
+  for (int fg : {16, 8, 4})
+    if ((long long)ctot * fg * 16 <= kUnionLdsWeightBytes) return fg;
+  return 0;
+}
+
 hipError_t union_corr(const BoxTables& bt, hipStream_t st, int nf, int which, const c64* w, c64* corr) {
   if (bt.natom == 0 || nf == 0 || bt.U.U == 0) return hipSuccess;
   prof_begin(st, KC_BOX_ADD);
-  const int per = kBoxThreads / 64;
-  if (bt.msp == 8)
-    hipLaunchKernelGGL(k_union_corr<8>, dim3((bt.U.U + per - 1) / per), dim3(kBoxThreads), 0, st, bt.U,
-                       which ? bt.U.p1 : bt.U.p, w, nf, corr);
-  else
-    hipLaunchKernelGGL(k_union_corr<16>, dim3((bt.U.U + per - 1) / per), dim3(kBoxThreads), 0, st, bt.U,
-                       which ? bt.U.p1 : bt.U.p, w, nf, corr);
+  const double* pv = which ? bt.pv1 : bt.pv;
+  const int fg = bt.nublk > 0 ? union_lds_fields(bt.ctot) : 0;
+  if (fg > 0) {
+    const size_t lds = (size_t)bt.ctot * fg * 16 + (size_t)kUnionBlockPairs * 12 + (kUnionBlockRows + 1) * 4;
+    const dim3 grid(bt.nublk, (nf + fg - 1) / fg);
+    if (fg == 16)
+      hipLaunchKernelGGL(k_union_corr_lds<16>, grid, dim3(kBoxThreads), lds, st, bt.ublk, bt.pptr, bt.pc, pv, w, nf,
+                         bt.ctot, corr);
+    else if (fg == 8)
+      hipLaunchKernelGGL(k_union_corr_lds<8>, grid, dim3(kBoxThreads), lds, st, bt.ublk, bt.pptr, bt.pc, pv, w, nf,
+                         bt.ctot, corr);
+    else
+      hipLaunchKernelGGL(k_union_corr_lds<4>, grid, dim3(kBoxThreads), lds, st, bt.ublk, bt.pptr, bt.pc, pv, w, nf,
+                         bt.ctot, corr);
+  } else {
+    const int per = kBoxThreads / 64;
+    hipLaunchKernelGGL(k_union_corr, dim3((bt.U.U + per - 1) / per), dim3(kBoxThreads), 0, st, bt.pptr, bt.pc, pv,
+                       bt.U.U, w, nf, corr);
+  }
   prof_end(st, KC_BOX_ADD);
   return hipGetLastError();
 }

@@ -149,6 +149,10 @@ struct opaw_ctx {
   int msp = 8, nchunks = 0, U = 0, S = 0;
   int2 *chunks = nullptr, *achunk = nullptr, *coef = nullptr;
   int *umap = nullptr, *uptr = nullptr, *upt = nullptr, *bpt = nullptr, *bk = nullptr;
+  int *pptr = nullptr, *pc = nullptr;
+  int2* ublk = nullptr;
+  int nublk = 0;
+  double *pv = nullptr, *pv1 = nullptr;
   UEntry* uent = nullptr;
   double *utp = nullptr, *utp1 = nullptr;
   int *smap = nullptr, *sptr = nullptr, *sphn = nullptr;
@@ -621,7 +625,8 @@ int opaw_finalize_tables(opaw_ctx* c) {
                   (void*)c->sph_g, (void*)c->qtab, (void*)c->d0tab, (void*)c->etab, (void*)c->dij, (void*)c->rhoij,
                   (void*)c->chunks, (void*)c->achunk, (void*)c->coef, (void*)c->umap, (void*)c->uptr, (void*)c->upt,
                   (void*)c->uent, (void*)c->utp, (void*)c->utp1, (void*)c->smap, (void*)c->sptr, (void*)c->sphn,
-                  (void*)c->sent, (void*)c->wl, (void*)c->bpt, (void*)c->bk})
+                  (void*)c->sent, (void*)c->wl, (void*)c->bpt, (void*)c->bk,
+                  (void*)c->pptr, (void*)c->pc, (void*)c->pv, (void*)c->pv1, (void*)c->ublk})
     dfree(c, p);
   c->prep_valid = false;
   c->coef_cap = 0;   // projection buffers depend on msp / chunk count
@@ -765,6 +770,44 @@ int opaw_finalize_tables(opaw_ctx* c) {
       uptr.push_back((int)ue.size());
     }
     c->U = (int)upt2.size();
+    // the same entries flattened to (coefficient, value) pairs per union point
+    {
+      std::vector<int> pptr(1, 0), pc;
+      std::vector<double> pv, pv1;
+      for (size_t k = 0; k < upt2.size(); ++k) {
+        for (int e = uptr[k]; e < uptr[k + 1]; ++e)
+          for (int i = 0; i < ue[e].ms; ++i) {
+            pc.push_back(ue[e].coff + i);
+            pv.push_back(tp[ue[e].toff + i]);
+            pv1.push_back(tp1[ue[e].toff + i]);
+          }
+        pptr.push_back((int)pc.size());
+      }
+      // blocks of consecutive points for the LDS kernel; none when one
+      // point alone has more pairs than a block holds
+      std::vector<int2> ublk;
+      bool fits = true;
+      for (size_t k = 0; k + 1 < pptr.size(); ++k)
+        if (pptr[k + 1] - pptr[k] > kUnionBlockPairs) fits = false;
+      if (fits) {
+        int first = 0;
+        const int U = (int)pptr.size() - 1;
+        while (first < U) {
+          int cnt = 0;
+          while (first + cnt < U && cnt < kUnionBlockRows &&
+                 pptr[first + cnt + 1] - pptr[first] <= kUnionBlockPairs)
+            ++cnt;
+          ublk.push_back(make_int2(first, cnt));
+          first += cnt;
+        }
+      }
+      c->nublk = (int)ublk.size();
+      if (c->nublk > 0) RCHK(upload(c, &c->ublk, ublk));
+      RCHK(upload(c, &c->pptr, pptr));
+      RCHK(upload(c, &c->pc, pc));
+      RCHK(upload(c, &c->pv, pv));
+      RCHK(upload(c, &c->pv1, pv1));
+    }
     // per box point: grid point and union index (atoms concatenated)
     std::vector<int> bpt, bk;
     for (int a = 0; a < natom; ++a) {
@@ -988,6 +1031,12 @@ static BoxTables box_tables(opaw_ctx* c) {
   b.p1 = c->P1;
   b.bpt = c->bpt;
   b.bk = c->bk;
+  b.pptr = c->pptr;
+  b.pc = c->pc;
+  b.pv = c->pv;
+  b.pv1 = c->pv1;
+  b.ublk = c->ublk;
+  b.nublk = c->nublk;
   b.U = UnionTabs{c->umap, c->uptr, c->uent, c->upt, c->utp, c->utp1, c->U};
   return b;
 }
@@ -1172,7 +1221,7 @@ static int flow_rebuild(opaw_ctx* c, int s0, int ns, const cx<R>* phi) {
 // rebuild), updates acc and y with the outer S^-1/2 gathered into the update,
 // and rebuilds H from the new y.
 template <typename R>
-static int flow_rk4(opaw_ctx* c, double dt) {
+static int flow_rk4(opaw_ctx* c, double dt, double* dip_out) {
   const int nf = c->nsys * c->nb;
   const long long n = (long long)nf * c->N;
   const double dv = dvol(c);
@@ -1184,7 +1233,6 @@ static int flow_rk4(opaw_ctx* c, double dt) {
   cx<R>* acc = (cx<R>*)c->wacc.p;
   cx<R>* h = (cx<R>*)c->wh.p;
   if (!c->prep_valid) RCHK(flow_prep<R>(c, p, nf));
-  const BoxTables bt = box_tables(c);
   prof_begin(c->st, KC_DENS);
   HCHK(field_norms<R>(c->st, p, c->N, nf, dv, c->part, c->nrm0));
   prof_end(c->st, KC_DENS);
@@ -1192,17 +1240,20 @@ static int flow_rk4(opaw_ctx* c, double dt) {
   for (int stage = 1; stage <= 4; ++stage) {
     const cx<R>* yin = (stage == 1) ? p : y;
     RCHK(flow_hs<R>(c, 0, yin, h, nf, c->nb));
+    // stage 4 also forms the post-step norms (norm1) of every orbital
     prof_begin(c->st, KC_RK4);
     HCHK(rk4_update<R>(c->st, stage, c->N, nf, h, p, acc, y, cst[stage - 1], dt, c->umap, (const c64*)c->corrC.p,
-                       nf));
+                       nf, c->part, dv, c->nrm1));
     prof_end(c->st, KC_RK4);
     if (stage < 4) RCHK(flow_rebuild<R>(c, 0, c->nsys, y));
   }
+  // norm restore; inside the time loop fused with the dipole of the step
   prof_begin(c->st, KC_DENS);
-  HCHK(field_norms<R>(c->st, p, c->N, nf, dv, c->part, c->nrm1));
-  prof_end(c->st, KC_DENS);
-  prof_begin(c->st, KC_DENS);
-  HCHK(scale_fields<R>(c->st, p, c->N, nf, c->nrm0, c->nrm1));
+  if (dip_out)
+    HCHK(scale_dipole<R>(c->st, p, c->nx, c->ny, c->nz, c->nsys, c->nb, c->dx, c->dy, c->dz, c->nrm0, c->nrm1,
+                         c->part, dip_out));
+  else
+    HCHK(scale_fields<R>(c->st, p, c->N, nf, c->nrm0, c->nrm1));
   prof_end(c->st, KC_DENS);
   c->prep_valid = false;
   return OPAW_OK;
@@ -1497,8 +1548,8 @@ int opaw_excite_psi_bar_pert(opaw_ctx* c, int from_sys, int to_sys, double sm, i
 int opaw_rk4_prop_shs(opaw_ctx* c, double dt) {
   CTX_CHECK(c);
   RCHK(need_tables(c));
-  if (c->prec == OPAW_PREC_F64) return flow_rk4<double>(c, dt);
-  return flow_rk4<float>(c, dt);
+  if (c->prec == OPAW_PREC_F64) return flow_rk4<double>(c, dt, nullptr);
+  return flow_rk4<float>(c, dt, nullptr);
 }
 
 static int dipoles_to(opaw_ctx* c, double* dst) {
@@ -1543,8 +1594,13 @@ int opaw_tddft_run(opaw_ctx* c, int nt, double dt, double sm, double* d_dip, dou
   }
   for (int it = 0; it < nt; ++it) {
     RCHK(opaw_make_hamiltonian_tddft(c, -1));
-    RCHK(opaw_rk4_prop_shs(c, dt));
-    RCHK(dipoles_to(c, c->dipbuf + (size_t)3 * c->nsys * it));
+    // rk4_prop_shs with the dipole of the step (plot_dip) fused into the
+    // norm restore
+    double* slot = c->dipbuf + (size_t)3 * c->nsys * it;
+    if (c->prec == OPAW_PREC_F64)
+      RCHK(flow_rk4<double>(c, dt, slot));
+    else
+      RCHK(flow_rk4<float>(c, dt, slot));
   }
   if (d_dip || dip) {
     std::vector<double> h((size_t)3 * c->nsys * nt);

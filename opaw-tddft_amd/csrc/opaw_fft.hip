@@ -232,18 +232,36 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fft_plane(const cx<R>* in, cx
   const int f = blockIdx.x / nz;
   const int z = blockIdx.x % nz;
   const long long plane0 = (long long)z * NX * NY;           // offset of the plane in a field
+  constexpr int PER = (NX * NY + kPlaneThreads - 1) / kPlaneThreads;
   const cx<R>* src = in + (long long)f * N + plane0;
-  for (int idx = threadIdx.x; idx < NX * NY; idx += kPlaneThreads) {
-    const int x = idx % NX, y = idx / NX;
-    cx<R> v = src[idx];
+  {
+    // all plane loads of a thread first, then the correction lookups
+    cx<R> v[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int idx = threadIdx.x + q * kPlaneThreads;
+      v[q] = (idx < NX * NY) ? src[idx] : mk<R>(0, 0);
+    }
     if (EPI == EPI_NONE && cpre) {
-      const int k = umap[plane0 + idx];
-      if (k >= 0) {
-        const c64 c = cpre[(long long)k * U + f];
-        v = mk<R>((R)((double)v.x + c.x), (R)((double)v.y + c.y));
+      int k[PER];
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int idx = threadIdx.x + q * kPlaneThreads;
+        k[q] = (idx < NX * NY) ? umap[plane0 + idx] : -1;
+      }
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        if (k[q] >= 0) {
+          const c64 c = cpre[(long long)k[q] * U + f];
+          v[q] = mk<R>((R)((double)v[q].x + c.x), (R)((double)v[q].y + c.y));
+        }
       }
     }
-    tile[y * LS + x] = v;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int idx = threadIdx.x + q * kPlaneThreads;
+      if (idx < NX * NY) tile[(idx / NX) * LS + idx % NX] = v[q];
+    }
   }
   __syncthreads();
   if constexpr (S < 0) {
@@ -254,33 +272,48 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fft_plane(const cx<R>* in, cx
     fft::lines<R, NX, NY, kPlaneThreads, false, LS, 1, S>(tile, twx);
   }
   cx<R>* dst = out + (long long)f * N + plane0;
-  for (int idx = threadIdx.x; idx < NX * NY; idx += kPlaneThreads) {
-    const int x = idx % NX, y = idx / NX;
-    cx<R> v = tile[y * LS + x];
-    if constexpr (EPI == EPI_HPSI) {
-      const long long pt = plane0 + idx;
-      const double vv = (double)vks[(long long)(f / nf_per_sys) * N + pt];
-      const cx<R> uu = uin[(long long)f * N + pt];
-      double ur = (double)uu.x, ui = (double)uu.y;
-      double ar = (double)v.x, ai = (double)v.y;
-      if (cpre || cnl) {
-        const int k = umap[pt];
-        if (k >= 0) {
-          if (cpre) {
-            const c64 c = cpre[(long long)k * U + f];
-            ur += c.x;
-            ui += c.y;
-          }
-          if (cnl) {
-            const c64 c = cnl[(long long)k * U + f];
-            ar += c.x;
-            ai += c.y;
-          }
+  if constexpr (EPI == EPI_HPSI) {
+    const R* vk = vks + (long long)(f / nf_per_sys) * N + plane0;
+    const cx<R>* us = uin + (long long)f * N + plane0;
+    cx<R> uu[PER];
+    R vv[PER];
+    int k[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int idx = threadIdx.x + q * kPlaneThreads;
+      const bool ok = idx < NX * NY;
+      uu[q] = ok ? us[idx] : mk<R>(0, 0);
+      vv[q] = ok ? vk[idx] : R(0);
+      k[q] = (ok && (cpre || cnl)) ? umap[plane0 + idx] : -1;
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int idx = threadIdx.x + q * kPlaneThreads;
+      if (idx >= NX * NY) continue;
+      const cx<R> t = tile[(idx / NX) * LS + idx % NX];
+      double ur = (double)uu[q].x, ui = (double)uu[q].y;
+      double ar = (double)t.x, ai = (double)t.y;
+      if (k[q] >= 0) {
+        if (cpre) {
+          const c64 c = cpre[(long long)k[q] * U + f];
+          ur += c.x;
+          ui += c.y;
+        }
+        if (cnl) {
+          const c64 c = cnl[(long long)k[q] * U + f];
+          ar += c.x;
+          ai += c.y;
         }
       }
-      v = mk<R>((R)(ar + vv * ur), (R)(ai + vv * ui));
+      const double w = (double)vv[q];
+      dst[idx] = mk<R>((R)(ar + w * ur), (R)(ai + w * ui));
     }
-    dst[idx] = v;
+  } else {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int idx = threadIdx.x + q * kPlaneThreads;
+      if (idx < NX * NY) dst[idx] = tile[(idx / NX) * LS + idx % NX];
+    }
   }
 }
 

@@ -20,13 +20,19 @@ struct SphereTabs {
   const int* sn;     // per-atom sphere point count (g stride)
 };
 
+// stage 4 also writes the per-field norms (dv sum|p|^2, then sqrt) to norms
 template <typename R>
 hipError_t rk4_update(hipStream_t st, int stage, long long N, int nf, const cx<R>* h, cx<R>* p, cx<R>* acc,
-                      cx<R>* y, double c, double dt, const int* umap, const c64* cc, int U);
+                      cx<R>* y, double c, double dt, const int* umap, const c64* cc, int U, double* part,
+                      double dv, double* norms);
 template <typename R>
 hipError_t field_norms(hipStream_t st, const cx<R>* u, long long N, int nf, double dv, double* part, double* out);
 template <typename R>
 hipError_t scale_fields(hipStream_t st, cx<R>* u, long long N, int nf, const double* num, const double* den);
+// scale_fields fused with the dipole of the scaled orbitals (out: 3 per system)
+template <typename R>
+hipError_t scale_dipole(hipStream_t st, cx<R>* u, int nx, int ny, int nz, int nsys, int nb, double dx, double dy,
+                        double dz, const double* num, const double* den, double* part, double* out);
 template <typename R>
 hipError_t density(hipStream_t st, const cx<R>* u, long long N, int nsys, int nb, double* dens, const int* umap,
                    const c64* cpre, int U);

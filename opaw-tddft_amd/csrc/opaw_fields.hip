@@ -45,46 +45,89 @@ __device__ __forceinline__ void chunk_range(long long n, int nchunk, int c, long
 }
 
 // ---------------------------------------------------------------------------
+// Streaming field kernels run on 2D grids: blockIdx.y is the field (or
+// system), blockIdx.x covers the grid points, kV points per thread.
+constexpr int kV = 4;
+
 // RK4 stage update (src/rk4.f90:35-56 and :22): k = -i hs, where
 // hs = h + cc (the outer S^-1/2 of shs as a compact union correction) when cc
 // is set
 //   stage 1: acc = k;        y = p + c k
 //   stage 2,3: acc += 2k;    y = p + c k
-//   stage 4: p = p + dt/6 (acc + k)
 template <typename R>
-__global__ __launch_bounds__(kT) void k_rk4_update(int stage, long long N, int nf, const cx<R>* __restrict__ h,
-                                                    cx<R>* __restrict__ p, cx<R>* __restrict__ acc,
-                                                    cx<R>* __restrict__ y, double c, double dt6,
-                                                    const int* __restrict__ umap, const c64* __restrict__ cc, int U) {
-  const long long n = N * nf;
-  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n; i += (long long)gridDim.x * kT) {
-    const cx<R> hh = h[i];
-    double hr = (double)hh.x, hi = (double)hh.y;
-    if (cc) {
-      const int f = (int)(i / N);
-      const long long pt = i - (long long)f * N;
-      const int k = umap[pt];
-      if (k >= 0) {
-        const c64 v = cc[(long long)k * U + f];
-        hr += v.x;
-        hi += v.y;
-      }
+__global__ __launch_bounds__(kT) void k_rk4_update(int stage, int N, const cx<R>* __restrict__ h,
+                                                    const cx<R>* __restrict__ p, cx<R>* __restrict__ acc,
+                                                    cx<R>* __restrict__ y, double c,
+                                                    const int* __restrict__ umap, const c64* __restrict__ cc,
+                                                    int U) {
+  const int f = blockIdx.y;
+  const long long off = (long long)f * N;
+  const int base = blockIdx.x * (kT * kV) + threadIdx.x;
+  cx<R> hh[kV], pp[kV], aa[kV];
+  int kk[kV];
+#pragma unroll
+  for (int j = 0; j < kV; ++j) {
+    const int r = base + j * kT;
+    const bool ok = r < N;
+    hh[j] = ok ? h[off + r] : mk<R>(0, 0);
+    pp[j] = ok ? p[off + r] : mk<R>(0, 0);
+    aa[j] = (ok && stage > 1) ? acc[off + r] : mk<R>(0, 0);
+    kk[j] = (ok && cc) ? umap[r] : -1;
+  }
+#pragma unroll
+  for (int j = 0; j < kV; ++j) {
+    const int r = base + j * kT;
+    if (r >= N) continue;
+    double hr = (double)hh[j].x, hi = (double)hh[j].y;
+    if (kk[j] >= 0) {
+      const c64 v = cc[(long long)kk[j] * U + f];
+      hr += v.x;
+      hi += v.y;
     }
     const double kr = hi, ki = -hr;
-    const cx<R> pp = p[i];
-    if (stage == 4) {
-      const cx<R> a = acc[i];
-      p[i] = mk<R>((R)((double)pp.x + dt6 * ((double)a.x + kr)), (R)((double)pp.y + dt6 * ((double)a.y + ki)));
-    } else {
-      if (stage == 1) {
-        acc[i] = mk<R>((R)kr, (R)ki);
-      } else {
-        const cx<R> a = acc[i];
-        acc[i] = mk<R>((R)((double)a.x + 2.0 * kr), (R)((double)a.y + 2.0 * ki));
-      }
-      y[i] = mk<R>((R)((double)pp.x + c * kr), (R)((double)pp.y + c * ki));
-    }
+    if (stage == 1)
+      acc[off + r] = mk<R>((R)kr, (R)ki);
+    else
+      acc[off + r] = mk<R>((R)((double)aa[j].x + 2.0 * kr), (R)((double)aa[j].y + 2.0 * ki));
+    y[off + r] = mk<R>((R)((double)pp[j].x + c * kr), (R)((double)pp[j].y + c * ki));
   }
+}
+
+// RK4 stage 4 (src/rk4.f90:22): p = p + dt/6 (acc + k) with k = -i (h + cc),
+// fused with the per-field partial sums of |p|^2 for the norm restore
+// (norm1, src/rk4.f90:92-102).  Grid (nchunk, nf); the chunking is the one of
+// k_norm_partial.
+template <typename R>
+__global__ __launch_bounds__(kT) void k_rk4_final(long long N, int nchunk, const cx<R>* __restrict__ h,
+                                                   cx<R>* __restrict__ p, const cx<R>* __restrict__ acc,
+                                                   double dt6, const int* __restrict__ umap,
+                                                   const c64* __restrict__ cc, int U,
+                                                   double* __restrict__ part) {
+  const int ch = blockIdx.x, f = blockIdx.y;
+  long long lo, hi;
+  chunk_range(N, nchunk, ch, lo, hi);
+  const long long off = (long long)f * N;
+  double v[1] = {0.0};
+  for (long long r = lo + threadIdx.x; r < hi; r += kT) {
+    const cx<R> hh = h[off + r];
+    const cx<R> pp = p[off + r];
+    const cx<R> a = acc[off + r];
+    double hr = (double)hh.x, hi2 = (double)hh.y;
+    if (cc) {
+      const int k = umap[r];
+      if (k >= 0) {
+        const c64 w = cc[(long long)k * U + f];
+        hr += w.x;
+        hi2 += w.y;
+      }
+    }
+    const double kr = hi2, ki = -hr;
+    const cx<R> q = mk<R>((R)((double)pp.x + dt6 * ((double)a.x + kr)), (R)((double)pp.y + dt6 * ((double)a.y + ki)));
+    p[off + r] = q;
+    v[0] += (double)q.x * (double)q.x + (double)q.y * (double)q.y;
+  }
+  block_sum<1>(v);
+  if (threadIdx.x == 0) part[(long long)f * nchunk + ch] = v[0];
 }
 
 // per-field partial sums of |psi|^2
@@ -120,44 +163,86 @@ __global__ __launch_bounds__(kT) void k_finish_rows(const double* __restrict__ p
 
 // psi_f *= num[f] / den[f]   (src/rk4.f90:92-102, norm1)
 template <typename R>
-__global__ __launch_bounds__(kT) void k_scale_fields(cx<R>* __restrict__ u, long long N, int nf,
-                                                      const double* __restrict__ num,
+__global__ __launch_bounds__(kT) void k_scale_fields(cx<R>* __restrict__ u, int N, const double* __restrict__ num,
                                                       const double* __restrict__ den) {
-  const long long n = N * nf;
-  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n; i += (long long)gridDim.x * kT) {
-    const int f = (int)(i / N);
-    const double s = num[f] / den[f];
-    const cx<R> v = u[i];
-    u[i] = mk<R>((R)((double)v.x * s), (R)((double)v.y * s));
+  const int f = blockIdx.y;
+  const double s = num[f] / den[f];
+  cx<R>* uf = u + (long long)f * N;
+  const int base = blockIdx.x * (kT * kV) + threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < kV; ++j) {
+    const int r = base + j * kT;
+    if (r < N) {
+      const cx<R> v = uf[r];
+      uf[r] = mk<R>((R)((double)v.x * s), (R)((double)v.y * s));
+    }
   }
+}
+
+// The norm restore fused with the dipole of the restored orbitals
+// (src/rk4.f90:92-102 then src/tddft_plot.f90:40-55): psi_f *= num[f]/den[f]
+// for the nb orbitals of system s, and the partial sums of r * 2 sum|psi|^2.
+// Grid (nchunk, nsys), the chunking of k_dipole_partial.
+template <typename R>
+__global__ __launch_bounds__(kT) void k_scale_dipole(cx<R>* __restrict__ u, int nx, int ny, int nz, int nb,
+                                                      int nchunk, double dx, double dy, double dz,
+                                                      const double* __restrict__ num,
+                                                      const double* __restrict__ den,
+                                                      double* __restrict__ part) {
+  const int ch = blockIdx.x, s = blockIdx.y;
+  const long long N = (long long)nx * ny * nz;
+  long long lo, hi;
+  chunk_range(N, nchunk, ch, lo, hi);
+  extern __shared__ double fac[];
+  for (int o = threadIdx.x; o < nb; o += kT) fac[o] = num[s * nb + o] / den[s * nb + o];
+  __syncthreads();
+  double v[3] = {0.0, 0.0, 0.0};
+  cx<R>* us = u + (long long)s * nb * N;
+  for (long long i = lo + threadIdx.x; i < hi; i += kT) {
+    double rho = 0.0;
+    for (int o = 0; o < nb; ++o) {
+      const cx<R> w = us[(long long)o * N + i];
+      const cx<R> q = mk<R>((R)((double)w.x * fac[o]), (R)((double)w.y * fac[o]));
+      us[(long long)o * N + i] = q;
+      rho += (double)q.x * (double)q.x + (double)q.y * (double)q.y;
+    }
+    rho *= 2.0;
+    const int ii = (int)i;
+    const int ix = ii % nx, t = ii / nx;
+    const int iy = t % ny, iz = t / ny;
+    v[0] += rho * ((ix - nx / 2) * dx);
+    v[1] += rho * ((iy - ny / 2) * dy);
+    v[2] += rho * ((iz - nz / 2) * dz);
+  }
+  block_sum<3>(v);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) part[((long long)s * 3 + k) * nchunk + ch] = v[k];
 }
 
 // dens[s][r] = 2 sum_{o<nb} |psi~_{s,o}(r)|^2 (src/tddft_dens.f90:65-76) with
 // psi~ = psi + cpre (compact S^-1/2 correction) when cpre is set (on the fly,
 // src/ncpaw_make_hamiltonian.f90:228-235)
 template <typename R>
-__global__ __launch_bounds__(kT) void k_density(const cx<R>* __restrict__ u, long long N, int nsys, int nb,
+__global__ __launch_bounds__(kT) void k_density(const cx<R>* __restrict__ u, int N, int nb,
                                                  double* __restrict__ dens, const int* __restrict__ umap,
                                                  const c64* __restrict__ cpre, int U) {
-  const long long n = N * nsys;
-  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n; i += (long long)gridDim.x * kT) {
-    const int s = (int)(i / N);
-    const long long r = i - (long long)s * N;
-    const cx<R>* us = u + (long long)s * nb * N + r;
-    const int k = cpre ? umap[r] : -1;
-    double acc = 0.0;
-    for (int o = 0; o < nb; ++o) {
-      const cx<R> v = us[(long long)o * N];
-      double vr = (double)v.x, vi = (double)v.y;
-      if (k >= 0) {
-        const c64 c = cpre[(long long)k * U + (s * nb + o)];
-        vr += c.x;
-        vi += c.y;
-      }
-      acc += vr * vr + vi * vi;
+  const int s = blockIdx.y;
+  const int r = blockIdx.x * kT + threadIdx.x;
+  if (r >= N) return;
+  const cx<R>* us = u + (long long)s * nb * N + r;
+  const int k = cpre ? umap[r] : -1;
+  double acc = 0.0;
+  for (int o = 0; o < nb; ++o) {
+    const cx<R> v = us[(long long)o * N];
+    double vr = (double)v.x, vi = (double)v.y;
+    if (k >= 0) {
+      const c64 c = cpre[(long long)k * U + (s * nb + o)];
+      vr += c.x;
+      vi += c.y;
     }
-    dens[i] = 2.0 * acc;
+    acc += vr * vr + vi * vi;
   }
+  dens[(long long)s * N + r] = 2.0 * acc;
 }
 
 // compensation charge (src/nhat.f90:13-28):
@@ -185,25 +270,23 @@ __global__ __launch_bounds__(64) void k_nhat_w(const AtomDesc* __restrict__ atom
 }
 
 __global__ __launch_bounds__(kT) void k_nhat_grid(SphereTabs S, const double* __restrict__ sph_g,
-                                                   const double* __restrict__ wl, int natom, int nl2, long long N,
-                                                   int nsys, double* __restrict__ nhat) {
-  const long long n = N * nsys;
-  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n; i += (long long)gridDim.x * kT) {
-    const int s = (int)(i / N);
-    const long long r = i - (long long)s * N;
-    const int k = S.smap[r];
-    double v = 0.0;
-    if (k >= 0) {
-      for (int e = S.sptr[k]; e < S.sptr[k + 1]; ++e) {
-        const int2 en = S.se[e];                  // (atom, g offset of L = 0)
-        const int a = en.x;
-        const int stride = S.sn[a];
-        const double* w = wl + ((long long)s * natom + a) * nl2;
-        for (int L = 0; L < nl2; ++L) v += sph_g[en.y + (long long)L * stride] * w[L];
-      }
+                                                   const double* __restrict__ wl, int natom, int nl2, int N,
+                                                   double* __restrict__ nhat) {
+  const int s = blockIdx.y;
+  const int r = blockIdx.x * kT + threadIdx.x;
+  if (r >= N) return;
+  const int k = S.smap[r];
+  double v = 0.0;
+  if (k >= 0) {
+    for (int e = S.sptr[k]; e < S.sptr[k + 1]; ++e) {
+      const int2 en = S.se[e];                  // (atom, g offset of L = 0)
+      const int a = en.x;
+      const int stride = S.sn[a];
+      const double* w = wl + ((long long)s * natom + a) * nl2;
+      for (int L = 0; L < nl2; ++L) v += sph_g[en.y + (long long)L * stride] * w[L];
     }
-    nhat[i] = v;
   }
+  nhat[(long long)s * N + r] = v;
 }
 
 // partial sums of two real fields per system (nhat and dens) for the rnel
@@ -241,30 +324,27 @@ __global__ __launch_bounds__(kT) void k_renorm_factor(const double* __restrict__
 // dens, nhat *= fac (when fac != null); rho pair for the Hartree solve:
 //   pair[q] = (dens[2q]+nhat[2q]) + i (dens[2q+1]+nhat[2q+1])
 __global__ __launch_bounds__(kT) void k_scale_pack(double* __restrict__ dens, double* __restrict__ nhat,
-                                                    const double* __restrict__ fac, long long N, int nsys,
+                                                    const double* __restrict__ fac, int N, int nsys,
                                                     c64* __restrict__ pair) {
-  const int npair = (nsys + 1) / 2;
-  const long long n = N * npair;
-  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n; i += (long long)gridDim.x * kT) {
-    const int q = (int)(i / N);
-    const long long r = i - (long long)q * N;
-    double v[2] = {0.0, 0.0};
-    for (int k = 0; k < 2; ++k) {
-      const int s = 2 * q + k;
-      if (s < nsys) {
-        const long long j = (long long)s * N + r;
-        double d = dens[j], h = nhat[j];
-        if (fac) {
-          d *= fac[s];
-          h *= fac[s];
-          dens[j] = d;
-          nhat[j] = h;
-        }
-        v[k] = d + h;
+  const int q = blockIdx.y;
+  const int r = blockIdx.x * kT + threadIdx.x;
+  if (r >= N) return;
+  double v[2] = {0.0, 0.0};
+  for (int k = 0; k < 2; ++k) {
+    const int s = 2 * q + k;
+    if (s < nsys) {
+      const long long j = (long long)s * N + r;
+      double d = dens[j], h = nhat[j];
+      if (fac) {
+        d *= fac[s];
+        h *= fac[s];
+        dens[j] = d;
+        nhat[j] = h;
       }
+      v[k] = d + h;
     }
-    pair[i] = mk(v[0], v[1]);
   }
+  pair[(long long)q * N + r] = mk(v[0], v[1]);
 }
 
 // ---------------------------------------------------------------------------
@@ -293,23 +373,22 @@ __device__ __forceinline__ double lda_vxc(double n) {
 // (src/tddft_pot.f90:130-149)
 __global__ __launch_bounds__(kT) void k_vks(const double* __restrict__ dens, const double* __restrict__ ncoret,
                                              const c64* __restrict__ vhpair, const double* __restrict__ vhmean,
-                                             const double* __restrict__ vloc, long long N, int nsys,
+                                             const double* __restrict__ vloc, int N,
                                              double* __restrict__ vxc, double* __restrict__ vh,
                                              double* __restrict__ vks, float* __restrict__ vksf) {
-  const long long n = N * nsys;
-  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n; i += (long long)gridDim.x * kT) {
-    const int s = (int)(i / N);
-    const long long r = i - (long long)s * N;
-    const double x = lda_vxc(dens[i] + ncoret[r]);
-    const c64 p = vhpair[(long long)(s / 2) * N + r];
-    double h = (s & 1) ? p.y : p.x;
-    if (vhmean) h -= vhmean[s];
-    const double v = h + x + vloc[r];
-    vxc[i] = x;
-    vh[i] = h;
-    vks[i] = v;
-    if (vksf) vksf[i] = (float)v;
-  }
+  const int s = blockIdx.y;
+  const int r = blockIdx.x * kT + threadIdx.x;
+  if (r >= N) return;
+  const long long i = (long long)s * N + r;
+  const double x = lda_vxc(dens[i] + ncoret[r]);
+  const c64 p = vhpair[(long long)(s / 2) * N + r];
+  double h = (s & 1) ? p.y : p.x;
+  if (vhmean) h -= vhmean[s];
+  const double v = h + x + vloc[r];
+  vxc[i] = x;
+  vh[i] = h;
+  vks[i] = v;
+  if (vksf) vksf[i] = (float)v;
 }
 
 // partial sums of the packed Hartree output (re -> system 2q, im -> 2q+1)
@@ -439,10 +518,9 @@ __global__ __launch_bounds__(kT) void k_dipole_partial(const cx<R>* __restrict__
       rho += (double)w.x * (double)w.x + (double)w.y * (double)w.y;
     }
     rho *= 2.0;
-    const int ix = (int)(i % nx);
-    const long long t = i / nx;
-    const int iy = (int)(t % ny);
-    const int iz = (int)(t / ny);
+    const int ii = (int)i;
+    const int ix = ii % nx, t = ii / nx;
+    const int iy = t % ny, iz = t / ny;
     v[0] += rho * ((ix - nx / 2) * dx);
     v[1] += rho * ((iy - ny / 2) * dy);
     v[2] += rho * ((iz - nz / 2) * dz);
@@ -585,11 +663,22 @@ static unsigned grid_for(long long n) {
   return (unsigned)b;
 }
 
+static dim3 grid_rows(long long N, int rows, int per_thread) {
+  return dim3((unsigned)((N + (long long)kT * per_thread - 1) / ((long long)kT * per_thread)), (unsigned)rows);
+}
+
 template <typename R>
 hipError_t rk4_update(hipStream_t st, int stage, long long N, int nf, const cx<R>* h, cx<R>* p, cx<R>* acc,
-                      cx<R>* y, double c, double dt, const int* umap, const c64* cc, int U) {
-  hipLaunchKernelGGL((k_rk4_update<R>), dim3(grid_for(N * nf)), dim3(kT), 0, st, stage, N, nf, h, p, acc, y, c,
-                     dt / 6.0, umap, cc, U);
+                      cx<R>* y, double c, double dt, const int* umap, const c64* cc, int U, double* part,
+                      double dv, double* norms) {
+  if (stage < 4) {
+    hipLaunchKernelGGL((k_rk4_update<R>), grid_rows(N, nf, kV), dim3(kT), 0, st, stage, (int)N, h, p, acc, y, c,
+                       umap, cc, U);
+  } else {
+    hipLaunchKernelGGL((k_rk4_final<R>), dim3(kChunks, nf), dim3(kT), 0, st, N, kChunks, h, p, acc, dt / 6.0, umap,
+                       cc, U, part);
+    hipLaunchKernelGGL(k_finish_rows, dim3(nf), dim3(kT), 0, st, part, nf, kChunks, dv, 1, norms);
+  }
   return hipGetLastError();
 }
 
@@ -602,14 +691,23 @@ hipError_t field_norms(hipStream_t st, const cx<R>* u, long long N, int nf, doub
 
 template <typename R>
 hipError_t scale_fields(hipStream_t st, cx<R>* u, long long N, int nf, const double* num, const double* den) {
-  hipLaunchKernelGGL((k_scale_fields<R>), dim3(grid_for(N * nf)), dim3(kT), 0, st, u, N, nf, num, den);
+  hipLaunchKernelGGL((k_scale_fields<R>), grid_rows(N, nf, kV), dim3(kT), 0, st, u, (int)N, num, den);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t scale_dipole(hipStream_t st, cx<R>* u, int nx, int ny, int nz, int nsys, int nb, double dx, double dy,
+                        double dz, const double* num, const double* den, double* part, double* out) {
+  hipLaunchKernelGGL((k_scale_dipole<R>), dim3(kChunks, nsys), dim3(kT), nb * sizeof(double), st, u, nx, ny, nz, nb,
+                     kChunks, dx, dy, dz, num, den, part);
+  hipLaunchKernelGGL(k_finish_rows, dim3(3 * nsys), dim3(kT), 0, st, part, 3 * nsys, kChunks, dx * dy * dz, 0, out);
   return hipGetLastError();
 }
 
 template <typename R>
 hipError_t density(hipStream_t st, const cx<R>* u, long long N, int nsys, int nb, double* dens, const int* umap,
                    const c64* cpre, int U) {
-  hipLaunchKernelGGL((k_density<R>), dim3(grid_for(N * nsys)), dim3(kT), 0, st, u, N, nsys, nb, dens, umap, cpre, U);
+  hipLaunchKernelGGL((k_density<R>), grid_rows(N, nsys, 1), dim3(kT), 0, st, u, (int)N, nb, dens, umap, cpre, U);
   return hipGetLastError();
 }
 
@@ -625,7 +723,7 @@ hipError_t nhat(hipStream_t st, const AtomDesc* atoms, int natom, const double* 
     hipLaunchKernelGGL(k_nhat_w, dim3(natom, nsys), dim3(64), 0, st, atoms, natom, rhoij, mtot, qtab, nl2, nsys,
                        wl);
   }
-  hipLaunchKernelGGL(k_nhat_grid, dim3(grid_for(N * nsys)), dim3(kT), 0, st, S, sph_g, wl, natom, nl2, N, nsys, nh);
+  hipLaunchKernelGGL(k_nhat_grid, grid_rows(N, nsys, 1), dim3(kT), 0, st, S, sph_g, wl, natom, nl2, (int)N, nh);
   return hipGetLastError();
 }
 
@@ -638,7 +736,7 @@ hipError_t renorm_factor(hipStream_t st, const double* nh, const double* dens, l
 
 hipError_t scale_pack(hipStream_t st, double* dens, double* nh, const double* fac, long long N, int nsys,
                       c64* pair) {
-  hipLaunchKernelGGL(k_scale_pack, dim3(grid_for(N * ((nsys + 1) / 2))), dim3(kT), 0, st, dens, nh, fac, N, nsys,
+  hipLaunchKernelGGL(k_scale_pack, grid_rows(N, (nsys + 1) / 2, 1), dim3(kT), 0, st, dens, nh, fac, (int)N, nsys,
                      pair);
   return hipGetLastError();
 }
@@ -654,7 +752,7 @@ hipError_t pair_means(hipStream_t st, const c64* pair, long long N, int nsys, do
 hipError_t vks_build(hipStream_t st, const double* dens, const double* ncoret, const c64* vhpair,
                      const double* vhmean, const double* vloc, long long N, int nsys, double* vxc, double* vh,
                      double* vks, float* vksf) {
-  hipLaunchKernelGGL(k_vks, dim3(grid_for(N * nsys)), dim3(kT), 0, st, dens, ncoret, vhpair, vhmean, vloc, N, nsys,
+  hipLaunchKernelGGL(k_vks, grid_rows(N, nsys, 1), dim3(kT), 0, st, dens, ncoret, vhpair, vhmean, vloc, (int)N,
                      vxc, vh, vks, vksf);
   return hipGetLastError();
 }
@@ -717,7 +815,9 @@ hipError_t convert_to_c64(hipStream_t st, const cx<R>* in, c64* out, long long n
 
 #define OPAW_INST(R)                                                                                            \
   template hipError_t rk4_update<R>(hipStream_t, int, long long, int, const cx<R>*, cx<R>*, cx<R>*, cx<R>*,     \
-                                    double, double, const int*, const c64*, int);                               \
+                                    double, double, const int*, const c64*, int, double*, double, double*);     \
+  template hipError_t scale_dipole<R>(hipStream_t, cx<R>*, int, int, int, int, int, double, double, double,     \
+                                      const double*, const double*, double*, double*);                          \
   template hipError_t field_norms<R>(hipStream_t, const cx<R>*, long long, int, double, double*, double*);      \
   template hipError_t scale_fields<R>(hipStream_t, cx<R>*, long long, int, const double*, const double*);       \
   template hipError_t density<R>(hipStream_t, const cx<R>*, long long, int, int, double*, const int*,          \

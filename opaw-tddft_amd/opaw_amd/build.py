@@ -25,7 +25,8 @@ LIB_PATH = PKG_DIR / LIB_NAME
 ARCH = os.environ.get("OPAW_OFFLOAD_ARCH", "gfx950")
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
-    # packed FP32 instructions stay off (see DESIGN notes on gfx950 codegen)
+    # packed FP32 instructions stay off: some of their forms have given
+    # silently wrong results on gfx950
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
