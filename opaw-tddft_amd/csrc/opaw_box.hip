@@ -41,54 +41,60 @@ __global__ __launch_bounds__(kBoxThreads) void k_proj(const AtomDesc* __restrict
   for (int i = 0; i < MSP; ++i) accr[i] = acci[i] = 0.0;
   const int kend = min(A.npt, cd.y + kProjChunk);
   constexpr int PPT = kProjChunk / kPointLanes;   // points per lane
-  if (f < nf) {
-    const cx<R>* uf = u + (long long)f * N;
-    // issue every load of the lane first: grid indices, field values,
-    // corrections, table values; then accumulate
-    int pt[PPT];
+  // Every load is unconditional (point index clamped into the chunk,
+  // projector index clamped to the atom's ms, field clamped to nf - 1) so the
+  // compiler issues them together; masked points contribute zero and
+  // projectors i >= ms and fields f >= nf are never written.
+  const int fc = min(f, nf - 1);
+  const cx<R>* uf = u + (long long)fc * N;
+  int kk[PPT];
+  bool ok[PPT];
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) {
+    const int k = cd.y + kl + q * kPointLanes;
+    ok[q] = k < kend;
+    kk[q] = ok[q] ? k : cd.y;
+  }
+  int pt[PPT];
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) pt[q] = bpt[A.boff + kk[q]];
+  double tv[PPT][MSP];
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) {
+    const double* t = tabs + A.tab + kk[q];
+#pragma unroll
+    for (int i = 0; i < MSP; ++i) tv[q][i] = t[(long long)(i < A.ms ? i : 0) * A.npt];
+  }
+  cx<R> v[PPT];
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) v[q] = uf[pt[q]];
+  double vr[PPT], vi[PPT];
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) {
+    vr[q] = ok[q] ? (double)v[q].x : 0.0;
+    vi[q] = ok[q] ? (double)v[q].y : 0.0;
+  }
+  if constexpr (PRE) {
+    int bq[PPT];
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) bq[q] = bk[A.boff + kk[q]];   // every box point is a union point
+    c64 cv[PPT];
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) cv[q] = cpre[(long long)bq[q] * nf + fc];
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
-      const int k = cd.y + kl + q * kPointLanes;
-      pt[q] = (k < kend) ? bpt[A.boff + k] : -1;
-    }
-    double vr[PPT], vi[PPT];
-#pragma unroll
-    for (int q = 0; q < PPT; ++q) {
-      cx<R> v = mk<R>(0, 0);
-      if (pt[q] >= 0) v = uf[pt[q]];
-      vr[q] = (double)v.x;
-      vi[q] = (double)v.y;
-    }
-    if constexpr (PRE) {
-      int kk[PPT];
-#pragma unroll
-      for (int q = 0; q < PPT; ++q) {
-        const int k = cd.y + kl + q * kPointLanes;
-        kk[q] = (k < kend) ? bk[A.boff + k] : -1;
-      }
-#pragma unroll
-      for (int q = 0; q < PPT; ++q) {
-        if (kk[q] >= 0) {
-          const c64 c = cpre[(long long)kk[q] * nf + f];
-          vr[q] += c.x;
-          vi[q] += c.y;
-        }
+      if (ok[q]) {
+        vr[q] += cv[q].x;
+        vi[q] += cv[q].y;
       }
     }
+  }
 #pragma unroll
-    for (int q = 0; q < PPT; ++q) {
-      const int k = cd.y + kl + q * kPointLanes;
-      if (k < kend) {
-        const double* t = tabs + A.tab + k;
+  for (int q = 0; q < PPT; ++q) {
 #pragma unroll
-        for (int i = 0; i < MSP; ++i) {
-          if (i < A.ms) {
-            const double tv = t[(long long)i * A.npt];
-            accr[i] += vr[q] * tv;
-            acci[i] += vi[q] * tv;
-          }
-        }
-      }
+    for (int i = 0; i < MSP; ++i) {
+      accr[i] += vr[q] * tv[q][i];
+      acci[i] += vi[q] * tv[q][i];
     }
   }
 #pragma unroll
@@ -325,16 +331,42 @@ __global__ __launch_bounds__(kBoxThreads) void k_union_corr_lds(const int2* __re
   const int2 blk = ublk[blockIdx.x];                      // (first point, point count)
   const int f0 = blockIdx.y * FG;
   const int q0 = pptr[blk.x];
-  for (int q = threadIdx.x; q < ctot * FG; q += kBoxThreads) {
-    const int c = q / FG, j = q % FG;
-    wl[q] = (f0 + j < nf) ? w[(long long)c * nf + f0 + j] : mk(0.0, 0.0);
+  const int np = pptr[blk.x + blk.y] - q0;                // >= 1: every union point has a pair
+  // staging: every thread issues its loads in batches (clamped indices)
+  const int nw = ctot * FG;
+  for (int b0 = 0; b0 < nw; b0 += kBoxThreads * 8) {
+    c64 t[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int q = min(b0 + s * kBoxThreads + (int)threadIdx.x, nw - 1);
+      t[s] = w[(long long)(q / FG) * nf + min(f0 + q % FG, nf - 1)];
+    }
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int q = b0 + s * kBoxThreads + (int)threadIdx.x;
+      if (q < nw) wl[q] = (f0 + q % FG < nf) ? t[s] : mk(0.0, 0.0);
+    }
   }
-  for (int r = threadIdx.x; r <= blk.y; r += kBoxThreads) rl[r] = pptr[blk.x + r] - q0;
-  const int q1 = pptr[blk.x + blk.y];
-  for (int q = q0 + threadIdx.x; q < q1; q += kBoxThreads) {
-    tl[q - q0] = pv[q];
-    cl[q - q0] = pc[q];
+  {
+    constexpr int PB = kUnionBlockPairs / kBoxThreads;
+    double tv[PB];
+    int cv[PB];
+#pragma unroll
+    for (int s = 0; s < PB; ++s) {
+      const int q = min(s * kBoxThreads + (int)threadIdx.x, np - 1);
+      tv[s] = pv[q0 + q];
+      cv[s] = pc[q0 + q];
+    }
+#pragma unroll
+    for (int s = 0; s < PB; ++s) {
+      const int q = s * kBoxThreads + (int)threadIdx.x;
+      if (q < np) {
+        tl[q] = tv[s];
+        cl[q] = cv[s];
+      }
+    }
   }
+  if ((int)threadIdx.x <= blk.y) rl[threadIdx.x] = pptr[blk.x + threadIdx.x] - q0;
   __syncthreads();
   constexpr int PL = kBoxThreads / FG;
   const int j = threadIdx.x % FG;
@@ -342,7 +374,25 @@ __global__ __launch_bounds__(kBoxThreads) void k_union_corr_lds(const int2* __re
   for (int r = threadIdx.x / FG; r < blk.y; r += PL) {
     const int a = rl[r], e = rl[r + 1];
     double xr = 0.0, xi = 0.0;
-    for (int p = a; p < e; ++p) {
+    int p = a;
+    for (; p + 4 <= e; p += 4) {
+      int cc[4];
+      double tt[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        cc[s] = cl[p + s];
+        tt[s] = tl[p + s];
+      }
+      c64 wv[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) wv[s] = wl[cc[s] * FG + j];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        xr += tt[s] * wv[s].x;
+        xi += tt[s] * wv[s].y;
+      }
+    }
+    for (; p < e; ++p) {
       const c64 wv = wl[cl[p] * FG + j];
       const double t = tl[p];
       xr += t * wv.x;

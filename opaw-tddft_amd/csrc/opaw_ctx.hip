@@ -150,6 +150,7 @@ struct opaw_ctx {
   int2 *chunks = nullptr, *achunk = nullptr, *coef = nullptr;
   int *umap = nullptr, *uptr = nullptr, *upt = nullptr, *bpt = nullptr, *bk = nullptr;
   int *pptr = nullptr, *pc = nullptr;
+  int* uzr = nullptr;        // nz+1: union points of plane z are [uzr[z], uzr[z+1])
   int2* ublk = nullptr;
   int nublk = 0;
   double *pv = nullptr, *pv1 = nullptr;
@@ -626,7 +627,8 @@ int opaw_finalize_tables(opaw_ctx* c) {
                   (void*)c->chunks, (void*)c->achunk, (void*)c->coef, (void*)c->umap, (void*)c->uptr, (void*)c->upt,
                   (void*)c->uent, (void*)c->utp, (void*)c->utp1, (void*)c->smap, (void*)c->sptr, (void*)c->sphn,
                   (void*)c->sent, (void*)c->wl, (void*)c->bpt, (void*)c->bk,
-                  (void*)c->pptr, (void*)c->pc, (void*)c->pv, (void*)c->pv1, (void*)c->ublk})
+                  (void*)c->pptr, (void*)c->pc, (void*)c->pv, (void*)c->pv1, (void*)c->ublk,
+                  (void*)c->uzr})
     dfree(c, p);
   c->prep_valid = false;
   c->coef_cap = 0;   // projection buffers depend on msp / chunk count
@@ -828,6 +830,13 @@ int opaw_finalize_tables(opaw_ctx* c) {
     RCHK(upload(c, &c->umap, umap));
     RCHK(upload(c, &c->uptr, uptr));
     RCHK(upload(c, &c->upt, upt2));
+    {
+      const long long plane = (long long)c->nx * c->ny;
+      std::vector<int> uzr(c->nz + 1);
+      for (int z = 0; z <= c->nz; ++z)
+        uzr[z] = (int)(std::lower_bound(upt2.begin(), upt2.end(), (long long)z * plane) - upt2.begin());
+      RCHK(upload(c, &c->uzr, uzr));
+    }
     RCHK(upload(c, &c->uent, ue));
     RCHK(upload(c, &c->utp, tp));
     RCHK(upload(c, &c->utp1, tp1));
@@ -1134,7 +1143,7 @@ static int flow_h_phi(opaw_ctx* c, int sys0, const cx<R>* in, cx<R>* out, int nf
   HCHK(union_corr(bt, c->st, nf, 0, (const c64*)c->wB.p, (c64*)c->corrNL.p));
   HCHK(apply_t_plus_v<R>(c->g, c->st, in, out, (cx<R>*)c->wtmp.p, nf, nf_per_sys, Fields<R>::ek(c),
                          Fields<R>::vks(c) + (size_t)sys0 * c->N, TW<R>::get(c, c->nx), TW<R>::get(c, c->ny),
-                         TW<R>::get(c, c->nz), c->umap, nullptr, (const c64*)c->corrNL.p, nf));
+                         TW<R>::get(c, c->nz), c->umap, c->upt, c->uzr, nullptr, (const c64*)c->corrNL.p, nf));
   return OPAW_OK;
 }
 
@@ -1150,7 +1159,8 @@ static int flow_hs(opaw_ctx* c, int sys0, const cx<R>* in, cx<R>* h, int nf, int
   HCHK(union_corr(bt, c->st, nf, 0, (const c64*)c->wB.p, (c64*)c->corrNL.p));
   HCHK(apply_t_plus_v<R>(c->g, c->st, in, h, (cx<R>*)c->wtmp.p, nf, nf_per_sys, Fields<R>::ek(c),
                          Fields<R>::vks(c) + (size_t)sys0 * c->N, TW<R>::get(c, c->nx), TW<R>::get(c, c->ny),
-                         TW<R>::get(c, c->nz), c->umap, (const c64*)c->corrA.p, (const c64*)c->corrNL.p, nf));
+                         TW<R>::get(c, c->nz), c->umap, c->upt, c->uzr, (const c64*)c->corrA.p,
+                         (const c64*)c->corrNL.p, nf));
   HCHK(box_coeffs<R>(c->g, bt, c->st, h, nf, 1, nullptr, dvol(c), (double*)c->ppart.p, (c64*)c->cC.p, 1, c->sn[1],
                      nullptr, nf, c->mtot, (c64*)c->wC.p));
   HCHK(union_corr(bt, c->st, nf, 1, (const c64*)c->wC.p, (c64*)c->corrC.p));
@@ -1233,6 +1243,7 @@ static int flow_rk4(opaw_ctx* c, double dt, double* dip_out) {
   cx<R>* acc = (cx<R>*)c->wacc.p;
   cx<R>* h = (cx<R>*)c->wh.p;
   if (!c->prep_valid) RCHK(flow_prep<R>(c, p, nf));
+  const BoxTables bt = box_tables(c);
   prof_begin(c->st, KC_DENS);
   HCHK(field_norms<R>(c->st, p, c->N, nf, dv, c->part, c->nrm0));
   prof_end(c->st, KC_DENS);
@@ -1240,10 +1251,11 @@ static int flow_rk4(opaw_ctx* c, double dt, double* dip_out) {
   for (int stage = 1; stage <= 4; ++stage) {
     const cx<R>* yin = (stage == 1) ? p : y;
     RCHK(flow_hs<R>(c, 0, yin, h, nf, c->nb));
+    // the outer S^-1/2 of shs: h += corrC on the union of boxes
+    HCHK(union_apply<R>(c->g, bt, c->st, h, nf, (const c64*)c->corrC.p));
     // stage 4 also forms the post-step norms (norm1) of every orbital
     prof_begin(c->st, KC_RK4);
-    HCHK(rk4_update<R>(c->st, stage, c->N, nf, h, p, acc, y, cst[stage - 1], dt, c->umap, (const c64*)c->corrC.p,
-                       nf, c->part, dv, c->nrm1));
+    HCHK(rk4_update<R>(c->st, stage, c->N, nf, h, p, acc, y, cst[stage - 1], dt, c->part, dv, c->nrm1));
     prof_end(c->st, KC_RK4);
     if (stage < 4) RCHK(flow_rebuild<R>(c, 0, c->nsys, y));
   }

@@ -220,10 +220,17 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fft_plane(const cx<R>* in, cx
                                                               const cx<R>* __restrict__ twyg,
                                                               const cx<R>* __restrict__ uin,
                                                               const R* __restrict__ vks, int nf_per_sys,
-                                                              const int* __restrict__ umap,
+                                                              const int* __restrict__ upt,
+                                                              const int* __restrict__ uzr,
                                                               const c64* __restrict__ cpre,
                                                               const c64* __restrict__ cnl, int U) {
   constexpr int LS = plane_stride<R, NX, NY>();
+  constexpr int NP = NX * NY;
+  constexpr int PER = (NP + kPlaneThreads - 1) / kPlaneThreads;
+  constexpr bool EXACT = NP % kPlaneThreads == 0;
+  // the epilogue's global operands are loaded before the transforms when
+  // they fit in a few registers, so their latency hides behind the FFT
+  constexpr bool EARLY = EPI == EPI_HPSI && PER * (3 * sizeof(R)) <= 96;
   __shared__ cx<R> tile[NY * LS];
   __shared__ cx<R> twx[NX];
   __shared__ cx<R> twy[NY];
@@ -231,36 +238,45 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fft_plane(const cx<R>* in, cx
   fft::load_twiddles(twy, twyg, NY);
   const int f = blockIdx.x / nz;
   const int z = blockIdx.x % nz;
-  const long long plane0 = (long long)z * NX * NY;           // offset of the plane in a field
-  constexpr int PER = (NX * NY + kPlaneThreads - 1) / kPlaneThreads;
+  const long long plane0 = (long long)z * NP;            // offset of the plane in a field
   const cx<R>* src = in + (long long)f * N + plane0;
+  // plane element idx of this thread's q-th slot, clamped for the loads
+  auto slot = [](int q) { return (int)threadIdx.x + q * kPlaneThreads; };
+  auto lidx = [&](int q) { return EXACT ? slot(q) : min(slot(q), NP - 1); };
+  auto tix = [](int idx) { return (idx / NX) * LS + idx % NX; };
+  const R* vk = nullptr;
+  const cx<R>* us = nullptr;
+  if constexpr (EPI == EPI_HPSI) {
+    vk = vks + (long long)(f / nf_per_sys) * N + plane0;
+    us = uin + (long long)f * N + plane0;
+  }
+  cx<R> uu[EARLY ? PER : 1];
+  R vv[EARLY ? PER : 1];
   {
-    // all plane loads of a thread first, then the correction lookups
     cx<R> v[PER];
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int idx = threadIdx.x + q * kPlaneThreads;
-      v[q] = (idx < NX * NY) ? src[idx] : mk<R>(0, 0);
-    }
-    if (EPI == EPI_NONE && cpre) {
-      int k[PER];
+    for (int q = 0; q < PER; ++q) v[q] = src[lidx(q)];
+    if constexpr (EARLY) {
 #pragma unroll
       for (int q = 0; q < PER; ++q) {
-        const int idx = threadIdx.x + q * kPlaneThreads;
-        k[q] = (idx < NX * NY) ? umap[plane0 + idx] : -1;
-      }
-#pragma unroll
-      for (int q = 0; q < PER; ++q) {
-        if (k[q] >= 0) {
-          const c64 c = cpre[(long long)k[q] * U + f];
-          v[q] = mk<R>((R)((double)v[q].x + c.x), (R)((double)v[q].y + c.y));
-        }
+        uu[q] = us[lidx(q)];
+        vv[q] = vk[lidx(q)];
       }
     }
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int idx = threadIdx.x + q * kPlaneThreads;
-      if (idx < NX * NY) tile[(idx / NX) * LS + idx % NX] = v[q];
+    for (int q = 0; q < PER; ++q)
+      if (EXACT || slot(q) < NP) tile[tix(slot(q))] = v[q];
+  }
+  // union points of this plane: a contiguous range (the union is sorted by
+  // grid point); forward pass: u = in + cpre there
+  const int k0 = (cpre || cnl) ? uzr[z] : 0, k1 = (cpre || cnl) ? uzr[z + 1] : 0;
+  if (EPI == EPI_NONE && cpre) {
+    __syncthreads();
+    for (int k = k0 + (int)threadIdx.x; k < k1; k += kPlaneThreads) {
+      const int loc = (int)(upt[k] - plane0);
+      const c64 c = cpre[(long long)k * U + f];
+      cx<R>& t = tile[tix(loc)];
+      t = mk<R>((R)((double)t.x + c.x), (R)((double)t.y + c.y));
     }
   }
   __syncthreads();
@@ -273,47 +289,53 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fft_plane(const cx<R>* in, cx
   }
   cx<R>* dst = out + (long long)f * N + plane0;
   if constexpr (EPI == EPI_HPSI) {
-    const R* vk = vks + (long long)(f / nf_per_sys) * N + plane0;
-    const cx<R>* us = uin + (long long)f * N + plane0;
-    cx<R> uu[PER];
-    R vv[PER];
-    int k[PER];
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int idx = threadIdx.x + q * kPlaneThreads;
-      const bool ok = idx < NX * NY;
-      uu[q] = ok ? us[idx] : mk<R>(0, 0);
-      vv[q] = ok ? vk[idx] : R(0);
-      k[q] = (ok && (cpre || cnl)) ? umap[plane0 + idx] : -1;
-    }
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int idx = threadIdx.x + q * kPlaneThreads;
-      if (idx >= NX * NY) continue;
-      const cx<R> t = tile[(idx / NX) * LS + idx % NX];
-      double ur = (double)uu[q].x, ui = (double)uu[q].y;
-      double ar = (double)t.x, ai = (double)t.y;
-      if (k[q] >= 0) {
+    // h = T u + vks u + cnl with u = uin + cpre: the union terms
+    // vks * cpre + cnl go into the tile first, then the plain epilogue
+    if (cpre || cnl) {
+      for (int k = k0 + (int)threadIdx.x; k < k1; k += kPlaneThreads) {
+        const int loc = (int)(upt[k] - plane0);
+        double ar = 0.0, ai = 0.0;
         if (cpre) {
-          const c64 c = cpre[(long long)k[q] * U + f];
-          ur += c.x;
-          ui += c.y;
+          const double w = (double)vk[loc];
+          const c64 c = cpre[(long long)k * U + f];
+          ar += w * c.x;
+          ai += w * c.y;
         }
         if (cnl) {
-          const c64 c = cnl[(long long)k[q] * U + f];
+          const c64 c = cnl[(long long)k * U + f];
           ar += c.x;
           ai += c.y;
         }
+        cx<R>& t = tile[tix(loc)];
+        t = mk<R>((R)((double)t.x + ar), (R)((double)t.y + ai));
       }
-      const double w = (double)vv[q];
-      dst[idx] = mk<R>((R)(ar + w * ur), (R)(ai + w * ui));
+      __syncthreads();
+    }
+    if constexpr (!EARLY) {
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        uu[0] = us[lidx(q)];
+        vv[0] = vk[lidx(q)];
+        if (EXACT || slot(q) < NP) {
+          const cx<R> t = tile[tix(slot(q))];
+          const double w = (double)vv[0];
+          dst[slot(q)] = mk<R>((R)((double)t.x + w * (double)uu[0].x), (R)((double)t.y + w * (double)uu[0].y));
+        }
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        if (EXACT || slot(q) < NP) {
+          const cx<R> t = tile[tix(slot(q))];
+          const double w = (double)vv[q];
+          dst[slot(q)] = mk<R>((R)((double)t.x + w * (double)uu[q].x), (R)((double)t.y + w * (double)uu[q].y));
+        }
+      }
     }
   } else {
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int idx = threadIdx.x + q * kPlaneThreads;
-      if (idx < NX * NY) dst[idx] = tile[(idx / NX) * LS + idx % NX];
-    }
+    for (int q = 0; q < PER; ++q)
+      if (EXACT || slot(q) < NP) dst[slot(q)] = tile[tix(slot(q))];
   }
 }
 
@@ -334,21 +356,21 @@ static bool plane_supported(int nx, int ny) {
 
 template <typename R, int S, int EPI>
 static hipError_t launch_plane(int n, hipStream_t st, const cx<R>* in, cx<R>* out, int nz, int nf, long long N,
-                               const cx<R>* tw, const cx<R>* uin, const R* vks, int nfps, const int* umap,
-                               const c64* cpre, const c64* cnl, int U) {
+                               const cx<R>* tw, const cx<R>* uin, const R* vks, int nfps, const int* upt,
+                               const int* uzr, const c64* cpre, const c64* cnl, int U) {
   dim3 grid((unsigned)((long long)nf * nz));
   switch (n) {
 #define OPAW_CASE(nn)                                                                                          \
   case nn:                                                                                                     \
     hipLaunchKernelGGL((k_fft_plane<R, nn, nn, S, EPI>), grid, dim3(kPlaneThreads), 0, st, in, out, nz, N, tw, tw, \
-                       uin, vks, nfps, umap, cpre, cnl, U);                                                     \
+                       uin, vks, nfps, upt, uzr, cpre, cnl, U);                                                     \
     return hipGetLastError();
     OPAW_PLANE_SIZES(OPAW_CASE)
 #undef OPAW_CASE
     case 128:
       if constexpr (sizeof(R) == 4) {
         hipLaunchKernelGGL((k_fft_plane<R, 128, 128, S, EPI>), grid, dim3(kPlaneThreads), 0, st, in, out, nz, N, tw,
-                           tw, uin, vks, nfps, umap, cpre, cnl, U);
+                           tw, uin, vks, nfps, upt, uzr, cpre, cnl, U);
         return hipGetLastError();
       }
       return hipErrorInvalidValue;
@@ -459,7 +481,8 @@ template <typename R>
 hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* in, cx<R>* out, cx<R>* tmp,
                           int nf, int nf_per_sys, const R* ek, const R* vks,
                           const cx<R>* twx, const cx<R>* twy, const cx<R>* twz,
-                          const int* umap, const c64* cpre, const c64* cnl, int U) {
+                          const int* umap, const int* upt, const int* uzr, const c64* cpre, const c64* cnl,
+                          int U) {
   const long long N = (long long)g.nx * g.ny * g.nz;
   View v{N, 0, 0, 0, g.nx, g.ny};
   XLines Lx{0, g.ny, 0, g.nz, nf, 0, g.nx, 0, g.nx};
@@ -469,13 +492,14 @@ hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* in, cx
   hipError_t e;
   if (plane_supported<R>(g.nx, g.ny)) {
     prof_begin(st, KC_FFT_X);
-    e = launch_plane<R, -1, EPI_NONE>(g.nx, st, in, tmp, g.nz, nf, N, twx, nullptr, nullptr, nf_per_sys, umap, cpre,
-                                      nullptr, U);
+    e = launch_plane<R, -1, EPI_NONE>(g.nx, st, in, tmp, g.nz, nf, N, twx, nullptr, nullptr, nf_per_sys, upt, uzr,
+                                      cpre, nullptr, U);
     prof_end(st, KC_FFT_X);
     if (e) return e;
     if ((e = fft_s_pass<R>(KC_FFT_Z, g.nz, 2, -1, EPI_MULT, st, tmp, v, tmp, v, Lz, twz, ek, vt))) return e;
     prof_begin(st, KC_FFT_XH);
-    e = launch_plane<R, 1, EPI_HPSI>(g.nx, st, tmp, out, g.nz, nf, N, twx, in, vks, nf_per_sys, umap, cpre, cnl, U);
+    e = launch_plane<R, 1, EPI_HPSI>(g.nx, st, tmp, out, g.nz, nf, N, twx, in, vks, nf_per_sys, upt, uzr, cpre, cnl,
+                                     U);
     prof_end(st, KC_FFT_XH);
     return e;
   }
@@ -522,10 +546,10 @@ hipError_t hartree_conv(const GridGeom& g, int scale, hipStream_t st, const c64*
 
 template hipError_t apply_t_plus_v<double>(const GridGeom&, hipStream_t, const c64*, c64*, c64*, int, int,
                                            const double*, const double*, const c64*, const c64*, const c64*,
-                                           const int*, const c64*, const c64*, int);
+                                           const int*, const int*, const int*, const c64*, const c64*, int);
 template hipError_t apply_t_plus_v<float>(const GridGeom&, hipStream_t, const c32*, c32*, c32*, int, int,
                                           const float*, const float*, const c32*, const c32*, const c32*,
-                                          const int*, const c64*, const c64*, int);
+                                          const int*, const int*, const int*, const c64*, const c64*, int);
 
 // plain 3D transform of nf fields (used by tests and the Hartree-free paths)
 template <typename R>

@@ -23,8 +23,7 @@ struct SphereTabs {
 // stage 4 also writes the per-field norms (dv sum|p|^2, then sqrt) to norms
 template <typename R>
 hipError_t rk4_update(hipStream_t st, int stage, long long N, int nf, const cx<R>* h, cx<R>* p, cx<R>* acc,
-                      cx<R>* y, double c, double dt, const int* umap, const c64* cc, int U, double* part,
-                      double dv, double* norms);
+                      cx<R>* y, double c, double dt, double* part, double dv, double* norms);
 template <typename R>
 hipError_t field_norms(hipStream_t st, const cx<R>* u, long long N, int nf, double dv, double* part, double* out);
 template <typename R>

@@ -48,44 +48,34 @@ __device__ __forceinline__ void chunk_range(long long n, int nchunk, int c, long
 // Streaming field kernels run on 2D grids: blockIdx.y is the field (or
 // system), blockIdx.x covers the grid points, kV points per thread.
 constexpr int kV = 4;
+// orbitals whose loads a thread issues together in the per-point orbital sums
+constexpr int kOrbBatch = 8;
 
-// RK4 stage update (src/rk4.f90:35-56 and :22): k = -i hs, where
-// hs = h + cc (the outer S^-1/2 of shs as a compact union correction) when cc
-// is set
+// RK4 stage update (src/rk4.f90:35-56 and :22), k = -i hs with hs the
+// complete shs output (its outer S^-1/2 add-back already in h):
 //   stage 1: acc = k;        y = p + c k
 //   stage 2,3: acc += 2k;    y = p + c k
-template <typename R>
-__global__ __launch_bounds__(kT) void k_rk4_update(int stage, int N, const cx<R>* __restrict__ h,
-                                                    const cx<R>* __restrict__ p, cx<R>* __restrict__ acc,
-                                                    cx<R>* __restrict__ y, double c,
-                                                    const int* __restrict__ umap, const c64* __restrict__ cc,
-                                                    int U) {
-  const int f = blockIdx.y;
-  const long long off = (long long)f * N;
+// Loads are unconditional (indices clamped) so a thread's kV loads are in
+// flight together; only the stores are guarded.
+template <typename R, bool FIRST>
+__global__ __launch_bounds__(kT) void k_rk4_update(int N, const cx<R>* __restrict__ h, const cx<R>* __restrict__ p,
+                                                    cx<R>* __restrict__ acc, cx<R>* __restrict__ y, double c) {
+  const long long off = (long long)blockIdx.y * N;
   const int base = blockIdx.x * (kT * kV) + threadIdx.x;
   cx<R> hh[kV], pp[kV], aa[kV];
-  int kk[kV];
 #pragma unroll
   for (int j = 0; j < kV; ++j) {
-    const int r = base + j * kT;
-    const bool ok = r < N;
-    hh[j] = ok ? h[off + r] : mk<R>(0, 0);
-    pp[j] = ok ? p[off + r] : mk<R>(0, 0);
-    aa[j] = (ok && stage > 1) ? acc[off + r] : mk<R>(0, 0);
-    kk[j] = (ok && cc) ? umap[r] : -1;
+    const long long i = off + min(base + j * kT, N - 1);
+    hh[j] = h[i];
+    pp[j] = p[i];
+    if constexpr (!FIRST) aa[j] = acc[i];
   }
 #pragma unroll
   for (int j = 0; j < kV; ++j) {
     const int r = base + j * kT;
     if (r >= N) continue;
-    double hr = (double)hh[j].x, hi = (double)hh[j].y;
-    if (kk[j] >= 0) {
-      const c64 v = cc[(long long)kk[j] * U + f];
-      hr += v.x;
-      hi += v.y;
-    }
-    const double kr = hi, ki = -hr;
-    if (stage == 1)
+    const double kr = (double)hh[j].y, ki = -(double)hh[j].x;
+    if constexpr (FIRST)
       acc[off + r] = mk<R>((R)kr, (R)ki);
     else
       acc[off + r] = mk<R>((R)((double)aa[j].x + 2.0 * kr), (R)((double)aa[j].y + 2.0 * ki));
@@ -93,38 +83,37 @@ __global__ __launch_bounds__(kT) void k_rk4_update(int stage, int N, const cx<R>
   }
 }
 
-// RK4 stage 4 (src/rk4.f90:22): p = p + dt/6 (acc + k) with k = -i (h + cc),
-// fused with the per-field partial sums of |p|^2 for the norm restore
-// (norm1, src/rk4.f90:92-102).  Grid (nchunk, nf); the chunking is the one of
-// k_norm_partial.
+// RK4 stage 4 (src/rk4.f90:22): p = p + dt/6 (acc + k), k = -i h, fused with
+// the per-field partial sums of |p|^2 for the norm restore (norm1,
+// src/rk4.f90:92-102).  Grid (nchunk, nf); the chunking of k_norm_partial.
 template <typename R>
 __global__ __launch_bounds__(kT) void k_rk4_final(long long N, int nchunk, const cx<R>* __restrict__ h,
                                                    cx<R>* __restrict__ p, const cx<R>* __restrict__ acc,
-                                                   double dt6, const int* __restrict__ umap,
-                                                   const c64* __restrict__ cc, int U,
-                                                   double* __restrict__ part) {
+                                                   double dt6, double* __restrict__ part) {
   const int ch = blockIdx.x, f = blockIdx.y;
   long long lo, hi;
   chunk_range(N, nchunk, ch, lo, hi);
   const long long off = (long long)f * N;
   double v[1] = {0.0};
-  for (long long r = lo + threadIdx.x; r < hi; r += kT) {
-    const cx<R> hh = h[off + r];
-    const cx<R> pp = p[off + r];
-    const cx<R> a = acc[off + r];
-    double hr = (double)hh.x, hi2 = (double)hh.y;
-    if (cc) {
-      const int k = umap[r];
-      if (k >= 0) {
-        const c64 w = cc[(long long)k * U + f];
-        hr += w.x;
-        hi2 += w.y;
-      }
+  for (long long b0 = lo; b0 < hi; b0 += (long long)kT * kV) {
+    cx<R> hh[kV], pp[kV], aa[kV];
+#pragma unroll
+    for (int j = 0; j < kV; ++j) {
+      const long long r = min(b0 + threadIdx.x + (long long)j * kT, hi - 1);
+      hh[j] = h[off + r];
+      pp[j] = p[off + r];
+      aa[j] = acc[off + r];
     }
-    const double kr = hi2, ki = -hr;
-    const cx<R> q = mk<R>((R)((double)pp.x + dt6 * ((double)a.x + kr)), (R)((double)pp.y + dt6 * ((double)a.y + ki)));
-    p[off + r] = q;
-    v[0] += (double)q.x * (double)q.x + (double)q.y * (double)q.y;
+#pragma unroll
+    for (int j = 0; j < kV; ++j) {
+      const long long r = b0 + threadIdx.x + (long long)j * kT;
+      if (r >= hi) continue;
+      const double kr = (double)hh[j].y, ki = -(double)hh[j].x;
+      const cx<R> q = mk<R>((R)((double)pp[j].x + dt6 * ((double)aa[j].x + kr)),
+                            (R)((double)pp[j].y + dt6 * ((double)aa[j].y + ki)));
+      p[off + r] = q;
+      v[0] += (double)q.x * (double)q.x + (double)q.y * (double)q.y;
+    }
   }
   block_sum<1>(v);
   if (threadIdx.x == 0) part[(long long)f * nchunk + ch] = v[0];
@@ -169,13 +158,13 @@ __global__ __launch_bounds__(kT) void k_scale_fields(cx<R>* __restrict__ u, int 
   const double s = num[f] / den[f];
   cx<R>* uf = u + (long long)f * N;
   const int base = blockIdx.x * (kT * kV) + threadIdx.x;
+  cx<R> v[kV];
+#pragma unroll
+  for (int j = 0; j < kV; ++j) v[j] = uf[min(base + j * kT, N - 1)];
 #pragma unroll
   for (int j = 0; j < kV; ++j) {
     const int r = base + j * kT;
-    if (r < N) {
-      const cx<R> v = uf[r];
-      uf[r] = mk<R>((R)((double)v.x * s), (R)((double)v.y * s));
-    }
+    if (r < N) uf[r] = mk<R>((R)((double)v[j].x * s), (R)((double)v[j].y * s));
   }
 }
 
@@ -200,11 +189,19 @@ __global__ __launch_bounds__(kT) void k_scale_dipole(cx<R>* __restrict__ u, int 
   cx<R>* us = u + (long long)s * nb * N;
   for (long long i = lo + threadIdx.x; i < hi; i += kT) {
     double rho = 0.0;
-    for (int o = 0; o < nb; ++o) {
-      const cx<R> w = us[(long long)o * N + i];
-      const cx<R> q = mk<R>((R)((double)w.x * fac[o]), (R)((double)w.y * fac[o]));
-      us[(long long)o * N + i] = q;
-      rho += (double)q.x * (double)q.x + (double)q.y * (double)q.y;
+    for (int o0 = 0; o0 < nb; o0 += kOrbBatch) {
+      cx<R> w[kOrbBatch];
+#pragma unroll
+      for (int q = 0; q < kOrbBatch; ++q) w[q] = us[(long long)min(o0 + q, nb - 1) * N + i];
+#pragma unroll
+      for (int q = 0; q < kOrbBatch; ++q) {
+        if (o0 + q < nb) {
+          const double s = fac[o0 + q];
+          const cx<R> v = mk<R>((R)((double)w[q].x * s), (R)((double)w[q].y * s));
+          us[(long long)(o0 + q) * N + i] = v;
+          rho += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+        }
+      }
     }
     rho *= 2.0;
     const int ii = (int)i;
@@ -232,15 +229,29 @@ __global__ __launch_bounds__(kT) void k_density(const cx<R>* __restrict__ u, int
   const cx<R>* us = u + (long long)s * nb * N + r;
   const int k = cpre ? umap[r] : -1;
   double acc = 0.0;
-  for (int o = 0; o < nb; ++o) {
-    const cx<R> v = us[(long long)o * N];
-    double vr = (double)v.x, vi = (double)v.y;
-    if (k >= 0) {
-      const c64 c = cpre[(long long)k * U + (s * nb + o)];
-      vr += c.x;
-      vi += c.y;
+  for (int o0 = 0; o0 < nb; o0 += kOrbBatch) {
+    cx<R> v[kOrbBatch];
+#pragma unroll
+    for (int q = 0; q < kOrbBatch; ++q) v[q] = us[(long long)min(o0 + q, nb - 1) * N];
+    double vr[kOrbBatch], vi[kOrbBatch];
+#pragma unroll
+    for (int q = 0; q < kOrbBatch; ++q) {
+      vr[q] = (double)v[q].x;
+      vi[q] = (double)v[q].y;
     }
-    acc += vr * vr + vi * vi;
+    if (k >= 0) {
+      c64 c[kOrbBatch];
+#pragma unroll
+      for (int q = 0; q < kOrbBatch; ++q) c[q] = cpre[(long long)k * U + s * nb + min(o0 + q, nb - 1)];
+#pragma unroll
+      for (int q = 0; q < kOrbBatch; ++q) {
+        vr[q] += c[q].x;
+        vi[q] += c[q].y;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kOrbBatch; ++q)
+      if (o0 + q < nb) acc += vr[q] * vr[q] + vi[q] * vi[q];
   }
   dens[(long long)s * N + r] = 2.0 * acc;
 }
@@ -453,17 +464,33 @@ __global__ __launch_bounds__(kT) void k_dij(const AtomDesc* __restrict__ atoms, 
     const double rv = rg[q];
     rho[q] = fabs(rv) > kRhoijSelect ? rv : 0.0;   // src/get_dij.f90:80-88 selection
   }
-  for (int q = threadIdx.x; q < nl2 * m2; q += kT) qq[q] = qtab[A.toff_q + q];
+  // loads are issued in batches with clamped indices (no load behind a
+  // branch), so a thread's loads are in flight together
+  const int nq = nl2 * m2;
+  for (int q0 = 0; q0 < nq; q0 += kT * 4) {
+    double t[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) t[u] = qtab[A.toff_q + min(q0 + u * kT + (int)threadIdx.x, nq - 1)];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = q0 + u * kT + (int)threadIdx.x;
+      if (q < nq) qq[q] = t[u];
+    }
+  }
   double acc[kMaxNl2];
 #pragma unroll
   for (int L = 0; L < kMaxNl2; ++L) acc[L] = 0.0;
   for (int p = threadIdx.x; p < A.sph_n; p += kT) {
     const long long r = sph_idx[A.sph_off + p];
-    double pot = vks[(long long)s * N + r];
-    if (subtract_vxc) pot -= vxc[(long long)s * N + r];
+    double g[kMaxNl2];
 #pragma unroll
     for (int L = 0; L < kMaxNl2; ++L)
-      if (L < nl2) acc[L] += pot * sph_g[(long long)A.sph_off * nl2 + (long long)L * A.sph_n + p];
+      g[L] = sph_g[(long long)A.sph_off * nl2 + (long long)min(L, nl2 - 1) * A.sph_n + p];
+    const double vk = vks[(long long)s * N + r];
+    const double vx = vxc[(long long)s * N + r];
+    const double pot = subtract_vxc ? vk - vx : vk;
+#pragma unroll
+    for (int L = 0; L < kMaxNl2; ++L) acc[L] += pot * g[L];
   }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
@@ -484,8 +511,14 @@ __global__ __launch_bounds__(kT) void k_dij(const AtomDesc* __restrict__ atoms, 
     const int i = q % ms, j = q / ms;
     double d = d0tab[A.toff_d0 + q];
     const double* e = etab + A.toff_e + (long long)q;   // E(i,j,k,l) at i + ms(j + ms(k + ms l))
-#pragma unroll 8
-    for (int kl = 0; kl < m2; ++kl) d += e[(long long)kl * m2] * rho[kl];
+    for (int kl0 = 0; kl0 < m2; kl0 += 16) {
+      double ev[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) ev[u] = e[(long long)min(kl0 + u, m2 - 1) * m2];
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        if (kl0 + u < m2) d += ev[u] * rho[kl0 + u];
+    }
     for (int L = 0; L < nl2; ++L) d += qq[L + nl2 * (i + ms * j)] * prod[L];
     M[q] = d;
   }
@@ -513,9 +546,13 @@ __global__ __launch_bounds__(kT) void k_dipole_partial(const cx<R>* __restrict__
   const cx<R>* us = u + (long long)s * nb * N;
   for (long long i = lo + threadIdx.x; i < hi; i += kT) {
     double rho = 0.0;
-    for (int o = 0; o < nb; ++o) {
-      const cx<R> w = us[(long long)o * N + i];
-      rho += (double)w.x * (double)w.x + (double)w.y * (double)w.y;
+    for (int o0 = 0; o0 < nb; o0 += kOrbBatch) {
+      cx<R> w[kOrbBatch];
+#pragma unroll
+      for (int q = 0; q < kOrbBatch; ++q) w[q] = us[(long long)min(o0 + q, nb - 1) * N + i];
+#pragma unroll
+      for (int q = 0; q < kOrbBatch; ++q)
+        if (o0 + q < nb) rho += (double)w[q].x * (double)w[q].x + (double)w[q].y * (double)w[q].y;
     }
     rho *= 2.0;
     const int ii = (int)i;
@@ -669,14 +706,13 @@ static dim3 grid_rows(long long N, int rows, int per_thread) {
 
 template <typename R>
 hipError_t rk4_update(hipStream_t st, int stage, long long N, int nf, const cx<R>* h, cx<R>* p, cx<R>* acc,
-                      cx<R>* y, double c, double dt, const int* umap, const c64* cc, int U, double* part,
-                      double dv, double* norms) {
-  if (stage < 4) {
-    hipLaunchKernelGGL((k_rk4_update<R>), grid_rows(N, nf, kV), dim3(kT), 0, st, stage, (int)N, h, p, acc, y, c,
-                       umap, cc, U);
+                      cx<R>* y, double c, double dt, double* part, double dv, double* norms) {
+  if (stage == 1) {
+    hipLaunchKernelGGL((k_rk4_update<R, true>), grid_rows(N, nf, kV), dim3(kT), 0, st, (int)N, h, p, acc, y, c);
+  } else if (stage < 4) {
+    hipLaunchKernelGGL((k_rk4_update<R, false>), grid_rows(N, nf, kV), dim3(kT), 0, st, (int)N, h, p, acc, y, c);
   } else {
-    hipLaunchKernelGGL((k_rk4_final<R>), dim3(kChunks, nf), dim3(kT), 0, st, N, kChunks, h, p, acc, dt / 6.0, umap,
-                       cc, U, part);
+    hipLaunchKernelGGL((k_rk4_final<R>), dim3(kChunks, nf), dim3(kT), 0, st, N, kChunks, h, p, acc, dt / 6.0, part);
     hipLaunchKernelGGL(k_finish_rows, dim3(nf), dim3(kT), 0, st, part, nf, kChunks, dv, 1, norms);
   }
   return hipGetLastError();
@@ -815,7 +851,7 @@ hipError_t convert_to_c64(hipStream_t st, const cx<R>* in, c64* out, long long n
 
 #define OPAW_INST(R)                                                                                            \
   template hipError_t rk4_update<R>(hipStream_t, int, long long, int, const cx<R>*, cx<R>*, cx<R>*, cx<R>*,     \
-                                    double, double, const int*, const c64*, int, double*, double, double*);     \
+                                    double, double, double*, double, double*);                                  \
   template hipError_t scale_dipole<R>(hipStream_t, cx<R>*, int, int, int, int, int, double, double, double,     \
                                       const double*, const double*, double*, double*);                          \
   template hipError_t field_norms<R>(hipStream_t, const cx<R>*, long long, int, double, double*, double*);      \

@@ -25,7 +25,8 @@ template <typename R>
 hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* in, cx<R>* out, cx<R>* tmp,
                           int nf, int nf_per_sys, const R* ek, const R* vks,
                           const cx<R>* twx, const cx<R>* twy, const cx<R>* twz,
-                          const int* umap, const c64* cpre, const c64* cnl, int U);
+                          const int* umap, const int* upt, const int* uzr, const c64* cpre, const c64* cnl,
+                          int U);
 
 template <typename R>
 hipError_t fft3d(const GridGeom& g, hipStream_t st, int dir, const cx<R>* in, cx<R>* out, int nf,
