@@ -82,6 +82,9 @@ def table_bytes(spec):
     return sum(int(a.nr[0] * a.nr[1] * a.nr[2]) * a.ms * 8 for a in spec.atoms)
 
 
+FAMILIES = ("fft_x", "fft_y", "fft_z", "fft_xh", "hartree", "box_proj", "box_add", "rk4", "dens", "pot", "other")
+
+
 def class_bytes(spec, nsys, nb, sc):
     """Algorithmic HBM bytes per launch of the single-kernel families of one
     shs / rebuild application over all nsys*nb fields (fft_x / fft_xh are the
@@ -113,18 +116,27 @@ def run_gpu(spec, phi0, prec, steps, warmup, nb):
     g = OpawGPU(spec, prec, nsys=2, nb=nb)
     g.write_orbitals(0, phi0)
     g.excite_psi_bar_pert(spec.sm, spec.ipol + 1)
-    # warmup, profiled for every family to find the dominant kernel
-    g.profile(True)
+    # warmup without instrumentation
     d_w, _ = g.opaw_tddft(warmup, spec.dt, spec.sm)
     torch.cuda.synchronize()
-    tot = {k: g.profile_read(k) for k in ("fft_x", "fft_y", "fft_z", "fft_xh", "hartree", "box_proj",
-                                          "box_add", "rk4", "dens", "pot", "other")}
-    g.profile_reset()
     sc = 16 if prec == "f64" else 8
     model = class_bytes(spec, 2, nb, sc)
-    dom = max(model, key=lambda k: tot[k][0])
+    # per-step time of every family, each measured alone (events around one
+    # family only, so no family's time includes another's event overhead)
+    fam = {}
+    d_f = []
+    for k in FAMILIES:
+        g.profile([k])
+        g.profile_reset()
+        d, _ = g.opaw_tddft(2, spec.dt, spec.sm)
+        torch.cuda.synchronize()
+        fam[k] = g.profile_read(k)
+        d_f.append(d)
+    g.profile(False)
+    dom = max(model, key=lambda k: fam[k][0])
     # timed region: events only around the dominant family's launches
     g.profile([dom])
+    g.profile_reset()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     d_t, _ = g.opaw_tddft(steps, spec.dt, spec.sm)
@@ -136,8 +148,8 @@ def run_gpu(spec, phi0, prec, steps, warmup, nb):
     sec = (t1 - t0) / steps
     avg = dom_ms / max(dom_n, 1) / 1e3
     achieved = model[dom] / avg / 1e9
-    breakdown = {k: round(v[0] / max(warmup, 1), 4) for k, v in tot.items()}
-    return dict(steps_per_s=1.0 / sec, ms_per_step=sec * 1e3, d_dip=np.concatenate([d_w, d_t]),
+    breakdown = {k: round(v[0] / 2.0, 4) for k, v in fam.items()}
+    return dict(steps_per_s=1.0 / sec, ms_per_step=sec * 1e3, d_dip=np.concatenate([d_w, *d_f, d_t]),
                 dominant=dom, dom_bytes=model[dom], dom_avg_s=avg, dom_launches=dom_n,
                 achieved_gbs=achieved, breakdown_ms_per_step=breakdown)
 

@@ -7,10 +7,9 @@ namespace opaw {
 
 constexpr int kProjChunk = 64;
 // union-correction blocks: at most this many points and (coefficient, value)
-// pairs per block, and the LDS budget of the staged weights
+// pairs per block
 constexpr int kUnionBlockRows = 64;
-constexpr int kUnionBlockPairs = 1024;
-constexpr int kUnionLdsWeightBytes = 48 * 1024;    // box points per projection workgroup
+constexpr int kUnionBlockPairs = 1024;    // box points per projection workgroup
 
 struct BoxTables {
   const AtomDesc* atoms;     // device
@@ -53,5 +52,10 @@ hipError_t box_weights(const BoxTables& bt, hipStream_t st, const c64* ca, int n
 // out_f(upt[k]) += corr(k, f), in place
 template <typename R>
 hipError_t union_apply(const GridGeom& g, const BoxTables& bt, hipStream_t st, cx<R>* out, int nf, const c64* corr);
+// union_corr added straight into the grid fields out (corr is scratch for the
+// fallback path); the per-field order of additions is that of union_apply
+template <typename R>
+hipError_t union_corr_apply(const GridGeom& g, const BoxTables& bt, hipStream_t st, int nf, int which,
+                            const c64* w, c64* corr, cx<R>* out);
 
 }  // namespace opaw

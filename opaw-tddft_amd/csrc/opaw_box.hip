@@ -316,18 +316,25 @@ hipError_t box_weights(const BoxTables& bt, hipStream_t st, const c64* ca, int n
 // those fields for every coefficient (ctot x FG) and the block's pairs into
 // LDS with coalesced loads, then every sum reads LDS only.  Lanes are
 // (point, field); the pair order per point is the one of k_union_corr.
-template <int FG>
+//
+// APPLY: instead of storing corr, add it into the grid fields out
+// (out_f(upt[k]) += corr(k, f)): the block's results are staged in LDS and
+// re-read point-fastest, so the read-modify-writes walk consecutive grid
+// points of one field.
+template <int FG, typename R, bool APPLY>
 __global__ __launch_bounds__(kBoxThreads) void k_union_corr_lds(const int2* __restrict__ ublk,
                                                                 const int* __restrict__ pptr,
                                                                 const int* __restrict__ pc,
                                                                 const double* __restrict__ pv,
                                                                 const c64* __restrict__ w, int nf, int ctot,
-                                                                c64* __restrict__ corr) {
+                                                                c64* __restrict__ corr, const int* __restrict__ upt,
+                                                                cx<R>* __restrict__ out, long long N) {
   extern __shared__ __align__(16) double smem[];
   c64* wl = reinterpret_cast<c64*>(smem);                 // [ctot][FG]
   double* tl = smem + 2 * (long long)ctot * FG;           // pair values
   int* cl = reinterpret_cast<int*>(tl + kUnionBlockPairs);  // pair coefficients
   int* rl = cl + kUnionBlockPairs;                        // row starts (block-relative)
+  c64* res = reinterpret_cast<c64*>(smem + 2 * (long long)ctot * FG + (kUnionBlockPairs * 12 + (kUnionBlockRows + 1) * 4 + 15) / 16 * 2);
   const int2 blk = ublk[blockIdx.x];                      // (first point, point count)
   const int f0 = blockIdx.y * FG;
   const int q0 = pptr[blk.x];
@@ -398,40 +405,82 @@ __global__ __launch_bounds__(kBoxThreads) void k_union_corr_lds(const int2* __re
       xr += t * wv.x;
       xi += t * wv.y;
     }
-    if (f < nf) corr[(long long)(blk.x + r) * nf + f] = mk(xr, xi);
+    if constexpr (APPLY)
+      res[r * FG + j] = mk(xr, xi);
+    else if (f < nf)
+      corr[(long long)(blk.x + r) * nf + f] = mk(xr, xi);
+  }
+  if constexpr (APPLY) {
+    __syncthreads();
+    for (int q = threadIdx.x; q < blk.y * FG; q += kBoxThreads) {
+      const int r = q % blk.y, jj = q / blk.y;
+      if (f0 + jj < nf) {
+        cx<R>& o = out[(long long)(f0 + jj) * N + upt[blk.x + r]];
+        const c64 v = res[r * FG + jj];
+        o = mk<R>((R)((double)o.x + v.x), (R)((double)o.y + v.y));
+      }
+    }
   }
 }
 
-// fields per workgroup of the LDS kernel for ctot coefficients (0: use the
-// global kernel)
-static int union_lds_fields(int ctot) {
+static size_t union_lds_bytes(int ctot, int fg, bool apply) {
+  size_t s = (size_t)ctot * fg * 16 + ((size_t)kUnionBlockPairs * 12 + (kUnionBlockRows + 1) * 4 + 15) / 16 * 16;
+  if (apply) s += (size_t)kUnionBlockRows * fg * 16;
+  return s;
+}
+
+// fields per workgroup of the LDS kernel for ctot coefficients, within 64 KiB
+// of LDS (0: use the global kernel)
+static int union_lds_fields(int ctot, bool apply) {
   for (int fg : {16, 8, 4})
-    if ((long long)ctot * fg * 16 <= kUnionLdsWeightBytes) return fg;
+    if (union_lds_bytes(ctot, fg, apply) <= 65536) return fg;
   return 0;
+}
+
+template <typename R, bool APPLY>
+static void launch_union_lds(const BoxTables& bt, hipStream_t st, int fg, int nf, const double* pv, const c64* w,
+                             c64* corr, cx<R>* out, long long N) {
+  const size_t lds = union_lds_bytes(bt.ctot, fg, APPLY);
+  const dim3 grid(bt.nublk, (nf + fg - 1) / fg);
+  if (fg == 16)
+    hipLaunchKernelGGL((k_union_corr_lds<16, R, APPLY>), grid, dim3(kBoxThreads), lds, st, bt.ublk, bt.pptr, bt.pc,
+                       pv, w, nf, bt.ctot, corr, bt.U.upt, out, N);
+  else if (fg == 8)
+    hipLaunchKernelGGL((k_union_corr_lds<8, R, APPLY>), grid, dim3(kBoxThreads), lds, st, bt.ublk, bt.pptr, bt.pc,
+                       pv, w, nf, bt.ctot, corr, bt.U.upt, out, N);
+  else
+    hipLaunchKernelGGL((k_union_corr_lds<4, R, APPLY>), grid, dim3(kBoxThreads), lds, st, bt.ublk, bt.pptr, bt.pc,
+                       pv, w, nf, bt.ctot, corr, bt.U.upt, out, N);
 }
 
 hipError_t union_corr(const BoxTables& bt, hipStream_t st, int nf, int which, const c64* w, c64* corr) {
   if (bt.natom == 0 || nf == 0 || bt.U.U == 0) return hipSuccess;
   prof_begin(st, KC_BOX_ADD);
   const double* pv = which ? bt.pv1 : bt.pv;
-  const int fg = bt.nublk > 0 ? union_lds_fields(bt.ctot) : 0;
+  const int fg = bt.nublk > 0 ? union_lds_fields(bt.ctot, false) : 0;
   if (fg > 0) {
-    const size_t lds = (size_t)bt.ctot * fg * 16 + (size_t)kUnionBlockPairs * 12 + (kUnionBlockRows + 1) * 4;
-    const dim3 grid(bt.nublk, (nf + fg - 1) / fg);
-    if (fg == 16)
-      hipLaunchKernelGGL(k_union_corr_lds<16>, grid, dim3(kBoxThreads), lds, st, bt.ublk, bt.pptr, bt.pc, pv, w, nf,
-                         bt.ctot, corr);
-    else if (fg == 8)
-      hipLaunchKernelGGL(k_union_corr_lds<8>, grid, dim3(kBoxThreads), lds, st, bt.ublk, bt.pptr, bt.pc, pv, w, nf,
-                         bt.ctot, corr);
-    else
-      hipLaunchKernelGGL(k_union_corr_lds<4>, grid, dim3(kBoxThreads), lds, st, bt.ublk, bt.pptr, bt.pc, pv, w, nf,
-                         bt.ctot, corr);
+    launch_union_lds<double, false>(bt, st, fg, nf, pv, w, corr, nullptr, 0);
   } else {
     const int per = kBoxThreads / 64;
     hipLaunchKernelGGL(k_union_corr, dim3((bt.U.U + per - 1) / per), dim3(kBoxThreads), 0, st, bt.pptr, bt.pc, pv,
                        bt.U.U, w, nf, corr);
   }
+  prof_end(st, KC_BOX_ADD);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t union_corr_apply(const GridGeom& g, const BoxTables& bt, hipStream_t st, int nf, int which,
+                            const c64* w, c64* corr, cx<R>* out) {
+  if (bt.natom == 0 || nf == 0 || bt.U.U == 0) return hipSuccess;
+  const int fg = bt.nublk > 0 ? union_lds_fields(bt.ctot, true) : 0;
+  if (fg == 0) {
+    hipError_t e = union_corr(bt, st, nf, which, w, corr);
+    if (e) return e;
+    return union_apply<R>(g, bt, st, out, nf, corr);
+  }
+  prof_begin(st, KC_BOX_ADD);
+  launch_union_lds<R, true>(bt, st, fg, nf, which ? bt.pv1 : bt.pv, w, corr, out, (long long)g.nx * g.ny * g.nz);
   prof_end(st, KC_BOX_ADD);
   return hipGetLastError();
 }
@@ -455,5 +504,9 @@ template hipError_t box_coeffs<float>(const GridGeom&, const BoxTables&, hipStre
                                       c64*);
 template hipError_t union_apply<double>(const GridGeom&, const BoxTables&, hipStream_t, c64*, int, const c64*);
 template hipError_t union_apply<float>(const GridGeom&, const BoxTables&, hipStream_t, c32*, int, const c64*);
+template hipError_t union_corr_apply<double>(const GridGeom&, const BoxTables&, hipStream_t, int, int, const c64*,
+                                             c64*, c64*);
+template hipError_t union_corr_apply<float>(const GridGeom&, const BoxTables&, hipStream_t, int, int, const c64*,
+                                            c64*, c32*);
 
 }  // namespace opaw

@@ -64,8 +64,10 @@ struct Prof {
       pool.pop_back();
       return e;
     }
+    // timing-only events: no system-scope fence, so recording one does not
+    // write back and invalidate the caches between the kernels it brackets
     hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
     return e;
   }
   void harvest() {
@@ -123,6 +125,11 @@ struct opaw_ctx {
   GridGeom g{};
   GridGeom gbig{};
   hipStream_t st = nullptr;
+  // side stream (non-blocking, default priority) for work that overlaps the
+  // main chain inside one entry point; forked from and joined back into st
+  // with the two events, so callers only ever see st
+  hipStream_t st2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   Prof prof;
   std::vector<void*> allocs;
   long long bytes = 0;
@@ -348,6 +355,12 @@ int opaw_create(opaw_ctx** out, int nx, int ny, int nz, double dx, double dy, do
   if (r == OPAW_OK) r = dalloc_t(c, &c->gmat, (size_t)nb * nb);
   if (r == OPAW_OK) r = dalloc_t(c, &c->tmat, (size_t)nb * nb);
   if (r == OPAW_OK) r = (hipMemset(c->phi, 0, nf * N * c->esize()) == hipSuccess) ? OPAW_OK : fail(OPAW_ERR_HIP, "memset");
+  if (r == OPAW_OK && (hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess ||
+                       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming | hipEventReleaseToDevice) !=
+                           hipSuccess ||
+                       hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming | hipEventReleaseToDevice) !=
+                           hipSuccess))
+    r = fail(OPAW_ERR_HIP, "side stream / events");
   if (r != OPAW_OK) {
     opaw_destroy(c);
     return r;
@@ -360,6 +373,9 @@ int opaw_destroy(opaw_ctx* c) {
   if (!c) return OPAW_OK;
   (void)hipDeviceSynchronize();
   c->prof.destroy();
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->st2) (void)hipStreamDestroy(c->st2);
   for (void* p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
   delete c;
@@ -1103,6 +1119,26 @@ static double dvol(opaw_ctx* c) { return c->dx * c->dy * c->dz; }
 // coefficients A (p1 of in, weights sn(-1/2), compact correction corrA) and
 // B (p of in + corrA = p of S^-1/2 in); the B weights are formed later with
 // the then-current D_ij
+// st2 starts after everything enqueued on st so far
+static int fork_side(opaw_ctx* c) {
+  HCHK(hipEventRecord(c->ev_fork, c->st));
+  HCHK(hipStreamWaitEvent(c->st2, c->ev_fork, 0));
+  return OPAW_OK;
+}
+
+// ev_join marks the end of the side work enqueued so far
+static int mark_side(opaw_ctx* c) {
+  HCHK(hipEventRecord(c->ev_join, c->st2));
+  return OPAW_OK;
+}
+
+// st continues only after the side work
+static int join_side(opaw_ctx* c) {
+  RCHK(mark_side(c));
+  HCHK(hipStreamWaitEvent(c->st, c->ev_join, 0));
+  return OPAW_OK;
+}
+
 template <typename R>
 static int flow_prep(opaw_ctx* c, const cx<R>* in, int nf) {
   RCHK(ensure_coef(c, nf));
@@ -1143,27 +1179,34 @@ static int flow_h_phi(opaw_ctx* c, int sys0, const cx<R>* in, cx<R>* out, int nf
   HCHK(union_corr(bt, c->st, nf, 0, (const c64*)c->wB.p, (c64*)c->corrNL.p));
   HCHK(apply_t_plus_v<R>(c->g, c->st, in, out, (cx<R>*)c->wtmp.p, nf, nf_per_sys, Fields<R>::ek(c),
                          Fields<R>::vks(c) + (size_t)sys0 * c->N, TW<R>::get(c, c->nx), TW<R>::get(c, c->ny),
-                         TW<R>::get(c, c->nz), c->umap, c->upt, c->uzr, nullptr, (const c64*)c->corrNL.p, nf));
+                         TW<R>::get(c, c->nz), c->umap, c->upt, c->uzr, nullptr, (const c64*)c->corrNL.p, nf,
+                         nullptr));
   return OPAW_OK;
 }
 
-// h = H S^-1/2 in from the A/B coefficients of `in` (weights B from D_ij now),
-// then the C coefficients of h (p1, weights sn(-1/2), compact corrC); the
-// outer S^-1/2 is left to the consumer (adds corrC)
+// h = S^-1/2 H S^-1/2 in from the A/B coefficients of `in` (weights B from
+// D_ij now): H S^-1/2 in, then the C coefficients of that (p1, weights
+// sn(-1/2)) and their add-back into h in place
 template <typename R>
 static int flow_hs(opaw_ctx* c, int sys0, const cx<R>* in, cx<R>* h, int nf, int nf_per_sys) {
   RCHK(ensure(c, c->wtmp, (size_t)nf * c->N * sizeof(cx<R>)));
   const BoxTables bt = box_tables(c);
-  HCHK(box_weights(bt, c->st, (const c64*)c->cB.p, nf, 2, nullptr, c->dij + (size_t)sys0 * c->mtot, nf_per_sys,
+  // the nonlocal weights D(sys) cB and their union correction run on the
+  // side stream while the forward plane and z passes (which need only
+  // corrA) run on st; the inverse plane pass waits for them
+  RCHK(fork_side(c));
+  HCHK(box_weights(bt, c->st2, (const c64*)c->cB.p, nf, 2, nullptr, c->dij + (size_t)sys0 * c->mtot, nf_per_sys,
                    c->mtot, (c64*)c->wB.p));
-  HCHK(union_corr(bt, c->st, nf, 0, (const c64*)c->wB.p, (c64*)c->corrNL.p));
+  HCHK(union_corr(bt, c->st2, nf, 0, (const c64*)c->wB.p, (c64*)c->corrNL.p));
+  RCHK(mark_side(c));
   HCHK(apply_t_plus_v<R>(c->g, c->st, in, h, (cx<R>*)c->wtmp.p, nf, nf_per_sys, Fields<R>::ek(c),
                          Fields<R>::vks(c) + (size_t)sys0 * c->N, TW<R>::get(c, c->nx), TW<R>::get(c, c->ny),
                          TW<R>::get(c, c->nz), c->umap, c->upt, c->uzr, (const c64*)c->corrA.p,
-                         (const c64*)c->corrNL.p, nf));
+                         (const c64*)c->corrNL.p, nf, c->ev_join));
   HCHK(box_coeffs<R>(c->g, bt, c->st, h, nf, 1, nullptr, dvol(c), (double*)c->ppart.p, (c64*)c->cC.p, 1, c->sn[1],
                      nullptr, nf, c->mtot, (c64*)c->wC.p));
-  HCHK(union_corr(bt, c->st, nf, 1, (const c64*)c->wC.p, (c64*)c->corrC.p));
+  // the outer S^-1/2 of shs: h += corrC on the union of boxes
+  HCHK(union_corr_apply<R>(c->g, bt, c->st, nf, 1, (const c64*)c->wC.p, (c64*)c->corrC.p, h));
   return OPAW_OK;
 }
 
@@ -1173,8 +1216,6 @@ static int flow_shs(opaw_ctx* c, int sys0, const cx<R>* in, cx<R>* out, int nf, 
   RCHK(flow_prep<R>(c, in, nf));
   c->prep_valid = false;
   RCHK(flow_hs<R>(c, sys0, in, out, nf, nf_per_sys));
-  const BoxTables bt = box_tables(c);
-  HCHK(union_apply<R>(c->g, bt, c->st, out, nf, (const c64*)c->corrC.p));
   return OPAW_OK;
 }
 
@@ -1189,19 +1230,28 @@ static int flow_rebuild(opaw_ctx* c, int s0, int ns, const cx<R>* phi) {
   const int nf = ns * c->nb;
   const long long N = c->N;
   const double dv = dvol(c);
-  RCHK(flow_prep<R>(c, phi, nf));
+  // flow_prep with the density pass (needs only corrA) on the side stream,
+  // overlapping the B projection, rho_ij and nhat on st
+  RCHK(ensure_coef(c, nf));
   const BoxTables bt = box_tables(c);
+  HCHK(box_coeffs<R>(c->g, bt, c->st, phi, nf, 1, nullptr, dv, (double*)c->ppart.p, (c64*)c->cA.p, 1, c->sn[1],
+                     nullptr, nf, c->mtot, (c64*)c->wA.p));
+  HCHK(union_corr(bt, c->st, nf, 1, (const c64*)c->wA.p, (c64*)c->corrA.p));
+  RCHK(fork_side(c));
+  prof_begin(c->st2, KC_DENS);
+  HCHK(density<R>(c->st2, phi, N, ns, c->nb, c->dens + s0 * N, c->umap, (const c64*)c->corrA.p, nf));
+  prof_end(c->st2, KC_DENS);
+  HCHK(box_coeffs<R>(c->g, bt, c->st, phi, nf, 0, (const c64*)c->corrA.p, dv, (double*)c->ppart.p, (c64*)c->cB.p, 0,
+                     nullptr, nullptr, nf, c->mtot, nullptr));
   prof_begin(c->st, KC_DENS);
   HCHK(rhoij(c->st, c->datoms, c->natom, (const c64*)c->cB.p, c->ctot, ns, c->nb, c->mtot,
              c->rhoij + (size_t)s0 * c->mtot));
   prof_end(c->st, KC_DENS);
   prof_begin(c->st, KC_DENS);
-  HCHK(density<R>(c->st, phi, N, ns, c->nb, c->dens + s0 * N, c->umap, (const c64*)c->corrA.p, nf));
-  prof_end(c->st, KC_DENS);
-  prof_begin(c->st, KC_DENS);
   HCHK(nhat(c->st, c->datoms, c->natom, c->rhoij + (size_t)s0 * c->mtot, c->mtot, c->qtab, c->nl2, sphere_tabs(c),
             c->sph_g, N, ns, c->wl, c->nhat + s0 * N));
   prof_end(c->st, KC_DENS);
+  RCHK(join_side(c));
   prof_begin(c->st, KC_DENS);
   HCHK(renorm_factor(c->st, c->nhat + s0 * N, c->dens + s0 * N, N, ns, dv, c->rnel, c->part, c->fac + s0));
   prof_end(c->st, KC_DENS);
@@ -1243,7 +1293,6 @@ static int flow_rk4(opaw_ctx* c, double dt, double* dip_out) {
   cx<R>* acc = (cx<R>*)c->wacc.p;
   cx<R>* h = (cx<R>*)c->wh.p;
   if (!c->prep_valid) RCHK(flow_prep<R>(c, p, nf));
-  const BoxTables bt = box_tables(c);
   prof_begin(c->st, KC_DENS);
   HCHK(field_norms<R>(c->st, p, c->N, nf, dv, c->part, c->nrm0));
   prof_end(c->st, KC_DENS);
@@ -1251,8 +1300,6 @@ static int flow_rk4(opaw_ctx* c, double dt, double* dip_out) {
   for (int stage = 1; stage <= 4; ++stage) {
     const cx<R>* yin = (stage == 1) ? p : y;
     RCHK(flow_hs<R>(c, 0, yin, h, nf, c->nb));
-    // the outer S^-1/2 of shs: h += corrC on the union of boxes
-    HCHK(union_apply<R>(c->g, bt, c->st, h, nf, (const c64*)c->corrC.p));
     // stage 4 also forms the post-step norms (norm1) of every orbital
     prof_begin(c->st, KC_RK4);
     HCHK(rk4_update<R>(c->st, stage, c->N, nf, h, p, acc, y, cst[stage - 1], dt, c->part, dv, c->nrm1));

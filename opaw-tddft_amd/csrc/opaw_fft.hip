@@ -144,13 +144,17 @@ __global__ __launch_bounds__(kThreads) void k_fft_x(const cx<R>* in, View vin,
 }
 
 // ---------------------------------------------------------------------------
-// y (AXIS=1) or z (AXIS=2) pass
+// y (AXIS=1) or z (AXIS=2) pass over XT x-columns per workgroup (16: more,
+// smaller workgroups measured faster than 32- or 64-wide tiles on the 64^3
+// z pass)
+constexpr int kSXT = 16;
+
 template <typename R, int N, int AXIS, int S, int EPI>
 __global__ __launch_bounds__(kThreads) void k_fft_s(const cx<R>* in, View vin,
                                                      cx<R>* out, View vout, SLines L,
                                                      const cx<R>* __restrict__ twg,
                                                      const R* __restrict__ tab, View vtab) {
-  constexpr int XT = 16;
+  constexpr int XT = kSXT;
   __shared__ cx<R> tile[XT * N];
   __shared__ cx<R> tw[N];
   fft::load_twiddles(tw, twg, N);
@@ -454,7 +458,7 @@ template <typename R>
 hipError_t fft_s_pass(int cls, int n, int axis, int dir, int epi, hipStream_t st, const cx<R>* in,
                       const View& vin, cx<R>* out, const View& vout, const SLines& L,
                       const cx<R>* tw, const R* tab, const View& vtab) {
-  const long long ntile = (long long)((L.nx + 15) / 16) * L.lo * L.nf;
+  const long long ntile = (long long)((L.nx + kSXT - 1) / kSXT) * L.lo * L.nf;
   if (ntile <= 0) return hipSuccess;
   dim3 grid((unsigned)ntile);
   hipError_t e;
@@ -482,7 +486,7 @@ hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* in, cx
                           int nf, int nf_per_sys, const R* ek, const R* vks,
                           const cx<R>* twx, const cx<R>* twy, const cx<R>* twz,
                           const int* umap, const int* upt, const int* uzr, const c64* cpre, const c64* cnl,
-                          int U) {
+                          int U, hipEvent_t before_inv) {
   const long long N = (long long)g.nx * g.ny * g.nz;
   View v{N, 0, 0, 0, g.nx, g.ny};
   XLines Lx{0, g.ny, 0, g.nz, nf, 0, g.nx, 0, g.nx};
@@ -497,6 +501,7 @@ hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* in, cx
     prof_end(st, KC_FFT_X);
     if (e) return e;
     if ((e = fft_s_pass<R>(KC_FFT_Z, g.nz, 2, -1, EPI_MULT, st, tmp, v, tmp, v, Lz, twz, ek, vt))) return e;
+    if (before_inv && (e = hipStreamWaitEvent(st, before_inv, 0))) return e;
     prof_begin(st, KC_FFT_XH);
     e = launch_plane<R, 1, EPI_HPSI>(g.nx, st, tmp, out, g.nz, nf, N, twx, in, vks, nf_per_sys, upt, uzr, cpre, cnl,
                                      U);
@@ -509,6 +514,7 @@ hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* in, cx
   if ((e = fft_s_pass<R>(KC_FFT_Y, g.ny, 1, -1, EPI_NONE, st, tmp, v, tmp, v, Ly, twy, nullptr, vt))) return e;
   if ((e = fft_s_pass<R>(KC_FFT_Z, g.nz, 2, -1, EPI_MULT, st, tmp, v, tmp, v, Lz, twz, ek, vt))) return e;
   if ((e = fft_s_pass<R>(KC_FFT_Y, g.ny, 1, 1, EPI_NONE, st, tmp, v, tmp, v, Ly, twy, nullptr, vt))) return e;
+  if (before_inv && (e = hipStreamWaitEvent(st, before_inv, 0))) return e;
   return fft_x_pass<R>(KC_FFT_XH, g.nx, 1, EPI_HPSI, st, tmp, v, out, v, Lx, twx, in, vks, nf_per_sys, umap, cpre, cnl,
                        U);
 }
@@ -546,10 +552,12 @@ hipError_t hartree_conv(const GridGeom& g, int scale, hipStream_t st, const c64*
 
 template hipError_t apply_t_plus_v<double>(const GridGeom&, hipStream_t, const c64*, c64*, c64*, int, int,
                                            const double*, const double*, const c64*, const c64*, const c64*,
-                                           const int*, const int*, const int*, const c64*, const c64*, int);
+                                           const int*, const int*, const int*, const c64*, const c64*, int,
+                                           hipEvent_t);
 template hipError_t apply_t_plus_v<float>(const GridGeom&, hipStream_t, const c32*, c32*, c32*, int, int,
                                           const float*, const float*, const c32*, const c32*, const c32*,
-                                          const int*, const int*, const int*, const c64*, const c64*, int);
+                                          const int*, const int*, const int*, const c64*, const c64*, int,
+                                          hipEvent_t);
 
 // plain 3D transform of nf fields (used by tests and the Hartree-free paths)
 template <typename R>

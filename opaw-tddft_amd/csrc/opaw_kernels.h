@@ -20,13 +20,16 @@ bool fft_size_supported(int n);
 // out_f = IFFT(ek FFT(u_f)) + vks(sys) u_f [+ cnl_f], where u_f = in_f [+ cpre_f]
 // (compact corrections over the union of boxes stored as corr[k*U + f]: U is
 // the row stride, i.e. the field count of the batch; umap maps a grid point to
-// its union index k or -1)
+// its union index k or -1, upt lists the union points in grid order and the
+// union points of plane z are [uzr[z], uzr[z+1])).  cnl is read only by the
+// last (inverse) pass, which first waits for before_inv when it is set, so
+// cnl may still be in production on another stream during the first passes.
 template <typename R>
 hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* in, cx<R>* out, cx<R>* tmp,
                           int nf, int nf_per_sys, const R* ek, const R* vks,
                           const cx<R>* twx, const cx<R>* twy, const cx<R>* twz,
                           const int* umap, const int* upt, const int* uzr, const c64* cpre, const c64* cnl,
-                          int U);
+                          int U, hipEvent_t before_inv);
 
 template <typename R>
 hipError_t fft3d(const GridGeom& g, hipStream_t st, int dir, const cx<R>* in, cx<R>* out, int nf,
