@@ -28,6 +28,10 @@ struct BoxTables {
   const int* pc;             // pair coefficient index
   const double* pv;          // pair value, projector table p
   const double* pv1;         // pair value, projector table p1
+  const int* optr;           // natom+1: overlap columns of atom a (k_bfix)
+  const int* ovoff;          // natom: offset of atom a's overlap values
+  const int* oc;             // overlap column -> coefficient index
+  const double* ov;          // overlap values O(i, l), row-major per atom
   const int2* ublk;          // union point blocks (first point, count) for the LDS kernel
   int nublk;                 // 0: no blocks (global kernel)
   UnionTabs U;               // union-of-boxes gather tables
@@ -52,6 +56,12 @@ hipError_t box_weights(const BoxTables& bt, hipStream_t st, const c64* ca, int n
 // out_f(upt[k]) += corr(k, f), in place
 template <typename R>
 hipError_t union_apply(const GridGeom& g, const BoxTables& bt, hipStream_t st, cx<R>* out, int nf, const c64* corr);
+// cA = dv P1^T u, wA = sn cA and cB = dv P^T u in one projection launch
+template <typename R>
+hipError_t box_coeffs_dual(const GridGeom& g, const BoxTables& bt, hipStream_t st, const cx<R>* u, int nf,
+                           double dv, double* part, c64* cA, const double* sn, c64* wA, c64* cB);
+// cB += O wA: projections of S^-1/2 u from those of u (fixed box overlaps)
+hipError_t box_bfix(const BoxTables& bt, hipStream_t st, int nf, const c64* wA, c64* cB);
 // union_corr added straight into the grid fields out (corr is scratch for the
 // fallback path); the per-field order of additions is that of union_apply
 template <typename R>

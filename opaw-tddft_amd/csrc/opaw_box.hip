@@ -27,10 +27,14 @@ constexpr int kPointLanes = kBoxThreads / kFieldsPerWG;   // 16
 template <typename R, int MSP, bool PRE>
 __global__ __launch_bounds__(kBoxThreads) void k_proj(const AtomDesc* __restrict__ atoms,
                                                       const int2* __restrict__ chunks, const int* __restrict__ bpt,
-                                                      const int* __restrict__ bk, const double* __restrict__ tabs,
+                                                      const int* __restrict__ bk, const double* __restrict__ tabs0,
+                                                      const double* __restrict__ tabs1,
                                                       const cx<R>* __restrict__ u, long long N, int nf,
                                                       const c64* __restrict__ cpre, int U,
-                                                      double* __restrict__ part) {
+                                                      double* __restrict__ part0, double* __restrict__ part1) {
+  // blockIdx.z selects the table set (two projections of the same u in one launch)
+  const double* __restrict__ tabs = blockIdx.z ? tabs1 : tabs0;
+  double* __restrict__ part = blockIdx.z ? part1 : part0;
   const int ch = blockIdx.x;
   const int kl = threadIdx.x % kPointLanes;
   const int f = blockIdx.y * kFieldsPerWG + threadIdx.x / kPointLanes;
@@ -124,10 +128,15 @@ __global__ __launch_bounds__(kBoxThreads) void k_proj(const AtomDesc* __restrict
 template <int MSP>
 __global__ __launch_bounds__(256) void k_proj_finish(const AtomDesc* __restrict__ atoms,
                                                      const int2* __restrict__ achunk,
-                                                     const double* __restrict__ part, int nf, int ctot, double dv,
-                                                     c64* __restrict__ ca, int mode, const double* __restrict__ sn,
+                                                     const double* __restrict__ part0, int nf, int ctot, double dv,
+                                                     c64* __restrict__ ca0, int mode0, const double* __restrict__ sn,
                                                      const double* __restrict__ dij, int nf_per_sys, int mtot,
-                                                     c64* __restrict__ w) {
+                                                     c64* __restrict__ w, const double* __restrict__ part1,
+                                                     c64* __restrict__ ca1) {
+  // blockIdx.z = 1: the second projection of a dual launch (coefficients only)
+  const double* __restrict__ part = blockIdx.z ? part1 : part0;
+  c64* __restrict__ ca = blockIdx.z ? ca1 : ca0;
+  const int mode = blockIdx.z ? 0 : mode0;
   const int a = blockIdx.x, f = blockIdx.y;
   const AtomDesc A = atoms[a];
   const int2 cr = achunk[a];                     // (first chunk, chunk count)
@@ -261,15 +270,16 @@ __global__ __launch_bounds__(kBoxThreads) void k_union_apply(const int* __restri
 // ---------------------------------------------------------------------------
 template <typename R, int MSP>
 static void launch_proj(const GridGeom& g, const BoxTables& bt, hipStream_t st, const cx<R>* u, int nf,
-                        const double* tab, const c64* cpre, double* part) {
+                        const double* tab, const c64* cpre, double* part, const double* tab1 = nullptr,
+                        double* part1 = nullptr) {
   const long long N = (long long)g.nx * g.ny * g.nz;
-  dim3 grid(bt.nchunks, (nf + kFieldsPerWG - 1) / kFieldsPerWG);
+  dim3 grid(bt.nchunks, (nf + kFieldsPerWG - 1) / kFieldsPerWG, tab1 ? 2 : 1);
   if (cpre)
     hipLaunchKernelGGL((k_proj<R, MSP, true>), grid, dim3(kBoxThreads), 0, st, bt.atoms, bt.chunks, bt.bpt, bt.bk,
-                       tab, u, N, nf, cpre, bt.U.U, part);
+                       tab, tab1, u, N, nf, cpre, bt.U.U, part, part1);
   else
     hipLaunchKernelGGL((k_proj<R, MSP, false>), grid, dim3(kBoxThreads), 0, st, bt.atoms, bt.chunks, bt.bpt, bt.bk,
-                       tab, u, N, nf, cpre, bt.U.U, part);
+                       tab, tab1, u, N, nf, cpre, bt.U.U, part, part1);
 }
 
 template <typename R>
@@ -290,10 +300,102 @@ hipError_t box_coeffs(const GridGeom& g, const BoxTables& bt, hipStream_t st, co
   const dim3 fg(bt.natom, nf);
   if (bt.msp == 8)
     hipLaunchKernelGGL(k_proj_finish<8>, fg, dim3(256), 0, st, bt.atoms, bt.achunk, part, nf, bt.ctot, dv, ca, mode,
-                       sn, dij, nf_per_sys, mtot, w);
+                       sn, dij, nf_per_sys, mtot, w, nullptr, nullptr);
   else
     hipLaunchKernelGGL(k_proj_finish<16>, fg, dim3(256), 0, st, bt.atoms, bt.achunk, part, nf, bt.ctot, dv, ca, mode,
-                       sn, dij, nf_per_sys, mtot, w);
+                       sn, dij, nf_per_sys, mtot, w, nullptr, nullptr);
+  prof_end(st, KC_BOX_ADD);
+  return hipGetLastError();
+}
+
+// Dual projection of one set of fields: cA = dv P1^T u with weights
+// wA = sn cA, and cB = dv P^T u, in one projection launch and one finish
+// launch (blockIdx.z selects the table set).  part must hold two partial sets.
+template <typename R>
+hipError_t box_coeffs_dual(const GridGeom& g, const BoxTables& bt, hipStream_t st, const cx<R>* u, int nf,
+                           double dv, double* part, c64* cA, const double* sn, c64* wA, c64* cB) {
+  if (bt.natom == 0 || nf == 0) return hipSuccess;
+  double* part1 = part + (size_t)bt.nchunks * nf * bt.msp * 2;
+  prof_begin(st, KC_BOX_PROJ);
+  if (bt.msp == 8)
+    launch_proj<R, 8>(g, bt, st, u, nf, bt.p1, nullptr, part, bt.p, part1);
+  else
+    launch_proj<R, 16>(g, bt, st, u, nf, bt.p1, nullptr, part, bt.p, part1);
+  prof_end(st, KC_BOX_PROJ);
+  hipError_t e = hipGetLastError();
+  if (e) return e;
+  prof_begin(st, KC_BOX_ADD);
+  const dim3 fg(bt.natom, nf, 2);
+  if (bt.msp == 8)
+    hipLaunchKernelGGL(k_proj_finish<8>, fg, dim3(256), 0, st, bt.atoms, bt.achunk, part, nf, bt.ctot, dv, cA, 1, sn,
+                       nullptr, nf, 0, wA, part1, cB);
+  else
+    hipLaunchKernelGGL(k_proj_finish<16>, fg, dim3(256), 0, st, bt.atoms, bt.achunk, part, nf, bt.ctot, dv, cA, 1, sn,
+                       nullptr, nf, 0, wA, part1, cB);
+  prof_end(st, KC_BOX_ADD);
+  return hipGetLastError();
+}
+
+// cB += O wA for every (atom a, field f): the projections of S^-1/2 u from
+// those of u, since P_a^T S^-1/2 u dv = P_a^T u dv + sum_b O_ab wA_b with the
+// fixed overlaps O_ab(i, j) = dv sum_r p_ai(r) p1_bj(r) over the common points
+// of the two boxes (host-built, FP64, grid order).  Per atom the nonzero
+// columns are flattened: column l is coefficient oc[optr[a] + l] and
+// O(i, l) = ov[ovoff[a] + i * L_a + l].  One wave per (atom, field): lanes
+// over columns, a fixed shuffle tree over the lanes (deterministic).
+template <int MSP>
+__global__ __launch_bounds__(64) void k_bfix(const AtomDesc* __restrict__ atoms, const int* __restrict__ optr,
+                                             const int* __restrict__ ovoff, const int* __restrict__ oc,
+                                             const double* __restrict__ ov, const c64* __restrict__ wA, int nf,
+                                             int ctot, c64* __restrict__ cB) {
+  const int a = blockIdx.x, f = blockIdx.y, lane = threadIdx.x;
+  const AtomDesc A = atoms[a];
+  const int l0 = optr[a], La = optr[a + 1] - l0;
+  const double* O = ov + ovoff[a];
+  double xr[MSP], xi[MSP];
+#pragma unroll
+  for (int i = 0; i < MSP; ++i) xr[i] = xi[i] = 0.0;
+  for (int lb = 0; lb < La; lb += 64) {
+    const int l = min(lb + lane, La - 1);
+    const bool on = lb + lane < La;
+    const int c = oc[l0 + l];
+    double o[MSP];
+#pragma unroll
+    for (int i = 0; i < MSP; ++i) o[i] = O[(long long)(i < A.ms ? i : 0) * La + l];
+    const c64 w = wA[(long long)c * nf + f];
+    const double wr = on ? w.x : 0.0, wi = on ? w.y : 0.0;
+#pragma unroll
+    for (int i = 0; i < MSP; ++i) {
+      xr[i] += o[i] * wr;
+      xi[i] += o[i] * wi;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < MSP; ++i) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      xr[i] += __shfl_xor(xr[i], off, 64);
+      xi[i] += __shfl_xor(xi[i], off, 64);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < MSP; ++i) {
+    if (lane == i && i < A.ms) {
+      c64& b = cB[(long long)f * ctot + A.coff + i];
+      b = mk(b.x + xr[i], b.y + xi[i]);
+    }
+  }
+}
+
+hipError_t box_bfix(const BoxTables& bt, hipStream_t st, int nf, const c64* wA, c64* cB) {
+  if (bt.natom == 0 || nf == 0) return hipSuccess;
+  prof_begin(st, KC_BOX_ADD);
+  if (bt.msp == 8)
+    hipLaunchKernelGGL(k_bfix<8>, dim3(bt.natom, nf), dim3(64), 0, st, bt.atoms, bt.optr, bt.ovoff, bt.oc, bt.ov, wA,
+                       nf, bt.ctot, cB);
+  else
+    hipLaunchKernelGGL(k_bfix<16>, dim3(bt.natom, nf), dim3(64), 0, st, bt.atoms, bt.optr, bt.ovoff, bt.oc, bt.ov, wA,
+                       nf, bt.ctot, cB);
   prof_end(st, KC_BOX_ADD);
   return hipGetLastError();
 }
@@ -504,6 +606,10 @@ template hipError_t box_coeffs<float>(const GridGeom&, const BoxTables&, hipStre
                                       c64*);
 template hipError_t union_apply<double>(const GridGeom&, const BoxTables&, hipStream_t, c64*, int, const c64*);
 template hipError_t union_apply<float>(const GridGeom&, const BoxTables&, hipStream_t, c32*, int, const c64*);
+template hipError_t box_coeffs_dual<double>(const GridGeom&, const BoxTables&, hipStream_t, const c64*, int, double,
+                                            double*, c64*, const double*, c64*, c64*);
+template hipError_t box_coeffs_dual<float>(const GridGeom&, const BoxTables&, hipStream_t, const c32*, int, double,
+                                           double*, c64*, const double*, c64*, c64*);
 template hipError_t union_corr_apply<double>(const GridGeom&, const BoxTables&, hipStream_t, int, int, const c64*,
                                              c64*, c64*);
 template hipError_t union_corr_apply<float>(const GridGeom&, const BoxTables&, hipStream_t, int, int, const c64*,

@@ -160,6 +160,8 @@ struct opaw_ctx {
   int* uzr = nullptr;        // nz+1: union points of plane z are [uzr[z], uzr[z+1])
   int2* ublk = nullptr;
   int nublk = 0;
+  int *optr = nullptr, *ovoff = nullptr, *oc = nullptr;   // box overlap tables (k_bfix)
+  double* ov = nullptr;
   double *pv = nullptr, *pv1 = nullptr;
   UEntry* uent = nullptr;
   double *utp = nullptr, *utp1 = nullptr;
@@ -644,7 +646,7 @@ int opaw_finalize_tables(opaw_ctx* c) {
                   (void*)c->uent, (void*)c->utp, (void*)c->utp1, (void*)c->smap, (void*)c->sptr, (void*)c->sphn,
                   (void*)c->sent, (void*)c->wl, (void*)c->bpt, (void*)c->bk,
                   (void*)c->pptr, (void*)c->pc, (void*)c->pv, (void*)c->pv1, (void*)c->ublk,
-                  (void*)c->uzr})
+                  (void*)c->uzr, (void*)c->optr, (void*)c->ovoff, (void*)c->oc, (void*)c->ov})
     dfree(c, p);
   c->prep_valid = false;
   c->coef_cap = 0;   // projection buffers depend on msp / chunk count
@@ -773,8 +775,22 @@ int opaw_finalize_tables(opaw_ctx* c) {
     std::vector<int> uptr(1, 0), upt2;
     std::vector<UEntry> ue;
     std::vector<double> tp, tp1;
+    // box overlaps O_ab(i, j) = sum over common points of p_ai p1_bj (times
+    // dv below), accumulated in grid order
+    std::vector<std::vector<double>> Oab((size_t)natom * natom);
     for (int k : order) {
       const int pt = upt[k];
+      for (auto& ea : at[k])
+        for (auto& eb : at[k]) {
+          const AtomDesc& Aa = c->hatoms[ea.first];
+          const AtomDesc& Ab = c->hatoms[eb.first];
+          std::vector<double>& O = Oab[(size_t)ea.first * natom + eb.first];
+          if (O.empty()) O.assign((size_t)Aa.ms * Ab.ms, 0.0);
+          for (int i = 0; i < Aa.ms; ++i) {
+            const double pa = c->hp[ea.first][(size_t)i * Aa.npt + ea.second];
+            for (int j = 0; j < Ab.ms; ++j) O[(size_t)i * Ab.ms + j] += pa * c->hp1[eb.first][(size_t)j * Ab.npt + eb.second];
+          }
+        }
       umap[pt] = (int)upt2.size();
       upt2.push_back(pt);
       for (auto& ap : at[k]) {   // atoms were appended in atom order
@@ -788,6 +804,31 @@ int opaw_finalize_tables(opaw_ctx* c) {
       uptr.push_back((int)ue.size());
     }
     c->U = (int)upt2.size();
+    {
+      const double dv = c->dx * c->dy * c->dz;
+      std::vector<int> optr(1, 0), ovoff, oc;
+      std::vector<double> ov;
+      for (int a = 0; a < natom; ++a) {
+        const AtomDesc& Aa = c->hatoms[a];
+        std::vector<std::pair<int, int>> cols;   // (b, j)
+        for (int b2 = 0; b2 < natom; ++b2)
+          if (!Oab[(size_t)a * natom + b2].empty())
+            for (int j = 0; j < c->hatoms[b2].ms; ++j) cols.push_back({b2, j});
+        const int La = (int)cols.size();
+        ovoff.push_back((int)ov.size());
+        for (auto& cj : cols) oc.push_back(c->hatoms[cj.first].coff + cj.second);
+        optr.push_back((int)oc.size());
+        for (int i = 0; i < Aa.ms; ++i)
+          for (int l = 0; l < La; ++l) {
+            const int b2 = cols[l].first, j = cols[l].second;
+            ov.push_back(dv * Oab[(size_t)a * natom + b2][(size_t)i * c->hatoms[b2].ms + j]);
+          }
+      }
+      RCHK(upload(c, &c->optr, optr));
+      RCHK(upload(c, &c->ovoff, ovoff));
+      RCHK(upload(c, &c->oc, oc));
+      RCHK(upload(c, &c->ov, ov));
+    }
     // the same entries flattened to (coefficient, value) pairs per union point
     {
       std::vector<int> pptr(1, 0), pc;
@@ -1062,6 +1103,10 @@ static BoxTables box_tables(opaw_ctx* c) {
   b.pv1 = c->pv1;
   b.ublk = c->ublk;
   b.nublk = c->nublk;
+  b.optr = c->optr;
+  b.ovoff = c->ovoff;
+  b.oc = c->oc;
+  b.ov = c->ov;
   b.U = UnionTabs{c->umap, c->uptr, c->uent, c->upt, c->utp, c->utp1, c->U};
   return b;
 }
@@ -1094,7 +1139,7 @@ static int ensure_coef(opaw_ctx* c, int nf) {
   if (nf <= c->coef_cap) return OPAW_OK;
   const size_t cb = (size_t)nf * std::max(1, c->ctot) * sizeof(c64);
   for (DevBuf* b : {&c->cA, &c->wA, &c->cB, &c->wB, &c->cC, &c->wC}) RCHK(ensure(c, *b, cb));
-  RCHK(ensure(c, c->ppart, (size_t)std::max(1, c->nchunks) * nf * c->msp * 2 * sizeof(double)));
+  RCHK(ensure(c, c->ppart, (size_t)2 * std::max(1, c->nchunks) * nf * c->msp * 2 * sizeof(double)));
   const size_t ub = (size_t)nf * std::max(1, c->U) * sizeof(c64);
   for (DevBuf* b : {&c->corrA, &c->corrNL, &c->corrC}) RCHK(ensure(c, *b, ub));
   c->coef_cap = nf;
@@ -1139,15 +1184,23 @@ static int join_side(opaw_ctx* c) {
   return OPAW_OK;
 }
 
+// Coefficients of `in` for shs and the rebuild: A = P1^T in (weights
+// wA = sn(-1/2) A, compact correction corrA = S^-1/2 in - in) and
+// B = P^T S^-1/2 in = P^T in + O wA.  With side != 0 the corrA pass runs on
+// the side stream (the caller joins it).
 template <typename R>
-static int flow_prep(opaw_ctx* c, const cx<R>* in, int nf) {
+static int flow_prep(opaw_ctx* c, const cx<R>* in, int nf, bool side = false) {
   RCHK(ensure_coef(c, nf));
   const BoxTables bt = box_tables(c);
-  HCHK(box_coeffs<R>(c->g, bt, c->st, in, nf, 1, nullptr, dvol(c), (double*)c->ppart.p, (c64*)c->cA.p, 1, c->sn[1],
-                     nullptr, nf, c->mtot, (c64*)c->wA.p));
-  HCHK(union_corr(bt, c->st, nf, 1, (const c64*)c->wA.p, (c64*)c->corrA.p));
-  HCHK(box_coeffs<R>(c->g, bt, c->st, in, nf, 0, (const c64*)c->corrA.p, dvol(c), (double*)c->ppart.p,
-                     (c64*)c->cB.p, 0, nullptr, nullptr, nf, c->mtot, nullptr));
+  HCHK(box_coeffs_dual<R>(c->g, bt, c->st, in, nf, dvol(c), (double*)c->ppart.p, (c64*)c->cA.p, c->sn[1],
+                          (c64*)c->wA.p, (c64*)c->cB.p));
+  if (side) {
+    RCHK(fork_side(c));
+    HCHK(union_corr(bt, c->st2, nf, 1, (const c64*)c->wA.p, (c64*)c->corrA.p));
+  } else {
+    HCHK(union_corr(bt, c->st, nf, 1, (const c64*)c->wA.p, (c64*)c->corrA.p));
+  }
+  HCHK(box_bfix(bt, c->st, nf, (const c64*)c->wA.p, (c64*)c->cB.p));
   return OPAW_OK;
 }
 
@@ -1230,19 +1283,12 @@ static int flow_rebuild(opaw_ctx* c, int s0, int ns, const cx<R>* phi) {
   const int nf = ns * c->nb;
   const long long N = c->N;
   const double dv = dvol(c);
-  // flow_prep with the density pass (needs only corrA) on the side stream,
-  // overlapping the B projection, rho_ij and nhat on st
-  RCHK(ensure_coef(c, nf));
-  const BoxTables bt = box_tables(c);
-  HCHK(box_coeffs<R>(c->g, bt, c->st, phi, nf, 1, nullptr, dv, (double*)c->ppart.p, (c64*)c->cA.p, 1, c->sn[1],
-                     nullptr, nf, c->mtot, (c64*)c->wA.p));
-  HCHK(union_corr(bt, c->st, nf, 1, (const c64*)c->wA.p, (c64*)c->corrA.p));
-  RCHK(fork_side(c));
+  // the coefficients with corrA and then the density pass (which needs
+  // corrA) on the side stream, overlapping B, rho_ij and nhat on st
+  RCHK(flow_prep<R>(c, phi, nf, true));
   prof_begin(c->st2, KC_DENS);
   HCHK(density<R>(c->st2, phi, N, ns, c->nb, c->dens + s0 * N, c->umap, (const c64*)c->corrA.p, nf));
   prof_end(c->st2, KC_DENS);
-  HCHK(box_coeffs<R>(c->g, bt, c->st, phi, nf, 0, (const c64*)c->corrA.p, dv, (double*)c->ppart.p, (c64*)c->cB.p, 0,
-                     nullptr, nullptr, nf, c->mtot, nullptr));
   prof_begin(c->st, KC_DENS);
   HCHK(rhoij(c->st, c->datoms, c->natom, (const c64*)c->cB.p, c->ctot, ns, c->nb, c->mtot,
              c->rhoij + (size_t)s0 * c->mtot));
