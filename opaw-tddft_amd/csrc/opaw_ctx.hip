@@ -162,6 +162,11 @@ struct opaw_ctx {
   int nublk = 0;
   int *optr = nullptr, *ovoff = nullptr, *oc = nullptr;   // box overlap tables (k_bfix)
   double* ov = nullptr;
+  // rebuild: nhat kept on the sphere union (nhs, nsys x S, before the rnel
+  // factor); the full nhat field of system s is stale while nhat_lazy[s]
+  double* nhs = nullptr;
+  double* rpart = nullptr;   // density / nhat partial sums of the rebuild
+  std::vector<char> nhat_lazy;
   double *pv = nullptr, *pv1 = nullptr;
   UEntry* uent = nullptr;
   double *utp = nullptr, *utp1 = nullptr;
@@ -646,7 +651,8 @@ int opaw_finalize_tables(opaw_ctx* c) {
                   (void*)c->uent, (void*)c->utp, (void*)c->utp1, (void*)c->smap, (void*)c->sptr, (void*)c->sphn,
                   (void*)c->sent, (void*)c->wl, (void*)c->bpt, (void*)c->bk,
                   (void*)c->pptr, (void*)c->pc, (void*)c->pv, (void*)c->pv1, (void*)c->ublk,
-                  (void*)c->uzr, (void*)c->optr, (void*)c->ovoff, (void*)c->oc, (void*)c->ov})
+                  (void*)c->uzr, (void*)c->optr, (void*)c->ovoff, (void*)c->oc, (void*)c->ov,
+                  (void*)c->nhs, (void*)c->rpart})
     dfree(c, p);
   c->prep_valid = false;
   c->coef_cap = 0;   // projection buffers depend on msp / chunk count
@@ -924,6 +930,11 @@ int opaw_finalize_tables(opaw_ctx* c) {
       sptr.push_back((int)se.size());
     }
     c->S = (int)spt.size();
+    RCHK(dalloc_t(c, &c->nhs, (size_t)c->nsys * std::max(1, c->S)));
+    // rebuild partial sums: density (one per 256 points) and nhat (one per
+    // 64 sphere points), per system
+    RCHK(dalloc_t(c, &c->rpart, (size_t)c->nsys * ((c->N + 255) / 256 + (c->S + 63) / 64 + 1)));
+    c->nhat_lazy.assign(c->nsys, 0);
     RCHK(upload(c, &c->smap, smap));
     RCHK(upload(c, &c->sptr, sptr));
     RCHK(upload(c, &c->sent, se));
@@ -980,6 +991,17 @@ static int check_sys(opaw_ctx* c, int sys) {
   return OPAW_OK;
 }
 
+// materialise the full nhat field of system sys from the rebuild's sparse
+// sphere values (the rebuild itself never needs it)
+static int nhat_full(opaw_ctx* c, int sys) {
+  if (sys < (int)c->nhat_lazy.size() && c->nhat_lazy[sys]) {
+    HCHK(nhat_expand(c->st, c->smap, c->nhs + (size_t)sys * std::max(1, c->S), c->fac + sys, c->N,
+                     c->nhat + (size_t)sys * c->N));
+    c->nhat_lazy[sys] = 0;
+  }
+  return OPAW_OK;
+}
+
 int opaw_write_vks(opaw_ctx* c, int sys, const double* vks) {
   CTX_CHECK(c);
   RCHK(check_sys(c, sys));
@@ -998,6 +1020,7 @@ int opaw_read_field(opaw_ctx* c, int which, int sys, double* out) {
   CTX_CHECK(c);
   RCHK(check_sys(c, sys));
   if (!out) return fail(OPAW_ERR_ARG, "null output");
+  if (which == OPAW_FIELD_NHAT) RCHK(nhat_full(c, sys));
   const double* src = nullptr;
   switch (which) {
     case OPAW_FIELD_DENS: src = c->dens; break;
@@ -1286,24 +1309,26 @@ static int flow_rebuild(opaw_ctx* c, int s0, int ns, const cx<R>* phi) {
   // the coefficients with corrA and then the density pass (which needs
   // corrA) on the side stream, overlapping B, rho_ij and nhat on st
   RCHK(flow_prep<R>(c, phi, nf, true));
+  const int dstride = (int)((N + 255) / 256);
+  double* dpart = c->rpart;
+  double* npart = c->rpart + (size_t)c->nsys * dstride;
+  int nd = 0, nn = 0;
   prof_begin(c->st2, KC_DENS);
-  HCHK(density<R>(c->st2, phi, N, ns, c->nb, c->dens + s0 * N, c->umap, (const c64*)c->corrA.p, nf));
+  HCHK(density_sum<R>(c->st2, phi, N, ns, c->nb, c->dens + s0 * N, c->umap, (const c64*)c->corrA.p, nf, dpart,
+                      &nd));
   prof_end(c->st2, KC_DENS);
   prof_begin(c->st, KC_DENS);
-  HCHK(rhoij(c->st, c->datoms, c->natom, (const c64*)c->cB.p, c->ctot, ns, c->nb, c->mtot,
-             c->rhoij + (size_t)s0 * c->mtot));
-  prof_end(c->st, KC_DENS);
-  prof_begin(c->st, KC_DENS);
-  HCHK(nhat(c->st, c->datoms, c->natom, c->rhoij + (size_t)s0 * c->mtot, c->mtot, c->qtab, c->nl2, sphere_tabs(c),
-            c->sph_g, N, ns, c->wl, c->nhat + s0 * N));
+  HCHK(rhoij_nhatw(c->st, c->datoms, c->natom, (const c64*)c->cB.p, c->ctot, ns, c->nb, c->mtot,
+                   c->rhoij + (size_t)s0 * c->mtot, c->qtab, c->nl2, c->wl));
+  HCHK(nhat_sparse(c->st, sphere_tabs(c), c->S, c->sph_g, c->wl, c->natom, c->nl2, ns,
+                   c->nhs + (size_t)s0 * std::max(1, c->S), npart, &nn));
   prof_end(c->st, KC_DENS);
   RCHK(join_side(c));
-  prof_begin(c->st, KC_DENS);
-  HCHK(renorm_factor(c->st, c->nhat + s0 * N, c->dens + s0 * N, N, ns, dv, c->rnel, c->part, c->fac + s0));
-  prof_end(c->st, KC_DENS);
   prof_begin(c->st, KC_POT);
-  HCHK(scale_pack(c->st, c->dens + s0 * N, c->nhat + s0 * N, c->fac + s0, N, ns, c->pair));
+  HCHK(renorm_pack(c->st, c->dens + s0 * N, c->smap, c->nhs + (size_t)s0 * std::max(1, c->S), std::max(1, c->S),
+                   dpart, nd, npart, nn, dv, c->rnel, 1, N, ns, c->pair, c->fac + s0));
   prof_end(c->st, KC_POT);
+  for (int s = s0; s < s0 + ns; ++s) c->nhat_lazy[s] = 1;
   const int npair = (ns + 1) / 2;
   for (int q = 0; q < npair; ++q)
     HCHK(hartree_conv(c->g, c->scale, c->st, c->pair + q * N, c->pair + q * N, c->hwork, c->vk,
@@ -1589,6 +1614,7 @@ int opaw_get_nhat(opaw_ctx* c, int sys) {
   const long long N = c->N;
   HCHK(nhat(c->st, c->datoms, c->natom, c->rhoij + (size_t)sys * c->mtot, c->mtot, c->qtab, c->nl2, sphere_tabs(c),
             c->sph_g, N, 1, c->wl, c->nhat + sys * N));
+  if (sys < (int)c->nhat_lazy.size()) c->nhat_lazy[sys] = 0;
   HCHK(renorm_factor(c->st, c->nhat + sys * N, c->dens + sys * N, N, 1, dvol(c), c->rnel, c->part, c->fac + sys));
   HCHK(scale_pack(c->st, c->dens + sys * N, c->nhat + sys * N, c->fac + sys, N, 1, c->pair));
   return OPAW_OK;
@@ -1599,6 +1625,7 @@ int opaw_get_pot_tddft_2(opaw_ctx* c, int sys) {
   RCHK(check_sys(c, sys));
   if (!c->pot_ready || !c->vk_ready) return fail(OPAW_ERR_STATE, "potentials / Hartree kernel not set");
   const long long N = c->N;
+  RCHK(nhat_full(c, sys));
   HCHK(scale_pack(c->st, c->dens + sys * N, c->nhat + sys * N, nullptr, N, 1, c->pair));
   HCHK(hartree_conv(c->g, c->scale, c->st, c->pair, c->pair, c->hwork, c->vk, c->tw64[c->gbig.nx],
                     c->tw64[c->gbig.ny], c->tw64[c->gbig.nz]));
@@ -1698,7 +1725,10 @@ int opaw_tddft_run(opaw_ctx* c, int nt, double dt, double sm, double* d_dip, dou
     c->dipcap = nt;
   }
   for (int it = 0; it < nt; ++it) {
-    RCHK(opaw_make_hamiltonian_tddft(c, -1));
+    if (c->prec == OPAW_PREC_F64)
+      RCHK(flow_rebuild<double>(c, 0, c->nsys, (const c64*)c->phi));
+    else
+      RCHK(flow_rebuild<float>(c, 0, c->nsys, (const c32*)c->phi));
     // rk4_prop_shs with the dipole of the step (plot_dip) fused into the
     // norm restore
     double* slot = c->dipbuf + (size_t)3 * c->nsys * it;

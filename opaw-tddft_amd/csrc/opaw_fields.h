@@ -35,6 +35,23 @@ hipError_t scale_dipole(hipStream_t st, cx<R>* u, int nx, int ny, int nz, int ns
 template <typename R>
 hipError_t density(hipStream_t st, const cx<R>* u, long long N, int nsys, int nb, double* dens, const int* umap,
                    const c64* cpre, int U);
+// density with per-block partial sums (nd partials per system)
+template <typename R>
+hipError_t density_sum(hipStream_t st, const cx<R>* u, long long N, int nsys, int nb, double* dens,
+                       const int* umap, const c64* cpre, int U, double* dpart, int* nd);
+// rho_ij and the compensation weights wl in one launch
+hipError_t rhoij_nhatw(hipStream_t st, const AtomDesc* atoms, int natom, const c64* ca, int ctot, int nsys, int nb,
+                       int mtot, double* rho, const double* qtab, int nl2, double* wl);
+// nhat on the sphere union (S points per system) with nn partial sums per system
+hipError_t nhat_sparse(hipStream_t st, const SphereTabs& T, int S, const double* sph_g, const double* wl, int natom,
+                       int nl2, int nsys, double* nhs, double* npart, int* nn);
+// rnel factor from the partials, dens *= fac, Hartree pair of fac (dens + nhat)
+hipError_t renorm_pack(hipStream_t st, double* dens, const int* smap, const double* nhs, int S, const double* dpart,
+                       int nd, const double* npart, int nn, double dv, double rnel, int renorm, long long N, int nsys,
+                       c64* pair, double* fac);
+// full nhat field of one system from the sparse values
+hipError_t nhat_expand(hipStream_t st, const int* smap, const double* nhs, const double* fac, long long N,
+                       double* nhat);
 hipError_t fill(hipStream_t st, double* p, long long n, double v);
 hipError_t nhat(hipStream_t st, const AtomDesc* atoms, int natom, const double* rhoij, int mtot,
                 const double* qtab, int nl2, const SphereTabs& S, const double* sph_g, long long N, int nsys,

@@ -692,6 +692,201 @@ __global__ __launch_bounds__(kT) void k_fill(double* __restrict__ p, long long n
 }
 
 // ---------------------------------------------------------------------------
+// Fused rebuild kernels (one system group of the per-stage rebuild).
+//
+// density with per-block partial sums: block (c, s) covers points
+// [c * kDensBlock, (c+1) * kDensBlock) of system s and writes
+// dpart[s * gridDim.x + c] = sum of its dens values
+constexpr int kDensBlock = kT;
+
+template <typename R>
+__global__ __launch_bounds__(kT) void k_density_sum(const cx<R>* __restrict__ u, int N, int nb,
+                                                     double* __restrict__ dens, const int* __restrict__ umap,
+                                                     const c64* __restrict__ cpre, int U,
+                                                     double* __restrict__ dpart) {
+  const int s = blockIdx.y;
+  double tot[1] = {0.0};
+  {
+    const int r = blockIdx.x * kDensBlock + threadIdx.x;
+    const int rc = min(r, N - 1);
+    const cx<R>* us = u + (long long)s * nb * N + rc;
+    const int k = cpre ? umap[rc] : -1;
+    double acc = 0.0;
+    for (int o0 = 0; o0 < nb; o0 += kOrbBatch) {
+      cx<R> v[kOrbBatch];
+#pragma unroll
+      for (int q = 0; q < kOrbBatch; ++q) v[q] = us[(long long)min(o0 + q, nb - 1) * N];
+      double vr[kOrbBatch], vi[kOrbBatch];
+#pragma unroll
+      for (int q = 0; q < kOrbBatch; ++q) {
+        vr[q] = (double)v[q].x;
+        vi[q] = (double)v[q].y;
+      }
+      if (k >= 0) {
+        c64 c[kOrbBatch];
+#pragma unroll
+        for (int q = 0; q < kOrbBatch; ++q) c[q] = cpre[(long long)k * U + s * nb + min(o0 + q, nb - 1)];
+#pragma unroll
+        for (int q = 0; q < kOrbBatch; ++q) {
+          vr[q] += c[q].x;
+          vi[q] += c[q].y;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kOrbBatch; ++q)
+        if (o0 + q < nb) acc += vr[q] * vr[q] + vi[q] * vi[q];
+    }
+    if (r < N) {
+      dens[(long long)s * N + r] = 2.0 * acc;
+      tot[0] = 2.0 * acc;
+    }
+  }
+  block_sum<1>(tot);
+  if (threadIdx.x == 0) dpart[(long long)s * gridDim.x + blockIdx.x] = tot[0];
+}
+
+// rho_ij (src/tddft_rhoij.f90:75-115) and the compensation weights
+// wl(L) = sum_ij rho_ij qijlm(L,i,j) (src/nhat.f90:13-28) of one (atom,
+// system) in one workgroup
+__global__ __launch_bounds__(64) void k_rhoij_nhatw(const AtomDesc* __restrict__ atoms, const c64* __restrict__ ca,
+                                                    int ctot, int nb, int mtot, double* __restrict__ rhoij,
+                                                    const double* __restrict__ qtab, int nl2, int natom,
+                                                    double* __restrict__ wl) {
+  const int a = blockIdx.x, s = blockIdx.y;
+  const AtomDesc A = atoms[a];
+  const int m2 = A.ms * A.ms, nq = nl2 * m2;
+  __shared__ double rho[kMaxMs * kMaxMs];
+  __shared__ double qq[kMaxNl2 * kMaxMs * kMaxMs];
+  // qijlm of the type into LDS (batched loads, clamped indices)
+  for (int t0 = 0; t0 < nq; t0 += 64 * 4) {
+    double t[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) t[u] = qtab[A.toff_q + min(t0 + u * 64 + (int)threadIdx.x, nq - 1)];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (t0 + u * 64 + (int)threadIdx.x < nq) qq[t0 + u * 64 + threadIdx.x] = t[u];
+  }
+  for (int q = threadIdx.x; q < m2; q += 64) {
+    const int i = q % A.ms, j = q / A.ms;
+    double acc = 0.0;
+    for (int o0 = 0; o0 < nb; o0 += kOrbBatch) {
+      c64 ci[kOrbBatch], cj[kOrbBatch];
+#pragma unroll
+      for (int u = 0; u < kOrbBatch; ++u) {
+        const c64* c = ca + (long long)(s * nb + min(o0 + u, nb - 1)) * ctot + A.coff;
+        ci[u] = c[i];
+        cj[u] = c[j];
+      }
+#pragma unroll
+      for (int u = 0; u < kOrbBatch; ++u)
+        if (o0 + u < nb) acc += 2.0 * (ci[u].x * cj[u].x + ci[u].y * cj[u].y);
+    }
+    rho[q] = acc;
+    rhoij[(long long)s * mtot + A.moff + q] = acc;
+  }
+  __syncthreads();
+  for (int L = threadIdx.x; L < nl2; L += 64) {
+    double w = 0.0;
+    for (int q = 0; q < m2; ++q) w += rho[q] * qq[L + nl2 * q];
+    wl[((long long)s * natom + a) * nl2 + L] = w;
+  }
+}
+
+// nhat on the union of the compensation spheres only (it is zero elsewhere):
+// nhs[s * S + k] = sum over the atoms of point k of sum_L g wl, with per-block
+// partial sums npart[s * gridDim.x + blk]
+template <int NLM>
+__global__ __launch_bounds__(64) void k_nhat_sparse(SphereTabs T, int S, const double* __restrict__ sph_g,
+                                                    const double* __restrict__ wl, int natom, int nl2,
+                                                    double* __restrict__ nhs, double* __restrict__ npart) {
+  const int s = blockIdx.y;
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  const int kc = min(k, S - 1);
+  double v = 0.0;
+  const int e0 = T.sptr[kc], e1 = T.sptr[kc + 1];
+  for (int e = e0; e < e1; ++e) {
+    const int2 en = T.se[e];
+    const int stride = T.sn[en.x];
+    const double* w = wl + ((long long)s * natom + en.x) * nl2;
+    double g[NLM], ww[NLM];
+#pragma unroll
+    for (int L = 0; L < NLM; ++L) {
+      const int l = min(L, nl2 - 1);
+      g[L] = sph_g[en.y + (long long)l * stride];
+      ww[L] = w[l];
+    }
+#pragma unroll
+    for (int L = 0; L < NLM; ++L)
+      if (L < nl2) v += g[L] * ww[L];
+  }
+  if (k < S) nhs[(long long)s * S + k] = v;
+  else v = 0.0;
+  // 64-thread block: one wave, fixed shuffle tree
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if (threadIdx.x == 0) npart[(long long)s * gridDim.x + blockIdx.x] = v;
+}
+
+// rnel renormalisation (src/nhat.f90:31-41) fused with the Hartree packing:
+// every workgroup forms fac[s] = rnel / (dv sum dens + dv sum nhat) from the
+// partial sums in the same fixed order (so all agree bit for bit), then
+// dens *= fac and pair[q] = fac (dens + nhat)[2q] + i fac (dens + nhat)[2q+1]
+// with nhat read from the sparse sphere values
+__global__ __launch_bounds__(kT) void k_renorm_pack(double* __restrict__ dens, const int* __restrict__ smap,
+                                                    const double* __restrict__ nhs, int S,
+                                                    const double* __restrict__ dpart, int nd,
+                                                    const double* __restrict__ npart, int nn, double dv,
+                                                    double rnel, int renorm, int N, int nsys,
+                                                    c64* __restrict__ pair, double* __restrict__ fac_out) {
+  const int q = blockIdx.y;
+  __shared__ double facs[2];
+  for (int k = 0; k < 2; ++k) {
+    const int s = 2 * q + k;
+    if (s >= nsys) break;
+    double v[2] = {0.0, 0.0};
+    for (int c = threadIdx.x; c < nd; c += kT) v[0] += dpart[(long long)s * nd + c];
+    for (int c = threadIdx.x; c < nn; c += kT) v[1] += npart[(long long)s * nn + c];
+    block_sum<2>(v);
+    if (threadIdx.x == 0) {
+      const double f = renorm ? rnel / (v[1] * dv + v[0] * dv) : 1.0;
+      facs[k] = f;
+      if (blockIdx.x == 0) fac_out[s] = f;
+    }
+  }
+  __syncthreads();
+  const int base = blockIdx.x * (kT * kV) + threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < kV; ++j) {
+    const int r = base + j * kT;
+    if (r >= N) continue;
+    const int k = smap[r];
+    double v[2] = {0.0, 0.0};
+    for (int t = 0; t < 2; ++t) {
+      const int s = 2 * q + t;
+      if (s < nsys) {
+        const long long i = (long long)s * N + r;
+        const double d = dens[i] * facs[t];
+        const double h = (k >= 0 ? nhs[(long long)s * S + k] : 0.0) * facs[t];
+        dens[i] = d;
+        v[t] = d + h;
+      }
+    }
+    pair[(long long)q * N + r] = mk(v[0], v[1]);
+  }
+}
+
+// full nhat field of one system from the sparse values: nhat = fac * nhs on
+// the sphere union, 0 elsewhere
+__global__ __launch_bounds__(kT) void k_nhat_expand(const int* __restrict__ smap, const double* __restrict__ nhs,
+                                                    const double* __restrict__ fac, int N,
+                                                    double* __restrict__ nhat) {
+  const int r = blockIdx.x * kT + threadIdx.x;
+  if (r >= N) return;
+  const int k = smap[r];
+  nhat[r] = k >= 0 ? fac[0] * nhs[k] : 0.0;
+}
+
+// ---------------------------------------------------------------------------
 // host wrappers
 static unsigned grid_for(long long n) {
   long long b = (n + kT - 1) / kT;
@@ -737,6 +932,54 @@ hipError_t scale_dipole(hipStream_t st, cx<R>* u, int nx, int ny, int nz, int ns
   hipLaunchKernelGGL((k_scale_dipole<R>), dim3(kChunks, nsys), dim3(kT), nb * sizeof(double), st, u, nx, ny, nz, nb,
                      kChunks, dx, dy, dz, num, den, part);
   hipLaunchKernelGGL(k_finish_rows, dim3(3 * nsys), dim3(kT), 0, st, part, 3 * nsys, kChunks, dx * dy * dz, 0, out);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t density_sum(hipStream_t st, const cx<R>* u, long long N, int nsys, int nb, double* dens,
+                       const int* umap, const c64* cpre, int U, double* dpart, int* nd) {
+  const int nc = (int)((N + kDensBlock - 1) / kDensBlock);
+  *nd = nc;
+  hipLaunchKernelGGL((k_density_sum<R>), dim3(nc, nsys), dim3(kT), 0, st, u, (int)N, nb, dens, umap, cpre, U,
+                     dpart);
+  return hipGetLastError();
+}
+
+hipError_t rhoij_nhatw(hipStream_t st, const AtomDesc* atoms, int natom, const c64* ca, int ctot, int nsys, int nb,
+                       int mtot, double* rho, const double* qtab, int nl2, double* wl) {
+  if (natom == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rhoij_nhatw, dim3(natom, nsys), dim3(64), 0, st, atoms, ca, ctot, nb, mtot, rho, qtab, nl2,
+                     natom, wl);
+  return hipGetLastError();
+}
+
+hipError_t nhat_sparse(hipStream_t st, const SphereTabs& T, int S, const double* sph_g, const double* wl, int natom,
+                       int nl2, int nsys, double* nhs, double* npart, int* nn) {
+  const int nc = (S + 63) / 64;
+  *nn = nc;
+  if (S == 0) return hipSuccess;
+  if (nl2 <= 9)
+    hipLaunchKernelGGL(k_nhat_sparse<9>, dim3(nc, nsys), dim3(64), 0, st, T, S, sph_g, wl, natom, nl2, nhs, npart);
+  else if (nl2 <= 16)
+    hipLaunchKernelGGL(k_nhat_sparse<16>, dim3(nc, nsys), dim3(64), 0, st, T, S, sph_g, wl, natom, nl2, nhs, npart);
+  else
+    hipLaunchKernelGGL(k_nhat_sparse<kMaxNl2>, dim3(nc, nsys), dim3(64), 0, st, T, S, sph_g, wl, natom, nl2, nhs,
+                       npart);
+  return hipGetLastError();
+}
+
+hipError_t renorm_pack(hipStream_t st, double* dens, const int* smap, const double* nhs, int S, const double* dpart,
+                       int nd, const double* npart, int nn, double dv, double rnel, int renorm, long long N, int nsys,
+                       c64* pair, double* fac) {
+  hipLaunchKernelGGL(k_renorm_pack, grid_rows(N, (nsys + 1) / 2, kV), dim3(kT), 0, st, dens, smap, nhs, S, dpart,
+                     nd, npart, nn, dv, rnel, renorm, (int)N, nsys, pair, fac);
+  return hipGetLastError();
+}
+
+hipError_t nhat_expand(hipStream_t st, const int* smap, const double* nhs, const double* fac, long long N,
+                       double* nhat) {
+  hipLaunchKernelGGL(k_nhat_expand, dim3((unsigned)((N + kT - 1) / kT)), dim3(kT), 0, st, smap, nhs, fac, (int)N,
+                     nhat);
   return hipGetLastError();
 }
 
@@ -858,6 +1101,8 @@ hipError_t convert_to_c64(hipStream_t st, const cx<R>* in, c64* out, long long n
   template hipError_t scale_fields<R>(hipStream_t, cx<R>*, long long, int, const double*, const double*);       \
   template hipError_t density<R>(hipStream_t, const cx<R>*, long long, int, int, double*, const int*,          \
                                  const c64*, int);                                                              \
+  template hipError_t density_sum<R>(hipStream_t, const cx<R>*, long long, int, int, double*, const int*,      \
+                                     const c64*, int, double*, int*);                                           \
   template hipError_t dipole<R>(hipStream_t, const cx<R>*, int, int, int, int, int, double, double, double,     \
                                 double*, double*);                                                              \
   template hipError_t kick<R>(hipStream_t, const cx<R>*, cx<R>*, int, int, int, int, int, double, double);      \
