@@ -245,14 +245,46 @@ def polyacene_geometry(nring, cc=2.65, ch=2.05):
     return C, H
 
 
+def coronene_geometry(cc=2.68, ch=2.05):
+    """Planar coronene C24H12 (D6h) in the xy plane: a central hexagon of side
+    cc and the six hexagons sharing its edges; C-H bonds radial from the
+    centre of the one ring an outer carbon belongs to."""
+    a = cc
+    centers = [(0.0, 0.0)] + [(math.sqrt(3.0) * a * math.cos(math.radians(60 * k)),
+                               math.sqrt(3.0) * a * math.sin(math.radians(60 * k))) for k in range(6)]
+    pts = {}
+    for ci, (cx, cy) in enumerate(centers):
+        for k in range(6):
+            th = math.radians(30 + 60 * k)
+            key = (round(cx + a * math.cos(th), 6), round(cy + a * math.sin(th), 6))
+            pts.setdefault(key, []).append(ci)
+    carbons = sorted(pts.keys())
+    hyd = []
+    for key in carbons:
+        if len(pts[key]) == 1:
+            cx, cy = centers[pts[key][0]]
+            vx, vy = key[0] - cx, key[1] - cy
+            nrm = math.hypot(vx, vy)
+            hyd.append((key[0] + ch * vx / nrm, key[1] + ch * vy / nrm))
+    C = np.array([[x, y, 0.0] for x, y in carbons])
+    H = np.array([[x, y, 0.0] for x, y in hyd])
+    return C, H
+
+
 def molecule_system(name="c10h8", nring=2, n=(64, 64, 64), d=0.5, seed=1234,
-                    dt=0.05, sm=1e-3, ipol=0) -> SystemSpec:
-    """Naphthalene-shaped (nring=2) nonperiodic MT system with synthetic
-    C (14^3 box, 8 projectors) and H (10^3 box, 5 projectors) tables."""
+                    dt=0.05, sm=1e-3, ipol=0, geometry="polyacene", cc=None) -> SystemSpec:
+    """Nonperiodic MT system with synthetic C (14^3 box, 8 projectors) and H
+    (10^3 box, 5 projectors) tables: the polyacene with nring rings
+    (naphthalene for nring=2; C-C 2.65 bohr) or coronene C24H12 (C-C 2.68)."""
     rng = np.random.default_rng(seed)
     nx, ny, nz = n
     dx = dy = dz = float(d)
-    Cxyz, Hxyz = polyacene_geometry(nring)
+    if geometry == "polyacene":
+        Cxyz, Hxyz = polyacene_geometry(nring, cc=cc or 2.65)
+    elif geometry == "coronene":
+        Cxyz, Hxyz = coronene_geometry(cc=cc or 2.68)
+    else:
+        raise ValueError(f"unknown geometry {geometry}")
     grid = (nx, ny, nz, dx, dy, dz, False)
     nl2 = 9
 
@@ -300,11 +332,12 @@ def _erf_over_r(r, rc):
     return np.where(r > 1e-12, erf(r / rc) / rs, 2.0 / (math.sqrt(math.pi) * rc))
 
 
-def hpsi_system(n=(96, 96, 96), d=0.4, natom=40, seed=2024, nl2=9) -> SystemSpec:
+def hpsi_system(n=(96, 96, 96), d=0.4, natom=40, seed=2024, nl2=9, nrad=2) -> SystemSpec:
     """Periodic synthetic H.psi system: V_KS = seeded sum of 40 Gaussians
     (depth -0.5..-2 Ha, width 0.5..1.5 bohr); natom atoms uniformly placed
-    with >= 2 bohr separation, 8 projectors r^l exp(-r^2/0.8) Y_lm (two radial
-    orders per l <= 1), cut at 3 bohr; random symmetric D_ij in [-1, 1]."""
+    with >= 2 bohr separation, 4 * nrad projectors r^l exp(-r^2/0.8) Y_lm
+    (nrad radial orders per l <= 1; 8 for nrad = 2), cut at 3 bohr; random
+    symmetric D_ij in [-1, 1]."""
     rng = np.random.default_rng(seed)
     nx, ny, nz = n
     dx = dy = dz = float(d)
@@ -328,9 +361,9 @@ def hpsi_system(n=(96, 96, 96), d=0.4, natom=40, seed=2024, nl2=9) -> SystemSpec
     def radial(R, l, nrad):
         return R ** (l + 2 * nrad) * np.exp(-R * R / 0.8)
 
-    types = [_make_type(rng, 8, nl2, 0.05, 0.02)]
-    atoms = [_make_atom(rng, grid, 0, c, nrough, radial, [(0, 0), (0, 1), (1, 0), (1, 1)],
-                        rcut, 0.3, 1.6, nl2, 0.7) for c in coords]
+    ms_l = [(l, r) for l in (0, 1) for r in range(nrad)]
+    types = [_make_type(rng, 4 * nrad, nl2, 0.05, 0.02)]
+    atoms = [_make_atom(rng, grid, 0, c, nrough, radial, ms_l, rcut, 0.3, 1.6, nl2, 0.7) for c in coords]
     rx = (np.arange(nx) - nx // 2) * dx
     ry = (np.arange(ny) - ny // 2) * dy
     rz = (np.arange(nz) - nz // 2) * dz

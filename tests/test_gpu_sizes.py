@@ -1,0 +1,271 @@
+"""GPU parity at the full sizes of the workloads, and the code paths the small
+cases do not reach, all through the C ABI.
+
+* H.psi on the periodic 96^3 system with 128 orbitals and 40 atoms x 8
+  projectors (the H.psi workload): two orbitals against the oracle (<= 1e-10),
+  batch independence (the two orbitals alone give the batch's rows), and the
+  mixed path against FP64 over all 128 orbitals (<= 2e-5).
+* One full time step of the benchmark workload (C10H8, 64^3 MT, 24 orbitals x
+  2 systems) against the oracle: orbitals <= 1e-10, d_dip <= 1e-8.
+* Grids with nx != ny (the five-pass line path of the kinetic operator, with
+  the corrections gathered per point) and odd radices: h_phi, shs and one
+  RK4 step against the oracle.
+* 12 projectors per atom (the 16-wide projection, overlap and finish kernels)
+  on a periodic system: operators and one RK4 step against the oracle.
+* No atoms at all, and a single orbital.
+* Coronene C24H12 on 96^3 @ 0.45 MT (the 4-system batch: unperturbed + kicks
+  along x, y, z; 54 orbitals): H.psi against the oracle, then time steps
+  whose per-orbital norms stay at their initial values (<= 1e-12) and whose
+  mixed-path kick responses stay within 1e-4 (relative L2) of FP64 over 20
+  steps.
+* The polyacene series n = 1..6 on grids from 48^3 to 128x64x48: H.psi of one
+  orbital against the oracle, and one time step whose norms are conserved
+  and whose mixed-path orbitals agree with FP64.
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import O, rel, state_from_spec
+from opaw_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _api():
+    import torch
+    from opaw_amd.api import OpawGPU
+    assert torch.cuda.is_available(), "GPU test without a GPU"
+    return torch, OpawGPU
+
+
+def _potentials(spec, st, seed):
+    rng = np.random.default_rng(seed)
+    for S in st.systems:
+        S.vks = spec.vloc + rng.uniform(-0.3, 0.3, spec.n)
+        S.dij = []
+        for a in spec.atoms:
+            m = rng.uniform(-1, 1, (a.ms, a.ms))
+            S.dij.append(0.5 * (m + m.T))
+
+
+def _push(g, st):
+    for s, S in enumerate(st.systems):
+        g.write_vks(s, S.vks)
+        for ia, d in enumerate(S.dij, start=1):
+            g.write_dij(s, ia, d)
+
+
+def _orbitals(spec, nb, seed, envelope=4.0):
+    return synth.lowdin_orthonormalise(synth.random_orbitals(spec, nb, seed=seed, envelope=envelope), spec.dv)
+
+
+def _norms(spec, phi):
+    return np.sqrt(spec.dv * np.sum(np.abs(phi) ** 2, axis=1))
+
+
+# ---------------------------------------------------------------------------
+def test_h_phi_periodic_96_full_batch():
+    torch, OpawGPU = _api()
+    spec = synth.hpsi_system(n=(96, 96, 96), d=0.4, natom=40, seed=2024)
+    nb = 128
+    st = state_from_spec(spec, nsys=1, with_vk=False)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(5)
+    x = torch.randn(nb, spec.n, dtype=torch.complex128, device="cuda", generator=gen)
+    g = OpawGPU(spec, "f64", nsys=1, nb=nb)
+    out = g.h_phi(x, 0)
+    sub = [0, 77]
+    phi_sub = x[sub].cpu().numpy()
+    ref = O.h_phi(st, phi_sub, 0)
+    got = out[sub].cpu().numpy()
+    assert rel(got, ref) < 1e-10
+    alone = g.h_phi(x[sub].contiguous(), 0).cpu().numpy()
+    assert np.max(np.abs(alone - got)) <= 1e-14 * np.max(np.abs(ref))
+    g.close()
+    g32 = OpawGPU(spec, "mixed", nsys=1, nb=nb)
+    out32 = g32.h_phi(x.to(torch.complex64), 0)
+    err = (out32.to(torch.complex128) - out).abs().max().item() / out.abs().max().item()
+    assert err < 2e-5, err
+    g32.close()
+
+
+def test_time_step_c10h8_64_full_size():
+    torch, OpawGPU = _api()
+    spec = synth.molecule_system(nring=2, n=(64, 64, 64), d=0.5, seed=1234)
+    nb = 24
+    spec.rnel = 2.0 * nb
+    phi0 = _orbitals(spec, nb, seed=7)
+    st = state_from_spec(spec, nsys=2)
+    phis = [phi0, O.excite_psi_bar_pert(st, phi0, spec.sm, spec.ipol)]
+    new, d_dip, _, _ = O.tddft_step(st, phis, spec.sm)
+    g = OpawGPU(spec, "f64", nsys=2, nb=nb)
+    g.write_orbitals(0, phi0)
+    g.excite_psi_bar_pert(spec.sm, spec.ipol + 1)
+    d_gpu, _ = g.opaw_tddft(1, spec.dt, spec.sm)
+    for s in (0, 1):
+        assert rel(g.read_orbitals(s), new[s]) < 1e-10, s
+    assert np.max(np.abs(d_gpu[0] - d_dip)) <= 1e-8 * np.max(np.abs(d_dip))
+    g.close()
+
+
+@pytest.mark.parametrize("shape", [(48, 40, 32), (40, 48, 24)])
+def test_line_path_grids(shape):
+    """nx != ny: the kinetic operator runs as five line passes (x, y, z, y, x)
+    and the x passes gather the union corrections per point."""
+    torch, OpawGPU = _api()
+    spec = synth.molecule_system(nring=2, n=shape, d=0.5, seed=21)
+    nb = 3
+    spec.rnel = 2.0 * nb
+    st = state_from_spec(spec, nsys=2)
+    _potentials(spec, st, 3)
+    phi = _orbitals(spec, nb, seed=8, envelope=3.0)
+    g = OpawGPU(spec, "f64", nsys=2, nb=nb)
+    _push(g, st)
+    x = g.to_device(phi)
+    for sys in (0, 1):
+        assert rel(g.h_phi(x, sys).cpu().numpy(), O.h_phi(st, phi, sys)) < 1e-10
+        assert rel(g.shs(x, sys).cpu().numpy(), O.shs(st, phi, sys)) < 1e-10
+    g.close()
+    # one self-consistent time step from scratch potentials
+    st2 = state_from_spec(spec, nsys=2)
+    phis = [phi, O.excite_psi_bar_pert(st2, phi, spec.sm, spec.ipol)]
+    new, d_dip, _, _ = O.tddft_step(st2, phis, spec.sm)
+    g = OpawGPU(spec, "f64", nsys=2, nb=nb)
+    g.write_orbitals(0, phi)
+    g.excite_psi_bar_pert(spec.sm, spec.ipol + 1)
+    d_gpu, _ = g.opaw_tddft(1, spec.dt, spec.sm)
+    for s in (0, 1):
+        assert rel(g.read_orbitals(s), new[s]) < 1e-10, s
+    assert np.max(np.abs(d_gpu[0] - d_dip)) <= 1e-8 * np.max(np.abs(d_dip))
+    g.close()
+
+
+def test_twelve_projectors_per_atom():
+    """ms = 12 > 8: the 16-wide projection, finish and overlap kernels."""
+    torch, OpawGPU = _api()
+    spec = synth.hpsi_system(n=(32, 32, 40), d=0.4, natom=5, seed=9, nrad=3)
+    assert all(a.ms == 12 for a in spec.atoms)
+    nb = 3
+    spec.rnel = 2.0 * nb
+    st = state_from_spec(spec, nsys=2)
+    _potentials(spec, st, 4)
+    phi = _orbitals(spec, nb, seed=6, envelope=3.0)
+    g = OpawGPU(spec, "f64", nsys=2, nb=nb)
+    _push(g, st)
+    x = g.to_device(phi)
+    assert rel(g.h_phi(x, 0).cpu().numpy(), O.h_phi(st, phi, 0)) < 1e-10
+    assert rel(g.sn_phi(x, -0.5).cpu().numpy(), O.sn_phi(st, phi, -0.5)) < 1e-12
+    assert rel(g.shs(x, 1).cpu().numpy(), O.shs(st, phi, 1)) < 1e-10
+    g.close()
+    st2 = state_from_spec(spec, nsys=2)
+    phis = [phi, O.excite_psi_bar_pert(st2, phi, spec.sm, spec.ipol)]
+    new, d_dip, _, _ = O.tddft_step(st2, phis, spec.sm)
+    g = OpawGPU(spec, "f64", nsys=2, nb=nb)
+    g.write_orbitals(0, phi)
+    g.excite_psi_bar_pert(spec.sm, spec.ipol + 1)
+    d_gpu, _ = g.opaw_tddft(1, spec.dt, spec.sm)
+    for s in (0, 1):
+        assert rel(g.read_orbitals(s), new[s]) < 1e-10, s
+    assert np.max(np.abs(d_gpu[0] - d_dip)) <= 1e-8 * np.max(np.abs(d_dip))
+    g.close()
+
+
+def test_no_atoms_radix_3_5():
+    torch, OpawGPU = _api()
+    spec = synth.hpsi_system(n=(24, 20, 40), d=0.4, natom=0, seed=12)
+    nb = 2
+    st = state_from_spec(spec, nsys=1, with_vk=False)
+    phi = _orbitals(spec, nb, seed=2)
+    g = OpawGPU(spec, "f64", nsys=1, nb=nb)
+    x = g.to_device(phi)
+    assert rel(g.h_phi(x, 0).cpu().numpy(), O.h_phi(st, phi, 0)) < 1e-10
+    assert rel(g.shs(x, 0).cpu().numpy(), O.shs(st, phi, 0)) < 1e-10
+    g.close()
+
+
+def test_single_orbital_step():
+    torch, OpawGPU = _api()
+    spec = synth.molecule_system(nring=1, n=(32, 32, 32), d=0.5, seed=11)
+    nb = 1
+    spec.rnel = 2.0
+    phi = _orbitals(spec, nb, seed=13, envelope=3.0)
+    st = state_from_spec(spec, nsys=2)
+    phis = [phi, O.excite_psi_bar_pert(st, phi, spec.sm, spec.ipol)]
+    new, d_dip, _, _ = O.tddft_step(st, phis, spec.sm)
+    g = OpawGPU(spec, "f64", nsys=2, nb=nb)
+    g.write_orbitals(0, phi)
+    g.excite_psi_bar_pert(spec.sm, spec.ipol + 1)
+    d_gpu, _ = g.opaw_tddft(1, spec.dt, spec.sm)
+    for s in (0, 1):
+        assert rel(g.read_orbitals(s), new[s]) < 1e-10, s
+    assert np.max(np.abs(d_gpu[0] - d_dip)) <= 1e-8 * np.max(np.abs(d_dip))
+    g.close()
+
+
+def test_coronene_96_four_systems():
+    torch, OpawGPU = _api()
+    spec = synth.molecule_system(geometry="coronene", n=(96, 96, 96), d=0.45, seed=77)
+    nb = spec.nocc
+    assert nb == 54
+    # H.psi of two orbitals against the oracle
+    st = state_from_spec(spec, nsys=1, with_vk=False)
+    _potentials(spec, st, 5)
+    phi0 = _orbitals(spec, nb, seed=17, envelope=5.0)
+    g = OpawGPU(spec, "f64", nsys=1, nb=nb)
+    _push(g, st)
+    sub = phi0[[0, nb - 1]]
+    assert rel(g.h_phi(g.to_device(sub), 0).cpu().numpy(), O.h_phi(st, sub, 0)) < 1e-10
+    g.close()
+    # unperturbed + kicks along x, y, z, 20 time steps in each precision
+    traces = {}
+    for prec in ("f64", "mixed"):
+        g = OpawGPU(spec, prec, nsys=4, nb=nb)
+        g.write_orbitals(0, phi0)
+        for k in (1, 2, 3):
+            g.excite_psi_bar_pert(spec.sm, k, from_sys=0, to_sys=k)
+        n0 = [_norms(spec, g.read_orbitals(s).astype(np.complex128)) for s in range(4)]
+        d, dip = g.opaw_tddft(20, spec.dt, spec.sm)
+        n1 = [_norms(spec, g.read_orbitals(s).astype(np.complex128)) for s in range(4)]
+        tol = 1e-12 if prec == "f64" else 1e-6
+        for s in range(4):
+            assert np.max(np.abs(n1[s] - n0[s])) < tol, (prec, s)
+        assert np.all(np.isfinite(dip))
+        traces[prec] = dip
+        g.close()
+    a, b = traces["mixed"], traces["f64"]
+    kicked = b[:, 3:] - np.tile(b[:, :3], 3)
+    kicked_m = a[:, 3:] - np.tile(a[:, :3], 3)
+    assert np.linalg.norm(kicked_m - kicked) <= 1e-4 * np.linalg.norm(kicked)
+
+
+POLYACENE_GRIDS = {1: (48, 48, 48), 2: (64, 48, 48), 3: (80, 64, 48), 4: (96, 64, 48), 5: (96, 64, 48),
+                   6: (128, 64, 48)}
+
+
+@pytest.mark.parametrize("nring", [1, 2, 3, 4, 5, 6])
+def test_polyacene_series(nring):
+    torch, OpawGPU = _api()
+    spec = synth.molecule_system(nring=nring, n=POLYACENE_GRIDS[nring], d=0.5, seed=100 + nring)
+    nb = spec.nocc
+    assert nb == 9 * nring + 6
+    st = state_from_spec(spec, nsys=1, with_vk=False)
+    _potentials(spec, st, 6)
+    phi0 = _orbitals(spec, nb, seed=30 + nring)
+    g = OpawGPU(spec, "f64", nsys=1, nb=nb)
+    _push(g, st)
+    one = phi0[[nb // 2]]
+    assert rel(g.h_phi(g.to_device(one), 0).cpu().numpy(), O.h_phi(st, one, 0)) < 1e-10
+    g.close()
+    out = {}
+    for prec in ("f64", "mixed"):
+        g = OpawGPU(spec, prec, nsys=2, nb=nb)
+        g.write_orbitals(0, phi0)
+        g.excite_psi_bar_pert(spec.sm, spec.ipol + 1)
+        g.opaw_tddft(1, spec.dt, spec.sm)
+        p = [g.read_orbitals(s).astype(np.complex128) for s in (0, 1)]
+        n1 = _norms(spec, p[0])
+        assert np.max(np.abs(n1 - _norms(spec, phi0))) < (1e-12 if prec == "f64" else 1e-6), prec
+        out[prec] = p
+        g.close()
+    for s in (0, 1):
+        assert rel(out["mixed"][s], out["f64"][s]) < 2e-5, s
