@@ -170,6 +170,118 @@ struct Stage {
   }
 };
 
+// ---------------------------------------------------------------------------
+// Runtime-length transforms (lengths outside the compiled list): the radix
+// plan is built on the host (radices 8, 4, 2, 3, 5, 7, 11, 13) and each stage
+// runs out of place between two LDS buffers.  Radices without a hand-written
+// butterfly use a direct DFT whose roots come from the twiddle table
+// (exp(-2 pi i m / N) = tw[m]).
+constexpr int kMaxPlan = 16;
+struct Plan {
+  int nr;
+  int rad[kMaxPlan];
+};
+
+// radix plan of n, or nr = 0 when n has a prime factor above 13
+inline Plan make_plan(int n) {
+  Plan P{};
+  P.nr = 0;
+  if (n < 2) return P;
+  int m = n;
+  while (m > 1 && P.nr < kMaxPlan) {
+    int r = 0;
+    if (m % 8 == 0 && m != 16) r = 8;
+    else if (m % 4 == 0) r = 4;
+    else if (m % 2 == 0) r = 2;
+    else {
+      const int odd[5] = {3, 5, 7, 11, 13};
+      for (int i = 0; i < 5; ++i)
+        if (m % odd[i] == 0) {
+          r = odd[i];
+          break;
+        }
+    }
+    if (r == 0) {
+      P.nr = 0;
+      return P;
+    }
+    P.rad[P.nr++] = r;
+    m /= r;
+  }
+  if (m != 1) P.nr = 0;
+  return P;
+}
+
+template <typename R, int RAD, int S>
+__host__ __device__ __forceinline__ void dft_direct(cx<R>* v, const cx<R>* tw, int N) {
+  cx<R> o[RAD];
+  const int step = N / RAD;
+#pragma unroll
+  for (int m = 0; m < RAD; ++m) {
+    cx<R> acc = v[0];
+#pragma unroll
+    for (int q = 1; q < RAD; ++q) {
+      cx<R> w = tw[((q * m) % RAD) * step];
+      if constexpr (S > 0) w.y = -w.y;
+      acc = acc + cmul(v[q], w);
+    }
+    o[m] = acc;
+  }
+#pragma unroll
+  for (int m = 0; m < RAD; ++m) v[m] = o[m];
+}
+
+// one butterfly of a runtime stage: src -> dst (same index scheme as Stage)
+template <typename R, int RAD, int S>
+__host__ __device__ __forceinline__ void gen_butterfly(const cx<R>* src, cx<R>* dst, const cx<R>* tw, int N, int l,
+                                                       int j, int M, int ns, int LSTR, int ESTR) {
+  cx<R> v[RAD];
+#pragma unroll
+  for (int r = 0; r < RAD; ++r) v[r] = src[l * LSTR + (j + r * M) * ESTR];
+  const int k = j % ns;
+  if (ns > 1) {
+    const int tws = N / (ns * RAD);
+#pragma unroll
+    for (int r = 1; r < RAD; ++r) {
+      cx<R> w = tw[r * k * tws];
+      if constexpr (S > 0) w.y = -w.y;
+      v[r] = cmul(v[r], w);
+    }
+  }
+  if constexpr (RAD == 2 || RAD == 3 || RAD == 4 || RAD == 5 || RAD == 8)
+    dft<R, RAD, S>(v);
+  else
+    dft_direct<R, RAD, S>(v, tw, N);
+  const int base = (j / ns) * ns * RAD + k;
+#pragma unroll
+  for (int r = 0; r < RAD; ++r) dst[l * LSTR + (base + r * ns) * ESTR] = v[r];
+}
+
+// all stages of a runtime plan for `lines` lines; element (l, j) of a buffer
+// at l*LSTR + j*ESTR; tid / nthreads walk the butterflies.  Returns the
+// buffer holding the result (a or b).
+template <typename R, int S>
+__host__ __device__ inline cx<R>* gen_stage(const cx<R>* src, cx<R>* dst, const cx<R>* tw, int N, int lines,
+                                            bool lfast, int LSTR, int ESTR, int rad, int ns, int tid,
+                                            int nthreads) {
+  const int M = N / rad, NB = lines * M;
+  for (int t = tid; t < NB; t += nthreads) {
+    const int l = lfast ? (t % lines) : (t / M);
+    const int j = lfast ? (t / lines) : (t % M);
+    switch (rad) {
+      case 2: gen_butterfly<R, 2, S>(src, dst, tw, N, l, j, M, ns, LSTR, ESTR); break;
+      case 3: gen_butterfly<R, 3, S>(src, dst, tw, N, l, j, M, ns, LSTR, ESTR); break;
+      case 4: gen_butterfly<R, 4, S>(src, dst, tw, N, l, j, M, ns, LSTR, ESTR); break;
+      case 5: gen_butterfly<R, 5, S>(src, dst, tw, N, l, j, M, ns, LSTR, ESTR); break;
+      case 7: gen_butterfly<R, 7, S>(src, dst, tw, N, l, j, M, ns, LSTR, ESTR); break;
+      case 8: gen_butterfly<R, 8, S>(src, dst, tw, N, l, j, M, ns, LSTR, ESTR); break;
+      case 11: gen_butterfly<R, 11, S>(src, dst, tw, N, l, j, M, ns, LSTR, ESTR); break;
+      default: gen_butterfly<R, 13, S>(src, dst, tw, N, l, j, M, ns, LSTR, ESTR); break;
+    }
+  }
+  return dst;
+}
+
 #if defined(__HIPCC__)
 template <typename R, int N, int NS, int LINES, int THREADS, bool LFAST, int LSTR, int ESTR, int S>
 __device__ __forceinline__ void stages(cx<R>* lds, const cx<R>* tw) {
@@ -191,6 +303,21 @@ __device__ __forceinline__ void stages(cx<R>* lds, const cx<R>* tw) {
 template <typename R, int N, int LINES, int THREADS, bool LFAST, int LSTR, int ESTR, int S>
 __device__ __forceinline__ void lines(cx<R>* lds, const cx<R>* tw) {
   stages<R, N, 1, LINES, THREADS, LFAST, LSTR, ESTR, S>(lds, tw);
+}
+
+template <typename R, int S>
+__device__ __forceinline__ cx<R>* lines_gen(cx<R>* a, cx<R>* b, const cx<R>* tw, int N, int lines, bool lfast,
+                                            int LSTR, int ESTR, const Plan& P) {
+  int ns = 1;
+  for (int s = 0; s < P.nr; ++s) {
+    gen_stage<R, S>(a, b, tw, N, lines, lfast, LSTR, ESTR, P.rad[s], ns, threadIdx.x, blockDim.x);
+    __syncthreads();
+    cx<R>* t = a;
+    a = b;
+    b = t;
+    ns *= P.rad[s];
+  }
+  return a;
 }
 
 // Copy the forward twiddle table exp(-2 pi i m / N), m < N, into LDS.

@@ -66,7 +66,9 @@ template <int N>
 constexpr int x_stride() { return (N % 16 == 0) ? N + 8 : N; }
 
 // ---------------------------------------------------------------------------
-// x pass
+// x pass.  N > 0: a compiled length (static LDS, in-place register-staged
+// stages).  N == 0: the runtime length n with its radix plan (dynamic LDS
+// holding two tiles and the twiddles, out-of-place stages).
 template <typename R, int N, int S, int EPI>
 __global__ __launch_bounds__(kThreads) void k_fft_x(const cx<R>* in, View vin,
                                                      cx<R>* out, View vout, XLines L,
@@ -74,16 +76,30 @@ __global__ __launch_bounds__(kThreads) void k_fft_x(const cx<R>* in, View vin,
                                                      const cx<R>* __restrict__ u,
                                                      const R* __restrict__ vks, int nf_per_sys,
                                                      const int* __restrict__ umap, const c64* __restrict__ cpre,
-                                                     const c64* __restrict__ cnl, int U) {
-  constexpr int LINES = x_lines<N>();
-  constexpr int LS = x_stride<N>();
-  __shared__ cx<R> tile[LINES * LS];
-  __shared__ cx<R> tw[N];
-  fft::load_twiddles(tw, twg, N);
+                                                     const c64* __restrict__ cnl, int U, int n_rt,
+                                                     fft::Plan plan) {
+  const int n = N > 0 ? N : n_rt;
+  const int LINES = N > 0 ? x_lines<(N > 0 ? N : 1)>() : ((1024 / n) > 0 ? 1024 / n : 1);
+  const int LS = N > 0 ? x_stride<(N > 0 ? N : 1)>() : ((n % 16 == 0) ? n + 8 : n);
+  cx<R>* tile;
+  cx<R>* tile2 = nullptr;
+  cx<R>* tw;
+  if constexpr (N > 0) {
+    __shared__ cx<R> s_tile[x_lines<(N > 0 ? N : 1)>() * x_stride<(N > 0 ? N : 1)>()];
+    __shared__ cx<R> s_tw[N > 0 ? N : 1];
+    tile = s_tile;
+    tw = s_tw;
+  } else {
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    tile = reinterpret_cast<cx<R>*>(dyn_lds);
+    tile2 = tile + LINES * LS;
+    tw = tile2 + LINES * LS;
+  }
+  fft::load_twiddles(tw, twg, n);
   const long long nlines = (long long)L.ly * L.lz * L.nf;
   const long long line0 = (long long)blockIdx.x * LINES;
-  for (int idx = threadIdx.x; idx < LINES * N; idx += kThreads) {
-    const int l = idx / N, x = idx % N;
+  for (int idx = threadIdx.x; idx < LINES * n; idx += kThreads) {
+    const int l = idx / n, x = idx % n;
     const long long gl = line0 + l;
     cx<R> val = mk<R>(0, 0);
     if (gl < nlines && x >= L.la && x < L.lb) {
@@ -103,9 +119,14 @@ __global__ __launch_bounds__(kThreads) void k_fft_x(const cx<R>* in, View vin,
     tile[l * LS + x] = val;
   }
   __syncthreads();
-  fft::lines<R, N, LINES, kThreads, false, LS, 1, S>(tile, tw);
-  for (int idx = threadIdx.x; idx < LINES * N; idx += kThreads) {
-    const int l = idx / N, x = idx % N;
+  const cx<R>* res = tile;
+  if constexpr (N > 0)
+    fft::lines<R, (N > 0 ? N : 1), x_lines<(N > 0 ? N : 1)>(), kThreads, false, x_stride<(N > 0 ? N : 1)>(), 1, S>(
+        tile, tw);
+  else
+    res = fft::lines_gen<R, S>(tile, tile2, tw, n, LINES, false, LS, 1, plan);
+  for (int idx = threadIdx.x; idx < LINES * n; idx += kThreads) {
+    const int l = idx / n, x = idx % n;
     const long long gl = line0 + l;
     if (gl < nlines && x >= L.sa && x < L.sb) {
       const int y = L.y0 + (int)(gl % L.ly);
@@ -113,7 +134,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_x(const cx<R>* in, View vin,
       const int z = L.z0 + (int)(r % L.lz);
       const int f = (int)(r / L.lz);
       const long long o = vaddr(vout, f, x, y, z);
-      cx<R> val = tile[l * LS + x];
+      cx<R> val = res[l * LS + x];
       if constexpr (EPI == EPI_HPSI) {
         const int sys = f / nf_per_sys;
         const long long pt = o - (long long)f * vout.fs;
@@ -153,11 +174,27 @@ template <typename R, int N, int AXIS, int S, int EPI>
 __global__ __launch_bounds__(kThreads) void k_fft_s(const cx<R>* in, View vin,
                                                      cx<R>* out, View vout, SLines L,
                                                      const cx<R>* __restrict__ twg,
-                                                     const R* __restrict__ tab, View vtab) {
-  constexpr int XT = kSXT;
-  __shared__ cx<R> tile[XT * N];
-  __shared__ cx<R> tw[N];
-  fft::load_twiddles(tw, twg, N);
+                                                     const R* __restrict__ tab, View vtab, int n_rt, int xt_rt,
+                                                     fft::Plan plan) {
+  // N == 0: runtime length n_rt over xt_rt columns (dynamic LDS: two tiles
+  // and the twiddles)
+  const int n = N > 0 ? N : n_rt;
+  const int XT = N > 0 ? kSXT : xt_rt;
+  cx<R>* tile;
+  cx<R>* tile2 = nullptr;
+  cx<R>* tw;
+  if constexpr (N > 0) {
+    __shared__ cx<R> s_tile[kSXT * (N > 0 ? N : 1)];
+    __shared__ cx<R> s_tw[N > 0 ? N : 1];
+    tile = s_tile;
+    tw = s_tw;
+  } else {
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    tile = reinterpret_cast<cx<R>*>(dyn_lds);
+    tile2 = tile + XT * n;
+    tw = tile2 + XT * n;
+  }
+  fft::load_twiddles(tw, twg, n);
   const int xtiles = (L.nx + XT - 1) / XT;
   long long b = blockIdx.x;
   const int xt = (int)(b % xtiles);
@@ -165,7 +202,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_s(const cx<R>* in, View vin,
   const int o = L.o0 + (int)(b % L.lo);
   const int f = (int)(b / L.lo);
   const int x0 = xt * XT;
-  for (int idx = threadIdx.x; idx < XT * N; idx += kThreads) {
+  for (int idx = threadIdx.x; idx < XT * n; idx += kThreads) {
     const int c = idx % XT, j = idx / XT;
     const int x = x0 + c;
     cx<R> val = mk<R>(0, 0);
@@ -176,26 +213,33 @@ __global__ __launch_bounds__(kThreads) void k_fft_s(const cx<R>* in, View vin,
     tile[idx] = val;
   }
   __syncthreads();
-  fft::lines<R, N, XT, kThreads, true, 1, XT, S>(tile, tw);
+  cx<R>* res = tile;
+  if constexpr (N > 0)
+    fft::lines<R, (N > 0 ? N : 1), kSXT, kThreads, true, 1, kSXT, S>(tile, tw);
+  else
+    res = fft::lines_gen<R, S>(tile, tile2, tw, n, XT, true, 1, XT, plan);
   if constexpr (EPI == EPI_MULT) {
-    for (int idx = threadIdx.x; idx < XT * N; idx += kThreads) {
+    for (int idx = threadIdx.x; idx < XT * n; idx += kThreads) {
       const int c = idx % XT, j = idx / XT;
       const int x = x0 + c;
       if (x < L.nx) {
         const long long a = (AXIS == 1) ? vaddr(vtab, 0, x, j, o) : vaddr(vtab, 0, x, o, j);
         const R t = tab[a];
-        tile[idx] = mk(tile[idx].x * t, tile[idx].y * t);
+        res[idx] = mk(res[idx].x * t, res[idx].y * t);
       }
     }
     __syncthreads();
-    fft::lines<R, N, XT, kThreads, true, 1, XT, -S>(tile, tw);
+    if constexpr (N > 0)
+      fft::lines<R, (N > 0 ? N : 1), kSXT, kThreads, true, 1, kSXT, -S>(tile, tw);
+    else
+      res = fft::lines_gen<R, -S>(res, res == tile ? tile2 : tile, tw, n, XT, true, 1, XT, plan);
   }
-  for (int idx = threadIdx.x; idx < XT * N; idx += kThreads) {
+  for (int idx = threadIdx.x; idx < XT * n; idx += kThreads) {
     const int c = idx % XT, j = idx / XT;
     const int x = x0 + c;
     if (x < L.nx && j >= L.sa && j < L.sb) {
       const long long a = (AXIS == 1) ? vaddr(vout, f, x, j, o) : vaddr(vout, f, x, o, j);
-      out[a] = tile[idx];
+      out[a] = res[idx];
     }
   }
 }
@@ -388,7 +432,7 @@ static hipError_t launch_plane(int n, hipStream_t st, const cx<R>* in, cx<R>* ou
   X(4) X(5) X(6) X(8) X(10) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) X(80) X(96) \
   X(128) X(160) X(192) X(256)
 
-bool fft_size_supported(int n) {
+static bool fft_size_compiled(int n) {
   switch (n) {
 #define OPAW_CASE(n) case n: return true;
     OPAW_FFT_SIZES(OPAW_CASE)
@@ -397,40 +441,75 @@ bool fft_size_supported(int n) {
   }
 }
 
+// runtime-length transforms: dynamic LDS of the x pass (two tiles of
+// 1024/n lines plus twiddles) and of the y/z pass (two tiles of xt columns
+// plus twiddles) stay within 64 KiB
+static constexpr int kMaxRuntimeN = 256;
+
+static int x_lines_rt(int n) {
+  int l = 1024 / n;
+  return l > 0 ? l : 1;
+}
+
+static int s_xt_rt(int n, int esize) {
+  int xt = kSXT;
+  while (xt > 1 && (size_t)(2 * xt + 1) * n * esize > 65536) xt /= 2;
+  return xt;
+}
+
+bool fft_size_supported(int n) {
+  if (fft_size_compiled(n)) return true;
+  return n >= 2 && n <= kMaxRuntimeN && fft::make_plan(n).nr > 0;
+}
+
 template <typename R, int S, int EPI>
 static hipError_t launch_x_n(int n, dim3 grid, hipStream_t st, const cx<R>* in, View vin, cx<R>* out,
                              View vout, XLines L, const cx<R>* tw, const cx<R>* u, const R* vks,
                              int nfps, const int* umap, const c64* cpre, const c64* cnl, int U) {
+  fft::Plan none{};
   switch (n) {
 #define OPAW_CASE(nn)                                                                   \
   case nn:                                                                              \
     hipLaunchKernelGGL((k_fft_x<R, nn, S, EPI>), grid, dim3(kThreads), 0, st, in, vin, \
-                       out, vout, L, tw, u, vks, nfps, umap, cpre, cnl, U);                         \
+                       out, vout, L, tw, u, vks, nfps, umap, cpre, cnl, U, nn, none);   \
     return hipGetLastError();
     OPAW_FFT_SIZES(OPAW_CASE)
 #undef OPAW_CASE
-    default: return hipErrorInvalidValue;
+    default: {
+      const fft::Plan P = fft::make_plan(n);
+      if (P.nr == 0 || n > kMaxRuntimeN) return hipErrorInvalidValue;
+      const int lines = x_lines_rt(n);
+      const int ls = (n % 16 == 0) ? n + 8 : n;
+      const size_t lds = (size_t)(2 * lines * ls + n) * sizeof(cx<R>);
+      hipLaunchKernelGGL((k_fft_x<R, 0, S, EPI>), grid, dim3(kThreads), lds, st, in, vin, out, vout, L, tw, u, vks,
+                         nfps, umap, cpre, cnl, U, n, P);
+      return hipGetLastError();
+    }
   }
 }
 
 template <typename R, int AXIS, int S, int EPI>
 static hipError_t launch_s_n(int n, dim3 grid, hipStream_t st, const cx<R>* in, View vin, cx<R>* out,
                              View vout, SLines L, const cx<R>* tw, const R* tab, View vtab) {
+  fft::Plan none{};
   switch (n) {
 #define OPAW_CASE(nn)                                                                        \
   case nn:                                                                                   \
     hipLaunchKernelGGL((k_fft_s<R, nn, AXIS, S, EPI>), grid, dim3(kThreads), 0, st, in, vin, \
-                       out, vout, L, tw, tab, vtab);                                         \
+                       out, vout, L, tw, tab, vtab, nn, kSXT, none);                         \
     return hipGetLastError();
     OPAW_FFT_SIZES(OPAW_CASE)
 #undef OPAW_CASE
-    default: return hipErrorInvalidValue;
+    default: {
+      const fft::Plan P = fft::make_plan(n);
+      if (P.nr == 0 || n > kMaxRuntimeN) return hipErrorInvalidValue;
+      const int xt = s_xt_rt(n, (int)sizeof(cx<R>));
+      const size_t lds = (size_t)(2 * xt * n + n) * sizeof(cx<R>);
+      hipLaunchKernelGGL((k_fft_s<R, 0, AXIS, S, EPI>), grid, dim3(kThreads), lds, st, in, vin, out, vout, L, tw,
+                         tab, vtab, n, xt, P);
+      return hipGetLastError();
+    }
   }
-}
-
-static int x_lines_rt(int n) {
-  int l = 1024 / n;
-  return l > 0 ? l : 1;
 }
 
 template <typename R>
@@ -458,7 +537,8 @@ template <typename R>
 hipError_t fft_s_pass(int cls, int n, int axis, int dir, int epi, hipStream_t st, const cx<R>* in,
                       const View& vin, cx<R>* out, const View& vout, const SLines& L,
                       const cx<R>* tw, const R* tab, const View& vtab) {
-  const long long ntile = (long long)((L.nx + kSXT - 1) / kSXT) * L.lo * L.nf;
+  const int xt = fft_size_compiled(n) ? kSXT : s_xt_rt(n, (int)sizeof(cx<R>));
+  const long long ntile = (long long)((L.nx + xt - 1) / xt) * L.lo * L.nf;
   if (ntile <= 0) return hipSuccess;
   dim3 grid((unsigned)ntile);
   hipError_t e;

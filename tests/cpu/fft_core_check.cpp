@@ -52,8 +52,50 @@ void run() {
       std::printf("%d %d %d %d %.17e %.17e\n", N, S, l, j, lds[l * LSTR + j * ESTR].x, lds[l * LSTR + j * ESTR].y);
 }
 
+// runtime-plan transforms (lengths outside the compiled list): every stage
+// out of place, all butterflies of a stage on one emulated thread
+void run_gen(int N, int LINES, bool LFAST, int S) {
+  const int LSTR = LFAST ? 1 : N;
+  const int ESTR = LFAST ? LINES : 1;
+  const Plan P = make_plan(N);
+  if (P.nr == 0) {
+    std::printf("bad plan %d\n", N);
+    std::exit(1);
+  }
+  std::vector<cx<double>> tw(N), a(N * LINES), b(N * LINES);
+  for (int m = 0; m < N; ++m) {
+    long double ang = -2.0L * 3.14159265358979323846264338327950288L * m / N;
+    tw[m] = mk<double>((double)std::cos(ang), (double)std::sin(ang));
+  }
+  for (int l = 0; l < LINES; ++l)
+    for (int j = 0; j < N; ++j)
+      a[l * LSTR + j * ESTR] = mk<double>(std::sin(0.37 * (j + 1) + l), std::cos(0.11 * j * j + 0.5 * l));
+  cx<double>* src = a.data();
+  cx<double>* dst = b.data();
+  int ns = 1;
+  for (int s = 0; s < P.nr; ++s) {
+    if (S < 0)
+      gen_stage<double, -1>(src, dst, tw.data(), N, LINES, LFAST, LSTR, ESTR, P.rad[s], ns, 0, 1);
+    else
+      gen_stage<double, 1>(src, dst, tw.data(), N, LINES, LFAST, LSTR, ESTR, P.rad[s], ns, 0, 1);
+    std::swap(src, dst);
+    ns *= P.rad[s];
+  }
+  for (int l = 0; l < LINES; ++l)
+    for (int j = 0; j < N; ++j)
+      std::printf("%d %d %d %d %.17e %.17e\n", N, S, l, j, src[l * LSTR + j * ESTR].x, src[l * LSTR + j * ESTR].y);
+}
+
 #include <array>
 int main() {
+  // radices 7, 11, 13 and their mixes with 2, 3, 5 (e.g. the 56 x 52 x 32
+  // naphthalene grid and its doubled 112 x 104 x 64 box)
+  const int gen_sizes[] = {7, 14, 28, 56, 112, 224, 13, 26, 52, 104, 208, 11, 22, 44, 77, 91, 143, 36, 60, 100, 18, 27};
+  int k = 0;
+  for (int n : gen_sizes) {
+    run_gen(n, 1 + k % 3, k % 2 == 0, (k % 4 < 2) ? -1 : 1);
+    ++k;
+  }
   run<8, 3, false, -1>();
   run<12, 5, true, -1>();
   run<16, 2, false, 1>();

@@ -51,7 +51,13 @@ def test_library_loads_and_exports_every_symbol():
     # compiled FFT lengths cover every grid of the workloads (and their MT doubles)
     for n in (48, 64, 96, 128, 192, 256, 32, 12, 10, 8):
         assert L.opaw_fft_length_supported(n) == 1, n
-    assert L.opaw_fft_length_supported(7) == 0
+    # lengths outside the compiled list run on the runtime-plan path when their
+    # prime factors are <= 13 (the reference's own 56 x 52 x 32 grid and its
+    # doubled 112 x 104 x 64 box)
+    for n in (7, 13, 52, 56, 104, 112, 143, 224):
+        assert L.opaw_fft_length_supported(n) == 1, n
+    for n in (17, 34, 512, 1):
+        assert L.opaw_fft_length_supported(n) == 0, n
 
 
 def test_null_context_is_an_error_not_a_crash():

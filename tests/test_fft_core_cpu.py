@@ -1,6 +1,7 @@
 """The Stockham stage index logic of csrc/fft_core.h, compiled for the host
 with g++ and run on emulated threads, against numpy.fft (every radix mix and
-both LDS layouts used by the kernels)."""
+both LDS layouts used by the kernels), including the runtime-plan stages of
+lengths outside the compiled list (radices 7, 11, 13)."""
 import os
 import shutil
 import subprocess

@@ -249,7 +249,8 @@ def test_error_paths():
     spec.atoms[0].start[0] = 25
     with pytest.raises(OpawError, match="cannot go outside"):
         OpawGPU(spec, "f64", nsys=1, nb=1)
+    # a grid length with a prime factor above 13 has no FFT plan
     spec = synth.hpsi_system(n=(12, 10, 8), d=0.4, natom=0, seed=1)
-    spec.nx = 14
+    spec.nx = 17
     with pytest.raises(OpawError, match="FFT length"):
         OpawGPU(spec, "f64", nsys=1, nb=1)

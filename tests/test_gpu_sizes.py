@@ -21,6 +21,10 @@ cases do not reach, all through the C ABI.
 * The polyacene series n = 1..6 on grids from 48^3 to 128x64x48: H.psi of one
   orbital against the oracle, and one time step whose norms are conserved
   and whose mixed-path orbitals agree with FP64.
+* Lengths outside the compiled list (radices 7, 11, 13 on the runtime-plan
+  path): 3D FFTs against numpy, and the naphthalene on the reference's own
+  56 x 52 x 32 grid (doubled box 112 x 104 x 64) through H.psi, shs and one
+  time step against the oracle.
 """
 import numpy as np
 import pytest
@@ -269,3 +273,55 @@ def test_polyacene_series(nring):
         g.close()
     for s in (0, 1):
         assert rel(out["mixed"][s], out["f64"][s]) < 2e-5, s
+
+
+@pytest.mark.parametrize("shape", [(56, 52, 32), (28, 13, 22), (7, 11, 104)])
+@pytest.mark.parametrize("prec", ["f64", "mixed"])
+def test_fft3d_runtime_lengths(shape, prec):
+    torch, OpawGPU = _api()
+    nx, ny, nz = shape
+    spec = synth.hpsi_system(n=shape, d=0.4, natom=0, seed=1)
+    g = OpawGPU(spec, prec, nsys=1, nb=2)
+    rng = np.random.default_rng(0)
+    f = rng.standard_normal((2, spec.n)) + 1j * rng.standard_normal((2, spec.n))
+    x = g.to_device(f)
+    fw = g.fft3d(x, -1).cpu().numpy()
+    bw = g.fft3d(x, +1).cpu().numpy()
+    ref_f = np.fft.fftn(f.reshape(2, nz, ny, nx), axes=(1, 2, 3)).reshape(2, -1)
+    ref_b = (np.fft.ifftn(f.reshape(2, nz, ny, nx), axes=(1, 2, 3)) * spec.n).reshape(2, -1)
+    tol = 1e-13 if prec == "f64" else 2e-6
+    assert rel(fw, ref_f) < tol
+    assert rel(bw, ref_b) < tol
+    g.close()
+
+
+def test_naphthalene_reference_grid_56_52_32():
+    """The grid of the reference's naphthalene input (nx 56, ny 52, nz 32 at
+    0.5 bohr): radix 7 and 13 line passes, MT Hartree on 112 x 104 x 64."""
+    torch, OpawGPU = _api()
+    spec = synth.molecule_system(nring=2, n=(56, 52, 32), d=0.5, seed=31)
+    nb = 4
+    spec.rnel = 2.0 * nb
+    st = state_from_spec(spec, nsys=2)
+    _potentials(spec, st, 7)
+    phi = _orbitals(spec, nb, seed=9, envelope=3.0)
+    g = OpawGPU(spec, "f64", nsys=2, nb=nb)
+    _push(g, st)
+    x = g.to_device(phi)
+    for sys in (0, 1):
+        assert rel(g.h_phi(x, sys).cpu().numpy(), O.h_phi(st, phi, sys)) < 1e-10
+        assert rel(g.shs(x, sys).cpu().numpy(), O.shs(st, phi, sys)) < 1e-10
+    g.close()
+    st2 = state_from_spec(spec, nsys=2)
+    phis = [phi, O.excite_psi_bar_pert(st2, phi, spec.sm, spec.ipol)]
+    new, d_dip, _, _ = O.tddft_step(st2, phis, spec.sm)
+    for prec, tol in (("f64", 1e-10), ("mixed", 2e-5)):
+        g = OpawGPU(spec, prec, nsys=2, nb=nb)
+        g.write_orbitals(0, phi)
+        g.excite_psi_bar_pert(spec.sm, spec.ipol + 1)
+        d_gpu, _ = g.opaw_tddft(1, spec.dt, spec.sm)
+        for s in (0, 1):
+            assert rel(g.read_orbitals(s).astype(np.complex128), new[s]) < tol, (prec, s)
+        if prec == "f64":
+            assert np.max(np.abs(d_gpu[0] - d_dip)) <= 1e-8 * np.max(np.abs(d_dip))
+        g.close()
