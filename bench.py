@@ -87,17 +87,20 @@ FAMILIES = ("fft_x", "fft_y", "fft_z", "fft_xh", "hartree", "box_proj", "box_add
 
 def class_bytes(spec, nsys, nb, sc):
     """Algorithmic HBM bytes per launch of the single-kernel families of one
-    shs / rebuild application over all nsys*nb fields (fft_x / fft_xh are the
-    xy-plane passes on these grids; rk4 averages its four stage variants,
-    which move 4, 5, 5 and 4 field arrays)."""
+    time step over all nsys*nb fields.  On these grids fft_x / fft_xh are the
+    xy-plane passes, and RK4 stages 1-3 ride in the inverse plane pass:
+    fft_xh averages its four launches (stage 1 reads the transformed field and
+    p, writes y and acc: 4 arrays; stages 2-3 also read y and acc: 6; stage 4
+    is plain H.psi, reading the transformed field and y and writing h: 3),
+    plus vks; rk4 is the stage-4 update alone (reads h, p, acc, writes p)."""
     N, nf, Nb, T = spec.n, nsys * nb, box_points(spec), table_bytes(spec)
     sr = sc // 2
     return {
         "fft_x": 2 * nf * N * sc,
         "fft_z": 2 * nf * N * sc + N * sr,
-        "fft_xh": 3 * nf * N * sc + nsys * N * sr,
+        "fft_xh": 4.75 * nf * N * sc + nsys * N * sr,
         "box_proj": nf * Nb * sc + T,
-        "rk4": 4.5 * nf * N * sc,
+        "rk4": 4 * nf * N * sc,
     }
 
 

@@ -60,6 +60,14 @@ hipError_t union_apply(const GridGeom& g, const BoxTables& bt, hipStream_t st, c
 template <typename R>
 hipError_t box_coeffs_dual(const GridGeom& g, const BoxTables& bt, hipStream_t st, const cx<R>* u, int nf,
                            double dv, double* part, c64* cA, const double* sn, c64* wA, c64* cB);
+// RK4 stage add-back of the weights w (p1 tables): kc = -i corr,
+// y += a1 kc, acc += a2 kc on the union of boxes
+template <typename R>
+hipError_t union_rk4_apply(const GridGeom& g, const BoxTables& bt, hipStream_t st, int nf, const c64* w, c64* corr,
+                           cx<R>* y, cx<R>* acc, double a1, double a2);
+// P1 coefficients and sn weights of fields stored only on the union of boxes
+hipError_t box_coeffs_compact(const GridGeom& g, const BoxTables& bt, hipStream_t st, const c64* vals, int nf,
+                              double dv, double* part, c64* ca, const double* sn, c64* w);
 // cB += O wA: projections of S^-1/2 u from those of u (fixed box overlaps)
 hipError_t box_bfix(const BoxTables& bt, hipStream_t st, int nf, const c64* wA, c64* cB);
 // union_corr added straight into the grid fields out (corr is scratch for the

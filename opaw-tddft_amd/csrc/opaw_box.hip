@@ -24,7 +24,10 @@ constexpr int kBoxThreads = 256;
 constexpr int kFieldsPerWG = 16;
 constexpr int kPointLanes = kBoxThreads / kFieldsPerWG;   // 16
 
-template <typename R, int MSP, bool PRE>
+// SRC: 0 = the grid fields u, 1 = u + cpre (compact correction),
+// 2 = the compact values cpre alone (fields stored only on the union of
+// boxes, cpre[k*nf + f]; u is not read)
+template <typename R, int MSP, int SRC>
 __global__ __launch_bounds__(kBoxThreads) void k_proj(const AtomDesc* __restrict__ atoms,
                                                       const int2* __restrict__ chunks, const int* __restrict__ bpt,
                                                       const int* __restrict__ bk, const double* __restrict__ tabs0,
@@ -50,7 +53,7 @@ __global__ __launch_bounds__(kBoxThreads) void k_proj(const AtomDesc* __restrict
   // compiler issues them together; masked points contribute zero and
   // projectors i >= ms and fields f >= nf are never written.
   const int fc = min(f, nf - 1);
-  const cx<R>* uf = u + (long long)fc * N;
+  const cx<R>* uf = (SRC != 2) ? u + (long long)fc * N : nullptr;
   int kk[PPT];
   bool ok[PPT];
 #pragma unroll
@@ -61,7 +64,7 @@ __global__ __launch_bounds__(kBoxThreads) void k_proj(const AtomDesc* __restrict
   }
   int pt[PPT];
 #pragma unroll
-  for (int q = 0; q < PPT; ++q) pt[q] = bpt[A.boff + kk[q]];
+  for (int q = 0; q < PPT; ++q) pt[q] = (SRC != 2) ? bpt[A.boff + kk[q]] : 0;
   double tv[PPT][MSP];
 #pragma unroll
   for (int q = 0; q < PPT; ++q) {
@@ -69,16 +72,21 @@ __global__ __launch_bounds__(kBoxThreads) void k_proj(const AtomDesc* __restrict
 #pragma unroll
     for (int i = 0; i < MSP; ++i) tv[q][i] = t[(long long)(i < A.ms ? i : 0) * A.npt];
   }
-  cx<R> v[PPT];
-#pragma unroll
-  for (int q = 0; q < PPT; ++q) v[q] = uf[pt[q]];
   double vr[PPT], vi[PPT];
+  if constexpr (SRC != 2) {
+    cx<R> v[PPT];
 #pragma unroll
-  for (int q = 0; q < PPT; ++q) {
-    vr[q] = ok[q] ? (double)v[q].x : 0.0;
-    vi[q] = ok[q] ? (double)v[q].y : 0.0;
+    for (int q = 0; q < PPT; ++q) v[q] = uf[pt[q]];
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+      vr[q] = ok[q] ? (double)v[q].x : 0.0;
+      vi[q] = ok[q] ? (double)v[q].y : 0.0;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) vr[q] = vi[q] = 0.0;
   }
-  if constexpr (PRE) {
+  if constexpr (SRC != 0) {
     int bq[PPT];
 #pragma unroll
     for (int q = 0; q < PPT; ++q) bq[q] = bk[A.boff + kk[q]];   // every box point is a union point
@@ -274,11 +282,14 @@ static void launch_proj(const GridGeom& g, const BoxTables& bt, hipStream_t st, 
                         double* part1 = nullptr) {
   const long long N = (long long)g.nx * g.ny * g.nz;
   dim3 grid(bt.nchunks, (nf + kFieldsPerWG - 1) / kFieldsPerWG, tab1 ? 2 : 1);
-  if (cpre)
-    hipLaunchKernelGGL((k_proj<R, MSP, true>), grid, dim3(kBoxThreads), 0, st, bt.atoms, bt.chunks, bt.bpt, bt.bk,
+  if (cpre && !u)
+    hipLaunchKernelGGL((k_proj<R, MSP, 2>), grid, dim3(kBoxThreads), 0, st, bt.atoms, bt.chunks, bt.bpt, bt.bk,
+                       tab, tab1, u, N, nf, cpre, bt.U.U, part, part1);
+  else if (cpre)
+    hipLaunchKernelGGL((k_proj<R, MSP, 1>), grid, dim3(kBoxThreads), 0, st, bt.atoms, bt.chunks, bt.bpt, bt.bk,
                        tab, tab1, u, N, nf, cpre, bt.U.U, part, part1);
   else
-    hipLaunchKernelGGL((k_proj<R, MSP, false>), grid, dim3(kBoxThreads), 0, st, bt.atoms, bt.chunks, bt.bpt, bt.bk,
+    hipLaunchKernelGGL((k_proj<R, MSP, 0>), grid, dim3(kBoxThreads), 0, st, bt.atoms, bt.chunks, bt.bpt, bt.bk,
                        tab, tab1, u, N, nf, cpre, bt.U.U, part, part1);
 }
 
@@ -423,14 +434,19 @@ hipError_t box_weights(const BoxTables& bt, hipStream_t st, const c64* ca, int n
 // (out_f(upt[k]) += corr(k, f)): the block's results are staged in LDS and
 // re-read point-fastest, so the read-modify-writes walk consecutive grid
 // points of one field.
-template <int FG, typename R, bool APPLY>
+template <int FG, typename R, int MODE>
 __global__ __launch_bounds__(kBoxThreads) void k_union_corr_lds(const int2* __restrict__ ublk,
                                                                 const int* __restrict__ pptr,
                                                                 const int* __restrict__ pc,
                                                                 const double* __restrict__ pv,
                                                                 const c64* __restrict__ w, int nf, int ctot,
                                                                 c64* __restrict__ corr, const int* __restrict__ upt,
-                                                                cx<R>* __restrict__ out, long long N) {
+                                                                cx<R>* __restrict__ out, long long N,
+                                                                cx<R>* __restrict__ out2, double a1, double a2) {
+  // MODE 0: corr(k, f) = sum; 1: out_f(upt[k]) += sum; 2 (RK4 stage
+  // add-back of the outer S^-1/2, linear in h): with kc = -i sum,
+  // out_f += a1 kc and out2_f += a2 kc at upt[k]
+  constexpr bool APPLY = MODE != 0;
   extern __shared__ __align__(16) double smem[];
   c64* wl = reinterpret_cast<c64*>(smem);                 // [ctot][FG]
   double* tl = smem + 2 * (long long)ctot * FG;           // pair values
@@ -517,9 +533,18 @@ __global__ __launch_bounds__(kBoxThreads) void k_union_corr_lds(const int2* __re
     for (int q = threadIdx.x; q < blk.y * FG; q += kBoxThreads) {
       const int r = q % blk.y, jj = q / blk.y;
       if (f0 + jj < nf) {
-        cx<R>& o = out[(long long)(f0 + jj) * N + upt[blk.x + r]];
+        const long long gi = (long long)(f0 + jj) * N + upt[blk.x + r];
         const c64 v = res[r * FG + jj];
-        o = mk<R>((R)((double)o.x + v.x), (R)((double)o.y + v.y));
+        if constexpr (MODE == 1) {
+          cx<R>& o = out[gi];
+          o = mk<R>((R)((double)o.x + v.x), (R)((double)o.y + v.y));
+        } else {
+          const double kr = v.y, ki = -v.x;
+          cx<R>& o = out[gi];
+          o = mk<R>((R)((double)o.x + a1 * kr), (R)((double)o.y + a1 * ki));
+          cx<R>& o2 = out2[gi];
+          o2 = mk<R>((R)((double)o2.x + a2 * kr), (R)((double)o2.y + a2 * ki));
+        }
       }
     }
   }
@@ -539,20 +564,21 @@ static int union_lds_fields(int ctot, bool apply) {
   return 0;
 }
 
-template <typename R, bool APPLY>
+template <typename R, int MODE>
 static void launch_union_lds(const BoxTables& bt, hipStream_t st, int fg, int nf, const double* pv, const c64* w,
-                             c64* corr, cx<R>* out, long long N) {
-  const size_t lds = union_lds_bytes(bt.ctot, fg, APPLY);
+                             c64* corr, cx<R>* out, long long N, cx<R>* out2 = nullptr, double a1 = 0.0,
+                             double a2 = 0.0) {
+  const size_t lds = union_lds_bytes(bt.ctot, fg, MODE != 0);
   const dim3 grid(bt.nublk, (nf + fg - 1) / fg);
   if (fg == 16)
-    hipLaunchKernelGGL((k_union_corr_lds<16, R, APPLY>), grid, dim3(kBoxThreads), lds, st, bt.ublk, bt.pptr, bt.pc,
-                       pv, w, nf, bt.ctot, corr, bt.U.upt, out, N);
+    hipLaunchKernelGGL((k_union_corr_lds<16, R, MODE>), grid, dim3(kBoxThreads), lds, st, bt.ublk, bt.pptr, bt.pc,
+                       pv, w, nf, bt.ctot, corr, bt.U.upt, out, N, out2, a1, a2);
   else if (fg == 8)
-    hipLaunchKernelGGL((k_union_corr_lds<8, R, APPLY>), grid, dim3(kBoxThreads), lds, st, bt.ublk, bt.pptr, bt.pc,
-                       pv, w, nf, bt.ctot, corr, bt.U.upt, out, N);
+    hipLaunchKernelGGL((k_union_corr_lds<8, R, MODE>), grid, dim3(kBoxThreads), lds, st, bt.ublk, bt.pptr, bt.pc,
+                       pv, w, nf, bt.ctot, corr, bt.U.upt, out, N, out2, a1, a2);
   else
-    hipLaunchKernelGGL((k_union_corr_lds<4, R, APPLY>), grid, dim3(kBoxThreads), lds, st, bt.ublk, bt.pptr, bt.pc,
-                       pv, w, nf, bt.ctot, corr, bt.U.upt, out, N);
+    hipLaunchKernelGGL((k_union_corr_lds<4, R, MODE>), grid, dim3(kBoxThreads), lds, st, bt.ublk, bt.pptr, bt.pc,
+                       pv, w, nf, bt.ctot, corr, bt.U.upt, out, N, out2, a1, a2);
 }
 
 hipError_t union_corr(const BoxTables& bt, hipStream_t st, int nf, int which, const c64* w, c64* corr) {
@@ -561,7 +587,7 @@ hipError_t union_corr(const BoxTables& bt, hipStream_t st, int nf, int which, co
   const double* pv = which ? bt.pv1 : bt.pv;
   const int fg = bt.nublk > 0 ? union_lds_fields(bt.ctot, false) : 0;
   if (fg > 0) {
-    launch_union_lds<double, false>(bt, st, fg, nf, pv, w, corr, nullptr, 0);
+    launch_union_lds<double, 0>(bt, st, fg, nf, pv, w, corr, nullptr, 0);
   } else {
     const int per = kBoxThreads / 64;
     hipLaunchKernelGGL(k_union_corr, dim3((bt.U.U + per - 1) / per), dim3(kBoxThreads), 0, st, bt.pptr, bt.pc, pv,
@@ -582,7 +608,72 @@ hipError_t union_corr_apply(const GridGeom& g, const BoxTables& bt, hipStream_t 
     return union_apply<R>(g, bt, st, out, nf, corr);
   }
   prof_begin(st, KC_BOX_ADD);
-  launch_union_lds<R, true>(bt, st, fg, nf, which ? bt.pv1 : bt.pv, w, corr, out, (long long)g.nx * g.ny * g.nz);
+  launch_union_lds<R, 1>(bt, st, fg, nf, which ? bt.pv1 : bt.pv, w, corr, out, (long long)g.nx * g.ny * g.nz);
+  prof_end(st, KC_BOX_ADD);
+  return hipGetLastError();
+}
+
+// RK4 stage add-back: with kc = -i corr(k, f) of the weights w, y += a1 kc
+// and acc += a2 kc on the union of boxes (fallback without union blocks:
+// corr then two scattered passes)
+template <typename R>
+__global__ __launch_bounds__(kBoxThreads) void k_union_rk4(const int* __restrict__ upt, int U, cx<R>* __restrict__ y,
+                                                           cx<R>* __restrict__ acc, long long N, int nf,
+                                                           const c64* __restrict__ corr, double a1, double a2) {
+  const int k = blockIdx.x * kBoxThreads + threadIdx.x;
+  const int f = blockIdx.y;
+  if (k >= U) return;
+  const c64 v = corr[(long long)k * nf + f];
+  const double kr = v.y, ki = -v.x;
+  const long long gi = (long long)f * N + upt[k];
+  const cx<R> a = y[gi];
+  y[gi] = mk<R>((R)((double)a.x + a1 * kr), (R)((double)a.y + a1 * ki));
+  const cx<R> b = acc[gi];
+  acc[gi] = mk<R>((R)((double)b.x + a2 * kr), (R)((double)b.y + a2 * ki));
+}
+
+template <typename R>
+hipError_t union_rk4_apply(const GridGeom& g, const BoxTables& bt, hipStream_t st, int nf, const c64* w, c64* corr,
+                           cx<R>* y, cx<R>* acc, double a1, double a2) {
+  if (bt.natom == 0 || nf == 0 || bt.U.U == 0) return hipSuccess;
+  const long long N = (long long)g.nx * g.ny * g.nz;
+  const int fg = bt.nublk > 0 ? union_lds_fields(bt.ctot, true) : 0;
+  if (fg == 0) {
+    hipError_t e = union_corr(bt, st, nf, 1, w, corr);
+    if (e) return e;
+    prof_begin(st, KC_BOX_ADD);
+    hipLaunchKernelGGL((k_union_rk4<R>), dim3((bt.U.U + kBoxThreads - 1) / kBoxThreads, nf), dim3(kBoxThreads), 0,
+                       st, bt.U.upt, bt.U.U, y, acc, N, nf, corr, a1, a2);
+    prof_end(st, KC_BOX_ADD);
+    return hipGetLastError();
+  }
+  prof_begin(st, KC_BOX_ADD);
+  launch_union_lds<R, 2>(bt, st, fg, nf, bt.pv1, w, corr, y, N, acc, a1, a2);
+  prof_end(st, KC_BOX_ADD);
+  return hipGetLastError();
+}
+
+// coefficients (P1, weights sn) of fields stored only on the union of boxes
+// (vals[k*nf + f])
+hipError_t box_coeffs_compact(const GridGeom& g, const BoxTables& bt, hipStream_t st, const c64* vals, int nf,
+                              double dv, double* part, c64* ca, const double* sn, c64* w) {
+  if (bt.natom == 0 || nf == 0) return hipSuccess;
+  prof_begin(st, KC_BOX_PROJ);
+  if (bt.msp == 8)
+    launch_proj<double, 8>(g, bt, st, nullptr, nf, bt.p1, vals, part);
+  else
+    launch_proj<double, 16>(g, bt, st, nullptr, nf, bt.p1, vals, part);
+  prof_end(st, KC_BOX_PROJ);
+  hipError_t e = hipGetLastError();
+  if (e) return e;
+  prof_begin(st, KC_BOX_ADD);
+  const dim3 fg(bt.natom, nf);
+  if (bt.msp == 8)
+    hipLaunchKernelGGL(k_proj_finish<8>, fg, dim3(256), 0, st, bt.atoms, bt.achunk, part, nf, bt.ctot, dv, ca, 1, sn,
+                       nullptr, nf, 0, w, nullptr, nullptr);
+  else
+    hipLaunchKernelGGL(k_proj_finish<16>, fg, dim3(256), 0, st, bt.atoms, bt.achunk, part, nf, bt.ctot, dv, ca, 1, sn,
+                       nullptr, nf, 0, w, nullptr, nullptr);
   prof_end(st, KC_BOX_ADD);
   return hipGetLastError();
 }
@@ -610,6 +701,10 @@ template hipError_t box_coeffs_dual<double>(const GridGeom&, const BoxTables&, h
                                             double*, c64*, const double*, c64*, c64*);
 template hipError_t box_coeffs_dual<float>(const GridGeom&, const BoxTables&, hipStream_t, const c32*, int, double,
                                            double*, c64*, const double*, c64*, c64*);
+template hipError_t union_rk4_apply<double>(const GridGeom&, const BoxTables&, hipStream_t, int, const c64*, c64*,
+                                            c64*, c64*, double, double);
+template hipError_t union_rk4_apply<float>(const GridGeom&, const BoxTables&, hipStream_t, int, const c64*, c64*,
+                                           c32*, c32*, double, double);
 template hipError_t union_corr_apply<double>(const GridGeom&, const BoxTables&, hipStream_t, int, int, const c64*,
                                              c64*, c64*);
 template hipError_t union_corr_apply<float>(const GridGeom&, const BoxTables&, hipStream_t, int, int, const c64*,

@@ -210,6 +210,7 @@ struct opaw_ctx {
   // compact corrections over the union of boxes (nf x U): S^-1/2 of the
   // input (A), nonlocal add-back (NL), S^-1/2 of H S^-1/2 input (C)
   DevBuf corrA, corrNL, corrC;
+  DevBuf hU;   // h on the union of boxes (fused RK4 stages)
   int coef_cap = 0;
   c64* pair = nullptr;
   c64* hwork = nullptr;
@@ -1164,7 +1165,7 @@ static int ensure_coef(opaw_ctx* c, int nf) {
   for (DevBuf* b : {&c->cA, &c->wA, &c->cB, &c->wB, &c->cC, &c->wC}) RCHK(ensure(c, *b, cb));
   RCHK(ensure(c, c->ppart, (size_t)2 * std::max(1, c->nchunks) * nf * c->msp * 2 * sizeof(double)));
   const size_t ub = (size_t)nf * std::max(1, c->U) * sizeof(c64);
-  for (DevBuf* b : {&c->corrA, &c->corrNL, &c->corrC}) RCHK(ensure(c, *b, ub));
+  for (DevBuf* b : {&c->corrA, &c->corrNL, &c->corrC, &c->hU}) RCHK(ensure(c, *b, ub));
   c->coef_cap = nf;
   return OPAW_OK;
 }
@@ -1286,6 +1287,36 @@ static int flow_hs(opaw_ctx* c, int sys0, const cx<R>* in, cx<R>* h, int nf, int
   return OPAW_OK;
 }
 
+// One RK4 stage 1-3 with the update fused into the last kinetic pass (plane
+// path): y = p + c k and acc = k (first) or acc += 2k, k = -i shs(yin), from
+// the A/B coefficients of yin.  The kinetic pass writes h only on the union
+// of boxes (hU); its C coefficients give the outer S^-1/2 add-back, which is
+// linear in h and is applied to y and acc afterwards.
+template <typename R>
+static int flow_hs_rk4(opaw_ctx* c, const cx<R>* yin, cx<R>* y, cx<R>* acc, const cx<R>* p, int nf, bool first,
+                       double cst) {
+  RCHK(ensure(c, c->wtmp, (size_t)nf * c->N * sizeof(cx<R>)));
+  const BoxTables bt = box_tables(c);
+  RCHK(fork_side(c));
+  HCHK(box_weights(bt, c->st2, (const c64*)c->cB.p, nf, 2, nullptr, c->dij, c->nb, c->mtot, (c64*)c->wB.p));
+  HCHK(union_corr(bt, c->st2, nf, 0, (const c64*)c->wB.p, (c64*)c->corrNL.p));
+  RCHK(mark_side(c));
+  PlaneRk4<R> rk;
+  rk.p = p;
+  rk.acc = acc;
+  rk.hU = (c64*)c->hU.p;
+  rk.c = cst;
+  rk.first = first ? 1 : 0;
+  HCHK(apply_t_plus_v<R>(c->g, c->st, yin, y, (cx<R>*)c->wtmp.p, nf, c->nb, Fields<R>::ek(c), Fields<R>::vks(c),
+                         TW<R>::get(c, c->nx), TW<R>::get(c, c->ny), TW<R>::get(c, c->nz), c->umap, c->upt, c->uzr,
+                         (const c64*)c->corrA.p, (const c64*)c->corrNL.p, nf, c->ev_join, &rk));
+  HCHK(box_coeffs_compact(c->g, bt, c->st, (const c64*)c->hU.p, nf, dvol(c), (double*)c->ppart.p, (c64*)c->cC.p,
+                          c->sn[1], (c64*)c->wC.p));
+  HCHK(union_rk4_apply<R>(c->g, bt, c->st, nf, (const c64*)c->wC.p, (c64*)c->corrC.p, y, acc, cst,
+                          first ? 1.0 : 2.0));
+  return OPAW_OK;
+}
+
 // out = S^-1/2 H S^-1/2 in
 template <typename R>
 static int flow_shs(opaw_ctx* c, int sys0, const cx<R>* in, cx<R>* out, int nf, int nf_per_sys) {
@@ -1368,8 +1399,15 @@ static int flow_rk4(opaw_ctx* c, double dt, double* dip_out) {
   HCHK(field_norms<R>(c->st, p, c->N, nf, dv, c->part, c->nrm0));
   prof_end(c->st, KC_DENS);
   const double cst[4] = {dt / 2.0, dt / 2.0, dt, dt};
+  const bool fused = plane_supported<R>(c->nx, c->ny);
   for (int stage = 1; stage <= 4; ++stage) {
     const cx<R>* yin = (stage == 1) ? p : y;
+    if (fused && stage < 4) {
+      // stages 1-3: the update rides in the last kinetic pass
+      RCHK(flow_hs_rk4<R>(c, yin, y, acc, p, nf, stage == 1, cst[stage - 1]));
+      RCHK(flow_rebuild<R>(c, 0, c->nsys, y));
+      continue;
+    }
     RCHK(flow_hs<R>(c, 0, yin, h, nf, c->nb));
     // stage 4 also forms the post-step norms (norm1) of every orbital
     prof_begin(c->st, KC_RK4);

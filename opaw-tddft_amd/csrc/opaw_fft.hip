@@ -52,7 +52,7 @@ struct SLines {
   int la, lb, sa, sb;       // windows along the transform axis
 };
 
-enum { EPI_NONE = 0, EPI_MULT = 1, EPI_HPSI = 2 };
+enum { EPI_NONE = 0, EPI_MULT = 1, EPI_HPSI = 2, EPI_RK4F = 3, EPI_RK4 = 4 };
 
 constexpr int kThreads = 256;
 
@@ -251,6 +251,12 @@ __global__ __launch_bounds__(kThreads) void k_fft_s(const cx<R>* in, View vin,
 //   EPI_NONE, S=-1: u = in (+ cpre) -> FFT_xy -> out
 //   EPI_HPSI, S=+1: in -> IFFT_xy -> out = result + vks * u + cnl,
 //                   u = uin (+ cpre)
+//   EPI_RK4F / EPI_RK4 (RK4 stage 1 / stages 2-3, src/rk4.f90:35-56): the
+//                   same h, then k = -i h, acc = k (or acc += 2k) and
+//                   out = p + c k; h itself is stored only on the union of
+//                   boxes, compactly in hU[k*U + f] (FP64), for the C
+//                   projection whose add-back is applied to acc and out
+//                   afterwards (it is linear in h)
 constexpr int kPlaneThreads = 512;
 
 template <typename R, int NX, int NY>
@@ -266,19 +272,23 @@ template <typename R, int NX, int NY, int S, int EPI>
 __global__ __launch_bounds__(kPlaneThreads) void k_fft_plane(const cx<R>* in, cx<R>* out, int nz, long long N,
                                                               const cx<R>* __restrict__ twxg,
                                                               const cx<R>* __restrict__ twyg,
-                                                              const cx<R>* __restrict__ uin,
+                                                              const cx<R>* uin,
                                                               const R* __restrict__ vks, int nf_per_sys,
                                                               const int* __restrict__ upt,
                                                               const int* __restrict__ uzr,
                                                               const c64* __restrict__ cpre,
-                                                              const c64* __restrict__ cnl, int U) {
+                                                              const c64* __restrict__ cnl, int U,
+                                                              PlaneRk4<R> rk) {
+  // uin may alias out (RK4 stages 2-3 update y in place: each element is read
+  // and written by the same thread), so neither is __restrict__
+  constexpr bool HP = EPI == EPI_HPSI || EPI == EPI_RK4F || EPI == EPI_RK4;
   constexpr int LS = plane_stride<R, NX, NY>();
   constexpr int NP = NX * NY;
   constexpr int PER = (NP + kPlaneThreads - 1) / kPlaneThreads;
   constexpr bool EXACT = NP % kPlaneThreads == 0;
   // the epilogue's global operands are loaded before the transforms when
   // they fit in a few registers, so their latency hides behind the FFT
-  constexpr bool EARLY = EPI == EPI_HPSI && PER * (3 * sizeof(R)) <= 96;
+  constexpr bool EARLY = HP && PER * (3 * sizeof(R)) <= 96;
   __shared__ cx<R> tile[NY * LS];
   __shared__ cx<R> twx[NX];
   __shared__ cx<R> twy[NY];
@@ -294,7 +304,7 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fft_plane(const cx<R>* in, cx
   auto tix = [](int idx) { return (idx / NX) * LS + idx % NX; };
   const R* vk = nullptr;
   const cx<R>* us = nullptr;
-  if constexpr (EPI == EPI_HPSI) {
+  if constexpr (HP) {
     vk = vks + (long long)(f / nf_per_sys) * N + plane0;
     us = uin + (long long)f * N + plane0;
   }
@@ -317,7 +327,8 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fft_plane(const cx<R>* in, cx
   }
   // union points of this plane: a contiguous range (the union is sorted by
   // grid point); forward pass: u = in + cpre there
-  const int k0 = (cpre || cnl) ? uzr[z] : 0, k1 = (cpre || cnl) ? uzr[z + 1] : 0;
+  const bool useu = cpre || cnl || (EPI == EPI_RK4F || EPI == EPI_RK4);
+  const int k0 = useu ? uzr[z] : 0, k1 = useu ? uzr[z + 1] : 0;
   if (EPI == EPI_NONE && cpre) {
     __syncthreads();
     for (int k = k0 + (int)threadIdx.x; k < k1; k += kPlaneThreads) {
@@ -336,7 +347,7 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fft_plane(const cx<R>* in, cx
     fft::lines<R, NX, NY, kPlaneThreads, false, LS, 1, S>(tile, twx);
   }
   cx<R>* dst = out + (long long)f * N + plane0;
-  if constexpr (EPI == EPI_HPSI) {
+  if constexpr (HP) {
     // h = T u + vks u + cnl with u = uin + cpre: the union terms
     // vks * cpre + cnl go into the tile first, then the plain epilogue
     if (cpre || cnl) {
@@ -359,25 +370,69 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fft_plane(const cx<R>* in, cx
       }
       __syncthreads();
     }
-    if constexpr (!EARLY) {
+    if constexpr (EPI == EPI_HPSI) {
+      if constexpr (!EARLY) {
 #pragma unroll
-      for (int q = 0; q < PER; ++q) {
-        uu[0] = us[lidx(q)];
-        vv[0] = vk[lidx(q)];
-        if (EXACT || slot(q) < NP) {
-          const cx<R> t = tile[tix(slot(q))];
-          const double w = (double)vv[0];
-          dst[slot(q)] = mk<R>((R)((double)t.x + w * (double)uu[0].x), (R)((double)t.y + w * (double)uu[0].y));
+        for (int q = 0; q < PER; ++q) {
+          uu[0] = us[lidx(q)];
+          vv[0] = vk[lidx(q)];
+          if (EXACT || slot(q) < NP) {
+            const cx<R> t = tile[tix(slot(q))];
+            const double w = (double)vv[0];
+            dst[slot(q)] = mk<R>((R)((double)t.x + w * (double)uu[0].x), (R)((double)t.y + w * (double)uu[0].y));
+          }
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          if (EXACT || slot(q) < NP) {
+            const cx<R> t = tile[tix(slot(q))];
+            const double w = (double)vv[q];
+            dst[slot(q)] = mk<R>((R)((double)t.x + w * (double)uu[q].x), (R)((double)t.y + w * (double)uu[q].y));
+          }
         }
       }
     } else {
+      // RK4 stage update in the epilogue
+      const cx<R>* pb = rk.p + (long long)f * N + plane0;
+      cx<R>* ab = rk.acc + (long long)f * N + plane0;
+      for (int q0 = 0; q0 < PER; q0 += 4) {
+        cx<R> u4[4], p4[4], a4[4];
+        R v4[4];
 #pragma unroll
-      for (int q = 0; q < PER; ++q) {
-        if (EXACT || slot(q) < NP) {
-          const cx<R> t = tile[tix(slot(q))];
-          const double w = (double)vv[q];
-          dst[slot(q)] = mk<R>((R)((double)t.x + w * (double)uu[q].x), (R)((double)t.y + w * (double)uu[q].y));
+        for (int s = 0; s < 4; ++s) {
+          const int q = min(q0 + s, PER - 1);
+          if constexpr (EARLY) {
+            u4[s] = uu[q];
+            v4[s] = vv[q];
+          } else {
+            u4[s] = us[lidx(q)];
+            v4[s] = vk[lidx(q)];
+          }
+          p4[s] = pb[lidx(q)];
+          if constexpr (EPI == EPI_RK4) a4[s] = ab[lidx(q)];
         }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int q = q0 + s;
+          if (q < PER && (EXACT || slot(q) < NP)) {
+            cx<R>& tt = tile[tix(slot(q))];
+            const double w = (double)v4[s];
+            const cx<R> h = mk<R>((R)((double)tt.x + w * (double)u4[s].x), (R)((double)tt.y + w * (double)u4[s].y));
+            tt = h;   // kept for the compact copy on the union of boxes
+            const double kr = (double)h.y, ki = -(double)h.x;
+            if constexpr (EPI == EPI_RK4F)
+              ab[slot(q)] = mk<R>((R)kr, (R)ki);
+            else
+              ab[slot(q)] = mk<R>((R)((double)a4[s].x + 2.0 * kr), (R)((double)a4[s].y + 2.0 * ki));
+            dst[slot(q)] = mk<R>((R)((double)p4[s].x + rk.c * kr), (R)((double)p4[s].y + rk.c * ki));
+          }
+        }
+      }
+      __syncthreads();
+      for (int k = k0 + (int)threadIdx.x; k < k1; k += kPlaneThreads) {
+        const cx<R> h = tile[tix((int)(upt[k] - plane0))];
+        rk.hU[(long long)k * U + f] = mk((double)h.x, (double)h.y);
       }
     }
   } else {
@@ -391,7 +446,7 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fft_plane(const cx<R>* in, cx
 #define OPAW_PLANE_SIZES(X) X(16) X(24) X(32) X(48) X(64) X(80) X(96)
 
 template <typename R>
-static bool plane_supported(int nx, int ny) {
+bool plane_supported(int nx, int ny) {
   if (nx != ny) return false;
   switch (nx) {
 #define OPAW_CASE(n) case n: return true;
@@ -401,24 +456,27 @@ static bool plane_supported(int nx, int ny) {
     default: return false;
   }
 }
+template bool plane_supported<double>(int, int);
+template bool plane_supported<float>(int, int);
 
 template <typename R, int S, int EPI>
 static hipError_t launch_plane(int n, hipStream_t st, const cx<R>* in, cx<R>* out, int nz, int nf, long long N,
                                const cx<R>* tw, const cx<R>* uin, const R* vks, int nfps, const int* upt,
-                               const int* uzr, const c64* cpre, const c64* cnl, int U) {
+                               const int* uzr, const c64* cpre, const c64* cnl, int U,
+                               PlaneRk4<R> rk = PlaneRk4<R>{}) {
   dim3 grid((unsigned)((long long)nf * nz));
   switch (n) {
 #define OPAW_CASE(nn)                                                                                          \
   case nn:                                                                                                     \
     hipLaunchKernelGGL((k_fft_plane<R, nn, nn, S, EPI>), grid, dim3(kPlaneThreads), 0, st, in, out, nz, N, tw, tw, \
-                       uin, vks, nfps, upt, uzr, cpre, cnl, U);                                                     \
+                       uin, vks, nfps, upt, uzr, cpre, cnl, U, rk);                                                     \
     return hipGetLastError();
     OPAW_PLANE_SIZES(OPAW_CASE)
 #undef OPAW_CASE
     case 128:
       if constexpr (sizeof(R) == 4) {
         hipLaunchKernelGGL((k_fft_plane<R, 128, 128, S, EPI>), grid, dim3(kPlaneThreads), 0, st, in, out, nz, N, tw,
-                           tw, uin, vks, nfps, upt, uzr, cpre, cnl, U);
+                           tw, uin, vks, nfps, upt, uzr, cpre, cnl, U, rk);
         return hipGetLastError();
       }
       return hipErrorInvalidValue;
@@ -566,7 +624,7 @@ hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* in, cx
                           int nf, int nf_per_sys, const R* ek, const R* vks,
                           const cx<R>* twx, const cx<R>* twy, const cx<R>* twz,
                           const int* umap, const int* upt, const int* uzr, const c64* cpre, const c64* cnl,
-                          int U, hipEvent_t before_inv) {
+                          int U, hipEvent_t before_inv, const PlaneRk4<R>* rk) {
   const long long N = (long long)g.nx * g.ny * g.nz;
   View v{N, 0, 0, 0, g.nx, g.ny};
   XLines Lx{0, g.ny, 0, g.nz, nf, 0, g.nx, 0, g.nx};
@@ -583,11 +641,19 @@ hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* in, cx
     if ((e = fft_s_pass<R>(KC_FFT_Z, g.nz, 2, -1, EPI_MULT, st, tmp, v, tmp, v, Lz, twz, ek, vt))) return e;
     if (before_inv && (e = hipStreamWaitEvent(st, before_inv, 0))) return e;
     prof_begin(st, KC_FFT_XH);
-    e = launch_plane<R, 1, EPI_HPSI>(g.nx, st, tmp, out, g.nz, nf, N, twx, in, vks, nf_per_sys, upt, uzr, cpre, cnl,
-                                     U);
+    if (rk == nullptr)
+      e = launch_plane<R, 1, EPI_HPSI>(g.nx, st, tmp, out, g.nz, nf, N, twx, in, vks, nf_per_sys, upt, uzr, cpre,
+                                       cnl, U);
+    else if (rk->first)
+      e = launch_plane<R, 1, EPI_RK4F>(g.nx, st, tmp, out, g.nz, nf, N, twx, in, vks, nf_per_sys, upt, uzr, cpre,
+                                       cnl, U, *rk);
+    else
+      e = launch_plane<R, 1, EPI_RK4>(g.nx, st, tmp, out, g.nz, nf, N, twx, in, vks, nf_per_sys, upt, uzr, cpre,
+                                      cnl, U, *rk);
     prof_end(st, KC_FFT_XH);
     return e;
   }
+  if (rk) return hipErrorInvalidValue;   // the RK4 epilogue exists on the plane path only
   if ((e = fft_x_pass<R>(KC_FFT_X, g.nx, -1, EPI_NONE, st, in, v, tmp, v, Lx, twx, nullptr, nullptr, 1, umap, cpre,
                          nullptr, U)))
     return e;
@@ -633,11 +699,11 @@ hipError_t hartree_conv(const GridGeom& g, int scale, hipStream_t st, const c64*
 template hipError_t apply_t_plus_v<double>(const GridGeom&, hipStream_t, const c64*, c64*, c64*, int, int,
                                            const double*, const double*, const c64*, const c64*, const c64*,
                                            const int*, const int*, const int*, const c64*, const c64*, int,
-                                           hipEvent_t);
+                                           hipEvent_t, const PlaneRk4<double>*);
 template hipError_t apply_t_plus_v<float>(const GridGeom&, hipStream_t, const c32*, c32*, c32*, int, int,
                                           const float*, const float*, const c32*, const c32*, const c32*,
                                           const int*, const int*, const int*, const c64*, const c64*, int,
-                                          hipEvent_t);
+                                          hipEvent_t, const PlaneRk4<float>*);
 
 // plain 3D transform of nf fields (used by tests and the Hartree-free paths)
 template <typename R>

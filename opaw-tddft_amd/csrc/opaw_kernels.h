@@ -17,6 +17,22 @@ void prof_end(hipStream_t st, int cls);
 
 bool fft_size_supported(int n);
 
+// RK4 stage fused into the last kinetic pass (plane path only): with it,
+// apply_t_plus_v writes y = p + c k and acc (= k, or += 2k when !first),
+// k = -i h, instead of h, and h on the union of boxes into hU[k*U + f]
+template <typename R>
+struct PlaneRk4 {
+  const cx<R>* p = nullptr;
+  cx<R>* acc = nullptr;
+  c64* hU = nullptr;
+  double c = 0.0;
+  int first = 0;
+};
+
+// whether the kinetic operator of an nx x ny grid runs on the xy-plane path
+template <typename R>
+bool plane_supported(int nx, int ny);
+
 // out_f = IFFT(ek FFT(u_f)) + vks(sys) u_f [+ cnl_f], where u_f = in_f [+ cpre_f]
 // (compact corrections over the union of boxes stored as corr[k*U + f]: U is
 // the row stride, i.e. the field count of the batch; umap maps a grid point to
@@ -29,7 +45,7 @@ hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* in, cx
                           int nf, int nf_per_sys, const R* ek, const R* vks,
                           const cx<R>* twx, const cx<R>* twy, const cx<R>* twz,
                           const int* umap, const int* upt, const int* uzr, const c64* cpre, const c64* cnl,
-                          int U, hipEvent_t before_inv);
+                          int U, hipEvent_t before_inv, const PlaneRk4<R>* rk = nullptr);
 
 template <typename R>
 hipError_t fft3d(const GridGeom& g, hipStream_t st, int dir, const cx<R>* in, cx<R>* out, int nf,
