@@ -253,7 +253,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_s(const cx<R>* in, View vin,
 //                   u = uin (+ cpre)
 //   EPI_RK4F / EPI_RK4 (RK4 stage 1 / stages 2-3, src/rk4.f90:35-56): the
 //                   same h, then k = -i h, acc = k (or acc += 2k) and
-//                   out = p + c k; h itself is stored only on the union of
+//                   out = p + c k (stage 1 transforms p itself: uin is p); h itself is stored only on the union of
 //                   boxes, compactly in hU[k*U + f] (FP64), for the C
 //                   projection whose add-back is applied to acc and out
 //                   afterwards (it is linear in h)
@@ -409,7 +409,11 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fft_plane(const cx<R>* in, cx
             u4[s] = us[lidx(q)];
             v4[s] = vk[lidx(q)];
           }
-          p4[s] = pb[lidx(q)];
+          // stage 1 transforms p itself (uin == p): no second read
+          if constexpr (EPI == EPI_RK4F)
+            p4[s] = u4[s];
+          else
+            p4[s] = pb[lidx(q)];
           if constexpr (EPI == EPI_RK4) a4[s] = ab[lidx(q)];
         }
 #pragma unroll

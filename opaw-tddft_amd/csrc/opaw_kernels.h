@@ -19,7 +19,8 @@ bool fft_size_supported(int n);
 
 // RK4 stage fused into the last kinetic pass (plane path only): with it,
 // apply_t_plus_v writes y = p + c k and acc (= k, or += 2k when !first),
-// k = -i h, instead of h, and h on the union of boxes into hU[k*U + f]
+// k = -i h, instead of h, and h on the union of boxes into hU[k*U + f].
+// With first, the transformed input must be p itself (p is not read again).
 template <typename R>
 struct PlaneRk4 {
   const cx<R>* p = nullptr;
