@@ -181,6 +181,45 @@ int opaw_orthog_phi(opaw_ctx* ctx, int sys, int h_type, double* eig);
  * (h_type 0); orbitals rotated in place, eig (nb) ascending. */
 int opaw_diagh(opaw_ctx* ctx, int sys, int h_type, double* eig);
 
+/* ---- on-disk formats (host code; no device work except the restart copies)
+ * Binary files are Fortran unformatted sequential files (4-byte record
+ * markers, 4-byte integers, 8-byte reals); text files are read with Fortran
+ * list-directed input.  Orbital arrays here are host complex*16, nstates
+ * fields of nx*ny*nz points, x fastest. */
+/* writewf (src/write_wf.f90:1-120; short_header = 0: wf_bar.bin / .txt with
+ * orbmn_wf and ihomo) or write_wf_for_sgw_debug (src/write_wf.f90:122-219;
+ * short_header = 1: the 8 header records that read_wf_tddft reads).  The
+ * stored orbital is sign(Re phi) |phi| (src/write_wf.f90:55-63). */
+int opaw_write_wf(const char* path, int binary, int short_header, int nx, int ny, int nz, double dx, double dy,
+                  double dz, int nstates, int ihomo, const double* eig, const void* phi);
+/* read_wf_tddft (src/read_wf.f90:130-228): checks the grid, the spacings and
+ * nsp = 1 against the caller's (src/read_wf.f90:181-187), accepts both header
+ * layouts above; returns nstates (<= max_states), ihomo (0 when the file has
+ * none), eig (nstates) and phi (nstates fields, imaginary part 0); eig, phi
+ * and ihomo may be NULL. */
+int opaw_read_wf(const char* path, int binary, int nx, int ny, int nz, double dx, double dy, double dz,
+                 int max_states, int* nstates, int* ihomo, double* eig, void* phi);
+/* eig / eig_bar: `write(1,*) eig` (src/diag.f90:109-115), read back with
+ * `read(1,*) eig` (src/write_wf.f90:33) */
+int opaw_write_eig(const char* path, int n, const double* eig);
+int opaw_read_eig(const char* path, int n, double* eig);
+/* plot_dip's files (src/tddft_plot.f90:100-105) for nt steps starting at
+ * it = it0+1: d_dip.dat lines `(' ',4e18.8)` of (it-1)*dt, d_dip(3) and
+ * dip.dat lines (it-1)*dt, dip(3*nsys) (list-directed); d_dip (nt,3) and
+ * dip (nt, 3*nsys) as returned by opaw_tddft_run.  append = 1 continues
+ * existing files; either path/array pair may be NULL. */
+int opaw_write_dipoles(const char* d_dip_path, const char* dip_path, int append, int it0, int nt, double dt,
+                       const double* d_dip, const double* dip, int nsys);
+/* write_to_restart (src/tddft_write_to_restart.f90:1-67): dir/info holds
+ * `nb nds it` (nds = 1: one process), dir/phi_bar.bin system 0 and
+ * dir/phi_bar_pert.bin system 1 (phi_bar_pert<s>.bin for further systems),
+ * one complex*16 record per orbital.  Creates dir when missing. */
+int opaw_write_restart(opaw_ctx* ctx, const char* dir, int it);
+/* read_phi_bar (src/read_phi_bar.f90:1-51): loads every system's orbitals
+ * from dir and returns it (the time loop restarts at it+1,
+ * src/tddft.f90:26-28); nb and nds of dir/info must match the context. */
+int opaw_read_restart(opaw_ctx* ctx, const char* dir, int* it);
+
 /* ---- profiling: HIP events around the launches of selected kernel families
  * mask: bit k enables family k (OPAW_K_*); 0 disables, -1 enables all. */
 int opaw_profile_enable(opaw_ctx* ctx, int mask);

@@ -1832,3 +1832,56 @@ int opaw_profile_read(opaw_ctx* c, int kclass, double* total_ms, long long* laun
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// services of the context to the file formats (opaw_io.hip): orbitals of one
+// system as complex*16 on the host, whatever the storage precision
+namespace opaw {
+
+int io_fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int ctx_shape(opaw_ctx* c, int* nsys, int* nb, long long* n) {
+  if (!c) return fail(OPAW_ERR_ARG, "null context");
+  *nsys = c->nsys;
+  *nb = c->nb;
+  *n = c->N;
+  return OPAW_OK;
+}
+
+int ctx_orbitals_to_host(opaw_ctx* c, int sys, std::vector<std::complex<double>>& out, int* nb, long long* n) {
+  if (!c) return fail(OPAW_ERR_ARG, "null context");
+  RCHK(check_sys(c, sys));
+  const size_t cnt = (size_t)c->nb * c->N;
+  out.resize(cnt);
+  if (c->prec == OPAW_PREC_F64) {
+    RCHK(opaw_read_orbitals(c, sys, out.data()));
+  } else {
+    std::vector<std::complex<float>> f(cnt);
+    RCHK(opaw_read_orbitals(c, sys, f.data()));
+    for (size_t k = 0; k < cnt; ++k) out[k] = std::complex<double>(f[k].real(), f[k].imag());
+  }
+  *nb = c->nb;
+  *n = c->N;
+  return OPAW_OK;
+}
+
+int ctx_orbitals_from_host(opaw_ctx* c, int sys, const std::vector<std::complex<double>>& in) {
+  if (!c) return fail(OPAW_ERR_ARG, "null context");
+  RCHK(check_sys(c, sys));
+  const size_t cnt = (size_t)c->nb * c->N;
+  if (in.size() < cnt) return fail(OPAW_ERR_ARG, "orbital buffer too small");
+  if (c->prec == OPAW_PREC_F64) return opaw_write_orbitals(c, sys, in.data());
+  std::vector<std::complex<float>> f(cnt);
+  for (size_t k = 0; k < cnt; ++k) f[k] = std::complex<float>((float)in[k].real(), (float)in[k].imag());
+  return opaw_write_orbitals(c, sys, f.data());
+}
+
+}  // namespace opaw

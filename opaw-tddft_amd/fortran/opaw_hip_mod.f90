@@ -102,6 +102,18 @@ module opaw_hip_mod
       integer(c_int), value :: from_sys, to_sys, ipol
       real(c_double), value :: sm
     end function
+    integer(c_int) function opaw_write_restart(ctx, dir, it) bind(C, name="opaw_write_restart")
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char) :: dir(*)
+      integer(c_int), value :: it
+    end function
+    integer(c_int) function opaw_read_restart(ctx, dir, it) bind(C, name="opaw_read_restart")
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char) :: dir(*)
+      integer(c_int) :: it
+    end function
     function opaw_last_error() bind(C, name="opaw_last_error")
       import :: c_ptr
       type(c_ptr) :: opaw_last_error
@@ -111,8 +123,9 @@ module opaw_hip_mod
   public :: opaw_create, opaw_destroy, opaw_ek_prep, opaw_set_atoms, opaw_set_projectors, opaw_set_sphere, &
             opaw_set_type, opaw_finalize_tables, opaw_set_potentials, opaw_vk_prep, opaw_write_orbitals, &
             opaw_read_orbitals, opaw_make_hamiltonian_tddft, opaw_rk4_prop_shs, opaw_plot_dip, &
-            opaw_excite_psi_bar_pert, opaw_check
-  public :: gpu_rk4_prop_shs, gpu_ncpaw_make_hamiltonian_tddft, gpu_plot_dip
+            opaw_excite_psi_bar_pert, opaw_write_restart, opaw_read_restart, opaw_check
+  public :: gpu_rk4_prop_shs, gpu_ncpaw_make_hamiltonian_tddft, gpu_plot_dip, gpu_write_to_restart, &
+            gpu_read_phi_bar
 
 contains
 
@@ -155,6 +168,23 @@ contains
     real(c_double), intent(in) :: sm
     real(c_double) :: d_dip(3), dip(3), dip_pert(3)
     call opaw_check(opaw_plot_dip(opaw_ctx, sm, d_dip, dip, dip_pert), 'plot_dip')
+  end subroutine
+
+  ! drop-in for write_to_restart (src/tddft_write_to_restart.f90:1): the
+  ! resident orbitals go to ./tddft_save
+  subroutine gpu_write_to_restart(it)
+    integer, intent(in) :: it
+    call opaw_check(opaw_write_restart(opaw_ctx, './tddft_save'//c_null_char, int(it, c_int)), &
+                    'write_to_restart')
+  end subroutine
+
+  ! drop-in for read_phi_bar (src/read_phi_bar.f90:1): loads ./tddft_save
+  ! into the resident orbitals and returns it_restart
+  subroutine gpu_read_phi_bar(it_restart)
+    integer, intent(out) :: it_restart
+    integer(c_int) :: it
+    call opaw_check(opaw_read_restart(opaw_ctx, './tddft_save'//c_null_char, it), 'read_phi_bar')
+    it_restart = it
   end subroutine
 
 end module opaw_hip_mod

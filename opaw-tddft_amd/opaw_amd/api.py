@@ -268,6 +268,17 @@ class OpawGPU:
         lib.call("opaw_diagh", self.ctx, sys, h_type, _dp(eig))
         return eig
 
+    # -------------------------------------------------------------- restart
+    def write_to_restart(self, directory: str, it: int):
+        """write_to_restart (src/tddft_write_to_restart.f90:1-67)."""
+        lib.call("opaw_write_restart", self.ctx, str(directory).encode(), int(it))
+
+    def read_phi_bar(self, directory: str) -> int:
+        """read_phi_bar (src/read_phi_bar.f90:1-51); returns it_restart."""
+        it = C.c_int()
+        lib.call("opaw_read_restart", self.ctx, str(directory).encode(), C.byref(it))
+        return it.value
+
     # -------------------------------------------------------------- profile
     def profile(self, classes=None):
         """classes: None / True = every kernel family, False / [] = off, or an

@@ -69,6 +69,13 @@ SIGNATURES = {
     "opaw_tddft_run": (I, [P, I, D, D, DP, DP]),
     "opaw_orthog_phi": (I, [P, I, I, DP]),
     "opaw_diagh": (I, [P, I, I, DP]),
+    "opaw_write_wf": (I, [C.c_char_p, I, I, I, I, I, D, D, D, I, I, DP, P]),
+    "opaw_read_wf": (I, [C.c_char_p, I, I, I, I, D, D, D, I, IP, IP, DP, P]),
+    "opaw_write_eig": (I, [C.c_char_p, I, DP]),
+    "opaw_read_eig": (I, [C.c_char_p, I, DP]),
+    "opaw_write_dipoles": (I, [C.c_char_p, C.c_char_p, I, I, I, D, DP, DP, I]),
+    "opaw_write_restart": (I, [P, C.c_char_p, I]),
+    "opaw_read_restart": (I, [P, C.c_char_p, IP]),
     "opaw_profile_enable": (I, [P, I]),
     "opaw_profile_reset": (I, [P]),
     "opaw_profile_read": (I, [P, I, DP, LLP]),
