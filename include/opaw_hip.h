@@ -220,6 +220,23 @@ int opaw_write_restart(opaw_ctx* ctx, const char* dir, int it);
  * src/tddft.f90:26-28); nb and nds of dir/info must match the context. */
 int opaw_read_restart(opaw_ctx* ctx, const char* dir, int* it);
 
+/* ---- absorption spectrum (post-processing of the d_dip series; the
+ * reference leaves it to external scripts, README.md:7-11).  For nsig host
+ * series d (series s at d + s*nt, sample n at t = n*dt, the rows of d_dip.dat):
+ *   alpha_s(w) = -dt * sum_n c_n f(t_n) d_s(t_n) exp(i w t_n),
+ *   c_n = 1 - [n = 0]/2 - [n = nt-1]/2 (trapezoid; 0 when nt = 1);
+ *   f = 1 (damp_kind 0), exp(-t/tau) (1),
+ *   exp(-t^2/(2 tau^2)) (2);  w_k = w0 + k*dw (Hartree), k < nw;
+ *   strength_s(w) = (2 w / pi) Im alpha_s(w).
+ * With the reference's kick phi exp(-i sm r) (src/tddft_excite.f90:11-12)
+ * and d = d_dip, alpha is the polarizability (the electrons' dipole is
+ * minus the density dipole of src/tddft_plot.f90:40-57) and strength > 0
+ * at absorption peaks.
+ * alpha (nsig, nw) complex interleaved and strength (nsig, nw) are host
+ * arrays; either may be NULL.  Runs on the current device, synchronously. */
+int opaw_spectrum(const double* d, int nsig, int nt, double dt, int damp_kind, double tau, double w0, double dw,
+                  int nw, double* alpha, double* strength);
+
 /* ---- profiling: HIP events around the launches of selected kernel families
  * mask: bit k enables family k (OPAW_K_*); 0 disables, -1 enables all. */
 int opaw_profile_enable(opaw_ctx* ctx, int mask);

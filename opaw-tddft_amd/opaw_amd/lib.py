@@ -76,6 +76,7 @@ SIGNATURES = {
     "opaw_write_dipoles": (I, [C.c_char_p, C.c_char_p, I, I, I, D, DP, DP, I]),
     "opaw_write_restart": (I, [P, C.c_char_p, I]),
     "opaw_read_restart": (I, [P, C.c_char_p, IP]),
+    "opaw_spectrum": (I, [DP, I, I, D, I, D, D, D, I, DP, DP]),
     "opaw_profile_enable": (I, [P, I]),
     "opaw_profile_reset": (I, [P]),
     "opaw_profile_read": (I, [P, I, DP, LLP]),
