@@ -237,6 +237,31 @@ int opaw_read_restart(opaw_ctx* ctx, const char* dir, int* it);
 int opaw_spectrum(const double* d, int nsig, int nt, double dt, int damp_kind, double tau, double w0, double dw,
                   int nw, double* alpha, double* strength);
 
+/* ---- PAW set-up, grid-sized part (prep_paw, src/prepare_paw.f90:1-49).
+ * Stateless; host arrays in and out; runs on the current device.
+ * get_local_p3d (src/p3d.f90:2-114) for one atom: for each of ms m-states,
+ * rad(nr, ms) / rad1(nr, ms) (Fortran layout: the radial ptilde / ptilde1 of
+ * the state's radial function) linearly interpolated in r and times the real
+ * Y_lm of lm(2, ms) = (l, m), sampled at the fine points
+ * corner + (i-1) dxf (nfine(3) of them, minimum image of the atom at coord
+ * in the box of half lengths half(3)), then contracted onto the rough box
+ * with the spline matrices bx(nrough(1), nfine(1)), by, bz of get_b
+ * (src/read_paw.f90:277-345).  Outputs local_p3d(nrough, ms) and
+ * local_p3d1 (before transform_p3d). */
+int opaw_paw_box(int nr, const double* rr, int ms, const double* rad, const double* rad1, const int* lm,
+                 const int* nrough, const int* nfine, const double* dxf, const double* corner, const double* coord,
+                 const double* half, const double* bx, const double* by, const double* bz, double* local_p3d,
+                 double* local_p3d1);
+/* map_localgrid (src/map_localgrid.f90:4-58) and get_localg3d
+ * (src/g3d.f90:1-119) for one atom: the grid points with (minimum-image)
+ * distance < rcut in grid order, local_grid(ngrid, 3) 1-based, and
+ * local_g3d(ngrid, nl2) = g_l(r) Y_L for r <= rcomp (else 0) from the shape
+ * functions gl(nrc, 2 nl - 1).  With max_points = 0 only ngrid is returned
+ * (size query); otherwise ngrid <= max_points is required. */
+int opaw_paw_sphere(int nx, int ny, int nz, double dx, double dy, double dz, const double* coord, double rcut,
+                    double rcomp, int nr, const double* rr, int nl2, int nrc, const double* gl, int max_points,
+                    int* ngrid, int* local_grid, double* local_g3d);
+
 /* ---- profiling: HIP events around the launches of selected kernel families
  * mask: bit k enables family k (OPAW_K_*); 0 disables, -1 enables all. */
 int opaw_profile_enable(opaw_ctx* ctx, int mask);

@@ -77,6 +77,8 @@ SIGNATURES = {
     "opaw_write_restart": (I, [P, C.c_char_p, I]),
     "opaw_read_restart": (I, [P, C.c_char_p, IP]),
     "opaw_spectrum": (I, [DP, I, I, D, I, D, D, D, I, DP, DP]),
+    "opaw_paw_box": (I, [I, DP, I, DP, DP, IP, IP, IP, DP, DP, DP, DP, DP, DP, DP, DP, DP]),
+    "opaw_paw_sphere": (I, [I, I, I, D, D, D, DP, D, D, I, DP, I, I, DP, I, IP, IP, DP]),
     "opaw_profile_enable": (I, [P, I]),
     "opaw_profile_reset": (I, [P]),
     "opaw_profile_read": (I, [P, I, DP, LLP]),
