@@ -262,6 +262,58 @@ int opaw_paw_sphere(int nx, int ny, int nz, double dx, double dy, double dz, con
                     double rcomp, int nr, const double* rr, int nl2, int nrc, const double* gl, int max_points,
                     int* ngrid, int* local_grid, double* local_g3d);
 
+/* ---- ground state (FP64 contexts) -----------------------------------------
+ * The SCF pieces of the reference's main loop (src/main.f90:66-95):
+ * ncpaw_make_hamiltonian, set_hminmax, chebyr_m, orthog_phi, diagh.
+ * h_type 1: the operator is shs (orthogonal orbitals), 0: h_phi. */
+/* orbitals 1..nocc of each system make its density and rho_ij
+ * (acc_dens1 / acc_coeff1, src/update_dens.f90:60-79); default nb.  The
+ * propagation entry points require nocc = nb. */
+int opaw_set_nocc(opaw_ctx* ctx, int nocc);
+/* DIIS history length ndi (src/0_parameter.f90:8, 4) and mixing mix_diis of
+ * the V_KS extrapolation; resets the history.  ndi = 0 disables it. */
+int opaw_set_diis(opaw_ctx* ctx, int ndi, double mix);
+/* ncpaw_make_hamiltonian (src/ncpaw_make_hamiltonian.f90:1-34, h_type 1):
+ * psi~ = S^-1/2 phi of the occupied orbitals, rho_ij, density, nhat with the
+ * rnel renormalisation, V_KS (get_pot, src/pot.f90:1-99) followed by
+ * diis_general (src/diis.f90:12-153) when use_diis, then D_ij. */
+int opaw_ncpaw_make_hamiltonian(opaw_ctx* ctx, int sys, int use_diis);
+/* get_hminmax (src/hminmax.f90:18-108): two power iterations of niter steps
+ * from random real starts (splitmix64 from seed), the interval widened by
+ * fctr (1.2 in the reference). */
+int opaw_hminmax(opaw_ctx* ctx, int sys, int h_type, int niter, double fctr, unsigned long long seed, double* hmin,
+                 double* hmax);
+/* chebyr_m (src/chebyr_m.f90:1-48): the degree-nch Chebyshev filter on
+ * [hbot, hmax] applied to all nb resident orbitals of system sys. */
+int opaw_chebyshev_filter(opaw_ctx* ctx, int sys, int h_type, int degree, double hbot, double hmax);
+/* res_i = |H phi_i - eig_i phi_i| (with dv) of the resident orbitals. */
+int opaw_residuals(opaw_ctx* ctx, int sys, int h_type, const double* eig, double* res);
+/* Block preconditioned eigensolver on the resident (orthonormal) orbitals;
+ * no counterpart in the reference, which has no preconditioner.  Per
+ * iteration: Rayleigh quotients, residuals r, w = M r in k space with
+ * M = TPA(x) exp(g(log(1+x), e/t)), x = |k|^2/2 / t, t the band kinetic
+ * energy, g the MLP of mlp (NULL: plain Teter-Payne-Allan), then
+ * Rayleigh-Ritz on [phi, w, p] (LOBPCG).  Stops when the residuals of the
+ * first nconv bands (0: all) are <= tol, or after maxit.
+ * eig (nb) out; hist (maxit) the largest residual per iteration (may be
+ * NULL); iters the iterations done.  record > 0 keeps (phi, r) every
+ * record-th iteration as training material. */
+int opaw_block_solve(opaw_ctx* ctx, int sys, int h_type, int maxit, double tol, int nconv, const double* mlp, int nh,
+                     int record, double* eig, double* hist, int* iters);
+/* MLP of the preconditioner: 2 -> nh -> nh -> 1 with tanh hidden layers;
+ * parameters W1 (nh x 2), b1, W2 (nh x nh), b2, w3 (nh), b3. */
+int opaw_mlp_param_count(int nh);
+/* Training pairs: prepare turns the recorded (phi_t, r_t) of system sys into
+ * (r_t(k), e_t(k)) with e_t = phi* (phi*^H phi_t) dv - phi_t, phi* the
+ * resident converged orbitals; loss = mean over pairs and bands of
+ * 1 - |<e, M r>|^2 / (|e|^2 |M r|^2) and its gradient w.r.t. the MLP
+ * parameters (grad may be NULL); pair downloads one prepared pair. */
+int opaw_precond_prepare(opaw_ctx* ctx, int sys);
+int opaw_precond_clear(opaw_ctx* ctx);
+int opaw_precond_count(opaw_ctx* ctx, int* nsnap, int* nready);
+int opaw_precond_loss(opaw_ctx* ctx, const double* mlp, int nh, double* loss, double* grad);
+int opaw_precond_pair(opaw_ctx* ctx, int idx, void* rk, void* ek, double* t, double* y, int* nb);
+
 /* ---- profiling: HIP events around the launches of selected kernel families
  * mask: bit k enables family k (OPAW_K_*); 0 disables, -1 enables all. */
 int opaw_profile_enable(opaw_ctx* ctx, int mask);

@@ -13,6 +13,7 @@
 
 #include "opaw_box.h"
 #include "opaw_fields.h"
+#include "opaw_gs.h"
 #include "opaw_hip.h"
 
 using namespace opaw;
@@ -218,6 +219,26 @@ struct opaw_ctx {
   c64* gmat = nullptr;
   c64* tmat = nullptr;
 
+  // orbitals 1..nocc of a system make its density and rho_ij (default nb)
+  int nocc = 0;
+  // ground-state solver: DIIS on V_KS per system, block-solver buffers and
+  // the recorded (residual, error) pairs that train the preconditioner
+  int diis_L = 4;
+  double diis_mix = 0.15;
+  struct Diis {
+    int K = 0, Kb = 0, stage = 0;
+    double *bank = nullptr, *err = nullptr;
+  };
+  std::vector<Diis> diis;
+  DevBuf gsY, gsHY, gsK, gsT, gsPart, gsG, gsM, gsSmall;
+  struct Snap {
+    DevBuf phi, r;     // phi_t and r_t; after prepare: r_t and e_t in k space
+    std::vector<double> t, y;
+    int sys = 0, nb = 0;
+    bool ready = false;
+  };
+  std::vector<Snap> snaps;
+
   size_t esize() const { return prec == OPAW_PREC_F64 ? sizeof(c64) : sizeof(c32); }
 };
 
@@ -331,6 +352,8 @@ int opaw_create(opaw_ctx** out, int nx, int ny, int nz, double dx, double dy, do
   c->prec = precision;
   c->nsys = nsys;
   c->nb = nb;
+  c->nocc = nb;
+  c->diis.resize(nsys);
   c->N = (long long)nx * ny * nz;
   c->scale = scale;
   c->g = GridGeom{nx, ny, nz};
@@ -1625,7 +1648,7 @@ int opaw_tddft_rhoij_2(opaw_ctx* c, int sys, const void* psi) {
   RCHK(check_sys(c, sys));
   if (!c->tables_ready) return fail(OPAW_ERR_STATE, "tables not finalised");
   if (!psi) return fail(OPAW_ERR_ARG, "null psi");
-  const int nb = c->nb;
+  const int nb = c->nocc;   // occupied orbitals only
   RCHK(ensure_coef(c, nb));
   c->prep_valid = false;
   RCHK(opaw_proj(c, psi, nb, 0, c->cB.p));
@@ -1639,9 +1662,9 @@ int opaw_tddft_dens_2(opaw_ctx* c, int sys, const void* psi) {
   RCHK(check_sys(c, sys));
   if (!psi) return fail(OPAW_ERR_ARG, "null psi");
   if (c->prec == OPAW_PREC_F64)
-    HCHK(density<double>(c->st, (const c64*)psi, c->N, 1, c->nb, c->dens + sys * c->N, nullptr, nullptr, 0));
+    HCHK(density<double>(c->st, (const c64*)psi, c->N, 1, c->nocc, c->dens + sys * c->N, nullptr, nullptr, 0));
   else
-    HCHK(density<float>(c->st, (const c32*)psi, c->N, 1, c->nb, c->dens + sys * c->N, nullptr, nullptr, 0));
+    HCHK(density<float>(c->st, (const c32*)psi, c->N, 1, c->nocc, c->dens + sys * c->N, nullptr, nullptr, 0));
   return OPAW_OK;
 }
 
@@ -1686,6 +1709,7 @@ int opaw_get_dij(opaw_ctx* c, int sys) {
 
 int opaw_make_hamiltonian_tddft(opaw_ctx* c, int sys) {
   CTX_CHECK(c);
+  if (c->nocc != c->nb) return fail(OPAW_ERR_STATE, "the propagation runs on occupied orbitals only (nocc = nb)");
   RCHK(need_tables(c));
   int s0 = 0, ns = c->nsys;
   if (sys >= 0) {
@@ -1717,6 +1741,7 @@ int opaw_excite_psi_bar_pert(opaw_ctx* c, int from_sys, int to_sys, double sm, i
 
 int opaw_rk4_prop_shs(opaw_ctx* c, double dt) {
   CTX_CHECK(c);
+  if (c->nocc != c->nb) return fail(OPAW_ERR_STATE, "the propagation runs on occupied orbitals only (nocc = nb)");
   RCHK(need_tables(c));
   if (c->prec == OPAW_PREC_F64) return flow_rk4<double>(c, dt, nullptr);
   return flow_rk4<float>(c, dt, nullptr);
@@ -1755,6 +1780,7 @@ int opaw_plot_dip(opaw_ctx* c, double sm, double* d_dip, double* dip, double* di
 
 int opaw_tddft_run(opaw_ctx* c, int nt, double dt, double sm, double* d_dip, double* dip) {
   CTX_CHECK(c);
+  if (c->nocc != c->nb) return fail(OPAW_ERR_STATE, "the propagation runs on occupied orbitals only (nocc = nb)");
   RCHK(need_tables(c));
   if (nt <= 0) return fail(OPAW_ERR_ARG, "nt must be positive");
   if (c->dipcap < nt) {
@@ -1885,3 +1911,692 @@ int ctx_orbitals_from_host(opaw_ctx* c, int sys, const std::vector<std::complex<
 }
 
 }  // namespace opaw
+
+// ---------------------------------------------------------------------------
+// ground state (FP64 contexts): the SCF pieces of the reference's main loop
+// (src/main.f90:66-95) and a block preconditioned solver whose k-space
+// preconditioner carries a learned correction
+static int need_f64(opaw_ctx* c) {
+  if (c->prec != OPAW_PREC_F64) return fail(OPAW_ERR_ARG, "the ground-state solver runs in FP64 contexts");
+  return OPAW_OK;
+}
+
+static double* gs_part(opaw_ctx* c, size_t count, int* rc) {
+  *rc = ensure(c, c->gsPart, count * sizeof(double));
+  return (double*)c->gsPart.p;
+}
+
+// dense solve B x = rhs (partial pivoting), B row-major n x n
+static bool small_solve(int n, std::vector<double> B, std::vector<double>& x) {
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    for (int i = k + 1; i < n; ++i)
+      if (std::fabs(B[(size_t)i * n + k]) > std::fabs(B[(size_t)p * n + k])) p = i;
+    if (B[(size_t)p * n + k] == 0.0) return false;
+    if (p != k) {
+      for (int j = 0; j < n; ++j) std::swap(B[(size_t)k * n + j], B[(size_t)p * n + j]);
+      std::swap(x[k], x[p]);
+    }
+    for (int i = k + 1; i < n; ++i) {
+      const double f = B[(size_t)i * n + k] / B[(size_t)k * n + k];
+      for (int j = k; j < n; ++j) B[(size_t)i * n + j] -= f * B[(size_t)k * n + j];
+      x[i] -= f * x[k];
+    }
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    double s = x[k];
+    for (int j = k + 1; j < n; ++j) s -= B[(size_t)k * n + j] * x[j];
+    x[k] = s / B[(size_t)k * n + k];
+  }
+  return true;
+}
+
+// diis_general (src/diis.f90:12-153) on V_KS of system sys
+static int diis_step(opaw_ctx* c, int sys) {
+  auto& D = c->diis[sys];
+  const int L = c->diis_L;
+  const long long N = c->N;
+  double* vec = c->vks + (size_t)sys * N;
+  const size_t fb = (size_t)N * sizeof(double);
+  if (!D.bank) {
+    RCHK(dalloc_t(c, &D.bank, (size_t)L * N));
+    RCHK(dalloc_t(c, &D.err, (size_t)L * N));
+  }
+  if (D.stage == 0) {
+    D.stage = 1;
+    D.K = 0;
+    D.Kb = 1;
+    HCHK(hipMemcpyAsync(D.bank, vec, fb, hipMemcpyDeviceToDevice, c->st));
+    return OPAW_OK;
+  }
+  // stackerr
+  if (D.K == L)
+    for (int k = 0; k < L - 1; ++k)
+      HCHK(hipMemcpyAsync(D.err + (size_t)k * N, D.err + (size_t)(k + 1) * N, fb, hipMemcpyDeviceToDevice, c->st));
+  D.K = std::min(D.K + 1, L);
+  const int K = D.K;
+  HCHK(sub_real(c->st, vec, D.bank + (size_t)(D.Kb - 1) * N, N, D.err + (size_t)(K - 1) * N));
+  // minimize: [B 1; 1 0] C = e_M
+  int rc;
+  double* part = gs_part(c, (size_t)kGsChunks * K * K, &rc);
+  RCHK(rc);
+  HCHK(real_dots(c->st, D.err, N, K, part));
+  std::vector<double> hp((size_t)kGsChunks * K * K);
+  HCHK(hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HCHK(hipStreamSynchronize(c->st));
+  const int M = K + 1;
+  std::vector<double> B((size_t)M * M, 0.0), C(M, 0.0);
+  for (int i = 0; i < K; ++i)
+    for (int j = 0; j < K; ++j) {
+      double s = 0.0;
+      for (int ch = 0; ch < kGsChunks; ++ch) s += hp[((size_t)ch * K + i) * K + j];
+      B[(size_t)i * M + j] = s * dvol(c);
+    }
+  for (int i = 0; i < K; ++i) B[(size_t)i * M + K] = B[(size_t)K * M + i] = 1.0;
+  C[K] = 1.0;
+  if (!small_solve(M, B, C)) return fail(OPAW_ERR_ARG, "DIIS matrix is singular");
+  double sc = 0.0;
+  for (int i = 0; i < K; ++i) sc += C[i];
+  for (int i = 0; i < K; ++i) C[i] /= sc;
+  RCHK(ensure(c, c->gsSmall, (size_t)M * sizeof(double)));
+  HCHK(hipMemcpyAsync(c->gsSmall.p, C.data(), K * sizeof(double), hipMemcpyHostToDevice, c->st));
+  // predict_output, then bankstack
+  HCHK(diis_predict(c->st, D.bank, D.err, (const double*)c->gsSmall.p, K, c->diis_mix, N, vec));
+  if (D.Kb == L)
+    for (int k = 0; k < L - 1; ++k)
+      HCHK(hipMemcpyAsync(D.bank + (size_t)k * N, D.bank + (size_t)(k + 1) * N, fb, hipMemcpyDeviceToDevice, c->st));
+  if (D.Kb != K) return fail(OPAW_ERR_STATE, "DIIS bank and error stacks out of step (Kb %d, K %d)", D.Kb, K);
+  D.Kb = std::min(D.Kb + 1, L);
+  HCHK(hipMemcpyAsync(D.bank + (size_t)(D.Kb - 1) * N, vec, fb, hipMemcpyDeviceToDevice, c->st));
+  HCHK(hipStreamSynchronize(c->st));
+  return OPAW_OK;
+}
+
+// 1 x 1 Gram of two single fields: dv <a|b>
+static int dot1(opaw_ctx* c, const c64* a, const c64* b, std::complex<double>* out) {
+  HCHK(gram<double>(c->st, a, b, c->N, 1, dvol(c), c->gpart, c->gmat));
+  HCHK(hipMemcpyAsync(out, c->gmat, sizeof(c64), hipMemcpyDeviceToHost, c->st));
+  HCHK(hipStreamSynchronize(c->st));
+  return OPAW_OK;
+}
+
+// S^-1/2 H S^-1/2 (h_type 1) or H (h_type 0) of nf fields of system sys
+static int apply_h(opaw_ctx* c, int sys, int h_type, const c64* in, c64* out, int nf) {
+  if (h_type == 1) return flow_shs<double>(c, sys, in, out, nf, nf);
+  return flow_h_phi<double>(c, sys, in, out, nf, nf);
+}
+
+// generalised eigenproblem H C = S C w on an m-dimensional basis (row-major
+// host matrices); directions with overlap below 1e-12 of the largest are
+// dropped; returns the lowest nout eigenpairs, C column-major m x nout
+static int subspace_eig(int m, const std::vector<std::complex<double>>& S0, const std::vector<std::complex<double>>& H0,
+                        int nout, std::vector<std::complex<double>>& Ccol, std::vector<double>& w) {
+  // unit-norm columns first (D S D, D H D): small search directions are not
+  // linearly dependent ones, and the rank test below must not drop them
+  std::vector<double> dsc(m);
+  for (int i = 0; i < m; ++i) {
+    const double sii = S0[(size_t)i * m + i].real();
+    dsc[i] = sii > 0.0 ? 1.0 / std::sqrt(sii) : 0.0;
+  }
+  std::vector<std::complex<double>> S((size_t)m * m), H((size_t)m * m);
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j) {
+      S[(size_t)i * m + j] = S0[(size_t)i * m + j] * (dsc[i] * dsc[j]);
+      H[(size_t)i * m + j] = H0[(size_t)i * m + j] * (dsc[i] * dsc[j]);
+    }
+  std::vector<double> ws;
+  std::vector<std::complex<double>> Vs;
+  herm_eig(m, S, ws, Vs);
+  const double top = ws.back();
+  std::vector<int> keep;
+  for (int k = 0; k < m; ++k)
+    if (ws[k] > 1e-12 * top) keep.push_back(k);
+  const int r = (int)keep.size();
+  if (r < nout) return fail(OPAW_ERR_ARG, "subspace rank %d below the %d bands", r, nout);
+  std::vector<std::complex<double>> X((size_t)m * r);   // row-major m x r
+  for (int i = 0; i < m; ++i)
+    for (int k = 0; k < r; ++k) X[(size_t)i * r + k] = Vs[(size_t)i * m + keep[k]] / std::sqrt(ws[keep[k]]);
+  std::vector<std::complex<double>> HX((size_t)m * r), Hr((size_t)r * r);
+  for (int i = 0; i < m; ++i)
+    for (int k = 0; k < r; ++k) {
+      std::complex<double> s = 0.0;
+      for (int j = 0; j < m; ++j) s += H[(size_t)i * m + j] * X[(size_t)j * r + k];
+      HX[(size_t)i * r + k] = s;
+    }
+  for (int a = 0; a < r; ++a)
+    for (int b = 0; b < r; ++b) {
+      std::complex<double> s = 0.0;
+      for (int i = 0; i < m; ++i) s += std::conj(X[(size_t)i * r + a]) * HX[(size_t)i * r + b];
+      Hr[(size_t)a * r + b] = s;
+    }
+  for (int a = 0; a < r; ++a)
+    for (int b = a + 1; b < r; ++b) {
+      const std::complex<double> h = 0.5 * (Hr[(size_t)a * r + b] + std::conj(Hr[(size_t)b * r + a]));
+      Hr[(size_t)a * r + b] = h;
+      Hr[(size_t)b * r + a] = std::conj(h);
+    }
+  std::vector<double> wh;
+  std::vector<std::complex<double>> U;
+  herm_eig(r, Hr, wh, U);
+  Ccol.assign((size_t)m * nout, 0.0);
+  w.assign(wh.begin(), wh.begin() + nout);
+  for (int j = 0; j < nout; ++j)
+    for (int i = 0; i < m; ++i) {
+      std::complex<double> s = 0.0;
+      for (int k = 0; k < r; ++k) s += X[(size_t)i * r + k] * U[(size_t)k * r + j];
+      Ccol[i + (size_t)m * j] = s * dsc[i];
+    }
+  return OPAW_OK;
+}
+
+// gram of m fields (row-major m x m on the host), with the solver's buffers
+static int gram_m(opaw_ctx* c, const c64* a, const c64* b, int m, std::vector<std::complex<double>>& out) {
+  RCHK(ensure(c, c->gsG, (size_t)kGramChunks * m * m * sizeof(c64)));
+  RCHK(ensure(c, c->gsM, (size_t)m * m * sizeof(c64)));
+  HCHK(gram<double>(c->st, a, b, c->N, m, dvol(c), (c64*)c->gsG.p, (c64*)c->gsM.p));
+  out.resize((size_t)m * m);
+  HCHK(hipMemcpyAsync(out.data(), c->gsM.p, (size_t)m * m * sizeof(c64), hipMemcpyDeviceToHost, c->st));
+  HCHK(hipStreamSynchronize(c->st));
+  return OPAW_OK;
+}
+
+// per-band kinetic energy sum_k ek |f(k)|^2 / sum_k |f(k)|^2 of nf k-space fields
+static int band_kinetic(opaw_ctx* c, const c64* fk, int nf, std::vector<double>& t) {
+  int rc;
+  double* part = gs_part(c, (size_t)kGsChunks * nf * 2, &rc);
+  RCHK(rc);
+  HCHK(kinetic_partials(c->st, fk, c->ek, c->N, nf, part));
+  std::vector<double> h((size_t)kGsChunks * nf * 2);
+  HCHK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HCHK(hipStreamSynchronize(c->st));
+  t.assign(nf, 0.0);
+  for (int i = 0; i < nf; ++i) {
+    double a = 0.0, b = 0.0;
+    for (int ch = 0; ch < kGsChunks; ++ch) {
+      a += h[((size_t)ch * nf + i) * 2];
+      b += h[((size_t)ch * nf + i) * 2 + 1];
+    }
+    t[i] = (b > 0.0 ? a / b : 0.0) * (double)c->N;   // ek carries 1/N
+    if (!(t[i] > 1e-6)) t[i] = 1e-6;
+  }
+  return OPAW_OK;
+}
+
+static int upload_small(opaw_ctx* c, size_t offset_doubles, const std::vector<double>& v) {
+  RCHK(ensure(c, c->gsSmall, (offset_doubles + v.size()) * sizeof(double)));
+  HCHK(hipMemcpyAsync((double*)c->gsSmall.p + offset_doubles, v.data(), v.size() * sizeof(double),
+                      hipMemcpyHostToDevice, c->st));
+  return OPAW_OK;
+}
+
+extern "C" {
+
+int opaw_set_nocc(opaw_ctx* c, int nocc) {
+  CTX_CHECK(c);
+  if (nocc < 1 || nocc > c->nb) return fail(OPAW_ERR_ARG, "nocc must be in 1..nb");
+  c->nocc = nocc;
+  return OPAW_OK;
+}
+
+int opaw_set_diis(opaw_ctx* c, int ndi, double mix) {
+  CTX_CHECK(c);
+  if (ndi < 0 || ndi > 64) return fail(OPAW_ERR_ARG, "ndi must be in 0..64");
+  for (auto& D : c->diis) {
+    dfree(c, D.bank);
+    dfree(c, D.err);
+    D = opaw_ctx::Diis();
+  }
+  c->diis_L = ndi;
+  c->diis_mix = mix;
+  return OPAW_OK;
+}
+
+int opaw_ncpaw_make_hamiltonian(opaw_ctx* c, int sys, int use_diis) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  RCHK(need_f64(c));
+  RCHK(need_tables(c));
+  const long long N = c->N;
+  const int no = c->nocc;
+  RCHK(ensure(c, c->wu, (size_t)no * N * sizeof(c64)));
+  const c64* phi = (const c64*)c->phi + (size_t)sys * c->nb * N;
+  RCHK(flow_sn_phi<double>(c, phi, (c64*)c->wu.p, no, -0.5));
+  RCHK(opaw_tddft_rhoij_2(c, sys, c->wu.p));
+  RCHK(opaw_tddft_dens_2(c, sys, c->wu.p));
+  RCHK(opaw_get_nhat(c, sys));
+  RCHK(opaw_get_pot_tddft_2(c, sys));
+  if (use_diis && c->diis_L > 0) RCHK(diis_step(c, sys));
+  if (c->vksf) HCHK(to_f32(c->st, c->vks + (size_t)sys * N, c->vksf + (size_t)sys * N, N));
+  RCHK(opaw_get_dij(c, sys));
+  c->prep_valid = false;
+  HCHK(hipStreamSynchronize(c->st));
+  return OPAW_OK;
+}
+
+int opaw_hminmax(opaw_ctx* c, int sys, int h_type, int niter, double fctr, unsigned long long seed, double* hmin,
+                 double* hmax) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  RCHK(need_f64(c));
+  RCHK(need_tables(c));
+  if (niter < 1 || !hmin || !hmax) return fail(OPAW_ERR_ARG, "bad hminmax arguments");
+  const long long N = c->N;
+  RCHK(ensure(c, c->gsT, 2 * (size_t)N * sizeof(c64)));
+  c64* z = (c64*)c->gsT.p;
+  c64* hz = z + N;
+  std::vector<std::complex<double>> host((size_t)N);
+  double e[2] = {0.0, 0.0};
+  for (int pass = 0; pass < 2; ++pass) {
+    unsigned long long x = seed + 0x9E3779B97F4A7C15ull * (unsigned long long)(pass + 1);
+    for (long long r = 0; r < N; ++r) {   // splitmix64, uniform in [0, 1)
+      unsigned long long v = (x += 0x9E3779B97F4A7C15ull);
+      v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ull;
+      v = (v ^ (v >> 27)) * 0x94D049BB133111EBull;
+      v ^= v >> 31;
+      host[(size_t)r] = std::complex<double>((double)(v >> 11) * 0x1.0p-53, 0.0);
+    }
+    HCHK(hipMemcpy(z, host.data(), (size_t)N * sizeof(c64), hipMemcpyHostToDevice));
+    for (int it = 0; it < niter; ++it) {
+      std::complex<double> nz;
+      RCHK(dot1(c, z, z, &nz));
+      HCHK(scale_complex(c->st, z, N, 1.0 / std::sqrt(nz.real())));
+      RCHK(apply_h(c, sys, h_type, z, hz, 1));
+      if (pass == 1) HCHK(axpy_complex(c->st, hz, z, e[0], N, hz));
+      std::complex<double> q, n2;
+      RCHK(dot1(c, z, hz, &q));
+      RCHK(dot1(c, z, z, &n2));
+      e[pass] = q.real() / n2.real();
+      HCHK(hipMemcpyAsync(z, hz, (size_t)N * sizeof(c64), hipMemcpyDeviceToDevice, c->st));
+    }
+  }
+  // get_hminmax (src/hminmax.f90:89-98)
+  const double e2 = e[1] + e[0];
+  double hx = std::max(e[0], e2), hn = std::min(e[0], e2);
+  const double havg = 0.5 * (hn + hx), dh = (hx - hn) * fctr;
+  *hmax = havg + dh / 2.0;
+  *hmin = havg - dh / 2.0;
+  return OPAW_OK;
+}
+
+int opaw_chebyshev_filter(opaw_ctx* c, int sys, int h_type, int degree, double hbot, double hmax) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  RCHK(need_f64(c));
+  RCHK(need_tables(c));
+  if (degree < 1 || !(hmax > hbot)) return fail(OPAW_ERR_ARG, "bad Chebyshev filter arguments");
+  const long long N = c->N;
+  const int nb = c->nb;
+  const size_t fb = (size_t)nb * N * sizeof(c64);
+  RCHK(ensure(c, c->gsY, 3 * fb));
+  c64* phi = (c64*)c->phi + (size_t)sys * nb * N;
+  c64* A = (c64*)c->gsY.p;
+  c64* B = A + (size_t)nb * N;
+  c64* Cc = B + (size_t)nb * N;
+  HCHK(hipMemcpyAsync(A, phi, fb, hipMemcpyDeviceToDevice, c->st));
+  HCHK(hipMemcpyAsync(B, phi, fb, hipMemcpyDeviceToDevice, c->st));
+  const double havg = 0.5 * (hmax + hbot), dh = 0.5 * (hmax - hbot);
+  for (int k = 1; k <= degree; ++k) {
+    RCHK(apply_h(c, sys, h_type, B, Cc, nb));
+    HCHK(cheb_combine<double>(c->st, Cc, B, A, (long long)nb * N, havg, dh, k == 1));
+    c64* t = A;
+    A = B;
+    B = Cc;
+    Cc = t;
+  }
+  HCHK(hipMemcpyAsync(phi, B, fb, hipMemcpyDeviceToDevice, c->st));
+  HCHK(hipStreamSynchronize(c->st));
+  c->prep_valid = false;
+  return OPAW_OK;
+}
+
+int opaw_residuals(opaw_ctx* c, int sys, int h_type, const double* eig, double* res) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  RCHK(need_f64(c));
+  RCHK(need_tables(c));
+  if (!eig || !res) return fail(OPAW_ERR_ARG, "null argument");
+  const long long N = c->N;
+  const int nb = c->nb;
+  RCHK(ensure(c, c->gsHY, (size_t)nb * N * sizeof(c64)));
+  const c64* phi = (const c64*)c->phi + (size_t)sys * nb * N;
+  c64* hp = (c64*)c->gsHY.p;
+  RCHK(apply_h(c, sys, h_type, phi, hp, nb));
+  RCHK(upload_small(c, 0, std::vector<double>(eig, eig + nb)));
+  int rc;
+  double* part = gs_part(c, (size_t)kGsChunks * nb, &rc);
+  RCHK(rc);
+  HCHK(residual_partials<double>(c->st, hp, phi, (const double*)c->gsSmall.p, N, nb, part));
+  std::vector<double> h((size_t)kGsChunks * nb);
+  HCHK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HCHK(hipStreamSynchronize(c->st));
+  for (int i = 0; i < nb; ++i) {
+    double s = 0.0;
+    for (int ch = 0; ch < kGsChunks; ++ch) s += h[(size_t)ch * nb + i];
+    res[i] = std::sqrt(s * dvol(c));
+  }
+  return OPAW_OK;
+}
+
+// Block preconditioned eigensolver (LOBPCG) on the resident orbitals of
+// system sys (h_type 1: shs; the orbitals must be orthonormal).  Each
+// iteration: Rayleigh quotients e_i, residuals r_i = H phi_i - e_i phi_i,
+// w_i = M_i r_i in k space (Teter-Payne-Allan ratio of x = |k|^2/2 / t_i, t_i
+// the band's kinetic energy, times exp(g(log(1+x), e_i/t_i)) when an MLP is
+// given), then Rayleigh-Ritz on [phi, w, p] with p the previous step's
+// (w, p) part of the new vectors.  Stops when the residual norms of the
+// first nconv bands are <= tol; hist[it] = their largest at iteration it.
+int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, int nconv, const double* mlp, int nh,
+                     int record, double* eig, double* hist, int* iters) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  RCHK(need_f64(c));
+  RCHK(need_tables(c));
+  if (maxit < 1 || !eig) return fail(OPAW_ERR_ARG, "bad block_solve arguments");
+  MlpShape shape;
+  shape.nh = nh;
+  if (mlp && (nh < 1 || nh > 16)) return fail(OPAW_ERR_ARG, "MLP hidden width must be 1..16");
+  const long long N = c->N;
+  const int nb = c->nb;
+  if (nconv <= 0 || nconv > nb) nconv = nb;
+  const size_t fb = (size_t)nb * N * sizeof(c64);
+  RCHK(ensure(c, c->gsY, 3 * fb));
+  RCHK(ensure(c, c->gsHY, 3 * fb));
+  RCHK(ensure(c, c->gsK, fb));
+  RCHK(ensure(c, c->gsT, 4 * fb));
+  c64* phi = (c64*)c->phi + (size_t)sys * nb * N;
+  c64* Y = (c64*)c->gsY.p;        // [phi | w | p]
+  c64* HY = (c64*)c->gsHY.p;      // H of the same
+  c64* Kb = (c64*)c->gsK.p;
+  c64* T0 = (c64*)c->gsT.p;
+  c64* T1 = T0 + (size_t)nb * N;
+  c64* T2 = T1 + (size_t)nb * N;
+  c64* T3 = T2 + (size_t)nb * N;
+  const size_t off = (size_t)nb * N;
+  const c64 *twx = c->tw64[c->nx], *twy = c->tw64[c->ny], *twz = c->tw64[c->nz];
+  const int npar = mlp_param_count(shape);
+  double* dmlp = nullptr;
+  RCHK(ensure(c, c->gsSmall, (size_t)(3 * nb + (mlp ? npar : 0)) * sizeof(double)));
+  if (mlp) {
+    dmlp = (double*)c->gsSmall.p + 3 * nb;
+    HCHK(hipMemcpy(dmlp, mlp, npar * sizeof(double), hipMemcpyHostToDevice));
+  }
+  double* deig = (double*)c->gsSmall.p;
+  double* dt = deig + nb;
+  double* dy = dt + nb;
+  HCHK(hipMemcpyAsync(Y, phi, fb, hipMemcpyDeviceToDevice, c->st));
+  RCHK(apply_h(c, sys, h_type, Y, HY, nb));
+  std::vector<double> e(nb), t, y(nb);
+  bool have_p = false;
+  int it = 0;
+  for (; it < maxit; ++it) {
+    std::vector<std::complex<double>> G;
+    RCHK(gram_m(c, Y, HY, nb, G));
+    for (int i = 0; i < nb; ++i) e[i] = G[(size_t)i * nb + i].real();
+    HCHK(hipMemcpyAsync(deig, e.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->st));
+    c64* W = Y + off;
+    HCHK(residual_vectors(c->st, HY, Y, deig, N, nb, W));
+    int rc;
+    double* part = gs_part(c, (size_t)kGsChunks * nb, &rc);
+    RCHK(rc);
+    HCHK(residual_partials<double>(c->st, HY, Y, deig, N, nb, part));
+    std::vector<double> h((size_t)kGsChunks * nb);
+    HCHK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HCHK(hipStreamSynchronize(c->st));
+    double worst = 0.0;
+    for (int i = 0; i < nconv; ++i) {
+      double sm = 0.0;
+      for (int ch = 0; ch < kGsChunks; ++ch) sm += h[(size_t)ch * nb + i];
+      worst = std::max(worst, std::sqrt(sm * dvol(c)));
+    }
+    if (hist) hist[it] = worst;
+    if (worst <= tol) break;
+    // band kinetic energies and the preconditioned residuals
+    HCHK(fft3d<double>(c->g, c->st, -1, Y, Kb, nb, twx, twy, twz));
+    RCHK(band_kinetic(c, Kb, nb, t));
+    for (int i = 0; i < nb; ++i) y[i] = e[i] / t[i];
+    HCHK(hipMemcpyAsync(dt, t.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HCHK(hipMemcpyAsync(dy, y.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->st));
+    if (record > 0 && it % record == 0) {
+      opaw_ctx::Snap sp;
+      sp.sys = sys;
+      sp.nb = nb;
+      sp.t = t;
+      sp.y = y;
+      RCHK(ensure(c, sp.phi, fb));
+      RCHK(ensure(c, sp.r, fb));
+      HCHK(hipMemcpyAsync(sp.phi.p, Y, fb, hipMemcpyDeviceToDevice, c->st));
+      HCHK(hipMemcpyAsync(sp.r.p, W, fb, hipMemcpyDeviceToDevice, c->st));
+      c->snaps.push_back(sp);
+    }
+    HCHK(fft3d<double>(c->g, c->st, -1, W, Kb, nb, twx, twy, twz));
+    HCHK(precond_apply(c->st, Kb, N, nb, c->ek, dt, dy, dmlp, shape));
+    HCHK(fft3d<double>(c->g, c->st, 1, Kb, W, nb, twx, twy, twz));
+    RCHK(apply_h(c, sys, h_type, W, HY + off, nb));
+    // Rayleigh-Ritz on [phi, w, p] (LOBPCG; no p in the first iteration)
+    const int m = have_p ? 3 * nb : 2 * nb;
+    std::vector<std::complex<double>> S, H, Ccol;
+    RCHK(gram_m(c, Y, Y, m, S));
+    RCHK(gram_m(c, Y, HY, m, H));
+    for (int a = 0; a < m; ++a)
+      for (int b = a + 1; b < m; ++b) {
+        const std::complex<double> hs = 0.5 * (S[(size_t)a * m + b] + std::conj(S[(size_t)b * m + a]));
+        S[(size_t)a * m + b] = hs;
+        S[(size_t)b * m + a] = std::conj(hs);
+      }
+    std::vector<double> w;
+    RCHK(subspace_eig(m, S, H, nb, Ccol, w));
+    // C (m x nb) and its lower block C[nb:m] ((m - nb) x nb), column-major
+    std::vector<std::complex<double>> Cp((size_t)(m - nb) * nb);
+    for (int j = 0; j < nb; ++j)
+      for (int i = nb; i < m; ++i) Cp[(i - nb) + (size_t)(m - nb) * j] = Ccol[i + (size_t)m * j];
+    RCHK(ensure(c, c->gsM, (size_t)(m + (m - nb)) * nb * sizeof(c64)));
+    c64* dC = (c64*)c->gsM.p;
+    c64* dCp = dC + (size_t)m * nb;
+    HCHK(hipMemcpyAsync(dC, Ccol.data(), Ccol.size() * sizeof(c64), hipMemcpyHostToDevice, c->st));
+    HCHK(hipMemcpyAsync(dCp, Cp.data(), Cp.size() * sizeof(c64), hipMemcpyHostToDevice, c->st));
+    HCHK(rotate_rect(c->st, Y, T0, N, m, nb, dC));
+    HCHK(rotate_rect(c->st, HY, T1, N, m, nb, dC));
+    HCHK(rotate_rect(c->st, Y + off, T2, N, m - nb, nb, dCp));
+    HCHK(rotate_rect(c->st, HY + off, T3, N, m - nb, nb, dCp));
+    HCHK(hipMemcpyAsync(Y, T0, fb, hipMemcpyDeviceToDevice, c->st));
+    HCHK(hipMemcpyAsync(HY, T1, fb, hipMemcpyDeviceToDevice, c->st));
+    HCHK(hipMemcpyAsync(Y + 2 * off, T2, fb, hipMemcpyDeviceToDevice, c->st));
+    HCHK(hipMemcpyAsync(HY + 2 * off, T3, fb, hipMemcpyDeviceToDevice, c->st));
+    have_p = true;
+    e = w;
+  }
+  if (iters) *iters = it;
+  HCHK(hipMemcpyAsync(phi, Y, fb, hipMemcpyDeviceToDevice, c->st));
+  HCHK(hipStreamSynchronize(c->st));
+  std::copy(e.begin(), e.end(), eig);
+  c->prep_valid = false;
+  return OPAW_OK;
+}
+
+// Turns the recorded (phi_t, r_t) of system sys into training pairs with the
+// resident (converged, orthonormal) orbitals phi*: e_t = phi* (phi*^H phi_t dv)
+// - phi_t, the part of phi_t the step should remove; r_t and e_t go to k space.
+int opaw_precond_prepare(opaw_ctx* c, int sys) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  RCHK(need_f64(c));
+  const long long N = c->N;
+  const int nb = c->nb;
+  const size_t fb = (size_t)nb * N * sizeof(c64);
+  const c64* ref = (const c64*)c->phi + (size_t)sys * nb * N;
+  RCHK(ensure(c, c->gsT, 2 * fb));
+  c64* T0 = (c64*)c->gsT.p;
+  const c64 *twx = c->tw64[c->nx], *twy = c->tw64[c->ny], *twz = c->tw64[c->nz];
+  for (auto& sp : c->snaps) {
+    if (sp.ready || sp.sys != sys) continue;
+    std::vector<std::complex<double>> G;   // G[a][j] = dv <ref_a | phi_j>
+    RCHK(ensure(c, c->gsG, (size_t)kGramChunks * nb * nb * sizeof(c64)));
+    RCHK(ensure(c, c->gsM, (size_t)nb * nb * sizeof(c64)));
+    HCHK(gram<double>(c->st, ref, (const c64*)sp.phi.p, N, nb, dvol(c), (c64*)c->gsG.p, (c64*)c->gsM.p));
+    // rotate_rect wants T[a + nb j] = G[a][j]: the row-major gram is that transposed
+    G.resize((size_t)nb * nb);
+    HCHK(hipMemcpyAsync(G.data(), c->gsM.p, G.size() * sizeof(c64), hipMemcpyDeviceToHost, c->st));
+    HCHK(hipStreamSynchronize(c->st));
+    std::vector<std::complex<double>> Tc((size_t)nb * nb);
+    for (int a = 0; a < nb; ++a)
+      for (int j = 0; j < nb; ++j) Tc[a + (size_t)nb * j] = G[(size_t)a * nb + j];
+    HCHK(hipMemcpyAsync(c->gsM.p, Tc.data(), Tc.size() * sizeof(c64), hipMemcpyHostToDevice, c->st));
+    HCHK(rotate_rect(c->st, ref, T0, N, nb, nb, (const c64*)c->gsM.p));
+    HCHK(axpy_complex(c->st, T0, (const c64*)sp.phi.p, 1.0, (long long)nb * N, T0));
+    // k space: phi slot <- e_t(k), r slot <- r_t(k)
+    HCHK(fft3d<double>(c->g, c->st, -1, T0, (c64*)sp.phi.p, nb, twx, twy, twz));
+    HCHK(fft3d<double>(c->g, c->st, -1, (const c64*)sp.r.p, T0, nb, twx, twy, twz));
+    HCHK(hipMemcpyAsync(sp.r.p, T0, fb, hipMemcpyDeviceToDevice, c->st));
+    sp.ready = true;
+  }
+  HCHK(hipStreamSynchronize(c->st));
+  return OPAW_OK;
+}
+
+int opaw_precond_clear(opaw_ctx* c) {
+  CTX_CHECK(c);
+  for (auto& sp : c->snaps) {
+    dfree(c, sp.phi.p);
+    dfree(c, sp.r.p);
+  }
+  c->snaps.clear();
+  return OPAW_OK;
+}
+
+int opaw_precond_count(opaw_ctx* c, int* nsnap, int* nready) {
+  CTX_CHECK(c);
+  int r = 0;
+  for (auto& sp : c->snaps) r += sp.ready ? 1 : 0;
+  if (nsnap) *nsnap = (int)c->snaps.size();
+  if (nready) *nready = r;
+  return OPAW_OK;
+}
+
+// Loss of the preconditioner over the prepared pairs and its gradient:
+// per pair and band, loss = 1 - |<e, M r>|^2 / (|e|^2 |M r|^2) (k space; a
+// scale-free misalignment, since Rayleigh-Ritz only sees the direction of
+// M r); loss = mean over pairs and bands with |e| above 1e-12 of the largest.
+// grad (mlp_param_count) may be NULL.
+int opaw_precond_loss(opaw_ctx* c, const double* mlp, int nh, double* loss, double* grad) {
+  CTX_CHECK(c);
+  RCHK(need_f64(c));
+  if (!mlp || !loss || nh < 1 || nh > 16) return fail(OPAW_ERR_ARG, "bad precond_loss arguments");
+  MlpShape shape;
+  shape.nh = nh;
+  const int npar = mlp_param_count(shape);
+  const long long N = c->N;
+  int nready = 0;
+  int nbmax = 0;
+  for (auto& sp : c->snaps)
+    if (sp.ready) {
+      ++nready;
+      nbmax = std::max(nbmax, sp.nb);
+    }
+  if (nready == 0) return fail(OPAW_ERR_STATE, "no prepared training pairs");
+  constexpr int kGradChunks = 512;
+  RCHK(ensure(c, c->gsSmall, (size_t)(npar + 5 * nbmax) * sizeof(double)));
+  double* dP = (double*)c->gsSmall.p;
+  double* dt = dP + npar;
+  double* dy = dt + nbmax;
+  double* dA = dy + nbmax;        // 2 per band
+  double* dB = dA + 2 * nbmax;
+  HCHK(hipMemcpy(dP, mlp, npar * sizeof(double), hipMemcpyHostToDevice));
+  int rc;
+  double* part = gs_part(c, std::max((size_t)kGsChunks * nbmax * 4, (size_t)kGradChunks * npar), &rc);
+  RCHK(rc);
+  // first pass: per-band sums, to find the largest |e| and count the bands
+  struct Band { double ar, ai, b, ee; };
+  std::vector<std::vector<Band>> sums;
+  double eemax = 0.0;
+  for (auto& sp : c->snaps) {
+    if (!sp.ready) continue;
+    const int nb = sp.nb;
+    HCHK(hipMemcpyAsync(dt, sp.t.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HCHK(hipMemcpyAsync(dy, sp.y.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HCHK(precond_loss_sums(c->st, (const c64*)sp.r.p, (const c64*)sp.phi.p, N, nb, c->ek, dt, dy, dP, shape, part));
+    std::vector<double> h((size_t)kGsChunks * nb * 4);
+    HCHK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HCHK(hipStreamSynchronize(c->st));
+    std::vector<Band> bs(nb, Band{0, 0, 0, 0});
+    for (int i = 0; i < nb; ++i) {
+      for (int ch = 0; ch < kGsChunks; ++ch) {
+        const double* q = &h[((size_t)ch * nb + i) * 4];
+        bs[i].ar += q[0];
+        bs[i].ai += q[1];
+        bs[i].b += q[2];
+        bs[i].ee += q[3];
+      }
+      eemax = std::max(eemax, bs[i].ee);
+    }
+    sums.push_back(bs);
+  }
+  int count = 0;
+  for (auto& bs : sums)
+    for (auto& b : bs)
+      if (b.ee > 1e-12 * eemax && b.b > 0.0) ++count;
+  if (count == 0) return fail(OPAW_ERR_STATE, "every training pair has a zero error vector");
+  double L = 0.0;
+  std::vector<double> g(npar, 0.0);
+  int si = 0;
+  for (auto& sp : c->snaps) {
+    if (!sp.ready) continue;
+    const auto& bs = sums[si++];
+    const int nb = sp.nb;
+    std::vector<double> cA(2 * nb, 0.0), cB(nb, 0.0);
+    for (int i = 0; i < nb; ++i) {
+      const Band& b = bs[i];
+      if (!(b.ee > 1e-12 * eemax && b.b > 0.0)) continue;
+      const double a2 = b.ar * b.ar + b.ai * b.ai;
+      L += 1.0 - a2 / (b.ee * b.b);
+      // dL/dg = [-2 Re(conj(A) conj(e) r) + 2 |A|^2 m |r|^2 / B] m / (Ee B), over count
+      const double s = 1.0 / (b.ee * b.b * count);
+      cA[2 * i] = -2.0 * b.ar * s;       // conj(A) = ar - i ai: Re(conj(A) z) = ar Re z + ai Im z
+      cA[2 * i + 1] = 2.0 * b.ai * s;    // kernel forms cA0 Re z - cA1 Im z
+      cB[i] = 2.0 * a2 / b.b * s;
+    }
+    if (!grad) continue;
+    HCHK(hipMemcpyAsync(dt, sp.t.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HCHK(hipMemcpyAsync(dy, sp.y.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HCHK(hipMemcpyAsync(dA, cA.data(), 2 * nb * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HCHK(hipMemcpyAsync(dB, cB.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HCHK(precond_grad(c->st, (const c64*)sp.r.p, (const c64*)sp.phi.p, N, nb, c->ek, dt, dy, dP, shape, dA, dB, part,
+                      kGradChunks));
+    std::vector<double> h((size_t)kGradChunks * npar);
+    HCHK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HCHK(hipStreamSynchronize(c->st));
+    for (int ch = 0; ch < kGradChunks; ++ch)
+      for (int p = 0; p < npar; ++p) g[p] += h[(size_t)ch * npar + p];
+  }
+  *loss = L / count;
+  if (grad) std::copy(g.begin(), g.end(), grad);
+  return OPAW_OK;
+}
+
+// prepared pair idx (0-based among the prepared ones): r(k) and e(k) of its
+// nb bands (complex*16, nb x N, k space, unnormalised forward FFT), the band
+// kinetic energies t and features y; any output may be NULL
+int opaw_precond_pair(opaw_ctx* c, int idx, void* rk, void* ek, double* t, double* y, int* nb) {
+  CTX_CHECK(c);
+  int k = -1;
+  for (auto& sp : c->snaps) {
+    if (!sp.ready) continue;
+    if (++k != idx) continue;
+    const size_t fb = (size_t)sp.nb * c->N * sizeof(c64);
+    HCHK(hipStreamSynchronize(c->st));
+    if (rk) HCHK(hipMemcpy(rk, sp.r.p, fb, hipMemcpyDeviceToHost));
+    if (ek) HCHK(hipMemcpy(ek, sp.phi.p, fb, hipMemcpyDeviceToHost));
+    if (t) std::copy(sp.t.begin(), sp.t.end(), t);
+    if (y) std::copy(sp.y.begin(), sp.y.end(), y);
+    if (nb) *nb = sp.nb;
+    return OPAW_OK;
+  }
+  return fail(OPAW_ERR_ARG, "no prepared pair %d", idx);
+}
+
+int opaw_mlp_param_count(int nh) {
+  MlpShape s;
+  s.nh = nh;
+  return mlp_param_count(s);
+}
+
+}  // extern "C"

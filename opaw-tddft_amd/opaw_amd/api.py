@@ -268,6 +268,70 @@ class OpawGPU:
         lib.call("opaw_diagh", self.ctx, sys, h_type, _dp(eig))
         return eig
 
+    # ---------------------------------------------------------- ground state
+    def set_nocc(self, nocc: int):
+        lib.call("opaw_set_nocc", self.ctx, int(nocc))
+
+    def set_diis(self, ndi: int = 4, mix: float = 0.15):
+        lib.call("opaw_set_diis", self.ctx, int(ndi), float(mix))
+
+    def ncpaw_make_hamiltonian(self, sys: int = 0, diis: bool = True):
+        lib.call("opaw_ncpaw_make_hamiltonian", self.ctx, sys, int(diis))
+
+    def hminmax(self, sys: int = 0, h_type: int = 1, niter: int = 25, fctr: float = 1.2, seed: int = 12345):
+        lo, hi = C.c_double(), C.c_double()
+        lib.call("opaw_hminmax", self.ctx, sys, h_type, int(niter), float(fctr), C.c_ulonglong(seed),
+                 C.byref(lo), C.byref(hi))
+        return lo.value, hi.value
+
+    def chebyshev_filter(self, hbot: float, hmax: float, degree: int = 10, sys: int = 0, h_type: int = 1):
+        lib.call("opaw_chebyshev_filter", self.ctx, sys, h_type, int(degree), float(hbot), float(hmax))
+
+    def residuals(self, eig, sys: int = 0, h_type: int = 1):
+        e = np.ascontiguousarray(eig, dtype=np.float64)
+        r = np.zeros(self.nb)
+        lib.call("opaw_residuals", self.ctx, sys, h_type, _dp(e), _dp(r))
+        return r
+
+    def block_solve(self, maxit: int, tol: float, mlp=None, nh: int = 16, record: int = 0, sys: int = 0,
+                    h_type: int = 1, nconv: int = 0):
+        eig = np.zeros(self.nb)
+        hist = np.zeros(maxit)
+        its = C.c_int()
+        p = None if mlp is None else np.ascontiguousarray(mlp, dtype=np.float64)
+        lib.call("opaw_block_solve", self.ctx, sys, h_type, int(maxit), float(tol), int(nconv),
+                 None if p is None else _dp(p), int(nh), int(record), _dp(eig), _dp(hist), C.byref(its))
+        return eig, hist[:max(its.value + (its.value < maxit), 1)], its.value
+
+    def precond_prepare(self, sys: int = 0):
+        lib.call("opaw_precond_prepare", self.ctx, sys)
+
+    def precond_clear(self):
+        lib.call("opaw_precond_clear", self.ctx)
+
+    def precond_count(self):
+        a, b = C.c_int(), C.c_int()
+        lib.call("opaw_precond_count", self.ctx, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def precond_loss(self, mlp, nh: int = 16, grad: bool = True):
+        p = np.ascontiguousarray(mlp, dtype=np.float64)
+        loss = C.c_double()
+        g = np.zeros_like(p) if grad else None
+        lib.call("opaw_precond_loss", self.ctx, _dp(p), int(nh), C.byref(loss), None if g is None else _dp(g))
+        return loss.value, g
+
+    def precond_pair(self, idx: int):
+        nb = C.c_int()
+        lib.call("opaw_precond_pair", self.ctx, int(idx), None, None, None, None, C.byref(nb))
+        rk = np.zeros((nb.value, self.N), np.complex128)
+        ek = np.zeros_like(rk)
+        t = np.zeros(nb.value)
+        y = np.zeros(nb.value)
+        lib.call("opaw_precond_pair", self.ctx, int(idx), rk.ctypes.data_as(C.c_void_p),
+                 ek.ctypes.data_as(C.c_void_p), _dp(t), _dp(y), C.byref(nb))
+        return rk, ek, t, y
+
     # -------------------------------------------------------------- restart
     def write_to_restart(self, directory: str, it: int):
         """write_to_restart (src/tddft_write_to_restart.f90:1-67)."""

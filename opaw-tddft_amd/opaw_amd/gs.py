@@ -1,0 +1,134 @@
+"""Ground-state SCF on the GPU and the learned preconditioner.
+
+scf_chebyshev mirrors the reference's SCF loop (src/main.f90:66-95): per
+iteration ncpaw_make_hamiltonian (density of the occupied orbitals, V_KS with
+DIIS, D_ij), set_hminmax (power-iteration bounds for the first iterations),
+chebyr_m (degree-10 Chebyshev filter on [mu, hmax]), orthog_phi, diagh, and
+mu = eig(nocc+1) + dmua (src/diag.f90:108).
+
+scf_block replaces filter + orthog + diagh by a few iterations of the block
+preconditioned solver (opaw_block_solve), whose k-space preconditioner is
+the Teter-Payne-Allan ratio, optionally corrected by a small MLP.  The MLP
+is trained on (residual, error) pairs recorded by the solver itself
+(train_preconditioner: Adam on the loss and gradient the library computes
+on the GPU).
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import lib
+
+
+@dataclass
+class ScfResult:
+    eig: np.ndarray
+    residual: float            # largest eigen-residual of the occupied bands at the end
+    iterations: int
+    seconds: float
+    history: List[float] = field(default_factory=list)   # largest residual per SCF iteration
+    h_applications: int = 0    # band applications of shs
+
+
+def _occupied_residual(g, eig, nocc):
+    return float(np.max(g.residuals(eig)[:nocc]))
+
+
+def scf_chebyshev(g, nocc: int, nscf: int = 30, tol: float = 1e-8, degree: int = 10, dmua: float = 0.1,
+                  iscf_hminmax: int = 5, seed: int = 1234) -> ScfResult:
+    """The reference's SCF (src/main.f90:66-95) on the resident orbitals of
+    system 0 (h_type 1).  The first mu is the lower bound from hminmax plus
+    a quarter of the interval."""
+    g.torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = []
+    mu = None
+    hlo = hhi = None
+    eig = np.zeros(g.nb)
+    napp = 0
+    it = 0
+    for it in range(1, nscf + 1):
+        g.ncpaw_make_hamiltonian(0, diis=True)
+        if it <= iscf_hminmax or hhi is None:
+            hlo, hhi = g.hminmax(0, 1, 25, 1.2, seed + it)
+            napp += 50
+        if mu is None:
+            mu = hlo + 0.25 * (hhi - hlo)
+        g.chebyshev_filter(mu, hhi, degree)
+        g.orthog_phi(0, 1)
+        eig = g.diagh(0, 1)
+        napp += degree * g.nb + g.nb
+        mu = eig[nocc] + dmua if nocc < g.nb else eig[-1] + dmua
+        res = _occupied_residual(g, eig, nocc)
+        napp += g.nb
+        hist.append(res)
+        if res <= tol:
+            break
+    g.torch.cuda.synchronize()
+    return ScfResult(eig=eig, residual=hist[-1], iterations=it, seconds=time.perf_counter() - t0, history=hist,
+                     h_applications=napp)
+
+
+def scf_block(g, nocc: int, nscf: int = 30, tol: float = 1e-8, inner: int = 3, mlp=None, nh: int = 16,
+              record: int = 0) -> ScfResult:
+    """SCF with the block preconditioned solver in place of the filter."""
+    g.torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = []
+    eig = np.zeros(g.nb)
+    napp = 0
+    it = 0
+    g.orthog_phi(0, 1)
+    for it in range(1, nscf + 1):
+        g.ncpaw_make_hamiltonian(0, diis=True)
+        eig, h, its = g.block_solve(inner, 0.0, mlp=mlp, nh=nh, record=record, nconv=nocc)
+        napp += g.nb * (2 * its + 1)
+        res = _occupied_residual(g, eig, nocc)
+        napp += g.nb
+        hist.append(res)
+        if res <= tol:
+            break
+    g.torch.cuda.synchronize()
+    return ScfResult(eig=eig, residual=hist[-1], iterations=it, seconds=time.perf_counter() - t0, history=hist,
+                     h_applications=napp)
+
+
+def mlp_init(nh: int = 16, seed: int = 0, scale: float = 0.1) -> np.ndarray:
+    """Small random weights; the output bias and weights start at zero, so
+    the untrained preconditioner equals Teter-Payne-Allan."""
+    n = lib.load().opaw_mlp_param_count(nh)
+    rng = np.random.default_rng(seed)
+    p = rng.normal(0.0, scale, n)
+    p[4 * nh + nh * nh:] = 0.0       # w3, b3
+    return p
+
+
+def train_preconditioner(g, nh: int = 16, steps: int = 300, lr: float = 1e-2, seed: int = 0,
+                         params: Optional[np.ndarray] = None, log_every: int = 0):
+    """Adam on the misalignment loss of the prepared pairs.  Returns
+    (params, losses)."""
+    p = mlp_init(nh, seed) if params is None else np.array(params, dtype=np.float64)
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    b1, b2, eps = 0.9, 0.999, 1e-8
+    losses = []
+    best = (np.inf, p.copy())
+    for k in range(1, steps + 1):
+        loss, grad = g.precond_loss(p, nh)
+        losses.append(loss)
+        if loss < best[0]:
+            best = (loss, p.copy())
+        m = b1 * m + (1 - b1) * grad
+        v = b2 * v + (1 - b2) * grad * grad
+        p = p - lr * (m / (1 - b1 ** k)) / (np.sqrt(v / (1 - b2 ** k)) + eps)
+        if log_every and k % log_every == 0:
+            print(f"[train] step {k} loss {loss:.6f}", flush=True)
+    loss, _ = g.precond_loss(p, nh, grad=False)
+    losses.append(loss)
+    if loss < best[0]:
+        best = (loss, p.copy())
+    return best[1], losses

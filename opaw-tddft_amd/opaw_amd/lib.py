@@ -79,6 +79,19 @@ SIGNATURES = {
     "opaw_spectrum": (I, [DP, I, I, D, I, D, D, D, I, DP, DP]),
     "opaw_paw_box": (I, [I, DP, I, DP, DP, IP, IP, IP, DP, DP, DP, DP, DP, DP, DP, DP, DP]),
     "opaw_paw_sphere": (I, [I, I, I, D, D, D, DP, D, D, I, DP, I, I, DP, I, IP, IP, DP]),
+    "opaw_set_nocc": (I, [P, I]),
+    "opaw_set_diis": (I, [P, I, D]),
+    "opaw_ncpaw_make_hamiltonian": (I, [P, I, I]),
+    "opaw_hminmax": (I, [P, I, I, I, D, C.c_ulonglong, DP, DP]),
+    "opaw_chebyshev_filter": (I, [P, I, I, I, D, D]),
+    "opaw_residuals": (I, [P, I, I, DP, DP]),
+    "opaw_block_solve": (I, [P, I, I, I, D, I, DP, I, I, DP, DP, IP]),
+    "opaw_mlp_param_count": (I, [I]),
+    "opaw_precond_prepare": (I, [P, I]),
+    "opaw_precond_clear": (I, [P]),
+    "opaw_precond_count": (I, [P, IP, IP]),
+    "opaw_precond_loss": (I, [P, DP, I, DP, DP]),
+    "opaw_precond_pair": (I, [P, I, P, P, DP, DP, IP]),
     "opaw_profile_enable": (I, [P, I]),
     "opaw_profile_reset": (I, [P]),
     "opaw_profile_read": (I, [P, I, DP, LLP]),

@@ -1,0 +1,180 @@
+"""GPU: the ground-state solver against the oracle (Chebyshev filter,
+power-iteration bounds, make_hamiltonian with occupied-only density and
+DIIS, residuals), the block preconditioned solver's convergence checked on
+the CPU (Rayleigh-Ritz and residuals of the GPU's orbitals with the
+oracle's shs), and the preconditioner loss and gradient against the
+oracle."""
+import numpy as np
+import pytest
+
+from tests.helpers import O, rel, state_from_spec
+from oracle import gs_oracle as GO
+from opaw_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _api():
+    import torch
+    from opaw_amd.api import OpawGPU
+    assert torch.cuda.is_available(), "GPU test without a GPU"
+    return OpawGPU
+
+
+def _system(nb=6, nocc=3, seed=11):
+    spec = synth.molecule_system(nring=1, n=(32, 32, 32), d=0.5, seed=seed)
+    spec.rnel = 2.0 * nocc
+    st = state_from_spec(spec, nsys=1)
+    rng = np.random.default_rng(seed)
+    for S in st.systems:
+        S.vks = spec.vloc + rng.uniform(-0.3, 0.3, spec.n)
+        S.dij = []
+        for a in spec.atoms:
+            m = rng.uniform(-1, 1, (a.ms, a.ms))
+            S.dij.append(0.5 * (m + m.T))
+    phi = synth.lowdin_orthonormalise(synth.random_orbitals(spec, nb, seed=seed + 1, envelope=3.0), spec.dv)
+    return spec, st, phi
+
+
+def _gpu(spec, st, nb):
+    OpawGPU = _api()
+    g = OpawGPU(spec, "f64", nsys=1, nb=nb)
+    g.write_vks(0, st.systems[0].vks)
+    for ia, d in enumerate(st.systems[0].dij, start=1):
+        g.write_dij(0, ia, d)
+    return g
+
+
+def test_chebyshev_filter_matches_oracle():
+    spec, st, phi = _system()
+    g = _gpu(spec, st, phi.shape[0])
+    g.write_orbitals(0, phi)
+    g.chebyshev_filter(-0.4, 6.0, degree=4)
+    ref = GO.cheb_filter(st, phi, 0, 4, -0.4, 6.0)
+    assert rel(g.read_orbitals(0), ref) < 1e-10
+    g.close()
+
+
+def test_hminmax_matches_oracle_and_brackets_the_spectrum():
+    spec, st, phi = _system()
+    g = _gpu(spec, st, phi.shape[0])
+    lo, hi = g.hminmax(0, 1, niter=6, fctr=1.2, seed=77)
+    rlo, rhi = GO.hminmax(st, 0, 6, 1.2, 77, spec.n)
+    assert abs(lo - rlo) < 1e-9 * abs(rhi) and abs(hi - rhi) < 1e-9 * abs(rhi)
+    assert lo < hi
+    g.close()
+
+
+def test_make_hamiltonian_occupied_density_and_diis():
+    spec, st, phi = _system(nb=5, nocc=3)
+    nocc = 3
+    g = _gpu(spec, st, phi.shape[0])
+    g.set_nocc(nocc)
+    g.set_diis(4, 0.15)
+    diis = GO.Diis(4, 0.15, spec.dv)
+    rng = np.random.default_rng(5)
+    for k in range(5):
+        cur = synth.lowdin_orthonormalise(phi + 0.2 * k * (rng.standard_normal(phi.shape) +
+                                                         1j * rng.standard_normal(phi.shape)), spec.dv)
+        g.write_orbitals(0, cur)
+        g.ncpaw_make_hamiltonian(0, diis=True)
+        pt = O.sn_phi(st, cur[:nocc], -0.5)
+        rho = O.tddft_rhoij_2(st, pt)
+        dens = O.tddft_dens_2(st, pt)
+        nh, dens = O.get_nhat(st, rho, dens)
+        vks, vxc, vh = O.get_pot_tddft_2(st, dens, nh)
+        vks = diis(vks)
+        dij = O.get_dij(st, rho, vks, vxc)
+        assert rel(g.read_field("dens", 0), dens) < 1e-12, k
+        assert rel(g.read_field("vks", 0), vks) < 1e-11, k
+        for ia in range(1, len(spec.atoms) + 1):
+            assert rel(g.read_dij(0, ia), dij[ia - 1]) < 1e-10, (k, ia)
+    g.close()
+
+
+def test_block_solver_converges_and_oracle_agrees():
+    spec, st, phi = _system(nb=6)
+    g = _gpu(spec, st, phi.shape[0])
+    g.write_orbitals(0, phi)
+    nconv = 4
+    eig, hist, its = g.block_solve(300, 1e-9, nconv=nconv)
+    print(f"block solver: {its} iterations, residual {hist[-1]:.2e}", flush=True)
+    assert hist[-1] <= 1e-9
+    res = g.residuals(eig)
+    assert np.max(res[:nconv]) <= 1.1e-9
+    out = g.read_orbitals(0)
+    G = (out.conj() @ out.T) * spec.dv
+    assert np.max(np.abs(G - np.eye(len(eig)))) < 1e-10
+    # the oracle's shs on the GPU's orbitals: same Ritz values, small residuals
+    H = O.shs(st, out, 0)
+    ritz = np.real(np.einsum("ir,ir->i", out.conj(), H)) * spec.dv
+    assert np.max(np.abs(ritz - eig)) < 1e-10
+    r = np.sqrt(np.sum(np.abs(H - eig[:, None] * out) ** 2, axis=1) * spec.dv)
+    assert np.max(r[:nconv]) < 2e-9
+    g.close()
+
+
+def test_preconditioner_loss_and_gradient_match_oracle():
+    spec, st, phi = _system(nb=4)
+    g = _gpu(spec, st, phi.shape[0])
+    g.write_orbitals(0, phi)
+    g.block_solve(8, 0.0, record=3)
+    g.block_solve(300, 1e-10, nconv=3)
+    g.precond_prepare(0)
+    nsnap, nready = g.precond_count()
+    assert nsnap == nready >= 2
+    pairs = [g.precond_pair(k) for k in range(nready)]
+    from opaw_amd.gs import mlp_init
+    nh = 8
+    P = mlp_init(nh, seed=3, scale=0.4)
+    P[4 * nh + nh * nh:] = np.random.default_rng(4).normal(0, 0.3, nh + 1)
+    ek = O.ek_prep(st.grid, spec.ekcut)
+    L, grad = g.precond_loss(P, nh)
+    Lr, gr = GO.precond_loss(pairs, ek, spec.n, P, nh)
+    assert abs(L - Lr) < 1e-11
+    assert np.max(np.abs(grad - gr)) < 1e-9 * max(1.0, np.max(np.abs(gr)))
+    g.precond_clear()
+    assert g.precond_count() == (0, 0)
+    g.close()
+
+
+def test_trained_preconditioner_on_held_out_system():
+    """Train the MLP on pairs recorded while solving one system, then on a
+    held-out system compare with plain Teter-Payne-Allan: the misalignment
+    loss of the held-out pairs, and LOBPCG iterations to 1e-8."""
+    from opaw_amd import gs
+    OpawGPU = _api()
+    nb, nocc, nh = 8, 6, 8
+
+    def make(seed):
+        spec = synth.molecule_system(nring=1, n=(32, 32, 32), d=0.5, seed=seed)
+        spec.rnel = 2.0 * nocc
+        phi = synth.lowdin_orthonormalise(synth.random_orbitals(spec, nb, seed=seed + 1, envelope=3.0), spec.dv)
+        g = OpawGPU(spec, "f64", nsys=1, nb=nb)
+        g.set_nocc(nocc)
+        g.write_orbitals(0, phi)
+        g.ncpaw_make_hamiltonian(0, diis=False)
+        return g, phi
+
+    g, _ = make(21)
+    g.block_solve(200, 1e-9, record=1, nconv=nocc)
+    g.precond_prepare(0)
+    P, losses = gs.train_preconditioner(g, nh=nh, steps=80, lr=2e-2, seed=1)
+    assert losses[-1] < losses[0]
+    g.close()
+    h, phi = make(33)
+    its = {}
+    for name, p in (("tpa", None), ("mlp", P)):
+        h.write_orbitals(0, phi)
+        _, hist, n = h.block_solve(200, 1e-8, mlp=p, nh=nh, record=(1 if name == "tpa" else 0), nconv=nocc)
+        assert hist[-1] <= 1e-8, name
+        its[name] = n
+    h.precond_prepare(0)
+    zero = gs.mlp_init(nh)                       # output weights zero: exactly TPA
+    l_tpa, _ = h.precond_loss(zero, nh, grad=False)
+    l_mlp, _ = h.precond_loss(P, nh, grad=False)
+    print(f"held-out loss TPA {l_tpa:.4f} MLP {l_mlp:.4f}; LOBPCG iterations TPA {its['tpa']} MLP {its['mlp']}",
+          flush=True)
+    assert l_mlp < l_tpa
+    assert its["mlp"] <= 1.2 * its["tpa"]
+    h.close()
