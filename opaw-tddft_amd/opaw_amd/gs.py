@@ -74,8 +74,11 @@ def scf_chebyshev(g, nocc: int, nscf: int = 30, tol: float = 1e-8, degree: int =
 
 
 def scf_block(g, nocc: int, nscf: int = 30, tol: float = 1e-8, inner: int = 3, mlp=None, nh: int = 16,
-              record: int = 0) -> ScfResult:
-    """SCF with the block preconditioned solver in place of the filter."""
+              record: int = 0, eta: float = 0.0, inner_max: int = 30) -> ScfResult:
+    """SCF with the block preconditioned solver in place of the filter.
+    eta = 0: `inner` LOBPCG iterations per SCF iteration; eta > 0: each
+    inner solve runs (up to inner_max iterations) to max(tol, eta * the
+    previous SCF residual), so the preconditioner's quality sets the work."""
     g.torch.cuda.synchronize()
     t0 = time.perf_counter()
     hist = []
@@ -83,13 +86,19 @@ def scf_block(g, nocc: int, nscf: int = 30, tol: float = 1e-8, inner: int = 3, m
     napp = 0
     it = 0
     g.orthog_phi(0, 1)
+    prev = None
     for it in range(1, nscf + 1):
         g.ncpaw_make_hamiltonian(0, diis=True)
-        eig, h, its = g.block_solve(inner, 0.0, mlp=mlp, nh=nh, record=record, nconv=nocc)
+        if eta > 0.0:
+            inner_tol = tol if prev is None else max(tol, eta * prev)
+            eig, h, its = g.block_solve(inner_max, inner_tol, mlp=mlp, nh=nh, record=record, nconv=nocc)
+        else:
+            eig, h, its = g.block_solve(inner, 0.0, mlp=mlp, nh=nh, record=record, nconv=nocc)
         napp += g.nb * (2 * its + 1)
         res = _occupied_residual(g, eig, nocc)
         napp += g.nb
         hist.append(res)
+        prev = res
         if res <= tol:
             break
     g.torch.cuda.synchronize()
