@@ -15,6 +15,7 @@
 #include "opaw_fields.h"
 #include "opaw_gs.h"
 #include "opaw_hip.h"
+#include "opaw_linalg.h"
 
 using namespace opaw;
 
@@ -1451,66 +1452,12 @@ static int flow_rk4(opaw_ctx* c, double dt, double* dip_out) {
 }
 
 // ---------------------------------------------------------------------------
-// host Hermitian eigensolver (cyclic complex Jacobi), FP64.  A is n x n
-// row-major A[i*n+j] = A(i,j); returns ascending eigenvalues and column
-// eigenvectors V[i*n+k] = V(i,k).
+// host Hermitian eigensolver, FP64 (Householder tridiagonalisation + implicit
+// QL, opaw_linalg.h).  A is n x n row-major A[i*n+j] = A(i,j); returns
+// ascending eigenvalues and column eigenvectors V[i*n+k] = V(i,k).
 static void herm_eig(int n, std::vector<std::complex<double>> A, std::vector<double>& w,
                      std::vector<std::complex<double>>& V) {
-  using C = std::complex<double>;
-  V.assign((size_t)n * n, C(0, 0));
-  for (int i = 0; i < n; ++i) V[(size_t)i * n + i] = 1.0;
-  double scale = 0.0;
-  for (auto& v : A) scale += std::norm(v);
-  scale = std::sqrt(scale);
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0.0;
-    for (int p = 0; p < n; ++p)
-      for (int q = p + 1; q < n; ++q) off += std::norm(A[(size_t)p * n + q]);
-    if (std::sqrt(off) <= 1e-17 * scale || off == 0.0) break;
-    for (int p = 0; p < n; ++p)
-      for (int q = p + 1; q < n; ++q) {
-        const C apq = A[(size_t)p * n + q];
-        const double r = std::abs(apq);
-        if (r == 0.0) continue;
-        const C ph = apq / r;  // e^{i alpha}
-        const double app = A[(size_t)p * n + p].real(), aqq = A[(size_t)q * n + q].real();
-        const double theta = (aqq - app) / (2.0 * r);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-        const double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
-        const C em = std::conj(ph);  // e^{-i alpha}
-        // A <- A W (columns), W = [[c, s], [-s e^{-ia}, c e^{-ia}]]
-        for (int k = 0; k < n; ++k) {
-          const C akp = A[(size_t)k * n + p], akq = A[(size_t)k * n + q];
-          A[(size_t)k * n + p] = cs * akp - sn * em * akq;
-          A[(size_t)k * n + q] = sn * akp + cs * em * akq;
-        }
-        // A <- W^H A (rows)
-        for (int k = 0; k < n; ++k) {
-          const C apk = A[(size_t)p * n + k], aqk = A[(size_t)q * n + k];
-          A[(size_t)p * n + k] = cs * apk - sn * ph * aqk;
-          A[(size_t)q * n + k] = sn * apk + cs * ph * aqk;
-        }
-        A[(size_t)p * n + q] = 0.0;
-        A[(size_t)q * n + p] = 0.0;
-        A[(size_t)p * n + p] = A[(size_t)p * n + p].real();
-        A[(size_t)q * n + q] = A[(size_t)q * n + q].real();
-        for (int k = 0; k < n; ++k) {
-          const C vkp = V[(size_t)k * n + p], vkq = V[(size_t)k * n + q];
-          V[(size_t)k * n + p] = cs * vkp - sn * em * vkq;
-          V[(size_t)k * n + q] = sn * vkp + cs * em * vkq;
-        }
-      }
-  }
-  std::vector<int> ord(n);
-  for (int i = 0; i < n; ++i) ord[i] = i;
-  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return A[(size_t)a * n + a].real() < A[(size_t)b * n + b].real(); });
-  std::vector<C> V2((size_t)n * n);
-  w.resize(n);
-  for (int k = 0; k < n; ++k) {
-    w[k] = A[(size_t)ord[k] * n + ord[k]].real();
-    for (int i = 0; i < n; ++i) V2[(size_t)i * n + k] = V[(size_t)i * n + ord[k]];
-  }
-  V.swap(V2);
+  herm_eig_tridiag(n, std::move(A), w, V);
 }
 
 // hm = dv * <a_i | b_j> for nb fields; downloads the matrix (row-major)
