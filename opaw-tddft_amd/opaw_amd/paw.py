@@ -156,10 +156,9 @@ def cg3(l1, m1, l2, m2, l3, m3) -> float:
 
 
 def natural_spline_y2(x: np.ndarray, y: np.ndarray) -> np.ndarray:
-    """Second derivatives of the natural cubic spline through (x, y)
-    (Numerical Recipes `spline` with natural end conditions).  The
-    reference calls spline_dn from a library shipped only as an object
-    file; this is the published algorithm, not a restatement of it."""
+    """Second derivatives of the natural cubic spline through (x, y):
+    SPLINE_dn (src/lib/library_NRF_spline.f:11-55), the Numerical Recipes
+    routine with yp1 = ypn = 1e32, i.e. natural end conditions."""
     n = len(x)
     y2 = np.zeros(n)
     u = np.zeros(n)
@@ -176,7 +175,8 @@ def natural_spline_y2(x: np.ndarray, y: np.ndarray) -> np.ndarray:
 
 
 def natural_spline_eval(x: np.ndarray, y: np.ndarray, y2: np.ndarray, t: float) -> float:
-    """Numerical Recipes `splint`."""
+    """SPLINT_dn (src/lib/library_NRF_splint.f:6-33): bisection for the
+    interval, then the cubic in the interval."""
     klo, khi = 0, len(x) - 1
     while khi - klo > 1:
         k = (khi + klo) // 2

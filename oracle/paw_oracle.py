@@ -19,9 +19,9 @@ Pinning (tests/test_paw_cpu.py):
   sphere       membership vs a brute-force distance test over all images;
                g3d vs numpy.interp of g_l and scipy's spherical harmonics
                made real (with the reference's single-precision pi)
-The spline matrices themselves follow the published natural cubic spline
-(Numerical Recipes): the reference's spline_dn is shipped only as an object
-file, so their agreement with the reference is parity unpinned.
+The spline matrices come from SPLINE_dn / SPLINT_dn
+(src/lib/library_NRF_spline.f, src/lib/library_NRF_splint.f), the Numerical
+Recipes natural cubic spline, checked against scipy's natural CubicSpline.
 """
 from __future__ import annotations
 
