@@ -258,6 +258,16 @@ int opaw_paw_box(int nr, const double* rr, int ms, const double* rad, const doub
  * local_g3d(ngrid, nl2) = g_l(r) Y_L for r <= rcomp (else 0) from the shape
  * functions gl(nrc, 2 nl - 1).  With max_points = 0 only ngrid is returned
  * (size query); otherwise ngrid <= max_points is required. */
+/* vloc_tuma_prep -> vloc_prep (src/vloc/vloc_prep.f90:1-79, make_cvkb
+ * src/vloc/make_cvkb.f90, add_formfactor src/vloc/formfactor1.f90): the local
+ * potential of a nonperiodic context.  Per type t, the radial potential
+ * vr (the dataset's vloc, already times Y_00) on rr (nr_of_type[t] points;
+ * rr_cat / vr_cat concatenate the types) is splined onto the doubled box,
+ * Fourier transformed, shifted and summed over the atoms of the type with
+ * structure factors; the inverse transform cropped to the grid is vloc_out
+ * (nx*ny*nz).  atom_type is 1-based, coords (3, natom) in bohr. */
+int opaw_vloc_tuma(opaw_ctx* ctx, int ntype, const int* nr_of_type, const double* rr_cat, const double* vr_cat,
+                   int natom, const int* atom_type, const double* coords, double* vloc_out);
 int opaw_paw_sphere(int nx, int ny, int nz, double dx, double dy, double dz, const double* coord, double rcut,
                     double rcomp, int nr, const double* rr, int nl2, int nrc, const double* gl, int max_points,
                     int* ngrid, int* local_grid, double* local_g3d);

@@ -16,6 +16,7 @@
 #include "opaw_gs.h"
 #include "opaw_hip.h"
 #include "opaw_linalg.h"
+#include "opaw_paw.h"
 
 using namespace opaw;
 
@@ -2547,3 +2548,100 @@ int opaw_mlp_param_count(int nh) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// local potential of the nonperiodic (Martyna-Tuckerman) path:
+// vloc_tuma_prep -> vloc_prep (src/vloc/vloc_prep.f90:1-79) with make_cvkb
+// (src/vloc/make_cvkb.f90) and add_formfactor (src/vloc/formfactor1.f90)
+extern "C" int opaw_vloc_tuma(opaw_ctx* c, int ntype, const int* nr_of_type, const double* rr_cat,
+                              const double* vr_cat, int natom, const int* atom_type, const double* coords,
+                              double* vloc_out) {
+  CTX_CHECK(c);
+  if (c->periodic) return fail(OPAW_ERR_ARG, "vloc_tuma_prep is the nonperiodic path");
+  if (ntype < 1 || natom < 1 || !nr_of_type || !rr_cat || !vr_cat || !atom_type || !coords || !vloc_out)
+    return fail(OPAW_ERR_ARG, "bad vloc arguments");
+  for (int a = 0; a < natom; ++a)
+    if (atom_type[a] < 1 || atom_type[a] > ntype) return fail(OPAW_ERR_ARG, "atom %d has type %d", a + 1, atom_type[a]);
+  const GridGeom& B = c->gbig;
+  const long long Nb = (long long)B.nx * B.ny * B.nz;
+  const long long N = c->N;
+  std::vector<void*> tmp;
+  auto cleanup = [&]() {
+    for (void* p : tmp) dfree(c, p);
+  };
+  auto alloc = [&](size_t bytes) -> void* {
+    void* p = nullptr;
+    if (dalloc(c, &p, bytes) != OPAW_OK) return nullptr;
+    tmp.push_back(p);
+    return p;
+  };
+  c64* A = (c64*)alloc((size_t)Nb * sizeof(c64));
+  c64* cvn = (c64*)alloc((size_t)Nb * sizeof(c64));
+  double* dv_out = (double*)alloc((size_t)N * sizeof(double));
+  if (!A || !cvn || !dv_out) {
+    cleanup();
+    return OPAW_ERR_ALLOC;
+  }
+  int rc = OPAW_OK;
+  auto H = [&](hipError_t e) {
+    if (e != hipSuccess && rc == OPAW_OK) rc = fail(OPAW_ERR_HIP, "%s", hipGetErrorString(e));
+    return rc == OPAW_OK;
+  };
+  const c64 *twx = c->tw64[B.nx], *twy = c->tw64[B.ny], *twz = c->tw64[B.nz];
+  H(hipMemsetAsync(cvn, 0, (size_t)Nb * sizeof(c64), c->st));
+  size_t off = 0;
+  for (int t = 0; t < ntype && rc == OPAW_OK; ++t) {
+    const int nr = nr_of_type[t];
+    if (nr < 3) {
+      rc = fail(OPAW_ERR_ARG, "type %d: radial grid too short", t + 1);
+      break;
+    }
+    const double* rr = rr_cat + off;
+    const double* vr = vr_cat + off;
+    off += nr;
+    // make_cvkb: spline of vr on max(rr, 0.1 dr1) (SPLINE_dn, natural)
+    const double dr1 = rr[1] - rr[0];
+    std::vector<double> xs(nr), y2(nr, 0.0), u(nr, 0.0);
+    for (int i = 0; i < nr; ++i) xs[i] = std::max(rr[i], 0.1 * dr1);
+    for (int i = 1; i < nr - 1; ++i) {
+      const double sig = (xs[i] - xs[i - 1]) / (xs[i + 1] - xs[i - 1]);
+      const double p = sig * y2[i - 1] + 2.0;
+      y2[i] = (sig - 1.0) / p;
+      u[i] = (6.0 * ((vr[i + 1] - vr[i]) / (xs[i + 1] - xs[i]) - (vr[i] - vr[i - 1]) / (xs[i] - xs[i - 1])) /
+                  (xs[i + 1] - xs[i - 1]) -
+              sig * u[i - 1]) /
+             p;
+    }
+    y2[nr - 1] = 0.0;
+    for (int k = nr - 2; k >= 0; --k) y2[k] = y2[k] * y2[k + 1] + u[k];
+    const double charge = vr[nr - 1] * rr[nr - 1];
+    double* d = (double*)alloc((size_t)4 * nr * sizeof(double));
+    std::vector<double> at;
+    for (int a = 0; a < natom; ++a)
+      if (atom_type[a] == t + 1) at.insert(at.end(), coords + 3 * a, coords + 3 * a + 3);
+    double* dc = at.empty() ? nullptr : (double*)alloc(at.size() * sizeof(double));
+    if (!d || (!at.empty() && !dc)) {
+      rc = OPAW_ERR_ALLOC;
+      break;
+    }
+    if (!H(hipMemcpy(d, rr, nr * sizeof(double), hipMemcpyHostToDevice)) ||
+        !H(hipMemcpy(d + nr, xs.data(), nr * sizeof(double), hipMemcpyHostToDevice)) ||
+        !H(hipMemcpy(d + 2 * nr, vr, nr * sizeof(double), hipMemcpyHostToDevice)) ||
+        !H(hipMemcpy(d + 3 * nr, y2.data(), nr * sizeof(double), hipMemcpyHostToDevice)))
+      break;
+    if (at.empty()) continue;
+    if (!H(hipMemcpy(dc, at.data(), at.size() * sizeof(double), hipMemcpyHostToDevice))) break;
+    if (!H(vloc_sample(c->st, d, d + nr, d + 2 * nr, d + 3 * nr, nr, charge, dr1, B.nx, B.ny, B.nz, c->dx, c->dy,
+                       c->dz, A)))
+      break;
+    if (!H(fft3d<double>(B, c->st, -1, A, c->hwork, 1, twx, twy, twz))) break;
+    if (!H(vloc_accumulate(c->st, c->hwork, cvn, B.nx, B.ny, B.nz, c->dx, c->dy, c->dz, dc, (int)at.size() / 3)))
+      break;
+  }
+  if (rc == OPAW_OK && H(fft3d<double>(B, c->st, 1, cvn, A, 1, twx, twy, twz)) &&
+      H(vloc_crop(c->st, A, B.nx, B.ny, B.nz, c->nx, c->ny, c->nz, 1.0 / ((double)Nb * dvol(c)), dv_out)) &&
+      H(hipStreamSynchronize(c->st)))
+    H(hipMemcpy(vloc_out, dv_out, (size_t)N * sizeof(double), hipMemcpyDeviceToHost));
+  cleanup();
+  return rc;
+}

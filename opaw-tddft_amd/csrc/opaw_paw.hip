@@ -361,3 +361,123 @@ extern "C" int opaw_paw_sphere(int nx, int ny, int nz, double dx, double dy, dou
   PCHK(hipMemcpy(local_g3d, dg3d.p, (size_t)nl2 * total * sizeof(double), hipMemcpyDeviceToHost));
   return OPAW_OK;
 }
+
+// ---------------------------------------------------------------------------
+// local potential of vloc_tuma_prep (nonperiodic, Martyna-Tuckerman box)
+#include "opaw_paw.h"
+
+namespace opaw {
+namespace {
+
+__global__ __launch_bounds__(256) void k_vloc_sample(const double* __restrict__ rr, const double* __restrict__ xs,
+                                                     const double* __restrict__ ys, const double* __restrict__ y2s,
+                                                     int nr, double charge, double dr1, int nxb, int nyb, int nzb,
+                                                     double dx, double dy, double dz, c64* __restrict__ cv) {
+  const long long n = (long long)nxb * nyb * nzb;
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int ix = (int)(p % nxb), iy = (int)((p / nxb) % nyb), iz = (int)(p / ((long long)nxb * nyb));
+  const double x = (ix - nxb / 2) * dx, y = (iy - nyb / 2) * dy, z = (iz - nzb / 2) * dz;
+  const double rs = fmax(1e-10, sqrt(x * x + y * y + z * z));
+  const double rl = fmax(rs, 0.1 * dr1);
+  double v;
+  if (rs < rr[0]) {
+    v = ys[0];
+  } else if (rs > rr[nr - 1]) {
+    v = charge / rs;
+  } else {
+    // SPLINT_dn (src/lib/library_NRF_splint.f:6-33)
+    int klo = 0, khi = nr - 1;
+    while (khi - klo > 1) {
+      const int k = (khi + klo) / 2;
+      if (xs[k] > rl)
+        khi = k;
+      else
+        klo = k;
+    }
+    const double h = xs[khi] - xs[klo];
+    const double a = (xs[khi] - rl) / h, b = (rl - xs[klo]) / h;
+    v = a * ys[klo] + b * ys[khi] + ((a * a * a - a) * y2s[klo] + (b * b * b - b) * y2s[khi]) * (h * h) / 6.0;
+  }
+  cv[p] = mk(v, 0.0);
+}
+
+__device__ __forceinline__ double fold_k(int i, int n, double d) {
+  const double pi = 3.14159265358979323846;
+  double k = (double)i * (2.0 * pi / ((double)n * d));
+  if (k > pi / d) k -= 2.0 * pi / d;
+  return k;
+}
+
+__global__ __launch_bounds__(256) void k_vloc_accumulate(const c64* __restrict__ cvkb, c64* __restrict__ cvn, int nxb,
+                                                         int nyb, int nzb, double dx, double dy, double dz,
+                                                         const double* __restrict__ coords, int natom) {
+  const long long n = (long long)nxb * nyb * nzb;
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int ix = (int)(p % nxb), iy = (int)((p / nxb) % nyb), iz = (int)(p / ((long long)nxb * nyb));
+  const double kx = fold_k(ix, nxb, dx), ky = fold_k(iy, nyb, dy), kz = fold_k(iz, nzb, dz);
+  // shift_cvkb: exp(-i r0.k), r0 = -(n/2) d
+  const double r0x = -(double)nxb / 2.0 * dx, r0y = -(double)nyb / 2.0 * dy, r0z = -(double)nzb / 2.0 * dz;
+  double s, c;
+  sincos(-(r0x * kx + r0y * ky + r0z * kz), &s, &c);
+  // add_formfactor: sum over atoms of exp(-i (R - rstart).k), zero on the
+  // Nyquist planes, rstart = -(n d)/2
+  double fr = 0.0, fi = 0.0;
+  if (ix != nxb / 2 && iy != nyb / 2 && iz != nzb / 2) {
+    const double sx = -0.5 * dx * nxb, sy = -0.5 * dy * nyb, sz = -0.5 * dz * nzb;
+    for (int a = 0; a < natom; ++a) {
+      double cs, sn;
+      sincos(-((coords[3 * a] - sx) * kx), &sn, &cs);
+      double cs2, sn2;
+      sincos(-((coords[3 * a + 1] - sy) * ky + (coords[3 * a + 2] - sz) * kz), &sn2, &cs2);
+      fr += cs * cs2 - sn * sn2;
+      fi += cs * sn2 + sn * cs2;
+    }
+  }
+  const double dv = dx * dy * dz;
+  const c64 v = cvkb[p];
+  // (v dv) (c + i s) (fr + i fi)
+  const double ar = (v.x * c - v.y * s) * dv, ai = (v.x * s + v.y * c) * dv;
+  const c64 o = cvn[p];
+  cvn[p] = mk(o.x + ar * fr - ai * fi, o.y + ar * fi + ai * fr);
+}
+
+__global__ __launch_bounds__(256) void k_vloc_crop(const c64* __restrict__ cvn, int nxb, int nyb, int nzb, int nx,
+                                                   int ny, int nz, double scale, double* __restrict__ vloc) {
+  const long long n = (long long)nx * ny * nz;
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int ix = (int)(p % nx), iy = (int)((p / nx) % ny), iz = (int)(p / ((long long)nx * ny));
+  const int bx = nxb / 2 - nx / 2 + ix, by = nyb / 2 - ny / 2 + iy, bz = nzb / 2 - nz / 2 + iz;
+  vloc[p] = cvn[(size_t)bx + (size_t)nxb * ((size_t)by + (size_t)nyb * bz)].x * scale;
+}
+
+}  // namespace
+
+hipError_t vloc_sample(hipStream_t st, const double* rr, const double* x, const double* y, const double* y2, int nr,
+                       double charge, double dr1, int nxb, int nyb, int nzb, double dx, double dy, double dz,
+                       c64* cv) {
+  const long long n = (long long)nxb * nyb * nzb;
+  hipLaunchKernelGGL(k_vloc_sample, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, rr, x, y, y2, nr, charge, dr1,
+                     nxb, nyb, nzb, dx, dy, dz, cv);
+  return hipGetLastError();
+}
+
+hipError_t vloc_accumulate(hipStream_t st, const c64* cvkb, c64* cvn, int nxb, int nyb, int nzb, double dx,
+                           double dy, double dz, const double* coords, int natom) {
+  const long long n = (long long)nxb * nyb * nzb;
+  hipLaunchKernelGGL(k_vloc_accumulate, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cvkb, cvn, nxb, nyb, nzb,
+                     dx, dy, dz, coords, natom);
+  return hipGetLastError();
+}
+
+hipError_t vloc_crop(hipStream_t st, const c64* cvn, int nxb, int nyb, int nzb, int nx, int ny, int nz, double scale,
+                     double* vloc) {
+  const long long n = (long long)nx * ny * nz;
+  hipLaunchKernelGGL(k_vloc_crop, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cvn, nxb, nyb, nzb, nx, ny, nz,
+                     scale, vloc);
+  return hipGetLastError();
+}
+
+}  // namespace opaw

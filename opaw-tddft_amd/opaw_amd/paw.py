@@ -692,17 +692,43 @@ def kinetic_differences(ds: PawDataset) -> np.ndarray:
     return read_radial_block(ds.filename, "<kinetic_energy_differences>", ds.ns * ds.ns).reshape(ds.ns, ds.ns)
 
 
+def vloc_tuma(geom: Geometry, symbols: Sequence[str], coords: np.ndarray, datasets: Dict[str, PawDataset]
+              ) -> np.ndarray:
+    """vloc_tuma_prep (src/vloc/vloc_prep.f90:1-79) on the GPU for a
+    nonperiodic grid: the datasets' local potentials (vloc, already times
+    Y_00) placed at the atoms through the doubled-box transform of
+    make_cvkb / add_formfactor.  Returns vloc_tot (nx*ny*nz)."""
+    if geom.periodic:
+        raise ValueError("vloc_tuma_prep is the nonperiodic path (periodic runs use libpaw's atm2fft)")
+    tsyms, amap = type_map(symbols)
+    nr = np.array([datasets[t].nr for t in tsyms], np.int32)
+    rr = np.ascontiguousarray(np.concatenate([datasets[t].rr for t in tsyms]))
+    vr = np.ascontiguousarray(np.concatenate([datasets[t].vloc for t in tsyms]))
+    types = np.ascontiguousarray(amap + 1, dtype=np.int32)
+    xyz = np.ascontiguousarray(np.asarray(coords, dtype=np.float64))
+    out = np.zeros(geom.nx * geom.ny * geom.nz)
+    L = lib.load()
+    ctx = C.c_void_p()
+    lib.call("opaw_create", C.byref(ctx), geom.nx, geom.ny, geom.nz, geom.dx, geom.dy, geom.dz, 0, 0, 1, 1)
+    try:
+        lib.call("opaw_vloc_tuma", ctx, len(tsyms), _ip(nr), _dp(rr), _dp(vr), len(symbols), _ip(types), _dp(xyz),
+                 _dp(out))
+    finally:
+        L.opaw_destroy(ctx)
+    return out
+
+
 def system_from_datasets(geom: Geometry, symbols: Sequence[str], coords: np.ndarray,
                          datasets: Dict[str, PawDataset], nocc: int, name: str = "paw",
                          dt: float = 0.05, sm: float = 1e-3, ipol: int = 0):
     """The tables of prep_paw for every atom: box projectors and spheres on
-    the GPU, transform_p3d on the host.  The local potential, the
-    pseudo-core density and the on-site D_ij come from libpaw in the
-    reference (atm2fft, vloc_tuma_prep, pawinit/pawdij: src/vloc.f90,
-    src/vloc/*.f90, src/get_dij.f90); here they are stand-ins, marked as
-    such in extra["standins"]: vloc and ncoret are the radial
-    blochl_local_ionic_potential * Y_00 and pseudo_core_density * Y_00
-    sampled at the grid points (nearest image), dij0 is the dataset's
+    the GPU, transform_p3d on the host, and for a nonperiodic grid the local
+    potential of vloc_tuma_prep on the GPU (vloc_tuma).  The pseudo-core
+    density and the on-site D_ij come from libpaw in the reference (atm2fft,
+    pawinit/pawdij: src/vloc.f90, src/get_dij.f90); here they are stand-ins,
+    marked as such in extra["standins"]: ncoret is pseudo_core_density *
+    Y_00 sampled at the grid points (nearest image; vloc likewise on a
+    periodic grid, where the reference uses atm2fft), dij0 is the dataset's
     kinetic-energy differences and eijkl is zero."""
     from . import synth
     tsyms, amap = type_map(symbols)
@@ -745,14 +771,20 @@ def system_from_datasets(geom: Geometry, symbols: Sequence[str], coords: np.ndar
             e = np.minimum(e, np.abs(2.0 * hk - e))
             d2 = d2 + e * e
         r = np.sqrt(d2)
-        vloc += np.interp(r, ds.rr, ds.vloc)
+        if geom.periodic:
+            vloc += np.interp(r, ds.rr, ds.vloc)
         core = read_radial_block(ds.filename, "<pseudo_core_density", ds.nr) / SQ4PI
         ncore += np.interp(r, ds.rr, core, right=0.0)
+    standins = ("ncoret", "dij0", "eijkl")
+    if geom.periodic:
+        standins = ("vloc",) + standins
+    else:
+        vloc = vloc_tuma(geom, symbols, coords, datasets).reshape(vloc.shape)
     from .synth import ekcut_for
     return synth.SystemSpec(name=name, nx=geom.nx, ny=geom.ny, nz=geom.nz, dx=geom.dx, dy=geom.dy, dz=geom.dz,
                             periodic=geom.periodic, ekcut=ekcut_for(geom.dx, geom.dy, geom.dz), atoms=atoms,
                             types=types, vloc=vloc.reshape(-1), ncoret=ncore.reshape(-1), rnel=2.0 * nocc,
                             nocc=nocc, dt=dt, sm=sm, ipol=ipol,
-                            extra=dict(standins=("vloc", "ncoret", "dij0", "eijkl"), type_symbols=tsyms,
+                            extra=dict(standins=standins, type_symbols=tsyms,
                                        natom_c=sum(1 for s in symbols if s == "C"),
                                        natom_h=sum(1 for s in symbols if s == "H"), N=geom.nx * geom.ny * geom.nz))

@@ -126,3 +126,73 @@ def sphere(nx, ny, nz, dx, dy, dz, coord, rcut, rcomp, rr, gl, nl2, pi):
         g3d[:, L] = (gl[i, l] * wi + gl[j, l] * wj) * ylm_array(l, m, xx, yy, zz, pi)
     g3d[r > rcomp] = 0.0
     return idx, g3d
+
+
+def natural_spline(x, y):
+    """SPLINE_dn (src/lib/library_NRF_spline.f): natural end conditions."""
+    n = len(x)
+    y2 = np.zeros(n)
+    u = np.zeros(n)
+    for i in range(1, n - 1):
+        sig = (x[i] - x[i - 1]) / (x[i + 1] - x[i - 1])
+        p = sig * y2[i - 1] + 2.0
+        y2[i] = (sig - 1.0) / p
+        u[i] = (6.0 * ((y[i + 1] - y[i]) / (x[i + 1] - x[i]) - (y[i] - y[i - 1]) / (x[i] - x[i - 1]))
+                / (x[i + 1] - x[i - 1]) - sig * u[i - 1]) / p
+    for k in range(n - 2, -1, -1):
+        y2[k] = y2[k] * y2[k + 1] + u[k]
+    return y2
+
+
+def splint(x, y, y2, t):
+    """SPLINT_dn for many points."""
+    khi = np.clip(np.searchsorted(x, t, side="right"), 1, len(x) - 1)
+    klo = khi - 1
+    h = x[khi] - x[klo]
+    a = (x[khi] - t) / h
+    b = (t - x[klo]) / h
+    return a * y[klo] + b * y[khi] + ((a ** 3 - a) * y2[klo] + (b ** 3 - b) * y2[khi]) * h * h / 6.0
+
+
+def vloc_tuma(nx, ny, nz, dx, dy, dz, types, atom_type, coords):
+    """vloc_prep (src/vloc/vloc_prep.f90:15-79) for the nonperiodic box:
+    types = [(rr, vr)], atom_type 0-based, coords (natom, 3)."""
+    nxb, nyb, nzb = 2 * nx, 2 * ny, 2 * nz
+    dv = dx * dy * dz
+    iz, iy, ix = np.meshgrid(np.arange(nzb), np.arange(nyb), np.arange(nxb), indexing="ij")
+    X, Y, Z = (ix - nxb // 2) * dx, (iy - nyb // 2) * dy, (iz - nzb // 2) * dz
+    rs = np.maximum(1e-10, np.sqrt(X * X + Y * Y + Z * Z))
+
+    def kfold(n, d):
+        k = np.arange(n) * (2 * np.pi / (n * d))
+        return np.where(k > np.pi / d, k - 2 * np.pi / d, k)
+
+    kx, ky, kz = kfold(nxb, dx), kfold(nyb, dy), kfold(nzb, dz)
+    KZ, KY, KX = np.meshgrid(kz, ky, kx, indexing="ij")
+    r0 = (-nxb / 2 * dx, -nyb / 2 * dy, -nzb / 2 * dz)
+    shift = np.exp(-1j * (r0[0] * KX + r0[1] * KY + r0[2] * KZ))
+    nyq = (np.arange(nxb)[None, None, :] == nxb // 2) | (np.arange(nyb)[None, :, None] == nyb // 2) | \
+          (np.arange(nzb)[:, None, None] == nzb // 2)
+    rstart = (-0.5 * dx * nxb, -0.5 * dy * nyb, -0.5 * dz * nzb)
+    cvn = np.zeros((nzb, nyb, nxb), np.complex128)
+    for t, (rr, vr) in enumerate(types):
+        idx = [a for a in range(len(atom_type)) if atom_type[a] == t]
+        if not idx:
+            continue
+        dr1 = rr[1] - rr[0]
+        xs = np.maximum(rr, 0.1 * dr1)
+        y2 = natural_spline(xs, vr)
+        charge = vr[-1] * rr[-1]
+        rl = np.maximum(rs, 0.1 * dr1)
+        v = splint(xs, vr, y2, rl)
+        v = np.where(rs < rr[0], vr[0], np.where(rs > rr[-1], charge / rs, v))
+        cvkb = np.fft.fftn(v) * dv * shift
+        form = np.zeros_like(cvn)
+        for a in idx:
+            c = coords[a]
+            form += np.exp(-1j * ((c[0] - rstart[0]) * KX + (c[1] - rstart[1]) * KY + (c[2] - rstart[2]) * KZ))
+        form[nyq] = 0.0
+        cvn += form * cvkb
+    out = np.fft.ifftn(cvn) * (nxb * nyb * nzb) / (nxb * nyb * nzb * dv)
+    mx, my, mz = nxb // 2 - nx // 2, nyb // 2 - ny // 2, nzb // 2 - nz // 2
+    return out[mz:mz + nz, my:my + ny, mx:mx + nx].real.reshape(-1)

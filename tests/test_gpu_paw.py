@@ -103,3 +103,15 @@ def test_naphthalene_from_datasets_one_step(naph):
     assert np.max(np.abs(d_gpu[0] - d_dip)) <= 1e-8 * np.max(np.abs(d_dip))
     g.close()
     del torch
+
+
+def test_vloc_tuma_matches_oracle(naph):
+    _torch()
+    geom, syms, xyz, ds = naph
+    v = paw.vloc_tuma(geom, syms, xyz, ds)
+    tsyms, amap = paw.type_map(syms)
+    ref = PO.vloc_tuma(geom.nx, geom.ny, geom.nz, geom.dx, geom.dy, geom.dz,
+                       [(ds[t].rr, ds[t].vloc) for t in tsyms], list(amap), xyz)
+    assert rel(v, ref) < 1e-12
+    # far from the molecule the potential is the ions' Coulomb tail -Z_val / r
+    assert v.min() < -1.0

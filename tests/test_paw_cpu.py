@@ -274,3 +274,30 @@ def test_set_up_entry_points_reject_bad_arguments():
     with pytest.raises(lib.OpawError, match="bad \\(l, m\\)"):
         lib.call("opaw_paw_box", 4, dp(z), 1, dp(z), dp(z), ip(lm), ip(sh), ip(sh), dp(z), dp(z), dp(z), dp(z),
                  dp(z), dp(z), dp(z), dp(z), dp(z))
+
+
+def test_vloc_tuma_oracle_is_the_direct_sum_for_smooth_potentials():
+    """The doubled-box transform route of vloc_prep reproduces the direct
+    real-space sum sum_a v(|r - R_a|) for a smooth (band-limited) radial
+    potential: v = -Z erf(r / rc) / r with rc well above the spacing."""
+    from scipy.special import erf
+    nx, ny, nz, d = 24, 20, 16, 0.4
+    rr = 1e-3 * (np.exp(0.01 * np.arange(1500)) - 1) + 1e-8
+    types = [(rr, -4.0 * erf(rr / 1.2) / rr), (rr, -1.0 * erf(rr / 1.0) / rr)]
+    atom_type = [0, 1, 0]
+    coords = np.array([[0.3, -0.2, 0.1], [-1.1, 0.9, -0.4], [1.6, 0.5, 0.7]])
+    v = PO.vloc_tuma(nx, ny, nz, d, d, d, types, atom_type, coords)
+    iz, iy, ix = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    P = np.stack([(-nx / 2 + ix) * d, (-ny / 2 + iy) * d, (-nz / 2 + iz) * d], -1).reshape(-1, 3)
+    ref = np.zeros(len(P))
+    for t, c in zip(atom_type, coords):
+        r = np.linalg.norm(P - c, axis=1)
+        Z, rc = (4.0, 1.2) if t == 0 else (1.0, 1.0)
+        ref += np.where(r > 1e-12, -Z * erf(r / rc) / np.maximum(r, 1e-12), -Z * 2 / (np.sqrt(np.pi) * rc))
+    assert np.max(np.abs(v - ref)) < 5e-4 * np.max(np.abs(ref))   # Nyquist planes and the finite box
+    # the spline restatement is scipy's natural spline
+    x = rr[:50]
+    y = np.sin(x * 40)
+    cs = CubicSpline(x, y, bc_type="natural")
+    t = np.linspace(x[0], x[-1], 77)
+    assert np.max(np.abs(PO.splint(x, y, PO.natural_spline(x, y), t) - cs(t))) < 1e-12
