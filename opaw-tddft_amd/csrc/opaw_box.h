@@ -34,6 +34,10 @@ struct BoxTables {
   const double* ov;          // overlap values O(i, l), row-major per atom
   const int2* ublk;          // union point blocks (first point, count) for the LDS kernel
   int nublk;                 // 0: no blocks (global kernel)
+  const int* ucptr;          // nublk+1: coefficient list of block b is ucol[ucptr[b] .. ucptr[b+1])
+  const int* ucol;           // coefficients referenced by each block (sorted)
+  const int* pcl;            // pair coefficient as an index into its block's list
+  int ucmax;                 // longest block coefficient list
   UnionTabs U;               // union-of-boxes gather tables
 };
 

@@ -438,6 +438,8 @@ template <int FG, typename R, int MODE>
 __global__ __launch_bounds__(kBoxThreads) void k_union_corr_lds(const int2* __restrict__ ublk,
                                                                 const int* __restrict__ pptr,
                                                                 const int* __restrict__ pc,
+                                                                const int* __restrict__ ucptr,
+                                                                const int* __restrict__ ucol,
                                                                 const double* __restrict__ pv,
                                                                 const c64* __restrict__ w, int nf, int ctot,
                                                                 c64* __restrict__ corr, const int* __restrict__ upt,
@@ -457,14 +459,18 @@ __global__ __launch_bounds__(kBoxThreads) void k_union_corr_lds(const int2* __re
   const int f0 = blockIdx.y * FG;
   const int q0 = pptr[blk.x];
   const int np = pptr[blk.x + blk.y] - q0;                // >= 1: every union point has a pair
-  // staging: every thread issues its loads in batches (clamped indices)
-  const int nw = ctot * FG;
+  // staging of the block's own coefficient rows (ctot bounds the list),
+  // every thread issuing its loads in batches (clamped indices); pc holds
+  // indices into the list
+  const int c0 = ucptr[blockIdx.x];
+  const int nc = ucptr[blockIdx.x + 1] - c0;
+  const int nw = nc * FG;
   for (int b0 = 0; b0 < nw; b0 += kBoxThreads * 8) {
     c64 t[8];
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
       const int q = min(b0 + s * kBoxThreads + (int)threadIdx.x, nw - 1);
-      t[s] = w[(long long)(q / FG) * nf + min(f0 + q % FG, nf - 1)];
+      t[s] = w[(long long)ucol[c0 + q / FG] * nf + min(f0 + q % FG, nf - 1)];
     }
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
@@ -550,6 +556,7 @@ __global__ __launch_bounds__(kBoxThreads) void k_union_corr_lds(const int2* __re
   }
 }
 
+// LDS of the kernel for coefficient lists of at most ctot rows
 static size_t union_lds_bytes(int ctot, int fg, bool apply) {
   size_t s = (size_t)ctot * fg * 16 + ((size_t)kUnionBlockPairs * 12 + (kUnionBlockRows + 1) * 4 + 15) / 16 * 16;
   if (apply) s += (size_t)kUnionBlockRows * fg * 16;
@@ -568,24 +575,24 @@ template <typename R, int MODE>
 static void launch_union_lds(const BoxTables& bt, hipStream_t st, int fg, int nf, const double* pv, const c64* w,
                              c64* corr, cx<R>* out, long long N, cx<R>* out2 = nullptr, double a1 = 0.0,
                              double a2 = 0.0) {
-  const size_t lds = union_lds_bytes(bt.ctot, fg, MODE != 0);
+  const size_t lds = union_lds_bytes(bt.ucmax, fg, MODE != 0);
   const dim3 grid(bt.nublk, (nf + fg - 1) / fg);
   if (fg == 16)
-    hipLaunchKernelGGL((k_union_corr_lds<16, R, MODE>), grid, dim3(kBoxThreads), lds, st, bt.ublk, bt.pptr, bt.pc,
-                       pv, w, nf, bt.ctot, corr, bt.U.upt, out, N, out2, a1, a2);
+    hipLaunchKernelGGL((k_union_corr_lds<16, R, MODE>), grid, dim3(kBoxThreads), lds, st, bt.ublk, bt.pptr, bt.pcl,
+                       bt.ucptr, bt.ucol, pv, w, nf, bt.ucmax, corr, bt.U.upt, out, N, out2, a1, a2);
   else if (fg == 8)
-    hipLaunchKernelGGL((k_union_corr_lds<8, R, MODE>), grid, dim3(kBoxThreads), lds, st, bt.ublk, bt.pptr, bt.pc,
-                       pv, w, nf, bt.ctot, corr, bt.U.upt, out, N, out2, a1, a2);
+    hipLaunchKernelGGL((k_union_corr_lds<8, R, MODE>), grid, dim3(kBoxThreads), lds, st, bt.ublk, bt.pptr, bt.pcl,
+                       bt.ucptr, bt.ucol, pv, w, nf, bt.ucmax, corr, bt.U.upt, out, N, out2, a1, a2);
   else
-    hipLaunchKernelGGL((k_union_corr_lds<4, R, MODE>), grid, dim3(kBoxThreads), lds, st, bt.ublk, bt.pptr, bt.pc,
-                       pv, w, nf, bt.ctot, corr, bt.U.upt, out, N, out2, a1, a2);
+    hipLaunchKernelGGL((k_union_corr_lds<4, R, MODE>), grid, dim3(kBoxThreads), lds, st, bt.ublk, bt.pptr, bt.pcl,
+                       bt.ucptr, bt.ucol, pv, w, nf, bt.ucmax, corr, bt.U.upt, out, N, out2, a1, a2);
 }
 
 hipError_t union_corr(const BoxTables& bt, hipStream_t st, int nf, int which, const c64* w, c64* corr) {
   if (bt.natom == 0 || nf == 0 || bt.U.U == 0) return hipSuccess;
   prof_begin(st, KC_BOX_ADD);
   const double* pv = which ? bt.pv1 : bt.pv;
-  const int fg = bt.nublk > 0 ? union_lds_fields(bt.ctot, false) : 0;
+  const int fg = bt.nublk > 0 ? union_lds_fields(bt.ucmax, false) : 0;
   if (fg > 0) {
     launch_union_lds<double, 0>(bt, st, fg, nf, pv, w, corr, nullptr, 0);
   } else {
@@ -601,7 +608,7 @@ template <typename R>
 hipError_t union_corr_apply(const GridGeom& g, const BoxTables& bt, hipStream_t st, int nf, int which,
                             const c64* w, c64* corr, cx<R>* out) {
   if (bt.natom == 0 || nf == 0 || bt.U.U == 0) return hipSuccess;
-  const int fg = bt.nublk > 0 ? union_lds_fields(bt.ctot, true) : 0;
+  const int fg = bt.nublk > 0 ? union_lds_fields(bt.ucmax, true) : 0;
   if (fg == 0) {
     hipError_t e = union_corr(bt, st, nf, which, w, corr);
     if (e) return e;
@@ -637,7 +644,7 @@ hipError_t union_rk4_apply(const GridGeom& g, const BoxTables& bt, hipStream_t s
                            cx<R>* y, cx<R>* acc, double a1, double a2) {
   if (bt.natom == 0 || nf == 0 || bt.U.U == 0) return hipSuccess;
   const long long N = (long long)g.nx * g.ny * g.nz;
-  const int fg = bt.nublk > 0 ? union_lds_fields(bt.ctot, true) : 0;
+  const int fg = bt.nublk > 0 ? union_lds_fields(bt.ucmax, true) : 0;
   if (fg == 0) {
     hipError_t e = union_corr(bt, st, nf, 1, w, corr);
     if (e) return e;

@@ -163,6 +163,8 @@ struct opaw_ctx {
   int* uzr = nullptr;        // nz+1: union points of plane z are [uzr[z], uzr[z+1])
   int2* ublk = nullptr;
   int nublk = 0;
+  int *ucptr = nullptr, *ucol = nullptr, *pcl = nullptr;
+  int ucmax = 0;
   int *optr = nullptr, *ovoff = nullptr, *oc = nullptr;   // box overlap tables (k_bfix)
   double* ov = nullptr;
   // rebuild: nhat kept on the sphere union (nhs, nsys x S, before the rnel
@@ -894,6 +896,27 @@ int opaw_finalize_tables(opaw_ctx* c) {
       }
       c->nublk = (int)ublk.size();
       if (c->nublk > 0) RCHK(upload(c, &c->ublk, ublk));
+      // per block: the coefficients its pairs reference (sorted), and each
+      // pair's coefficient as an index into that list, so a block stages
+      // only the weight rows it uses
+      if (c->nublk > 0) {
+        std::vector<int> ucptr(1, 0), ucol, pcl(pc.size(), 0);
+        c->ucmax = 0;
+        for (const int2& bk2 : ublk) {
+          const int q0 = pptr[bk2.x], q1 = pptr[bk2.x + bk2.y];
+          std::vector<int> cols(pc.begin() + q0, pc.begin() + q1);
+          std::sort(cols.begin(), cols.end());
+          cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+          for (int q = q0; q < q1; ++q)
+            pcl[q] = (int)(std::lower_bound(cols.begin(), cols.end(), pc[q]) - cols.begin());
+          ucol.insert(ucol.end(), cols.begin(), cols.end());
+          ucptr.push_back((int)ucol.size());
+          c->ucmax = std::max(c->ucmax, (int)cols.size());
+        }
+        RCHK(upload(c, &c->ucptr, ucptr));
+        RCHK(upload(c, &c->ucol, ucol));
+        RCHK(upload(c, &c->pcl, pcl));
+      }
       RCHK(upload(c, &c->pptr, pptr));
       RCHK(upload(c, &c->pc, pc));
       RCHK(upload(c, &c->pv, pv));
@@ -1152,6 +1175,10 @@ static BoxTables box_tables(opaw_ctx* c) {
   b.pv1 = c->pv1;
   b.ublk = c->ublk;
   b.nublk = c->nublk;
+  b.ucptr = c->ucptr;
+  b.ucol = c->ucol;
+  b.pcl = c->pcl;
+  b.ucmax = c->ucmax;
   b.optr = c->optr;
   b.ovoff = c->ovoff;
   b.oc = c->oc;
