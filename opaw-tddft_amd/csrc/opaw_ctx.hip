@@ -234,14 +234,19 @@ struct opaw_ctx {
     double *bank = nullptr, *err = nullptr;
   };
   std::vector<Diis> diis;
-  DevBuf gsY, gsHY, gsK, gsT, gsPart, gsG, gsM, gsSmall;
+  DevBuf gsY, gsHY, gsK, gsT, gsPart, gsG, gsM, gsSmall, gsSmallC;
   struct Snap {
     DevBuf phi, r;     // phi_t and r_t; after prepare: r_t and e_t in k space
-    std::vector<double> t, y;
+    DevBuf ty;         // after prepare: t then y on the device
+    DevBuf agg;        // after prepare: shell sums z (nb x nsh complex), then w2 (nb x nsh)
+    std::vector<double> t, y, ee;   // ee: per band sum |e(k)|^2
     int sys = 0, nb = 0;
     bool ready = false;
   };
   std::vector<Snap> snaps;
+  // k shells (sets of k points with equal ek) for the training sums
+  DevBuf shPerm, shOff, shEk;
+  int nsh = 0;
 
   size_t esize() const { return prec == OPAW_PREC_F64 ? sizeof(c64) : sizeof(c32); }
 };
@@ -2300,21 +2305,28 @@ int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, in
   double* dy = dt + nb;
   HCHK(hipMemcpyAsync(Y, phi, fb, hipMemcpyDeviceToDevice, c->st));
   RCHK(apply_h(c, sys, h_type, Y, HY, nb));
-  std::vector<double> e(nb), t, y(nb);
-  bool have_p = false;
-  int it = 0;
-  for (; it < maxit; ++it) {
+  std::vector<double> e(nb), t(nb), y(nb);
+  {
+    // Rayleigh quotients of the start; later iterations carry the Ritz values
     std::vector<std::complex<double>> G;
     RCHK(gram_m(c, Y, HY, nb, G));
     for (int i = 0; i < nb; ++i) e[i] = G[(size_t)i * nb + i].real();
+  }
+  bool have_p = false;
+  int rc;
+  // one partial-sum buffer for the residual norms and the kinetic sums
+  double* part = gs_part(c, (size_t)kGsChunks * nb * 3, &rc);
+  RCHK(rc);
+  std::vector<double> h((size_t)kGsChunks * nb * 3);
+  int it = 0;
+  for (; it < maxit; ++it) {
     HCHK(hipMemcpyAsync(deig, e.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->st));
     c64* W = Y + off;
     HCHK(residual_vectors(c->st, HY, Y, deig, N, nb, W));
-    int rc;
-    double* part = gs_part(c, (size_t)kGsChunks * nb, &rc);
-    RCHK(rc);
     HCHK(residual_partials<double>(c->st, HY, Y, deig, N, nb, part));
-    std::vector<double> h((size_t)kGsChunks * nb);
+    // band kinetic energies from the same download
+    HCHK(fft3d<double>(c->g, c->st, -1, Y, Kb, nb, twx, twy, twz));
+    HCHK(kinetic_partials(c->st, Kb, c->ek, N, nb, part + (size_t)kGsChunks * nb));
     HCHK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HCHK(hipStreamSynchronize(c->st));
     double worst = 0.0;
@@ -2325,10 +2337,17 @@ int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, in
     }
     if (hist) hist[it] = worst;
     if (worst <= tol) break;
-    // band kinetic energies and the preconditioned residuals
-    HCHK(fft3d<double>(c->g, c->st, -1, Y, Kb, nb, twx, twy, twz));
-    RCHK(band_kinetic(c, Kb, nb, t));
-    for (int i = 0; i < nb; ++i) y[i] = e[i] / t[i];
+    const double* kp = h.data() + (size_t)kGsChunks * nb;
+    for (int i = 0; i < nb; ++i) {
+      double a = 0.0, b = 0.0;
+      for (int ch = 0; ch < kGsChunks; ++ch) {
+        a += kp[((size_t)ch * nb + i) * 2];
+        b += kp[((size_t)ch * nb + i) * 2 + 1];
+      }
+      t[i] = (b > 0.0 ? a / b : 0.0) * (double)N;   // ek carries 1/N
+      if (!(t[i] > 1e-6)) t[i] = 1e-6;
+      y[i] = e[i] / t[i];
+    }
     HCHK(hipMemcpyAsync(dt, t.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->st));
     HCHK(hipMemcpyAsync(dy, y.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->st));
     if (record > 0 && it % record == 0) {
@@ -2347,11 +2366,18 @@ int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, in
     HCHK(precond_apply(c->st, Kb, N, nb, c->ek, dt, dy, dmlp, shape));
     HCHK(fft3d<double>(c->g, c->st, 1, Kb, W, nb, twx, twy, twz));
     RCHK(apply_h(c, sys, h_type, W, HY + off, nb));
-    // Rayleigh-Ritz on [phi, w, p] (LOBPCG; no p in the first iteration)
+    // Rayleigh-Ritz on [phi, w, p] (LOBPCG; no p in the first iteration);
+    // both Gram matrices in one download
     const int m = have_p ? 3 * nb : 2 * nb;
-    std::vector<std::complex<double>> S, H, Ccol;
-    RCHK(gram_m(c, Y, Y, m, S));
-    RCHK(gram_m(c, Y, HY, m, H));
+    std::vector<std::complex<double>> S((size_t)m * m), H((size_t)m * m), Ccol;
+    RCHK(ensure(c, c->gsG, (size_t)kGramChunks * m * m * sizeof(c64)));
+    RCHK(ensure(c, c->gsM, (size_t)2 * m * m * sizeof(c64)));
+    HCHK(gram<double>(c->st, Y, Y, N, m, dvol(c), (c64*)c->gsG.p, (c64*)c->gsM.p));
+    HCHK(gram<double>(c->st, Y, HY, N, m, dvol(c), (c64*)c->gsG.p, (c64*)c->gsM.p + (size_t)m * m));
+    HCHK(hipMemcpyAsync(S.data(), c->gsM.p, (size_t)m * m * sizeof(c64), hipMemcpyDeviceToHost, c->st));
+    HCHK(hipMemcpyAsync(H.data(), (c64*)c->gsM.p + (size_t)m * m, (size_t)m * m * sizeof(c64),
+                        hipMemcpyDeviceToHost, c->st));
+    HCHK(hipStreamSynchronize(c->st));
     for (int a = 0; a < m; ++a)
       for (int b = a + 1; b < m; ++b) {
         const std::complex<double> hs = 0.5 * (S[(size_t)a * m + b] + std::conj(S[(size_t)b * m + a]));
@@ -2364,8 +2390,8 @@ int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, in
     std::vector<std::complex<double>> Cp((size_t)(m - nb) * nb);
     for (int j = 0; j < nb; ++j)
       for (int i = nb; i < m; ++i) Cp[(i - nb) + (size_t)(m - nb) * j] = Ccol[i + (size_t)m * j];
-    RCHK(ensure(c, c->gsM, (size_t)(m + (m - nb)) * nb * sizeof(c64)));
-    c64* dC = (c64*)c->gsM.p;
+    RCHK(ensure(c, c->gsSmallC, (size_t)(m + (m - nb)) * nb * sizeof(c64)));
+    c64* dC = (c64*)c->gsSmallC.p;
     c64* dCp = dC + (size_t)m * nb;
     HCHK(hipMemcpyAsync(dC, Ccol.data(), Ccol.size() * sizeof(c64), hipMemcpyHostToDevice, c->st));
     HCHK(hipMemcpyAsync(dCp, Cp.data(), Cp.size() * sizeof(c64), hipMemcpyHostToDevice, c->st));
@@ -2385,6 +2411,33 @@ int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, in
   HCHK(hipStreamSynchronize(c->st));
   std::copy(e.begin(), e.end(), eig);
   c->prep_valid = false;
+  return OPAW_OK;
+}
+
+// k shells: the k points sorted by ek, grouped by equal value
+static int ensure_shells(opaw_ctx* c) {
+  if (c->nsh > 0) return OPAW_OK;
+  const long long N = c->N;
+  std::vector<double> ek((size_t)N);
+  HCHK(hipMemcpy(ek.data(), c->ek, (size_t)N * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<int> perm((size_t)N);
+  for (long long k = 0; k < N; ++k) perm[(size_t)k] = (int)k;
+  std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return ek[a] < ek[b]; });
+  std::vector<int> off;
+  std::vector<double> eks;
+  for (long long q = 0; q < N; ++q)
+    if (q == 0 || ek[perm[(size_t)q]] != ek[perm[(size_t)q - 1]]) {
+      off.push_back((int)q);
+      eks.push_back(ek[perm[(size_t)q]]);
+    }
+  off.push_back((int)N);
+  RCHK(ensure(c, c->shPerm, (size_t)N * sizeof(int)));
+  RCHK(ensure(c, c->shOff, off.size() * sizeof(int)));
+  RCHK(ensure(c, c->shEk, eks.size() * sizeof(double)));
+  HCHK(hipMemcpy(c->shPerm.p, perm.data(), (size_t)N * sizeof(int), hipMemcpyHostToDevice));
+  HCHK(hipMemcpy(c->shOff.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+  HCHK(hipMemcpy(c->shEk.p, eks.data(), eks.size() * sizeof(double), hipMemcpyHostToDevice));
+  c->nsh = (int)eks.size();
   return OPAW_OK;
 }
 
@@ -2422,6 +2475,33 @@ int opaw_precond_prepare(opaw_ctx* c, int sys) {
     HCHK(fft3d<double>(c->g, c->st, -1, T0, (c64*)sp.phi.p, nb, twx, twy, twz));
     HCHK(fft3d<double>(c->g, c->st, -1, (const c64*)sp.r.p, T0, nb, twx, twy, twz));
     HCHK(hipMemcpyAsync(sp.r.p, T0, fb, hipMemcpyDeviceToDevice, c->st));
+    RCHK(ensure(c, sp.ty, 2 * (size_t)nb * sizeof(double)));
+    HCHK(hipMemcpyAsync(sp.ty.p, sp.t.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HCHK(hipMemcpyAsync((double*)sp.ty.p + nb, sp.y.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->st));
+    // shell sums, and |e_i|^2 from the full-sample sums (independent of M)
+    RCHK(ensure_shells(c));
+    const int nsh = c->nsh;
+    RCHK(ensure(c, sp.agg, (size_t)nb * nsh * (sizeof(c64) + sizeof(double))));
+    c64* z = (c64*)sp.agg.p;
+    double* w2 = (double*)(z + (size_t)nb * nsh);
+    HCHK(shell_aggregate(c->st, (const c64*)sp.r.p, (const c64*)sp.phi.p, N, nb, (const int*)c->shPerm.p,
+                         (const int*)c->shOff.p, nsh, z, w2));
+    {
+      int rc;
+      double* part = gs_part(c, (size_t)kGsChunks * nb * 4, &rc);
+      RCHK(rc);
+      const double* dt = (const double*)sp.ty.p;
+      MlpShape shape;
+      HCHK(precond_loss_sums(c->st, (const c64*)sp.r.p, (const c64*)sp.phi.p, N, nb, c->ek, dt, dt + nb, nullptr,
+                             shape, part));
+      std::vector<double> h((size_t)kGsChunks * nb * 4);
+      HCHK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+      HCHK(hipStreamSynchronize(c->st));
+      sp.ee.assign(nb, 0.0);
+      for (int i = 0; i < nb; ++i)
+        for (int ch = 0; ch < kGsChunks; ++ch) sp.ee[i] += h[((size_t)ch * nb + i) * 4 + 3];
+    }
+    HCHK(hipStreamSynchronize(c->st));
     sp.ready = true;
   }
   HCHK(hipStreamSynchronize(c->st));
@@ -2433,6 +2513,8 @@ int opaw_precond_clear(opaw_ctx* c) {
   for (auto& sp : c->snaps) {
     dfree(c, sp.phi.p);
     dfree(c, sp.r.p);
+    dfree(c, sp.ty.p);
+    dfree(c, sp.agg.p);
   }
   c->snaps.clear();
   return OPAW_OK;
@@ -2460,90 +2542,83 @@ int opaw_precond_loss(opaw_ctx* c, const double* mlp, int nh, double* loss, doub
   shape.nh = nh;
   const int npar = mlp_param_count(shape);
   const long long N = c->N;
-  int nready = 0;
+  std::vector<opaw_ctx::Snap*> ready;
   int nbmax = 0;
   for (auto& sp : c->snaps)
     if (sp.ready) {
-      ++nready;
+      ready.push_back(&sp);
       nbmax = std::max(nbmax, sp.nb);
     }
-  if (nready == 0) return fail(OPAW_ERR_STATE, "no prepared training pairs");
+  const int np = (int)ready.size();
+  if (np == 0) return fail(OPAW_ERR_STATE, "no prepared training pairs");
+  const int nsh = c->nsh;
   constexpr int kGradChunks = 512;
-  RCHK(ensure(c, c->gsSmall, (size_t)(npar + 5 * nbmax) * sizeof(double)));
+  // device scratch: parameters, gradient accumulator, per-pair coefficients
+  // (3 per band), per-pair per-band loss sums (3)
+  RCHK(ensure(c, c->gsSmall, (size_t)(2 * npar + 6 * (size_t)np * nbmax) * sizeof(double)));
   double* dP = (double*)c->gsSmall.p;
-  double* dt = dP + npar;
-  double* dy = dt + nbmax;
-  double* dA = dy + nbmax;        // 2 per band
-  double* dB = dA + 2 * nbmax;
-  HCHK(hipMemcpy(dP, mlp, npar * sizeof(double), hipMemcpyHostToDevice));
-  int rc;
-  double* part = gs_part(c, std::max((size_t)kGsChunks * nbmax * 4, (size_t)kGradChunks * npar), &rc);
-  RCHK(rc);
-  // first pass: per-band sums, to find the largest |e| and count the bands
-  struct Band { double ar, ai, b, ee; };
-  std::vector<std::vector<Band>> sums;
-  double eemax = 0.0;
-  for (auto& sp : c->snaps) {
-    if (!sp.ready) continue;
-    const int nb = sp.nb;
-    HCHK(hipMemcpyAsync(dt, sp.t.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->st));
-    HCHK(hipMemcpyAsync(dy, sp.y.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->st));
-    HCHK(precond_loss_sums(c->st, (const c64*)sp.r.p, (const c64*)sp.phi.p, N, nb, c->ek, dt, dy, dP, shape, part));
-    std::vector<double> h((size_t)kGsChunks * nb * 4);
-    HCHK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    HCHK(hipStreamSynchronize(c->st));
-    std::vector<Band> bs(nb, Band{0, 0, 0, 0});
-    for (int i = 0; i < nb; ++i) {
-      for (int ch = 0; ch < kGsChunks; ++ch) {
-        const double* q = &h[((size_t)ch * nb + i) * 4];
-        bs[i].ar += q[0];
-        bs[i].ai += q[1];
-        bs[i].b += q[2];
-        bs[i].ee += q[3];
-      }
-      eemax = std::max(eemax, bs[i].ee);
-    }
-    sums.push_back(bs);
+  double* dacc = dP + npar;
+  double* dcoef = dacc + npar;
+  double* dsum = dcoef + 3 * (size_t)np * nbmax;
+  HCHK(hipMemcpyAsync(dP, mlp, npar * sizeof(double), hipMemcpyHostToDevice, c->st));
+  for (int k = 0; k < np; ++k) {
+    const auto* sp = ready[k];
+    const double* dt = (const double*)sp->ty.p;
+    const c64* z = (const c64*)sp->agg.p;
+    const double* w2 = (const double*)(z + (size_t)sp->nb * nsh);
+    HCHK(loss_shells(c->st, z, w2, nsh, sp->nb, (const double*)c->shEk.p, N, dt, dt + sp->nb, dP, shape,
+                     dsum + (size_t)k * nbmax * 3));
   }
+  std::vector<double> h((size_t)np * nbmax * 3);
+  HCHK(hipMemcpyAsync(h.data(), dsum, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HCHK(hipStreamSynchronize(c->st));
+  double eemax = 0.0;
+  for (auto* sp : ready)
+    for (double e : sp->ee) eemax = std::max(eemax, e);
   int count = 0;
-  for (auto& bs : sums)
-    for (auto& b : bs)
-      if (b.ee > 1e-12 * eemax && b.b > 0.0) ++count;
+  for (int k = 0; k < np; ++k)
+    for (int i = 0; i < ready[k]->nb; ++i)
+      if (ready[k]->ee[i] > 1e-12 * eemax && h[((size_t)k * nbmax + i) * 3 + 2] > 0.0) ++count;
   if (count == 0) return fail(OPAW_ERR_STATE, "every training pair has a zero error vector");
   double L = 0.0;
-  std::vector<double> g(npar, 0.0);
-  int si = 0;
-  for (auto& sp : c->snaps) {
-    if (!sp.ready) continue;
-    const auto& bs = sums[si++];
-    const int nb = sp.nb;
-    std::vector<double> cA(2 * nb, 0.0), cB(nb, 0.0);
+  std::vector<double> coef((size_t)3 * np * nbmax, 0.0);
+  for (int k = 0; k < np; ++k) {
+    const int nb = ready[k]->nb;
+    double* cA = coef.data() + (size_t)3 * k * nbmax;   // 2 per band, then cB
+    double* cB = cA + 2 * nbmax;
     for (int i = 0; i < nb; ++i) {
-      const Band& b = bs[i];
-      if (!(b.ee > 1e-12 * eemax && b.b > 0.0)) continue;
-      const double a2 = b.ar * b.ar + b.ai * b.ai;
-      L += 1.0 - a2 / (b.ee * b.b);
-      // dL/dg = [-2 Re(conj(A) conj(e) r) + 2 |A|^2 m |r|^2 / B] m / (Ee B), over count
-      const double s = 1.0 / (b.ee * b.b * count);
-      cA[2 * i] = -2.0 * b.ar * s;       // conj(A) = ar - i ai: Re(conj(A) z) = ar Re z + ai Im z
-      cA[2 * i + 1] = 2.0 * b.ai * s;    // kernel forms cA0 Re z - cA1 Im z
-      cB[i] = 2.0 * a2 / b.b * s;
+      const double* q = &h[((size_t)k * nbmax + i) * 3];
+      const double ar = q[0], ai = q[1], bb = q[2], ee = ready[k]->ee[i];
+      if (!(ee > 1e-12 * eemax && bb > 0.0)) continue;
+      const double a2 = ar * ar + ai * ai;
+      L += 1.0 - a2 / (ee * bb);
+      // dL/dg = [-2 Re(conj(A) z) + 2 |A|^2 m w2 / B] m / (Ee B), over count
+      const double sc = 1.0 / (ee * bb * count);
+      cA[2 * i] = -2.0 * ar * sc;       // conj(A) = ar - i ai: Re(conj(A) z) = ar Re z + ai Im z
+      cA[2 * i + 1] = 2.0 * ai * sc;    // kernel forms cA0 Re z - cA1 Im z
+      cB[i] = 2.0 * a2 / bb * sc;
     }
-    if (!grad) continue;
-    HCHK(hipMemcpyAsync(dt, sp.t.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->st));
-    HCHK(hipMemcpyAsync(dy, sp.y.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->st));
-    HCHK(hipMemcpyAsync(dA, cA.data(), 2 * nb * sizeof(double), hipMemcpyHostToDevice, c->st));
-    HCHK(hipMemcpyAsync(dB, cB.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->st));
-    HCHK(precond_grad(c->st, (const c64*)sp.r.p, (const c64*)sp.phi.p, N, nb, c->ek, dt, dy, dP, shape, dA, dB, part,
-                      kGradChunks));
-    std::vector<double> h((size_t)kGradChunks * npar);
-    HCHK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    HCHK(hipStreamSynchronize(c->st));
-    for (int ch = 0; ch < kGradChunks; ++ch)
-      for (int p = 0; p < npar; ++p) g[p] += h[(size_t)ch * npar + p];
   }
   *loss = L / count;
-  if (grad) std::copy(g.begin(), g.end(), grad);
+  if (!grad) return OPAW_OK;
+  int rc;
+  double* gpart = gs_part(c, (size_t)kGradChunks * npar, &rc);
+  RCHK(rc);
+  HCHK(hipMemcpyAsync(dcoef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+  HCHK(hipMemsetAsync(dacc, 0, npar * sizeof(double), c->st));
+  for (int k = 0; k < np; ++k) {
+    const auto* sp = ready[k];
+    const double* dt = (const double*)sp->ty.p;
+    const c64* z = (const c64*)sp->agg.p;
+    const double* w2 = (const double*)(z + (size_t)sp->nb * nsh);
+    const double* cA = dcoef + (size_t)3 * k * nbmax;
+    const int nchunk = (int)std::min<long long>(kGradChunks, ((long long)sp->nb * nsh + 63) / 64);
+    HCHK(grad_shells(c->st, z, w2, nsh, sp->nb, (const double*)c->shEk.p, N, dt, dt + sp->nb, dP, shape, cA,
+                     cA + 2 * nbmax, gpart, nchunk));
+    HCHK(colsum_accumulate(c->st, gpart, nchunk, npar, dacc));
+  }
+  HCHK(hipMemcpyAsync(grad, dacc, npar * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HCHK(hipStreamSynchronize(c->st));
   return OPAW_OK;
 }
 

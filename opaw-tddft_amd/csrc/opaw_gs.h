@@ -23,6 +23,8 @@ hipError_t real_dots(hipStream_t st, const double* a, long long N, int K, double
 // out = sum_k coef_k (bank_k + mix err_k)
 hipError_t diis_predict(hipStream_t st, const double* bank, const double* err, const double* coef, int K, double mix,
                         long long N, double* out);
+// acc[p] += sum over rows of part (nrow x npar), in a fixed order
+hipError_t colsum_accumulate(hipStream_t st, const double* part, int nrow, int npar, double* acc);
 hipError_t sub_real(hipStream_t st, const double* a, const double* b, long long n, double* out);
 hipError_t to_f32(hipStream_t st, const double* in, float* out, long long n);
 hipError_t scale_complex(hipStream_t st, c64* a, long long n, double s);
@@ -53,5 +55,17 @@ hipError_t precond_loss_sums(hipStream_t st, const c64* rk, const c64* ekf, long
 hipError_t precond_grad(hipStream_t st, const c64* rk, const c64* ekf, long long N, int nb, const double* ek,
                         const double* tkin, const double* yfeat, const double* params, MlpShape s,
                         const double* coefA, const double* coefB, double* gpart, int nchunk);
+
+// training on k shells (sets of k with equal ek): per pair and band, the
+// shell sums z_s = sum conj(e) r and w2_s = sum |r|^2 (perm lists the k of
+// shell s in [off[s], off[s+1])); the loss sums (A, B per band) and the
+// parameter gradient over the (band, shell) samples
+hipError_t shell_aggregate(hipStream_t st, const c64* rk, const c64* ekf, long long N, int nb, const int* perm,
+                           const int* off, int nsh, c64* z, double* w2);
+hipError_t loss_shells(hipStream_t st, const c64* z, const double* w2, int nsh, int nb, const double* eks, long long N,
+                       const double* tkin, const double* yfeat, const double* params, MlpShape s, double* out);
+hipError_t grad_shells(hipStream_t st, const c64* z, const double* w2, int nsh, int nb, const double* eks, long long N,
+                       const double* tkin, const double* yfeat, const double* params, MlpShape s, const double* coefA,
+                       const double* coefB, double* gpart, int nchunk);
 
 }  // namespace opaw

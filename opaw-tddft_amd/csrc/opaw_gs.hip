@@ -12,6 +12,7 @@
 #include "opaw_gs.h"
 
 namespace opaw {
+int mlp_param_count(const MlpShape& s);
 namespace {
 
 constexpr int kTB = 256;
@@ -369,7 +370,192 @@ __global__ __launch_bounds__(kTB) void k_sub_real(const double* __restrict__ a, 
   if (r < n) out[r] = a[r] - b[r];
 }
 
+// acc[p] += sum over rows b < nrow of part[b * npar + p]: one block per
+// parameter, strided partial sums and a fixed-order tree
+__global__ __launch_bounds__(kTB) void k_colsum_acc(const double* __restrict__ part, int nrow, int npar,
+                                                    double* __restrict__ acc) {
+  const int p = blockIdx.x;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nrow; b += kTB) s += part[(size_t)b * npar + p];
+  __shared__ double red[kTB];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = kTB / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) acc[p] += red[0];
+}
+
+// shell sums of one prepared pair: z[i][s] = sum over k of shell s of
+// conj(e_i(k)) r_i(k), w2[i][s] = sum |r_i(k)|^2; the k of shell s are
+// perm[off[s] .. off[s+1]); block = (shell s, band i)
+__global__ __launch_bounds__(kTB) void k_shell_agg(const c64* __restrict__ rk, const c64* __restrict__ ekf,
+                                                   long long N, const int* __restrict__ perm,
+                                                   const int* __restrict__ off, int nsh, c64* __restrict__ z,
+                                                   double* __restrict__ w2) {
+  const int sh = blockIdx.x, i = blockIdx.y;
+  const int a = off[sh], b = off[sh + 1];
+  double zr = 0.0, zi = 0.0, ww = 0.0;
+  for (int q = a + (int)threadIdx.x; q < b; q += kTB) {
+    const long long k = perm[q];
+    const c64 r = rk[(size_t)i * N + k], e = ekf[(size_t)i * N + k];
+    zr += e.x * r.x + e.y * r.y;
+    zi += e.x * r.y - e.y * r.x;
+    ww += r.x * r.x + r.y * r.y;
+  }
+  __shared__ double red[3][kTB];
+  red[0][threadIdx.x] = zr;
+  red[1][threadIdx.x] = zi;
+  red[2][threadIdx.x] = ww;
+  __syncthreads();
+  for (int o = kTB / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+      for (int t = 0; t < 3; ++t) red[t][threadIdx.x] += red[t][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    z[(size_t)i * nsh + sh] = mk(red[0][0], red[1][0]);
+    w2[(size_t)i * nsh + sh] = red[2][0];
+  }
+}
+
+// per band: A = sum_s M_s z_s, B = sum_s M_s^2 w2_s (one block per band)
+__global__ __launch_bounds__(kTB) void k_loss_shells(const c64* __restrict__ z, const double* __restrict__ w2, int nsh,
+                                                     const double* __restrict__ eks, double N,
+                                                     const double* __restrict__ tkin, const double* __restrict__ yfeat,
+                                                     const double* __restrict__ P, int nh, double* __restrict__ out) {
+  const int i = blockIdx.x;
+  double ar = 0.0, ai = 0.0, bb = 0.0;
+  for (int sh = threadIdx.x; sh < nsh; sh += kTB) {
+    Fwd f;
+    const double m = multiplier(P, nh, eks[sh] * N / tkin[i], yfeat[i], f);
+    const c64 v = z[(size_t)i * nsh + sh];
+    ar += m * v.x;
+    ai += m * v.y;
+    bb += m * m * w2[(size_t)i * nsh + sh];
+  }
+  __shared__ double red[3][kTB];
+  red[0][threadIdx.x] = ar;
+  red[1][threadIdx.x] = ai;
+  red[2][threadIdx.x] = bb;
+  __syncthreads();
+  for (int o = kTB / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+      for (int t = 0; t < 3; ++t) red[t][threadIdx.x] += red[t][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) out[(size_t)i * 3 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// parameter gradient over the (band, shell) samples q = i * nsh + s, as
+// k_precond_grad with conj(e) r -> z_s and |r|^2 -> w2_s
+__global__ __launch_bounds__(kTile) void k_grad_shells(const c64* __restrict__ z, const double* __restrict__ w2,
+                                                       int nsh, int nb, const double* __restrict__ eks, double N,
+                                                       const double* __restrict__ tkin,
+                                                       const double* __restrict__ yfeat, const double* __restrict__ P,
+                                                       int nh, const double* __restrict__ coefA,
+                                                       const double* __restrict__ coefB, int npar,
+                                                       double* __restrict__ gpart, int nchunk) {
+  __shared__ double s_u[kTile][2];
+  __shared__ double s_h1[kTile][kMaxH];
+  __shared__ double s_dz1[kTile][kMaxH];
+  __shared__ double s_h2[kTile][kMaxH];
+  __shared__ double s_dz2[kTile][kMaxH];
+  __shared__ double s_dg[kTile];
+  const long long total = (long long)nsh * nb;
+  long long lo, hi;
+  chunk_bounds(total, nchunk, blockIdx.x, lo, hi);
+  const double* W2 = P + 3 * nh;
+  const double* w3 = W2 + nh * nh + nh;
+  double acc[kSlots];
+  for (int slot = 0; slot < kSlots; ++slot) acc[slot] = 0.0;
+  for (long long t0 = lo; t0 < hi; t0 += kTile) {
+    const long long q = t0 + threadIdx.x;
+    if (q < hi) {
+      const int i = (int)(q / nsh);
+      const int sh = (int)(q - (long long)i * nsh);
+      Fwd f;
+      const double m = multiplier(P, nh, eks[sh] * N / tkin[i], yfeat[i], f);
+      const c64 v = z[q];
+      const double dg = (coefA[2 * i] * v.x - coefA[2 * i + 1] * v.y + coefB[i] * m * w2[q]) * m;
+      s_u[threadIdx.x][0] = f.u0;
+      s_u[threadIdx.x][1] = f.u1;
+      s_dg[threadIdx.x] = dg;
+      double dz2[kMaxH];
+      for (int a = 0; a < nh; ++a) {
+        dz2[a] = dg * w3[a] * (1.0 - f.h2[a] * f.h2[a]);
+        s_h2[threadIdx.x][a] = f.h2[a];
+        s_dz2[threadIdx.x][a] = dz2[a];
+        s_h1[threadIdx.x][a] = f.h1[a];
+      }
+      for (int b = 0; b < nh; ++b) {
+        double dh1 = 0.0;
+        for (int a = 0; a < nh; ++a) dh1 += W2[a * nh + b] * dz2[a];
+        s_dz1[threadIdx.x][b] = dh1 * (1.0 - f.h1[b] * f.h1[b]);
+      }
+    }
+    __syncthreads();
+    const int ns = (int)(hi - t0 < kTile ? hi - t0 : kTile);
+    for (int slot = 0; slot < kSlots; ++slot) {
+      const int p = threadIdx.x + slot * kTile;
+      if (p >= npar) break;
+      double sm = 0.0;
+      if (p < 2 * nh) {
+        const int a = p / 2, cc = p % 2;
+        for (int t = 0; t < ns; ++t) sm += s_dz1[t][a] * s_u[t][cc];
+      } else if (p < 3 * nh) {
+        const int a = p - 2 * nh;
+        for (int t = 0; t < ns; ++t) sm += s_dz1[t][a];
+      } else if (p < 3 * nh + nh * nh) {
+        const int a = (p - 3 * nh) / nh, b = (p - 3 * nh) % nh;
+        for (int t = 0; t < ns; ++t) sm += s_dz2[t][a] * s_h1[t][b];
+      } else if (p < 4 * nh + nh * nh) {
+        const int a = p - 3 * nh - nh * nh;
+        for (int t = 0; t < ns; ++t) sm += s_dz2[t][a];
+      } else if (p < 5 * nh + nh * nh) {
+        const int a = p - 4 * nh - nh * nh;
+        for (int t = 0; t < ns; ++t) sm += s_dg[t] * s_h2[t][a];
+      } else {
+        for (int t = 0; t < ns; ++t) sm += s_dg[t];
+      }
+      acc[slot] += sm;
+    }
+    __syncthreads();
+  }
+  for (int slot = 0; slot < kSlots; ++slot) {
+    const int p = threadIdx.x + slot * kTile;
+    if (p < npar) gpart[(size_t)blockIdx.x * npar + p] = acc[slot];
+  }
+}
+
 }  // namespace
+
+hipError_t shell_aggregate(hipStream_t st, const c64* rk, const c64* ekf, long long N, int nb, const int* perm,
+                           const int* off, int nsh, c64* z, double* w2) {
+  hipLaunchKernelGGL(k_shell_agg, dim3(nsh, nb), dim3(kTB), 0, st, rk, ekf, N, perm, off, nsh, z, w2);
+  return hipGetLastError();
+}
+
+hipError_t loss_shells(hipStream_t st, const c64* z, const double* w2, int nsh, int nb, const double* eks, long long N,
+                       const double* tkin, const double* yfeat, const double* params, MlpShape s, double* out) {
+  hipLaunchKernelGGL(k_loss_shells, dim3(nb), dim3(kTB), 0, st, z, w2, nsh, eks, (double)N, tkin, yfeat, params, s.nh,
+                     out);
+  return hipGetLastError();
+}
+
+hipError_t grad_shells(hipStream_t st, const c64* z, const double* w2, int nsh, int nb, const double* eks, long long N,
+                       const double* tkin, const double* yfeat, const double* params, MlpShape s, const double* coefA,
+                       const double* coefB, double* gpart, int nchunk) {
+  hipLaunchKernelGGL(k_grad_shells, dim3(nchunk), dim3(kTile), 0, st, z, w2, nsh, nb, eks, (double)N, tkin, yfeat,
+                     params, s.nh, coefA, coefB, mlp_param_count(s), gpart, nchunk);
+  return hipGetLastError();
+}
+
+hipError_t colsum_accumulate(hipStream_t st, const double* part, int nrow, int npar, double* acc) {
+  hipLaunchKernelGGL(k_colsum_acc, dim3(npar), dim3(kTB), 0, st, part, nrow, npar, acc);
+  return hipGetLastError();
+}
 
 hipError_t sub_real(hipStream_t st, const double* a, const double* b, long long n, double* out) {
   hipLaunchKernelGGL(k_sub_real, dim3(blocks_for(n)), dim3(kTB), 0, st, a, b, n, out);

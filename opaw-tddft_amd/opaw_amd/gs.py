@@ -31,7 +31,7 @@ class ScfResult:
     iterations: int
     seconds: float
     history: List[float] = field(default_factory=list)   # largest residual per SCF iteration
-    h_applications: int = 0    # band applications of shs
+    h_applications: int = 0    # band applications of shs (power iterations, filter, residual checks, LOBPCG)
 
 
 def _occupied_residual(g, eig, nocc):
@@ -94,7 +94,7 @@ def scf_block(g, nocc: int, nscf: int = 30, tol: float = 1e-8, inner: int = 3, m
             eig, h, its = g.block_solve(inner_max, inner_tol, mlp=mlp, nh=nh, record=record, nconv=nocc)
         else:
             eig, h, its = g.block_solve(inner, 0.0, mlp=mlp, nh=nh, record=record, nconv=nocc)
-        napp += g.nb * (2 * its + 1)
+        napp += g.nb * (its + 1)
         res = _occupied_residual(g, eig, nocc)
         napp += g.nb
         hist.append(res)
