@@ -114,6 +114,51 @@ module opaw_hip_mod
       character(kind=c_char) :: dir(*)
       integer(c_int) :: it
     end function
+    integer(c_int) function opaw_set_nocc(ctx, nocc) bind(C, name="opaw_set_nocc")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nocc
+    end function
+    integer(c_int) function opaw_set_diis(ctx, ndi, mix) bind(C, name="opaw_set_diis")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ndi
+      real(c_double), value :: mix
+    end function
+    integer(c_int) function opaw_ncpaw_make_hamiltonian(ctx, sys, use_diis) &
+        bind(C, name="opaw_ncpaw_make_hamiltonian")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: sys, use_diis
+    end function
+    integer(c_int) function opaw_hminmax(ctx, sys, h_type, niter, fctr, seed, hmin, hmax) &
+        bind(C, name="opaw_hminmax")
+      import :: c_ptr, c_int, c_double, c_long_long
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: sys, h_type, niter
+      real(c_double), value :: fctr
+      integer(c_long_long), value :: seed
+      real(c_double) :: hmin, hmax
+    end function
+    integer(c_int) function opaw_chebyshev_filter(ctx, sys, h_type, degree, hbot, hmax) &
+        bind(C, name="opaw_chebyshev_filter")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: sys, h_type, degree
+      real(c_double), value :: hbot, hmax
+    end function
+    integer(c_int) function opaw_orthog_phi(ctx, sys, h_type, eig) bind(C, name="opaw_orthog_phi")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: sys, h_type
+      real(c_double) :: eig(*)
+    end function
+    integer(c_int) function opaw_diagh(ctx, sys, h_type, eig) bind(C, name="opaw_diagh")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: sys, h_type
+      real(c_double) :: eig(*)
+    end function
     function opaw_last_error() bind(C, name="opaw_last_error")
       import :: c_ptr
       type(c_ptr) :: opaw_last_error
@@ -123,9 +168,11 @@ module opaw_hip_mod
   public :: opaw_create, opaw_destroy, opaw_ek_prep, opaw_set_atoms, opaw_set_projectors, opaw_set_sphere, &
             opaw_set_type, opaw_finalize_tables, opaw_set_potentials, opaw_vk_prep, opaw_write_orbitals, &
             opaw_read_orbitals, opaw_make_hamiltonian_tddft, opaw_rk4_prop_shs, opaw_plot_dip, &
-            opaw_excite_psi_bar_pert, opaw_write_restart, opaw_read_restart, opaw_check
+            opaw_excite_psi_bar_pert, opaw_write_restart, opaw_read_restart, opaw_set_nocc, opaw_set_diis, &
+            opaw_ncpaw_make_hamiltonian, opaw_hminmax, opaw_chebyshev_filter, opaw_orthog_phi, opaw_diagh, &
+            opaw_check
   public :: gpu_rk4_prop_shs, gpu_ncpaw_make_hamiltonian_tddft, gpu_plot_dip, gpu_write_to_restart, &
-            gpu_read_phi_bar
+            gpu_read_phi_bar, gpu_ncpaw_make_hamiltonian, gpu_get_hminmax, gpu_chebyr_m, gpu_orthog_phi, gpu_diagh
 
 contains
 
@@ -185,6 +232,39 @@ contains
     integer(c_int) :: it
     call opaw_check(opaw_read_restart(opaw_ctx, './tddft_save'//c_null_char, it), 'read_phi_bar')
     it_restart = it
+  end subroutine
+
+  ! ground-state SCF drop-ins (src/main.f90:66-95, h_type 1); the orbitals
+  ! phit_tot of the single system stay resident between the calls
+
+  ! drop-in for ncpaw_make_hamiltonian (src/ncpaw_make_hamiltonian.f90:1)
+  subroutine gpu_ncpaw_make_hamiltonian()
+    call opaw_check(opaw_ncpaw_make_hamiltonian(opaw_ctx, 0_c_int, 1_c_int), 'ncpaw_make_hamiltonian')
+  end subroutine
+
+  ! drop-in for get_hminmax (src/hminmax.f90:18): niter 25, fctr 1.2
+  subroutine gpu_get_hminmax(hmin, hmax)
+    real(c_double), intent(out) :: hmin, hmax
+    call opaw_check(opaw_hminmax(opaw_ctx, 0_c_int, 1_c_int, 25_c_int, 1.2d0, 12345_c_long_long, hmin, hmax), &
+                    'get_hminmax')
+  end subroutine
+
+  ! drop-in for chebyr_m(hmax, hbot) (src/chebyr_m.f90:1), nch = 10
+  subroutine gpu_chebyr_m(hmax, hbot)
+    real(c_double), intent(in) :: hmax, hbot
+    call opaw_check(opaw_chebyshev_filter(opaw_ctx, 0_c_int, 1_c_int, 10_c_int, hbot, hmax), 'chebyr_m')
+  end subroutine
+
+  ! drop-in for orthog_phi (src/orthog_phi.f90:1)
+  subroutine gpu_orthog_phi(ovl)
+    real(c_double) :: ovl(*)
+    call opaw_check(opaw_orthog_phi(opaw_ctx, 0_c_int, 1_c_int, ovl), 'orthog_phi')
+  end subroutine
+
+  ! drop-in for diagh (src/diag.f90:1): eig ascending
+  subroutine gpu_diagh(eig)
+    real(c_double) :: eig(*)
+    call opaw_check(opaw_diagh(opaw_ctx, 0_c_int, 1_c_int, eig), 'diagh')
   end subroutine
 
 end module opaw_hip_mod
