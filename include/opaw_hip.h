@@ -63,6 +63,11 @@ enum {
 const char* opaw_last_error(void);
 int opaw_version(void);
 int opaw_fft_length_supported(int n);
+/* 1 when the kinetic operator of nx x ny planes runs as the one-visit xy-plane
+ * kernel at this precision (OPAW_PREC_F64 / OPAW_PREC_MIXED), 0 when it runs
+ * as line passes.  A property of this library's kernels; the reference's
+ * FFTW calls (src/lib/library_fftw_v3.f90:56-83) have no such split. */
+int opaw_plane_path(int nx, int ny, int precision);
 
 /* ---- context: grid and orbital arrays ------------------------------------
  * Replaces the grid part of main_mod (src/4_main.f90:5-50) and the orbital
@@ -171,6 +176,16 @@ int opaw_plot_dip(opaw_ctx* ctx, double sm, double* d_dip, double* dip, double* 
  * rebuild of all systems, rk4_prop_shs, and the dipole; d_dip (nt,3) and
  * dip (nt, 3*nsys) may be NULL.  No host synchronisation inside the loop. */
 int opaw_tddft_run(opaw_ctx* ctx, int nt, double dt, double sm, double* d_dip, double* dip);
+/* The same time loop for several independent contexts at once (molecules of
+ * their own grids, orbital counts and kick systems), replacing one
+ * opaw_tddft (src/tddft.f90:67-91) run per molecule.  Each context runs on a
+ * stream of its own, handed over from and back to its caller's stream, so the
+ * kernels of different molecules overlap on the device.  graph != 0: after
+ * the first step, one time step per context is captured as a HIP graph and
+ * replayed for the remaining nt-1 steps.  dip receives, context after
+ * context, each one's (nt, 3*nsys) series (may be NULL).  No context may be
+ * used by another thread during the call. */
+int opaw_tddft_run_batch(opaw_ctx* const* ctxs, int nctx, int nt, double dt, int graph, double* dip);
 
 /* ---- orthogonalisation / subspace diagonalisation ------------------------- */
 /* orthog_phi (src/orthog_phi.f90:1-194): canonical phi U lambda^-1/2 of the

@@ -133,6 +133,9 @@ struct opaw_ctx {
   // with the two events, so callers only ever see st
   hipStream_t st2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // own non-blocking stream of the multi-context time loop (created on first
+  // use), so the molecules of one batch run side by side
+  hipStream_t bst = nullptr;
   Prof prof;
   std::vector<void*> allocs;
   long long bytes = 0;
@@ -205,6 +208,7 @@ struct opaw_ctx {
   double* part = nullptr;
   double* dipbuf = nullptr;
   int dipcap = 0;
+  double* dipstep = nullptr;  // 3 nsys: the dipole slot of a replayed time step
 
   // orbitals + work
   void* phi = nullptr;
@@ -336,6 +340,10 @@ extern "C" {
 const char* opaw_last_error(void) { return g_err.c_str(); }
 int opaw_version(void) { return 10000; }
 int opaw_fft_length_supported(int n) { return fft_size_supported(n) ? 1 : 0; }
+int opaw_plane_path(int nx, int ny, int precision) {
+  if (precision == OPAW_PREC_F64) return plane_supported<double>(nx, ny) ? 1 : 0;
+  return plane_supported<float>(nx, ny) ? 1 : 0;
+}
 
 int opaw_create(opaw_ctx** out, int nx, int ny, int nz, double dx, double dy, double dz, int periodic,
                 int precision, int nsys, int nb) {
@@ -416,6 +424,7 @@ int opaw_destroy(opaw_ctx* c) {
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->st2) (void)hipStreamDestroy(c->st2);
+  if (c->bst) (void)hipStreamDestroy(c->bst);
   for (void* p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
   delete c;
@@ -1758,40 +1767,154 @@ int opaw_plot_dip(opaw_ctx* c, double sm, double* d_dip, double* dip, double* di
   return OPAW_OK;
 }
 
-int opaw_tddft_run(opaw_ctx* c, int nt, double dt, double sm, double* d_dip, double* dip) {
-  CTX_CHECK(c);
+// one iteration of the time loop for every system of c: the rebuild, then
+// rk4_prop_shs with the dipole of the step written to slot (3 nsys doubles,
+// device).  Enqueues only; no host synchronisation.
+static int tddft_step(opaw_ctx* c, double dt, double* slot) {
+  if (c->prec == OPAW_PREC_F64) {
+    RCHK(flow_rebuild<double>(c, 0, c->nsys, (const c64*)c->phi));
+    return flow_rk4<double>(c, dt, slot);
+  }
+  RCHK(flow_rebuild<float>(c, 0, c->nsys, (const c32*)c->phi));
+  return flow_rk4<float>(c, dt, slot);
+}
+
+static int tddft_ready(opaw_ctx* c, int nt) {
   if (c->nocc != c->nb) return fail(OPAW_ERR_STATE, "the propagation runs on occupied orbitals only (nocc = nb)");
   RCHK(need_tables(c));
+  if (!c->pot_ready) return fail(OPAW_ERR_STATE, "potentials not set (opaw_set_potentials)");
+  if (!c->vk_ready) return fail(OPAW_ERR_STATE, "Hartree kernel not set (opaw_vk_prep)");
   if (nt <= 0) return fail(OPAW_ERR_ARG, "nt must be positive");
   if (c->dipcap < nt) {
     dfree(c, c->dipbuf);
+    c->dipbuf = nullptr;
+    c->dipcap = 0;
     RCHK(dalloc_t(c, &c->dipbuf, (size_t)3 * c->nsys * nt));
     c->dipcap = nt;
   }
+  return OPAW_OK;
+}
+
+// dipole series of the last nt steps: dip (nt, 3 nsys) and, from systems 0
+// and 1, d_dip (nt, 3) = (dip_pert - dip) / sm; either may be NULL
+static int dipoles_download(opaw_ctx* c, int nt, double sm, double* d_dip, double* dip) {
+  if (!d_dip && !dip) return OPAW_OK;
+  std::vector<double> h((size_t)3 * c->nsys * nt);
+  HCHK(hipStreamSynchronize(c->st));
+  HCHK(hipMemcpy(h.data(), c->dipbuf, h.size() * sizeof(double), hipMemcpyDeviceToHost));
   for (int it = 0; it < nt; ++it) {
-    if (c->prec == OPAW_PREC_F64)
-      RCHK(flow_rebuild<double>(c, 0, c->nsys, (const c64*)c->phi));
-    else
-      RCHK(flow_rebuild<float>(c, 0, c->nsys, (const c32*)c->phi));
-    // rk4_prop_shs with the dipole of the step (plot_dip) fused into the
-    // norm restore
-    double* slot = c->dipbuf + (size_t)3 * c->nsys * it;
-    if (c->prec == OPAW_PREC_F64)
-      RCHK(flow_rk4<double>(c, dt, slot));
-    else
-      RCHK(flow_rk4<float>(c, dt, slot));
+    const double* r = h.data() + (size_t)3 * c->nsys * it;
+    if (dip)
+      for (int k = 0; k < 3 * c->nsys; ++k) dip[(size_t)3 * c->nsys * it + k] = r[k];
+    if (d_dip)
+      for (int k = 0; k < 3; ++k) d_dip[(size_t)3 * it + k] = c->nsys > 1 ? (r[3 + k] - r[k]) / sm : 0.0;
   }
-  if (d_dip || dip) {
-    std::vector<double> h((size_t)3 * c->nsys * nt);
-    HCHK(hipStreamSynchronize(c->st));
-    HCHK(hipMemcpy(h.data(), c->dipbuf, h.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (int it = 0; it < nt; ++it) {
-      const double* r = h.data() + (size_t)3 * c->nsys * it;
-      if (dip)
-        for (int k = 0; k < 3 * c->nsys; ++k) dip[(size_t)3 * c->nsys * it + k] = r[k];
-      if (d_dip)
-        for (int k = 0; k < 3; ++k) d_dip[(size_t)3 * it + k] = c->nsys > 1 ? (r[3 + k] - r[k]) / sm : 0.0;
+  return OPAW_OK;
+}
+
+int opaw_tddft_run(opaw_ctx* c, int nt, double dt, double sm, double* d_dip, double* dip) {
+  CTX_CHECK(c);
+  RCHK(tddft_ready(c, nt));
+  for (int it = 0; it < nt; ++it) RCHK(tddft_step(c, dt, c->dipbuf + (size_t)3 * c->nsys * it));
+  return dipoles_download(c, nt, sm, d_dip, dip);
+}
+
+// one time step of c recorded (not run) from its stream into an executable
+// graph; the dipole of a replay lands in c->dipstep
+static int capture_step(opaw_ctx* c, double dt, hipGraphExec_t* exec) {
+  *exec = nullptr;
+  HCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+  const int r = tddft_step(c, dt, c->dipstep);
+  hipGraph_t gr = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->st, &gr);
+  if (r != OPAW_OK || e != hipSuccess) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return r != OPAW_OK ? r : fail(OPAW_ERR_HIP, "stream capture of a time step: %s", hipGetErrorString(e));
+  }
+  const hipError_t ei = hipGraphInstantiate(exec, gr, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(gr);
+  if (ei != hipSuccess) {
+    *exec = nullptr;
+    return fail(OPAW_ERR_HIP, "graph instantiation of a time step: %s", hipGetErrorString(ei));
+  }
+  return OPAW_OK;
+}
+
+// the time loop of several contexts, each on its own stream: step 0 runs
+// eagerly (it also sizes every work buffer), then either the remaining steps
+// are enqueued context after context, or one step per context is captured
+// and its graph replayed
+static int batch_loop(opaw_ctx* const* cs, int nctx, int nt, double dt, int graph) {
+  for (int k = 0; k < nctx; ++k) {
+    ProfScope ps(cs[k]);
+    RCHK(tddft_step(cs[k], dt, cs[k]->dipbuf));
+  }
+  if (nt == 1) return OPAW_OK;
+  if (!graph) {
+    for (int it = 1; it < nt; ++it)
+      for (int k = 0; k < nctx; ++k) {
+        ProfScope ps(cs[k]);
+        RCHK(tddft_step(cs[k], dt, cs[k]->dipbuf + (size_t)3 * cs[k]->nsys * it));
+      }
+    return OPAW_OK;
+  }
+  std::vector<hipGraphExec_t> ex(nctx, nullptr);
+  int r = OPAW_OK;
+  for (int k = 0; k < nctx && r == OPAW_OK; ++k) r = capture_step(cs[k], dt, &ex[k]);
+  for (int it = 1; it < nt && r == OPAW_OK; ++it)
+    for (int k = 0; k < nctx && r == OPAW_OK; ++k) {
+      opaw_ctx* c = cs[k];
+      const size_t nd = (size_t)3 * c->nsys;
+      hipError_t e = hipGraphLaunch(ex[k], c->st);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(c->dipbuf + nd * it, c->dipstep, nd * sizeof(double), hipMemcpyDeviceToDevice, c->st);
+      if (e != hipSuccess) r = fail(OPAW_ERR_HIP, "graph replay of a time step: %s", hipGetErrorString(e));
     }
+  // the replays must have ended before their graphs go
+  for (int k = 0; k < nctx; ++k)
+    if (ex[k]) {
+      const hipError_t e = hipStreamSynchronize(cs[k]->st);
+      if (e != hipSuccess && r == OPAW_OK) r = fail(OPAW_ERR_HIP, "graph replay: %s", hipGetErrorString(e));
+      (void)hipGraphExecDestroy(ex[k]);
+    }
+  return r;
+}
+
+int opaw_tddft_run_batch(opaw_ctx* const* cs, int nctx, int nt, double dt, int graph, double* dip) {
+  if (!cs || nctx <= 0) return fail(OPAW_ERR_ARG, "no contexts");
+  for (int k = 0; k < nctx; ++k) {
+    opaw_ctx* c = cs[k];
+    if (!c) return fail(OPAW_ERR_ARG, "null context %d", k);
+    for (int j = 0; j < k; ++j)
+      if (cs[j] == c) return fail(OPAW_ERR_ARG, "context %d appears twice", k);
+    ProfScope ps(c);
+    RCHK(tddft_ready(c, nt));
+    if (graph && c->prof.on) return fail(OPAW_ERR_STATE, "graph replay with kernel timing enabled (context %d)", k);
+    if (!c->dipstep) RCHK(dalloc_t(c, &c->dipstep, (size_t)3 * c->nsys));
+    if (!c->bst) HCHK(hipStreamCreateWithFlags(&c->bst, hipStreamNonBlocking));
+  }
+  // each context moves from its caller's stream to its own and back
+  std::vector<hipStream_t> home(nctx);
+  for (int k = 0; k < nctx; ++k) {
+    opaw_ctx* c = cs[k];
+    home[k] = c->st;
+    HCHK(hipEventRecord(c->ev_fork, c->st));
+    HCHK(hipStreamWaitEvent(c->bst, c->ev_fork, 0));
+    c->st = c->bst;
+  }
+  int r = batch_loop(cs, nctx, nt, dt, graph);
+  for (int k = 0; k < nctx; ++k) {
+    opaw_ctx* c = cs[k];
+    hipError_t e = hipEventRecord(c->ev_join, c->bst);
+    if (e == hipSuccess) e = hipStreamWaitEvent(home[k], c->ev_join, 0);
+    if (e != hipSuccess && r == OPAW_OK) r = fail(OPAW_ERR_HIP, "stream hand-back: %s", hipGetErrorString(e));
+    c->st = home[k];
+  }
+  if (r != OPAW_OK || !dip) return r;
+  size_t off = 0;
+  for (int k = 0; k < nctx; ++k) {
+    RCHK(dipoles_download(cs[k], nt, 0.0, nullptr, dip + off));
+    off += (size_t)3 * cs[k]->nsys * nt;
   }
   return OPAW_OK;
 }

@@ -446,46 +446,48 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fft_plane(const cx<R>* in, cx
   }
 }
 
-// square planes with a compiled plane kernel (FP32 also 128 x 128)
-#define OPAW_PLANE_SIZES(X) X(16) X(24) X(32) X(48) X(64) X(80) X(96)
+// (nx, ny) planes with a compiled plane kernel: the squares and the
+// rectangles of the polyacene grids (64 x 48, 80 x 64, 96 x 64, 128 x 64);
+// a pair runs on the plane path where its unpadded plane and twiddles fit
+// the 160 KB of LDS (all in FP32; FP64 up to 96 x 96 and 128 x 64)
+#define OPAW_PLANE_PAIRS(X)                                                                              \
+  X(16, 16) X(24, 24) X(32, 32) X(48, 48) X(64, 64) X(80, 80) X(96, 96) X(128, 128) X(64, 48) X(80, 64) \
+  X(96, 64) X(128, 64)
+
+template <typename R, int NX, int NY>
+constexpr bool plane_fits() {
+  return (long)NY * NX * (long)sizeof(cx<R>) + (NX + NY) * (long)sizeof(cx<R>) <= 163840;
+}
 
 template <typename R>
 bool plane_supported(int nx, int ny) {
-  if (nx != ny) return false;
-  switch (nx) {
-#define OPAW_CASE(n) case n: return true;
-    OPAW_PLANE_SIZES(OPAW_CASE)
+#define OPAW_CASE(a, b) \
+  if (nx == a && ny == b) return plane_fits<R, a, b>();
+  OPAW_PLANE_PAIRS(OPAW_CASE)
 #undef OPAW_CASE
-    case 128: return sizeof(R) == 4;
-    default: return false;
-  }
+  return false;
 }
 template bool plane_supported<double>(int, int);
 template bool plane_supported<float>(int, int);
 
 template <typename R, int S, int EPI>
-static hipError_t launch_plane(int n, hipStream_t st, const cx<R>* in, cx<R>* out, int nz, int nf, long long N,
-                               const cx<R>* tw, const cx<R>* uin, const R* vks, int nfps, const int* upt,
-                               const int* uzr, const c64* cpre, const c64* cnl, int U,
+static hipError_t launch_plane(int nx, int ny, hipStream_t st, const cx<R>* in, cx<R>* out, int nz, int nf,
+                               long long N, const cx<R>* twx, const cx<R>* twy, const cx<R>* uin, const R* vks,
+                               int nfps, const int* upt, const int* uzr, const c64* cpre, const c64* cnl, int U,
                                PlaneRk4<R> rk = PlaneRk4<R>{}) {
   dim3 grid((unsigned)((long long)nf * nz));
-  switch (n) {
-#define OPAW_CASE(nn)                                                                                          \
-  case nn:                                                                                                     \
-    hipLaunchKernelGGL((k_fft_plane<R, nn, nn, S, EPI>), grid, dim3(kPlaneThreads), 0, st, in, out, nz, N, tw, tw, \
-                       uin, vks, nfps, upt, uzr, cpre, cnl, U, rk);                                                     \
-    return hipGetLastError();
-    OPAW_PLANE_SIZES(OPAW_CASE)
-#undef OPAW_CASE
-    case 128:
-      if constexpr (sizeof(R) == 4) {
-        hipLaunchKernelGGL((k_fft_plane<R, 128, 128, S, EPI>), grid, dim3(kPlaneThreads), 0, st, in, out, nz, N, tw,
-                           tw, uin, vks, nfps, upt, uzr, cpre, cnl, U, rk);
-        return hipGetLastError();
-      }
-      return hipErrorInvalidValue;
-    default: return hipErrorInvalidValue;
+#define OPAW_CASE(a, b)                                                                                      \
+  if (nx == a && ny == b) {                                                                                  \
+    if constexpr (plane_fits<R, a, b>()) {                                                                   \
+      hipLaunchKernelGGL((k_fft_plane<R, a, b, S, EPI>), grid, dim3(kPlaneThreads), 0, st, in, out, nz, N, twx, \
+                         twy, uin, vks, nfps, upt, uzr, cpre, cnl, U, rk);                                   \
+      return hipGetLastError();                                                                              \
+    }                                                                                                        \
+    return hipErrorInvalidValue;                                                                             \
   }
+  OPAW_PLANE_PAIRS(OPAW_CASE)
+#undef OPAW_CASE
+  return hipErrorInvalidValue;
 }
 
 // ---------------------------------------------------------------------------
@@ -638,22 +640,22 @@ hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* in, cx
   hipError_t e;
   if (plane_supported<R>(g.nx, g.ny)) {
     prof_begin(st, KC_FFT_X);
-    e = launch_plane<R, -1, EPI_NONE>(g.nx, st, in, tmp, g.nz, nf, N, twx, nullptr, nullptr, nf_per_sys, upt, uzr,
-                                      cpre, nullptr, U);
+    e = launch_plane<R, -1, EPI_NONE>(g.nx, g.ny, st, in, tmp, g.nz, nf, N, twx, twy, nullptr, nullptr, nf_per_sys,
+                                      upt, uzr, cpre, nullptr, U);
     prof_end(st, KC_FFT_X);
     if (e) return e;
     if ((e = fft_s_pass<R>(KC_FFT_Z, g.nz, 2, -1, EPI_MULT, st, tmp, v, tmp, v, Lz, twz, ek, vt))) return e;
     if (before_inv && (e = hipStreamWaitEvent(st, before_inv, 0))) return e;
     prof_begin(st, KC_FFT_XH);
     if (rk == nullptr)
-      e = launch_plane<R, 1, EPI_HPSI>(g.nx, st, tmp, out, g.nz, nf, N, twx, in, vks, nf_per_sys, upt, uzr, cpre,
-                                       cnl, U);
+      e = launch_plane<R, 1, EPI_HPSI>(g.nx, g.ny, st, tmp, out, g.nz, nf, N, twx, twy, in, vks, nf_per_sys, upt,
+                                       uzr, cpre, cnl, U);
     else if (rk->first)
-      e = launch_plane<R, 1, EPI_RK4F>(g.nx, st, tmp, out, g.nz, nf, N, twx, in, vks, nf_per_sys, upt, uzr, cpre,
-                                       cnl, U, *rk);
+      e = launch_plane<R, 1, EPI_RK4F>(g.nx, g.ny, st, tmp, out, g.nz, nf, N, twx, twy, in, vks, nf_per_sys, upt,
+                                       uzr, cpre, cnl, U, *rk);
     else
-      e = launch_plane<R, 1, EPI_RK4>(g.nx, st, tmp, out, g.nz, nf, N, twx, in, vks, nf_per_sys, upt, uzr, cpre,
-                                      cnl, U, *rk);
+      e = launch_plane<R, 1, EPI_RK4>(g.nx, g.ny, st, tmp, out, g.nz, nf, N, twx, twy, in, vks, nf_per_sys, upt,
+                                      uzr, cpre, cnl, U, *rk);
     prof_end(st, KC_FFT_XH);
     return e;
   }

@@ -29,6 +29,7 @@ SIGNATURES = {
     "opaw_last_error": (C.c_char_p, []),
     "opaw_version": (I, []),
     "opaw_fft_length_supported": (I, [I]),
+    "opaw_plane_path": (I, [I, I, I]),
     "opaw_create": (I, [C.POINTER(P), I, I, I, D, D, D, I, I, I, I]),
     "opaw_destroy": (I, [P]),
     "opaw_set_stream": (I, [P, P]),
@@ -67,6 +68,7 @@ SIGNATURES = {
     "opaw_rk4_prop_shs": (I, [P, D]),
     "opaw_plot_dip": (I, [P, D, DP, DP, DP]),
     "opaw_tddft_run": (I, [P, I, D, D, DP, DP]),
+    "opaw_tddft_run_batch": (I, [C.POINTER(C.c_void_p), I, I, D, I, DP]),
     "opaw_orthog_phi": (I, [P, I, I, DP]),
     "opaw_diagh": (I, [P, I, I, DP]),
     "opaw_write_wf": (I, [C.c_char_p, I, I, I, I, I, D, D, D, I, I, DP, P]),

@@ -86,8 +86,47 @@ void run_gen(int N, int LINES, bool LFAST, int S) {
       std::printf("%d %d %d %d %.17e %.17e\n", N, S, l, j, src[l * LSTR + j * ESTR].x, src[l * LSTR + j * ESTR].y);
 }
 
+// the xy-plane pass of the kinetic operator: NY rows of length NX (row
+// stride LS, the padded LDS row) and then NX columns of length NY, as the
+// plane kernel runs them with 512 threads; prints element (y, x)
+template <int NX, int NY, int LS, int S>
+void run_plane() {
+  constexpr int THREADS = 512;
+  std::vector<cx<double>> twx(NX), twy(NY), lds(NY * LS);
+  for (int m = 0; m < NX; ++m) {
+    long double a = -2.0L * 3.14159265358979323846264338327950288L * m / NX;
+    twx[m] = mk<double>((double)std::cos(a), (double)std::sin(a));
+  }
+  for (int m = 0; m < NY; ++m) {
+    long double a = -2.0L * 3.14159265358979323846264338327950288L * m / NY;
+    twy[m] = mk<double>((double)std::cos(a), (double)std::sin(a));
+  }
+  for (int y = 0; y < NY; ++y)
+    for (int x = 0; x < NX; ++x)
+      lds[y * LS + x] = mk<double>(std::sin(0.37 * (x + 1) + 0.7 * y), std::cos(0.11 * x * x + 0.5 * y));
+  if (S < 0) {
+    host_stages<NX, 1, NY, THREADS, false, LS, 1, S>(lds.data(), twx.data());
+    host_stages<NY, 1, NX, THREADS, true, 1, LS, S>(lds.data(), twy.data());
+  } else {
+    host_stages<NY, 1, NX, THREADS, true, 1, LS, S>(lds.data(), twy.data());
+    host_stages<NX, 1, NY, THREADS, false, LS, 1, S>(lds.data(), twx.data());
+  }
+  for (int y = 0; y < NY; ++y)
+    for (int x = 0; x < NX; ++x)
+      std::printf("%d %d %d %d %d %.17e %.17e\n", NX, NY, S, y, x, lds[y * LS + x].x, lds[y * LS + x].y);
+}
+
 #include <array>
-int main() {
+#include <cstring>
+int main(int argc, char** argv) {
+  if (argc > 1 && std::strcmp(argv[1], "plane") == 0) {
+    run_plane<64, 48, 72, -1>();
+    run_plane<80, 64, 88, 1>();
+    run_plane<96, 64, 104, -1>();
+    run_plane<128, 64, 136, 1>();
+    run_plane<48, 48, 56, -1>();
+    return 0;
+  }
   // radices 7, 11, 13 and their mixes with 2, 3, 5 (e.g. the 56 x 52 x 32
   // naphthalene grid and its doubled 112 x 104 x 64 box)
   const int gen_sizes[] = {7, 14, 28, 56, 112, 224, 13, 26, 52, 104, 208, 11, 22, 44, 77, 91, 143, 36, 60, 100, 18, 27};

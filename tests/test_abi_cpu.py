@@ -58,6 +58,14 @@ def test_library_loads_and_exports_every_symbol():
         assert L.opaw_fft_length_supported(n) == 1, n
     for n in (17, 34, 512, 1):
         assert L.opaw_fft_length_supported(n) == 0, n
+    # xy planes with a one-visit plane kernel: the squares and the polyacene
+    # rectangles; FP64 only where the plane fits the LDS
+    F64, MIX = lib.OPAW_PREC_F64, lib.OPAW_PREC_MIXED
+    for nx, ny in ((48, 48), (64, 64), (96, 96), (64, 48), (80, 64), (96, 64), (128, 64)):
+        assert L.opaw_plane_path(nx, ny, F64) == 1 and L.opaw_plane_path(nx, ny, MIX) == 1, (nx, ny)
+    assert L.opaw_plane_path(128, 128, MIX) == 1 and L.opaw_plane_path(128, 128, F64) == 0
+    for nx, ny in ((48, 40), (40, 48), (56, 52), (48, 64), (192, 192)):
+        assert L.opaw_plane_path(nx, ny, F64) == 0 and L.opaw_plane_path(nx, ny, MIX) == 0, (nx, ny)
 
 
 def test_null_context_is_an_error_not_a_crash():

@@ -7,9 +7,11 @@ cases do not reach, all through the C ABI.
   mixed path against FP64 over all 128 orbitals (<= 2e-5).
 * One full time step of the benchmark workload (C10H8, 64^3 MT, 24 orbitals x
   2 systems) against the oracle: orbitals <= 1e-10, d_dip <= 1e-8.
-* Grids with nx != ny (the five-pass line path of the kinetic operator, with
-  the corrections gathered per point) and odd radices: h_phi, shs and one
-  RK4 step against the oracle.
+* Rectangular xy planes with their own plane kernel (64 x 48, 96 x 64,
+  128 x 64): h_phi, shs and one RK4 step against the oracle, mixed against FP64.
+* Grids with nx != ny and no plane kernel (the five-pass line path of the
+  kinetic operator, with the corrections gathered per point) and odd radices:
+  h_phi, shs and one RK4 step against the oracle.
 * 12 projectors per atom (the 16-wide projection, overlap and finish kernels)
   on a periodic system: operators and one RK4 step against the oracle.
 * No atoms at all, and a single orbital.
@@ -110,6 +112,42 @@ def test_time_step_c10h8_64_full_size():
         assert rel(g.read_orbitals(s), new[s]) < 1e-10, s
     assert np.max(np.abs(d_gpu[0] - d_dip)) <= 1e-8 * np.max(np.abs(d_dip))
     g.close()
+
+
+@pytest.mark.parametrize("shape", [(64, 48, 40), (128, 64, 24), (96, 64, 32)])
+def test_rectangular_plane_grids(shape):
+    """nx != ny with a compiled rectangular plane kernel (the polyacene
+    grids): H.psi, shs and one time step against the oracle in FP64, and the
+    mixed time step against FP64."""
+    torch, OpawGPU = _api()
+    from opaw_amd import lib
+    assert lib.load().opaw_plane_path(shape[0], shape[1], lib.OPAW_PREC_F64) == 1
+    spec = synth.molecule_system(nring=2, n=shape, d=0.5, seed=23)
+    nb = 3
+    spec.rnel = 2.0 * nb
+    st = state_from_spec(spec, nsys=2)
+    _potentials(spec, st, 4)
+    phi = _orbitals(spec, nb, seed=8, envelope=3.0)
+    g = OpawGPU(spec, "f64", nsys=2, nb=nb)
+    _push(g, st)
+    x = g.to_device(phi)
+    for sys in (0, 1):
+        assert rel(g.h_phi(x, sys).cpu().numpy(), O.h_phi(st, phi, sys)) < 1e-10
+        assert rel(g.shs(x, sys).cpu().numpy(), O.shs(st, phi, sys)) < 1e-10
+    g.close()
+    st2 = state_from_spec(spec, nsys=2)
+    phis = [phi, O.excite_psi_bar_pert(st2, phi, spec.sm, spec.ipol)]
+    new, d_dip, _, _ = O.tddft_step(st2, phis, spec.sm)
+    for prec, tol in (("f64", 1e-10), ("mixed", 2e-5)):
+        g = OpawGPU(spec, prec, nsys=2, nb=nb)
+        g.write_orbitals(0, phi)
+        g.excite_psi_bar_pert(spec.sm, spec.ipol + 1)
+        d_gpu, _ = g.opaw_tddft(1, spec.dt, spec.sm)
+        for s in (0, 1):
+            assert rel(g.read_orbitals(s).astype(np.complex128), new[s]) < tol, (prec, s)
+        if prec == "f64":
+            assert np.max(np.abs(d_gpu[0] - d_dip)) <= 1e-8 * np.max(np.abs(d_dip))
+        g.close()
 
 
 @pytest.mark.parametrize("shape", [(48, 40, 32), (40, 48, 24)])
