@@ -133,6 +133,12 @@ struct opaw_ctx {
   // with the two events, so callers only ever see st
   hipStream_t st2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // time loop: the rebuild's potential chain stays on st2 for the next shs
+  // stage (pot_pending), whose forward passes wait only for corrA (ev_corr);
+  // ev_nhat hands the main stream's rho_ij / nhat to that chain
+  hipEvent_t ev_corr = nullptr, ev_nhat = nullptr;
+  bool pot_pending = false;
+  double* mpart = nullptr;   // partials of the periodic potential means (side stream)
   // own non-blocking stream of the multi-context time loop (created on first
   // use), so the molecules of one batch run side by side
   hipStream_t bst = nullptr;
@@ -395,6 +401,7 @@ int opaw_create(opaw_ctx** out, int nx, int ny, int nz, double dx, double dy, do
   if (r == OPAW_OK) r = dalloc_t(c, &c->nrm1, nf);
   if (r == OPAW_OK) r = dalloc_t(c, &c->means, nsys + 1);
   if (r == OPAW_OK) r = dalloc_t(c, &c->part, (size_t)kChunks * std::max<size_t>(nf, 4 * (size_t)nsys + 4));
+  if (r == OPAW_OK) r = dalloc_t(c, &c->mpart, (size_t)kChunks * ((size_t)nsys + 2));
   if (r == OPAW_OK) r = dalloc(c, &c->phi, nf * N * c->esize());
   if (r == OPAW_OK) r = dalloc_t(c, &c->pair, (size_t)((nsys + 1) / 2) * N);
   if (r == OPAW_OK) r = dalloc_t(c, &c->hwork, (size_t)c->gbig.nx * c->gbig.ny * c->gbig.nz);
@@ -407,6 +414,10 @@ int opaw_create(opaw_ctx** out, int nx, int ny, int nz, double dx, double dy, do
                        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming | hipEventReleaseToDevice) !=
                            hipSuccess ||
                        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming | hipEventReleaseToDevice) !=
+                           hipSuccess ||
+                       hipEventCreateWithFlags(&c->ev_corr, hipEventDisableTiming | hipEventReleaseToDevice) !=
+                           hipSuccess ||
+                       hipEventCreateWithFlags(&c->ev_nhat, hipEventDisableTiming | hipEventReleaseToDevice) !=
                            hipSuccess))
     r = fail(OPAW_ERR_HIP, "side stream / events");
   if (r != OPAW_OK) {
@@ -423,6 +434,8 @@ int opaw_destroy(opaw_ctx* c) {
   c->prof.destroy();
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->ev_corr) (void)hipEventDestroy(c->ev_corr);
+  if (c->ev_nhat) (void)hipEventDestroy(c->ev_nhat);
   if (c->st2) (void)hipStreamDestroy(c->st2);
   if (c->bst) (void)hipStreamDestroy(c->bst);
   for (void* p : c->allocs) (void)hipFree(p);
@@ -1274,6 +1287,26 @@ static int join_side(opaw_ctx* c) {
   return OPAW_OK;
 }
 
+// after a failed time step that left a pipelined rebuild unconsumed: fold
+// the side stream back into st
+static int settle_side(opaw_ctx* c, int r) {
+  if (c->pot_pending) {
+    c->pot_pending = false;
+    (void)join_side(c);
+  }
+  return r;
+}
+
+// side stream of an shs stage: after a pipelined rebuild st2 already runs
+// that rebuild's potential chain (the stage's weights must follow it there)
+// and st needs only corrA before its forward passes; otherwise fork
+static int side_for_stage(opaw_ctx* c) {
+  if (!c->pot_pending) return fork_side(c);
+  c->pot_pending = false;
+  HCHK(hipStreamWaitEvent(c->st, c->ev_corr, 0));
+  return OPAW_OK;
+}
+
 // Coefficients of `in` for shs and the rebuild: A = P1^T in (weights
 // wA = sn(-1/2) A, compact correction corrA = S^-1/2 in - in) and
 // B = P^T S^-1/2 in = P^T in + O wA.  With side != 0 the corrA pass runs on
@@ -1337,7 +1370,7 @@ static int flow_hs(opaw_ctx* c, int sys0, const cx<R>* in, cx<R>* h, int nf, int
   // the nonlocal weights D(sys) cB and their union correction run on the
   // side stream while the forward plane and z passes (which need only
   // corrA) run on st; the inverse plane pass waits for them
-  RCHK(fork_side(c));
+  RCHK(side_for_stage(c));
   HCHK(box_weights(bt, c->st2, (const c64*)c->cB.p, nf, 2, nullptr, c->dij + (size_t)sys0 * c->mtot, nf_per_sys,
                    c->mtot, (c64*)c->wB.p));
   HCHK(union_corr(bt, c->st2, nf, 0, (const c64*)c->wB.p, (c64*)c->corrNL.p));
@@ -1363,7 +1396,7 @@ static int flow_hs_rk4(opaw_ctx* c, const cx<R>* yin, cx<R>* y, cx<R>* acc, cons
                        double cst) {
   RCHK(ensure(c, c->wtmp, (size_t)nf * c->N * sizeof(cx<R>)));
   const BoxTables bt = box_tables(c);
-  RCHK(fork_side(c));
+  RCHK(side_for_stage(c));
   HCHK(box_weights(bt, c->st2, (const c64*)c->cB.p, nf, 2, nullptr, c->dij, c->nb, c->mtot, (c64*)c->wB.p));
   HCHK(union_corr(bt, c->st2, nf, 0, (const c64*)c->wB.p, (c64*)c->corrNL.p));
   RCHK(mark_side(c));
@@ -1396,16 +1429,23 @@ static int flow_shs(opaw_ctx* c, int sys0, const cx<R>* in, cx<R>* out, int nf, 
 // psi~ = S^-1/2 phi (never stored: its coefficients A and B and its density are
 // gathered), rho_ij, density, nhat + rnel renormalisation, V_H + V_xc -> V_KS,
 // D_ij (src/ncpaw_make_hamiltonian.f90:213-246).
+//
+// pipelined (the time loop, whose next call is an shs stage): the potential
+// chain (renormalisation, Hartree, V_KS, D_ij) is left on the side stream and
+// the stage consumes it (side_for_stage): its forward kinetic passes need
+// only corrA, so they overlap the chain, and only its inverse pass waits.
 template <typename R>
-static int flow_rebuild(opaw_ctx* c, int s0, int ns, const cx<R>* phi) {
+static int flow_rebuild(opaw_ctx* c, int s0, int ns, const cx<R>* phi, bool pipelined = false) {
   if (!c->pot_ready) return fail(OPAW_ERR_STATE, "potentials not set (opaw_set_potentials)");
   if (!c->vk_ready) return fail(OPAW_ERR_STATE, "Hartree kernel not set (opaw_vk_prep)");
+  if (c->pot_pending) return fail(OPAW_ERR_STATE, "a pipelined rebuild was not consumed");
   const int nf = ns * c->nb;
   const long long N = c->N;
   const double dv = dvol(c);
   // the coefficients with corrA and then the density pass (which needs
   // corrA) on the side stream, overlapping B, rho_ij and nhat on st
   RCHK(flow_prep<R>(c, phi, nf, true));
+  if (pipelined) HCHK(hipEventRecord(c->ev_corr, c->st2));
   const int dstride = (int)((N + 255) / 256);
   double* dpart = c->rpart;
   double* npart = c->rpart + (size_t)c->nsys * dstride;
@@ -1420,26 +1460,37 @@ static int flow_rebuild(opaw_ctx* c, int s0, int ns, const cx<R>* phi) {
   HCHK(nhat_sparse(c->st, sphere_tabs(c), c->S, c->sph_g, c->wl, c->natom, c->nl2, ns,
                    c->nhs + (size_t)s0 * std::max(1, c->S), npart, &nn));
   prof_end(c->st, KC_DENS);
-  RCHK(join_side(c));
-  prof_begin(c->st, KC_POT);
-  HCHK(renorm_pack(c->st, c->dens + s0 * N, c->smap, c->nhs + (size_t)s0 * std::max(1, c->S), std::max(1, c->S),
+  hipStream_t sp = c->st;   // stream of the potential chain
+  if (pipelined) {
+    HCHK(hipEventRecord(c->ev_nhat, c->st));
+    HCHK(hipStreamWaitEvent(c->st2, c->ev_nhat, 0));
+    sp = c->st2;
+  } else {
+    RCHK(join_side(c));
+  }
+  prof_begin(sp, KC_POT);
+  HCHK(renorm_pack(sp, c->dens + s0 * N, c->smap, c->nhs + (size_t)s0 * std::max(1, c->S), std::max(1, c->S),
                    dpart, nd, npart, nn, dv, c->rnel, 1, N, ns, c->pair, c->fac + s0));
-  prof_end(c->st, KC_POT);
+  prof_end(sp, KC_POT);
   for (int s = s0; s < s0 + ns; ++s) c->nhat_lazy[s] = 1;
+  // the Hartree solve stays FP64 in both precisions: the kick response is the
+  // difference of two nearly equal systems, and an FP32 V_H moves the mixed
+  // dipole trace by ~4e-4 (relative) over 2000 steps
   const int npair = (ns + 1) / 2;
   for (int q = 0; q < npair; ++q)
-    HCHK(hartree_conv(c->g, c->scale, c->st, c->pair + q * N, c->pair + q * N, c->hwork, c->vk,
+    HCHK(hartree_conv(c->g, c->scale, sp, c->pair + q * N, c->pair + q * N, c->hwork, c->vk,
                       c->tw64[c->gbig.nx], c->tw64[c->gbig.ny], c->tw64[c->gbig.nz]));
-  if (c->periodic) HCHK(pair_means(c->st, c->pair, N, ns, c->part, c->means));
-  prof_begin(c->st, KC_POT);
-  HCHK(vks_build(c->st, c->dens + s0 * N, c->ncoret, c->pair, c->periodic ? c->means : nullptr, c->vloc, N, ns,
+  if (c->periodic) HCHK(pair_means(sp, c->pair, N, ns, c->mpart, c->means));
+  prof_begin(sp, KC_POT);
+  HCHK(vks_build(sp, c->dens + s0 * N, c->ncoret, c->pair, c->periodic ? c->means : nullptr, c->vloc, N, ns,
                  c->vxc + s0 * N, c->vh + s0 * N, c->vks + s0 * N, c->vksf + s0 * N));
-  prof_end(c->st, KC_POT);
-  prof_begin(c->st, KC_POT);
-  HCHK(dij(c->st, c->datoms, c->natom, c->rhoij + (size_t)s0 * c->mtot, c->mtot, c->qtab, c->d0tab, c->etab, c->nl2,
+  prof_end(sp, KC_POT);
+  prof_begin(sp, KC_POT);
+  HCHK(dij(sp, c->datoms, c->natom, c->rhoij + (size_t)s0 * c->mtot, c->mtot, c->qtab, c->d0tab, c->etab, c->nl2,
            c->sph_idx, c->sph_g, c->vks + s0 * N, c->vxc + s0 * N, c->usexcnhat == 0 ? 1 : 0, N, ns, dv,
            c->dij + (size_t)s0 * c->mtot));
-  prof_end(c->st, KC_POT);
+  prof_end(sp, KC_POT);
+  c->pot_pending = pipelined;
   c->prep_valid = (s0 == 0 && ns == c->nsys && (const void*)phi == c->phi);
   return OPAW_OK;
 }
@@ -1471,7 +1522,7 @@ static int flow_rk4(opaw_ctx* c, double dt, double* dip_out) {
     if (fused && stage < 4) {
       // stages 1-3: the update rides in the last kinetic pass
       RCHK(flow_hs_rk4<R>(c, yin, y, acc, p, nf, stage == 1, cst[stage - 1]));
-      RCHK(flow_rebuild<R>(c, 0, c->nsys, y));
+      RCHK(flow_rebuild<R>(c, 0, c->nsys, y, true));
       continue;
     }
     RCHK(flow_hs<R>(c, 0, yin, h, nf, c->nb));
@@ -1479,7 +1530,7 @@ static int flow_rk4(opaw_ctx* c, double dt, double* dip_out) {
     prof_begin(c->st, KC_RK4);
     HCHK(rk4_update<R>(c->st, stage, c->N, nf, h, p, acc, y, cst[stage - 1], dt, c->part, dv, c->nrm1));
     prof_end(c->st, KC_RK4);
-    if (stage < 4) RCHK(flow_rebuild<R>(c, 0, c->nsys, y));
+    if (stage < 4) RCHK(flow_rebuild<R>(c, 0, c->nsys, y, true));
   }
   // norm restore; inside the time loop fused with the dipole of the step
   prof_begin(c->st, KC_DENS);
@@ -1732,8 +1783,8 @@ int opaw_rk4_prop_shs(opaw_ctx* c, double dt) {
   CTX_CHECK(c);
   if (c->nocc != c->nb) return fail(OPAW_ERR_STATE, "the propagation runs on occupied orbitals only (nocc = nb)");
   RCHK(need_tables(c));
-  if (c->prec == OPAW_PREC_F64) return flow_rk4<double>(c, dt, nullptr);
-  return flow_rk4<float>(c, dt, nullptr);
+  if (c->prec == OPAW_PREC_F64) return settle_side(c, flow_rk4<double>(c, dt, nullptr));
+  return settle_side(c, flow_rk4<float>(c, dt, nullptr));
 }
 
 static int dipoles_to(opaw_ctx* c, double* dst) {
@@ -1771,12 +1822,15 @@ int opaw_plot_dip(opaw_ctx* c, double sm, double* d_dip, double* dip, double* di
 // rk4_prop_shs with the dipole of the step written to slot (3 nsys doubles,
 // device).  Enqueues only; no host synchronisation.
 static int tddft_step(opaw_ctx* c, double dt, double* slot) {
+  int r;
   if (c->prec == OPAW_PREC_F64) {
-    RCHK(flow_rebuild<double>(c, 0, c->nsys, (const c64*)c->phi));
-    return flow_rk4<double>(c, dt, slot);
+    r = flow_rebuild<double>(c, 0, c->nsys, (const c64*)c->phi, true);
+    if (r == OPAW_OK) r = flow_rk4<double>(c, dt, slot);
+  } else {
+    r = flow_rebuild<float>(c, 0, c->nsys, (const c32*)c->phi, true);
+    if (r == OPAW_OK) r = flow_rk4<float>(c, dt, slot);
   }
-  RCHK(flow_rebuild<float>(c, 0, c->nsys, (const c32*)c->phi));
-  return flow_rk4<float>(c, dt, slot);
+  return settle_side(c, r);
 }
 
 static int tddft_ready(opaw_ctx* c, int nt) {
