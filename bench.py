@@ -17,6 +17,11 @@ Both precisions run from the same initial orbitals: FP64, and the mixed path
 (relative L2) of the FP64 trace, else the FP64 rate.  The CPU baseline is the
 numpy oracle timed on a bounded sample (one full step) on this host.
 
+Beside the headline, `batched_polyacenes` reports the batched uneven
+workload: the polyacenes n = 1..6 on their own grids, 4 systems each, mixed
+path, all advanced together by opaw_tddft_run_batch, against the same
+molecules run one after another.
+
 Prints ONE JSON line.
 """
 import argparse
@@ -43,6 +48,9 @@ def parse():
     ap.add_argument("--precision", choices=["both", "f64", "mixed"], default="both")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="first run a plain 1-read/1-write kernel over MIB MiB (counter calibration)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched polyacene series (reported beside the headline line)")
+    ap.add_argument("--batch-steps", type=int, default=10)
     return ap.parse_args()
 
 
@@ -157,6 +165,46 @@ def run_gpu(spec, phi0, prec, steps, warmup, nb):
                 achieved_gbs=achieved, breakdown_ms_per_step=breakdown)
 
 
+def run_batch(steps):
+    """The batched uneven workload: polyacenes C(4n+2)H(2n+4), n = 1..6, on
+    their grids (48^3 ... 128x64x48 at 0.5 bohr), 9n+6 orbitals, 4 systems each
+    (unperturbed + kicks along x, y, z), mixed path; one batch step advances
+    all six molecules by one time step (opaw_tddft_run_batch, one stream per
+    molecule).  Also the same molecules run one after another."""
+    import numpy as np
+    import torch
+    from opaw_amd import batch
+    specs = [batch.polyacene_spec(n) for n in range(1, 7)]
+    phis = [batch.polyacene_orbitals(s, seed=20 + k) for k, s in enumerate(specs)]
+    dt = specs[0].dt
+    gs = [batch.kicked_context(s, p, "mixed", 4) for s, p in zip(specs, phis)]
+    alone = []
+    for g in gs:
+        g.opaw_tddft(2, dt, g.spec.sm)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.opaw_tddft(steps, dt, g.spec.sm)
+        torch.cuda.synchronize()
+        alone.append((time.perf_counter() - t0) / steps)
+    batch.tddft_batch(gs, 2, dt, graph=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch.tddft_batch(gs, steps, dt, graph=False)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    model = sum(step_bytes_model(g.spec, 4, g.nb, 8, 4) for g in gs)
+    for g in gs:
+        g.close()
+    return {"workload": "polyacenes n=1..6 (C6H6 ... C26H16), grids 48^3 ... 128x64x48 @0.5 bohr, "
+                        "15 ... 60 orbitals x 4 systems each, mixed path",
+            "steps": steps, "batch_steps_per_s": 1.0 / sec, "ms_per_batch_step": sec * 1e3,
+            "sequential_ms_per_batch_step": sum(alone) * 1e3,
+            "speedup_over_sequential": sum(alone) / sec,
+            "alone_ms_per_step": [round(a * 1e3, 3) for a in alone],
+            "hbm_model_gbs": model / sec / 1e9,
+            "propagations_per_batch_step": 6 * 4}
+
+
 def cpu_baseline(spec, phi0):
     """numpy oracle, one full time step (both systems), single thread."""
     try:
@@ -202,6 +250,13 @@ def main():
         res[prec] = run_gpu(spec, phi0, prec, a.steps, a.warmup, nb)
         print(f"[bench] {prec}: {res[prec]['steps_per_s']:.2f} steps/s, dominant {res[prec]['dominant']} "
               f"{res[prec]['achieved_gbs']:.0f} GB/s", file=sys.stderr, flush=True)
+
+    batched = None
+    if not a.no_batch:
+        batched = run_batch(a.batch_steps)
+        print(f"[bench] batched polyacenes: {batched['ms_per_batch_step']:.2f} ms per batch step "
+              f"({batched['speedup_over_sequential']:.2f}x the molecules one after another)",
+              file=sys.stderr, flush=True)
 
     dip_err = None
     if "f64" in res and "mixed" in res:
@@ -250,6 +305,7 @@ def main():
         if ("f64" in res and "mixed" in res) else None,
         "dipole_rel_l2_mixed_vs_fp64": dip_err,
         "breakdown_ms_per_step": r["breakdown_ms_per_step"],
+        "batched_polyacenes": batched,
     }
     print(json.dumps(out))
 

@@ -71,3 +71,35 @@ def kicked_context(spec, phi0: np.ndarray, precision: str, nsys: int = 4):
     for s in range(1, nsys):
         g.excite_psi_bar_pert(spec.sm, s, from_sys=0, to_sys=s)
     return g
+
+
+def main(argv=None):
+    """Time the batched polyacene series: python -m opaw_amd.batch."""
+    import argparse
+    import time
+    import torch
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", choices=["f64", "mixed"], default="mixed")
+    ap.add_argument("--graph", action="store_true", help="replay one captured step per molecule")
+    a = ap.parse_args(argv)
+    specs = [polyacene_spec(n) for n in range(1, 7)]
+    phis = [polyacene_orbitals(s, seed=20 + k) for k, s in enumerate(specs)]
+    gs = [kicked_context(s, p, a.precision, 4) for s, p in zip(specs, phis)]
+    dt = specs[0].dt
+    tddft_batch(gs, a.warmup, dt, graph=a.graph)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dips = tddft_batch(gs, a.steps, dt, graph=a.graph)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / a.steps
+    for g, d in zip(gs, dips):
+        assert np.all(np.isfinite(d))
+        g.close()
+    print(f"polyacenes n=1..6 x 4 systems, {a.precision}, graph={a.graph}: {sec * 1e3:.3f} ms per batch step "
+          f"({a.steps} steps)", flush=True)
+
+
+if __name__ == "__main__":
+    main()
