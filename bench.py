@@ -82,6 +82,28 @@ def copy_calibration(mib):
     print(f"[bench] copy calibration: 3 x {mib} MiB read + {mib} MiB write", file=sys.stderr, flush=True)
 
 
+def stream_bandwidth(mib=1024, reps=20):
+    """Achievable streaming rate on this device: out = in + 1 over mib MiB of
+    float32 (one read and one write per element), timed with events."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.add(a, 1.0, out=b)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        torch.add(a, 1.0, out=b)
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) / 1e3 / reps
+    del a, b
+    return 2 * mib * (1 << 20) / sec / 1e9
+
+
 def box_points(spec):
     return sum(int(a.nr[0] * a.nr[1] * a.nr[2]) for a in spec.atoms)
 
@@ -245,6 +267,8 @@ def main():
     print(f"[bench] workload ready: {spec.nx}x{spec.ny}x{spec.nz}, {len(spec.atoms)} atoms, {nb} orbitals",
           file=sys.stderr, flush=True)
 
+    stream_gbs = stream_bandwidth()
+    print(f"[bench] streaming read+write rate of this device: {stream_gbs:.0f} GB/s", file=sys.stderr, flush=True)
     res = {}
     for prec in (["f64", "mixed"] if a.precision == "both" else [a.precision]):
         res[prec] = run_gpu(spec, phi0, prec, a.steps, a.warmup, nb)
@@ -296,7 +320,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": r["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": r["achieved_gbs"] / HBM_PEAK_GBS, "traffic": measured_traffic(key, r["dominant"]),
                      "kernel": r["dominant"], "bytes_per_launch": r["dom_bytes"],
-                     "avg_launch_us": r["dom_avg_s"] * 1e6, "launches": r["dom_launches"]},
+                     "avg_launch_us": r["dom_avg_s"] * 1e6, "launches": r["dom_launches"],
+                     "stream_measured_gbs": stream_gbs, "frac_of_measured_stream": r["achieved_gbs"] / stream_gbs},
         "cpu_baseline": cpu,
         "hbm_model_gbs": model_gbs,
         "fp64_steps_per_s": res["f64"]["steps_per_s"] if "f64" in res else None,

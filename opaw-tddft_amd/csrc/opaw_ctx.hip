@@ -1370,9 +1370,12 @@ static int flow_hs(opaw_ctx* c, int sys0, const cx<R>* in, cx<R>* h, int nf, int
   // the nonlocal weights D(sys) cB and their union correction run on the
   // side stream while the forward plane and z passes (which need only
   // corrA) run on st; the inverse plane pass waits for them
+  // after a pipelined rebuild its D_ij kernel has formed the weights
+  const bool have_w = c->pot_pending && sys0 == 0 && nf == c->nsys * c->nb && nf_per_sys == c->nb;
   RCHK(side_for_stage(c));
-  HCHK(box_weights(bt, c->st2, (const c64*)c->cB.p, nf, 2, nullptr, c->dij + (size_t)sys0 * c->mtot, nf_per_sys,
-                   c->mtot, (c64*)c->wB.p));
+  if (!have_w)
+    HCHK(box_weights(bt, c->st2, (const c64*)c->cB.p, nf, 2, nullptr, c->dij + (size_t)sys0 * c->mtot, nf_per_sys,
+                     c->mtot, (c64*)c->wB.p));
   HCHK(union_corr(bt, c->st2, nf, 0, (const c64*)c->wB.p, (c64*)c->corrNL.p));
   RCHK(mark_side(c));
   HCHK(apply_t_plus_v<R>(c->g, c->st, in, h, (cx<R>*)c->wtmp.p, nf, nf_per_sys, Fields<R>::ek(c),
@@ -1396,8 +1399,11 @@ static int flow_hs_rk4(opaw_ctx* c, const cx<R>* yin, cx<R>* y, cx<R>* acc, cons
                        double cst) {
   RCHK(ensure(c, c->wtmp, (size_t)nf * c->N * sizeof(cx<R>)));
   const BoxTables bt = box_tables(c);
+  // after a pipelined rebuild its D_ij kernel has formed the weights
+  const bool have_w = c->pot_pending && nf == c->nsys * c->nb;
   RCHK(side_for_stage(c));
-  HCHK(box_weights(bt, c->st2, (const c64*)c->cB.p, nf, 2, nullptr, c->dij, c->nb, c->mtot, (c64*)c->wB.p));
+  if (!have_w)
+    HCHK(box_weights(bt, c->st2, (const c64*)c->cB.p, nf, 2, nullptr, c->dij, c->nb, c->mtot, (c64*)c->wB.p));
   HCHK(union_corr(bt, c->st2, nf, 0, (const c64*)c->wB.p, (c64*)c->corrNL.p));
   RCHK(mark_side(c));
   PlaneRk4<R> rk;
@@ -1439,6 +1445,7 @@ static int flow_rebuild(opaw_ctx* c, int s0, int ns, const cx<R>* phi, bool pipe
   if (!c->pot_ready) return fail(OPAW_ERR_STATE, "potentials not set (opaw_set_potentials)");
   if (!c->vk_ready) return fail(OPAW_ERR_STATE, "Hartree kernel not set (opaw_vk_prep)");
   if (c->pot_pending) return fail(OPAW_ERR_STATE, "a pipelined rebuild was not consumed");
+  if (pipelined && (s0 != 0 || ns != c->nsys)) return fail(OPAW_ERR_ARG, "a pipelined rebuild covers every system");
   const int nf = ns * c->nb;
   const long long N = c->N;
   const double dv = dvol(c);
@@ -1485,10 +1492,13 @@ static int flow_rebuild(opaw_ctx* c, int s0, int ns, const cx<R>* phi, bool pipe
   HCHK(vks_build(sp, c->dens + s0 * N, c->ncoret, c->pair, c->periodic ? c->means : nullptr, c->vloc, N, ns,
                  c->vxc + s0 * N, c->vh + s0 * N, c->vks + s0 * N, c->vksf + s0 * N));
   prof_end(sp, KC_POT);
+  // pipelined: the D_ij kernel also forms the next stage's nonlocal weights
+  // wB = D cB (cB of these orbitals, from the prep above)
+  const DijWeights wt{(const c64*)c->cB.p, (c64*)c->wB.p, c->ctot, c->nsys * c->nb, c->nb, s0 * c->nb};
   prof_begin(sp, KC_POT);
   HCHK(dij(sp, c->datoms, c->natom, c->rhoij + (size_t)s0 * c->mtot, c->mtot, c->qtab, c->d0tab, c->etab, c->nl2,
            c->sph_idx, c->sph_g, c->vks + s0 * N, c->vxc + s0 * N, c->usexcnhat == 0 ? 1 : 0, N, ns, dv,
-           c->dij + (size_t)s0 * c->mtot));
+           c->dij + (size_t)s0 * c->mtot, pipelined ? &wt : nullptr));
   prof_end(sp, KC_POT);
   c->pot_pending = pipelined;
   c->prep_valid = (s0 == 0 && ns == c->nsys && (const void*)phi == c->phi);

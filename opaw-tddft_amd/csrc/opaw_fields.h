@@ -66,10 +66,17 @@ hipError_t vks_build(hipStream_t st, const double* dens, const double* ncoret, c
                      double* vks, float* vksf);
 hipError_t rhoij(hipStream_t st, const AtomDesc* atoms, int natom, const c64* ca, int ctot, int nsys, int nb,
                  int mtot, double* rho);
+// with wt: also the nonlocal weights w(c, f) = sum_j D_ij cB(f, c_j) of every
+// field of the rebuilt systems (what the weights kernel computes from D)
+struct DijWeights {
+  const c64* cb;   // coefficients, field-major cb[f * ctot + c]
+  c64* w;          // weights, coefficient-major w[c * nf + f]; null: none
+  int ctot, nf, nb, f0;   // f0: first field of the rebuild's first system
+};
 hipError_t dij(hipStream_t st, const AtomDesc* atoms, int natom, const double* rho, int mtot, const double* qtab,
                const double* d0tab, const double* etab, int nl2, const int* sph_idx, const double* sph_g,
                const double* vks, const double* vxc, int subtract_vxc, long long N, int nsys, double dv,
-               double* d);
+               double* d, const DijWeights* wt = nullptr);
 template <typename R>
 hipError_t dipole(hipStream_t st, const cx<R>* u, int nx, int ny, int nz, int nsys, int nb, double dx, double dy,
                   double dz, double* part, double* out);

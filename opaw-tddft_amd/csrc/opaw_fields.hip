@@ -450,7 +450,7 @@ __global__ __launch_bounds__(kT) void k_dij(const AtomDesc* __restrict__ atoms, 
                                              int nl2, const int* __restrict__ sph_idx,
                                              const double* __restrict__ sph_g, const double* __restrict__ vks,
                                              const double* __restrict__ vxc, int subtract_vxc, long long N,
-                                             double dv, double* __restrict__ dij) {
+                                             double dv, double* __restrict__ dij, DijWeights wt) {
   const int a = blockIdx.x, s = blockIdx.y;
   const AtomDesc A = atoms[a];
   const int ms = A.ms, m2 = ms * ms;
@@ -529,6 +529,23 @@ __global__ __launch_bounds__(kT) void k_dij(const AtomDesc* __restrict__ atoms, 
     double d = M[lo + ms * hi];
     if (fabs(d) > kDijMax) d = d / fabs(d) * kDijMax;
     dij[(long long)s * mtot + A.moff + q] = d;
+    rho[q] = d;   // the final D of the atom, for the weights below
+  }
+  if (!wt.w) return;
+  __syncthreads();
+  // nonlocal weights of the system's fields: w(coff+i, f) = sum_j D_ij
+  // cB(f, coff+j), in the order of the separate weights kernel (bit-identical)
+  for (int q = threadIdx.x; q < wt.nb * ms; q += kT) {
+    const int o = q % wt.nb, i = q / wt.nb;
+    const int f = wt.f0 + s * wt.nb + o;
+    const c64* row = wt.cb + (long long)f * wt.ctot + A.coff;
+    double xr = 0.0, xi = 0.0;
+    for (int j = 0; j < ms; ++j) {
+      const double d = rho[i + ms * j];
+      xr += d * row[j].x;
+      xi += d * row[j].y;
+    }
+    wt.w[(long long)(A.coff + i) * wt.nf + f] = mk(xr, xi);
   }
 }
 
@@ -1046,10 +1063,10 @@ hipError_t rhoij(hipStream_t st, const AtomDesc* atoms, int natom, const c64* ca
 hipError_t dij(hipStream_t st, const AtomDesc* atoms, int natom, const double* rho, int mtot, const double* qtab,
                const double* d0tab, const double* etab, int nl2, const int* sph_idx, const double* sph_g,
                const double* vks, const double* vxc, int subtract_vxc, long long N, int nsys, double dv,
-               double* d) {
+               double* d, const DijWeights* wt) {
   if (natom == 0) return hipSuccess;
   hipLaunchKernelGGL(k_dij, dim3(natom, nsys), dim3(kT), 0, st, atoms, rho, mtot, qtab, d0tab, etab, nl2, sph_idx,
-                     sph_g, vks, vxc, subtract_vxc, N, dv, d);
+                     sph_g, vks, vxc, subtract_vxc, N, dv, d, wt ? *wt : DijWeights{});
   return hipGetLastError();
 }
 
