@@ -406,7 +406,7 @@ int opaw_create(opaw_ctx** out, int nx, int ny, int nz, double dx, double dy, do
   if (r == OPAW_OK) r = dalloc_t(c, &c->pair, (size_t)((nsys + 1) / 2) * N);
   if (r == OPAW_OK) r = dalloc_t(c, &c->hwork, (size_t)c->gbig.nx * c->gbig.ny * c->gbig.nz);
   if (r == OPAW_OK) r = dalloc_t(c, &c->vk, (size_t)c->gbig.nx * c->gbig.ny * c->gbig.nz);
-  if (r == OPAW_OK) r = dalloc_t(c, &c->gpart, (size_t)kGramChunks * nb * nb);
+  if (r == OPAW_OK) r = dalloc_t(c, &c->gpart, gram_partials(nb));
   if (r == OPAW_OK) r = dalloc_t(c, &c->gmat, (size_t)nb * nb);
   if (r == OPAW_OK) r = dalloc_t(c, &c->tmat, (size_t)nb * nb);
   if (r == OPAW_OK) r = (hipMemset(c->phi, 0, nf * N * c->esize()) == hipSuccess) ? OPAW_OK : fail(OPAW_ERR_HIP, "memset");
@@ -2258,7 +2258,7 @@ static int subspace_eig(int m, const std::vector<std::complex<double>>& S0, cons
 
 // gram of m fields (row-major m x m on the host), with the solver's buffers
 static int gram_m(opaw_ctx* c, const c64* a, const c64* b, int m, std::vector<std::complex<double>>& out) {
-  RCHK(ensure(c, c->gsG, (size_t)kGramChunks * m * m * sizeof(c64)));
+  RCHK(ensure(c, c->gsG, gram_partials(m) * sizeof(c64)));
   RCHK(ensure(c, c->gsM, (size_t)m * m * sizeof(c64)));
   HCHK(gram<double>(c->st, a, b, c->N, m, dvol(c), (c64*)c->gsG.p, (c64*)c->gsM.p));
   out.resize((size_t)m * m);
@@ -2557,7 +2557,7 @@ int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, in
     // both Gram matrices in one download
     const int m = have_p ? 3 * nb : 2 * nb;
     std::vector<std::complex<double>> S((size_t)m * m), H((size_t)m * m), Ccol;
-    RCHK(ensure(c, c->gsG, (size_t)kGramChunks * m * m * sizeof(c64)));
+    RCHK(ensure(c, c->gsG, gram_partials(m) * sizeof(c64)));
     RCHK(ensure(c, c->gsM, (size_t)2 * m * m * sizeof(c64)));
     HCHK(gram<double>(c->st, Y, Y, N, m, dvol(c), (c64*)c->gsG.p, (c64*)c->gsM.p));
     HCHK(gram<double>(c->st, Y, HY, N, m, dvol(c), (c64*)c->gsG.p, (c64*)c->gsM.p + (size_t)m * m));
@@ -2645,7 +2645,7 @@ int opaw_precond_prepare(opaw_ctx* c, int sys) {
   for (auto& sp : c->snaps) {
     if (sp.ready || sp.sys != sys) continue;
     std::vector<std::complex<double>> G;   // G[a][j] = dv <ref_a | phi_j>
-    RCHK(ensure(c, c->gsG, (size_t)kGramChunks * nb * nb * sizeof(c64)));
+    RCHK(ensure(c, c->gsG, gram_partials(nb) * sizeof(c64)));
     RCHK(ensure(c, c->gsM, (size_t)nb * nb * sizeof(c64)));
     HCHK(gram<double>(c->st, ref, (const c64*)sp.phi.p, N, nb, dvol(c), (c64*)c->gsG.p, (c64*)c->gsM.p));
     // rotate_rect wants T[a + nb j] = G[a][j]: the row-major gram is that transposed

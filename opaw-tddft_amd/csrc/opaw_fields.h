@@ -6,7 +6,12 @@
 namespace opaw {
 
 constexpr int kChunks = 256;          // fixed chunk count of deterministic reductions
-constexpr int kGramChunks = 64;
+constexpr int kGramChunks = 64;         // Gram partials of the tiled kernel (more than 64 fields)
+constexpr int kGramBlockChunks = 512;   // Gram partials of the whole-block kernel (up to 64 fields)
+// partial-sum entries the Gram of m fields needs
+inline size_t gram_partials(int m) {
+  return (size_t)(m <= 64 ? kGramBlockChunks : kGramChunks) * (size_t)m * (size_t)m;
+}
 constexpr double kXcDensThreshold = 1e-15;  // per-point LDA density cutoff
 constexpr double kRhoijSelect = 1e-10;      // src/get_dij.f90:82
 constexpr double kDijMax = 500.0;           // src/0_parameter.f90 param_dij_max

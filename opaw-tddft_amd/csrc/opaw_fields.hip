@@ -658,6 +658,81 @@ __global__ __launch_bounds__(256) void k_gram_partial(const cx<R>* __restrict__ 
   if (i < nb && j < nb) part[((long long)c * nb + i) * nb + j] = mk(gr, gi);
 }
 
+// Gram partials with the whole m x m block in one workgroup per point chunk
+// (m <= 16 RB), so every element of a and b is read once:
+// G[c][i][j] = sum_{r in chunk c} conj(a_i(r)) b_j(r).  Thread (tx, ty) of a
+// 16 x 16 grid owns i = tx + 16 ri, j = ty + 16 rj (ri, rj < RB).  Points go
+// through LDS 16 at a time; the next 16 are loaded into registers while the
+// current ones are summed.
+template <typename R, int RB>
+__global__ __launch_bounds__(256) void k_gram_block(const cx<R>* __restrict__ a, const cx<R>* __restrict__ b,
+                                                     long long N, int m, int nchunk, c64* __restrict__ part) {
+  constexpr int P = 16;
+  constexpr int M = 16 * RB;
+  __shared__ c64 sa[M][P + 1], sb[M][P + 1];
+  const int c = blockIdx.x;
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  long long lo, hi;
+  chunk_range(N, nchunk, c, lo, hi);
+  // loader slots: RB elements of a and of b per thread, element (o, p) with
+  // o = slot / P, p = slot % P (16 consecutive points of one orbital per
+  // 16 lanes); orbital and point are clamped, out-of-range values zeroed
+  cx<R> va[RB], vb[RB];
+  auto load = [&](long long r0) {
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+      const int slot = (int)threadIdx.x + 256 * q;
+      const int o = slot / P, p = slot % P;
+      const long long r = min(r0 + p, hi - 1);
+      const int oc = min(o, m - 1);
+      va[q] = a[(long long)oc * N + r];
+      vb[q] = b[(long long)oc * N + r];
+    }
+  };
+  double gr[RB][RB], gi[RB][RB];
+#pragma unroll
+  for (int x = 0; x < RB; ++x)
+#pragma unroll
+    for (int y = 0; y < RB; ++y) gr[x][y] = gi[x][y] = 0.0;
+  if (lo < hi) load(lo);
+  for (long long r0 = lo; r0 < hi; r0 += P) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+      const int slot = (int)threadIdx.x + 256 * q;
+      const int o = slot / P, p = slot % P;
+      const bool ok = o < m && r0 + p < hi;
+      sa[o][p] = ok ? mk((double)va[q].x, (double)va[q].y) : mk(0.0, 0.0);
+      sb[o][p] = ok ? mk((double)vb[q].x, (double)vb[q].y) : mk(0.0, 0.0);
+    }
+    __syncthreads();
+    if (r0 + P < hi) load(r0 + P);
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      c64 x[RB], y[RB];
+#pragma unroll
+      for (int q = 0; q < RB; ++q) {
+        x[q] = sa[tx + 16 * q][k];
+        y[q] = sb[ty + 16 * q][k];
+      }
+#pragma unroll
+      for (int u = 0; u < RB; ++u)
+#pragma unroll
+        for (int v = 0; v < RB; ++v) {
+          gr[u][v] += x[u].x * y[v].x + x[u].y * y[v].y;
+          gi[u][v] += x[u].x * y[v].y - x[u].y * y[v].x;
+        }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < RB; ++u)
+#pragma unroll
+    for (int v = 0; v < RB; ++v) {
+      const int i = tx + 16 * u, j = ty + 16 * v;
+      if (i < m && j < m) part[((long long)c * m + i) * m + j] = mk(gr[u][v], gi[u][v]);
+    }
+}
+
 __global__ void k_gram_finish(const c64* __restrict__ part, int nb, int nchunk, double dv, c64* __restrict__ g) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nb * nb) return;
@@ -1090,9 +1165,22 @@ hipError_t kick(hipStream_t st, const cx<R>* src, cx<R>* dst, int nx, int ny, in
 template <typename R>
 hipError_t gram(hipStream_t st, const cx<R>* a, const cx<R>* b, long long N, int nb, double dv, c64* part, c64* g) {
   const int t = (nb + 15) / 16;
-  hipLaunchKernelGGL((k_gram_partial<R>), dim3(t, t, kGramChunks), dim3(256), 0, st, a, b, N, nb, kGramChunks,
-                     part);
-  hipLaunchKernelGGL(k_gram_finish, dim3((nb * nb + 255) / 256), dim3(256), 0, st, part, nb, kGramChunks, dv, g);
+  if (t > 4) {
+    hipLaunchKernelGGL((k_gram_partial<R>), dim3(t, t, kGramChunks), dim3(256), 0, st, a, b, N, nb, kGramChunks,
+                       part);
+    hipLaunchKernelGGL(k_gram_finish, dim3((nb * nb + 255) / 256), dim3(256), 0, st, part, nb, kGramChunks, dv, g);
+    return hipGetLastError();
+  }
+  const dim3 grid(kGramBlockChunks);
+  if (t == 1)
+    hipLaunchKernelGGL((k_gram_block<R, 1>), grid, dim3(256), 0, st, a, b, N, nb, kGramBlockChunks, part);
+  else if (t == 2)
+    hipLaunchKernelGGL((k_gram_block<R, 2>), grid, dim3(256), 0, st, a, b, N, nb, kGramBlockChunks, part);
+  else if (t == 3)
+    hipLaunchKernelGGL((k_gram_block<R, 3>), grid, dim3(256), 0, st, a, b, N, nb, kGramBlockChunks, part);
+  else
+    hipLaunchKernelGGL((k_gram_block<R, 4>), grid, dim3(256), 0, st, a, b, N, nb, kGramBlockChunks, part);
+  hipLaunchKernelGGL(k_gram_finish, dim3((nb * nb + 255) / 256), dim3(256), 0, st, part, nb, kGramBlockChunks, dv, g);
   return hipGetLastError();
 }
 
