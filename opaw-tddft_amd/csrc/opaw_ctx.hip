@@ -2223,18 +2223,22 @@ static int subspace_eig(int m, const std::vector<std::complex<double>>& S0, cons
   std::vector<std::complex<double>> X((size_t)m * r);   // row-major m x r
   for (int i = 0; i < m; ++i)
     for (int k = 0; k < r; ++k) X[(size_t)i * r + k] = Vs[(size_t)i * m + keep[k]] / std::sqrt(ws[keep[k]]);
+  // products in real arithmetic (la::mul / la::cmul: no library call per
+  // complex product)
   std::vector<std::complex<double>> HX((size_t)m * r), Hr((size_t)r * r);
   for (int i = 0; i < m; ++i)
-    for (int k = 0; k < r; ++k) {
-      std::complex<double> s = 0.0;
-      for (int j = 0; j < m; ++j) s += H[(size_t)i * m + j] * X[(size_t)j * r + k];
-      HX[(size_t)i * r + k] = s;
+    for (int j = 0; j < m; ++j) {
+      const std::complex<double> h = H[(size_t)i * m + j];
+      const std::complex<double>* xj = &X[(size_t)j * r];
+      std::complex<double>* out = &HX[(size_t)i * r];
+      for (int k = 0; k < r; ++k) out[k] += la::mul(h, xj[k]);
     }
-  for (int a = 0; a < r; ++a)
-    for (int b = 0; b < r; ++b) {
-      std::complex<double> s = 0.0;
-      for (int i = 0; i < m; ++i) s += std::conj(X[(size_t)i * r + a]) * HX[(size_t)i * r + b];
-      Hr[(size_t)a * r + b] = s;
+  for (int i = 0; i < m; ++i)
+    for (int a = 0; a < r; ++a) {
+      const std::complex<double> xa = X[(size_t)i * r + a];
+      const std::complex<double>* hx = &HX[(size_t)i * r];
+      std::complex<double>* out = &Hr[(size_t)a * r];
+      for (int b = 0; b < r; ++b) out[b] += la::cmul(xa, hx[b]);
     }
   for (int a = 0; a < r; ++a)
     for (int b = a + 1; b < r; ++b) {
@@ -2250,7 +2254,7 @@ static int subspace_eig(int m, const std::vector<std::complex<double>>& S0, cons
   for (int j = 0; j < nout; ++j)
     for (int i = 0; i < m; ++i) {
       std::complex<double> s = 0.0;
-      for (int k = 0; k < r; ++k) s += X[(size_t)i * r + k] * U[(size_t)k * r + j];
+      for (int k = 0; k < r; ++k) s += la::mul(X[(size_t)i * r + k], U[(size_t)k * r + j]);
       Ccol[i + (size_t)m * j] = s * dsc[i];
     }
   return OPAW_OK;

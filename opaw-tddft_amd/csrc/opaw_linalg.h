@@ -2,6 +2,11 @@
 // Hermitian matrix by Householder reduction to a real tridiagonal matrix and
 // the implicit QL iteration with Wilkinson shifts (the textbook route of
 // Golub and Van Loan, sections 8.3-8.4), O(n^3).
+//
+// Complex products are written out in real arithmetic (a std::complex
+// product compiles to a library call that handles infinities), the
+// reflections touch only the trailing block they change, and the QL
+// rotations run on rows of the transposed eigenvector matrix.
 #pragma once
 #include <algorithm>
 #include <cmath>
@@ -11,53 +16,99 @@
 
 namespace opaw {
 
+namespace la {
+using C = std::complex<double>;
+// a b
+inline C mul(C a, C b) { return C(a.real() * b.real() - a.imag() * b.imag(), a.real() * b.imag() + a.imag() * b.real()); }
+// conj(a) b
+inline C cmul(C a, C b) { return C(a.real() * b.real() + a.imag() * b.imag(), a.real() * b.imag() - a.imag() * b.real()); }
+// a conj(b)
+inline C mulc(C a, C b) { return C(a.real() * b.real() + a.imag() * b.imag(), a.imag() * b.real() - a.real() * b.imag()); }
+}  // namespace la
+
 // A (n x n, row-major, Hermitian) -> ascending eigenvalues w and
 // eigenvectors as the columns of V (row-major: V[i * n + k] is component i of
 // eigenvector k)
 inline void herm_eig_tridiag(int n, std::vector<std::complex<double>> A, std::vector<double>& w,
                              std::vector<std::complex<double>>& V) {
   using C = std::complex<double>;
+  using la::cmul;
+  using la::mul;
+  using la::mulc;
   w.assign(n, 0.0);
   V.assign((size_t)n * n, C(0.0));
   if (n == 0) return;
-  // Q accumulates the Householder reflections (row-major), A <- Q^H A Q
-  std::vector<C> Q((size_t)n * n, C(0.0));
-  for (int i = 0; i < n; ++i) Q[(size_t)i * n + i] = 1.0;
-  std::vector<C> v(n), p(n), q(n);
+  // A <- Q^H A Q with Q = H_0 H_1 ... H_{n-3}; the reflection vectors are
+  // kept (row k of Vh, zero when step k is the identity) and Q is formed
+  // backwards afterwards, where H_k only meets the block Q[k+1:, k+1:]
+  std::vector<C> Vh((size_t)n * n, C(0.0));
+  std::vector<char> used(n, 0);
+  std::vector<C> p(n), q(n);
   for (int k = 0; k + 2 < n; ++k) {
+    C* v = &Vh[(size_t)k * n];
+    const int k1 = k + 1;
     double xn = 0.0;
-    for (int i = k + 1; i < n; ++i) xn += std::norm(A[(size_t)i * n + k]);
+    for (int i = k1; i < n; ++i) xn += std::norm(A[(size_t)i * n + k]);
     xn = std::sqrt(xn);
     if (xn == 0.0) continue;
-    const C x0 = A[(size_t)(k + 1) * n + k];
+    const C x0 = A[(size_t)k1 * n + k];
     const C ph = std::abs(x0) > 0.0 ? x0 / std::abs(x0) : C(1.0);
     const C alpha = -ph * xn;
-    std::fill(v.begin(), v.end(), C(0.0));
-    for (int i = k + 1; i < n; ++i) v[i] = A[(size_t)i * n + k];
-    v[k + 1] -= alpha;
+    for (int i = k1; i < n; ++i) v[i] = A[(size_t)i * n + k];
+    v[k1] -= alpha;
     double vn = 0.0;
-    for (int i = k + 1; i < n; ++i) vn += std::norm(v[i]);
+    for (int i = k1; i < n; ++i) vn += std::norm(v[i]);
     if (vn == 0.0) continue;
-    vn = std::sqrt(vn);
-    for (int i = k + 1; i < n; ++i) v[i] /= vn;
-    // H = I - 2 v v^H;  A <- H A H:  p = A v, K = v^H p, q = p - K v,
-    // A <- A - 2 v q^H - 2 q v^H
-    for (int i = 0; i < n; ++i) {
-      C s = 0.0;
-      for (int j = k + 1; j < n; ++j) s += A[(size_t)i * n + j] * v[j];
-      p[i] = s;
+    vn = 1.0 / std::sqrt(vn);
+    for (int i = k1; i < n; ++i) v[i] *= vn;
+    // H = I - 2 v v^H with v on [k1, n): column k becomes alpha e_k1, and the
+    // trailing block B = A[k1:, k1:] becomes H B H:  p = B v, K = v^H p,
+    // q = p - K v, B <- B - 2 v q^H - 2 q v^H
+    for (int i = k1; i < n; ++i) {
+      const C* row = &A[(size_t)i * n];
+      double sr = 0.0, si = 0.0;
+      for (int j = k1; j < n; ++j) {
+        const C a = row[j], b = v[j];
+        sr += a.real() * b.real() - a.imag() * b.imag();
+        si += a.real() * b.imag() + a.imag() * b.real();
+      }
+      p[i] = C(sr, si);
     }
     C K = 0.0;
-    for (int i = k + 1; i < n; ++i) K += std::conj(v[i]) * p[i];
-    for (int i = 0; i < n; ++i) q[i] = p[i] - K * v[i];
-    for (int i = 0; i < n; ++i)
-      for (int j = 0; j < n; ++j)
-        A[(size_t)i * n + j] -= 2.0 * (v[i] * std::conj(q[j]) + q[i] * std::conj(v[j]));
-    // Q <- Q H
-    for (int i = 0; i < n; ++i) {
-      C s = 0.0;
-      for (int j = k + 1; j < n; ++j) s += Q[(size_t)i * n + j] * v[j];
-      for (int j = k + 1; j < n; ++j) Q[(size_t)i * n + j] -= 2.0 * s * std::conj(v[j]);
+    for (int i = k1; i < n; ++i) K += cmul(v[i], p[i]);
+    for (int i = k1; i < n; ++i) q[i] = p[i] - mul(K, v[i]);
+    for (int i = k1; i < n; ++i) {
+      C* row = &A[(size_t)i * n];
+      const C vi2 = 2.0 * v[i], qi2 = 2.0 * q[i];
+      for (int j = k1; j < n; ++j) row[j] -= mulc(vi2, q[j]) + mulc(qi2, v[j]);
+    }
+    A[(size_t)k1 * n + k] = alpha;
+    A[(size_t)k * n + k1] = std::conj(alpha);
+    for (int i = k1 + 1; i < n; ++i) {
+      A[(size_t)i * n + k] = 0.0;
+      A[(size_t)k * n + i] = 0.0;
+    }
+    used[k] = 1;
+  }
+  // Q = H_0 (H_1 (... (H_{n-3} I))): H_k Q = Q - 2 v (v^H Q) on rows and
+  // columns k+1.. (the rest of Q is still the identity there)
+  std::vector<C> Q((size_t)n * n, C(0.0));
+  for (int i = 0; i < n; ++i) Q[(size_t)i * n + i] = 1.0;
+  std::vector<C> t(n);
+  for (int k = n - 3; k >= 0; --k) {
+    if (!used[k]) continue;
+    const int k1 = k + 1;
+    const C* v = &Vh[(size_t)k * n];
+    for (int j = k1; j < n; ++j) t[j] = 0.0;
+    for (int i = k1; i < n; ++i) {
+      const C vi = std::conj(v[i]);
+      const C* row = &Q[(size_t)i * n];
+      for (int j = k1; j < n; ++j) t[j] += mul(vi, row[j]);
+    }
+    for (int i = k1; i < n; ++i) {
+      const C vi2 = 2.0 * v[i];
+      C* row = &Q[(size_t)i * n];
+      for (int j = k1; j < n; ++j) row[j] -= mul(vi2, t[j]);
     }
   }
   // real tridiagonal: d (diagonal), e (sub-diagonal magnitudes) after the
@@ -69,11 +120,13 @@ inline void herm_eig_tridiag(int n, std::vector<std::complex<double>> A, std::ve
     const C sub = A[(size_t)(i + 1) * n + i];
     const double m = std::abs(sub);
     e[i] = m;
-    D[i + 1] = D[i] * (m > 0.0 ? sub / m : C(1.0));
+    D[i + 1] = mul(D[i], m > 0.0 ? sub / m : C(1.0));
   }
-  // implicit QL on (d, e) with the eigenvectors of the tridiagonal in Z
-  std::vector<double> Z((size_t)n * n, 0.0);
-  for (int i = 0; i < n; ++i) Z[(size_t)i * n + i] = 1.0;
+  // implicit QL on (d, e); Zt holds the eigenvectors of the tridiagonal as
+  // rows (Zt[i * n + r] is component r of vector i), so a rotation of
+  // vectors i and i+1 runs over two contiguous rows
+  std::vector<double> Zt((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) Zt[(size_t)i * n + i] = 1.0;
   for (int l = 0; l < n; ++l) {
     int iter = 0;
     for (;;) {
@@ -105,10 +158,12 @@ inline void herm_eig_tridiag(int n, std::vector<std::complex<double>> A, std::ve
         pp = s * r;
         d[i + 1] = g + pp;
         g = c * r - b;
+        double* zi = &Zt[(size_t)i * n];
+        double* zi1 = &Zt[(size_t)(i + 1) * n];
         for (int k = 0; k < n; ++k) {
-          f = Z[(size_t)k * n + i + 1];
-          Z[(size_t)k * n + i + 1] = s * Z[(size_t)k * n + i] + c * f;
-          Z[(size_t)k * n + i] = c * Z[(size_t)k * n + i] - s * f;
+          const double fz = zi1[k];
+          zi1[k] = s * zi[k] + c * fz;
+          zi[k] = c * zi[k] - s * fz;
         }
       }
       if (r == 0.0 && i >= l) continue;
@@ -121,13 +176,19 @@ inline void herm_eig_tridiag(int n, std::vector<std::complex<double>> A, std::ve
   std::iota(ord.begin(), ord.end(), 0);
   std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return d[a] < d[b]; });
   // eigenvectors: (Q D) Z
-  for (int k = 0; k < n; ++k) {
-    const int o = ord[k];
-    w[k] = d[o];
-    for (int i = 0; i < n; ++i) {
-      C s = 0.0;
-      for (int j = 0; j < n; ++j) s += Q[(size_t)i * n + j] * D[j] * Z[(size_t)j * n + o];
-      V[(size_t)i * n + k] = s;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) Q[(size_t)i * n + j] = mul(Q[(size_t)i * n + j], D[j]);
+  for (int k = 0; k < n; ++k) w[k] = d[ord[k]];
+  for (int i = 0; i < n; ++i) {
+    const C* qi = &Q[(size_t)i * n];
+    for (int k = 0; k < n; ++k) {
+      const double* z = &Zt[(size_t)ord[k] * n];
+      double sr = 0.0, si = 0.0;
+      for (int j = 0; j < n; ++j) {
+        sr += qi[j].real() * z[j];
+        si += qi[j].imag() * z[j];
+      }
+      V[(size_t)i * n + k] = C(sr, si);
     }
   }
 }
