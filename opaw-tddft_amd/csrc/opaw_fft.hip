@@ -169,6 +169,9 @@ __global__ __launch_bounds__(kThreads) void k_fft_x(const cx<R>* in, View vin,
 // smaller workgroups measured faster than 32- or 64-wide tiles on the 64^3
 // z pass)
 constexpr int kSXT = 16;
+// columns per workgroup of a compiled length: 32 for lines of at most 48
+// points, so a workgroup still moves a useful amount of data
+constexpr int sxt_of(int n) { return n <= 48 ? 2 * kSXT : kSXT; }
 
 template <typename R, int N, int AXIS, int S, int EPI>
 __global__ __launch_bounds__(kThreads) void k_fft_s(const cx<R>* in, View vin,
@@ -179,12 +182,13 @@ __global__ __launch_bounds__(kThreads) void k_fft_s(const cx<R>* in, View vin,
   // N == 0: runtime length n_rt over xt_rt columns (dynamic LDS: two tiles
   // and the twiddles)
   const int n = N > 0 ? N : n_rt;
-  const int XT = N > 0 ? kSXT : xt_rt;
+  constexpr int CXT = sxt_of(N > 0 ? N : 256);
+  const int XT = N > 0 ? CXT : xt_rt;
   cx<R>* tile;
   cx<R>* tile2 = nullptr;
   cx<R>* tw;
   if constexpr (N > 0) {
-    __shared__ cx<R> s_tile[kSXT * (N > 0 ? N : 1)];
+    __shared__ cx<R> s_tile[CXT * (N > 0 ? N : 1)];
     __shared__ cx<R> s_tw[N > 0 ? N : 1];
     tile = s_tile;
     tw = s_tw;
@@ -215,7 +219,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_s(const cx<R>* in, View vin,
   __syncthreads();
   cx<R>* res = tile;
   if constexpr (N > 0)
-    fft::lines<R, (N > 0 ? N : 1), kSXT, kThreads, true, 1, kSXT, S>(tile, tw);
+    fft::lines<R, (N > 0 ? N : 1), CXT, kThreads, true, 1, CXT, S>(tile, tw);
   else
     res = fft::lines_gen<R, S>(tile, tile2, tw, n, XT, true, 1, XT, plan);
   if constexpr (EPI == EPI_MULT) {
@@ -230,7 +234,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_s(const cx<R>* in, View vin,
     }
     __syncthreads();
     if constexpr (N > 0)
-      fft::lines<R, (N > 0 ? N : 1), kSXT, kThreads, true, 1, kSXT, -S>(tile, tw);
+      fft::lines<R, (N > 0 ? N : 1), CXT, kThreads, true, 1, CXT, -S>(tile, tw);
     else
       res = fft::lines_gen<R, -S>(res, res == tile ? tile2 : tile, tw, n, XT, true, 1, XT, plan);
   }
@@ -560,7 +564,7 @@ static hipError_t launch_s_n(int n, dim3 grid, hipStream_t st, const cx<R>* in, 
 #define OPAW_CASE(nn)                                                                        \
   case nn:                                                                                   \
     hipLaunchKernelGGL((k_fft_s<R, nn, AXIS, S, EPI>), grid, dim3(kThreads), 0, st, in, vin, \
-                       out, vout, L, tw, tab, vtab, nn, kSXT, none);                         \
+                       out, vout, L, tw, tab, vtab, nn, sxt_of(nn), none);                   \
     return hipGetLastError();
     OPAW_FFT_SIZES(OPAW_CASE)
 #undef OPAW_CASE
@@ -601,7 +605,7 @@ template <typename R>
 hipError_t fft_s_pass(int cls, int n, int axis, int dir, int epi, hipStream_t st, const cx<R>* in,
                       const View& vin, cx<R>* out, const View& vout, const SLines& L,
                       const cx<R>* tw, const R* tab, const View& vtab) {
-  const int xt = fft_size_compiled(n) ? kSXT : s_xt_rt(n, (int)sizeof(cx<R>));
+  const int xt = fft_size_compiled(n) ? sxt_of(n) : s_xt_rt(n, (int)sizeof(cx<R>));
   const long long ntile = (long long)((L.nx + xt - 1) / xt) * L.lo * L.nf;
   if (ntile <= 0) return hipSuccess;
   dim3 grid((unsigned)ntile);
