@@ -20,7 +20,8 @@ numpy oracle timed on a bounded sample (one full step) on this host.
 Beside the headline, `batched_polyacenes` reports the batched uneven
 workload: the polyacenes n = 1..6 on their own grids, 4 systems each, mixed
 path, all advanced together by opaw_tddft_run_batch, against the same
-molecules run one after another.
+molecules run one after another; `coronene` the C24H12 96^3 workload with
+kicks along x, y, z in both precisions.
 
 Prints ONE JSON line.
 """
@@ -51,6 +52,9 @@ def parse():
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched polyacene series (reported beside the headline line)")
     ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--no-coronene", action="store_true",
+                    help="skip the coronene 96^3 four-system workload (reported beside the headline line)")
+    ap.add_argument("--coronene-steps", type=int, default=10)
     return ap.parse_args()
 
 
@@ -227,6 +231,37 @@ def run_batch(steps):
             "propagations_per_batch_step": 6 * 4}
 
 
+def run_coronene(steps):
+    """Coronene C24H12 (D6h, C-C 2.68, C-H 2.05 bohr), 96^3 @0.45 bohr MT, 54
+    orbitals, 4 systems (unperturbed + kicks along x, y, z), FP64 and mixed
+    from the same start: steps/s of each and the mixed kick responses
+    against FP64 (relative L2 over every step run)."""
+    import numpy as np
+    import torch
+    from opaw_amd import synth, batch
+    spec = synth.molecule_system(geometry="coronene", n=(96, 96, 96), d=0.45, seed=77)
+    phi0 = synth.lowdin_orthonormalise(synth.random_orbitals(spec, spec.nocc, seed=17, envelope=5.0), spec.dv)
+    out, resp = {}, {}
+    for prec in ("f64", "mixed"):
+        g = batch.kicked_context(spec, phi0, prec, 4)
+        _, d0 = g.opaw_tddft(2, spec.dt, spec.sm)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, d1 = g.opaw_tddft(steps, spec.dt, spec.sm)
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t0) / steps
+        sc, sr = (16, 8) if prec == "f64" else (8, 4)
+        out[prec] = {"steps_per_s": 1.0 / sec, "ms_per_step": sec * 1e3,
+                     "hbm_model_gbs": step_bytes_model(spec, 4, spec.nocc, sc, sr) / sec / 1e9}
+        resp[prec] = batch.kick_responses(np.concatenate([d0, d1]), spec.sm)
+        g.close()
+    err = float(np.linalg.norm(resp["mixed"] - resp["f64"]) / np.linalg.norm(resp["f64"]))
+    return {"workload": "coronene C24H12, 96^3 MT @0.45 bohr, 54 orbitals x 4 systems (kicks x, y, z)",
+            "steps": steps, "fp64": out["f64"], "mixed": out["mixed"],
+            "mixed_over_fp64": out["mixed"]["steps_per_s"] / out["f64"]["steps_per_s"],
+            "kick_response_rel_l2_mixed_vs_fp64": err}
+
+
 def cpu_baseline(spec, phi0):
     """numpy oracle, one full time step (both systems), single thread."""
     try:
@@ -282,6 +317,13 @@ def main():
               f"({batched['speedup_over_sequential']:.2f}x the molecules one after another)",
               file=sys.stderr, flush=True)
 
+    coronene = None
+    if not a.no_coronene:
+        coronene = run_coronene(a.coronene_steps)
+        print(f"[bench] coronene 96^3 x 4 systems: FP64 {coronene['fp64']['steps_per_s']:.1f}, mixed "
+              f"{coronene['mixed']['steps_per_s']:.1f} steps/s, kick responses within "
+              f"{coronene['kick_response_rel_l2_mixed_vs_fp64']:.1e}", file=sys.stderr, flush=True)
+
     dip_err = None
     if "f64" in res and "mixed" in res:
         d64, d32 = res["f64"]["d_dip"], res["mixed"]["d_dip"]
@@ -331,6 +373,7 @@ def main():
         "dipole_rel_l2_mixed_vs_fp64": dip_err,
         "breakdown_ms_per_step": r["breakdown_ms_per_step"],
         "batched_polyacenes": batched,
+        "coronene": coronene,
     }
     print(json.dumps(out))
 
