@@ -137,6 +137,7 @@ struct opaw_ctx {
   // stage (pot_pending), whose forward passes wait only for corrA (ev_corr);
   // ev_nhat hands the main stream's rho_ij / nhat to that chain
   hipEvent_t ev_corr = nullptr, ev_nhat = nullptr;
+  hipEvent_t ev_mid = nullptr;   // the forward kinetic passes (st2) have ended
   bool pot_pending = false;
   double* mpart = nullptr;   // partials of the periodic potential means (side stream)
   // own non-blocking stream of the multi-context time loop (created on first
@@ -418,6 +419,8 @@ int opaw_create(opaw_ctx** out, int nx, int ny, int nz, double dx, double dy, do
                        hipEventCreateWithFlags(&c->ev_corr, hipEventDisableTiming | hipEventReleaseToDevice) !=
                            hipSuccess ||
                        hipEventCreateWithFlags(&c->ev_nhat, hipEventDisableTiming | hipEventReleaseToDevice) !=
+                           hipSuccess ||
+                       hipEventCreateWithFlags(&c->ev_mid, hipEventDisableTiming | hipEventReleaseToDevice) !=
                            hipSuccess))
     r = fail(OPAW_ERR_HIP, "side stream / events");
   if (r != OPAW_OK) {
@@ -436,6 +439,7 @@ int opaw_destroy(opaw_ctx* c) {
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->ev_corr) (void)hipEventDestroy(c->ev_corr);
   if (c->ev_nhat) (void)hipEventDestroy(c->ev_nhat);
+  if (c->ev_mid) (void)hipEventDestroy(c->ev_mid);
   if (c->st2) (void)hipStreamDestroy(c->st2);
   if (c->bst) (void)hipStreamDestroy(c->bst);
   for (void* p : c->allocs) (void)hipFree(p);
@@ -1297,16 +1301,6 @@ static int settle_side(opaw_ctx* c, int r) {
   return r;
 }
 
-// side stream of an shs stage: after a pipelined rebuild st2 already runs
-// that rebuild's potential chain (the stage's weights must follow it there)
-// and st needs only corrA before its forward passes; otherwise fork
-static int side_for_stage(opaw_ctx* c) {
-  if (!c->pot_pending) return fork_side(c);
-  c->pot_pending = false;
-  HCHK(hipStreamWaitEvent(c->st, c->ev_corr, 0));
-  return OPAW_OK;
-}
-
 // Coefficients of `in` for shs and the rebuild: A = P1^T in (weights
 // wA = sn(-1/2) A, compact correction corrA = S^-1/2 in - in) and
 // B = P^T S^-1/2 in = P^T in + O wA.  With side != 0 the corrA pass runs on
@@ -1343,6 +1337,44 @@ static int flow_sn_phi(opaw_ctx* c, const cx<R>* in, cx<R>* out, int nf, double 
   return OPAW_OK;
 }
 
+// The kinetic pass of an shs stage, out = T u + V_KS u + corrNL with
+// u = in + corrA, and corrNL the union add-back of the weights D(sys) cB.
+//  - After a pipelined rebuild (pot_pending), st holds the rebuild's
+//    potential chain, whose D_ij kernel has formed the weights: corrNL
+//    follows it on st; the forward plane and z passes run on st2 as soon as
+//    corrA exists (ev_corr), and the inverse pass runs on st after them
+//    (ev_mid).  The critical chain stays on st, whose only wait (for the
+//    forward passes) has normally been met by the time st reaches it.
+//  - Otherwise the weights and corrNL run on st2 while st runs the forward
+//    passes, and the inverse pass waits for them (ev_join).
+template <typename R>
+static int kinetic_stage(opaw_ctx* c, int sys0, const cx<R>* in, cx<R>* out, int nf, int nf_per_sys,
+                         const PlaneRk4<R>* rk) {
+  RCHK(ensure(c, c->wtmp, (size_t)nf * c->N * sizeof(cx<R>)));
+  const BoxTables bt = box_tables(c);
+  const R* vks = Fields<R>::vks(c) + (size_t)sys0 * c->N;
+  const bool pending = c->pot_pending && sys0 == 0 && nf == c->nsys * c->nb && nf_per_sys == c->nb;
+  c->pot_pending = false;
+  if (pending) {
+    HCHK(union_corr(bt, c->st, nf, 0, (const c64*)c->wB.p, (c64*)c->corrNL.p));
+    HCHK(hipStreamWaitEvent(c->st2, c->ev_corr, 0));
+    HCHK(apply_t_plus_v<R>(c->g, c->st2, in, out, (cx<R>*)c->wtmp.p, nf, nf_per_sys, Fields<R>::ek(c), vks,
+                           TW<R>::get(c, c->nx), TW<R>::get(c, c->ny), TW<R>::get(c, c->nz), c->umap, c->upt,
+                           c->uzr, (const c64*)c->corrA.p, (const c64*)c->corrNL.p, nf, nullptr, rk, c->st,
+                           c->ev_mid));
+    return OPAW_OK;
+  }
+  RCHK(fork_side(c));
+  HCHK(box_weights(bt, c->st2, (const c64*)c->cB.p, nf, 2, nullptr, c->dij + (size_t)sys0 * c->mtot, nf_per_sys,
+                   c->mtot, (c64*)c->wB.p));
+  HCHK(union_corr(bt, c->st2, nf, 0, (const c64*)c->wB.p, (c64*)c->corrNL.p));
+  RCHK(mark_side(c));
+  HCHK(apply_t_plus_v<R>(c->g, c->st, in, out, (cx<R>*)c->wtmp.p, nf, nf_per_sys, Fields<R>::ek(c), vks,
+                         TW<R>::get(c, c->nx), TW<R>::get(c, c->ny), TW<R>::get(c, c->nz), c->umap, c->upt, c->uzr,
+                         (const c64*)c->corrA.p, (const c64*)c->corrNL.p, nf, c->ev_join, rk));
+  return OPAW_OK;
+}
+
 // out = H(sys) in for nf fields; field f uses system sys0 + f / nf_per_sys
 template <typename R>
 static int flow_h_phi(opaw_ctx* c, int sys0, const cx<R>* in, cx<R>* out, int nf, int nf_per_sys) {
@@ -1365,23 +1397,8 @@ static int flow_h_phi(opaw_ctx* c, int sys0, const cx<R>* in, cx<R>* out, int nf
 // sn(-1/2)) and their add-back into h in place
 template <typename R>
 static int flow_hs(opaw_ctx* c, int sys0, const cx<R>* in, cx<R>* h, int nf, int nf_per_sys) {
-  RCHK(ensure(c, c->wtmp, (size_t)nf * c->N * sizeof(cx<R>)));
+  RCHK(kinetic_stage<R>(c, sys0, in, h, nf, nf_per_sys, nullptr));
   const BoxTables bt = box_tables(c);
-  // the nonlocal weights D(sys) cB and their union correction run on the
-  // side stream while the forward plane and z passes (which need only
-  // corrA) run on st; the inverse plane pass waits for them
-  // after a pipelined rebuild its D_ij kernel has formed the weights
-  const bool have_w = c->pot_pending && sys0 == 0 && nf == c->nsys * c->nb && nf_per_sys == c->nb;
-  RCHK(side_for_stage(c));
-  if (!have_w)
-    HCHK(box_weights(bt, c->st2, (const c64*)c->cB.p, nf, 2, nullptr, c->dij + (size_t)sys0 * c->mtot, nf_per_sys,
-                     c->mtot, (c64*)c->wB.p));
-  HCHK(union_corr(bt, c->st2, nf, 0, (const c64*)c->wB.p, (c64*)c->corrNL.p));
-  RCHK(mark_side(c));
-  HCHK(apply_t_plus_v<R>(c->g, c->st, in, h, (cx<R>*)c->wtmp.p, nf, nf_per_sys, Fields<R>::ek(c),
-                         Fields<R>::vks(c) + (size_t)sys0 * c->N, TW<R>::get(c, c->nx), TW<R>::get(c, c->ny),
-                         TW<R>::get(c, c->nz), c->umap, c->upt, c->uzr, (const c64*)c->corrA.p,
-                         (const c64*)c->corrNL.p, nf, c->ev_join));
   HCHK(box_coeffs<R>(c->g, bt, c->st, h, nf, 1, nullptr, dvol(c), (double*)c->ppart.p, (c64*)c->cC.p, 1, c->sn[1],
                      nullptr, nf, c->mtot, (c64*)c->wC.p));
   // the outer S^-1/2 of shs: h += corrC on the union of boxes
@@ -1397,24 +1414,14 @@ static int flow_hs(opaw_ctx* c, int sys0, const cx<R>* in, cx<R>* h, int nf, int
 template <typename R>
 static int flow_hs_rk4(opaw_ctx* c, const cx<R>* yin, cx<R>* y, cx<R>* acc, const cx<R>* p, int nf, bool first,
                        double cst) {
-  RCHK(ensure(c, c->wtmp, (size_t)nf * c->N * sizeof(cx<R>)));
-  const BoxTables bt = box_tables(c);
-  // after a pipelined rebuild its D_ij kernel has formed the weights
-  const bool have_w = c->pot_pending && nf == c->nsys * c->nb;
-  RCHK(side_for_stage(c));
-  if (!have_w)
-    HCHK(box_weights(bt, c->st2, (const c64*)c->cB.p, nf, 2, nullptr, c->dij, c->nb, c->mtot, (c64*)c->wB.p));
-  HCHK(union_corr(bt, c->st2, nf, 0, (const c64*)c->wB.p, (c64*)c->corrNL.p));
-  RCHK(mark_side(c));
   PlaneRk4<R> rk;
   rk.p = p;
   rk.acc = acc;
   rk.hU = (c64*)c->hU.p;
   rk.c = cst;
   rk.first = first ? 1 : 0;
-  HCHK(apply_t_plus_v<R>(c->g, c->st, yin, y, (cx<R>*)c->wtmp.p, nf, c->nb, Fields<R>::ek(c), Fields<R>::vks(c),
-                         TW<R>::get(c, c->nx), TW<R>::get(c, c->ny), TW<R>::get(c, c->nz), c->umap, c->upt, c->uzr,
-                         (const c64*)c->corrA.p, (const c64*)c->corrNL.p, nf, c->ev_join, &rk));
+  RCHK(kinetic_stage<R>(c, 0, yin, y, nf, c->nb, &rk));
+  const BoxTables bt = box_tables(c);
   HCHK(box_coeffs_compact(c->g, bt, c->st, (const c64*)c->hU.p, nf, dvol(c), (double*)c->ppart.p, (c64*)c->cC.p,
                           c->sn[1], (c64*)c->wC.p));
   HCHK(union_rk4_apply<R>(c->g, bt, c->st, nf, (const c64*)c->wC.p, (c64*)c->corrC.p, y, acc, cst,
@@ -1436,10 +1443,12 @@ static int flow_shs(opaw_ctx* c, int sys0, const cx<R>* in, cx<R>* out, int nf, 
 // gathered), rho_ij, density, nhat + rnel renormalisation, V_H + V_xc -> V_KS,
 // D_ij (src/ncpaw_make_hamiltonian.f90:213-246).
 //
-// pipelined (the time loop, whose next call is an shs stage): the potential
-// chain (renormalisation, Hartree, V_KS, D_ij) is left on the side stream and
-// the stage consumes it (side_for_stage): its forward kinetic passes need
-// only corrA, so they overlap the chain, and only its inverse pass waits.
+// pipelined (the time loop, whose next call is an shs stage): st runs the
+// critical chain (coefficients, corrA, density, renormalisation, Hartree,
+// V_KS, D_ij with the next stage's weights) and st2 the B fix-up, rho_ij and
+// nhat (handed back by ev_nhat); the stage consumes the state
+// (kinetic_stage), running its forward kinetic passes on st2 (they need only
+// corrA, marked by ev_corr) while st finishes the chain.
 template <typename R>
 static int flow_rebuild(opaw_ctx* c, int s0, int ns, const cx<R>* phi, bool pipelined = false) {
   if (!c->pot_ready) return fail(OPAW_ERR_STATE, "potentials not set (opaw_set_potentials)");
@@ -1449,29 +1458,42 @@ static int flow_rebuild(opaw_ctx* c, int s0, int ns, const cx<R>* phi, bool pipe
   const int nf = ns * c->nb;
   const long long N = c->N;
   const double dv = dvol(c);
-  // the coefficients with corrA and then the density pass (which needs
-  // corrA) on the side stream, overlapping B, rho_ij and nhat on st
-  RCHK(flow_prep<R>(c, phi, nf, true));
-  if (pipelined) HCHK(hipEventRecord(c->ev_corr, c->st2));
+  hipStream_t sd, sr;   // streams of the density pass and of rho_ij + nhat
+  if (pipelined) {
+    RCHK(ensure_coef(c, nf));
+    const BoxTables bt = box_tables(c);
+    HCHK(box_coeffs_dual<R>(c->g, bt, c->st, phi, nf, dv, (double*)c->ppart.p, (c64*)c->cA.p, c->sn[1],
+                            (c64*)c->wA.p, (c64*)c->cB.p));
+    RCHK(fork_side(c));
+    HCHK(box_bfix(bt, c->st2, nf, (const c64*)c->wA.p, (c64*)c->cB.p));
+    HCHK(union_corr(bt, c->st, nf, 1, (const c64*)c->wA.p, (c64*)c->corrA.p));
+    HCHK(hipEventRecord(c->ev_corr, c->st));
+    sd = c->st;
+    sr = c->st2;
+  } else {
+    // the coefficients with corrA and then the density pass (which needs
+    // corrA) on the side stream, overlapping B, rho_ij and nhat on st
+    RCHK(flow_prep<R>(c, phi, nf, true));
+    sd = c->st2;
+    sr = c->st;
+  }
   const int dstride = (int)((N + 255) / 256);
   double* dpart = c->rpart;
   double* npart = c->rpart + (size_t)c->nsys * dstride;
   int nd = 0, nn = 0;
-  prof_begin(c->st2, KC_DENS);
-  HCHK(density_sum<R>(c->st2, phi, N, ns, c->nb, c->dens + s0 * N, c->umap, (const c64*)c->corrA.p, nf, dpart,
-                      &nd));
-  prof_end(c->st2, KC_DENS);
-  prof_begin(c->st, KC_DENS);
-  HCHK(rhoij_nhatw(c->st, c->datoms, c->natom, (const c64*)c->cB.p, c->ctot, ns, c->nb, c->mtot,
+  prof_begin(sd, KC_DENS);
+  HCHK(density_sum<R>(sd, phi, N, ns, c->nb, c->dens + s0 * N, c->umap, (const c64*)c->corrA.p, nf, dpart, &nd));
+  prof_end(sd, KC_DENS);
+  prof_begin(sr, KC_DENS);
+  HCHK(rhoij_nhatw(sr, c->datoms, c->natom, (const c64*)c->cB.p, c->ctot, ns, c->nb, c->mtot,
                    c->rhoij + (size_t)s0 * c->mtot, c->qtab, c->nl2, c->wl));
-  HCHK(nhat_sparse(c->st, sphere_tabs(c), c->S, c->sph_g, c->wl, c->natom, c->nl2, ns,
+  HCHK(nhat_sparse(sr, sphere_tabs(c), c->S, c->sph_g, c->wl, c->natom, c->nl2, ns,
                    c->nhs + (size_t)s0 * std::max(1, c->S), npart, &nn));
-  prof_end(c->st, KC_DENS);
-  hipStream_t sp = c->st;   // stream of the potential chain
+  prof_end(sr, KC_DENS);
+  const hipStream_t sp = c->st;   // stream of the potential chain
   if (pipelined) {
-    HCHK(hipEventRecord(c->ev_nhat, c->st));
-    HCHK(hipStreamWaitEvent(c->st2, c->ev_nhat, 0));
-    sp = c->st2;
+    HCHK(hipEventRecord(c->ev_nhat, c->st2));
+    HCHK(hipStreamWaitEvent(c->st, c->ev_nhat, 0));
   } else {
     RCHK(join_side(c));
   }
@@ -1522,9 +1544,13 @@ static int flow_rk4(opaw_ctx* c, double dt, double* dip_out) {
   cx<R>* acc = (cx<R>*)c->wacc.p;
   cx<R>* h = (cx<R>*)c->wh.p;
   if (!c->prep_valid) RCHK(flow_prep<R>(c, p, nf));
-  prof_begin(c->st, KC_DENS);
-  HCHK(field_norms<R>(c->st, p, c->N, nf, dv, c->part, c->nrm0));
-  prof_end(c->st, KC_DENS);
+  // after a pipelined rebuild st carries the critical chain: the norms go to
+  // st2, ahead of the first stage's forward passes (everything after those
+  // waits for them)
+  const hipStream_t sn = c->pot_pending ? c->st2 : c->st;
+  prof_begin(sn, KC_DENS);
+  HCHK(field_norms<R>(sn, p, c->N, nf, dv, c->part, c->nrm0));
+  prof_end(sn, KC_DENS);
   const double cst[4] = {dt / 2.0, dt / 2.0, dt, dt};
   const bool fused = plane_supported<R>(c->nx, c->ny);
   for (int stage = 1; stage <= 4; ++stage) {

@@ -634,7 +634,8 @@ hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* in, cx
                           int nf, int nf_per_sys, const R* ek, const R* vks,
                           const cx<R>* twx, const cx<R>* twy, const cx<R>* twz,
                           const int* umap, const int* upt, const int* uzr, const c64* cpre, const c64* cnl,
-                          int U, hipEvent_t before_inv, const PlaneRk4<R>* rk) {
+                          int U, hipEvent_t before_inv, const PlaneRk4<R>* rk, hipStream_t st_inv,
+                          hipEvent_t ev_mid) {
   const long long N = (long long)g.nx * g.ny * g.nz;
   View v{N, 0, 0, 0, g.nx, g.ny};
   XLines Lx{0, g.ny, 0, g.nz, nf, 0, g.nx, 0, g.nx};
@@ -642,6 +643,17 @@ hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* in, cx
   SLines Lz{g.nx, 0, g.ny, nf, 0, g.nz, 0, g.nz};
   View vt{0, 0, 0, 0, g.nx, g.ny};
   hipError_t e;
+  // the last pass runs on si: st, or st_inv once the passes before it (on st)
+  // have ended (ev_mid).  The split is keyed on the event: a null st_inv is
+  // the legacy default stream, a valid target.
+  hipStream_t si = st;
+  auto hand_over = [&]() -> hipError_t {
+    if (!ev_mid) return hipSuccess;
+    hipError_t r = hipEventRecord(ev_mid, st);
+    if (r == hipSuccess) r = hipStreamWaitEvent(st_inv, ev_mid, 0);
+    si = st_inv;
+    return r;
+  };
   if (plane_supported<R>(g.nx, g.ny)) {
     prof_begin(st, KC_FFT_X);
     e = launch_plane<R, -1, EPI_NONE>(g.nx, g.ny, st, in, tmp, g.nz, nf, N, twx, twy, nullptr, nullptr, nf_per_sys,
@@ -649,18 +661,19 @@ hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* in, cx
     prof_end(st, KC_FFT_X);
     if (e) return e;
     if ((e = fft_s_pass<R>(KC_FFT_Z, g.nz, 2, -1, EPI_MULT, st, tmp, v, tmp, v, Lz, twz, ek, vt))) return e;
-    if (before_inv && (e = hipStreamWaitEvent(st, before_inv, 0))) return e;
-    prof_begin(st, KC_FFT_XH);
+    if ((e = hand_over())) return e;
+    if (before_inv && (e = hipStreamWaitEvent(si, before_inv, 0))) return e;
+    prof_begin(si, KC_FFT_XH);
     if (rk == nullptr)
-      e = launch_plane<R, 1, EPI_HPSI>(g.nx, g.ny, st, tmp, out, g.nz, nf, N, twx, twy, in, vks, nf_per_sys, upt,
+      e = launch_plane<R, 1, EPI_HPSI>(g.nx, g.ny, si, tmp, out, g.nz, nf, N, twx, twy, in, vks, nf_per_sys, upt,
                                        uzr, cpre, cnl, U);
     else if (rk->first)
-      e = launch_plane<R, 1, EPI_RK4F>(g.nx, g.ny, st, tmp, out, g.nz, nf, N, twx, twy, in, vks, nf_per_sys, upt,
+      e = launch_plane<R, 1, EPI_RK4F>(g.nx, g.ny, si, tmp, out, g.nz, nf, N, twx, twy, in, vks, nf_per_sys, upt,
                                        uzr, cpre, cnl, U, *rk);
     else
-      e = launch_plane<R, 1, EPI_RK4>(g.nx, g.ny, st, tmp, out, g.nz, nf, N, twx, twy, in, vks, nf_per_sys, upt,
+      e = launch_plane<R, 1, EPI_RK4>(g.nx, g.ny, si, tmp, out, g.nz, nf, N, twx, twy, in, vks, nf_per_sys, upt,
                                       uzr, cpre, cnl, U, *rk);
-    prof_end(st, KC_FFT_XH);
+    prof_end(si, KC_FFT_XH);
     return e;
   }
   if (rk) return hipErrorInvalidValue;   // the RK4 epilogue exists on the plane path only
@@ -670,8 +683,9 @@ hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* in, cx
   if ((e = fft_s_pass<R>(KC_FFT_Y, g.ny, 1, -1, EPI_NONE, st, tmp, v, tmp, v, Ly, twy, nullptr, vt))) return e;
   if ((e = fft_s_pass<R>(KC_FFT_Z, g.nz, 2, -1, EPI_MULT, st, tmp, v, tmp, v, Lz, twz, ek, vt))) return e;
   if ((e = fft_s_pass<R>(KC_FFT_Y, g.ny, 1, 1, EPI_NONE, st, tmp, v, tmp, v, Ly, twy, nullptr, vt))) return e;
-  if (before_inv && (e = hipStreamWaitEvent(st, before_inv, 0))) return e;
-  return fft_x_pass<R>(KC_FFT_XH, g.nx, 1, EPI_HPSI, st, tmp, v, out, v, Lx, twx, in, vks, nf_per_sys, umap, cpre, cnl,
+  if ((e = hand_over())) return e;
+  if (before_inv && (e = hipStreamWaitEvent(si, before_inv, 0))) return e;
+  return fft_x_pass<R>(KC_FFT_XH, g.nx, 1, EPI_HPSI, si, tmp, v, out, v, Lx, twx, in, vks, nf_per_sys, umap, cpre, cnl,
                        U);
 }
 
@@ -709,11 +723,11 @@ hipError_t hartree_conv(const GridGeom& g, int scale, hipStream_t st, const c64*
 template hipError_t apply_t_plus_v<double>(const GridGeom&, hipStream_t, const c64*, c64*, c64*, int, int,
                                            const double*, const double*, const c64*, const c64*, const c64*,
                                            const int*, const int*, const int*, const c64*, const c64*, int,
-                                           hipEvent_t, const PlaneRk4<double>*);
+                                           hipEvent_t, const PlaneRk4<double>*, hipStream_t, hipEvent_t);
 template hipError_t apply_t_plus_v<float>(const GridGeom&, hipStream_t, const c32*, c32*, c32*, int, int,
                                           const float*, const float*, const c32*, const c32*, const c32*,
                                           const int*, const int*, const int*, const c64*, const c64*, int,
-                                          hipEvent_t, const PlaneRk4<float>*);
+                                          hipEvent_t, const PlaneRk4<float>*, hipStream_t, hipEvent_t);
 
 // plain 3D transform of nf fields (used by tests and the Hartree-free paths)
 template <typename R>

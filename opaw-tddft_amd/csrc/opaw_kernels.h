@@ -41,12 +41,16 @@ bool plane_supported(int nx, int ny);
 // union points of plane z are [uzr[z], uzr[z+1])).  cnl is read only by the
 // last (inverse) pass, which first waits for before_inv when it is set, so
 // cnl may still be in production on another stream during the first passes.
+// With ev_mid set, the passes before the last run on st and the last pass on
+// st_inv (which may be the null stream), after them: ev_mid is recorded on st
+// and st_inv waits for it.
 template <typename R>
 hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* in, cx<R>* out, cx<R>* tmp,
                           int nf, int nf_per_sys, const R* ek, const R* vks,
                           const cx<R>* twx, const cx<R>* twy, const cx<R>* twz,
                           const int* umap, const int* upt, const int* uzr, const c64* cpre, const c64* cnl,
-                          int U, hipEvent_t before_inv, const PlaneRk4<R>* rk = nullptr);
+                          int U, hipEvent_t before_inv, const PlaneRk4<R>* rk = nullptr,
+                          hipStream_t st_inv = nullptr, hipEvent_t ev_mid = nullptr);
 
 template <typename R>
 hipError_t fft3d(const GridGeom& g, hipStream_t st, int dir, const cx<R>* in, cx<R>* out, int nf,
