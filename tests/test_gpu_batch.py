@@ -15,6 +15,8 @@ checked against the oracle in the other GPU tests.
 * The polyacene series n = 1..6 (48^3 ... 128 x 64 x 48; 15 ... 60 orbitals;
   4 systems each: unperturbed and kicks along x, y, z), mixed path, graph
   replay: equal to the molecules run alone, norms conserved.
+* The time loop on the legacy null stream (twice) and on a stream of its
+  own: bit-identical.
 * Argument errors: a context twice, a null context, nt <= 0.
 """
 import numpy as np
@@ -101,6 +103,29 @@ def test_batch_equals_alone_graph_and_plain():
         for s in range(g.nsys):
             assert np.array_equal(g.read_orbitals(s), rorb[s]), (k, s)
         g.close()
+
+
+def test_time_loop_same_on_null_and_own_stream():
+    """The time loop splits its work over the caller's stream and a side
+    stream.  Run on the legacy default (null) stream twice and on a stream of
+    its own, the same context must give bit-identical dipoles and orbitals
+    (a missing hand-over between the streams shows up as run-to-run noise)."""
+    torch, OpawGPU = _api()
+    spec = batch.polyacene_spec(1)
+    phi = batch.polyacene_orbitals(spec, seed=3)
+    runs = []
+    for mode in ("null", "null", "own"):
+        s = torch.cuda.Stream() if mode == "own" else None
+        g = OpawGPU(spec, "f64", nsys=2, nb=phi.shape[0], stream=(s.cuda_stream if s is not None else 0))
+        g.write_orbitals(0, phi)
+        g.excite_psi_bar_pert(spec.sm, 1, from_sys=0, to_sys=1)
+        _, dip = g.opaw_tddft(3, spec.dt, spec.sm)
+        torch.cuda.synchronize()
+        runs.append((dip, g.read_orbitals(0), g.read_orbitals(1)))
+        g.close()
+    for k in (1, 2):
+        for a, b in zip(runs[0], runs[k]):
+            assert np.array_equal(a, b), k
 
 
 def test_batch_step_vs_oracle():
