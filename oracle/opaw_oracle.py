@@ -178,13 +178,33 @@ def ekcut_default(g: Grid) -> float:
 
 
 # ---------------------------------------------------------------------------
+# Threads of the 3D transforms.  1 (the default, used by every test): numpy's
+# pocketfft.  More: scipy.fft's threaded pocketfft, which only the timed CPU
+# baseline asks for (the reference runs one FFTW transform per MPI rank).
+FFT_WORKERS = 1
+
+
+def _fftn(a, axes=None):
+    if FFT_WORKERS > 1:
+        import scipy.fft
+        return scipy.fft.fftn(a, axes=axes, workers=FFT_WORKERS)
+    return np.fft.fftn(a, axes=axes)
+
+
+def _ifftn(a, axes=None):
+    if FFT_WORKERS > 1:
+        import scipy.fft
+        return scipy.fft.ifftn(a, axes=axes, workers=FFT_WORKERS)
+    return np.fft.ifftn(a, axes=axes)
+
+
 # FFTW semantics: src/lib/library_fftw_v3.f90:56-83 (unnormalised, forward e^-i)
 def fft3d_forward(g: Grid, f):
-    return np.fft.fftn(to3(g, f), axes=(-3, -2, -1)).reshape(f.shape)
+    return _fftn(to3(g, f), axes=(-3, -2, -1)).reshape(f.shape)
 
 
 def fft3d_backward(g: Grid, f):
-    return (np.fft.ifftn(to3(g, f), axes=(-3, -2, -1)) * g.n).reshape(f.shape)
+    return (_ifftn(to3(g, f), axes=(-3, -2, -1)) * g.n).reshape(f.shape)
 
 
 # ---------------------------------------------------------------------------
@@ -349,7 +369,7 @@ def vh_sub(st: OpawState, rho: np.ndarray) -> np.ndarray:
     densb[oz:oz + g.nz, oy:oy + g.ny, ox:ox + g.nx] = to3(g, rho)
     vkb = st.vk.reshape(nzb, nyb, nxb)
     # c2r(r2c(densb) * vk) unnormalised both ways
-    vhb = np.real(np.fft.ifftn(np.fft.fftn(densb) * vkb)) * (nxb * nyb * nzb)
+    vhb = np.real(_ifftn(_fftn(densb) * vkb)) * (nxb * nyb * nzb)
     return vhb[oz:oz + g.nz, oy:oy + g.ny, ox:ox + g.nx].reshape(-1).copy()
 
 
