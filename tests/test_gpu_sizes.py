@@ -150,6 +150,30 @@ def test_rectangular_plane_grids(shape):
         g.close()
 
 
+def test_mixed_planes_odd_count():
+    """An odd number of xy planes (3 orbitals x 45 planes on a periodic
+    64 x 48 x 45 grid: the MT doubling needs even sizes, and 45 runs on the
+    runtime-planned z pass): FP64 H.psi against the oracle, mixed H.psi and
+    shs against FP64."""
+    torch, OpawGPU = _api()
+    spec = synth.hpsi_system(n=(64, 48, 45), d=0.4, natom=6, seed=29)
+    nb = 3
+    st = state_from_spec(spec, nsys=1, with_vk=False)
+    _potentials(spec, st, 8)
+    phi = _orbitals(spec, nb, seed=12, envelope=3.0)
+    out = {}
+    for prec in ("f64", "mixed"):
+        g = OpawGPU(spec, prec, nsys=1, nb=nb)
+        _push(g, st)
+        x = g.to_device(phi)
+        out[prec] = (g.h_phi(x, 0).cpu().numpy().astype(np.complex128),
+                     g.shs(x, 0).cpu().numpy().astype(np.complex128))
+        g.close()
+    assert rel(out["f64"][0], O.h_phi(st, phi, 0)) < 1e-10
+    for k in (0, 1):
+        assert rel(out["mixed"][k], out["f64"][k]) < 2e-5, k
+
+
 @pytest.mark.parametrize("shape", [(48, 40, 32), (40, 48, 24)])
 def test_line_path_grids(shape):
     """nx != ny: the kinetic operator runs as five line passes (x, y, z, y, x)
