@@ -255,8 +255,11 @@ struct opaw_ctx {
     bool ready = false;
   };
   std::vector<Snap> snaps;
-  // k shells (sets of k points with equal ek) for the training sums
+  // k shells (sets of k points with equal ek) for the training sums and the
+  // preconditioner table
   DevBuf shPerm, shOff, shEk;
+  DevBuf shOf;        // shell of each k point
+  DevBuf gsTab;       // preconditioner multipliers per (band, shell)
   int nsh = 0;
 
   size_t esize() const { return prec == OPAW_PREC_F64 ? sizeof(c64) : sizeof(c32); }
@@ -2482,6 +2485,8 @@ int opaw_residuals(opaw_ctx* c, int sys, int h_type, const double* eig, double* 
 // given), then Rayleigh-Ritz on [phi, w, p] with p the previous step's
 // (w, p) part of the new vectors.  Stops when the residual norms of the
 // first nconv bands are <= tol; hist[it] = their largest at iteration it.
+static int ensure_shells(opaw_ctx* c);
+
 int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, int nconv, const double* mlp, int nh,
                      int record, double* eig, double* hist, int* iters) {
   CTX_CHECK(c);
@@ -2520,6 +2525,8 @@ int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, in
   double* deig = (double*)c->gsSmall.p;
   double* dt = deig + nb;
   double* dy = dt + nb;
+  RCHK(ensure_shells(c));
+  RCHK(ensure(c, c->gsTab, (size_t)nb * c->nsh * sizeof(double)));
   HCHK(hipMemcpyAsync(Y, phi, fb, hipMemcpyDeviceToDevice, c->st));
   RCHK(apply_h(c, sys, h_type, Y, HY, nb));
   std::vector<double> e(nb), t(nb), y(nb);
@@ -2580,7 +2587,8 @@ int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, in
       c->snaps.push_back(sp);
     }
     HCHK(fft3d<double>(c->g, c->st, -1, W, Kb, nb, twx, twy, twz));
-    HCHK(precond_apply(c->st, Kb, N, nb, c->ek, dt, dy, dmlp, shape));
+    HCHK(precond_apply_shells(c->st, Kb, N, nb, (const int*)c->shOf.p, (const double*)c->shEk.p, c->nsh, dt, dy, dmlp,
+                              shape, (double*)c->gsTab.p));
     HCHK(fft3d<double>(c->g, c->st, 1, Kb, W, nb, twx, twy, twz));
     RCHK(apply_h(c, sys, h_type, W, HY + off, nb));
     // Rayleigh-Ritz on [phi, w, p] (LOBPCG; no p in the first iteration);
@@ -2648,6 +2656,11 @@ static int ensure_shells(opaw_ctx* c) {
       eks.push_back(ek[perm[(size_t)q]]);
     }
   off.push_back((int)N);
+  std::vector<int> of((size_t)N);
+  for (size_t s = 0; s + 1 < off.size(); ++s)
+    for (int q = off[s]; q < off[s + 1]; ++q) of[(size_t)perm[(size_t)q]] = (int)s;
+  RCHK(ensure(c, c->shOf, (size_t)N * sizeof(int)));
+  HCHK(hipMemcpy(c->shOf.p, of.data(), (size_t)N * sizeof(int), hipMemcpyHostToDevice));
   RCHK(ensure(c, c->shPerm, (size_t)N * sizeof(int)));
   RCHK(ensure(c, c->shOff, off.size() * sizeof(int)));
   RCHK(ensure(c, c->shEk, eks.size() * sizeof(double)));

@@ -48,6 +48,11 @@ struct MlpShape {
 int mlp_param_count(const MlpShape& s);
 hipError_t precond_apply(hipStream_t st, c64* rk, long long N, int nb, const double* ek, const double* tkin,
                          const double* yfeat, const double* params, MlpShape s);
+// the same product with M evaluated once per (band, k shell) into tab
+// (nb x nsh); shell_of[k] is the shell of k point k, eks[s] its ek
+hipError_t precond_apply_shells(hipStream_t st, c64* rk, long long N, int nb, const int* shell_of, const double* eks,
+                                int nsh, const double* tkin, const double* yfeat, const double* params, MlpShape s,
+                                double* tab);
 // training: for each band, A_i = sum conj(e) M r, B_i = sum M^2 |r|^2 (k space),
 // loss_i = 1 - |A_i|^2 / (|e_i|^2 B_i); grads of sum_i loss_i w.r.t. params
 hipError_t precond_loss_sums(hipStream_t st, const c64* rk, const c64* ekf, long long N, int nb, const double* ek,

@@ -173,6 +173,35 @@ __global__ __launch_bounds__(kTB) void k_precond(c64* __restrict__ rk, long long
   rk[q] = mk(rk[q].x * m, rk[q].y * m);
 }
 
+// The multiplier depends on k only through ek(k), so it is evaluated once per
+// (band, k shell): tab[i * nsh + s] = M_i(eks[s]) / N, with the same
+// arithmetic as k_precond.
+__global__ __launch_bounds__(kTB) void k_precond_table(const double* __restrict__ eks, int nsh, long long N, int nb,
+                                                       const double* __restrict__ tkin,
+                                                       const double* __restrict__ yfeat, const double* __restrict__ P,
+                                                       int nh, double* __restrict__ tab) {
+  const long long q = (long long)blockIdx.x * kTB + threadIdx.x;
+  if (q >= (long long)nsh * nb) return;
+  const int i = (int)(q / nsh);
+  const int sh = (int)(q - (long long)i * nsh);
+  Fwd f;
+  tab[q] = multiplier(P, nh, eks[sh] * (double)N / tkin[i], yfeat[i], f) / (double)N;
+}
+
+// rk[i * N + k] *= tab[i * nsh + shell(k)]; one thread per k over all bands
+__global__ __launch_bounds__(kTB) void k_precond_lookup(c64* __restrict__ rk, long long N, int nb,
+                                                        const int* __restrict__ shell_of,
+                                                        const double* __restrict__ tab, int nsh) {
+  const long long k = (long long)blockIdx.x * kTB + threadIdx.x;
+  if (k >= N) return;
+  const int sh = shell_of[k];
+  for (int i = 0; i < nb; ++i) {
+    const double m = tab[(size_t)i * nsh + sh];
+    const size_t q = (size_t)i * N + k;
+    rk[q] = mk(rk[q].x * m, rk[q].y * m);
+  }
+}
+
 // block = (chunk c, band i): ReA, ImA, B, Ee partials
 __global__ __launch_bounds__(kTB) void k_loss_sums(const c64* __restrict__ rk, const c64* __restrict__ ekf, long long N,
                                                    int nb, const double* __restrict__ ek, const double* __restrict__ tkin,
@@ -623,6 +652,15 @@ int mlp_param_count(const MlpShape& s) { return s.nh * s.nin + s.nh + s.nh * s.n
 hipError_t precond_apply(hipStream_t st, c64* rk, long long N, int nb, const double* ek, const double* tkin,
                          const double* yfeat, const double* params, MlpShape s) {
   hipLaunchKernelGGL(k_precond, dim3(blocks_for(N * nb)), dim3(kTB), 0, st, rk, N, nb, ek, tkin, yfeat, params, s.nh);
+  return hipGetLastError();
+}
+
+hipError_t precond_apply_shells(hipStream_t st, c64* rk, long long N, int nb, const int* shell_of, const double* eks,
+                                int nsh, const double* tkin, const double* yfeat, const double* params, MlpShape s,
+                                double* tab) {
+  hipLaunchKernelGGL(k_precond_table, dim3(blocks_for((long long)nsh * nb)), dim3(kTB), 0, st, eks, nsh, N, nb, tkin,
+                     yfeat, params, s.nh, tab);
+  hipLaunchKernelGGL(k_precond_lookup, dim3(blocks_for(N)), dim3(kTB), 0, st, rk, N, nb, shell_of, tab, nsh);
   return hipGetLastError();
 }
 

@@ -114,6 +114,58 @@ def test_block_solver_converges_and_oracle_agrees():
     g.close()
 
 
+@pytest.mark.parametrize("learned", [False, True])
+def test_one_preconditioned_update_matches_oracle(learned):
+    """One LOBPCG update on the GPU (residuals r = H phi - e phi, band
+    kinetic energies t, w = M r in k space with M evaluated per (band, k
+    shell) and looked up per k point, Rayleigh-Ritz on [phi, w]) against the
+    same update formed on the CPU with the oracle's shs, FFTs and the
+    multiplier evaluated at every k point.  A k point given the wrong shell,
+    or a wrong multiplier, changes w and moves the Ritz values and the
+    updated span at first order."""
+    import scipy.linalg
+    spec, st, phi = _system(nb=4)
+    nh = 8
+    P = None
+    if learned:
+        from opaw_amd.gs import mlp_init
+        P = mlp_init(nh, seed=3, scale=0.4)
+        P[4 * nh + nh * nh:] = np.random.default_rng(4).normal(0, 0.3, nh + 1)
+    g = _gpu(spec, st, phi.shape[0])
+    g.write_orbitals(0, phi)
+    eig, _, its = g.block_solve(1, 0.0, mlp=P, nh=nh)
+    assert its == 1
+    out = g.read_orbitals(0)
+    g.close()
+    # the same update on the CPU
+    N, dv, grid = spec.n, spec.dv, st.grid
+    ek = O.ek_prep(grid, spec.ekcut)                 # carries 1/N
+    H = O.shs(st, phi, 0)
+    e = np.real(np.sum(phi.conj() * H, axis=1)) * dv
+    r = H - e[:, None] * phi
+    F = O.fft3d_forward(grid, phi)
+    t = np.array([np.sum(ek * np.abs(f) ** 2) / np.sum(np.abs(f) ** 2) * N for f in F])
+    t = np.maximum(t, 1e-6)
+    y = e / t
+    Rk = O.fft3d_forward(grid, r)
+    for i in range(len(e)):
+        m, _ = GO.multiplier(P, nh, ek * N / t[i], y[i])
+        Rk[i] *= m / N
+    W = O.fft3d_backward(grid, Rk)
+    Yb = np.concatenate([phi, W])
+    HYb = np.concatenate([H, O.shs(st, W, 0)])
+    S = (Yb.conj() @ Yb.T) * dv
+    Hm = (Yb.conj() @ HYb.T) * dv
+    w, V = scipy.linalg.eigh(0.5 * (Hm + Hm.conj().T), 0.5 * (S + S.conj().T))
+    nb = phi.shape[0]
+    assert np.max(np.abs(eig - w[:nb])) < 1e-10 * max(1.0, np.max(np.abs(w[:nb])))
+    ref = V[:, :nb].T @ Yb                            # updated orbitals (rows)
+    # the GPU's orbitals lie in the oracle's updated span (first order in w)
+    Q = np.linalg.qr(ref.T)[0]
+    resid = out.T - Q @ (Q.conj().T @ out.T)
+    assert np.linalg.norm(resid) < 1e-9 * np.linalg.norm(out)
+
+
 def test_preconditioner_loss_and_gradient_match_oracle():
     spec, st, phi = _system(nb=4)
     g = _gpu(spec, st, phi.shape[0])
