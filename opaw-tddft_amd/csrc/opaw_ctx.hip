@@ -2223,69 +2223,13 @@ static int apply_h(opaw_ctx* c, int sys, int h_type, const c64* in, c64* out, in
 }
 
 // generalised eigenproblem H C = S C w on an m-dimensional basis (row-major
-// host matrices); directions with overlap below 1e-12 of the largest are
+// host matrices): Cholesky when the unit-scaled overlap is safely positive
+// definite, else directions with overlap below 1e-12 of the largest are
 // dropped; returns the lowest nout eigenpairs, C column-major m x nout
 static int subspace_eig(int m, const std::vector<std::complex<double>>& S0, const std::vector<std::complex<double>>& H0,
                         int nout, std::vector<std::complex<double>>& Ccol, std::vector<double>& w) {
-  // unit-norm columns first (D S D, D H D): small search directions are not
-  // linearly dependent ones, and the rank test below must not drop them
-  std::vector<double> dsc(m);
-  for (int i = 0; i < m; ++i) {
-    const double sii = S0[(size_t)i * m + i].real();
-    dsc[i] = sii > 0.0 ? 1.0 / std::sqrt(sii) : 0.0;
-  }
-  std::vector<std::complex<double>> S((size_t)m * m), H((size_t)m * m);
-  for (int i = 0; i < m; ++i)
-    for (int j = 0; j < m; ++j) {
-      S[(size_t)i * m + j] = S0[(size_t)i * m + j] * (dsc[i] * dsc[j]);
-      H[(size_t)i * m + j] = H0[(size_t)i * m + j] * (dsc[i] * dsc[j]);
-    }
-  std::vector<double> ws;
-  std::vector<std::complex<double>> Vs;
-  herm_eig(m, S, ws, Vs);
-  const double top = ws.back();
-  std::vector<int> keep;
-  for (int k = 0; k < m; ++k)
-    if (ws[k] > 1e-12 * top) keep.push_back(k);
-  const int r = (int)keep.size();
+  const int r = pencil_lowest(m, S0, H0, nout, 1e-12, Ccol, w);
   if (r < nout) return fail(OPAW_ERR_ARG, "subspace rank %d below the %d bands", r, nout);
-  std::vector<std::complex<double>> X((size_t)m * r);   // row-major m x r
-  for (int i = 0; i < m; ++i)
-    for (int k = 0; k < r; ++k) X[(size_t)i * r + k] = Vs[(size_t)i * m + keep[k]] / std::sqrt(ws[keep[k]]);
-  // products in real arithmetic (la::mul / la::cmul: no library call per
-  // complex product)
-  std::vector<std::complex<double>> HX((size_t)m * r), Hr((size_t)r * r);
-  for (int i = 0; i < m; ++i)
-    for (int j = 0; j < m; ++j) {
-      const std::complex<double> h = H[(size_t)i * m + j];
-      const std::complex<double>* xj = &X[(size_t)j * r];
-      std::complex<double>* out = &HX[(size_t)i * r];
-      for (int k = 0; k < r; ++k) out[k] += la::mul(h, xj[k]);
-    }
-  for (int i = 0; i < m; ++i)
-    for (int a = 0; a < r; ++a) {
-      const std::complex<double> xa = X[(size_t)i * r + a];
-      const std::complex<double>* hx = &HX[(size_t)i * r];
-      std::complex<double>* out = &Hr[(size_t)a * r];
-      for (int b = 0; b < r; ++b) out[b] += la::cmul(xa, hx[b]);
-    }
-  for (int a = 0; a < r; ++a)
-    for (int b = a + 1; b < r; ++b) {
-      const std::complex<double> h = 0.5 * (Hr[(size_t)a * r + b] + std::conj(Hr[(size_t)b * r + a]));
-      Hr[(size_t)a * r + b] = h;
-      Hr[(size_t)b * r + a] = std::conj(h);
-    }
-  std::vector<double> wh;
-  std::vector<std::complex<double>> U;
-  herm_eig(r, Hr, wh, U);
-  Ccol.assign((size_t)m * nout, 0.0);
-  w.assign(wh.begin(), wh.begin() + nout);
-  for (int j = 0; j < nout; ++j)
-    for (int i = 0; i < m; ++i) {
-      std::complex<double> s = 0.0;
-      for (int k = 0; k < r; ++k) s += la::mul(X[(size_t)i * r + k], U[(size_t)k * r + j]);
-      Ccol[i + (size_t)m * j] = s * dsc[i];
-    }
   return OPAW_OK;
 }
 
