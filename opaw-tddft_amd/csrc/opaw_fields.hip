@@ -733,16 +733,35 @@ __global__ __launch_bounds__(256) void k_gram_block(const cx<R>* __restrict__ a,
     }
 }
 
-__global__ void k_gram_finish(const c64* __restrict__ part, int nb, int nchunk, double dv, c64* __restrict__ g) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nb * nb) return;
+// g[q] = dv * sum over chunks c of part[c * nb * nb + q].  A workgroup takes
+// kGfQ consecutive entries with kGfL lanes each: lane l sums chunks l,
+// l + kGfL, ... (the kGfQ entries of one chunk are adjacent in memory), and
+// the lane sums are added in LDS in a fixed tree, so the order is fixed.
+constexpr int kGfQ = 8, kGfL = 32;
+__global__ __launch_bounds__(kGfQ* kGfL) void k_gram_finish(const c64* __restrict__ part, int nb, int nchunk,
+                                                             double dv, c64* __restrict__ g) {
+  const int ql = threadIdx.x % kGfQ, l = threadIdx.x / kGfQ;
+  const int nq = nb * nb;
+  const int q = blockIdx.x * kGfQ + ql;
   double sr = 0.0, si = 0.0;
-  for (int c = 0; c < nchunk; ++c) {
-    const c64 v = part[(long long)c * nb * nb + q];
-    sr += v.x;
-    si += v.y;
+  if (q < nq)
+    for (int c = l; c < nchunk; c += kGfL) {
+      const c64 v = part[(long long)c * nq + q];
+      sr += v.x;
+      si += v.y;
+    }
+  __shared__ double rr[kGfQ * kGfL], ri[kGfQ * kGfL];
+  rr[threadIdx.x] = sr;
+  ri[threadIdx.x] = si;
+  __syncthreads();
+  for (int o = kGfL / 2; o > 0; o >>= 1) {
+    if (l < o) {
+      rr[threadIdx.x] += rr[threadIdx.x + o * kGfQ];
+      ri[threadIdx.x] += ri[threadIdx.x + o * kGfQ];
+    }
+    __syncthreads();
   }
-  g[q] = mk(sr * dv, si * dv);
+  if (l == 0 && q < nq) g[q] = mk(rr[ql] * dv, ri[ql] * dv);
 }
 
 // out_j = sum_i T[i + nb*j] in_i   (nb x nb complex FP64 T, column-major)
@@ -1168,7 +1187,8 @@ hipError_t gram(hipStream_t st, const cx<R>* a, const cx<R>* b, long long N, int
   if (t > 4) {
     hipLaunchKernelGGL((k_gram_partial<R>), dim3(t, t, kGramChunks), dim3(256), 0, st, a, b, N, nb, kGramChunks,
                        part);
-    hipLaunchKernelGGL(k_gram_finish, dim3((nb * nb + 255) / 256), dim3(256), 0, st, part, nb, kGramChunks, dv, g);
+    hipLaunchKernelGGL(k_gram_finish, dim3((nb * nb + kGfQ - 1) / kGfQ), dim3(kGfQ * kGfL), 0, st, part, nb,
+                       kGramChunks, dv, g);
     return hipGetLastError();
   }
   const dim3 grid(kGramBlockChunks);
@@ -1180,7 +1200,8 @@ hipError_t gram(hipStream_t st, const cx<R>* a, const cx<R>* b, long long N, int
     hipLaunchKernelGGL((k_gram_block<R, 3>), grid, dim3(256), 0, st, a, b, N, nb, kGramBlockChunks, part);
   else
     hipLaunchKernelGGL((k_gram_block<R, 4>), grid, dim3(256), 0, st, a, b, N, nb, kGramBlockChunks, part);
-  hipLaunchKernelGGL(k_gram_finish, dim3((nb * nb + 255) / 256), dim3(256), 0, st, part, nb, kGramBlockChunks, dv, g);
+  hipLaunchKernelGGL(k_gram_finish, dim3((nb * nb + kGfQ - 1) / kGfQ), dim3(kGfQ * kGfL), 0, st, part, nb,
+                     kGramBlockChunks, dv, g);
   return hipGetLastError();
 }
 

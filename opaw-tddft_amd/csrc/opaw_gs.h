@@ -46,10 +46,8 @@ struct MlpShape {
   int nin = 2, nh = 16;   // 2 -> nh -> nh -> 1, tanh hidden layers
 };
 int mlp_param_count(const MlpShape& s);
-hipError_t precond_apply(hipStream_t st, c64* rk, long long N, int nb, const double* ek, const double* tkin,
-                         const double* yfeat, const double* params, MlpShape s);
-// the same product with M evaluated once per (band, k shell) into tab
-// (nb x nsh); shell_of[k] is the shell of k point k, eks[s] its ek
+// w = M r applied in place to rk with M evaluated once per (band, k shell)
+// into tab (nb x nsh); shell_of[k] is the shell of k point k, eks[s] its ek
 hipError_t precond_apply_shells(hipStream_t st, c64* rk, long long N, int nb, const int* shell_of, const double* eks,
                                 int nsh, const double* tkin, const double* yfeat, const double* params, MlpShape s,
                                 double* tab);

@@ -160,22 +160,9 @@ __device__ __forceinline__ double multiplier(const double* __restrict__ P, int n
   return m;
 }
 
-// in place: rk[i*N + k] *= M_i(k) / N
-__global__ __launch_bounds__(kTB) void k_precond(c64* __restrict__ rk, long long N, int nb, const double* __restrict__ ek,
-                                                 const double* __restrict__ tkin, const double* __restrict__ yfeat,
-                                                 const double* __restrict__ P, int nh) {
-  const long long q = (long long)blockIdx.x * kTB + threadIdx.x;
-  if (q >= N * nb) return;
-  const int i = (int)(q / N);
-  const long long k = q - (long long)i * N;
-  Fwd f;
-  const double m = multiplier(P, nh, ek[k] * (double)N / tkin[i], yfeat[i], f) / (double)N;
-  rk[q] = mk(rk[q].x * m, rk[q].y * m);
-}
-
-// The multiplier depends on k only through ek(k), so it is evaluated once per
-// (band, k shell): tab[i * nsh + s] = M_i(eks[s]) / N, with the same
-// arithmetic as k_precond.
+// w = M r in k space, with the FFT's 1/N: the multiplier depends on k only
+// through ek(k), so it is evaluated once per (band, k shell):
+// tab[i * nsh + s] = M_i(eks[s]) / N.
 __global__ __launch_bounds__(kTB) void k_precond_table(const double* __restrict__ eks, int nsh, long long N, int nb,
                                                        const double* __restrict__ tkin,
                                                        const double* __restrict__ yfeat, const double* __restrict__ P,
@@ -327,29 +314,57 @@ __global__ __launch_bounds__(kTile) void k_precond_grad(const c64* __restrict__ 
   }
 }
 
-// out_j = sum_{i<m} T[i + m j] in_i for j < nout (FP64 fields)
-__global__ __launch_bounds__(kTB) void k_rotate_rect(const c64* __restrict__ in, c64* __restrict__ out, long long N,
-                                                     int m, int nout, const c64* __restrict__ T) {
-  constexpr int JT = 8;
-  const int j0 = blockIdx.y * JT;
-  const long long r = (long long)blockIdx.x * kTB + threadIdx.x;
-  if (r >= N) return;
-  double ar[JT], ai[JT];
+// out_j = sum_{i<m} T[i + m j] in_i for j < nout (FP64 fields).  A thread
+// keeps kRotJ outputs of one point, so for nout <= kRotJ every input is read
+// once; the coefficients go through LDS, kRotRows rows of T at a time, and
+// four input fields are loaded before their products are added.
+constexpr int kRotJ = 16, kRotTB = 128, kRotRows = 64;
+__global__ __launch_bounds__(kRotTB) void k_rotate_rect(const c64* __restrict__ in, c64* __restrict__ out,
+                                                        long long N, int m, int nout, const c64* __restrict__ T) {
+  __shared__ c64 sT[kRotRows * kRotJ];   // sT[(i - i0) * kRotJ + q] = T[i + m (j0 + q)]
+  const int j0 = blockIdx.y * kRotJ;
+  const long long r = (long long)blockIdx.x * kRotTB + threadIdx.x;
+  const bool live = r < N;
+  double ar[kRotJ], ai[kRotJ];
 #pragma unroll
-  for (int q = 0; q < JT; ++q) ar[q] = ai[q] = 0.0;
-  for (int i = 0; i < m; ++i) {
-    const c64 w = in[(size_t)i * N + r];
+  for (int q = 0; q < kRotJ; ++q) ar[q] = ai[q] = 0.0;
+  for (int i0 = 0; i0 < m; i0 += kRotRows) {
+    const int mi = m - i0 < kRotRows ? m - i0 : kRotRows;
+    __syncthreads();
+    for (int t = threadIdx.x; t < mi * kRotJ; t += kRotTB) {
+      const int i = t / kRotJ, q = t % kRotJ;
+      sT[t] = j0 + q < nout ? T[i0 + i + (size_t)m * (j0 + q)] : mk(0.0, 0.0);
+    }
+    __syncthreads();
+    if (!live) continue;
+    const c64* src = in + (size_t)i0 * N + r;
+    int i = 0;
+    for (; i + 4 <= mi; i += 4) {
+      c64 w[4];
 #pragma unroll
-    for (int q = 0; q < JT; ++q) {
-      if (j0 + q < nout) {
-        const c64 t = T[i + (size_t)m * (j0 + q)];
+      for (int u = 0; u < 4; ++u) w[u] = src[(size_t)(i + u) * N];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int q = 0; q < kRotJ; ++q) {
+          const c64 t = sT[(i + u) * kRotJ + q];
+          ar[q] += w[u].x * t.x - w[u].y * t.y;
+          ai[q] += w[u].x * t.y + w[u].y * t.x;
+        }
+    }
+    for (; i < mi; ++i) {
+      const c64 w = src[(size_t)i * N];
+#pragma unroll
+      for (int q = 0; q < kRotJ; ++q) {
+        const c64 t = sT[i * kRotJ + q];
         ar[q] += w.x * t.x - w.y * t.y;
         ai[q] += w.x * t.y + w.y * t.x;
       }
     }
   }
+  if (!live) return;
 #pragma unroll
-  for (int q = 0; q < JT; ++q)
+  for (int q = 0; q < kRotJ; ++q)
     if (j0 + q < nout) out[(size_t)(j0 + q) * N + r] = mk(ar[q], ai[q]);
 }
 
@@ -592,7 +607,8 @@ hipError_t sub_real(hipStream_t st, const double* a, const double* b, long long 
 }
 
 hipError_t rotate_rect(hipStream_t st, const c64* in, c64* out, long long N, int m, int nout, const c64* T) {
-  hipLaunchKernelGGL(k_rotate_rect, dim3(blocks_for(N), (nout + 7) / 8), dim3(kTB), 0, st, in, out, N, m, nout, T);
+  hipLaunchKernelGGL(k_rotate_rect, dim3((unsigned)((N + kRotTB - 1) / kRotTB), (nout + kRotJ - 1) / kRotJ),
+                     dim3(kRotTB), 0, st, in, out, N, m, nout, T);
   return hipGetLastError();
 }
 
@@ -648,12 +664,6 @@ hipError_t axpy_complex(hipStream_t st, const c64* a, const c64* b, double s, lo
 }
 
 int mlp_param_count(const MlpShape& s) { return s.nh * s.nin + s.nh + s.nh * s.nh + s.nh + s.nh + 1; }
-
-hipError_t precond_apply(hipStream_t st, c64* rk, long long N, int nb, const double* ek, const double* tkin,
-                         const double* yfeat, const double* params, MlpShape s) {
-  hipLaunchKernelGGL(k_precond, dim3(blocks_for(N * nb)), dim3(kTB), 0, st, rk, N, nb, ek, tkin, yfeat, params, s.nh);
-  return hipGetLastError();
-}
 
 hipError_t precond_apply_shells(hipStream_t st, c64* rk, long long N, int nb, const int* shell_of, const double* eks,
                                 int nsh, const double* tkin, const double* yfeat, const double* params, MlpShape s,
