@@ -2445,19 +2445,18 @@ int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, in
   const int nb = c->nb;
   if (nconv <= 0 || nconv > nb) nconv = nb;
   const size_t fb = (size_t)nb * N * sizeof(c64);
-  RCHK(ensure(c, c->gsY, 3 * fb));
-  RCHK(ensure(c, c->gsHY, 3 * fb));
+  // two [phi | w | p] blocks and their H images: the Rayleigh-Ritz rotation
+  // writes the next iteration's phi and p into the other block
+  RCHK(ensure(c, c->gsY, 6 * fb));
+  RCHK(ensure(c, c->gsHY, 6 * fb));
   RCHK(ensure(c, c->gsK, fb));
-  RCHK(ensure(c, c->gsT, 4 * fb));
+  const size_t off = (size_t)nb * N;
   c64* phi = (c64*)c->phi + (size_t)sys * nb * N;
   c64* Y = (c64*)c->gsY.p;        // [phi | w | p]
   c64* HY = (c64*)c->gsHY.p;      // H of the same
+  c64* Yn = Y + 3 * off;
+  c64* HYn = HY + 3 * off;
   c64* Kb = (c64*)c->gsK.p;
-  c64* T0 = (c64*)c->gsT.p;
-  c64* T1 = T0 + (size_t)nb * N;
-  c64* T2 = T1 + (size_t)nb * N;
-  c64* T3 = T2 + (size_t)nb * N;
-  const size_t off = (size_t)nb * N;
   const c64 *twx = c->tw64[c->nx], *twy = c->tw64[c->ny], *twz = c->tw64[c->nz];
   const int npar = mlp_param_count(shape);
   double* dmlp = nullptr;
@@ -2564,14 +2563,12 @@ int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, in
     c64* dCp = dC + (size_t)m * nb;
     HCHK(hipMemcpyAsync(dC, Ccol.data(), Ccol.size() * sizeof(c64), hipMemcpyHostToDevice, c->st));
     HCHK(hipMemcpyAsync(dCp, Cp.data(), Cp.size() * sizeof(c64), hipMemcpyHostToDevice, c->st));
-    HCHK(rotate_rect(c->st, Y, T0, N, m, nb, dC));
-    HCHK(rotate_rect(c->st, HY, T1, N, m, nb, dC));
-    HCHK(rotate_rect(c->st, Y + off, T2, N, m - nb, nb, dCp));
-    HCHK(rotate_rect(c->st, HY + off, T3, N, m - nb, nb, dCp));
-    HCHK(hipMemcpyAsync(Y, T0, fb, hipMemcpyDeviceToDevice, c->st));
-    HCHK(hipMemcpyAsync(HY, T1, fb, hipMemcpyDeviceToDevice, c->st));
-    HCHK(hipMemcpyAsync(Y + 2 * off, T2, fb, hipMemcpyDeviceToDevice, c->st));
-    HCHK(hipMemcpyAsync(HY + 2 * off, T3, fb, hipMemcpyDeviceToDevice, c->st));
+    HCHK(rotate_rect(c->st, Y, Yn, N, m, nb, dC));
+    HCHK(rotate_rect(c->st, HY, HYn, N, m, nb, dC));
+    HCHK(rotate_rect(c->st, Y + off, Yn + 2 * off, N, m - nb, nb, dCp));
+    HCHK(rotate_rect(c->st, HY + off, HYn + 2 * off, N, m - nb, nb, dCp));
+    std::swap(Y, Yn);
+    std::swap(HY, HYn);
     have_p = true;
     e = w;
   }

@@ -141,3 +141,62 @@ def train_preconditioner(g, nh: int = 16, steps: int = 300, lr: float = 1e-2, se
     if loss < best[0]:
         best = (loss, p.copy())
     return best[1], losses
+
+
+# the held-out comparison of the solvers: grid -> (rings, bands, occupied)
+SCF_SYSTEMS = {32: (1, 8, 6), 48: (2, 16, 12)}
+
+
+def _scf_system(grid: int, seed: int):
+    from . import synth
+    from .api import OpawGPU
+    nring, nb, nocc = SCF_SYSTEMS[grid]
+    spec = synth.molecule_system(nring=nring, n=(grid, grid, grid), d=0.5, seed=seed)
+    spec.rnel = 2.0 * nocc
+    phi = synth.lowdin_orthonormalise(synth.random_orbitals(spec, nb, seed=seed + 1, envelope=3.0), spec.dv)
+    g = OpawGPU(spec, "f64", nsys=1, nb=nb)
+    g.set_nocc(nocc)
+    g.write_orbitals(0, phi)
+    return g, nocc
+
+
+def main(argv=None):
+    """Ground-state SCFs to 1e-8 on synthetic molecules (32^3: one ring, 8
+    bands, 6 occupied; 48^3: two rings, 16 bands, 12 occupied): the
+    Chebyshev-filter SCF, LOBPCG with Teter-Payne-Allan, and LOBPCG with the
+    learned preconditioner, trained on the pairs of one other system's SCF.
+    python -m opaw_amd.gs [--grid 48] [--methods cheb tpa mlp]"""
+    import argparse
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--grid", type=int, choices=sorted(SCF_SYSTEMS), default=48)
+    ap.add_argument("--methods", nargs="+", choices=["cheb", "tpa", "mlp"], default=["cheb", "tpa", "mlp"])
+    ap.add_argument("--seeds", type=int, nargs="+", default=[33, 45], help="held-out systems")
+    ap.add_argument("--train-seed", type=int, default=21)
+    ap.add_argument("--nh", type=int, default=8, help="MLP hidden width")
+    ap.add_argument("--steps", type=int, default=80, help="Adam steps")
+    a = ap.parse_args(argv)
+    P = None
+    if "mlp" in a.methods:
+        g, nocc = _scf_system(a.grid, a.train_seed)
+        r = scf_block(g, nocc, nscf=80, tol=1e-8, eta=0.1, record=1)
+        g.precond_prepare(0)
+        t0 = time.perf_counter()
+        P, losses = train_preconditioner(g, nh=a.nh, steps=a.steps, lr=2e-2, seed=1)
+        print(f"[gs] {a.grid}^3 training: {g.precond_count()[1]} pairs from a {r.iterations}-iteration SCF, "
+              f"{a.steps} Adam steps in {time.perf_counter() - t0:.1f} s, loss {losses[0]:.3f} -> {losses[-1]:.3f}",
+              flush=True)
+        g.close()
+    for seed in a.seeds:
+        for name in a.methods:
+            g, nocc = _scf_system(a.grid, seed)
+            if name == "cheb":
+                r = scf_chebyshev(g, nocc, nscf=80, tol=1e-8)
+            else:
+                r = scf_block(g, nocc, nscf=80, tol=1e-8, eta=0.1, mlp=(P if name == "mlp" else None), nh=a.nh)
+            print(f"[gs] {a.grid}^3 seed {seed} {name}: {r.iterations} SCF iterations, {r.seconds:.3f} s, "
+                  f"residual {r.residual:.1e}, {r.h_applications} band H applications", flush=True)
+            g.close()
+
+
+if __name__ == "__main__":
+    main()
