@@ -33,7 +33,7 @@ __global__ __launch_bounds__(kBoxThreads) void k_proj(const AtomDesc* __restrict
                                                       const int* __restrict__ bk, const double* __restrict__ tabs0,
                                                       const double* __restrict__ tabs1,
                                                       const cx<R>* __restrict__ u, long long N, int nf,
-                                                      const c64* __restrict__ cpre, int U,
+                                                      const c64* __restrict__ cpre, int /*table sets, = gridDim.z*/,
                                                       double* __restrict__ part0, double* __restrict__ part1) {
   // blockIdx.z selects the table set (two projections of the same u in one launch)
   const double* __restrict__ tabs = blockIdx.z ? tabs1 : tabs0;

@@ -138,7 +138,7 @@ __global__ __launch_bounds__(kT) void k_norm_partial(const cx<R>* __restrict__ u
 
 // out[r] = scale * sum_c part[r*nchunk + c]  (then sqrt if want_sqrt); one
 // workgroup per row, fixed reduction tree
-__global__ __launch_bounds__(kT) void k_finish_rows(const double* __restrict__ part, int rows, int nchunk,
+__global__ __launch_bounds__(kT) void k_finish_rows(const double* __restrict__ part, int /*rows: one per workgroup*/, int nchunk,
                                                      double scale, int want_sqrt, double* __restrict__ out) {
   const int r = blockIdx.x;
   double v[1] = {0.0};
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(kT) void k_density(const cx<R>* __restrict__ u, int
 // sphere entries of that point in atom order (no atomics).
 __global__ __launch_bounds__(64) void k_nhat_w(const AtomDesc* __restrict__ atoms, int natom,
                                                 const double* __restrict__ rhoij, int mtot,
-                                                const double* __restrict__ qtab, int nl2, int nsys,
+                                                const double* __restrict__ qtab, int nl2, int /*nsys: one per blockIdx.y*/,
                                                 double* __restrict__ wl) {
   const int a = blockIdx.x, s = blockIdx.y;
   const AtomDesc A = atoms[a];
@@ -320,7 +320,7 @@ __global__ __launch_bounds__(kT) void k_sum2_partial(const double* __restrict__ 
 }
 
 // fac[s] = rnel / (dv*sum nhat + dv*sum dens); one workgroup per system
-__global__ __launch_bounds__(kT) void k_renorm_factor(const double* __restrict__ part, int nsys, int nchunk,
+__global__ __launch_bounds__(kT) void k_renorm_factor(const double* __restrict__ part, int /*nsys: one per workgroup*/, int nchunk,
                                                        double dv, double rnel, double* __restrict__ fac) {
   const int s = blockIdx.x;
   double v[2] = {0.0, 0.0};
@@ -501,7 +501,7 @@ __global__ __launch_bounds__(kT) void k_dij(const AtomDesc* __restrict__ atoms, 
     }
   }
   __syncthreads();
-  if (threadIdx.x < nl2) {
+  if ((int)threadIdx.x < nl2) {
     double t = 0.0;
     for (int k = 0; k < kT / 64; ++k) t += red[k][threadIdx.x];
     prod[threadIdx.x] = t * dv;
