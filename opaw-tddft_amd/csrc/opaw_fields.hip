@@ -6,6 +6,8 @@
 // accumulate in FP64 for both precisions.
 #include "opaw_fields.h"
 
+#include <type_traits>
+
 namespace opaw {
 
 constexpr int kT = 256;
@@ -733,6 +735,81 @@ __global__ __launch_bounds__(256) void k_gram_block(const cx<R>* __restrict__ a,
     }
 }
 
+// FP64 Gram of up to 16 RB fields on the matrix cores, same partials as
+// k_gram_block.  With G = A^H B, Re G = Ar^T Br + Ai^T Bi and
+// Im G = Ar^T Bi - Ai^T Br: four v_mfma_f64_16x16x4f64 per 16 x 16 tile and
+// 4 points.  Wave w owns rows 16w..16w+15 of the block (RB tiles); 16
+// points of every field go through LDS per step (16 consecutive points of
+// one field per 16 lanes), the next 16 are loaded into registers meanwhile.
+// Operand layout of the 16x16x4 f64 product: lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16]; lane l receives
+// D[l / 16 + 4 i][l % 16] in element i.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <int RB>
+__global__ __launch_bounds__(64 * RB) void k_gram_mfma(const c64* __restrict__ a, const c64* __restrict__ b,
+                                                       long long N, int m, int nchunk, c64* __restrict__ part) {
+  constexpr int P = 16;
+  constexpr int M = 16 * RB;
+  constexpr int NT = 64 * RB;
+  constexpr int SL = M * P / NT;   // loader slots per thread: 4
+  __shared__ c64 sa[M][P + 1], sb[M][P + 1];
+  const int c = blockIdx.x;
+  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int lr = lane % 16, lk = lane / 16;
+  long long lo, hi;
+  chunk_range(N, nchunk, c, lo, hi);
+  c64 va[SL], vb[SL];
+  auto load = [&](long long r0) {
+#pragma unroll
+    for (int q = 0; q < SL; ++q) {
+      const int slot = (int)threadIdx.x + NT * q;
+      const int o = slot / P, p = slot % P;
+      const long long r = min(r0 + p, hi - 1);
+      const int oc = min(o, m - 1);
+      va[q] = a[(long long)oc * N + r];
+      vb[q] = b[(long long)oc * N + r];
+    }
+  };
+  f64x4 gr[RB], gi[RB];
+#pragma unroll
+  for (int v = 0; v < RB; ++v)
+    for (int i = 0; i < 4; ++i) gr[v][i] = gi[v][i] = 0.0;
+  if (lo < hi) load(lo);
+  for (long long r0 = lo; r0 < hi; r0 += P) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < SL; ++q) {
+      const int slot = (int)threadIdx.x + NT * q;
+      const int o = slot / P, p = slot % P;
+      const bool ok = o < m && r0 + p < hi;
+      sa[o][p] = ok ? va[q] : mk(0.0, 0.0);
+      sb[o][p] = ok ? vb[q] : mk(0.0, 0.0);
+    }
+    __syncthreads();
+    if (r0 + P < hi) load(r0 + P);
+#pragma unroll
+    for (int kk = 0; kk < P / 4; ++kk) {
+      const c64 x = sa[16 * w + lr][4 * kk + lk];
+#pragma unroll
+      for (int v = 0; v < RB; ++v) {
+        const c64 y = sb[16 * v + lr][4 * kk + lk];
+        gr[v] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.x, gr[v], 0, 0, 0);
+        gr[v] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, y.y, gr[v], 0, 0, 0);
+        gi[v] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.y, gi[v], 0, 0, 0);
+        gi[v] = __builtin_amdgcn_mfma_f64_16x16x4f64(-x.y, y.x, gi[v], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < RB; ++v)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int i = 16 * w + lk + 4 * e, j = 16 * v + lr;
+      if (i < m && j < m) part[((long long)c * m + i) * m + j] = mk(gr[v][e], gi[v][e]);
+    }
+}
+
 // g[q] = dv * sum over chunks c of part[c * nb * nb + q].  A workgroup takes
 // kGfQ consecutive entries with kGfL lanes each: lane l sums chunks l,
 // l + kGfL, ... (the kGfQ entries of one chunk are adjacent in memory), and
@@ -1192,7 +1269,16 @@ hipError_t gram(hipStream_t st, const cx<R>* a, const cx<R>* b, long long N, int
     return hipGetLastError();
   }
   const dim3 grid(kGramBlockChunks);
-  if (t == 1)
+  if constexpr (std::is_same<R, double>::value) {
+    if (t == 1)
+      hipLaunchKernelGGL((k_gram_mfma<1>), grid, dim3(64), 0, st, a, b, N, nb, kGramBlockChunks, part);
+    else if (t == 2)
+      hipLaunchKernelGGL((k_gram_mfma<2>), grid, dim3(128), 0, st, a, b, N, nb, kGramBlockChunks, part);
+    else if (t == 3)
+      hipLaunchKernelGGL((k_gram_mfma<3>), grid, dim3(192), 0, st, a, b, N, nb, kGramBlockChunks, part);
+    else
+      hipLaunchKernelGGL((k_gram_mfma<4>), grid, dim3(256), 0, st, a, b, N, nb, kGramBlockChunks, part);
+  } else if (t == 1)
     hipLaunchKernelGGL((k_gram_block<R, 1>), grid, dim3(256), 0, st, a, b, N, nb, kGramBlockChunks, part);
   else if (t == 2)
     hipLaunchKernelGGL((k_gram_block<R, 2>), grid, dim3(256), 0, st, a, b, N, nb, kGramBlockChunks, part);

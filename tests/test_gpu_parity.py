@@ -231,6 +231,29 @@ def test_orthog_and_diagh_match_oracle(per):
     g.close()
 
 
+@pytest.mark.parametrize("nb", [13, 24, 40, 64])
+def test_orthog_many_orbitals_matches_oracle(nb):
+    """orthog_phi on 13 ... 64 FP64 orbitals: the Gram matrices of 1 to 4
+    16-wide tiles (ragged ones included) on the matrix cores, over point
+    chunks of 22-23 points; eigenvalues of the overlap as the oracle's,
+    orthonormal output spanning the oracle's subspace"""
+    spec = synth.hpsi_system(n=(24, 24, 20), d=0.5, natom=4, seed=8)
+    st = state_from_spec(spec, nsys=2)
+    _random_potentials(st, spec, 2)
+    raw = synth.random_orbitals(spec, nb, seed=nb)
+    torch, g = _gpu(spec, st, nb)
+    g.write_orbitals(0, raw)
+    eig = g.orthog_phi(0, 1)
+    ref, eig_ref, _ = O.orthog_phi(st, raw)
+    assert rel(eig, eig_ref) < 1e-12
+    out = g.read_orbitals(0)
+    G = (out.conj() @ out.T) * spec.dv
+    assert np.max(np.abs(G - np.eye(nb))) < 1e-12
+    M = (ref.conj() @ out.T) * spec.dv
+    assert np.max(np.abs(M.conj().T @ M - np.eye(nb))) < 1e-10
+    g.close()
+
+
 def test_mixed_precision_operators(mol):
     spec, st, phi = mol
     torch, g = _gpu(spec, st, phi.shape[0], prec="mixed")
