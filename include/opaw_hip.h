@@ -322,7 +322,9 @@ int opaw_residuals(opaw_ctx* ctx, int sys, int h_type, const double* eig, double
  * first nconv bands (0: all) are <= tol, or after maxit.
  * eig (nb) out; hist (maxit) the largest residual per iteration (may be
  * NULL); iters the iterations done.  record > 0 keeps (phi, r) every
- * record-th iteration as training material. */
+ * record-th iteration as training material.  The context keeps a device
+ * work space of 13 nb N complex FP64 values for it (two [phi, w, p] blocks,
+ * their H images, one k-space block). */
 int opaw_block_solve(opaw_ctx* ctx, int sys, int h_type, int maxit, double tol, int nconv, const double* mlp, int nh,
                      int record, double* eig, double* hist, int* iters);
 /* MLP of the preconditioner: 2 -> nh -> nh -> 1 with tanh hidden layers;
