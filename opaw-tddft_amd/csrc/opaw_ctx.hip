@@ -259,6 +259,8 @@ struct opaw_ctx {
   // preconditioner table
   DevBuf shPerm, shOff, shEk;
   DevBuf shOf;        // shell of each k point
+  void* pin = nullptr;   // pinned host staging of the block solver's small transfers
+  size_t pin_bytes = 0;
   DevBuf gsTab;       // preconditioner multipliers per (band, shell)
   int nsh = 0;
 
@@ -447,6 +449,7 @@ int opaw_destroy(opaw_ctx* c) {
   if (c->bst) (void)hipStreamDestroy(c->bst);
   for (void* p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
+  if (c->pin) (void)hipHostFree(c->pin);
   delete c;
   return OPAW_OK;
 }
@@ -2431,6 +2434,21 @@ int opaw_residuals(opaw_ctx* c, int sys, int h_type, const double* eig, double* 
 // first nconv bands are <= tol; hist[it] = their largest at iteration it.
 static int ensure_shells(opaw_ctx* c);
 
+// pinned host memory (hipHostMalloc) for small transfers, grown on demand
+// and freed with the context
+static int ensure_pinned(opaw_ctx* c, size_t bytes) {
+  if (c->pin_bytes >= bytes) return OPAW_OK;
+  if (c->pin) {
+    HCHK(hipStreamSynchronize(c->st));
+    HCHK(hipHostFree(c->pin));
+    c->pin = nullptr;
+    c->pin_bytes = 0;
+  }
+  HCHK(hipHostMalloc(&c->pin, bytes, hipHostMallocDefault));
+  c->pin_bytes = bytes;
+  return OPAW_OK;
+}
+
 int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, int nconv, const double* mlp, int nh,
                      int record, double* eig, double* hist, int* iters) {
   CTX_CHECK(c);
@@ -2470,6 +2488,20 @@ int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, in
   double* dy = dt + nb;
   RCHK(ensure_shells(c));
   RCHK(ensure(c, c->gsTab, (size_t)nb * c->nsh * sizeof(double)));
+  // pinned staging, one area per transfer: the partial sums (down), the
+  // Gram matrices S and H (down), e then t, y (up), the coefficients C and
+  // Cp (up).  Each area is rewritten only after a stream synchronisation
+  // that follows its previous transfer.
+  const size_t mmax = (size_t)3 * nb;
+  const size_t pin_h = (size_t)kGsChunks * nb * 3 * sizeof(double);
+  const size_t pin_sh = 2 * mmax * mmax * sizeof(c64);
+  const size_t pin_ety = (3 * (size_t)nb * sizeof(double) + 15) / 16 * 16;   // keeps pc 16-byte aligned
+  const size_t pin_c = (mmax + (mmax - nb)) * nb * sizeof(c64);
+  RCHK(ensure_pinned(c, pin_h + pin_sh + pin_ety + pin_c));
+  double* ph = (double*)c->pin;
+  c64* psh = (c64*)((char*)c->pin + pin_h);
+  double* pety = (double*)((char*)psh + pin_sh);
+  c64* pc = (c64*)((char*)pety + pin_ety);
   HCHK(hipMemcpyAsync(Y, phi, fb, hipMemcpyDeviceToDevice, c->st));
   RCHK(apply_h(c, sys, h_type, Y, HY, nb));
   std::vector<double> e(nb), t(nb), y(nb);
@@ -2484,17 +2516,18 @@ int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, in
   // one partial-sum buffer for the residual norms and the kinetic sums
   double* part = gs_part(c, (size_t)kGsChunks * nb * 3, &rc);
   RCHK(rc);
-  std::vector<double> h((size_t)kGsChunks * nb * 3);
+  const double* h = ph;
   int it = 0;
   for (; it < maxit; ++it) {
-    HCHK(hipMemcpyAsync(deig, e.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->st));
+    std::copy(e.begin(), e.end(), pety);
+    HCHK(hipMemcpyAsync(deig, pety, nb * sizeof(double), hipMemcpyHostToDevice, c->st));
     c64* W = Y + off;
     HCHK(residual_vectors(c->st, HY, Y, deig, N, nb, W));
     HCHK(residual_partials<double>(c->st, HY, Y, deig, N, nb, part));
     // band kinetic energies from the same download
     HCHK(fft3d<double>(c->g, c->st, -1, Y, Kb, nb, twx, twy, twz));
     HCHK(kinetic_partials(c->st, Kb, c->ek, N, nb, part + (size_t)kGsChunks * nb));
-    HCHK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HCHK(hipMemcpyAsync(ph, part, pin_h, hipMemcpyDeviceToHost, c->st));
     HCHK(hipStreamSynchronize(c->st));
     double worst = 0.0;
     for (int i = 0; i < nconv; ++i) {
@@ -2504,7 +2537,7 @@ int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, in
     }
     if (hist) hist[it] = worst;
     if (worst <= tol) break;
-    const double* kp = h.data() + (size_t)kGsChunks * nb;
+    const double* kp = h + (size_t)kGsChunks * nb;
     for (int i = 0; i < nb; ++i) {
       double a = 0.0, b = 0.0;
       for (int ch = 0; ch < kGsChunks; ++ch) {
@@ -2515,8 +2548,9 @@ int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, in
       if (!(t[i] > 1e-6)) t[i] = 1e-6;
       y[i] = e[i] / t[i];
     }
-    HCHK(hipMemcpyAsync(dt, t.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->st));
-    HCHK(hipMemcpyAsync(dy, y.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->st));
+    std::copy(t.begin(), t.end(), pety + nb);
+    std::copy(y.begin(), y.end(), pety + 2 * nb);
+    HCHK(hipMemcpyAsync(dt, pety + nb, 2 * nb * sizeof(double), hipMemcpyHostToDevice, c->st));   // dy = dt + nb
     if (record > 0 && it % record == 0) {
       opaw_ctx::Snap sp;
       sp.sys = sys;
@@ -2542,10 +2576,10 @@ int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, in
     RCHK(ensure(c, c->gsM, (size_t)2 * m * m * sizeof(c64)));
     HCHK(gram<double>(c->st, Y, Y, N, m, dvol(c), (c64*)c->gsG.p, (c64*)c->gsM.p));
     HCHK(gram<double>(c->st, Y, HY, N, m, dvol(c), (c64*)c->gsG.p, (c64*)c->gsM.p + (size_t)m * m));
-    HCHK(hipMemcpyAsync(S.data(), c->gsM.p, (size_t)m * m * sizeof(c64), hipMemcpyDeviceToHost, c->st));
-    HCHK(hipMemcpyAsync(H.data(), (c64*)c->gsM.p + (size_t)m * m, (size_t)m * m * sizeof(c64),
-                        hipMemcpyDeviceToHost, c->st));
+    HCHK(hipMemcpyAsync(psh, c->gsM.p, 2 * (size_t)m * m * sizeof(c64), hipMemcpyDeviceToHost, c->st));
     HCHK(hipStreamSynchronize(c->st));
+    std::memcpy((void*)S.data(), psh, (size_t)m * m * sizeof(c64));
+    std::memcpy((void*)H.data(), psh + (size_t)m * m, (size_t)m * m * sizeof(c64));
     for (int a = 0; a < m; ++a)
       for (int b = a + 1; b < m; ++b) {
         const std::complex<double> hs = 0.5 * (S[(size_t)a * m + b] + std::conj(S[(size_t)b * m + a]));
@@ -2561,8 +2595,9 @@ int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, in
     RCHK(ensure(c, c->gsSmallC, (size_t)(m + (m - nb)) * nb * sizeof(c64)));
     c64* dC = (c64*)c->gsSmallC.p;
     c64* dCp = dC + (size_t)m * nb;
-    HCHK(hipMemcpyAsync(dC, Ccol.data(), Ccol.size() * sizeof(c64), hipMemcpyHostToDevice, c->st));
-    HCHK(hipMemcpyAsync(dCp, Cp.data(), Cp.size() * sizeof(c64), hipMemcpyHostToDevice, c->st));
+    std::memcpy((void*)pc, Ccol.data(), Ccol.size() * sizeof(c64));
+    std::memcpy((void*)(pc + Ccol.size()), Cp.data(), Cp.size() * sizeof(c64));
+    HCHK(hipMemcpyAsync(dC, pc, (Ccol.size() + Cp.size()) * sizeof(c64), hipMemcpyHostToDevice, c->st));   // dCp follows dC
     HCHK(rotate_rect(c->st, Y, Yn, N, m, nb, dC));
     HCHK(rotate_rect(c->st, HY, HYn, N, m, nb, dC));
     HCHK(rotate_rect(c->st, Y + off, Yn + 2 * off, N, m - nb, nb, dCp));
