@@ -1,7 +1,8 @@
 """GPU: the spectrum kernel (opaw_spectrum) against the oracle and the
-closed form, and the accuracy of the mixed path end to end: over a 2000-step C10H8 64^3 propagation (the reference's naphthalene
-run length, dt = 0.05) the mixed dipole trace stays within 1e-4 (relative
-L2) of FP64 and the absorption peaks of the two move by <= 0.01 eV."""
+closed form, and the accuracy of the mixed path end to end: over a
+2000-step C10H8 64^3 propagation (the reference's naphthalene run length,
+dt = 0.05) the mixed dipole trace stays within 1e-4 (relative L2) of FP64
+and the absorption peaks of the two move by <= 0.01 eV."""
 import numpy as np
 import pytest
 
