@@ -76,3 +76,18 @@ def test_loss_gradient_matches_finite_differences(nh):
         Pm[k] -= h
         fd = (GO.precond_loss(pairs, ek, N, Pp, nh, False)[0] - GO.precond_loss(pairs, ek, N, Pm, nh, False)[0]) / (2 * h)
         assert abs(fd - g[k]) < 1e-7 * max(1.0, abs(fd)), k
+
+
+def test_scf_comparison_command_parses_without_a_gpu():
+    """python -m opaw_amd.gs: the systems it builds and its arguments (the
+    runs themselves need the GPU)"""
+    import os
+    import subprocess
+    import sys
+    from opaw_amd import gs
+    assert gs.SCF_SYSTEMS == {32: (1, 8, 6), 48: (2, 16, 12)}
+    out = subprocess.run([sys.executable, "-m", "opaw_amd.gs", "--help"], capture_output=True, text=True,
+                         timeout=120, check=True,
+                         cwd=os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "opaw-tddft_amd"))
+    for flag in ("--grid", "--methods", "--seeds", "--nh", "--steps"):
+        assert flag in out.stdout
