@@ -107,6 +107,33 @@ int opaw_set_sphere(opaw_ctx* ctx, int ia, int ngrid, const int* local_grid, con
 /* per-type qijlm(nl2,ms,ms) (src/2_paw.f90), and the on-site D_ij model that
  * stands in for libpaw pawdenpot/pawdij: dij0(ms,ms) + eijkl(ms,ms,ms,ms).rho */
 int opaw_set_type(opaw_ctx* ctx, int it, const double* qijlm, const double* dij0, const double* eijkl);
+/* libpaw's on-site tables of type it, which replace the model's on-site part
+ * by what get_dij (src/get_dij.f90:14-30) computes through pawdenpot
+ * (src/libpaw/m_paw_denpot.F90:136) and pawdij
+ * (src/libpaw/4_m_pawdij.F90:168) for LDA with pawxcdev = 1: with them
+ * D_ij = dij0 + dijhartree + dijxc + dijhat, where dij0 and eijkl
+ * (dijhartree) come from opaw_set_type (libpaw's pawtab%dij0 and eijkl,
+ * unpacked to (ms,ms) and (ms,ms,ms,ms)), dijxc is computed on the device at
+ * every rebuild from rho_ij, and dijhat uses the atom's D^hat sphere
+ * (opaw_set_dij_sphere) with qijl instead of the compensation sphere.
+ * Radial tables on nr points of the dataset's mesh (pawtab%mesh_size):
+ * rad(nr) with rad(1) = 0, w(nr) the simp_gen weights (zero beyond
+ * int_meshsz), phiphj / tphitphj (nr, nij) the products phi_i phi_j of the
+ * radial pairs, coredens / tcoredens (nr) the core densities (tcoredens
+ * read only when usetcore), gnt (lm_size, ms, ms) the real Gaunt
+ * coefficient of (lm, lm_i, lm_j), kln (ms, ms) the 0-based radial pair of
+ * (i, j), qijl (lm_size, ms, ms) pawtab%qijl.  lm_size <= 9 and nij <= 10
+ * (partial waves with l <= 1, up to four of them), else
+ * OPAW_ERR_UNSUPPORTED. */
+int opaw_set_onsite(opaw_ctx* ctx, int it, int nr, int lm_size, int nij, int usetcore, const double* rad,
+                    const double* w, const double* phiphj, const double* tphitphj, const double* coredens,
+                    const double* tcoredens, const double* gnt, const int* kln, const double* qijl);
+/* pawfgrtab of atom ia for pawdijhat (nhatgrid with optcut = 0,
+ * src/libpaw/finegrid.f90; pawgylm, src/libpaw/2_m_paw_finegrid.F90:92):
+ * the nfgd grid points within rshp, ifftsph(nfgd) 1-based flat indices
+ * (x fastest), and gylm(nfgd, lm_size) = g_l(r) S_lm.  The atom's type must
+ * have on-site tables. */
+int opaw_set_dij_sphere(opaw_ctx* ctx, int ia, int nfgd, const int* ifftsph, const double* gylm);
 /* builds the plane lists and uploads everything; call after the set_* above */
 int opaw_finalize_tables(opaw_ctx* ctx);
 

@@ -73,6 +73,11 @@ struct AtomDesc {
   int toff_d0;     // offset of the type's dij0 table (ms*ms)
   int toff_e;      // offset of the type's eijkl table (ms^4)
   int boff;        // offset of this atom's box points in the per-box-point tables
+  int ot;          // on-site (libpaw) table set of the type, -1: none (model on-site D_ij)
+  // D^hat sphere: points [dh_off, dh_off + dh_n) of the D^hat point arrays,
+  // their g values (dh_nl moments, moment-major) from dh_goff; the type's
+  // q_ij^L table (dh_nl, ms, ms) at toff_qh
+  int dh_off, dh_n, dh_nl, dh_goff, toff_qh;
 };
 
 constexpr int kMaxMs = 16;

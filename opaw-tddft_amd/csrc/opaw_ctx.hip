@@ -16,6 +16,7 @@
 #include "opaw_gs.h"
 #include "opaw_hip.h"
 #include "opaw_linalg.h"
+#include "opaw_onsite.h"
 #include "opaw_paw.h"
 
 using namespace opaw;
@@ -137,6 +138,7 @@ struct opaw_ctx {
   // stage (pot_pending), whose forward passes wait only for corrA (ev_corr);
   // ev_nhat hands the main stream's rho_ij / nhat to that chain
   hipEvent_t ev_corr = nullptr, ev_nhat = nullptr;
+  hipEvent_t ev_onsite = nullptr;   // the on-site XC kernel (st2) has ended
   hipEvent_t ev_mid = nullptr;   // the forward kinetic passes (st2) have ended
   bool pot_pending = false;
   double* mpart = nullptr;   // partials of the periodic potential means (side stream)
@@ -195,6 +197,26 @@ struct opaw_ctx {
   int* sph_idx = nullptr;
   double* sph_g = nullptr;
   double *qtab = nullptr, *d0tab = nullptr, *etab = nullptr;
+  // libpaw on-site tables per type (opaw_set_onsite; nr == 0: the type keeps
+  // the model on-site D_ij) and libpaw D^hat spheres per atom
+  // (opaw_set_dij_sphere; empty: D^hat over the compensation sphere)
+  struct OnsiteHost {
+    int nr = 0, lm = 0, nij = 0, usetcore = 0;
+    std::vector<double> rad, w, inv2, pp, tpp, core, tcore, gnt, qd;
+    std::vector<int> kln;
+  };
+  std::vector<OnsiteHost> hon;
+  std::vector<std::vector<int>> hdh_idx;
+  std::vector<std::vector<double>> hdh_g;
+  OnsiteType* otypes = nullptr;
+  double* otab = nullptr;
+  int* oint = nullptr;
+  int* dh_idx = nullptr;
+  double* dh_g = nullptr;
+  double* dxc = nullptr;     // nsys x mtot: Dij^xc of the last rebuild
+  double* oscr = nullptr;    // on-site kernel scratch
+  int onr_max = 0;
+  bool any_onsite = false;
 
   // potentials
   double *vloc = nullptr, *ncoret = nullptr;
@@ -426,6 +448,8 @@ int opaw_create(opaw_ctx** out, int nx, int ny, int nz, double dx, double dy, do
                        hipEventCreateWithFlags(&c->ev_nhat, hipEventDisableTiming | hipEventReleaseToDevice) !=
                            hipSuccess ||
                        hipEventCreateWithFlags(&c->ev_mid, hipEventDisableTiming | hipEventReleaseToDevice) !=
+                           hipSuccess ||
+                       hipEventCreateWithFlags(&c->ev_onsite, hipEventDisableTiming | hipEventReleaseToDevice) !=
                            hipSuccess))
     r = fail(OPAW_ERR_HIP, "side stream / events");
   if (r != OPAW_OK) {
@@ -444,6 +468,7 @@ int opaw_destroy(opaw_ctx* c) {
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->ev_corr) (void)hipEventDestroy(c->ev_corr);
   if (c->ev_nhat) (void)hipEventDestroy(c->ev_nhat);
+  if (c->ev_onsite) (void)hipEventDestroy(c->ev_onsite);
   if (c->ev_mid) (void)hipEventDestroy(c->ev_mid);
   if (c->st2) (void)hipStreamDestroy(c->st2);
   if (c->bst) (void)hipStreamDestroy(c->bst);
@@ -633,6 +658,9 @@ int opaw_set_atoms(opaw_ctx* c, int natom, const int* atom_map, const int* ir_st
   c->hq.assign(ntype, {});
   c->hd0.assign(ntype, {});
   c->he.assign(ntype, {});
+  c->hon.assign(ntype, {});
+  c->hdh_idx.assign(natom, {});
+  c->hdh_g.assign(natom, {});
   c->tables_ready = false;
   return OPAW_OK;
 }
@@ -692,6 +720,65 @@ int opaw_set_type(opaw_ctx* c, int it, const double* qijlm, const double* dij0, 
   return OPAW_OK;
 }
 
+int opaw_set_onsite(opaw_ctx* c, int it, int nr, int lm_size, int nij, int usetcore, const double* rad,
+                    const double* w, const double* phiphj, const double* tphitphj, const double* coredens,
+                    const double* tcoredens, const double* gnt, const int* kln, const double* qijl) {
+  CTX_CHECK(c);
+  const int t = it - 1;
+  if (t < 0 || t >= c->ntype) return fail(OPAW_ERR_ARG, "type index %d outside 1..%d", it, c->ntype);
+  if (nr < 4) return fail(OPAW_ERR_ARG, "on-site mesh of %d points", nr);
+  if (lm_size < 1 || lm_size > kOnsiteMaxLm)
+    return fail(OPAW_ERR_UNSUPPORTED, "on-site moments %d outside 1..%d (partial waves with l <= 1)", lm_size,
+                kOnsiteMaxLm);
+  if (nij < 1 || nij > kOnsiteMaxNij)
+    return fail(OPAW_ERR_UNSUPPORTED, "radial pairs %d outside 1..%d", nij, kOnsiteMaxNij);
+  if (!rad || !w || !phiphj || !tphitphj || !coredens || !gnt || !kln || !qijl || (usetcore && !tcoredens))
+    return fail(OPAW_ERR_ARG, "null on-site table");
+  const int ms = c->type_ms[t];
+  const size_t m2 = (size_t)ms * ms;
+  for (size_t q = 0; q < m2; ++q)
+    if (kln[q] < 0 || kln[q] >= nij) return fail(OPAW_ERR_ARG, "radial pair index %d outside 0..%d", kln[q], nij - 1);
+  if (rad[0] != 0.0) return fail(OPAW_ERR_ARG, "the on-site mesh must start at r = 0");
+  auto& h = c->hon[t];
+  h.nr = nr;
+  h.lm = lm_size;
+  h.nij = nij;
+  h.usetcore = usetcore ? 1 : 0;
+  h.rad.assign(rad, rad + nr);
+  h.w.assign(w, w + nr);
+  h.inv2.assign(nr, 0.0);
+  for (int r = 1; r < nr; ++r) h.inv2[r] = 1.0 / (rad[r] * rad[r]);
+  h.pp.assign(phiphj, phiphj + (size_t)nr * nij);
+  h.tpp.assign(tphitphj, tphitphj + (size_t)nr * nij);
+  h.core.assign(coredens, coredens + nr);
+  h.tcore.assign(nr, 0.0);
+  if (usetcore) h.tcore.assign(tcoredens, tcoredens + nr);
+  h.gnt.assign(gnt, gnt + (size_t)lm_size * m2);
+  h.kln.assign(kln, kln + m2);
+  h.qd.assign(qijl, qijl + (size_t)lm_size * m2);
+  c->tables_ready = false;
+  return OPAW_OK;
+}
+
+int opaw_set_dij_sphere(opaw_ctx* c, int ia, int nfgd, const int* ifftsph, const double* gylm) {
+  CTX_CHECK(c);
+  const int a = ia - 1;
+  if (a < 0 || a >= c->natom) return fail(OPAW_ERR_ARG, "atom index %d outside 1..%d", ia, c->natom);
+  const auto& h = c->hon[c->atom_type[a]];
+  if (h.nr == 0) return fail(OPAW_ERR_STATE, "the type of atom %d has no on-site tables (opaw_set_onsite)", ia);
+  if (nfgd < 0 || (nfgd > 0 && (!ifftsph || !gylm))) return fail(OPAW_ERR_ARG, "bad D^hat sphere arguments");
+  std::vector<int> idx(nfgd);
+  for (int p = 0; p < nfgd; ++p) {
+    const long long f = (long long)ifftsph[p] - 1;
+    if (f < 0 || f >= c->N) return fail(OPAW_ERR_ARG, "ifftsph point %d of atom %d outside the grid", p + 1, ia);
+    idx[p] = (int)f;
+  }
+  c->hdh_idx[a] = idx;
+  c->hdh_g[a].assign(gylm, gylm + (size_t)nfgd * h.lm);
+  c->tables_ready = false;
+  return OPAW_OK;
+}
+
 }  // extern "C"
 
 template <typename T>
@@ -717,7 +804,8 @@ int opaw_finalize_tables(opaw_ctx* c) {
                   (void*)c->sent, (void*)c->wl, (void*)c->bpt, (void*)c->bk,
                   (void*)c->pptr, (void*)c->pc, (void*)c->pv, (void*)c->pv1, (void*)c->ublk,
                   (void*)c->uzr, (void*)c->optr, (void*)c->ovoff, (void*)c->oc, (void*)c->ov,
-                  (void*)c->nhs, (void*)c->rpart})
+                  (void*)c->nhs, (void*)c->rpart, (void*)c->otypes, (void*)c->otab, (void*)c->oint,
+                  (void*)c->dh_idx, (void*)c->dh_g, (void*)c->dxc, (void*)c->oscr})
     dfree(c, p);
   c->prep_valid = false;
   c->coef_cap = 0;   // projection buffers depend on msp / chunk count
@@ -739,7 +827,46 @@ int opaw_finalize_tables(opaw_ctx* c) {
     d0.insert(d0.end(), c->hd0[t].begin(), c->hd0[t].end());
     e.insert(e.end(), c->he[t].begin(), c->he[t].end());
   }
+  // libpaw on-site tables: one OnsiteType per type that has them, their
+  // arrays concatenated; the D^hat q_ij^L of those types appended to qtab
+  std::vector<int> type_ot(c->ntype, -1), toff_qd(c->ntype, -1);
+  std::vector<OnsiteType> ots;
+  std::vector<double> ot_d;
+  std::vector<int> ot_i;
+  c->onr_max = 0;
+  for (int t = 0; t < c->ntype; ++t) {
+    const auto& h = c->hon[t];
+    if (h.nr == 0) continue;
+    OnsiteType T{};
+    T.nr = h.nr;
+    T.lm = h.lm;
+    T.nij = h.nij;
+    T.usetcore = h.usetcore;
+    auto put = [&](const std::vector<double>& v) {
+      const long long off = (long long)ot_d.size();
+      ot_d.insert(ot_d.end(), v.begin(), v.end());
+      return off;
+    };
+    T.rad = put(h.rad);
+    T.w = put(h.w);
+    T.inv2 = put(h.inv2);
+    T.pp = put(h.pp);
+    T.tpp = put(h.tpp);
+    T.core = put(h.core);
+    T.tcore = put(h.tcore);
+    T.gnt = put(h.gnt);
+    T.kln = (int)ot_i.size();
+    ot_i.insert(ot_i.end(), h.kln.begin(), h.kln.end());
+    type_ot[t] = (int)ots.size();
+    ots.push_back(T);
+    toff_qd[t] = (int)q.size();
+    q.insert(q.end(), h.qd.begin(), h.qd.end());
+    c->onr_max = std::max(c->onr_max, h.nr);
+  }
+  c->any_onsite = !ots.empty();
   c->hatoms.assign(natom, AtomDesc{});
+  std::vector<int> dhidx;
+  std::vector<double> dhg;
   std::vector<double> P, P1, s, sm1, smh, sph, sp1;
   std::vector<int> sidx;
   std::vector<double> sg;
@@ -766,6 +893,26 @@ int opaw_finalize_tables(opaw_ctx* c) {
     A.toff_q = toff_q[t];
     A.toff_d0 = toff_d0[t];
     A.toff_e = toff_e[t];
+    A.ot = type_ot[t];
+    // D^hat: libpaw's sphere and q_ij^L for a type with on-site tables
+    // (pawdijhat over pawfgrtab), else the compensation sphere and qijlm
+    A.dh_off = (int)dhidx.size();
+    A.dh_goff = (int)dhg.size();
+    if (A.ot >= 0) {
+      if (c->hdh_idx[a].empty() && c->hdh_g[a].empty())
+        return fail(OPAW_ERR_STATE, "D^hat sphere of atom %d not set (opaw_set_dij_sphere)", a + 1);
+      A.dh_n = (int)c->hdh_idx[a].size();
+      A.dh_nl = c->hon[t].lm;
+      A.toff_qh = toff_qd[t];
+      dhidx.insert(dhidx.end(), c->hdh_idx[a].begin(), c->hdh_idx[a].end());
+      dhg.insert(dhg.end(), c->hdh_g[a].begin(), c->hdh_g[a].end());
+    } else {
+      A.dh_n = (int)c->hsph_idx[a].size();
+      A.dh_nl = c->nl2;
+      A.toff_qh = toff_q[t];
+      dhidx.insert(dhidx.end(), c->hsph_idx[a].begin(), c->hsph_idx[a].end());
+      dhg.insert(dhg.end(), c->hsph_g[a].begin(), c->hsph_g[a].end());
+    }
     P.insert(P.end(), c->hp[a].begin(), c->hp[a].end());
     P1.insert(P1.end(), c->hp1[a].begin(), c->hp1[a].end());
     for (int i = 0; i < A.ms; ++i) {
@@ -798,6 +945,21 @@ int opaw_finalize_tables(opaw_ctx* c) {
   RCHK(upload(c, &c->qtab, q));
   RCHK(upload(c, &c->d0tab, d0));
   RCHK(upload(c, &c->etab, e));
+  RCHK(upload(c, &c->dh_idx, dhidx));
+  RCHK(upload(c, &c->dh_g, dhg));
+  c->otypes = nullptr;
+  c->otab = nullptr;
+  c->oint = nullptr;
+  c->dxc = nullptr;
+  c->oscr = nullptr;
+  if (c->any_onsite) {
+    RCHK(upload(c, &c->otypes, ots));
+    RCHK(upload(c, &c->otab, ot_d));
+    RCHK(upload(c, &c->oint, ot_i));
+    RCHK(dalloc_t(c, &c->dxc, (size_t)c->nsys * std::max(1, moff)));
+    RCHK(dalloc_t(c, &c->oscr, (size_t)std::max(1, natom) * c->nsys * 4 * c->onr_max));
+    HCHK(hipMemset(c->dxc, 0, (size_t)c->nsys * std::max(1, moff) * sizeof(double)));
+  }
   // projection chunks and coefficient map
   {
     std::vector<int2> chunks, achunk(std::max(1, natom)), coef;
@@ -1503,6 +1665,16 @@ static int flow_rebuild(opaw_ctx* c, int s0, int ns, const cx<R>* phi, bool pipe
   } else {
     RCHK(join_side(c));
   }
+  // on-site Dij^xc from rho_ij: on the side stream beside the Hartree chain
+  // when pipelined (joined before D_ij), else in order on st
+  if (c->any_onsite) {
+    const hipStream_t so = pipelined ? c->st2 : c->st;
+    prof_begin(so, KC_POT);
+    HCHK(onsite_xc(so, c->datoms, c->natom, c->otypes, c->otab, c->oint, c->rhoij + (size_t)s0 * c->mtot, c->mtot,
+                   ns, c->oscr, c->onr_max, c->dxc + (size_t)s0 * c->mtot));
+    prof_end(so, KC_POT);
+    if (pipelined) HCHK(hipEventRecord(c->ev_onsite, c->st2));
+  }
   prof_begin(sp, KC_POT);
   HCHK(renorm_pack(sp, c->dens + s0 * N, c->smap, c->nhs + (size_t)s0 * std::max(1, c->S), std::max(1, c->S),
                    dpart, nd, npart, nn, dv, c->rnel, 1, N, ns, c->pair, c->fac + s0));
@@ -1523,10 +1695,12 @@ static int flow_rebuild(opaw_ctx* c, int s0, int ns, const cx<R>* phi, bool pipe
   // pipelined: the D_ij kernel also forms the next stage's nonlocal weights
   // wB = D cB (cB of these orbitals, from the prep above)
   const DijWeights wt{(const c64*)c->cB.p, (c64*)c->wB.p, c->ctot, c->nsys * c->nb, c->nb, s0 * c->nb};
+  if (pipelined && c->any_onsite) HCHK(hipStreamWaitEvent(sp, c->ev_onsite, 0));
   prof_begin(sp, KC_POT);
-  HCHK(dij(sp, c->datoms, c->natom, c->rhoij + (size_t)s0 * c->mtot, c->mtot, c->qtab, c->d0tab, c->etab, c->nl2,
-           c->sph_idx, c->sph_g, c->vks + s0 * N, c->vxc + s0 * N, c->usexcnhat == 0 ? 1 : 0, N, ns, dv,
-           c->dij + (size_t)s0 * c->mtot, pipelined ? &wt : nullptr));
+  HCHK(dij(sp, c->datoms, c->natom, c->rhoij + (size_t)s0 * c->mtot, c->mtot, c->qtab, c->d0tab, c->etab,
+           c->dh_idx, c->dh_g, c->vks + s0 * N, c->vxc + s0 * N, c->usexcnhat == 0 ? 1 : 0, N, ns, dv,
+           c->dxc ? c->dxc + (size_t)s0 * c->mtot : nullptr, c->dij + (size_t)s0 * c->mtot,
+           pipelined ? &wt : nullptr));
   prof_end(sp, KC_POT);
   c->pot_pending = pipelined;
   c->prep_valid = (s0 == 0 && ns == c->nsys && (const void*)phi == c->phi);
@@ -1783,9 +1957,12 @@ int opaw_get_dij(opaw_ctx* c, int sys) {
   RCHK(check_sys(c, sys));
   if (!c->tables_ready) return fail(OPAW_ERR_STATE, "tables not finalised");
   const long long N = c->N;
-  HCHK(dij(c->st, c->datoms, c->natom, c->rhoij + (size_t)sys * c->mtot, c->mtot, c->qtab, c->d0tab, c->etab, c->nl2,
-           c->sph_idx, c->sph_g, c->vks + sys * N, c->vxc + sys * N, c->usexcnhat == 0 ? 1 : 0, N, 1,
-           c->dx * c->dy * c->dz, c->dij + (size_t)sys * c->mtot));
+  if (c->any_onsite)
+    HCHK(onsite_xc(c->st, c->datoms, c->natom, c->otypes, c->otab, c->oint, c->rhoij + (size_t)sys * c->mtot,
+                   c->mtot, 1, c->oscr, c->onr_max, c->dxc + (size_t)sys * c->mtot));
+  HCHK(dij(c->st, c->datoms, c->natom, c->rhoij + (size_t)sys * c->mtot, c->mtot, c->qtab, c->d0tab, c->etab,
+           c->dh_idx, c->dh_g, c->vks + sys * N, c->vxc + sys * N, c->usexcnhat == 0 ? 1 : 0, N, 1,
+           c->dx * c->dy * c->dz, c->dxc ? c->dxc + (size_t)sys * c->mtot : nullptr, c->dij + (size_t)sys * c->mtot));
   return OPAW_OK;
 }
 

@@ -78,10 +78,14 @@ struct DijWeights {
   c64* w;          // weights, coefficient-major w[c * nf + f]; null: none
   int ctot, nf, nb, f0;   // f0: first field of the rebuild's first system
 };
+// D^hat: the sphere of atom a is points [dh_off, dh_off + dh_n) of sph_idx /
+// sph_g (g stored per atom, moment-major) with dh_nl moments, and q_ij^L at
+// toff_qh of qtab; dxc (per system, stride mtot) is added for atoms with
+// on-site tables (ot >= 0)
 hipError_t dij(hipStream_t st, const AtomDesc* atoms, int natom, const double* rho, int mtot, const double* qtab,
-               const double* d0tab, const double* etab, int nl2, const int* sph_idx, const double* sph_g,
+               const double* d0tab, const double* etab, const int* sph_idx, const double* sph_g,
                const double* vks, const double* vxc, int subtract_vxc, long long N, int nsys, double dv,
-               double* d, const DijWeights* wt = nullptr);
+               const double* dxc, double* d, const DijWeights* wt = nullptr);
 template <typename R>
 hipError_t dipole(hipStream_t st, const cx<R>* u, int nx, int ny, int nz, int nsys, int nb, double dx, double dy,
                   double dz, double* part, double* out);

@@ -440,22 +440,25 @@ __global__ void k_rhoij(const AtomDesc* __restrict__ atoms, const c64* __restric
   }
 }
 
-// D_ij (src/get_dij.f90:1-156): on-site model dij0 + eijkl . rho_sel plus the
-// grid term sum_L q_ij^L dv sum_sph (vks - vxc) g_L (pawdijhat,
-// src/libpaw/4_m_pawdij.F90:2227-2400), then transfer_dij's packed
-// upper-triangle symmetric fill and +-param_dij_max clip.  One workgroup per
-// (atom, system); rho_sel, qijlm and the sphere moments are staged in LDS so
-// the contractions run from LDS.
+// D_ij (src/get_dij.f90:1-156, pawdij src/libpaw/4_m_pawdij.F90:168-1198):
+// dij0 + dijhartree (eijkl . rho_sel, pawdijhartree) + dijxc (from the
+// on-site kernel, when the type has libpaw tables) + dijhat, the grid term
+// sum_L q_ij^L dv sum_sph (vks - vxc) g_L (pawdijhat,
+// src/libpaw/4_m_pawdij.F90:2227-2514) over the atom's D^hat sphere, then
+// transfer_dij's packed upper-triangle symmetric fill and +-param_dij_max
+// clip.  One workgroup per (atom, system); rho_sel, q_ij^L and the sphere
+// moments are staged in LDS so the contractions run from LDS.
 __global__ __launch_bounds__(kT) void k_dij(const AtomDesc* __restrict__ atoms, const double* __restrict__ rhoij,
                                              int mtot, const double* __restrict__ qtab,
                                              const double* __restrict__ d0tab, const double* __restrict__ etab,
-                                             int nl2, const int* __restrict__ sph_idx,
+                                             const int* __restrict__ sph_idx,
                                              const double* __restrict__ sph_g, const double* __restrict__ vks,
                                              const double* __restrict__ vxc, int subtract_vxc, long long N,
-                                             double dv, double* __restrict__ dij, DijWeights wt) {
+                                             double dv, const double* __restrict__ dxc, double* __restrict__ dij,
+                                             DijWeights wt) {
   const int a = blockIdx.x, s = blockIdx.y;
   const AtomDesc A = atoms[a];
-  const int ms = A.ms, m2 = ms * ms;
+  const int ms = A.ms, m2 = ms * ms, nl2 = A.dh_nl;
   __shared__ double red[kT / 64][kMaxNl2];
   __shared__ double prod[kMaxNl2];
   __shared__ double rho[kMaxMs * kMaxMs];
@@ -472,7 +475,7 @@ __global__ __launch_bounds__(kT) void k_dij(const AtomDesc* __restrict__ atoms, 
   for (int q0 = 0; q0 < nq; q0 += kT * 4) {
     double t[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) t[u] = qtab[A.toff_q + min(q0 + u * kT + (int)threadIdx.x, nq - 1)];
+    for (int u = 0; u < 4; ++u) t[u] = qtab[A.toff_qh + min(q0 + u * kT + (int)threadIdx.x, nq - 1)];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int q = q0 + u * kT + (int)threadIdx.x;
@@ -482,12 +485,12 @@ __global__ __launch_bounds__(kT) void k_dij(const AtomDesc* __restrict__ atoms, 
   double acc[kMaxNl2];
 #pragma unroll
   for (int L = 0; L < kMaxNl2; ++L) acc[L] = 0.0;
-  for (int p = threadIdx.x; p < A.sph_n; p += kT) {
-    const long long r = sph_idx[A.sph_off + p];
+  for (int p = threadIdx.x; p < A.dh_n; p += kT) {
+    const long long r = sph_idx[A.dh_off + p];
     double g[kMaxNl2];
 #pragma unroll
     for (int L = 0; L < kMaxNl2; ++L)
-      g[L] = sph_g[(long long)A.sph_off * nl2 + (long long)min(L, nl2 - 1) * A.sph_n + p];
+      g[L] = sph_g[(long long)A.dh_goff + (long long)min(L, nl2 - 1) * A.dh_n + p];
     const double vk = vks[(long long)s * N + r];
     const double vx = vxc[(long long)s * N + r];
     const double pot = subtract_vxc ? vk - vx : vk;
@@ -522,6 +525,7 @@ __global__ __launch_bounds__(kT) void k_dij(const AtomDesc* __restrict__ atoms, 
         if (kl0 + u < m2) d += ev[u] * rho[kl0 + u];
     }
     for (int L = 0; L < nl2; ++L) d += qq[L + nl2 * (i + ms * j)] * prod[L];
+    if (A.ot >= 0) d += dxc[(long long)s * mtot + A.moff + q];
     M[q] = d;
   }
   __syncthreads();
@@ -1232,12 +1236,12 @@ hipError_t rhoij(hipStream_t st, const AtomDesc* atoms, int natom, const c64* ca
 }
 
 hipError_t dij(hipStream_t st, const AtomDesc* atoms, int natom, const double* rho, int mtot, const double* qtab,
-               const double* d0tab, const double* etab, int nl2, const int* sph_idx, const double* sph_g,
+               const double* d0tab, const double* etab, const int* sph_idx, const double* sph_g,
                const double* vks, const double* vxc, int subtract_vxc, long long N, int nsys, double dv,
-               double* d, const DijWeights* wt) {
+               const double* dxc, double* d, const DijWeights* wt) {
   if (natom == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_dij, dim3(natom, nsys), dim3(kT), 0, st, atoms, rho, mtot, qtab, d0tab, etab, nl2, sph_idx,
-                     sph_g, vks, vxc, subtract_vxc, N, dv, d, wt ? *wt : DijWeights{});
+  hipLaunchKernelGGL(k_dij, dim3(natom, nsys), dim3(kT), 0, st, atoms, rho, mtot, qtab, d0tab, etab, sph_idx,
+                     sph_g, vks, vxc, subtract_vxc, N, dv, dxc, d, wt ? *wt : DijWeights{});
   return hipGetLastError();
 }
 

@@ -1,0 +1,264 @@
+// On-site exchange-correlation part of D_ij (libpaw pawdenpot + pawdij with
+// pawxcdev = 1), computed on the GPU at every Hamiltonian rebuild so the time
+// loop keeps D_ij on the device.
+//
+// Reference chain (src/get_dij.f90:14-30): set_rhoij selects |rho_ij| > 1e-10;
+// pawdensities (src/libpaw/m_paw_denpot.F90:1216-1574) builds the lm moments
+// n1_lm(r) = sum_ij rho_ij dltij G(lm, ij) phi_i phi_j / r^2 and the pseudo
+// ones from tphi; pawxcm (src/libpaw/2_m_pawxc.F90:3763-4513) expands the
+// LDA potential to first order in the non-spherical moments:
+//   v_00 = sqrt(4pi) v(n_s) + v''(n_s) sum_{lm>0} n_lm^2 / (2 sqrt(4pi)),
+//   v_lm = v'(n_s) n_lm,
+// with n_s = (n_00 / sqrt(4pi) + core) / 2 the spin-up spherical density
+// (clamped at 1e-14), v = xcpbe option 1 (ABINIT's ixc = 7 PW92,
+// src/libpaw/0_xc_m_xcpbe.F90:881-945) and v', v'' by finite differences at
+// n_s (1 +- 1e-4); pawdijxcm (src/libpaw/4_m_pawdij.F90:1959-2178)
+// integrates sum_lm G(lm, ij) int (v_lm phi_i phi_j - vt_lm tphi_i tphi_j)
+// with simp_gen's weights.
+//
+// One workgroup per (atom, system).  Each thread takes radial points in a
+// stride, evaluates the moments and the six LDA potentials there and adds
+// its share of every (pair, lm) integral in registers; the integrals are
+// reduced in a fixed order (wave shuffles, then waves in order), so results
+// are bit-identical run to run.  The rare low-density interpolation of
+// pawxcm (points with n < xc_denpos bracketed by denser points) is applied
+// afterwards by one thread from the spherical values kept in scratch.
+#include "opaw_onsite.h"
+
+namespace opaw {
+
+namespace {
+
+constexpr int kOT = 512;
+constexpr int kNP = kOnsiteMaxNij * kOnsiteMaxLm;
+constexpr double kSqfpi = 3.5449077018110318;         // sqrt(4 pi)
+constexpr double kDelta = 1.0e-4;                      // pawxcm delta
+constexpr double kRhoMin = 1.0e-14;                    // pawxcm rho_min
+constexpr double kDenpos = 1.0e-14;                    // xc_denpos of src/get_dij.f90:20
+constexpr double kRhoijSel = 1.0e-10;                  // src/get_dij.f90:82
+// xcpbe (option 1) constants
+constexpr double kTwoM13 = 0.7937005259840998;         // 2^(-1/3)
+constexpr double kThreeFourthDivPi = 0.238732414637843;
+constexpr double kSixPi2_13 = 3.8977770897207535;      // (6 pi^2)^(1/3)
+constexpr double kRsfac = 0.6203504908994000;
+constexpr double kSqRsfac = 0.7876233178997433;
+constexpr double kSqRsfacInv = 1.269642451250142;
+constexpr double kA = 0.031091, kA1 = 0.21370, kB1 = 7.5957, kB2 = 3.5876, kB3 = 1.6382, kB4 = 0.49294;
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// xcpbe option 1, nspden = 1: potential of the spin-up density rho_up
+__device__ __forceinline__ double lda_v(double rho_up) {
+  const double rum13 = 1.0 / cbrt(rho_up);
+  const double rhotmot = kTwoM13 * rum13;
+  const double rhotot = 2.0 * rho_up;
+  const double rhotmo6 = sqrt(rhotmot);
+  const double rhoto6 = rhotot * rhotmot * rhotmot * rhotmo6;
+  const double ex = -kThreeFourthDivPi * kSixPi2_13 * rum13 * rum13 * rho_up;
+  const double rs = kRsfac * rhotmot;
+  const double sqr_rs = kSqRsfac * rhotmo6;
+  const double rsm1_2 = kSqRsfacInv * rhoto6;
+  const double q0 = -2.0 * kA * (1.0 + kA1 * rs);
+  const double q1 = 2.0 * kA * (kB1 * sqr_rs + kB2 * rs + kB3 * rs * sqr_rs + kB4 * rs * rs);
+  const double q1p = kA * (kB1 * rsm1_2 + 2.0 * kB2 + 3.0 * kB3 * sqr_rs + 4.0 * kB4 * rs);
+  const double den = 1.0 / (q1 * q1 + q1);
+  const double lg = -log(q1 * q1 * den);
+  const double ecrs = q0 * lg;
+  const double decrs = -2.0 * kA * kA1 * lg - q0 * q1p * den;
+  return (4.0 / 3.0) * ex + ecrs - rs * (1.0 / 3.0) * decrs;
+}
+
+// pawxcm potential moments at one point from the density moments n (lm < L)
+// and the core density; returns the total spherical density (rho_updn(r,1))
+__device__ __forceinline__ double xc_moments(const double (&n)[kOnsiteMaxLm], int L, double core,
+                                             double (&v)[kOnsiteMaxLm]) {
+  const double r0 = n[0] + kSqfpi * core;
+  double rs = r0 * (0.5 / kSqfpi);
+  if (rs < kDenpos) rs = kDenpos;                      // mkdenpos
+  const double vi = lda_v(rs);
+  const double v1 = lda_v((1.0 + kDelta) * rs);
+  const double v2 = lda_v((1.0 - kDelta) * rs);
+  const double rinv = rs > kRhoMin ? (0.5 / kDelta) / rs : 0.0;
+  const double d1 = (v1 - v2) * 0.5 * rinv;
+  const double d2 = (v1 + v2 - 2.0 * vi) * rinv * rinv;
+  double v1sum = 0.0;
+#pragma unroll
+  for (int lm = 1; lm < kOnsiteMaxLm; ++lm)
+    if (lm < L) v1sum += n[lm] * n[lm];
+  v[0] = vi * kSqfpi + v1sum * d2 * (0.5 / kSqfpi);
+#pragma unroll
+  for (int lm = 1; lm < kOnsiteMaxLm; ++lm) v[lm] = lm < L ? d1 * n[lm] : 0.0;
+  return r0;
+}
+
+// moments of the (pseudo) density at radial point r >= 1 from the pair
+// coefficients C (nij x L)
+__device__ __forceinline__ void moments_at(const double* __restrict__ C, const double* __restrict__ pp, int nr,
+                                           int P, int L, double inv2, int r, double (&n)[kOnsiteMaxLm]) {
+#pragma unroll
+  for (int lm = 0; lm < kOnsiteMaxLm; ++lm) n[lm] = 0.0;
+  for (int k = 0; k < P; ++k) {
+    const double f = pp[(long long)k * nr + r] * inv2;
+#pragma unroll
+    for (int lm = 0; lm < kOnsiteMaxLm; ++lm)
+      if (lm < L) n[lm] += C[k * kOnsiteMaxLm + lm] * f;
+  }
+}
+
+__device__ __forceinline__ void moments(const double* __restrict__ C, const double* __restrict__ pp,
+                                        const double* __restrict__ inv2, int nr, int P, int L, int r,
+                                        double (&n)[kOnsiteMaxLm]) {
+  if (r > 0) {
+    moments_at(C, pp, nr, P, L, inv2[r], r, n);
+    return;
+  }
+  // r = 0: pawrad_deducer0, f(1) = f(4) + 3 (f(2) - f(3))
+  double a[kOnsiteMaxLm], b[kOnsiteMaxLm], c[kOnsiteMaxLm];
+  moments_at(C, pp, nr, P, L, inv2[1], 1, a);
+  moments_at(C, pp, nr, P, L, inv2[2], 2, b);
+  moments_at(C, pp, nr, P, L, inv2[3], 3, c);
+#pragma unroll
+  for (int lm = 0; lm < kOnsiteMaxLm; ++lm) n[lm] = c[lm] + 3.0 * (a[lm] - b[lm]);
+}
+
+__global__ __launch_bounds__(kOT) void k_onsite_xc(const AtomDesc* __restrict__ atoms,
+                                                    const OnsiteType* __restrict__ ots,
+                                                    const double* __restrict__ otab, const int* __restrict__ oint,
+                                                    const double* __restrict__ rhoij, int mtot,
+                                                    double* __restrict__ scratch, int nr_max,
+                                                    double* __restrict__ dxc) {
+  const int a = blockIdx.x, s = blockIdx.y;
+  const AtomDesc A = atoms[a];
+  if (A.ot < 0) return;                                // uniform over the workgroup
+  const OnsiteType T = ots[A.ot];
+  const int ms = A.ms, m2 = ms * ms, L = T.lm, P = T.nij, nr = T.nr;
+  const double* gnt = otab + T.gnt;
+  const int* kln = oint + T.kln;
+  const double* pp = otab + T.pp;
+  const double* tpp = otab + T.tpp;
+  const double* inv2 = otab + T.inv2;
+  const double* w = otab + T.w;
+  const double* core = otab + T.core;
+  const double* tcore = otab + T.tcore;
+  __shared__ double rho[kMaxMs * kMaxMs];
+  __shared__ double C[kNP];
+  __shared__ double I[kNP];
+  __shared__ double red[kOT / 64][kNP];
+  __shared__ int flag;
+  const double* rg = rhoij + (long long)s * mtot + A.moff;
+  for (int q = threadIdx.x; q < m2; q += kOT) {
+    const double v = rg[q];
+    rho[q] = fabs(v) > kRhoijSel ? v : 0.0;            // set_rhoij selection
+  }
+  if (threadIdx.x == 0) flag = 0;
+  __syncthreads();
+  // pair coefficients C(k, lm) = sum over (i, j) with kln(i, j) = k of
+  // rho_ij G(lm, i, j): both triangles of rho, i.e. dltij times the packed
+  // element, as pawdensities accumulates it
+  for (int q = threadIdx.x; q < kNP; q += kOT) {
+    const int k = q / kOnsiteMaxLm, lm = q % kOnsiteMaxLm;
+    double acc = 0.0;
+    if (k < P && lm < L)
+      for (int ij = 0; ij < m2; ++ij)
+        if (kln[ij] == k) acc += rho[ij] * gnt[(long long)lm * m2 + ij];
+    C[q] = acc;
+  }
+  __syncthreads();
+  double acc[kNP];
+#pragma unroll
+  for (int q = 0; q < kNP; ++q) acc[q] = 0.0;
+  double* sc = scratch + (long long)(s * gridDim.x + a) * 4 * nr_max;
+  const double tolr = kDenpos * (1.0 + 1e-6);
+  bool low = false;
+  for (int r = threadIdx.x; r < nr; r += kOT) {
+    double n[kOnsiteMaxLm], v[kOnsiteMaxLm], vt[kOnsiteMaxLm];
+    moments(C, pp, inv2, nr, P, L, r, n);
+    const double r0 = xc_moments(n, L, core[r], v);
+    moments(C, tpp, inv2, nr, P, L, r, n);
+    const double r0t = xc_moments(n, L, T.usetcore ? tcore[r] : 0.0, vt);
+    sc[r] = r0;
+    sc[nr_max + r] = v[0];
+    sc[2 * nr_max + r] = r0t;
+    sc[3 * nr_max + r] = vt[0];
+    low = low || r0 < tolr || r0t < tolr;
+    const double wr = w[r];
+#pragma unroll
+    for (int k = 0; k < kOnsiteMaxNij; ++k) {
+      if (k < P) {
+        const double f = wr * pp[(long long)k * nr + r];
+        const double ft = wr * tpp[(long long)k * nr + r];
+#pragma unroll
+        for (int lm = 0; lm < kOnsiteMaxLm; ++lm) acc[k * kOnsiteMaxLm + lm] += v[lm] * f - vt[lm] * ft;
+      }
+    }
+  }
+  if (low) flag = 1;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < kNP; ++q) {
+    const double t = wsum(acc[q]);
+    if (lane == 0) red[wv][q] = t;
+  }
+  __threadfence_block();
+  __syncthreads();
+  for (int q = threadIdx.x; q < kNP; q += kOT) {
+    double t = 0.0;
+    for (int k = 0; k < kOT / 64; ++k) t += red[k][q];
+    I[q] = t;
+  }
+  __syncthreads();
+  if (flag && threadIdx.x == 0) {
+    // pawxcm's interpolation of v_00 across points with rho < tol_rho that
+    // lie between denser points: the correction of those points' share of
+    // the l = 0 integrals, for the all-electron (h = 0) and pseudo (h = 1)
+    // moments
+    const double* rad = otab + T.rad;
+    for (int h = 0; h < 2; ++h) {
+      const double* r0v = sc + 2 * h * nr_max;
+      const double* v0 = sc + (2 * h + 1) * nr_max;
+      const double sign = h == 0 ? 1.0 : -1.0;
+      const double* pw = h == 0 ? pp : tpp;
+      int ir1 = 0, ir2 = 0;
+      for (int ir = 1; ir <= nr; ++ir) {
+        if (r0v[ir - 1] < tolr) {
+          if (ir1 == 0) ir1 = ir - 1;
+          ir2 = ir + 1;
+        } else if (ir1 > 0) {
+          if (ir1 > 1 || ir2 < nr) {
+            const double fct = (v0[ir2 - 1] - v0[ir1 - 1]) / (rad[ir2 - 1] - rad[ir1 - 1]);
+            for (int jr = ir1 + 1; jr < ir2; ++jr) {
+              const double vn = v0[ir1 - 1] + fct * (rad[jr - 1] - rad[ir1 - 1]);
+              const double dvv = (vn - v0[jr - 1]) * w[jr - 1] * sign;
+              for (int k = 0; k < P; ++k) I[k * kOnsiteMaxLm] += dvv * pw[(long long)k * nr + jr - 1];
+            }
+          }
+          ir1 = ir2 = 0;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < m2; q += kOT) {
+    const int k = kln[q];
+    double d = 0.0;
+    for (int lm = 0; lm < L; ++lm) d += gnt[(long long)lm * m2 + q] * I[k * kOnsiteMaxLm + lm];
+    dxc[(long long)s * mtot + A.moff + q] = d;
+  }
+}
+
+}  // namespace
+
+hipError_t onsite_xc(hipStream_t st, const AtomDesc* atoms, int natom, const OnsiteType* ot, const double* otab,
+                     const int* oint, const double* rhoij, int mtot, int nsys, double* scratch, int nr_max,
+                     double* dxc) {
+  if (natom == 0 || !ot) return hipSuccess;
+  hipLaunchKernelGGL(k_onsite_xc, dim3(natom, nsys), dim3(kOT), 0, st, atoms, ot, otab, oint, rhoij, mtot, scratch,
+                     nr_max, dxc);
+  return hipGetLastError();
+}
+
+}  // namespace opaw

@@ -110,6 +110,24 @@ class OpawGPU:
             d0 = _f64(t.dij0.T)
             e = _f64(np.transpose(t.eijkl, (3, 2, 1, 0)))             # eijkl(ms,ms,ms,ms)
             lib.call("opaw_set_type", self.ctx, it, _dp(q), _dp(d0), _dp(e))
+        for it, t in enumerate(types, start=1):
+            o = getattr(t, "onsite", None)
+            if o is None:
+                continue
+            arrs = dict(rad=_f64(o["rad"]), w=_f64(o["w"]), pp=_f64(np.asarray(o["phiphj"]).T),
+                        tpp=_f64(np.asarray(o["tphitphj"]).T), core=_f64(o["coredens"]),
+                        tcore=_f64(o["tcoredens"]), gnt=_f64(np.transpose(o["gnt"], (0, 2, 1))),
+                        kln=_i32(np.asarray(o["kln"]).T), qd=_f64(np.transpose(o["qd"], (2, 1, 0))))
+            self._keep.append(arrs)
+            lib.call("opaw_set_onsite", self.ctx, it, int(o["nr"]), int(o["lm"]), int(o["nij"]), int(o["usetcore"]),
+                     _dp(arrs["rad"]), _dp(arrs["w"]), _dp(arrs["pp"]), _dp(arrs["tpp"]), _dp(arrs["core"]),
+                     _dp(arrs["tcore"]), _dp(arrs["gnt"]), _ip(arrs["kln"]), _dp(arrs["qd"]))
+        for ia, a in enumerate(s.atoms, start=1):
+            if getattr(a, "dsph_idx", None) is None:
+                continue
+            idx = _i32(np.asarray(a.dsph_idx) + 1)
+            g = _f64(np.asarray(a.dsph_g).T)                          # gylm(nfgd, lm)
+            lib.call("opaw_set_dij_sphere", self.ctx, ia, int(idx.size), _ip(idx), _dp(g))
         lib.call("opaw_finalize_tables", self.ctx)
         lib.call("opaw_set_potentials", self.ctx, _dp(_f64(s.vloc)), _dp(_f64(s.ncoret)), float(s.rnel),
                  int(usexcnhat))

@@ -41,6 +41,8 @@ SIGNATURES = {
     "opaw_set_projectors": (I, [P, I, DP, DP, DP, DP]),
     "opaw_set_sphere": (I, [P, I, I, IP, DP]),
     "opaw_set_type": (I, [P, I, DP, DP, DP]),
+    "opaw_set_onsite": (I, [P, I, I, I, I, I, DP, DP, DP, DP, DP, DP, DP, IP, DP]),
+    "opaw_set_dij_sphere": (I, [P, I, I, IP, DP]),
     "opaw_finalize_tables": (I, [P]),
     "opaw_set_potentials": (I, [P, DP, DP, D, I]),
     "opaw_vk_prep": (I, [P]),

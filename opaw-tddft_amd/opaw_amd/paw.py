@@ -721,32 +721,33 @@ def vloc_tuma(geom: Geometry, symbols: Sequence[str], coords: np.ndarray, datase
 def system_from_datasets(geom: Geometry, symbols: Sequence[str], coords: np.ndarray,
                          datasets: Dict[str, PawDataset], nocc: int, name: str = "paw",
                          dt: float = 0.05, sm: float = 1e-3, ipol: int = 0):
-    """The tables of prep_paw for every atom: box projectors and spheres on
-    the GPU, transform_p3d on the host, and for a nonperiodic grid the local
-    potential of vloc_tuma_prep on the GPU (vloc_tuma).  The pseudo-core
-    density and the on-site D_ij come from libpaw in the reference (atm2fft,
-    pawinit/pawdij: src/vloc.f90, src/get_dij.f90); here they are stand-ins,
-    marked as such in extra["standins"]: ncoret is pseudo_core_density *
-    Y_00 sampled at the grid points (nearest image; vloc likewise on a
-    periodic grid, where the reference uses atm2fft), dij0 is the dataset's
-    kinetic-energy differences and eijkl is zero."""
-    from . import synth
+    """The tables of prep_paw and prep_libpaw for every atom and type.
+
+    OPAW's own tables (prep_paw, src/prepare_paw.f90): box projectors and the
+    compensation spheres on the GPU, transform_p3d on the host, and for a
+    nonperiodic grid the local potential of vloc_tuma_prep on the GPU
+    (vloc_tuma).  libpaw's (prep_libpaw, src/prepare_libpaw.f90, restated in
+    opaw_amd.libpaw): dij0, eijkl and the radial tables of the on-site
+    D_ij, the D^hat spheres, and from atm2fft the pseudo-core density ncoret
+    and the starting density (extra["dens0"], the superposition of the
+    neutralised atomic pseudo valence densities, src/vloc.f90:24-33).  A
+    periodic grid keeps one stand-in, listed in extra["standins"]: v_loc
+    sampled from the radial tables, where the reference uses atm2fft's
+    local-potential branch."""
+    from . import libpaw, synth
     tsyms, amap = type_map(symbols)
     dv = geom.dx * geom.dy * geom.dz
     nl2 = max(datasets[s].nl2 for s in tsyms)
+    lp = {s: libpaw.load_type(datasets[s].filename) for s in tsyms}
     types = []
     for s in tsyms:
-        ds = datasets[s]
+        ds, t = datasets[s], lp[s]
         if ds.nl2 != nl2:
             raise ValueError("every type needs the same number of compensation channels")
-        K = kinetic_differences(ds)
-        ms = ds.ms
-        dij0 = np.zeros((ms, ms))
-        for i in range(ms):
-            for j in range(ms):
-                if ds.mstate[i] == ds.mstate[j] and ds.lstate[ds.ms_ls[i]] == ds.lstate[ds.ms_ls[j]]:
-                    dij0[i, j] = K[ds.ms_ls[i], ds.ms_ls[j]]
-        types.append(synth.TypeSpec(ms=ms, qijlm=ds.qijlm, dij0=dij0, eijkl=np.zeros((ms, ms, ms, ms))))
+        if t.lmn_size != ds.ms or any(t.lstate[ds.ms_ls[i]] != ds.lstate[ds.ms_ls[i]] for i in range(ds.ms)):
+            raise ValueError(f"{s}: libpaw's lmn order differs from readpaw's m-states")
+        types.append(synth.TypeSpec(ms=ds.ms, qijlm=ds.qijlm, dij0=t.unpack(t.dij0), eijkl=t.eijkl_full(),
+                                    onsite=libpaw.onsite_spec(t)))
     atoms = []
     for ia, (sym, c) in enumerate(zip(symbols, coords)):
         ds = datasets[sym]
@@ -755,36 +756,36 @@ def system_from_datasets(geom: Geometry, symbols: Sequence[str], coords: np.ndar
         p3d, p3d1 = box_projectors(ds, geom, c, irs)
         p1, s, sinv = transform_p3d(p3d1, ds.sij, dv)
         idx, g3d = sphere(ds, geom, c)
+        didx, dg = libpaw.dij_sphere(geom.nx, geom.ny, geom.nz, geom.dx, geom.dy, geom.dz, c, lp[sym])
         atoms.append(synth.AtomSpec(itype=int(amap[ia]), coord=np.asarray(c, dtype=np.float64), start=irs - 1,
-                                    nr=nrough, p=p3d, p1=p1, s=s, sinv=sinv, sph_idx=idx, sph_g=g3d))
-    h = geom.half
-    xs = [-h[k] + np.arange(n) * d for k, (n, d) in enumerate(((geom.nx, geom.dx), (geom.ny, geom.dy),
-                                                              (geom.nz, geom.dz)))]
-    Z, Y, X = np.meshgrid(xs[2], xs[1], xs[0], indexing="ij")
-    vloc = np.zeros(Z.shape)
-    ncore = np.zeros(Z.shape)
-    for sym, c in zip(symbols, coords):
-        ds = datasets[sym]
-        d2 = 0.0
-        for v, ck, hk in ((X, c[0], h[0]), (Y, c[1], h[1]), (Z, c[2], h[2])):
-            e = np.abs(v - ck)
-            e = np.minimum(e, np.abs(2.0 * hk - e))
-            d2 = d2 + e * e
-        r = np.sqrt(d2)
-        if geom.periodic:
-            vloc += np.interp(r, ds.rr, ds.vloc)
-        core = read_radial_block(ds.filename, "<pseudo_core_density", ds.nr) / SQ4PI
-        ncore += np.interp(r, ds.rr, core, right=0.0)
-    standins = ("ncoret", "dij0", "eijkl")
+                                    nr=nrough, p=p3d, p1=p1, s=s, sinv=sinv, sph_idx=idx, sph_g=g3d,
+                                    dsph_idx=didx, dsph_g=dg))
+    ncore, dens0 = libpaw.grid_densities(geom.nx, geom.ny, geom.nz, geom.dx, geom.dy, geom.dz,
+                                         np.asarray(coords, dtype=np.float64), amap, [lp[t] for t in tsyms])
+    standins: Tuple[str, ...] = ()
     if geom.periodic:
-        standins = ("vloc",) + standins
+        h = geom.half
+        xs = [-h[k] + np.arange(n) * d for k, (n, d) in enumerate(((geom.nx, geom.dx), (geom.ny, geom.dy),
+                                                                  (geom.nz, geom.dz)))]
+        Z, Y, X = np.meshgrid(xs[2], xs[1], xs[0], indexing="ij")
+        vloc = np.zeros(Z.shape)
+        for sym, c in zip(symbols, coords):
+            ds = datasets[sym]
+            d2 = 0.0
+            for v, ck, hk in ((X, c[0], h[0]), (Y, c[1], h[1]), (Z, c[2], h[2])):
+                e = np.abs(v - ck)
+                e = np.minimum(e, np.abs(2.0 * hk - e))
+                d2 = d2 + e * e
+            vloc += np.interp(np.sqrt(d2), ds.rr, ds.vloc)
+        vloc = vloc.reshape(-1)
+        standins = ("vloc",)
     else:
-        vloc = vloc_tuma(geom, symbols, coords, datasets).reshape(vloc.shape)
+        vloc = vloc_tuma(geom, symbols, coords, datasets)
     from .synth import ekcut_for
     return synth.SystemSpec(name=name, nx=geom.nx, ny=geom.ny, nz=geom.nz, dx=geom.dx, dy=geom.dy, dz=geom.dz,
                             periodic=geom.periodic, ekcut=ekcut_for(geom.dx, geom.dy, geom.dz), atoms=atoms,
-                            types=types, vloc=vloc.reshape(-1), ncoret=ncore.reshape(-1), rnel=2.0 * nocc,
+                            types=types, vloc=vloc, ncoret=ncore, rnel=2.0 * nocc,
                             nocc=nocc, dt=dt, sm=sm, ipol=ipol,
-                            extra=dict(standins=standins, type_symbols=tsyms,
+                            extra=dict(standins=standins, type_symbols=tsyms, dens0=dens0,
                                        natom_c=sum(1 for s in symbols if s == "C"),
                                        natom_h=sum(1 for s in symbols if s == "H"), N=geom.nx * geom.ny * geom.nz))

@@ -38,6 +38,9 @@ class AtomSpec:
     sinv: np.ndarray           # (ms,)
     sph_idx: np.ndarray        # (ngrid,) int64 flat grid index
     sph_g: np.ndarray          # (ngrid, nl2)
+    # libpaw's D^hat sphere (pawfgrtab): flat indices and g_l S_lm (nfgd, lm)
+    dsph_idx: Optional[np.ndarray] = None
+    dsph_g: Optional[np.ndarray] = None
 
     @property
     def ms(self):
@@ -50,6 +53,8 @@ class TypeSpec:
     qijlm: np.ndarray          # (nl2, ms, ms)
     dij0: np.ndarray           # (ms, ms)
     eijkl: np.ndarray          # (ms, ms, ms, ms)
+    # libpaw on-site tables (opaw_amd.libpaw.onsite_spec); None: model on-site D_ij
+    onsite: Optional[dict] = None
 
 
 @dataclass

@@ -16,8 +16,11 @@ def state_from_spec(spec, nsys=2, with_vk=True):
     g = grid_of(spec)
     atoms = [O.AtomBox(itype=a.itype, start=np.asarray(a.start), nr=np.asarray(a.nr), p=a.p, p1=a.p1,
                        s=np.asarray(a.s), sinv=np.asarray(a.sinv), sph_idx=np.asarray(a.sph_idx),
-                       sph_g=np.asarray(a.sph_g)) for a in spec.atoms]
-    types = [O.AtomType(ms=t.ms, qijlm=t.qijlm, dij0=t.dij0, eijkl=t.eijkl) for t in spec.types]
+                       sph_g=np.asarray(a.sph_g), dsph_idx=getattr(a, "dsph_idx", None),
+                       dsph_g=getattr(a, "dsph_g", None)) for a in spec.atoms]
+    types = [O.AtomType(ms=t.ms, qijlm=t.qijlm, dij0=t.dij0, eijkl=t.eijkl,
+                        onsite=(dict(getattr(t, "onsite")) if getattr(t, "onsite", None) is not None else None))
+             for t in spec.types]
     systems = []
     for _ in range(nsys):
         vks = spec.vks0.copy() if spec.vks0 is not None else np.zeros(g.n)

@@ -86,6 +86,9 @@ class AtomBox:
     # compensation-charge sphere (src/3_atom.f90:24-26): flat grid indices
     sph_idx: Optional[np.ndarray] = None   # (ngrid,) int64
     sph_g: Optional[np.ndarray] = None     # (ngrid, nl2) local_g3d
+    # libpaw's D^hat sphere (pawfgrtab: points within rshp, g_l S_lm)
+    dsph_idx: Optional[np.ndarray] = None
+    dsph_g: Optional[np.ndarray] = None
 
     @property
     def ms(self) -> int:
@@ -99,6 +102,7 @@ class AtomType:
     qijlm: np.ndarray            # (nl2, ms, ms)
     dij0: np.ndarray             # (ms, ms) on-site constant part
     eijkl: np.ndarray            # (ms, ms, ms, ms) on-site Hartree kernel
+    onsite: Optional[dict] = None  # libpaw on-site tables; None: the model below
 
 
 @dataclass
@@ -416,20 +420,25 @@ def get_pot_tddft_2(st: OpawState, dens: np.ndarray, nhat: np.ndarray):
 
 
 # ---------------------------------------------------------------------------
-# D_ij: src/get_dij.f90:1-156 with the grid part of libpaw pawdijhat
-# (src/libpaw/4_m_pawdij.F90:2227-2400) and an on-site model
-#   D_onsite = dij0 + sum_kl eijkl(i,j,k,l) rhoij_sel(k,l)
-# standing in for pawdenpot/pawdij's radial terms.
+# D_ij: src/get_dij.f90:1-156.  A type with libpaw tables runs pawdenpot +
+# pawdij (libpaw_oracle.onsite_dij: dij0 + dijhartree + dijxc + dijhat over
+# libpaw's own D^hat sphere).  A synthetic type keeps the model
+#   D = dij0 + sum_kl eijkl(i,j,k,l) rhoij_sel(k,l) + D^hat
+# with D^hat over the compensation sphere.
 def get_dij(st: OpawState, rhoij: List[np.ndarray], vks: np.ndarray, vxc: np.ndarray):
     g = st.grid
     pot = vks - vxc if st.usexcnhat == 0 else vks
     out = []
     for ia, a in enumerate(st.atoms):
         t = st.types[a.itype]
-        rho = np.where(np.abs(rhoij[ia]) > RHOIJ_SELECT, rhoij[ia], 0.0)
-        d = t.dij0 + np.einsum("ijkl,kl->ij", t.eijkl, rho)
-        prod = (pot[a.sph_idx] @ a.sph_g) * g.dv          # (nl2,)
-        d = d + np.einsum("lij,l->ij", t.qijlm, prod)
+        if t.onsite is not None:
+            from . import libpaw_oracle as LO
+            d = LO.onsite_dij(t.onsite, rhoij[ia], pot, a.dsph_idx, a.dsph_g, g.dv)
+        else:
+            rho = np.where(np.abs(rhoij[ia]) > RHOIJ_SELECT, rhoij[ia], 0.0)
+            d = t.dij0 + np.einsum("ijkl,kl->ij", t.eijkl, rho)
+            prod = (pot[a.sph_idx] @ a.sph_g) * g.dv          # (nl2,)
+            d = d + np.einsum("lij,l->ij", t.qijlm, prod)
         # transfer_dij: packed (row<=col) element, clip, symmetric fill
         up = np.triu(d)
         full = up + np.triu(d, 1).T
