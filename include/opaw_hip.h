@@ -104,8 +104,12 @@ int opaw_set_projectors(opaw_ctx* ctx, int ia, const double* local_p3d, const do
 /* compensation sphere: local_grid(ngrid,3) 1-based, local_g3d(ngrid,nl2)
  * (src/3_atom.f90:24-26, src/g3d.f90). */
 int opaw_set_sphere(opaw_ctx* ctx, int ia, int ngrid, const int* local_grid, const double* local_g3d);
-/* per-type qijlm(nl2,ms,ms) (src/2_paw.f90), and the on-site D_ij model that
- * stands in for libpaw pawdenpot/pawdij: dij0(ms,ms) + eijkl(ms,ms,ms,ms).rho */
+/* per-type qijlm(nl2,ms,ms) of the compensation charge (src/2_paw.f90), and
+ * the constant and Hartree parts of the on-site D_ij, dij0(ms,ms) +
+ * eijkl(ms,ms,ms,ms).rho_sel (libpaw's pawtab%dij0 and pawdijhartree,
+ * src/libpaw/4_m_pawdij.F90:1228-1313, unpacked).  A type without
+ * opaw_set_onsite tables keeps just these (with D^hat over the compensation
+ * sphere): the model of the synthetic workloads. */
 int opaw_set_type(opaw_ctx* ctx, int it, const double* qijlm, const double* dij0, const double* eijkl);
 /* libpaw's on-site tables of type it, which replace the model's on-site part
  * by what get_dij (src/get_dij.f90:14-30) computes through pawdenpot
@@ -330,6 +334,13 @@ int opaw_set_diis(opaw_ctx* ctx, int ndi, double mix);
  * rnel renormalisation, V_KS (get_pot, src/pot.f90:1-99) followed by
  * diis_general (src/diis.f90:12-153) when use_diis, then D_ij. */
 int opaw_ncpaw_make_hamiltonian(opaw_ctx* ctx, int sys, int use_diis);
+/* The first SCF iteration (src/main.f90:46-69): iscf = 1 skips update_dens,
+ * so the Hamiltonian comes from the starting density dens (nx*ny*nz host
+ * array; the reference's is atm2fft's superposition of atomic pseudo
+ * valence densities, src/vloc.f90:24-29) with rho_ij = 0 and nhat = 0
+ * (src/alloc_atom.f90:24, src/main.f90:46-52): get_pot (src/pot.f90) with
+ * DIIS when use_diis, then get_dij. */
+int opaw_ncpaw_make_hamiltonian_density(opaw_ctx* ctx, int sys, const double* dens, int use_diis);
 /* get_hminmax (src/hminmax.f90:18-108): two power iterations of niter steps
  * from random real starts (splitmix64 from seed), the interval widened by
  * fctr (1.2 in the reference). */

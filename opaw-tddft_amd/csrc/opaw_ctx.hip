@@ -2497,6 +2497,28 @@ int opaw_ncpaw_make_hamiltonian(opaw_ctx* c, int sys, int use_diis) {
   return OPAW_OK;
 }
 
+int opaw_ncpaw_make_hamiltonian_density(opaw_ctx* c, int sys, const double* dens, int use_diis) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  RCHK(need_f64(c));
+  RCHK(need_tables(c));
+  if (!dens) return fail(OPAW_ERR_ARG, "null density");
+  const long long N = c->N;
+  // iscf = 1: the starting density, rho_ij = 0 (so nhat = 0 and the on-site
+  // terms see the core densities only)
+  HCHK(hipMemcpyAsync(c->dens + sys * N, dens, N * sizeof(double), hipMemcpyHostToDevice, c->st));
+  HCHK(fill(c->st, c->rhoij + (size_t)sys * c->mtot, c->mtot, 0.0));
+  HCHK(fill(c->st, c->nhat + (size_t)sys * N, N, 0.0));
+  if (sys < (int)c->nhat_lazy.size()) c->nhat_lazy[sys] = 0;
+  RCHK(opaw_get_pot_tddft_2(c, sys));
+  if (use_diis && c->diis_L > 0) RCHK(diis_step(c, sys));
+  if (c->vksf) HCHK(to_f32(c->st, c->vks + (size_t)sys * N, c->vksf + (size_t)sys * N, N));
+  RCHK(opaw_get_dij(c, sys));
+  c->prep_valid = false;
+  HCHK(hipStreamSynchronize(c->st));
+  return OPAW_OK;
+}
+
 int opaw_hminmax(opaw_ctx* c, int sys, int h_type, int niter, double fctr, unsigned long long seed, double* hmin,
                  double* hmax) {
   CTX_CHECK(c);

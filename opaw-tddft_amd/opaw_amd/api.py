@@ -296,6 +296,10 @@ class OpawGPU:
     def ncpaw_make_hamiltonian(self, sys: int = 0, diis: bool = True):
         lib.call("opaw_ncpaw_make_hamiltonian", self.ctx, sys, int(diis))
 
+    def ncpaw_make_hamiltonian_density(self, dens: np.ndarray, sys: int = 0, diis: bool = True):
+        """SCF iteration 1: the Hamiltonian of a starting density, rho_ij = 0."""
+        lib.call("opaw_ncpaw_make_hamiltonian_density", self.ctx, sys, _dp(_f64(dens)), int(diis))
+
     def hminmax(self, sys: int = 0, h_type: int = 1, niter: int = 25, fctr: float = 1.2, seed: int = 12345):
         lo, hi = C.c_double(), C.c_double()
         lib.call("opaw_hminmax", self.ctx, sys, h_type, int(niter), float(fctr), C.c_ulonglong(seed),

@@ -39,10 +39,12 @@ def _occupied_residual(g, eig, nocc):
 
 
 def scf_chebyshev(g, nocc: int, nscf: int = 30, tol: float = 1e-8, degree: int = 10, dmua: float = 0.1,
-                  iscf_hminmax: int = 5, seed: int = 1234) -> ScfResult:
+                  iscf_hminmax: int = 5, seed: int = 1234, dens0: Optional[np.ndarray] = None) -> ScfResult:
     """The reference's SCF (src/main.f90:66-95) on the resident orbitals of
     system 0 (h_type 1).  The first mu is the lower bound from hminmax plus
-    a quarter of the interval."""
+    a quarter of the interval.  With dens0 (the atomic superposition of
+    get_vloc_ncoret), iteration 1 builds H from it with rho_ij = 0, as the
+    reference does; otherwise from the starting orbitals."""
     g.torch.cuda.synchronize()
     t0 = time.perf_counter()
     hist = []
@@ -52,7 +54,10 @@ def scf_chebyshev(g, nocc: int, nscf: int = 30, tol: float = 1e-8, degree: int =
     napp = 0
     it = 0
     for it in range(1, nscf + 1):
-        g.ncpaw_make_hamiltonian(0, diis=True)
+        if it == 1 and dens0 is not None:
+            g.ncpaw_make_hamiltonian_density(dens0, 0, diis=True)
+        else:
+            g.ncpaw_make_hamiltonian(0, diis=True)
         if it <= iscf_hminmax or hhi is None:
             hlo, hhi = g.hminmax(0, 1, 25, 1.2, seed + it)
             napp += 50

@@ -11,6 +11,11 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "libopaw_hip.so"
 
 OPAW_OK = 0
+OPAW_ERR_ARG = -1
+OPAW_ERR_HIP = -2
+OPAW_ERR_ALLOC = -3
+OPAW_ERR_STATE = -4
+OPAW_ERR_UNSUPPORTED = -5
 OPAW_PREC_F64 = 0
 OPAW_PREC_MIXED = 1
 FIELD = dict(dens=0, nhat=1, vxc=2, vh=3, vks=4)
@@ -87,6 +92,7 @@ SIGNATURES = {
     "opaw_set_nocc": (I, [P, I]),
     "opaw_set_diis": (I, [P, I, D]),
     "opaw_ncpaw_make_hamiltonian": (I, [P, I, I]),
+    "opaw_ncpaw_make_hamiltonian_density": (I, [P, I, DP, I]),
     "opaw_hminmax": (I, [P, I, I, I, D, C.c_ulonglong, DP, DP]),
     "opaw_chebyshev_filter": (I, [P, I, I, I, D, D]),
     "opaw_residuals": (I, [P, I, I, DP, DP]),
@@ -118,6 +124,15 @@ def load(path: Path | None = None):
     if _LIB is not None:
         return _LIB
     p = Path(path) if path else LIB_PATH
+    # torch ships its own HIP/HSA runtime; it must initialise the device
+    # before this library's runtime (from /opt/rocm) does, or torch then
+    # finds no GPU.  Device pointers are shared between the two.
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except ImportError:
+        pass
     if not p.exists():
         raise OpawError("load", -99, f"{p} not built: run __graft_entry__.build() "
                         "(python -c 'import __graft_entry__ as g; g.build()')")
