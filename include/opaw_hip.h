@@ -282,6 +282,19 @@ int opaw_read_restart(opaw_ctx* ctx, const char* dir, int* it);
  * arrays; either may be NULL.  Runs on the current device, synchronously. */
 int opaw_spectrum(const double* d, int nsig, int nt, double dt, int damp_kind, double tau, double w0, double dw,
                   int nw, double* alpha, double* strength);
+/* opaw_spectrum on caller DEVICE buffers with no allocation: no counterpart
+ * in the reference; for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops).  d (nsig * nt), alpha
+ * (2 nsig nw) and strength (nsig nw) are device memory of the current
+ * device (checked; OPAW_ERR_ARG otherwise, alpha and strength may not be
+ * NULL); the weighted series lives in the library's static per-device
+ * scratch array of 2^21 doubles (16 MiB), so nsig * nt <= 2^21 (else
+ * OPAW_ERR_ARG).  Enqueued on `stream` (a hipStream_t; NULL: the null
+ * stream) with no host synchronisation; results are the same bits as
+ * opaw_spectrum's.  Calls that overlap on one device share the scratch
+ * array and race. */
+int opaw_spectrum_noalloc(const double* d, int nsig, int nt, double dt, int damp_kind, double tau, double w0,
+                          double dw, int nw, double* alpha, double* strength, void* stream);
 
 /* ---- PAW set-up, grid-sized part (prep_paw, src/prepare_paw.f90:1-49).
  * Stateless; host arrays in and out; runs on the current device.

@@ -20,6 +20,7 @@
 // sums a strided subset of samples with FP64 sincos and the wave reduces by
 // butterfly shuffles (fixed order: results do not depend on the launch).
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "opaw_common.h"
@@ -90,6 +91,49 @@ struct DevMem {
   }
 };
 
+// Work space of opaw_spectrum_noalloc: the weighted series g of up to
+// kSpecScratchMax samples (nsig * nt), in a file-scope device array so the
+// entry point allocates nothing.  Each device holds its own copy for the
+// life of the process; its device address is looked up once per device
+// (hipGetSymbolAddress) and kept in scratch_ptr.
+constexpr long long kSpecScratchMax = 1LL << 21;   // 2 Mi doubles, 16 MiB per device
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kSpecScratchMax];
+static double* scratch_ptr[kMaxDevices] = {};
+static std::mutex scratch_mu;
+
+// the device address of this device's scratch array (nullptr and a message
+// on failure)
+int scratch_of_current_device(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return opaw::io_fail(OPAW_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return opaw::io_fail(OPAW_ERR_HIP, "device %d outside 0..%d", dev, kMaxDevices - 1);
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (!scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return opaw::io_fail(OPAW_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    scratch_ptr[dev] = (double*)p;
+  }
+  *out = scratch_ptr[dev];
+  return OPAW_OK;
+}
+
+// a pointer the kernels may use: device (or managed) memory of the current device
+bool is_device_pointer(const void* p) {
+  if (!p) return false;
+  hipPointerAttribute_t a{};
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();   // a host pointer leaves an error behind
+    return false;
+  }
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return false;
+  return (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged) && a.device == dev;
+}
+
 }  // namespace
 
 extern "C" int opaw_spectrum(const double* d, int nsig, int nt, double dt, int damp_kind, double tau, double w0,
@@ -127,5 +171,42 @@ extern "C" int opaw_spectrum(const double* d, int nsig, int nt, double dt, int d
   if (strength && !chk(hipMemcpy(strength, ds.p, nout * sizeof(double), hipMemcpyDeviceToHost)))
     return io_fail(OPAW_ERR_HIP, "%s", hipGetErrorString(e));
   if (!chk(hipDeviceSynchronize())) return io_fail(OPAW_ERR_HIP, "%s", hipGetErrorString(e));
+  return OPAW_OK;
+}
+
+// The same spectrum on caller device buffers, with no allocation: the
+// weighted series lives in this library's static scratch array (at most
+// kSpecScratchMax = 2^21 samples, nsig * nt), and both kernels are enqueued
+// on `stream` (nullptr: the null stream) without a host synchronisation.
+// d (nsig * nt), alpha (2 * nsig * nw) and strength (nsig * nw) must be
+// device memory of the current device.  Calls that overlap on one device
+// share the scratch array and race.
+extern "C" int opaw_spectrum_noalloc(const double* d, int nsig, int nt, double dt, int damp_kind, double tau,
+                                     double w0, double dw, int nw, double* alpha, double* strength, void* stream) {
+  using opaw::io_fail;
+  if (nsig < 1 || nsig > 65535) return io_fail(OPAW_ERR_ARG, "nsig must be 1..65535");
+  if (nt < 1 || nw < 1) return io_fail(OPAW_ERR_ARG, "nt and nw must be positive");
+  if (!(dt > 0.0)) return io_fail(OPAW_ERR_ARG, "dt must be positive");
+  if (damp_kind < 0 || damp_kind > 2) return io_fail(OPAW_ERR_ARG, "damp_kind must be 0, 1 or 2");
+  if (damp_kind != 0 && !(tau > 0.0)) return io_fail(OPAW_ERR_ARG, "tau must be positive");
+  const long long total = (long long)nsig * nt;
+  const long long nout = (long long)nsig * nw;
+  if (total > kSpecScratchMax)
+    return io_fail(OPAW_ERR_ARG, "nsig * nt = %lld exceeds the scratch of %lld samples", total, kSpecScratchMax);
+  if (nout > (1LL << 31)) return io_fail(OPAW_ERR_ARG, "spectrum too large");
+  if (!is_device_pointer(d) || !is_device_pointer(alpha) || !is_device_pointer(strength))
+    return io_fail(OPAW_ERR_ARG, "d, alpha and strength must be device memory of the current device");
+  double* g = nullptr;
+  const int r = scratch_of_current_device(&g);
+  if (r != OPAW_OK) return r;
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned gb = (unsigned)((total + 255) / 256);
+  hipLaunchKernelGGL(k_spec_weight, dim3(gb), dim3(256), 0, st, d, g, nt, total, dt, damp_kind, tau);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return io_fail(OPAW_ERR_HIP, "%s", hipGetErrorString(e));
+  hipLaunchKernelGGL(k_spec_ft, dim3((unsigned)((nw + kSpecWaves - 1) / kSpecWaves), (unsigned)nsig), dim3(256), 0,
+                     st, (const double*)g, nt, dt, w0, dw, nw, alpha, strength);
+  e = hipGetLastError();
+  if (e != hipSuccess) return io_fail(OPAW_ERR_HIP, "%s", hipGetErrorString(e));
   return OPAW_OK;
 }

@@ -86,6 +86,7 @@ SIGNATURES = {
     "opaw_write_restart": (I, [P, C.c_char_p, I]),
     "opaw_read_restart": (I, [P, C.c_char_p, IP]),
     "opaw_spectrum": (I, [DP, I, I, D, I, D, D, D, I, DP, DP]),
+    "opaw_spectrum_noalloc": (I, [P, I, I, D, I, D, D, D, I, P, P, P]),
     "opaw_paw_box": (I, [I, DP, I, DP, DP, IP, IP, IP, DP, DP, DP, DP, DP, DP, DP, DP, DP]),
     "opaw_vloc_tuma": (I, [P, I, IP, DP, DP, I, IP, DP, DP]),
     "opaw_paw_sphere": (I, [I, I, I, D, D, D, DP, D, D, I, DP, I, I, DP, I, IP, IP, DP]),
