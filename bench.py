@@ -3,24 +3,29 @@
 Workload (one "step" = one iteration of the reference time loop,
 src/tddft.f90:67-91 with dyn=3): the Hamiltonian rebuild of both systems,
 rk4_prop_shs with a self-consistent rebuild after stages 1-3 and the norm
-restore, and the dipole of both systems.  The system is the naphthalene-shaped
-synthetic OPAW system: C10H8 D2h geometry (C-C 2.65, C-H 2.05 bohr), 64^3
-grid at 0.5 bohr, Martyna-Tuckerman Hartree on the 128^3 doubled box, 10 C
-atoms (14^3 projector boxes, 8 projectors) and 8 H atoms (10^3 boxes, 5
-projectors), 24 occupied orbitals, 2 systems (unperturbed + kick 1e-3 along
-x), dt = 0.05 a.u.  Projector/compensation tables, vloc, ncoret and the
-on-site D_ij model are seeded synthetic tables of the reference shapes.
+restore, and the dipole of both systems.  The system is naphthalene C10H8 in
+its D2h form (C-C 2.65, C-H 2.05 bohr) built from the reference's JTH LDA
+C/H datasets: 64^3 grid at 0.5 bohr, Martyna-Tuckerman Hartree on the 128^3
+doubled box, 10 C atoms (14^3 projector boxes, 8 projectors) and 8 H atoms
+(10^3 boxes, 5 projectors), libpaw's on-site D_ij at every rebuild, 24
+occupied orbitals, 2 systems (unperturbed + kick 1e-3 along x), dt = 0.05
+a.u.  The orbitals are the ground state the run computes first (GPU SCF
+from the atomic-superposition density to an occupied residual of 1e-8), as
+the reference starts its propagation; set-up and ground state are outside
+the timed region and reported beside it.
 
-Both precisions run from the same initial orbitals: FP64, and the mixed path
-(FP32 orbitals and FFTs; FP64 density, rho_ij, D_ij, Hartree and dipoles).
+Both precisions run from the ground state: FP64, and the mixed path (FP32
+orbitals and FFTs; FP64 density, rho_ij, D_ij, Hartree and dipoles).
 `value` is the mixed-path rate when its dipole trace stays within 1e-4
 (relative L2) of the FP64 trace, else the FP64 rate.  The CPU baseline is the
 FP64 oracle timed on a bounded sample (one full step) on this host, with its
 3D FFTs threaded over the host cores this run may use (and once more on one
 thread).
 
-Beside the headline, `batched_polyacenes` reports the batched uneven
-workload: the polyacenes n = 1..6 on their own grids, 4 systems each, mixed
+Beside the headline, `hpsi_96` times one H.psi over the 128 orbitals of the
+synthetic 96^3 periodic workload (40 atoms x 8 projectors) in FP64 and FP32
+against its 5 s_c N bytes-per-orbital model; `batched_polyacenes` reports
+the batched uneven workload: the polyacenes n = 1..6 on their own grids, 4 systems each, mixed
 path, all advanced together by opaw_tddft_run_batch, against the same
 molecules run one after another; `coronene` the C24H12 96^3 workload with
 kicks along x, y, z in both precisions.
@@ -57,7 +62,11 @@ def parse():
     ap.add_argument("--no-coronene", action="store_true",
                     help="skip the coronene 96^3 four-system workload (reported beside the headline line)")
     ap.add_argument("--coronene-steps", type=int, default=10)
+    ap.add_argument("--no-hpsi", action="store_true", help="skip the 96^3 H.psi leg")
     return ap.parse_args()
+
+
+GOLD = os.path.join(_HERE, "tests", "golden", "jth")   # the reference's JTH C/H datasets
 
 
 TRAFFIC_FILE = os.path.join(_HERE, "profiles", "hbm_traffic_c10h8_64.json")
@@ -264,6 +273,52 @@ def run_coronene(steps):
             "kick_response_rel_l2_mixed_vs_fp64": err}
 
 
+def run_hpsi(reps=20):
+    """One H.psi (h_phi) over 128 orbitals of the synthetic 96^3 periodic
+    workload at 0.4 bohr (40 atoms x 8 projectors, r^l exp(-r^2/0.8) radial
+    form, V_KS of 40 Gaussians): orbitals seeded N(0,1), Gaussian-smoothed
+    (sigma 1 bohr) and orthonormalised on the device; ms per apply in both
+    precisions against the model 5 s_c N bytes per orbital (FFT read+write,
+    inverse read, epilogue reads psi and writes H psi)."""
+    import torch
+    from opaw_amd import synth
+    from opaw_amd.api import OpawGPU
+    spec = synth.hpsi_system(n=(96, 96, 96), d=0.4, natom=40, seed=2024)
+    nb = 128
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(5)
+    x = torch.randn(nb, spec.nz, spec.ny, spec.nx, dtype=torch.complex128, device="cuda", generator=gen)
+    k = [torch.fft.fftfreq(n, d=spec.dx, device="cuda") * 2 * math.pi for n in (spec.nz, spec.ny, spec.nx)]
+    k2 = k[0][:, None, None] ** 2 + k[1][None, :, None] ** 2 + k[2][None, None, :] ** 2
+    x = torch.fft.ifftn(torch.fft.fftn(x, dim=(1, 2, 3)) * torch.exp(-0.5 * k2), dim=(1, 2, 3)).reshape(nb, -1)
+    gram = (x.conj() @ x.T) * spec.dv
+    ev, U = torch.linalg.eigh(gram)
+    x = ((U * ev.rsqrt()).T.conj() @ x).contiguous()
+    out = {}
+    for prec in ("f64", "mixed"):
+        g = OpawGPU(spec, prec, nsys=1, nb=nb)
+        xin = x if prec == "f64" else x.to(torch.complex64)
+        h = g.h_phi(xin, 0)
+        for _ in range(3):
+            g.h_phi(xin, 0, out=h)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            g.h_phi(xin, 0, out=h)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        sc = 16 if prec == "f64" else 8
+        model = 5 * sc * spec.n * nb
+        out[prec] = {"ms_per_apply": ms, "model_bytes": model, "achieved_gbs": model / (ms / 1e3) / 1e9,
+                     "frac": model / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+        g.close()
+    return {"workload": "synthetic H.psi, 96^3 periodic @0.4 bohr, 128 orbitals, 40 atoms x 8 projectors",
+            "reps": reps, "fp64": out["f64"], "fp32": out["mixed"]}
+
+
 def host_threads():
     """Host threads this run may use: the OMP_NUM_THREADS share when set,
     else the visible cores, at most 16."""
@@ -310,16 +365,22 @@ def main():
         sys.exit(2)
     import numpy as np
     import torch
-    from opaw_amd import synth
+    from opaw_amd import workloads as W
     torch.cuda.init()
     if a.copy_calib:
         copy_calibration(a.copy_calib)
-    nb = a.norb
-    spec = synth.molecule_system(nring=2, n=(64, 64, 64), d=0.5, seed=1234)
-    spec.rnel = 2.0 * nb
-    phi0 = synth.lowdin_orthonormalise(synth.random_orbitals(spec, nb, seed=7, envelope=4.0), spec.dv)
-    print(f"[bench] workload ready: {spec.nx}x{spec.ny}x{spec.nz}, {len(spec.atoms)} atoms, {nb} orbitals",
-          file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    spec = W.acene_system(2, (64, 64, 64), 0.5, GOLD, name="C10H8-D2h-64")
+    t_setup = time.perf_counter() - t0
+    gst = W.ground_state(spec, tol=1e-8, nscf=60)
+    nb = spec.nocc
+    phi0 = gst.phi
+    ground = {"scf_iterations": gst.scf.iterations, "occupied_residual": gst.scf.residual,
+              "scf_seconds": gst.scf.seconds, "setup_seconds": t_setup, "bands": len(gst.eig),
+              "homo_lumo_gap_ha": float(gst.eig[nb] - gst.eig[nb - 1])}
+    print(f"[bench] workload ready: {spec.nx}x{spec.ny}x{spec.nz}, {len(spec.atoms)} atoms, {nb} orbitals; "
+          f"ground state in {gst.scf.iterations} SCF iterations ({gst.scf.seconds:.2f} s), residual "
+          f"{gst.scf.residual:.1e}", file=sys.stderr, flush=True)
 
     stream_gbs = stream_bandwidth()
     print(f"[bench] streaming read+write rate of this device: {stream_gbs:.0f} GB/s", file=sys.stderr, flush=True)
@@ -335,6 +396,13 @@ def main():
         print(f"[bench] batched polyacenes: {batched['ms_per_batch_step']:.2f} ms per batch step "
               f"({batched['speedup_over_sequential']:.2f}x the molecules one after another)",
               file=sys.stderr, flush=True)
+
+    hpsi = None
+    if not a.no_hpsi:
+        hpsi = run_hpsi()
+        print(f"[bench] H.psi 96^3 x 128: FP64 {hpsi['fp64']['ms_per_apply']:.3f} ms "
+              f"({hpsi['fp64']['frac']:.2f} of HBM peak), FP32 {hpsi['fp32']['ms_per_apply']:.3f} ms "
+              f"({hpsi['fp32']['frac']:.2f})", file=sys.stderr, flush=True)
 
     coronene = None
     if not a.no_coronene:
@@ -376,9 +444,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if key == "f64" else "f32 orbitals/FFT + f64 rho/rhoij/Dij/Hartree/dipole",
-        "data": "synthetic",
-        "config": {"workload": "C10H8-shaped synthetic OPAW system, 64^3 MT grid @0.5 bohr, "
-                               "24 orbitals x 2 systems, RK4 dt=0.05 with per-stage rebuild",
+        "data": "C10H8 D2h from the reference's JTH C/H datasets; ground state computed in the run",
+        "config": {"workload": "C10H8 D2h (C-C 2.65, C-H 2.05 bohr), JTH LDA datasets, 64^3 MT grid @0.5 bohr, "
+                               "24 orbitals x 2 systems from the GPU ground state, RK4 dt=0.05 with per-stage "
+                               "rebuild and libpaw on-site D_ij",
                    "grid": [spec.nx, spec.ny, spec.nz], "spacing_bohr": spec.dx, "boundary": "Martyna-Tuckerman",
                    "atoms": {"C": spec.extra["natom_c"], "H": spec.extra["natom_h"]}, "orbitals": nb,
                    "systems": 2, "dt": spec.dt, "kick": spec.sm, "parallelism": "single GPU"},
@@ -396,6 +465,8 @@ def main():
         if ("f64" in res and "mixed" in res) else None,
         "dipole_rel_l2_mixed_vs_fp64": dip_err,
         "breakdown_ms_per_step": r["breakdown_ms_per_step"],
+        "ground_state": ground,
+        "hpsi_96": hpsi,
         "batched_polyacenes": batched,
         "coronene": coronene,
     }

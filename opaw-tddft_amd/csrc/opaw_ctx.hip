@@ -957,7 +957,7 @@ int opaw_finalize_tables(opaw_ctx* c) {
     RCHK(upload(c, &c->otab, ot_d));
     RCHK(upload(c, &c->oint, ot_i));
     RCHK(dalloc_t(c, &c->dxc, (size_t)c->nsys * std::max(1, moff)));
-    RCHK(dalloc_t(c, &c->oscr, (size_t)std::max(1, natom) * c->nsys * 4 * c->onr_max));
+    RCHK(dalloc_t(c, &c->oscr, (size_t)std::max(1, natom) * c->nsys * onsite_scratch_doubles(c->onr_max)));
     HCHK(hipMemset(c->dxc, 0, (size_t)c->nsys * std::max(1, moff) * sizeof(double)));
   }
   // projection chunks and coefficient map

@@ -24,9 +24,12 @@ struct OnsiteType {
 // pawdensities (opt_dens = 1), pawxcm (pawxcdev = 1, ABINIT ixc = 7 LDA)
 // for the all-electron and pseudo moments, pawdijxcm.  dxc receives the
 // (ms, ms) matrices at the atoms' moff, per system (stride mtot); scratch
-// holds 4 nr_max doubles per (atom, system).
+// holds onsite_scratch_doubles(nr_max) doubles per (atom, system).
 hipError_t onsite_xc(hipStream_t st, const AtomDesc* atoms, int natom, const OnsiteType* ot, const double* otab,
                      const int* oint, const double* rhoij, int mtot, int nsys, double* scratch, int nr_max,
                      double* dxc);
+
+// Scratch doubles of one (atom, system) for types of at most nr_max points.
+size_t onsite_scratch_doubles(int nr_max);
 
 }  // namespace opaw

@@ -16,21 +16,26 @@
 // integrates sum_lm G(lm, ij) int (v_lm phi_i phi_j - vt_lm tphi_i tphi_j)
 // with simp_gen's weights.
 //
-// One workgroup per (atom, system).  Each thread takes radial points in a
-// stride, evaluates the moments and the six LDA potentials there and adds
-// its share of every (pair, lm) integral in registers; the integrals are
-// reduced in a fixed order (wave shuffles, then waves in order), so results
-// are bit-identical run to run.  The rare low-density interpolation of
-// pawxcm (points with n < xc_denpos bracketed by denser points) is applied
-// afterwards by one thread from the spherical values kept in scratch.
+// The radial mesh is cut into chunks of 128 points, one workgroup per
+// (chunk, atom, system), so a rebuild fills the device with a few hundred
+// workgroups: per point one thread evaluates the all-electron moments and
+// potentials and another the pseudo ones, the weighted potential moments and
+// the chunk's pair functions go to LDS, and one thread per (pair, lm) sums
+// the chunk point by point.  A second kernel adds the chunk partials in chunk
+// order, so results are bit-identical run to run, and applies the rare
+// low-density interpolation of pawxcm (points with n < xc_denpos bracketed by
+// denser points) from the spherical values kept in scratch.
 #include "opaw_onsite.h"
 
 namespace opaw {
 
 namespace {
 
-constexpr int kOT = 512;
-constexpr int kNP = kOnsiteMaxNij * kOnsiteMaxLm;
+constexpr int kPts = 128;                             // radial points per chunk (workgroup)
+constexpr int kOT = 2 * kPts;                          // all-electron + pseudo thread per point
+constexpr int kFin = 128;
+constexpr int kNP = kOnsiteMaxNij * kOnsiteMaxLm;     // (pair, moment) integrals
+static_assert(kNP <= kPts, "one thread per (pair, lm) integral in each half");
 constexpr double kSqfpi = 3.5449077018110318;         // sqrt(4 pi)
 constexpr double kDelta = 1.0e-4;                      // pawxcm delta
 constexpr double kRhoMin = 1.0e-14;                    // pawxcm rho_min
@@ -44,12 +49,6 @@ constexpr double kRsfac = 0.6203504908994000;
 constexpr double kSqRsfac = 0.7876233178997433;
 constexpr double kSqRsfacInv = 1.269642451250142;
 constexpr double kA = 0.031091, kA1 = 0.21370, kB1 = 7.5957, kB2 = 3.5876, kB3 = 1.6382, kB4 = 0.49294;
-
-__device__ __forceinline__ double wsum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
-}
 
 // xcpbe option 1, nspden = 1: potential of the spin-up density rho_up
 __device__ __forceinline__ double lda_v(double rho_up) {
@@ -125,29 +124,39 @@ __device__ __forceinline__ void moments(const double* __restrict__ C, const doub
   for (int lm = 0; lm < kOnsiteMaxLm; ++lm) n[lm] = c[lm] + 3.0 * (a[lm] - b[lm]);
 }
 
-__global__ __launch_bounds__(kOT) void k_onsite_xc(const AtomDesc* __restrict__ atoms,
-                                                    const OnsiteType* __restrict__ ots,
-                                                    const double* __restrict__ otab, const int* __restrict__ oint,
-                                                    const double* __restrict__ rhoij, int mtot,
-                                                    double* __restrict__ scratch, int nr_max,
-                                                    double* __restrict__ dxc) {
-  const int a = blockIdx.x, s = blockIdx.y;
+// Scratch of one (atom, system): r0, v0 (all-electron), r0t, v0t (pseudo)
+// at every radial point, then per chunk the kNP partial integrals and the
+// chunk's low-density flag.
+__host__ __device__ constexpr long long scratch_per(int nr_max) {
+  return 4LL * nr_max + (long long)((nr_max + kPts - 1) / kPts) * (kNP + 1);
+}
+
+// Grid (chunk, atom, system): threads [0, kPts) evaluate the all-electron
+// moments and potentials at the chunk's points, threads [kPts, 2 kPts) the
+// pseudo ones; then thread q < kNP of each half sums its (pair, lm) integral
+// over the chunk from LDS, and the chunk's partial (all-electron minus
+// pseudo) goes to scratch.
+__global__ __launch_bounds__(kOT) void k_onsite_points(const AtomDesc* __restrict__ atoms,
+                                                        const OnsiteType* __restrict__ ots,
+                                                        const double* __restrict__ otab,
+                                                        const int* __restrict__ oint,
+                                                        const double* __restrict__ rhoij, int mtot,
+                                                        double* __restrict__ scratch, int nr_max) {
+  const int c = blockIdx.x, a = blockIdx.y, s = blockIdx.z;
   const AtomDesc A = atoms[a];
   if (A.ot < 0) return;                                // uniform over the workgroup
   const OnsiteType T = ots[A.ot];
-  const int ms = A.ms, m2 = ms * ms, L = T.lm, P = T.nij, nr = T.nr;
+  const int nr = T.nr;
+  const int c0 = c * kPts;
+  if (c0 >= nr) return;                                // uniform: this type has fewer chunks
+  const int ms = A.ms, m2 = ms * ms, L = T.lm, P = T.nij;
   const double* gnt = otab + T.gnt;
   const int* kln = oint + T.kln;
-  const double* pp = otab + T.pp;
-  const double* tpp = otab + T.tpp;
-  const double* inv2 = otab + T.inv2;
-  const double* w = otab + T.w;
-  const double* core = otab + T.core;
-  const double* tcore = otab + T.tcore;
   __shared__ double rho[kMaxMs * kMaxMs];
   __shared__ double C[kNP];
-  __shared__ double I[kNP];
-  __shared__ double red[kOT / 64][kNP];
+  __shared__ double V[2][kPts][kOnsiteMaxLm];          // w v_lm, all-electron and pseudo
+  __shared__ double Pw[2][kOnsiteMaxNij][kPts + 1];    // pair functions of the chunk (padded rows)
+  __shared__ double red[2][kNP];
   __shared__ int flag;
   const double* rg = rhoij + (long long)s * mtot + A.moff;
   for (int q = threadIdx.x; q < m2; q += kOT) {
@@ -168,46 +177,70 @@ __global__ __launch_bounds__(kOT) void k_onsite_xc(const AtomDesc* __restrict__ 
     C[q] = acc;
   }
   __syncthreads();
-  double acc[kNP];
-#pragma unroll
-  for (int q = 0; q < kNP; ++q) acc[q] = 0.0;
-  double* sc = scratch + (long long)(s * gridDim.x + a) * 4 * nr_max;
+  const int h = threadIdx.x / kPts, i = threadIdx.x % kPts, r = c0 + i;
+  const double* pw = otab + (h == 0 ? T.pp : T.tpp);
+  double* sc = scratch + (long long)(s * gridDim.y + a) * scratch_per(nr_max);
   const double tolr = kDenpos * (1.0 + 1e-6);
-  bool low = false;
-  for (int r = threadIdx.x; r < nr; r += kOT) {
-    double n[kOnsiteMaxLm], v[kOnsiteMaxLm], vt[kOnsiteMaxLm];
-    moments(C, pp, inv2, nr, P, L, r, n);
-    const double r0 = xc_moments(n, L, core[r], v);
-    moments(C, tpp, inv2, nr, P, L, r, n);
-    const double r0t = xc_moments(n, L, T.usetcore ? tcore[r] : 0.0, vt);
-    sc[r] = r0;
-    sc[nr_max + r] = v[0];
-    sc[2 * nr_max + r] = r0t;
-    sc[3 * nr_max + r] = vt[0];
-    low = low || r0 < tolr || r0t < tolr;
-    const double wr = w[r];
+  if (r < nr) {
+    double n[kOnsiteMaxLm], v[kOnsiteMaxLm];
+    moments(C, pw, otab + T.inv2, nr, P, L, r, n);
+    const double cr = h == 0 ? otab[T.core + r] : (T.usetcore ? otab[T.tcore + r] : 0.0);
+    const double r0 = xc_moments(n, L, cr, v);
+    sc[2 * h * nr_max + r] = r0;
+    sc[(2 * h + 1) * nr_max + r] = v[0];
+    const double wr = otab[T.w + r];
 #pragma unroll
-    for (int k = 0; k < kOnsiteMaxNij; ++k) {
-      if (k < P) {
-        const double f = wr * pp[(long long)k * nr + r];
-        const double ft = wr * tpp[(long long)k * nr + r];
+    for (int lm = 0; lm < kOnsiteMaxLm; ++lm) V[h][i][lm] = wr * v[lm];
+    for (int k = 0; k < kOnsiteMaxNij; ++k) Pw[h][k][i] = k < P ? pw[(long long)k * nr + r] : 0.0;
+    if (r0 < tolr) flag = 1;
+  } else {
 #pragma unroll
-        for (int lm = 0; lm < kOnsiteMaxLm; ++lm) acc[k * kOnsiteMaxLm + lm] += v[lm] * f - vt[lm] * ft;
-      }
-    }
+    for (int lm = 0; lm < kOnsiteMaxLm; ++lm) V[h][i][lm] = 0.0;
+    for (int k = 0; k < kOnsiteMaxNij; ++k) Pw[h][k][i] = 0.0;
   }
-  if (low) flag = 1;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-  for (int q = 0; q < kNP; ++q) {
-    const double t = wsum(acc[q]);
-    if (lane == 0) red[wv][q] = t;
-  }
-  __threadfence_block();
   __syncthreads();
-  for (int q = threadIdx.x; q < kNP; q += kOT) {
+  if (i < kNP) {
+    const int k = i / kOnsiteMaxLm, lm = i % kOnsiteMaxLm;
+    const int len = min(kPts, nr - c0);
+    double acc = 0.0;
+    for (int j = 0; j < len; ++j) acc += V[h][j][lm] * Pw[h][k][j];
+    red[h][i] = acc;
+  }
+  __syncthreads();
+  double* part = sc + 4LL * nr_max + (long long)c * (kNP + 1);
+  if (threadIdx.x < kNP) part[threadIdx.x] = red[0][threadIdx.x] - red[1][threadIdx.x];
+  if (threadIdx.x == 0) part[kNP] = flag ? 1.0 : 0.0;
+}
+
+// Grid (atom, system): the chunk partials added in chunk order, pawxcm's
+// low-density interpolation where a chunk flagged it, and
+// Dij^xc(i, j) = sum_lm G(lm, i, j) I(kln(i, j), lm).
+__global__ __launch_bounds__(kFin) void k_onsite_finish(const AtomDesc* __restrict__ atoms,
+                                                         const OnsiteType* __restrict__ ots,
+                                                         const double* __restrict__ otab,
+                                                         const int* __restrict__ oint, int mtot,
+                                                         const double* __restrict__ scratch, int nr_max,
+                                                         double* __restrict__ dxc) {
+  const int a = blockIdx.x, s = blockIdx.y;
+  const AtomDesc A = atoms[a];
+  if (A.ot < 0) return;                                // uniform over the workgroup
+  const OnsiteType T = ots[A.ot];
+  const int ms = A.ms, m2 = ms * ms, L = T.lm, P = T.nij, nr = T.nr;
+  const int nch = (nr + kPts - 1) / kPts;
+  const double* gnt = otab + T.gnt;
+  const int* kln = oint + T.kln;
+  const double* sc = scratch + (long long)(s * gridDim.x + a) * scratch_per(nr_max);
+  const double* part = sc + 4LL * nr_max;
+  __shared__ double I[kNP];
+  __shared__ int flag;
+  if (threadIdx.x == 0) {
+    int f = 0;
+    for (int cc = 0; cc < nch; ++cc) f |= part[(long long)cc * (kNP + 1) + kNP] != 0.0;
+    flag = f;
+  }
+  for (int q = threadIdx.x; q < kNP; q += kFin) {
     double t = 0.0;
-    for (int k = 0; k < kOT / 64; ++k) t += red[k][q];
+    for (int cc = 0; cc < nch; ++cc) t += part[(long long)cc * (kNP + 1) + q];
     I[q] = t;
   }
   __syncthreads();
@@ -217,11 +250,13 @@ __global__ __launch_bounds__(kOT) void k_onsite_xc(const AtomDesc* __restrict__ 
     // the l = 0 integrals, for the all-electron (h = 0) and pseudo (h = 1)
     // moments
     const double* rad = otab + T.rad;
+    const double* w = otab + T.w;
+    const double tolr = kDenpos * (1.0 + 1e-6);
     for (int h = 0; h < 2; ++h) {
       const double* r0v = sc + 2 * h * nr_max;
       const double* v0 = sc + (2 * h + 1) * nr_max;
       const double sign = h == 0 ? 1.0 : -1.0;
-      const double* pw = h == 0 ? pp : tpp;
+      const double* pw = otab + (h == 0 ? T.pp : T.tpp);
       int ir1 = 0, ir2 = 0;
       for (int ir = 1; ir <= nr; ++ir) {
         if (r0v[ir - 1] < tolr) {
@@ -242,7 +277,7 @@ __global__ __launch_bounds__(kOT) void k_onsite_xc(const AtomDesc* __restrict__ 
     }
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < m2; q += kOT) {
+  for (int q = threadIdx.x; q < m2; q += kFin) {
     const int k = kln[q];
     double d = 0.0;
     for (int lm = 0; lm < L; ++lm) d += gnt[(long long)lm * m2 + q] * I[k * kOnsiteMaxLm + lm];
@@ -256,9 +291,14 @@ hipError_t onsite_xc(hipStream_t st, const AtomDesc* atoms, int natom, const Ons
                      const int* oint, const double* rhoij, int mtot, int nsys, double* scratch, int nr_max,
                      double* dxc) {
   if (natom == 0 || !ot) return hipSuccess;
-  hipLaunchKernelGGL(k_onsite_xc, dim3(natom, nsys), dim3(kOT), 0, st, atoms, ot, otab, oint, rhoij, mtot, scratch,
+  const int nch = (nr_max + kPts - 1) / kPts;
+  hipLaunchKernelGGL(k_onsite_points, dim3(nch, natom, nsys), dim3(kOT), 0, st, atoms, ot, otab, oint, rhoij, mtot,
+                     scratch, nr_max);
+  hipLaunchKernelGGL(k_onsite_finish, dim3(natom, nsys), dim3(kFin), 0, st, atoms, ot, otab, oint, mtot, scratch,
                      nr_max, dxc);
   return hipGetLastError();
 }
+
+size_t onsite_scratch_doubles(int nr_max) { return (size_t)scratch_per(nr_max); }
 
 }  // namespace opaw
