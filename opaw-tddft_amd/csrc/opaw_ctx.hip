@@ -1675,23 +1675,39 @@ static int flow_rebuild(opaw_ctx* c, int s0, int ns, const cx<R>* phi, bool pipe
     prof_end(so, KC_POT);
     if (pipelined) HCHK(hipEventRecord(c->ev_onsite, c->st2));
   }
-  prof_begin(sp, KC_POT);
-  HCHK(renorm_pack(sp, c->dens + s0 * N, c->smap, c->nhs + (size_t)s0 * std::max(1, c->S), std::max(1, c->S),
-                   dpart, nd, npart, nn, dv, c->rnel, 1, N, ns, c->pair, c->fac + s0));
-  prof_end(sp, KC_POT);
-  for (int s = s0; s < s0 + ns; ++s) c->nhat_lazy[s] = 1;
   // the Hartree solve stays FP64 in both precisions: the kick response is the
   // difference of two nearly equal systems, and an FP32 V_H moves the mixed
   // dipole trace by ~4e-4 (relative) over 2000 steps
   const int npair = (ns + 1) / 2;
-  for (int q = 0; q < npair; ++q)
-    HCHK(hartree_conv(c->g, c->scale, sp, c->pair + q * N, c->pair + q * N, c->hwork, c->vk,
-                      c->tw64[c->gbig.nx], c->tw64[c->gbig.ny], c->tw64[c->gbig.nz]));
-  if (c->periodic) HCHK(pair_means(sp, c->pair, N, ns, c->mpart, c->means));
-  prof_begin(sp, KC_POT);
-  HCHK(vks_build(sp, c->dens + s0 * N, c->ncoret, c->pair, c->periodic ? c->means : nullptr, c->vloc, N, ns,
-                 c->vxc + s0 * N, c->vh + s0 * N, c->vks + s0 * N, c->vksf + s0 * N));
-  prof_end(sp, KC_POT);
+  if (!c->periodic && hartree_fused_supported(c->g, c->scale)) {
+    // renormalisation + pair packing fused into the first Hartree pass; the
+    // renormalised density is stored by the V_KS kernel
+    const int S1 = std::max(1, c->S);
+    const HartreeRebuild H{c->dens + s0 * N, c->smap, c->nhs + (size_t)s0 * S1, S1, dpart, nd, npart, nn, dv,
+                           c->rnel, 1, c->fac + s0};
+    for (int q = 0; q < npair; ++q)
+      HCHK(hartree_fused(c->g, sp, H, q, ns, c->hwork, c->pair + q * N, c->vk, c->tw64[c->gbig.nx],
+                         c->tw64[c->gbig.ny], c->tw64[c->gbig.nz]));
+    for (int s = s0; s < s0 + ns; ++s) c->nhat_lazy[s] = 1;
+    prof_begin(sp, KC_POT);
+    HCHK(vks_build(sp, c->dens + s0 * N, c->ncoret, c->pair, nullptr, c->vloc, N, ns, c->vxc + s0 * N,
+                   c->vh + s0 * N, c->vks + s0 * N, c->vksf + s0 * N, c->fac + s0));
+    prof_end(sp, KC_POT);
+  } else {
+    prof_begin(sp, KC_POT);
+    HCHK(renorm_pack(sp, c->dens + s0 * N, c->smap, c->nhs + (size_t)s0 * std::max(1, c->S), std::max(1, c->S),
+                     dpart, nd, npart, nn, dv, c->rnel, 1, N, ns, c->pair, c->fac + s0));
+    prof_end(sp, KC_POT);
+    for (int s = s0; s < s0 + ns; ++s) c->nhat_lazy[s] = 1;
+    for (int q = 0; q < npair; ++q)
+      HCHK(hartree_conv(c->g, c->scale, sp, c->pair + q * N, c->pair + q * N, c->hwork, c->vk,
+                        c->tw64[c->gbig.nx], c->tw64[c->gbig.ny], c->tw64[c->gbig.nz]));
+    if (c->periodic) HCHK(pair_means(sp, c->pair, N, ns, c->mpart, c->means));
+    prof_begin(sp, KC_POT);
+    HCHK(vks_build(sp, c->dens + s0 * N, c->ncoret, c->pair, c->periodic ? c->means : nullptr, c->vloc, N, ns,
+                   c->vxc + s0 * N, c->vh + s0 * N, c->vks + s0 * N, c->vksf + s0 * N));
+    prof_end(sp, KC_POT);
+  }
   // pipelined: the D_ij kernel also forms the next stage's nonlocal weights
   // wB = D cB (cB of these orbitals, from the prep above)
   const DijWeights wt{(const c64*)c->cB.p, (c64*)c->wB.p, c->ctot, c->nsys * c->nb, c->nb, s0 * c->nb};

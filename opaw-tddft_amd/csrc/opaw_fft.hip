@@ -25,6 +25,7 @@
 // src/snphi.f90), and the H.psi epilogue rebuilds the same u for its vks * u
 // term and adds the compact nonlocal correction cnl.
 #include "fft_core.h"
+#include "opaw_fields.h"
 #include "opaw_kernels.h"
 
 namespace opaw {
@@ -718,6 +719,167 @@ hipError_t hartree_conv(const GridGeom& g, int scale, hipStream_t st, const c64*
   // x inverse: (y,z) in crop, store cropped x into the small grid
   XLines Lx2{oy, ny, oz, nz, 1, 0, NX, ox, ox + nx};
   return fft_x_pass<double>(KC_HARTREE, NX, 1, EPI_NONE, st, work, big, out, small, Lx2, twx, nullptr, nullptr, 1);
+}
+
+// ---------------------------------------------------------------------------
+// Fused first pass of the rebuild's Hartree solve on the doubled box
+// (NX = 2 nx, NY = 2 ny, NZ = 2 nz, the density embedded at offsets
+// o = n / 2 along each axis).
+//
+// k_hart_fwd, one workgroup per (small z plane, y parity p): the
+// renormalisation and pair packing of the plane (dens + compensation charge,
+// times the factor to rnel), its embedding in ny rows of length NX, the x
+// transform, and the y transform with the zero padding used exactly: with
+// o = ny / 2,
+//   Y[2j + p] = (-1)^j DFT_ny( x[m] w^(p (m + o)) )[j],  w = exp(-2 pi i / NY),
+// so each parity is one ny-point transform of the rows held in LDS, written
+// to the rows 2j + p of the doubled-box plane oz + z.  The z pass (forward,
+// * vk, inverse) and the cropped y and x inverse passes are the plain line
+// passes; V_KS follows as a separate kernel (an inverse fused with V_KS in
+// one workgroup per plane measured slower than the two line passes: one
+// LDS-filling workgroup per CU keeps too few loads in flight).
+constexpr int kHT = 512;
+
+template <int NX, int NY>
+constexpr bool hart_fits() {
+  // ny rows of NX + 8 plus the three twiddle tables
+  return (long)(NY / 2) * (NX + 8) * 16 + (NX + NY + NY / 2) * 16 <= 163840;
+}
+
+template <int NX, int NY>
+__global__ __launch_bounds__(kHT) void k_hart_fwd(HartreeRebuild H, int q, int nsys, int nz,
+                                                   c64* __restrict__ work, const c64* __restrict__ twxg,
+                                                   const c64* __restrict__ twyg) {
+  constexpr int nx = NX / 2, ny = NY / 2, ox = nx / 2, oy = ny / 2;
+  constexpr int LS = NX + 8;
+  __shared__ c64 tile[ny * LS];
+  __shared__ c64 twx[NX];
+  __shared__ c64 twy[NY];
+  __shared__ c64 twh[ny];
+  __shared__ double red[kHT / 64][4];
+  __shared__ double facs[2];
+  const int z = blockIdx.x, p = blockIdx.y;
+  const long long N = (long long)nx * ny * nz;
+  fft::load_twiddles(twx, twxg, NX);
+  fft::load_twiddles(twy, twyg, NY);
+  for (int m = threadIdx.x; m < ny; m += kHT) twh[m] = twyg[2 * m];   // exp(-2 pi i m / ny)
+  // renormalisation factors: every workgroup sums the partials in the same
+  // order (lanes, then waves), so all hold the same factors
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int k = 0; k < 2; ++k) {
+    const int s = 2 * q + k;
+    if (s >= nsys || !H.renorm) break;
+    for (int c = threadIdx.x; c < H.nd; c += kHT) v[2 * k] += H.dpart[(long long)s * H.nd + c];
+    for (int c = threadIdx.x; c < H.nn; c += kHT) v[2 * k + 1] += H.npart[(long long)s * H.nn + c];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][k] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    const int k = threadIdx.x, s = 2 * q + k;
+    double a = 0.0, b = 0.0;
+    for (int w = 0; w < kHT / 64; ++w) {
+      a += red[w][2 * k];
+      b += red[w][2 * k + 1];
+    }
+    const double f = (H.renorm && s < nsys) ? H.rnel / (b * H.dv + a * H.dv) : 1.0;
+    facs[k] = f;
+    if (z == 0 && p == 0 && s < nsys) H.fac[s] = f;
+  }
+  // the embedded rows: zero outside the x window
+  for (int e = threadIdx.x; e < ny * (NX - nx); e += kHT) {
+    const int y = e / (NX - nx), c = e % (NX - nx);
+    tile[y * LS + (c < ox ? c : c + nx)] = mk(0.0, 0.0);
+  }
+  // the plane's points, loads batched (sparse index first, then the values)
+  constexpr int NP = nx * ny;
+  constexpr int CNT = (NP + kHT - 1) / kHT;
+  auto pt = [&](int u) { return min((int)threadIdx.x + u * kHT, NP - 1); };
+  const long long r0 = (long long)z * NP;
+  int kk[CNT];
+#pragma unroll
+  for (int u = 0; u < CNT; ++u) kk[u] = H.smap[r0 + pt(u)];
+  double dd[2][CNT], hh[2][CNT];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int s = min(2 * q + t, nsys - 1);
+#pragma unroll
+    for (int u = 0; u < CNT; ++u) {
+      dd[t][u] = H.dens[(long long)s * N + r0 + pt(u)];
+      hh[t][u] = H.nhs[(long long)s * H.S + max(kk[u], 0)];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < CNT; ++u) {
+    const int e = (int)threadIdx.x + u * kHT;
+    if (NP % kHT == 0 || e < NP) {
+      double pr[2] = {0.0, 0.0};
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+        if (2 * q + t < nsys) pr[t] = dd[t][u] * facs[t] + (kk[u] >= 0 ? hh[t][u] : 0.0) * facs[t];
+      tile[(e / nx) * LS + ox + e % nx] = mk(pr[0], pr[1]);
+    }
+  }
+  __syncthreads();
+  fft::lines<double, NX, ny, kHT, false, LS, 1, -1>(tile, twx);
+  if (p) {
+    for (int e = threadIdx.x; e < ny * NX; e += kHT) {
+      const int y = e / NX, x = e % NX;
+      tile[y * LS + x] = cmul(tile[y * LS + x], twy[y + oy]);
+    }
+    __syncthreads();
+  }
+  fft::lines<double, ny, NX, kHT, true, 1, LS, -1>(tile, twh);
+  const int oz = nz / 2;
+  c64* dst = work + (long long)(oz + z) * NX * NY;
+  for (int e = threadIdx.x; e < ny * NX; e += kHT) {
+    const int j = e / NX, x = e % NX;
+    c64 t = tile[j * LS + x];
+    if (j & 1) t = mk(-t.x, -t.y);
+    dst[(long long)(2 * j + p) * NX + x] = t;
+  }
+}
+
+// small-grid (nx, ny) planes with a compiled fused path
+#define OPAW_HART_PAIRS(X) X(16, 16) X(32, 32) X(48, 48) X(64, 64) X(64, 48) X(48, 64)
+
+bool hartree_fused_supported(const GridGeom& g, int scale) {
+  if (scale != 2 || g.nx % 2 || g.ny % 2 || g.nz % 2) return false;
+#define OPAW_CASE(a, b) \
+  if (g.nx == a && g.ny == b) return hart_fits<2 * a, 2 * b>();
+  OPAW_HART_PAIRS(OPAW_CASE)
+#undef OPAW_CASE
+  return false;
+}
+
+hipError_t hartree_fused(const GridGeom& g, hipStream_t st, const HartreeRebuild& H, int q, int nsys, c64* work,
+                         c64* vh, const double* vk, const c64* twx, const c64* twy, const c64* twz) {
+  if (!hartree_fused_supported(g, 2)) return hipErrorNotSupported;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int NX = 2 * nx, NY = 2 * ny, NZ = 2 * nz, oz = nz / 2;
+  prof_begin(st, KC_HARTREE);
+#define OPAW_CASE(a, b)                                                                                    \
+  if (nx == a && ny == b)                                                                                  \
+    hipLaunchKernelGGL((k_hart_fwd<2 * a, 2 * b>), dim3(nz, 2), dim3(kHT), 0, st, H, q, nsys, nz, work, twx, \
+                       twy);
+  OPAW_HART_PAIRS(OPAW_CASE)
+#undef OPAW_CASE
+  prof_end(st, KC_HARTREE);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  View small{0, -nx / 2, -ny / 2, -oz, nx, ny};
+  View big{0, 0, 0, 0, NX, NY};
+  SLines Lz{NX, 0, NY, 1, oz, oz + nz, oz, oz + nz};
+  if ((e = fft_s_pass<double>(KC_HARTREE, NZ, 2, -1, EPI_MULT, st, work, big, work, big, Lz, twz, vk, big))) return e;
+  SLines Ly2{NX, oz, nz, 1, 0, NY, ny / 2, ny / 2 + ny};
+  if ((e = fft_s_pass<double>(KC_HARTREE, NY, 1, 1, EPI_NONE, st, work, big, work, big, Ly2, twy, nullptr, big))) return e;
+  XLines Lx2{ny / 2, ny, oz, nz, 1, 0, NX, nx / 2, nx / 2 + nx};
+  return fft_x_pass<double>(KC_HARTREE, NX, 1, EPI_NONE, st, work, big, vh, small, Lx2, twx, nullptr, nullptr, 1);
 }
 
 template hipError_t apply_t_plus_v<double>(const GridGeom&, hipStream_t, const c64*, c64*, c64*, int, int,

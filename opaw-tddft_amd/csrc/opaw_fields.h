@@ -16,6 +16,26 @@ constexpr double kXcDensThreshold = 1e-15;  // per-point LDA density cutoff
 constexpr double kRhoijSelect = 1e-10;      // src/get_dij.f90:82
 constexpr double kDijMax = 500.0;           // src/0_parameter.f90 param_dij_max
 
+// LDA: Slater exchange + Perdew-Wang 1992 correlation, unpolarised
+// (libxc ids 1 and 12 as called from src/tddft_pot.f90:163-177)
+__device__ __forceinline__ double lda_vxc(double n) {
+  if (!(n > kXcDensThreshold)) return 0.0;
+  const double pi = 3.14159265358979323846;
+  const double cx = cbrt(3.0 / pi);
+  const double n13 = cbrt(n);
+  const double vx = -cx * n13;
+  const double rs = cbrt(3.0 / (4.0 * pi * n));
+  const double A = 0.031091, a1 = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382, b4 = 0.49294;
+  const double srs = sqrt(rs);
+  const double q0 = -2.0 * A * (1.0 + a1 * rs);
+  const double q1 = 2.0 * A * (b1 * srs + b2 * rs + b3 * rs * srs + b4 * rs * rs);
+  const double dq1 = A * (b1 / srs + 2.0 * b2 + 3.0 * b3 * srs + 4.0 * b4 * rs);
+  const double lg = log1p(1.0 / q1);
+  const double ec = q0 * lg;
+  const double dec = -2.0 * A * a1 * lg - q0 * dq1 / (q1 * q1 + q1);
+  return vx + ec - rs / 3.0 * dec;
+}
+
 // union of the compensation spheres: point k owns entries [sptr[k],sptr[k+1])
 // = (atom, offset of its g value for L=0); g stride of atom a is sn[a]
 struct SphereTabs {
@@ -66,9 +86,11 @@ hipError_t renorm_factor(hipStream_t st, const double* nh, const double* dens, l
 hipError_t scale_pack(hipStream_t st, double* dens, double* nh, const double* fac, long long N, int nsys,
                       c64* pair);
 hipError_t pair_means(hipStream_t st, const c64* pair, long long N, int nsys, double* part, double* means);
-hipError_t vks_build(hipStream_t st, const double* dens, const double* ncoret, const c64* vhpair,
+// with fac: dens_s is first scaled by fac_s and stored (the renormalisation
+// whose factors the fused Hartree pass computed)
+hipError_t vks_build(hipStream_t st, double* dens, const double* ncoret, const c64* vhpair,
                      const double* vhmean, const double* vloc, long long N, int nsys, double* vxc, double* vh,
-                     double* vks, float* vksf);
+                     double* vks, float* vksf, const double* fac = nullptr);
 hipError_t rhoij(hipStream_t st, const AtomDesc* atoms, int natom, const c64* ca, int ctot, int nsys, int nb,
                  int mtot, double* rho);
 // with wt: also the nonlocal weights w(c, f) = sum_j D_ij cB(f, c_j) of every

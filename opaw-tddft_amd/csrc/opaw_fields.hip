@@ -360,40 +360,25 @@ __global__ __launch_bounds__(kT) void k_scale_pack(double* __restrict__ dens, do
   pair[(long long)q * N + r] = mk(v[0], v[1]);
 }
 
-// ---------------------------------------------------------------------------
-// LDA: Slater exchange + Perdew-Wang 1992 correlation, unpolarised
-// (libxc ids 1 and 12 as called from src/tddft_pot.f90:163-177)
-__device__ __forceinline__ double lda_vxc(double n) {
-  if (!(n > kXcDensThreshold)) return 0.0;
-  const double pi = 3.14159265358979323846;
-  const double cx = cbrt(3.0 / pi);
-  const double n13 = cbrt(n);
-  const double vx = -cx * n13;
-  const double rs = cbrt(3.0 / (4.0 * pi * n));
-  const double A = 0.031091, a1 = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382, b4 = 0.49294;
-  const double srs = sqrt(rs);
-  const double q0 = -2.0 * A * (1.0 + a1 * rs);
-  const double q1 = 2.0 * A * (b1 * srs + b2 * rs + b3 * rs * srs + b4 * rs * rs);
-  const double dq1 = A * (b1 / srs + 2.0 * b2 + 3.0 * b3 * srs + 4.0 * b4 * rs);
-  const double lg = log1p(1.0 / q1);
-  const double ec = q0 * lg;
-  const double dec = -2.0 * A * a1 * lg - q0 * dq1 / (q1 * q1 + q1);
-  return vx + ec - rs / 3.0 * dec;
-}
-
 // vxc[s] = LDA(dens[s] + ncoret); vh[s] from the packed Hartree output
 // (minus its mean when periodic); vks[s] = vh + vxc + vloc
 // (src/tddft_pot.f90:130-149)
-__global__ __launch_bounds__(kT) void k_vks(const double* __restrict__ dens, const double* __restrict__ ncoret,
+__global__ __launch_bounds__(kT) void k_vks(double* __restrict__ dens, const double* __restrict__ ncoret,
                                              const c64* __restrict__ vhpair, const double* __restrict__ vhmean,
                                              const double* __restrict__ vloc, int N,
                                              double* __restrict__ vxc, double* __restrict__ vh,
-                                             double* __restrict__ vks, float* __restrict__ vksf) {
+                                             double* __restrict__ vks, float* __restrict__ vksf,
+                                             const double* __restrict__ fac) {
   const int s = blockIdx.y;
   const int r = blockIdx.x * kT + threadIdx.x;
   if (r >= N) return;
   const long long i = (long long)s * N + r;
-  const double x = lda_vxc(dens[i] + ncoret[r]);
+  double dn = dens[i];
+  if (fac) {
+    dn *= fac[s];
+    dens[i] = dn;
+  }
+  const double x = lda_vxc(dn + ncoret[r]);
   const c64 p = vhpair[(long long)(s / 2) * N + r];
   double h = (s & 1) ? p.y : p.x;
   if (vhmean) h -= vhmean[s];
@@ -1220,11 +1205,11 @@ hipError_t pair_means(hipStream_t st, const c64* pair, long long N, int nsys, do
   return hipGetLastError();
 }
 
-hipError_t vks_build(hipStream_t st, const double* dens, const double* ncoret, const c64* vhpair,
+hipError_t vks_build(hipStream_t st, double* dens, const double* ncoret, const c64* vhpair,
                      const double* vhmean, const double* vloc, long long N, int nsys, double* vxc, double* vh,
-                     double* vks, float* vksf) {
+                     double* vks, float* vksf, const double* fac) {
   hipLaunchKernelGGL(k_vks, grid_rows(N, nsys, 1), dim3(kT), 0, st, dens, ncoret, vhpair, vhmean, vloc, (int)N,
-                     vxc, vh, vks, vksf);
+                     vxc, vh, vks, vksf, fac);
   return hipGetLastError();
 }
 

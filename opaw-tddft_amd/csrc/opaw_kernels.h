@@ -60,4 +60,28 @@ hipError_t hartree_conv(const GridGeom& g, int scale, hipStream_t st, const c64*
                         c64* work, const double* vk, const c64* twx, const c64* twy,
                         const c64* twz);
 
+// The rebuild's renormalisation and Martyna-Tuckerman Hartree solve for
+// doubled boxes whose (2 nx) x ny tile fits LDS, the renormalisation and pair
+// packing fused into the first transform pass.  Pair q = systems 2q, 2q+1:
+// fac_s = rnel / (dv (sum dpart_s + sum npart_s)) when renorm, else 1, is
+// stored in fac; rho_s = fac_s (dens_s + nhs_s[smap]) (dens is read, not
+// written: vks_build stores fac_s dens_s); vh[q] receives V_H of
+// rho_2q + i rho_2q+1 on the small grid.
+struct HartreeRebuild {
+  const double* dens;      // nsys x N
+  const int* smap;         // N: point -> sparse compensation index or -1
+  const double* nhs;       // nsys x S sparse compensation charge
+  int S;
+  const double* dpart;     // nsys x nd partial sums of dens
+  int nd;
+  const double* npart;     // nsys x nn partial sums of the compensation charge
+  int nn;
+  double dv, rnel;
+  int renorm;
+  double* fac;             // nsys renormalisation factors (out)
+};
+bool hartree_fused_supported(const GridGeom& g, int scale);
+hipError_t hartree_fused(const GridGeom& g, hipStream_t st, const HartreeRebuild& H, int q, int nsys, c64* work,
+                         c64* vh, const double* vk, const c64* twx, const c64* twy, const c64* twz);
+
 }  // namespace opaw

@@ -153,6 +153,8 @@ __global__ __launch_bounds__(kOT) void k_onsite_points(const AtomDesc* __restric
   const double* gnt = otab + T.gnt;
   const int* kln = oint + T.kln;
   __shared__ double rho[kMaxMs * kMaxMs];
+  __shared__ double G[kOnsiteMaxLm * kMaxMs * kMaxMs];
+  __shared__ int K[kMaxMs * kMaxMs];
   __shared__ double C[kNP];
   __shared__ double V[2][kPts][kOnsiteMaxLm];          // w v_lm, all-electron and pseudo
   __shared__ double Pw[2][kOnsiteMaxNij][kPts + 1];    // pair functions of the chunk (padded rows)
@@ -162,7 +164,9 @@ __global__ __launch_bounds__(kOT) void k_onsite_points(const AtomDesc* __restric
   for (int q = threadIdx.x; q < m2; q += kOT) {
     const double v = rg[q];
     rho[q] = fabs(v) > kRhoijSel ? v : 0.0;            // set_rhoij selection
+    K[q] = kln[q];
   }
+  for (int q = threadIdx.x; q < L * m2; q += kOT) G[q] = gnt[q];
   if (threadIdx.x == 0) flag = 0;
   __syncthreads();
   // pair coefficients C(k, lm) = sum over (i, j) with kln(i, j) = k of
@@ -172,8 +176,7 @@ __global__ __launch_bounds__(kOT) void k_onsite_points(const AtomDesc* __restric
     const int k = q / kOnsiteMaxLm, lm = q % kOnsiteMaxLm;
     double acc = 0.0;
     if (k < P && lm < L)
-      for (int ij = 0; ij < m2; ++ij)
-        if (kln[ij] == k) acc += rho[ij] * gnt[(long long)lm * m2 + ij];
+      for (int ij = 0; ij < m2; ++ij) acc += K[ij] == k ? rho[ij] * G[lm * m2 + ij] : 0.0;
     C[q] = acc;
   }
   __syncthreads();
@@ -182,8 +185,26 @@ __global__ __launch_bounds__(kOT) void k_onsite_points(const AtomDesc* __restric
   double* sc = scratch + (long long)(s * gridDim.y + a) * scratch_per(nr_max);
   const double tolr = kDenpos * (1.0 + 1e-6);
   if (r < nr) {
-    double n[kOnsiteMaxLm], v[kOnsiteMaxLm];
-    moments(C, pw, otab + T.inv2, nr, P, L, r, n);
+    double n[kOnsiteMaxLm], v[kOnsiteMaxLm], f[kOnsiteMaxNij];
+    // the point's pair functions, all loads in flight together
+#pragma unroll
+    for (int k = 0; k < kOnsiteMaxNij; ++k) f[k] = pw[(long long)min(k, P - 1) * nr + r];
+    if (r > 0) {
+      const double i2 = otab[T.inv2 + r];
+#pragma unroll
+      for (int lm = 0; lm < kOnsiteMaxLm; ++lm) n[lm] = 0.0;
+#pragma unroll
+      for (int k = 0; k < kOnsiteMaxNij; ++k) {
+        if (k < P) {
+          const double fk = f[k] * i2;
+#pragma unroll
+          for (int lm = 0; lm < kOnsiteMaxLm; ++lm)
+            if (lm < L) n[lm] += C[k * kOnsiteMaxLm + lm] * fk;
+        }
+      }
+    } else {
+      moments(C, pw, otab + T.inv2, nr, P, L, r, n);
+    }
     const double cr = h == 0 ? otab[T.core + r] : (T.usetcore ? otab[T.tcore + r] : 0.0);
     const double r0 = xc_moments(n, L, cr, v);
     sc[2 * h * nr_max + r] = r0;
@@ -191,7 +212,8 @@ __global__ __launch_bounds__(kOT) void k_onsite_points(const AtomDesc* __restric
     const double wr = otab[T.w + r];
 #pragma unroll
     for (int lm = 0; lm < kOnsiteMaxLm; ++lm) V[h][i][lm] = wr * v[lm];
-    for (int k = 0; k < kOnsiteMaxNij; ++k) Pw[h][k][i] = k < P ? pw[(long long)k * nr + r] : 0.0;
+#pragma unroll
+    for (int k = 0; k < kOnsiteMaxNij; ++k) Pw[h][k][i] = k < P ? f[k] : 0.0;
     if (r0 < tolr) flag = 1;
   } else {
 #pragma unroll
@@ -232,12 +254,14 @@ __global__ __launch_bounds__(kFin) void k_onsite_finish(const AtomDesc* __restri
   const double* sc = scratch + (long long)(s * gridDim.x + a) * scratch_per(nr_max);
   const double* part = sc + 4LL * nr_max;
   __shared__ double I[kNP];
+  __shared__ double G[kOnsiteMaxLm * kMaxMs * kMaxMs];
+  __shared__ int K[kMaxMs * kMaxMs];
   __shared__ int flag;
-  if (threadIdx.x == 0) {
-    int f = 0;
-    for (int cc = 0; cc < nch; ++cc) f |= part[(long long)cc * (kNP + 1) + kNP] != 0.0;
-    flag = f;
-  }
+  if (threadIdx.x == 0) flag = 0;
+  for (int q = threadIdx.x; q < m2; q += kFin) K[q] = kln[q];
+  for (int q = threadIdx.x; q < L * m2; q += kFin) G[q] = gnt[q];
+  __syncthreads();
+  if (threadIdx.x < nch && part[(long long)threadIdx.x * (kNP + 1) + kNP] != 0.0) flag = 1;
   for (int q = threadIdx.x; q < kNP; q += kFin) {
     double t = 0.0;
     for (int cc = 0; cc < nch; ++cc) t += part[(long long)cc * (kNP + 1) + q];
@@ -278,9 +302,9 @@ __global__ __launch_bounds__(kFin) void k_onsite_finish(const AtomDesc* __restri
   }
   __syncthreads();
   for (int q = threadIdx.x; q < m2; q += kFin) {
-    const int k = kln[q];
+    const int k = K[q];
     double d = 0.0;
-    for (int lm = 0; lm < L; ++lm) d += gnt[(long long)lm * m2 + q] * I[k * kOnsiteMaxLm + lm];
+    for (int lm = 0; lm < L; ++lm) d += G[lm * m2 + q] * I[k * kOnsiteMaxLm + lm];
     dxc[(long long)s * mtot + A.moff + q] = d;
   }
 }
@@ -292,6 +316,7 @@ hipError_t onsite_xc(hipStream_t st, const AtomDesc* atoms, int natom, const Ons
                      double* dxc) {
   if (natom == 0 || !ot) return hipSuccess;
   const int nch = (nr_max + kPts - 1) / kPts;
+  if (nch > kFin) return hipErrorInvalidValue;         // the finish kernel reads one flag per thread
   hipLaunchKernelGGL(k_onsite_points, dim3(nch, natom, nsys), dim3(kOT), 0, st, atoms, ot, otab, oint, rhoij, mtot,
                      scratch, nr_max);
   hipLaunchKernelGGL(k_onsite_finish, dim3(natom, nsys), dim3(kFin), 0, st, atoms, ot, otab, oint, mtot, scratch,
