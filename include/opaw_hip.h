@@ -193,6 +193,37 @@ int opaw_get_dij(opaw_ctx* ctx, int sys);                         /* src/get_dij
  * resident orbitals. */
 int opaw_make_hamiltonian_tddft(opaw_ctx* ctx, int sys);
 
+/* ---- host-array operators (drop-ins for a Fortran host) ---------------------
+ * The reference's operators take complex*16 arrays in host memory; these
+ * forms take the same (pin(nx,ny,nz), x fastest, nbatch fields back to back),
+ * convert to the context precision, run the device operator above and
+ * return complex*16.  The Fortran drop-ins h_phi / h_phi_pert, shs /
+ * shs_pert, sn_phi, proj / proj1 and rk4_prop_shs call these.
+ * h_phi(pin,hp,ik) / h_phi_pert (src/hphi.f90:1-10, src/hphi_pert.f90:1):
+ *   sys 0 / sys 1 */
+int opaw_h_phi_host(opaw_ctx* ctx, int sys, const double* pin, double* hp, int nbatch);
+/* shs(pin,pout,ik) / shs_pert (src/shs.f90:3-12, src/shs.f90:50): sys 0 / 1 */
+int opaw_shs_host(opaw_ctx* ctx, int sys, const double* pin, double* pout, int nbatch);
+/* sn_phi(pin,sp,ik,n) (src/snphi.f90:1-11) */
+int opaw_sn_phi_host(opaw_ctx* ctx, const double* pin, double* sp, int nbatch, double n);
+/* proj(ia,pin,ca,ms,ik) / proj1 (src/proj.f90:1-9, src/proj1.f90:1-9): the
+ * ms coefficients of atom ia (1-based) of one field, ca(is) = dv sum pin
+ * conj(p_is) over the atom's box; ms must equal the atom's projector count */
+int opaw_proj_host(opaw_ctx* ctx, int ia, const double* pin, int which_p1, double* ca, int ms);
+/* complex*16 forms of opaw_write_orbitals / opaw_read_orbitals, whatever the
+ * context precision (scatter_phi_bar / gather_phi_bar, src/tddft_scatter.f90) */
+int opaw_write_orbitals_c16(opaw_ctx* ctx, int sys, const double* host);
+int opaw_read_orbitals_c16(opaw_ctx* ctx, int sys, double* host);
+/* tddft_mod's dt (src/tddft_mod.f90), read by opaw_rk4_prop_shs_host */
+int opaw_set_dt(opaw_ctx* ctx, double dt);
+/* rk4_prop_shs(p,pp,n,nb) (src/rk4.f90:1-12) on host arrays p = phi_bar,
+ * pp = phi_bar_pert (n = nx ny nz points, nb orbitals): p and pp must equal
+ * the resident orbitals of systems 0 and 1 that the current Hamiltonian was
+ * built from (else OPAW_ERR_STATE, nothing changed: write them with
+ * opaw_write_orbitals_c16 and rebuild first); one step with the context's
+ * dt, the results returned in p and pp. */
+int opaw_rk4_prop_shs_host(opaw_ctx* ctx, double* p, double* pp, long long n, int nb);
+
 /* ---- time propagation ------------------------------------------------------ */
 /* excite_psi_bar_pert (src/tddft_excite.f90:1-16): orbitals(to) =
  * orbitals(from) * exp(-i sm r_ipol), ipol 1..3 */

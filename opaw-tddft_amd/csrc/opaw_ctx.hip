@@ -249,6 +249,8 @@ struct opaw_ctx {
   // input (A), nonlocal add-back (NL), S^-1/2 of H S^-1/2 input (C)
   DevBuf corrA, corrNL, corrC;
   DevBuf hU;   // h on the union of boxes (fused RK4 stages)
+  DevBuf hin, hout, hca, hflag;   // staging of the host-array entry points
+  double dt = 0.0;                // tddft_mod dt for the host-array rk4_prop_shs
   int coef_cap = 0;
   c64* pair = nullptr;
   c64* hwork = nullptr;
@@ -1901,6 +1903,148 @@ int opaw_proj(opaw_ctx* c, const void* pin, int nbatch, int which_p1, void* ca) 
     HCHK(box_coeffs<float>(c->g, bt, c->st, (const c32*)pin, nbatch, which_p1 ? 1 : 0, nullptr, dvol(c),
                            (double*)c->ppart.p, (c64*)ca, 0, nullptr, nullptr, nbatch, c->mtot, nullptr));
   return OPAW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// host-array entry points: complex*16 fields in host memory, in the
+// reference's layout (x fastest), converted to and from the context
+// precision around the device operators
+static int upload_c16(opaw_ctx* c, const double* host, void* dev, long long nelem) {
+  HCHK(hipStreamSynchronize(c->st));
+  if (c->prec == OPAW_PREC_F64) {
+    HCHK(hipMemcpy(dev, host, (size_t)nelem * sizeof(c64), hipMemcpyHostToDevice));
+    return OPAW_OK;
+  }
+  std::vector<c32> t((size_t)nelem);
+  for (long long i = 0; i < nelem; ++i) t[i] = mk<float>((float)host[2 * i], (float)host[2 * i + 1]);
+  HCHK(hipMemcpy(dev, t.data(), (size_t)nelem * sizeof(c32), hipMemcpyHostToDevice));
+  return OPAW_OK;
+}
+
+static int download_c16(opaw_ctx* c, const void* dev, double* host, long long nelem) {
+  HCHK(hipStreamSynchronize(c->st));
+  if (c->prec == OPAW_PREC_F64) {
+    HCHK(hipMemcpy(host, dev, (size_t)nelem * sizeof(c64), hipMemcpyDeviceToHost));
+    return OPAW_OK;
+  }
+  std::vector<c32> t((size_t)nelem);
+  HCHK(hipMemcpy(t.data(), dev, (size_t)nelem * sizeof(c32), hipMemcpyDeviceToHost));
+  for (long long i = 0; i < nelem; ++i) {
+    host[2 * i] = (double)t[i].x;
+    host[2 * i + 1] = (double)t[i].y;
+  }
+  return OPAW_OK;
+}
+
+// staging for nbatch fields in and out
+static int stage_fields(opaw_ctx* c, int nbatch) {
+  const size_t bytes = (size_t)nbatch * c->N * c->esize();
+  RCHK(ensure(c, c->hin, bytes));
+  RCHK(ensure(c, c->hout, bytes));
+  return OPAW_OK;
+}
+
+int opaw_set_dt(opaw_ctx* c, double dt) {
+  CTX_CHECK(c);
+  if (!(dt > 0.0) || !std::isfinite(dt)) return fail(OPAW_ERR_ARG, "dt must be positive and finite");
+  c->dt = dt;
+  return OPAW_OK;
+}
+
+int opaw_write_orbitals_c16(opaw_ctx* c, int sys, const double* host) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  if (!host) return fail(OPAW_ERR_ARG, "null orbitals");
+  c->prep_valid = false;
+  return upload_c16(c, host, opaw_orbitals_device(c, sys), (long long)c->nb * c->N);
+}
+
+int opaw_read_orbitals_c16(opaw_ctx* c, int sys, double* host) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  if (!host) return fail(OPAW_ERR_ARG, "null orbitals");
+  return download_c16(c, opaw_orbitals_device(c, sys), host, (long long)c->nb * c->N);
+}
+
+int opaw_h_phi_host(opaw_ctx* c, int sys, const double* pin, double* hp, int nbatch) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  if (!pin || !hp || nbatch <= 0) return fail(OPAW_ERR_ARG, "bad h_phi arguments");
+  RCHK(stage_fields(c, nbatch));
+  RCHK(upload_c16(c, pin, c->hin.p, (long long)nbatch * c->N));
+  RCHK(opaw_h_phi(c, sys, c->hin.p, c->hout.p, nbatch));
+  return download_c16(c, c->hout.p, hp, (long long)nbatch * c->N);
+}
+
+int opaw_shs_host(opaw_ctx* c, int sys, const double* pin, double* pout, int nbatch) {
+  CTX_CHECK(c);
+  RCHK(check_sys(c, sys));
+  if (!pin || !pout || nbatch <= 0) return fail(OPAW_ERR_ARG, "bad shs arguments");
+  RCHK(stage_fields(c, nbatch));
+  RCHK(upload_c16(c, pin, c->hin.p, (long long)nbatch * c->N));
+  RCHK(opaw_shs(c, sys, c->hin.p, c->hout.p, nbatch));
+  return download_c16(c, c->hout.p, pout, (long long)nbatch * c->N);
+}
+
+int opaw_sn_phi_host(opaw_ctx* c, const double* pin, double* sp, int nbatch, double n) {
+  CTX_CHECK(c);
+  if (!pin || !sp || nbatch <= 0) return fail(OPAW_ERR_ARG, "bad sn_phi arguments");
+  RCHK(stage_fields(c, nbatch));
+  RCHK(upload_c16(c, pin, c->hin.p, (long long)nbatch * c->N));
+  RCHK(opaw_sn_phi(c, c->hin.p, c->hout.p, nbatch, n));
+  return download_c16(c, c->hout.p, sp, (long long)nbatch * c->N);
+}
+
+int opaw_proj_host(opaw_ctx* c, int ia, const double* pin, int which_p1, double* ca, int ms) {
+  CTX_CHECK(c);
+  if (!c->tables_ready) return fail(OPAW_ERR_STATE, "tables not finalised");
+  if (ia < 1 || ia > c->natom) return fail(OPAW_ERR_ARG, "atom %d outside 1..%d", ia, c->natom);
+  const AtomDesc& A = c->hatoms[ia - 1];
+  if (ms != A.ms) return fail(OPAW_ERR_ARG, "atom %d has %d projectors, not %d", ia, A.ms, ms);
+  if (!pin || !ca) return fail(OPAW_ERR_ARG, "bad proj arguments");
+  RCHK(stage_fields(c, 1));
+  RCHK(ensure(c, c->hca, (size_t)std::max(1, c->ctot) * sizeof(c64)));
+  RCHK(upload_c16(c, pin, c->hin.p, c->N));
+  RCHK(opaw_proj(c, c->hin.p, 1, which_p1, c->hca.p));
+  std::vector<c64> all((size_t)std::max(1, c->ctot));
+  HCHK(hipStreamSynchronize(c->st));
+  HCHK(hipMemcpy(all.data(), c->hca.p, all.size() * sizeof(c64), hipMemcpyDeviceToHost));
+  for (int i = 0; i < ms; ++i) {
+    ca[2 * i] = all[A.coff + i].x;
+    ca[2 * i + 1] = all[A.coff + i].y;
+  }
+  return OPAW_OK;
+}
+
+int opaw_rk4_prop_shs_host(opaw_ctx* c, double* p, double* pp, long long n, int nb) {
+  CTX_CHECK(c);
+  if (!p || !pp) return fail(OPAW_ERR_ARG, "null p / pp");
+  if (n != c->N || nb != c->nb) return fail(OPAW_ERR_ARG, "p(n, nb) is (%lld, %d), the context holds (%lld, %d)", n, nb, c->N, c->nb);
+  if (c->nsys < 2) return fail(OPAW_ERR_STATE, "rk4_prop_shs propagates phi_bar and phi_bar_pert (nsys >= 2)");
+  if (!(c->dt > 0.0)) return fail(OPAW_ERR_STATE, "dt not set (opaw_set_dt)");
+  // p and pp must be the orbitals the current Hamiltonian was built from: the
+  // host copies are compared with the resident ones, word by word
+  const long long ne = (long long)nb * c->N;
+  RCHK(stage_fields(c, nb));
+  RCHK(ensure(c, c->hflag, sizeof(int)));
+  const size_t bytes = (size_t)ne * c->esize();
+  for (int s = 0; s < 2; ++s) {
+    RCHK(upload_c16(c, s == 0 ? p : pp, c->hin.p, ne));
+    HCHK(hipMemsetAsync(c->hflag.p, 0, sizeof(int), c->st));
+    HCHK(words_differ(c->st, (const unsigned*)c->hin.p, (const unsigned*)opaw_orbitals_device(c, s),
+                      (long long)(bytes / sizeof(unsigned)), (int*)c->hflag.p));
+    int flag = 0;
+    HCHK(hipStreamSynchronize(c->st));
+    HCHK(hipMemcpy(&flag, c->hflag.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (flag)
+      return fail(OPAW_ERR_STATE,
+                  "%s differs from the resident orbitals the Hamiltonian was built from: write it "
+                  "(opaw_write_orbitals_c16) and rebuild (opaw_make_hamiltonian_tddft) first",
+                  s == 0 ? "p (phi_bar)" : "pp (phi_bar_pert)");
+  }
+  RCHK(opaw_rk4_prop_shs(c, c->dt));
+  RCHK(download_c16(c, opaw_orbitals_device(c, 0), p, ne));
+  return download_c16(c, opaw_orbitals_device(c, 1), pp, ne);
 }
 
 int opaw_fft3d(opaw_ctx* c, int dir, const void* in, void* out, int nbatch) {

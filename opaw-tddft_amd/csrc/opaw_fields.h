@@ -86,6 +86,8 @@ hipError_t renorm_factor(hipStream_t st, const double* nh, const double* dens, l
 hipError_t scale_pack(hipStream_t st, double* dens, double* nh, const double* fac, long long N, int nsys,
                       c64* pair);
 hipError_t pair_means(hipStream_t st, const c64* pair, long long N, int nsys, double* part, double* means);
+// *flag (device, cleared by the caller) = 1 when a and b differ in any of n words
+hipError_t words_differ(hipStream_t st, const unsigned* a, const unsigned* b, long long n, int* flag);
 // with fac: dens_s is first scaled by fac_s and stored (the renormalisation
 // whose factors the fused Hartree pass computed)
 hipError_t vks_build(hipStream_t st, double* dens, const double* ncoret, const c64* vhpair,

@@ -1205,6 +1205,22 @@ hipError_t pair_means(hipStream_t st, const c64* pair, long long N, int nsys, do
   return hipGetLastError();
 }
 
+// *flag = 1 when a and b differ in any of n words
+__global__ __launch_bounds__(kT) void k_words_differ(const unsigned* __restrict__ a, const unsigned* __restrict__ b,
+                                                      long long n, int* __restrict__ flag) {
+  bool d = false;
+  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n; i += (long long)gridDim.x * kT)
+    d = d || a[i] != b[i];
+  if (d) *flag = 1;
+}
+
+hipError_t words_differ(hipStream_t st, const unsigned* a, const unsigned* b, long long n, int* flag) {
+  if (n <= 0) return hipSuccess;
+  const long long blocks = std::min<long long>((n + kT - 1) / kT, 4096);
+  hipLaunchKernelGGL(k_words_differ, dim3((unsigned)blocks), dim3(kT), 0, st, a, b, n, flag);
+  return hipGetLastError();
+}
+
 hipError_t vks_build(hipStream_t st, double* dens, const double* ncoret, const c64* vhpair,
                      const double* vhmean, const double* vloc, long long N, int nsys, double* vxc, double* vh,
                      double* vks, float* vksf, const double* fac) {

@@ -1,6 +1,7 @@
 ! iso_c_binding interfaces to libopaw_hip.so (include/opaw_hip.h) and the
-! drop-in wrappers a Fortran host would call in place of the reference's
-! external subroutines.  The module keeps one context (the device copy of
+! wrappers a Fortran host calls in place of the reference's module-state
+! routines (the operators and rk4_prop_shs under their own names are in
+! opaw_dropin.f90).  The module keeps one context (the device copy of
 ! main_mod / atom_mod / tddft_mod state); arrays cross the boundary in the
 ! reference layouts (column-major, complex*16 interleaved).
 module opaw_hip_mod
@@ -159,6 +160,55 @@ module opaw_hip_mod
       integer(c_int), value :: sys, h_type
       real(c_double) :: eig(*)
     end function
+    integer(c_int) function opaw_h_phi_host(ctx, sys, pin, hp, nbatch) bind(C, name="opaw_h_phi_host")
+      import :: c_ptr, c_int, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: sys, nbatch
+      complex(c_double_complex) :: pin(*), hp(*)
+    end function
+    integer(c_int) function opaw_shs_host(ctx, sys, pin, pout, nbatch) bind(C, name="opaw_shs_host")
+      import :: c_ptr, c_int, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: sys, nbatch
+      complex(c_double_complex) :: pin(*), pout(*)
+    end function
+    integer(c_int) function opaw_sn_phi_host(ctx, pin, sp, nbatch, n) bind(C, name="opaw_sn_phi_host")
+      import :: c_ptr, c_int, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nbatch
+      real(c_double), value :: n
+      complex(c_double_complex) :: pin(*), sp(*)
+    end function
+    integer(c_int) function opaw_proj_host(ctx, ia, pin, which_p1, ca, ms) bind(C, name="opaw_proj_host")
+      import :: c_ptr, c_int, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ia, which_p1, ms
+      complex(c_double_complex) :: pin(*), ca(*)
+    end function
+    integer(c_int) function opaw_write_orbitals_c16(ctx, sys, host) bind(C, name="opaw_write_orbitals_c16")
+      import :: c_ptr, c_int, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: sys
+      complex(c_double_complex) :: host(*)
+    end function
+    integer(c_int) function opaw_read_orbitals_c16(ctx, sys, host) bind(C, name="opaw_read_orbitals_c16")
+      import :: c_ptr, c_int, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: sys
+      complex(c_double_complex) :: host(*)
+    end function
+    integer(c_int) function opaw_set_dt(ctx, dt) bind(C, name="opaw_set_dt")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dt
+    end function
+    integer(c_int) function opaw_rk4_prop_shs_host(ctx, p, pp, n, nb) bind(C, name="opaw_rk4_prop_shs_host")
+      import :: c_ptr, c_int, c_long_long, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex) :: p(*), pp(*)
+      integer(c_long_long), value :: n
+      integer(c_int), value :: nb
+    end function
     function opaw_last_error() bind(C, name="opaw_last_error")
       import :: c_ptr
       type(c_ptr) :: opaw_last_error
@@ -170,9 +220,11 @@ module opaw_hip_mod
             opaw_read_orbitals, opaw_make_hamiltonian_tddft, opaw_rk4_prop_shs, opaw_plot_dip, &
             opaw_excite_psi_bar_pert, opaw_write_restart, opaw_read_restart, opaw_set_nocc, opaw_set_diis, &
             opaw_ncpaw_make_hamiltonian, opaw_hminmax, opaw_chebyshev_filter, opaw_orthog_phi, opaw_diagh, &
-            opaw_check
-  public :: gpu_rk4_prop_shs, gpu_ncpaw_make_hamiltonian_tddft, gpu_plot_dip, gpu_write_to_restart, &
-            gpu_read_phi_bar, gpu_ncpaw_make_hamiltonian, gpu_get_hminmax, gpu_chebyr_m, gpu_orthog_phi, gpu_diagh
+            opaw_h_phi_host, opaw_shs_host, opaw_sn_phi_host, opaw_proj_host, opaw_write_orbitals_c16, &
+            opaw_read_orbitals_c16, opaw_set_dt, opaw_rk4_prop_shs_host, opaw_check, opaw_require_ik
+  public :: gpu_set_dt, gpu_scatter_phi_bar, gpu_gather_phi_bar, gpu_ncpaw_make_hamiltonian_tddft, gpu_plot_dip, &
+            gpu_write_to_restart, gpu_read_phi_bar, gpu_ncpaw_make_hamiltonian, gpu_get_hminmax, gpu_chebyr_m, &
+            gpu_orthog_phi, gpu_diagh
 
 contains
 
@@ -194,15 +246,38 @@ contains
     end if
   end subroutine
 
-  ! drop-in for rk4_prop_shs(p, pp, n, nb) (src/rk4.f90:1): the orbitals are
-  ! resident on the device; p / pp are refreshed from the device afterwards
-  subroutine gpu_rk4_prop_shs(p, pp, n, nb, dt)
-    integer, intent(in) :: n, nb
-    complex(c_double_complex), target :: p(n, nb), pp(n, nb)
+  ! the context holds one kinetic table: the reference's k index ik is 1
+  subroutine opaw_require_ik(ik, what)
+    integer, intent(in) :: ik
+    character(len=*), intent(in) :: what
+    if (ik /= 1) then
+      write(6, *) 'opaw_hip: ', what, ': only ik = 1 (one kinetic table), got ', ik
+      error stop 'opaw_hip failure'
+    end if
+  end subroutine
+
+  ! tddft_mod's dt, read by the rk4_prop_shs drop-in
+  subroutine gpu_set_dt(dt)
     real(c_double), intent(in) :: dt
-    call opaw_check(opaw_rk4_prop_shs(opaw_ctx, dt), 'rk4_prop_shs')
-    call opaw_check(opaw_read_orbitals(opaw_ctx, 0_c_int, c_loc(p)), 'read phi_bar')
-    call opaw_check(opaw_read_orbitals(opaw_ctx, 1_c_int, c_loc(pp)), 'read phi_bar_pert')
+    call opaw_check(opaw_set_dt(opaw_ctx, dt), 'set_dt')
+  end subroutine
+
+  ! scatter_phi_bar (src/tddft_scatter.f90): phi_bar and phi_bar_pert
+  ! (n points x nb orbitals, complex*16) become the resident orbitals of
+  ! systems 0 and 1
+  subroutine gpu_scatter_phi_bar(p, pp, n, nb)
+    integer, intent(in) :: n, nb
+    complex(c_double_complex) :: p(n, nb), pp(n, nb)
+    call opaw_check(opaw_write_orbitals_c16(opaw_ctx, 0_c_int, p), 'scatter phi_bar')
+    call opaw_check(opaw_write_orbitals_c16(opaw_ctx, 1_c_int, pp), 'scatter phi_bar_pert')
+  end subroutine
+
+  ! gather_phi_bar: the resident orbitals back into phi_bar / phi_bar_pert
+  subroutine gpu_gather_phi_bar(p, pp, n, nb)
+    integer, intent(in) :: n, nb
+    complex(c_double_complex) :: p(n, nb), pp(n, nb)
+    call opaw_check(opaw_read_orbitals_c16(opaw_ctx, 0_c_int, p), 'gather phi_bar')
+    call opaw_check(opaw_read_orbitals_c16(opaw_ctx, 1_c_int, pp), 'gather phi_bar_pert')
   end subroutine
 
   ! drop-in for ncpaw_make_hamiltonian_tddft (src/ncpaw_make_hamiltonian.f90:118)

@@ -232,6 +232,58 @@ class OpawGPU:
     def proj1(self, pin):
         return self.proj(pin, True)
 
+    # ---------------------------------- host-array operators (Fortran drop-ins)
+    def _host_fields(self, pin) -> np.ndarray:
+        return np.ascontiguousarray(pin, dtype=np.complex128).reshape(-1, self.spec.n)
+
+    def h_phi_host(self, pin, sys: int = 0) -> np.ndarray:
+        """h_phi / h_phi_pert on complex*16 host arrays (nbatch, N)."""
+        x = self._host_fields(pin)
+        out = np.empty_like(x)
+        lib.call("opaw_h_phi_host", self.ctx, sys, _dp(x.view(np.float64)), _dp(out.view(np.float64)), x.shape[0])
+        return out
+
+    def shs_host(self, pin, sys: int = 0) -> np.ndarray:
+        x = self._host_fields(pin)
+        out = np.empty_like(x)
+        lib.call("opaw_shs_host", self.ctx, sys, _dp(x.view(np.float64)), _dp(out.view(np.float64)), x.shape[0])
+        return out
+
+    def sn_phi_host(self, pin, n: float) -> np.ndarray:
+        x = self._host_fields(pin)
+        out = np.empty_like(x)
+        lib.call("opaw_sn_phi_host", self.ctx, _dp(x.view(np.float64)), _dp(out.view(np.float64)), x.shape[0],
+                 float(n))
+        return out
+
+    def proj_host(self, ia: int, pin, ms: int, which_p1: bool = False) -> np.ndarray:
+        """proj(ia, pin, ca, ms, ik) with 1-based ia, one field."""
+        x = self._host_fields(pin)
+        ca = np.zeros(ms, dtype=np.complex128)
+        lib.call("opaw_proj_host", self.ctx, int(ia), _dp(x.view(np.float64)), 1 if which_p1 else 0,
+                 _dp(ca.view(np.float64)), int(ms))
+        return ca
+
+    def write_orbitals_c16(self, sys: int, phi):
+        x = self._host_fields(phi)
+        lib.call("opaw_write_orbitals_c16", self.ctx, sys, _dp(x.view(np.float64)))
+
+    def read_orbitals_c16(self, sys: int) -> np.ndarray:
+        out = np.empty((self.nb, self.spec.n), dtype=np.complex128)
+        lib.call("opaw_read_orbitals_c16", self.ctx, sys, _dp(out.view(np.float64)))
+        return out
+
+    def set_dt(self, dt: float):
+        lib.call("opaw_set_dt", self.ctx, float(dt))
+
+    def rk4_prop_shs_host(self, p: np.ndarray, pp: np.ndarray):
+        """rk4_prop_shs(p, pp, n, nb): p, pp complex128 (nb, N), updated in place."""
+        for a in (p, pp):
+            if a.dtype != np.complex128 or not a.flags.c_contiguous:
+                raise ValueError("p / pp must be C-contiguous complex128 (nb, N)")
+        lib.call("opaw_rk4_prop_shs_host", self.ctx, _dp(p.view(np.float64)), _dp(pp.view(np.float64)),
+                 int(self.spec.n), int(p.shape[0]))
+
     def fft3d(self, pin, direction: int, out=None):
         out = self.empty(pin.shape[0]) if out is None else out
         lib.call("opaw_fft3d", self.ctx, int(direction), self._ptr(pin), self._ptr(out), pin.shape[0])

@@ -64,6 +64,14 @@ SIGNATURES = {
     "opaw_sn_phi": (I, [P, P, P, I, D]),
     "opaw_shs": (I, [P, I, P, P, I]),
     "opaw_proj": (I, [P, P, I, I, P]),
+    "opaw_h_phi_host": (I, [P, I, DP, DP, I]),
+    "opaw_shs_host": (I, [P, I, DP, DP, I]),
+    "opaw_sn_phi_host": (I, [P, DP, DP, I, D]),
+    "opaw_proj_host": (I, [P, I, DP, I, DP, I]),
+    "opaw_write_orbitals_c16": (I, [P, I, DP]),
+    "opaw_read_orbitals_c16": (I, [P, I, DP]),
+    "opaw_set_dt": (I, [P, D]),
+    "opaw_rk4_prop_shs_host": (I, [P, DP, DP, C.c_longlong, I]),
     "opaw_fft3d": (I, [P, I, P, P, I]),
     "opaw_tddft_rhoij_2": (I, [P, I, P]),
     "opaw_tddft_dens_2": (I, [P, I, P]),
