@@ -7,7 +7,10 @@ occupied residual of 1e-8, hand-off through wf_bar.bin, RK4 propagation.
 Checks against the CPU oracle: the subspace eigenvalues on the GPU's final
 Hamiltonian (64^3), one full-size RK4 step of both systems from the ground
 state (64^3), and a 100-step dipole trace on the smallest grid at 0.5 bohr
-that holds the molecule's projector boxes (40 x 32 x 24).  The tolerances
+that holds the molecule's projector boxes (40 x 32 x 24).  And the mixed
+path against FP64 over the reference's naphthalene run length (2000 steps
+at dt = 0.05 from the 64^3 ground state): the dipole traces and the
+absorption peaks.  The tolerances
 of the steps carry the on-site XC's conditioning (v'' by a second
 difference with delta = 1e-4 in pawxcm: D_ij reproducible to ~1e-8, see
 tests/test_gpu_onsite.py)."""
@@ -128,3 +131,47 @@ def test_dipole_trace_100_steps(c1_small):
     err = np.linalg.norm(d_gpu - ref) / np.linalg.norm(ref)
     print(f"100-step d_dip relative L2 error {err:.2e}", flush=True)
     assert err <= 1e-8
+
+
+def test_2000_steps_mixed_vs_fp64(c1):
+    from oracle import spectrum_oracle as SO
+    from opaw_amd import spectrum
+    from opaw_amd.api import OpawGPU
+    spec = c1
+    gst = W.ground_state(spec, tol=1e-8, nscf=60)
+    nt, nocc = 2000, spec.nocc
+    dd = {}
+    for prec in ("f64", "mixed"):
+        print(f"propagating JTH C10H8 64^3 {prec} for {nt} steps", flush=True)
+        g = OpawGPU(spec, prec, nsys=2, nb=nocc)
+        g.write_orbitals(0, gst.phi if prec == "f64" else gst.phi.astype(np.complex64))
+        g.excite_psi_bar_pert(spec.sm, spec.ipol + 1)
+        dd[prec], _ = g.opaw_tddft(nt, spec.dt, spec.sm)
+        g.close()
+    err = np.linalg.norm(dd["mixed"] - dd["f64"]) / np.linalg.norm(dd["f64"])
+    print(f"dipole relative L2 (mixed vs f64, {nt} steps): {err:.3e}", flush=True)
+    assert err <= 1e-4
+    ne = 2 * nocc
+    v0 = dd["f64"][0, spec.ipol] / spec.dt
+    print(f"d_dip(dt)/dt = {v0:.4f} (free motion: -N_e = {-ne})", flush=True)
+    w = np.linspace(0.002, 3.0, 6000)   # Hartree
+    series = np.stack([dd["f64"][:, spec.ipol], dd["mixed"][:, spec.ipol]])
+    _, s = spectrum.kick_spectrum(series, spec.dt, w, damping="exp", tau=30.0)
+    fsum = s.sum(axis=1) * (w[1] - w[0])
+    print(f"f-sum over 0..3 Ha: f64 {fsum[0]:.3f}, mixed {fsum[1]:.3f}, N_e {ne}", flush=True)
+    assert abs(fsum[1] - fsum[0]) <= 1e-3 * abs(fsum[0])
+    # plausibility, not parity: free motion right after the kick (the
+    # projectors' nonlocality moves the velocity off -N_e by a few percent),
+    # most of the f-sum below 3 Ha, and the first bright band where
+    # naphthalene's pi -> pi* band lies (~5.9 eV measured)
+    assert abs(v0 + ne) < 0.1 * ne
+    assert abs(fsum[0] - ne) < 0.15 * ne
+    p64 = spectrum.peaks(w, s[0], rel_height=0.1)
+    p32 = spectrum.peaks(w, s[1], rel_height=0.1)
+    print("peaks (eV) f64  :", [round(p * SO.HARTREE_EV, 4) for p, _ in p64], flush=True)
+    print("peaks (eV) mixed:", [round(p * SO.HARTREE_EV, 4) for p, _ in p32], flush=True)
+    assert len(p64) >= 2 and len(p64) == len(p32)
+    assert 5.0 < p64[0][0] * SO.HARTREE_EV < 6.5
+    shift = max(abs(a - b) for (a, _), (b, _) in zip(p64, p32)) * SO.HARTREE_EV
+    print(f"largest peak shift: {shift:.2e} eV", flush=True)
+    assert shift <= 0.01
