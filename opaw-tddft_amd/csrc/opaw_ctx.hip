@@ -435,7 +435,8 @@ int opaw_create(opaw_ctx** out, int nx, int ny, int nz, double dx, double dy, do
   if (r == OPAW_OK) r = dalloc(c, &c->phi, nf * N * c->esize());
   if (r == OPAW_OK) r = dalloc_t(c, &c->pair, (size_t)((nsys + 1) / 2) * N);
   if (r == OPAW_OK) r = dalloc_t(c, &c->hwork, (size_t)c->gbig.nx * c->gbig.ny * c->gbig.nz);
-  if (r == OPAW_OK) r = dalloc_t(c, &c->vk, (size_t)c->gbig.nx * c->gbig.ny * c->gbig.nz);
+  if (r == OPAW_OK)
+    r = dalloc_t(c, &c->vk, (size_t)(c->gbig.nx / 2 + 1) * (c->gbig.ny / 2 + 1) * (c->gbig.nz / 2 + 1));
   if (r == OPAW_OK) r = dalloc_t(c, &c->gpart, gram_partials(nb));
   if (r == OPAW_OK) r = dalloc_t(c, &c->gmat, (size_t)nb * nb);
   if (r == OPAW_OK) r = dalloc_t(c, &c->tmat, (size_t)nb * nb);
@@ -575,7 +576,11 @@ __global__ void k_vk_finish(const c64* cvk, double* vk, int nx, int ny, int nz, 
     } else {
       re = (k2 > 1e-8) ? 4.0 * pi / k2 : 0.0;
     }
-    vk[i] = re / ((double)nx * (double)ny * (double)nz);
+    // vk is even in every k axis: only the octant 0..n/2 per axis is kept,
+    // (nx/2+1, ny/2+1, nz/2+1), and read at min(i, n - i)
+    if (ix <= nx / 2 && iy <= ny / 2 && iz <= nz / 2)
+      vk[ix + (long long)(nx / 2 + 1) * (iy + (long long)(ny / 2 + 1) * iz)] =
+          re / ((double)nx * (double)ny * (double)nz);
   }
 }
 

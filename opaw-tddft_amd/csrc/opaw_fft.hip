@@ -34,11 +34,28 @@ struct View {
   long long fs;   // field stride (elements)
   int sx, sy, sz; // coordinate offsets added before addressing
   int ldx, ldy;   // leading dimensions
+  // tables even in every k axis (the Hartree kernel vk): nonzero fold
+  // lengths store only the octant 0..n/2 per axis, read at min(i, n - i)
+  int fx = 0, fy = 0, fz = 0;
 };
 
 __device__ __forceinline__ long long vaddr(const View& v, int f, int x, int y, int z) {
   return (long long)f * v.fs + (long long)(x + v.sx) +
          (long long)v.ldx * ((long long)(y + v.sy) + (long long)v.ldy * (long long)(z + v.sz));
+}
+
+// address of a multiplier table entry, through the octant fold when set
+__device__ __forceinline__ long long taddr(const View& v, int x, int y, int z) {
+  if (v.fx) {
+    x += v.sx;
+    y += v.sy;
+    z += v.sz;
+    x = min(x, v.fx - x);
+    y = min(y, v.fy - y);
+    z = min(z, v.fz - z);
+    return (long long)x + (long long)v.ldx * ((long long)y + (long long)v.ldy * (long long)z);
+  }
+  return vaddr(v, 0, x, y, z);
 }
 
 struct XLines {
@@ -228,7 +245,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_s(const cx<R>* in, View vin,
       const int c = idx % XT, j = idx / XT;
       const int x = x0 + c;
       if (x < L.nx) {
-        const long long a = (AXIS == 1) ? vaddr(vtab, 0, x, j, o) : vaddr(vtab, 0, x, o, j);
+        const long long a = (AXIS == 1) ? taddr(vtab, x, j, o) : taddr(vtab, x, o, j);
         const R t = tab[a];
         res[idx] = mk(res[idx].x * t, res[idx].y * t);
       }
@@ -695,6 +712,15 @@ hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* in, cx
 // for one complex field rho (two real densities packed as re + i im; the
 // kernel vk is real and even so the two results do not mix).  work must hold
 // the doubled box.  Zero lines of the padded box are never touched.
+// the Hartree kernel vk is stored as its octant (NX/2+1, NY/2+1, NZ/2+1)
+static View vk_octant_view(int NX, int NY, int NZ) {
+  View v{0, 0, 0, 0, NX / 2 + 1, NY / 2 + 1};
+  v.fx = NX;
+  v.fy = NY;
+  v.fz = NZ;
+  return v;
+}
+
 hipError_t hartree_conv(const GridGeom& g, int scale, hipStream_t st, const c64* rho, c64* out,
                         c64* work, const double* vk, const c64* twx, const c64* twy,
                         const c64* twz) {
@@ -712,7 +738,8 @@ hipError_t hartree_conv(const GridGeom& g, int scale, hipStream_t st, const c64*
   if ((e = fft_s_pass<double>(KC_HARTREE, NY, 1, -1, EPI_NONE, st, work, big, work, big, Ly1, twy, nullptr, big))) return e;
   // z: forward, * vk, inverse; only the cropped z range is stored
   SLines Lz{NX, 0, NY, 1, oz, oz + nz, oz, oz + nz};
-  if ((e = fft_s_pass<double>(KC_HARTREE, NZ, 2, -1, EPI_MULT, st, work, big, work, big, Lz, twz, vk, big))) return e;
+  const View voct = vk_octant_view(NX, NY, NZ);
+  if ((e = fft_s_pass<double>(KC_HARTREE, NZ, 2, -1, EPI_MULT, st, work, big, work, big, Lz, twz, vk, voct))) return e;
   // y inverse: z in crop, store cropped y
   SLines Ly2{NX, oz, nz, 1, 0, NY, oy, oy + ny};
   if ((e = fft_s_pass<double>(KC_HARTREE, NY, 1, 1, EPI_NONE, st, work, big, work, big, Ly2, twy, nullptr, big))) return e;
@@ -875,7 +902,8 @@ hipError_t hartree_fused(const GridGeom& g, hipStream_t st, const HartreeRebuild
   View small{0, -nx / 2, -ny / 2, -oz, nx, ny};
   View big{0, 0, 0, 0, NX, NY};
   SLines Lz{NX, 0, NY, 1, oz, oz + nz, oz, oz + nz};
-  if ((e = fft_s_pass<double>(KC_HARTREE, NZ, 2, -1, EPI_MULT, st, work, big, work, big, Lz, twz, vk, big))) return e;
+  const View voct = vk_octant_view(NX, NY, NZ);
+  if ((e = fft_s_pass<double>(KC_HARTREE, NZ, 2, -1, EPI_MULT, st, work, big, work, big, Lz, twz, vk, voct))) return e;
   SLines Ly2{NX, oz, nz, 1, 0, NY, ny / 2, ny / 2 + ny};
   if ((e = fft_s_pass<double>(KC_HARTREE, NY, 1, 1, EPI_NONE, st, work, big, work, big, Ly2, twy, nullptr, big))) return e;
   XLines Lx2{ny / 2, ny, oz, nz, 1, 0, NX, nx / 2, nx / 2 + nx};

@@ -23,6 +23,8 @@
 #include <cstring>
 #include <string>
 #include <sys/stat.h>
+#include <new>
+#include <stdexcept>
 #include <vector>
 
 #include "opaw_hip.h"
@@ -171,7 +173,7 @@ bool same_real(double a, double b) { return std::fabs(a - b) <= 1e-10 * std::max
 extern "C" {
 
 int opaw_write_wf(const char* path, int binary, int short_header, int nx, int ny, int nz, double dx, double dy,
-                  double dz, int nstates, int ihomo, const double* eig, const void* phi) {
+                  double dz, int nstates, int ihomo, const double* eig, const void* phi) try {
   if (!path || !eig || !phi) return io_fail(OPAW_ERR_ARG, "null argument");
   if (nx <= 0 || ny <= 0 || nz <= 0 || nstates <= 0) return io_fail(OPAW_ERR_ARG, "bad shape");
   const long long N = (long long)nx * ny * nz;
@@ -251,10 +253,14 @@ int opaw_write_wf(const char* path, int binary, int short_header, int nx, int ny
   }
   ok = (std::fclose(f) == 0) && ok;
   return ok ? OPAW_OK : io_fail(OPAW_ERR_ARG, "write error on %s", path);
+} catch (const std::bad_alloc&) {
+  return io_fail(OPAW_ERR_ARG, "%s: out of host memory", "opaw_write_wf");
+} catch (const std::exception& e) {
+  return io_fail(OPAW_ERR_ARG, "%s: %s", "opaw_write_wf", e.what());
 }
 
 int opaw_read_wf(const char* path, int binary, int nx, int ny, int nz, double dx, double dy, double dz,
-                 int max_states, int* nstates, int* ihomo, double* eig, void* phi) {
+                 int max_states, int* nstates, int* ihomo, double* eig, void* phi) try {
   if (!path || !nstates) return io_fail(OPAW_ERR_ARG, "null argument");
   FILE* f = std::fopen(path, binary ? "rb" : "r");
   if (!f) return io_fail(OPAW_ERR_ARG, "cannot open %s: %s", path, std::strerror(errno));
@@ -276,12 +282,13 @@ int opaw_read_wf(const char* path, int binary, int nx, int ny, int nz, double dx
       if (!r.next() || !r.tag(t) || !r.r8(dv[k])) return bad("bad spacing header");
     if (!r.next() || !r.tag(t) || !r.i4(nsp)) return bad("bad nsp");
     if (!r.next() || !r.tag(t) || !r.i4(ns)) return bad("bad nstates");
+    if (ns < 0 || ns > max_states) return bad("nstates outside 0..max_states");
     // the reference's reader stops its header at nstates and skips the
     // eigenvalue tag, the eigenvalues and the orbital tag as three records;
     // files of writewf hold orbmn_wf and ihomo first
     if (!r.next() || !r.tag(t)) return bad("bad header");
     if (t == "orbmn_wf") {
-      r.i4(orbmn);
+      if (!r.i4(orbmn)) return bad("bad orbmn_wf");
       if (!r.next() || !r.tag(t) || !r.i4(ih)) return bad("bad ihomo");
       if (!r.next() || !r.tag(t)) return bad("missing evls");
     }
@@ -299,6 +306,7 @@ int opaw_read_wf(const char* path, int binary, int nx, int ny, int nz, double dx
       if (!r.next(t) || !r.r(dv[k])) return bad("bad spacing header");
     if (!r.next(t) || !r.i(nsp)) return bad("bad nsp");
     if (!r.next(t) || !r.i(ns)) return bad("bad nstates");
+    if (ns < 0 || ns > max_states) return bad("nstates outside 0..max_states");
     if (!r.next(t)) return bad("bad header");
     if (t == "orbmn_wf") {
       if (!r.i(orbmn) || !r.next(t) || !r.i(ih) || !r.next(t)) return bad("bad ihomo");
@@ -336,18 +344,26 @@ int opaw_read_wf(const char* path, int binary, int nx, int ny, int nz, double dx
   if (eig)
     for (int i = 0; i < ns; ++i) eig[i] = ev[(size_t)i];
   return OPAW_OK;
+} catch (const std::bad_alloc&) {
+  return io_fail(OPAW_ERR_ARG, "%s: out of host memory", "opaw_read_wf");
+} catch (const std::exception& e) {
+  return io_fail(OPAW_ERR_ARG, "%s: %s", "opaw_read_wf", e.what());
 }
 
-int opaw_write_eig(const char* path, int n, const double* eig) {
+int opaw_write_eig(const char* path, int n, const double* eig) try {
   if (!path || !eig || n <= 0) return io_fail(OPAW_ERR_ARG, "bad argument");
   FILE* f = std::fopen(path, "w");
   if (!f) return io_fail(OPAW_ERR_ARG, "cannot open %s: %s", path, std::strerror(errno));
   for (int i = 0; i < n; ++i) put_real(f, eig[i]);
   std::fprintf(f, "\n");
   return std::fclose(f) == 0 ? OPAW_OK : io_fail(OPAW_ERR_ARG, "write error on %s", path);
+} catch (const std::bad_alloc&) {
+  return io_fail(OPAW_ERR_ARG, "%s: out of host memory", "opaw_write_eig");
+} catch (const std::exception& e) {
+  return io_fail(OPAW_ERR_ARG, "%s: %s", "opaw_write_eig", e.what());
 }
 
-int opaw_read_eig(const char* path, int n, double* eig) {
+int opaw_read_eig(const char* path, int n, double* eig) try {
   if (!path || !eig || n <= 0) return io_fail(OPAW_ERR_ARG, "bad argument");
   FILE* f = std::fopen(path, "r");
   if (!f) return io_fail(OPAW_ERR_ARG, "cannot open %s: %s", path, std::strerror(errno));
@@ -359,10 +375,14 @@ int opaw_read_eig(const char* path, int n, double* eig) {
     }
   std::fclose(f);
   return OPAW_OK;
+} catch (const std::bad_alloc&) {
+  return io_fail(OPAW_ERR_ARG, "%s: out of host memory", "opaw_read_eig");
+} catch (const std::exception& e) {
+  return io_fail(OPAW_ERR_ARG, "%s: %s", "opaw_read_eig", e.what());
 }
 
 int opaw_write_dipoles(const char* d_dip_path, const char* dip_path, int append, int it0, int nt, double dt,
-                       const double* d_dip, const double* dip, int nsys) {
+                       const double* d_dip, const double* dip, int nsys) try {
   if (nt <= 0 || nsys <= 0) return io_fail(OPAW_ERR_ARG, "bad argument");
   const char* mode = append ? "a" : "w";
   if (d_dip_path && d_dip) {
@@ -387,6 +407,10 @@ int opaw_write_dipoles(const char* d_dip_path, const char* dip_path, int append,
     if (std::fclose(f) != 0) return io_fail(OPAW_ERR_ARG, "write error on %s", dip_path);
   }
   return OPAW_OK;
+} catch (const std::bad_alloc&) {
+  return io_fail(OPAW_ERR_ARG, "%s: out of host memory", "opaw_write_dipoles");
+} catch (const std::exception& e) {
+  return io_fail(OPAW_ERR_ARG, "%s: %s", "opaw_write_dipoles", e.what());
 }
 
 // restart files: system 0 -> phi_bar.bin, 1 -> phi_bar_pert.bin,
@@ -398,7 +422,7 @@ static std::string restart_file(const char* dir, int s) {
   return d + "/phi_bar_pert" + std::to_string(s) + ".bin";
 }
 
-int opaw_write_restart(opaw_ctx* ctx, const char* dir, int it) {
+int opaw_write_restart(opaw_ctx* ctx, const char* dir, int it) try {
   if (!ctx || !dir) return io_fail(OPAW_ERR_ARG, "null argument");
   int nsys = 0, nb = 0;
   long long n = 0;
@@ -430,9 +454,13 @@ int opaw_write_restart(opaw_ctx* ctx, const char* dir, int it) {
     if (!ok) return io_fail(OPAW_ERR_ARG, "write error on %s", fn.c_str());
   }
   return OPAW_OK;
+} catch (const std::bad_alloc&) {
+  return io_fail(OPAW_ERR_ARG, "%s: out of host memory", "opaw_write_restart");
+} catch (const std::exception& e) {
+  return io_fail(OPAW_ERR_ARG, "%s: %s", "opaw_write_restart", e.what());
 }
 
-int opaw_read_restart(opaw_ctx* ctx, const char* dir, int* it) {
+int opaw_read_restart(opaw_ctx* ctx, const char* dir, int* it) try {
   if (!ctx || !dir) return io_fail(OPAW_ERR_ARG, "null argument");
   int nsys = 0, nb = 0;
   long long n = 0;
@@ -469,6 +497,10 @@ int opaw_read_restart(opaw_ctx* ctx, const char* dir, int* it) {
   }
   if (it) *it = it_r;
   return OPAW_OK;
+} catch (const std::bad_alloc&) {
+  return io_fail(OPAW_ERR_ARG, "%s: out of host memory", "opaw_read_restart");
+} catch (const std::exception& e) {
+  return io_fail(OPAW_ERR_ARG, "%s: %s", "opaw_read_restart", e.what());
 }
 
 }  // extern "C"

@@ -31,6 +31,7 @@ there is no host fallback.
 from __future__ import annotations
 
 import ctypes as C
+import warnings
 from typing import Optional
 
 import numpy as np
@@ -61,6 +62,12 @@ class OpawGPU:
         import torch
         self.torch = torch
         self.spec = spec
+        standins = tuple((getattr(spec, "extra", None) or {}).get("standins", ()))
+        if standins:
+            # a dataset-built system whose tables include stand-ins for the
+            # reference's (v_loc on periodic grids): runs are not the reference's
+            warnings.warn(f"{getattr(spec, 'name', 'system')}: stand-in tables {standins} in place of the "
+                          "reference's", stacklevel=2)
         self.nsys = nsys
         self.nb = nb if nb is not None else spec.nocc
         self.N = spec.n

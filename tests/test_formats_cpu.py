@@ -251,3 +251,23 @@ def test_restart_records_and_info_match_reference(fortran, tmp_path):
     (tmp_path / "info").write_text(f" {NB_R} 1 {IT_R}\n")
     run("info", tmp_path / "info")
     assert struct.unpack("<3i", (d / "info.raw").read_bytes()) == (NB_R, 1, IT_R)
+
+
+@pytest.mark.parametrize("bad_ns", [-5, 2 ** 31 - 1])
+def test_wf_reader_rejects_corrupt_state_count(tmp_path, bad_ns):
+    """A negative or huge nstates record is refused before anything is
+    allocated: an error status, not an exception through the C ABI."""
+    _, eig, _ = _data()
+    phi, _ = _phi()
+    out = tmp_path / "wf.bin"
+    formats.write_wf(out, phi, eig, (NX, NY, NZ), (DX, DY, DZ), ihomo=2, binary=True, short_header=False)
+    raw = bytearray(out.read_bytes())
+    off = 0
+    for _ in range(7):                       # nx ny nz, dx dy dz, nsp
+        n = struct.unpack_from("<i", raw, off)[0]
+        off += 8 + n
+    n = struct.unpack_from("<i", raw, off)[0]   # the nstates record: tag, then the i4 value
+    struct.pack_into("<i", raw, off + 4 + n - 4, bad_ns)
+    out.write_bytes(bytes(raw))
+    with pytest.raises(lib.OpawError, match="nstates outside"):
+        formats.read_wf(out, (NX, NY, NZ), (DX, DY, DZ), 8)
