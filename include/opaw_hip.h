@@ -422,6 +422,11 @@ int opaw_precond_clear(opaw_ctx* ctx);
 int opaw_precond_count(opaw_ctx* ctx, int* nsnap, int* nready);
 int opaw_precond_loss(opaw_ctx* ctx, const double* mlp, int nh, double* loss, double* grad);
 int opaw_precond_pair(opaw_ctx* ctx, int idx, void* rk, void* ek, double* t, double* y, int* nb);
+/* Whether opaw_precond_prepare keeps each pair's k-space fields (default 1,
+ * for opaw_precond_pair) or frees them after reducing them to the shell sums
+ * the loss and gradient use (0: two band sets of device memory less per pair,
+ * for training sets of many large systems). */
+int opaw_precond_keep_fields(opaw_ctx* ctx, int keep);
 
 /* ---- profiling: HIP events around the launches of selected kernel families
  * mask: bit k enables family k (OPAW_K_*); 0 disables, -1 enables all. */

@@ -279,6 +279,7 @@ struct opaw_ctx {
     bool ready = false;
   };
   std::vector<Snap> snaps;
+  bool snap_keep = true;   // keep a prepared pair's k-space fields (opaw_precond_pair); else only its shell sums
   // k shells (sets of k points with equal ek) for the training sums and the
   // preconditioner table
   DevBuf shPerm, shOff, shEk;
@@ -3073,6 +3074,12 @@ int opaw_precond_prepare(opaw_ctx* c, int sys) {
     }
     HCHK(hipStreamSynchronize(c->st));
     sp.ready = true;
+    if (!c->snap_keep) {
+      dfree(c, sp.phi.p);
+      dfree(c, sp.r.p);
+      sp.phi = DevBuf{};
+      sp.r = DevBuf{};
+    }
   }
   HCHK(hipStreamSynchronize(c->st));
   return OPAW_OK;
@@ -3202,6 +3209,7 @@ int opaw_precond_pair(opaw_ctx* c, int idx, void* rk, void* ek, double* t, doubl
     if (!sp.ready) continue;
     if (++k != idx) continue;
     const size_t fb = (size_t)sp.nb * c->N * sizeof(c64);
+    if ((rk || ek) && !sp.r.p) return fail(OPAW_ERR_STATE, "pair %d kept only its shell sums", idx);
     HCHK(hipStreamSynchronize(c->st));
     if (rk) HCHK(hipMemcpy(rk, sp.r.p, fb, hipMemcpyDeviceToHost));
     if (ek) HCHK(hipMemcpy(ek, sp.phi.p, fb, hipMemcpyDeviceToHost));
@@ -3211,6 +3219,12 @@ int opaw_precond_pair(opaw_ctx* c, int idx, void* rk, void* ek, double* t, doubl
     return OPAW_OK;
   }
   return fail(OPAW_ERR_ARG, "no prepared pair %d", idx);
+}
+
+int opaw_precond_keep_fields(opaw_ctx* c, int keep) {
+  CTX_CHECK(c);
+  c->snap_keep = keep != 0;
+  return OPAW_OK;
 }
 
 int opaw_mlp_param_count(int nh) {

@@ -387,6 +387,9 @@ class OpawGPU:
     def precond_prepare(self, sys: int = 0):
         lib.call("opaw_precond_prepare", self.ctx, sys)
 
+    def precond_keep_fields(self, keep: bool):
+        lib.call("opaw_precond_keep_fields", self.ctx, 1 if keep else 0)
+
     def precond_clear(self):
         lib.call("opaw_precond_clear", self.ctx)
 

@@ -79,11 +79,20 @@ def scf_chebyshev(g, nocc: int, nscf: int = 30, tol: float = 1e-8, degree: int =
 
 
 def scf_block(g, nocc: int, nscf: int = 30, tol: float = 1e-8, inner: int = 3, mlp=None, nh: int = 16,
-              record: int = 0, eta: float = 0.0, inner_max: int = 30) -> ScfResult:
+              record: int = 0, eta: float = 0.0, inner_max: int = 30, first_tol: float = 1e-2,
+              dens0: Optional[np.ndarray] = None, record_max: int = 0) -> ScfResult:
     """SCF with the block preconditioned solver in place of the filter.
-    eta = 0: `inner` LOBPCG iterations per SCF iteration; eta > 0: each
+    Each iteration builds H from the current orbitals (iteration 1 from
+    dens0 when given, with rho_ij = 0) and runs the block solver on it.  The
+    solver's first residual is that of the orbitals it starts from, under
+    the H of their own density: the SCF has converged when it is <= tol
+    (iteration 2 on).
+    eta = 0: `inner` solver iterations per SCF iteration; eta > 0: each
     inner solve runs (up to inner_max iterations) to max(tol, eta * the
-    previous SCF residual), so the preconditioner's quality sets the work."""
+    current SCF residual estimate), the previous iteration's, first_tol at
+    iteration 1, so the preconditioner's quality sets the work.
+    record_max > 0: no further training pairs once the context holds that
+    many (each holds two band sets until prepared)."""
     g.torch.cuda.synchronize()
     t0 = time.perf_counter()
     hist = []
@@ -92,22 +101,26 @@ def scf_block(g, nocc: int, nscf: int = 30, tol: float = 1e-8, inner: int = 3, m
     it = 0
     g.orthog_phi(0, 1)
     prev = None
+    res = np.inf
     for it in range(1, nscf + 1):
-        g.ncpaw_make_hamiltonian(0, diis=True)
-        if eta > 0.0:
-            inner_tol = tol if prev is None else max(tol, eta * prev)
-            eig, h, its = g.block_solve(inner_max, inner_tol, mlp=mlp, nh=nh, record=record, nconv=nocc)
+        if it == 1 and dens0 is not None:
+            g.ncpaw_make_hamiltonian_density(dens0, 0, diis=True)
         else:
-            eig, h, its = g.block_solve(inner, 0.0, mlp=mlp, nh=nh, record=record, nconv=nocc)
+            g.ncpaw_make_hamiltonian(0, diis=True)
+        rec = record if not (record_max and g.precond_count()[0] >= record_max) else 0
+        if eta > 0.0:
+            inner_tol = max(tol, first_tol) if prev is None else max(tol, eta * prev)
+            eig, h, its = g.block_solve(inner_max, inner_tol, mlp=mlp, nh=nh, record=rec, nconv=nocc)
+        else:
+            eig, h, its = g.block_solve(inner, 0.0, mlp=mlp, nh=nh, record=rec, nconv=nocc)
         napp += g.nb * (its + 1)
-        res = _occupied_residual(g, eig, nocc)
-        napp += g.nb
+        res = float(h[0])              # the start orbitals under the H of their density
         hist.append(res)
         prev = res
-        if res <= tol:
+        if it > 1 and res <= tol:
             break
     g.torch.cuda.synchronize()
-    return ScfResult(eig=eig, residual=hist[-1], iterations=it, seconds=time.perf_counter() - t0, history=hist,
+    return ScfResult(eig=eig, residual=res, iterations=it, seconds=time.perf_counter() - t0, history=hist,
                      h_applications=napp)
 
 
@@ -123,8 +136,23 @@ def mlp_init(nh: int = 16, seed: int = 0, scale: float = 0.1) -> np.ndarray:
 
 def train_preconditioner(g, nh: int = 16, steps: int = 300, lr: float = 1e-2, seed: int = 0,
                          params: Optional[np.ndarray] = None, log_every: int = 0):
-    """Adam on the misalignment loss of the prepared pairs.  Returns
-    (params, losses)."""
+    """Adam on the misalignment loss of the prepared pairs of one context,
+    or of several (g a list: the sum of their losses).  Returns (params,
+    losses)."""
+    gl = g if isinstance(g, (list, tuple)) else [g]
+
+    class _Sum:
+        @staticmethod
+        def precond_loss(q, nh, grad=True):
+            tot, gt = 0.0, (np.zeros_like(q) if grad else None)
+            for h in gl:
+                l, gr = h.precond_loss(q, nh, grad=grad)
+                tot += l
+                if grad:
+                    gt += gr
+            return tot, gt
+
+    g = _Sum
     p = mlp_init(nh, seed) if params is None else np.array(params, dtype=np.float64)
     m = np.zeros_like(p)
     v = np.zeros_like(p)
@@ -146,6 +174,95 @@ def train_preconditioner(g, nh: int = 16, steps: int = 300, lr: float = 1e-2, se
     if loss < best[0]:
         best = (loss, p.copy())
     return best[1], losses
+
+
+# polyacene training and evaluation sets of the learned preconditioner:
+# rings -> grid at 0.5 bohr (the batched workload's grids for n = 1..6, 128^3
+# for the held-out n = 7, 8)
+ACENE_GRIDS = {1: (48, 48, 48), 2: (64, 48, 48), 3: (80, 64, 48), 4: (96, 64, 48), 5: (96, 64, 48),
+               6: (128, 64, 48), 7: (128, 128, 128), 8: (128, 128, 128)}
+
+
+def acene_context(nring: int, directory: str, distort=None, grid=None, seed: int = 2024):
+    """GPU context of a (distorted) JTH polyacene with the reference's band
+    count and seeded random start orbitals; returns (context, nocc, dens0)."""
+    from . import synth, workloads as W
+    from .api import OpawGPU
+    spec = W.acene_system(nring, grid or ACENE_GRIDS[nring], 0.5, directory, distort=distort,
+                          name=f"acene{nring}" + (f"-d{distort[1]}" if distort else ""))
+    nocc = spec.nocc
+    nb = W.nbands(nocc)
+    spec.rnel = 2.0 * nocc
+    phi = synth.lowdin_orthonormalise(synth.random_orbitals(spec, nb, seed=seed, envelope=4.0), spec.dv)
+    g = OpawGPU(spec, "f64", nsys=1, nb=nb)
+    g.set_nocc(nocc)
+    g.set_diis(4, 0.15)
+    g.write_orbitals(0, phi)
+    return g, nocc, spec.extra["dens0"]
+
+
+def acene_hamiltonian(nring: int, directory: str, distort=None, grid=None, seed: int = 2024):
+    """A polyacene context whose H is the reference's SCF iteration 1 (the
+    atomic superposition density, rho_ij = 0), with seeded random start
+    orbitals: the fixed operator the preconditioned eigensolver is judged
+    on.  Returns (context, nocc)."""
+    g, nocc, d0 = acene_context(nring, directory, distort=distort, grid=grid, seed=seed)
+    g.ncpaw_make_hamiltonian_density(d0, 0, diis=False)
+    g.orthog_phi(0, 1)
+    return g, nocc
+
+
+def train_on_acenes(directory: str, rings=(1, 2, 3, 4, 5, 6), sigma: float = 0.1, seed: int = 7, nh: int = 8,
+                    steps: int = 80, record: int = 2, tol: float = 1e-9, maxit: int = 300, log=print):
+    """Training pairs from Teter-Payne-Allan solves (to tol) of the
+    iteration-1 Hamiltonians of seeded sigma-bohr distortions of the
+    polyacenes `rings`, every `record`-th iteration, then Adam on the sum of
+    their losses.  Returns (params, losses)."""
+    gs_ = []
+    for n in rings:
+        g, nocc = acene_hamiltonian(n, directory, distort=(sigma, seed + n))
+        g.precond_keep_fields(False)
+        t0 = time.perf_counter()
+        _, hist, its = g.block_solve(maxit, tol, record=record, nconv=nocc)
+        g.precond_prepare(0)
+        log(f"[c5] train acene{n} (nb {g.nb}, nocc {nocc}): {its} iterations to {hist[-1]:.1e} in "
+            f"{time.perf_counter() - t0:.2f} s, {g.precond_count()[1]} pairs")
+        gs_.append(g)
+    t0 = time.perf_counter()
+    P, losses = train_preconditioner(gs_, nh=nh, steps=steps, lr=2e-2, seed=1)
+    log(f"[c5] {steps} Adam steps in {time.perf_counter() - t0:.1f} s, loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    for g in gs_:
+        g.close()
+    return P, losses
+
+
+@dataclass
+class SolveResult:
+    iterations: int
+    seconds: float
+    residual: float
+    h_applications: int
+    history: np.ndarray
+
+
+def evaluate_on_acene(nring: int, directory: str, P, nh: int = 8, tol: float = 1e-8, maxit: int = 400, grid=None,
+                      log=print):
+    """The eigensolver on the undistorted polyacene's iteration-1 H, from
+    the same start, to an occupied residual of tol: Teter-Payne-Allan and
+    the MLP.  Returns {name: SolveResult}."""
+    out = {}
+    for name, p in (("tpa", None), ("mlp", P)):
+        g, nocc = acene_hamiltonian(nring, directory, grid=grid)
+        g.torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, hist, its = g.block_solve(maxit, tol, mlp=p, nh=nh, nconv=nocc)
+        g.torch.cuda.synchronize()
+        r = SolveResult(its, time.perf_counter() - t0, float(hist[-1]), g.nb * (its + 1), hist)
+        g.close()
+        log(f"[c5] acene{nring} {name}: {r.iterations} iterations, {r.seconds:.2f} s, residual {r.residual:.1e}, "
+            f"{r.h_applications} band H applications")
+        out[name] = r
+    return out
 
 
 # the held-out comparison of the solvers: grid -> (rings, bands, occupied)

@@ -71,6 +71,7 @@ SIGNATURES = {
     "opaw_write_orbitals_c16": (I, [P, I, DP]),
     "opaw_read_orbitals_c16": (I, [P, I, DP]),
     "opaw_set_dt": (I, [P, D]),
+    "opaw_precond_keep_fields": (I, [P, I]),
     "opaw_rk4_prop_shs_host": (I, [P, DP, DP, C.c_longlong, I]),
     "opaw_fft3d": (I, [P, I, P, P, I]),
     "opaw_tddft_rhoij_2": (I, [P, I, P]),

@@ -67,10 +67,15 @@ def jth_datasets(geom: paw.Geometry, directory: str) -> dict:
             for s in ("H", "C")}
 
 
-def acene_system(nring: int, n: Tuple[int, int, int], d: float, directory: str, name: Optional[str] = None):
+def acene_system(nring: int, n: Tuple[int, int, int], d: float, directory: str, name: Optional[str] = None,
+                 distort: Optional[Tuple[float, int]] = None):
     """A polyacene on an n grid of spacing d, nonperiodic (Martyna-Tuckerman),
-    nocc = (valence electrons) / 2."""
+    nocc = (valence electrons) / 2.  distort = (sigma, seed): every atom
+    moved by a seeded Gaussian of standard deviation sigma (bohr) per axis."""
     syms, xyz = acene_geometry(nring)
+    if distort is not None:
+        sigma, seed = distort
+        xyz = xyz + np.random.default_rng(seed).normal(0.0, sigma, xyz.shape)
     geom = paw.Geometry(n[0], n[1], n[2], d, d, d, False)
     ds = jth_datasets(geom, directory)
     nval = int(round(sum(ds[s].val for s in syms)))
