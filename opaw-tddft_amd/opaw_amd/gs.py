@@ -287,7 +287,9 @@ def main(argv=None):
     bands, 6 occupied; 48^3: two rings, 16 bands, 12 occupied): the
     Chebyshev-filter SCF, LOBPCG with Teter-Payne-Allan, and LOBPCG with the
     learned preconditioner, trained on the pairs of one other system's SCF.
-    python -m opaw_amd.gs [--grid 48] [--methods cheb tpa mlp]"""
+    python -m opaw_amd.gs [--grid 48] [--methods cheb tpa mlp]
+    The polyacene workload: python -m opaw_amd.gs --acenes [--eval 7 8]
+    [--nh 16 --steps 300]."""
     import argparse
     ap = argparse.ArgumentParser(description=main.__doc__)
     ap.add_argument("--grid", type=int, choices=sorted(SCF_SYSTEMS), default=48)
@@ -296,7 +298,33 @@ def main(argv=None):
     ap.add_argument("--train-seed", type=int, default=21)
     ap.add_argument("--nh", type=int, default=8, help="MLP hidden width")
     ap.add_argument("--steps", type=int, default=80, help="Adam steps")
+    ap.add_argument("--acenes", action="store_true",
+                    help="the polyacene workload: train on distorted n = 1..6, evaluate on --eval rings at 128^3")
+    ap.add_argument("--eval", type=int, nargs="+", default=[7])
+    ap.add_argument("--solve", type=int, default=0, metavar="ITERS",
+                    help="only run ITERS Teter-Payne-Allan solver iterations on the first --eval polyacene (profiling)")
     a = ap.parse_args(argv)
+    import os
+    gold = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))),
+                        "tests", "golden", "jth")
+    if a.solve:
+        g, nocc = acene_hamiltonian(a.eval[0], gold)
+        g.torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, hist, its = g.block_solve(a.solve, 0.0, nconv=nocc)
+        g.torch.cuda.synchronize()
+        print(f"[gs] acene{a.eval[0]} ({g.nb} bands): {its} iterations in {time.perf_counter() - t0:.2f} s, "
+              f"residual {hist[-1]:.1e}", flush=True)
+        g.close()
+        return
+    if a.acenes:
+        P, _ = train_on_acenes(gold, nh=a.nh, steps=a.steps, log=lambda s: print(s, flush=True))
+        for n in a.eval:
+            out = evaluate_on_acene(n, gold, P, nh=a.nh, log=lambda s: print(s, flush=True))
+            t, m = out["tpa"], out["mlp"]
+            print(f"[gs] acene{n}: wall-time ratio TPA/MLP {t.seconds / m.seconds:.3f}, iterations "
+                  f"{t.iterations} / {m.iterations}", flush=True)
+        return
     P = None
     if "mlp" in a.methods:
         g, nocc = _scf_system(a.grid, a.train_seed)
