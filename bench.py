@@ -27,8 +27,9 @@ synthetic 96^3 periodic workload (40 atoms x 8 projectors) in FP64 and FP32
 against its 5 s_c N bytes-per-orbital model; `batched_polyacenes` reports
 the batched uneven workload: the polyacenes n = 1..6 on their own grids, 4 systems each, mixed
 path, all advanced together by opaw_tddft_run_batch, against the same
-molecules run one after another; `coronene` the C24H12 96^3 workload with
-kicks along x, y, z in both precisions.
+molecules run one after another; `coronene` the C24H12 96^3 workload from
+the JTH datasets and its GPU ground state, with kicks along x, y, z in both
+precisions.
 
 Prints ONE JSON line.
 """
@@ -243,15 +244,20 @@ def run_batch(steps):
 
 
 def run_coronene(steps):
-    """Coronene C24H12 (D6h, C-C 2.68, C-H 2.05 bohr), 96^3 @0.45 bohr MT, 54
-    orbitals, 4 systems (unperturbed + kicks along x, y, z), FP64 and mixed
-    from the same start: steps/s of each and the mixed kick responses
-    against FP64 (relative L2 over every step run)."""
+    """Coronene C24H12 (D6h, C-C 2.68, C-H 2.05 bohr) from the JTH datasets,
+    96^3 @0.45 bohr MT, 54 orbitals from its GPU ground state, 4 systems
+    (unperturbed + kicks along x, y, z), FP64 and mixed from the same start:
+    steps/s of each and the mixed kick responses against FP64 (relative L2
+    over every step run)."""
     import numpy as np
     import torch
-    from opaw_amd import synth, batch
-    spec = synth.molecule_system(geometry="coronene", n=(96, 96, 96), d=0.45, seed=77)
-    phi0 = synth.lowdin_orthonormalise(synth.random_orbitals(spec, spec.nocc, seed=17, envelope=5.0), spec.dv)
+    from opaw_amd import batch, workloads as W
+    t0 = time.perf_counter()
+    spec = W.coronene_system((96, 96, 96), 0.45, GOLD)
+    gst = W.ground_state(spec, tol=1e-8, nscf=80, degree=16)
+    ground = {"scf_iterations": gst.scf.iterations, "occupied_residual": gst.scf.residual,
+              "setup_and_scf_seconds": time.perf_counter() - t0, "bands": len(gst.eig)}
+    phi0 = gst.phi
     out, resp = {}, {}
     for prec in ("f64", "mixed"):
         g = batch.kicked_context(spec, phi0, prec, 4)
@@ -267,8 +273,9 @@ def run_coronene(steps):
         resp[prec] = batch.kick_responses(np.concatenate([d0, d1]), spec.sm)
         g.close()
     err = float(np.linalg.norm(resp["mixed"] - resp["f64"]) / np.linalg.norm(resp["f64"]))
-    return {"workload": "coronene C24H12, 96^3 MT @0.45 bohr, 54 orbitals x 4 systems (kicks x, y, z)",
-            "steps": steps, "fp64": out["f64"], "mixed": out["mixed"],
+    return {"workload": "coronene C24H12 D6h from the JTH datasets, 96^3 MT @0.45 bohr, 54 orbitals from the "
+                        "GPU ground state x 4 systems (kicks x, y, z)",
+            "ground_state": ground, "steps": steps, "fp64": out["f64"], "mixed": out["mixed"],
             "mixed_over_fp64": out["mixed"]["steps_per_s"] / out["f64"]["steps_per_s"],
             "kick_response_rel_l2_mixed_vs_fp64": err}
 

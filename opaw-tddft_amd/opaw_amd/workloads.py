@@ -4,7 +4,8 @@ datasets, and the ground state that starts their propagation.
 acene_geometry: polyacenes C(4n+2)H(2n+4) with regular hexagons of C-C bond
 a and C-H bond h along the outward ring radius, rings fused along x, in the
 z = 0 plane, centred at the origin (bohr).  n = 2 is naphthalene in its D2h
-form (a = 2.65, h = 2.05 bohr for the C10H8 workload).
+form (a = 2.65, h = 2.05 bohr for the C10H8 workload).  coronene_system:
+C24H12 in its D6h form (C-C 2.68, C-H 2.05 bohr).
 
 ground_state: the reference's main program up to writewf
 (src/main.f90:46-112): nb = max(ceil(1.3 nocc) + 5, 20) bands
@@ -82,6 +83,20 @@ def acene_system(nring: int, n: Tuple[int, int, int], d: float, directory: str, 
     return paw.system_from_datasets(geom, syms, xyz, ds, nocc=nval // 2, name=name or f"acene{nring}-jth")
 
 
+def coronene_system(n: Tuple[int, int, int], d: float, directory: str, name: Optional[str] = None,
+                    cc: float = 2.68, ch: float = 2.05):
+    """Coronene C24H12 in its D6h form (C-C cc, C-H ch bohr) from the JTH
+    datasets on an n grid of spacing d, nonperiodic (Martyna-Tuckerman);
+    nocc = 54."""
+    C, H = synth.coronene_geometry(cc=cc, ch=ch)
+    syms = ["C"] * len(C) + ["H"] * len(H)
+    xyz = np.concatenate([C, H])
+    geom = paw.Geometry(n[0], n[1], n[2], d, d, d, False)
+    ds = jth_datasets(geom, directory)
+    nval = int(round(sum(ds[s].val for s in syms)))
+    return paw.system_from_datasets(geom, syms, xyz, ds, nocc=nval // 2, name=name or "coronene-jth")
+
+
 def nbands(nocc: int) -> int:
     """get_rnel (src/rnel.f90:17-21) with rnel_1 = 1.3, rnel_2 = 5, rnel_3 = 20."""
     return max(math.ceil(nocc * 1.3) + 5, 20)
@@ -107,8 +122,10 @@ class GroundState:
 
 
 def ground_state(spec, tol: float = 1e-8, nscf: int = 60, seed: int = 2024, nb: Optional[int] = None,
-                 wf_path: Optional[str] = None) -> GroundState:
-    """The ground-state SCF on the GPU (FP64, h_type 1).  With wf_path the
+                 wf_path: Optional[str] = None, degree: int = 10) -> GroundState:
+    """The ground-state SCF on the GPU (FP64, h_type 1), Chebyshev filter of
+    the given degree (the reference's nch; 10 by default, 16 for coronene at
+    0.45 bohr, whose SCF stalls near 1e-7 with 10).  With wf_path the
     occupied orbitals are written there in the wf_bar.bin layout and read
     back, the hand-over the reference makes (src/write_wf.f90:122-219,
     src/read_wf.f90:130-228)."""
@@ -124,7 +141,7 @@ def ground_state(spec, tol: float = 1e-8, nscf: int = 60, seed: int = 2024, nb: 
         g.set_nocc(nocc)
         g.set_diis(4, 0.15)
         g.write_orbitals(0, phi)
-        res = gs.scf_chebyshev(g, nocc, nscf=nscf, tol=tol, dens0=spec.extra.get("dens0"))
+        res = gs.scf_chebyshev(g, nocc, nscf=nscf, tol=tol, degree=degree, dens0=spec.extra.get("dens0"))
         out = real_phase(g.read_orbitals(0)[:nocc])
     finally:
         g.close()
