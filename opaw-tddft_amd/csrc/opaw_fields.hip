@@ -799,6 +799,92 @@ __global__ __launch_bounds__(64 * RB) void k_gram_mfma(const c64* __restrict__ a
     }
 }
 
+// More than 64 FP64 fields: the same product on 64 x 64 tiles of G, one
+// workgroup (four waves, each 16 rows of the tile) per (tile pair, point
+// chunk).  Tile pairs vary fastest in the workgroup index, so the tiles of
+// one chunk run together and share the chunk's operands through the caches.
+// sym (a == b): only the tiles I <= J; an off-diagonal tile also writes its
+// conjugate transpose, G_ji = conj(G_ij), which is the same sum of the same
+// products.
+__global__ __launch_bounds__(256) void k_gram_mfma_tiles(const c64* __restrict__ a, const c64* __restrict__ b,
+                                                          long long N, int m, int T, int ntp, int sym, int nchunk,
+                                                          c64* __restrict__ part) {
+  constexpr int P = 16, M = 64, NT = 256;
+  constexpr int SL = M * P / NT;   // loader slots per thread: 4
+  __shared__ c64 sa[M][P + 1], sb[M][P + 1];
+  const int tp = blockIdx.x % ntp;
+  const int c = blockIdx.x / ntp;
+  int I, J;
+  if (sym) {
+    int k = tp;
+    I = 0;
+    while (k >= T - I) {
+      k -= T - I;
+      ++I;
+    }
+    J = I + k;
+  } else {
+    I = tp / T;
+    J = tp % T;
+  }
+  const int ia0 = M * I, jb0 = M * J;
+  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int lr = lane % 16, lk = lane / 16;
+  long long lo, hi;
+  chunk_range(N, nchunk, c, lo, hi);
+  c64 va[SL], vb[SL];
+  auto load = [&](long long r0) {
+#pragma unroll
+    for (int q = 0; q < SL; ++q) {
+      const int slot = (int)threadIdx.x + NT * q;
+      const int o = slot / P, p = slot % P;
+      const long long r = min(r0 + p, hi - 1);
+      va[q] = a[(long long)min(ia0 + o, m - 1) * N + r];
+      vb[q] = b[(long long)min(jb0 + o, m - 1) * N + r];
+    }
+  };
+  f64x4 gr[4], gi[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+    for (int i = 0; i < 4; ++i) gr[v][i] = gi[v][i] = 0.0;
+  if (lo < hi) load(lo);
+  for (long long r0 = lo; r0 < hi; r0 += P) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < SL; ++q) {
+      const int slot = (int)threadIdx.x + NT * q;
+      const int o = slot / P, p = slot % P;
+      const bool inr = r0 + p < hi;
+      sa[o][p] = (inr && ia0 + o < m) ? va[q] : mk(0.0, 0.0);
+      sb[o][p] = (inr && jb0 + o < m) ? vb[q] : mk(0.0, 0.0);
+    }
+    __syncthreads();
+    if (r0 + P < hi) load(r0 + P);
+#pragma unroll
+    for (int kk = 0; kk < P / 4; ++kk) {
+      const c64 x = sa[16 * w + lr][4 * kk + lk];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const c64 y = sb[16 * v + lr][4 * kk + lk];
+        gr[v] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.x, gr[v], 0, 0, 0);
+        gr[v] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, y.y, gr[v], 0, 0, 0);
+        gi[v] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.y, gi[v], 0, 0, 0);
+        gi[v] = __builtin_amdgcn_mfma_f64_16x16x4f64(-x.y, y.x, gi[v], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int i = ia0 + 16 * w + lk + 4 * e, j = jb0 + 16 * v + lr;
+      if (i < m && j < m) {
+        part[((long long)c * m + i) * m + j] = mk(gr[v][e], gi[v][e]);
+        if (sym && I != J) part[((long long)c * m + j) * m + i] = mk(gr[v][e], -gi[v][e]);
+      }
+    }
+}
+
 // g[q] = dv * sum over chunks c of part[c * nb * nb + q].  A workgroup takes
 // kGfQ consecutive entries with kGfL lanes each: lane l sums chunks l,
 // l + kGfL, ... (the kGfQ entries of one chunk are adjacent in memory), and
@@ -1267,8 +1353,16 @@ template <typename R>
 hipError_t gram(hipStream_t st, const cx<R>* a, const cx<R>* b, long long N, int nb, double dv, c64* part, c64* g) {
   const int t = (nb + 15) / 16;
   if (t > 4) {
-    hipLaunchKernelGGL((k_gram_partial<R>), dim3(t, t, kGramChunks), dim3(256), 0, st, a, b, N, nb, kGramChunks,
-                       part);
+    if constexpr (std::is_same<R, double>::value) {
+      const int T = (nb + 63) / 64;
+      const int sym = (const void*)a == (const void*)b;
+      const int ntp = sym ? T * (T + 1) / 2 : T * T;
+      hipLaunchKernelGGL(k_gram_mfma_tiles, dim3(ntp * kGramChunks), dim3(256), 0, st, a, b, N, nb, T, ntp, sym,
+                         kGramChunks, part);
+    } else {
+      hipLaunchKernelGGL((k_gram_partial<R>), dim3(t, t, kGramChunks), dim3(256), 0, st, a, b, N, nb, kGramChunks,
+                         part);
+    }
     hipLaunchKernelGGL(k_gram_finish, dim3((nb * nb + kGfQ - 1) / kGfQ), dim3(kGfQ * kGfL), 0, st, part, nb,
                        kGramChunks, dv, g);
     return hipGetLastError();

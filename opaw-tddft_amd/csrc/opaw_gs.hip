@@ -314,58 +314,73 @@ __global__ __launch_bounds__(kTile) void k_precond_grad(const c64* __restrict__ 
   }
 }
 
-// out_j = sum_{i<m} T[i + m j] in_i for j < nout (FP64 fields).  A thread
-// keeps kRotJ outputs of one point, so for nout <= kRotJ every input is read
-// once; the coefficients go through LDS, kRotRows rows of T at a time, and
-// four input fields are loaded before their products are added.
-constexpr int kRotJ = 16, kRotTB = 128, kRotRows = 64;
-__global__ __launch_bounds__(kRotTB) void k_rotate_rect(const c64* __restrict__ in, c64* __restrict__ out,
-                                                        long long N, int m, int nout, const c64* __restrict__ T) {
-  __shared__ c64 sT[kRotRows * kRotJ];   // sT[(i - i0) * kRotJ + q] = T[i + m (j0 + q)]
-  const int j0 = blockIdx.y * kRotJ;
-  const long long r = (long long)blockIdx.x * kRotTB + threadIdx.x;
-  const bool live = r < N;
-  double ar[kRotJ], ai[kRotJ];
+// out_j = sum_{i<m} T[i + m j] in_i for j < nout (FP64 fields, T column-major)
+// on the matrix cores: per workgroup 64 outputs x 64 points, four waves of
+// 16 outputs each.  The inputs and the coefficients of
+// 16 inputs at a time go through LDS (the coefficients with the input index
+// fastest: a column of T is contiguous), the next 16 are loaded into
+// registers meanwhile.  Complex products as four real 16x16x4 FP64 MFMAs:
+// Re += Tr Ir - Ti Ii, Im += Tr Ii + Ti Ir.  Lane l of a product holds
+// output row l/16 + 4e of point column l%16.
+typedef double f64x4r __attribute__((ext_vector_type(4)));
+constexpr int kRmP = 64, kRmJ = 64, kRmK = 16;
+__global__ __launch_bounds__(256) void k_rotate_mfma(const c64* __restrict__ in, c64* __restrict__ out, long long N,
+                                                      int m, int nout, const c64* __restrict__ T) {
+  __shared__ c64 sI[kRmK][kRmP + 1];
+  __shared__ c64 sT[kRmK][kRmJ + 1];
+  const long long p0 = (long long)blockIdx.x * kRmP;
+  const int j0 = blockIdx.y * kRmJ;
+  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int lr = lane % 16, lk = lane / 16;
+  c64 vi[4], vt[4];
+  auto load = [&](int i0) {
 #pragma unroll
-  for (int q = 0; q < kRotJ; ++q) ar[q] = ai[q] = 0.0;
-  for (int i0 = 0; i0 < m; i0 += kRotRows) {
-    const int mi = m - i0 < kRotRows ? m - i0 : kRotRows;
+    for (int q = 0; q < 4; ++q) {
+      const int slot = (int)threadIdx.x + 256 * q;
+      const int ki = slot / kRmP, ci = slot % kRmP;     // inputs: points fastest
+      const int kt = slot % kRmK, ct = slot / kRmK;     // coefficients: inputs fastest
+      vi[q] = in[(long long)min(i0 + ki, m - 1) * N + min(p0 + ci, N - 1)];
+      vt[q] = T[(long long)min(i0 + kt, m - 1) + (long long)m * min(j0 + ct, nout - 1)];
+    }
+  };
+  f64x4r cr[4], ci4[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+    for (int e = 0; e < 4; ++e) cr[v][e] = ci4[v][e] = 0.0;
+  load(0);
+  for (int i0 = 0; i0 < m; i0 += kRmK) {
     __syncthreads();
-    for (int t = threadIdx.x; t < mi * kRotJ; t += kRotTB) {
-      const int i = t / kRotJ, q = t % kRotJ;
-      sT[t] = j0 + q < nout ? T[i0 + i + (size_t)m * (j0 + q)] : mk(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int slot = (int)threadIdx.x + 256 * q;
+      const int ki = slot / kRmP, ci = slot % kRmP;
+      const int kt = slot % kRmK, ct = slot / kRmK;
+      sI[ki][ci] = (i0 + ki < m && p0 + ci < N) ? vi[q] : mk(0.0, 0.0);
+      sT[kt][ct] = (i0 + kt < m && j0 + ct < nout) ? vt[q] : mk(0.0, 0.0);
     }
     __syncthreads();
-    if (!live) continue;
-    const c64* src = in + (size_t)i0 * N + r;
-    int i = 0;
-    for (; i + 4 <= mi; i += 4) {
-      c64 w[4];
+    if (i0 + kRmK < m) load(i0 + kRmK);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) w[u] = src[(size_t)(i + u) * N];
+    for (int kk = 0; kk < kRmK / 4; ++kk) {
+      const c64 a = sT[4 * kk + lk][16 * w + lr];
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int q = 0; q < kRotJ; ++q) {
-          const c64 t = sT[(i + u) * kRotJ + q];
-          ar[q] += w[u].x * t.x - w[u].y * t.y;
-          ai[q] += w[u].x * t.y + w[u].y * t.x;
-        }
-    }
-    for (; i < mi; ++i) {
-      const c64 w = src[(size_t)i * N];
-#pragma unroll
-      for (int q = 0; q < kRotJ; ++q) {
-        const c64 t = sT[i * kRotJ + q];
-        ar[q] += w.x * t.x - w.y * t.y;
-        ai[q] += w.x * t.y + w.y * t.x;
+      for (int v = 0; v < 4; ++v) {
+        const c64 b = sI[4 * kk + lk][16 * v + lr];
+        cr[v] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, cr[v], 0, 0, 0);
+        cr[v] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, cr[v], 0, 0, 0);
+        ci4[v] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, ci4[v], 0, 0, 0);
+        ci4[v] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, ci4[v], 0, 0, 0);
       }
     }
   }
-  if (!live) return;
 #pragma unroll
-  for (int q = 0; q < kRotJ; ++q)
-    if (j0 + q < nout) out[(size_t)(j0 + q) * N + r] = mk(ar[q], ai[q]);
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int j = j0 + 16 * w + lk + 4 * e;
+      const long long p = p0 + 16 * v + lr;
+      if (j < nout && p < N) out[(long long)j * N + p] = mk(cr[v][e], ci4[v][e]);
+    }
 }
 
 // r_i = h_i - e_i phi_i
@@ -607,8 +622,8 @@ hipError_t sub_real(hipStream_t st, const double* a, const double* b, long long 
 }
 
 hipError_t rotate_rect(hipStream_t st, const c64* in, c64* out, long long N, int m, int nout, const c64* T) {
-  hipLaunchKernelGGL(k_rotate_rect, dim3((unsigned)((N + kRotTB - 1) / kRotTB), (nout + kRotJ - 1) / kRotJ),
-                     dim3(kRotTB), 0, st, in, out, N, m, nout, T);
+  hipLaunchKernelGGL(k_rotate_mfma, dim3((unsigned)((N + kRmP - 1) / kRmP), (nout + kRmJ - 1) / kRmJ), dim3(256), 0,
+                     st, in, out, N, m, nout, T);
   return hipGetLastError();
 }
 

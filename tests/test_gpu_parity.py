@@ -231,12 +231,15 @@ def test_orthog_and_diagh_match_oracle(per):
     g.close()
 
 
-@pytest.mark.parametrize("nb", [13, 24, 40, 64])
+@pytest.mark.parametrize("nb", [13, 24, 40, 64, 65, 100, 130])
 def test_orthog_many_orbitals_matches_oracle(nb):
-    """orthog_phi on 13 ... 64 FP64 orbitals: the Gram matrices of 1 to 4
+    """orthog_phi on 13 ... 130 FP64 orbitals: the Gram matrices of 1 to 4
     16-wide tiles (ragged ones included) on the matrix cores, over point
-    chunks of 22-23 points; eigenvalues of the overlap as the oracle's,
-    orthonormal output spanning the oracle's subspace"""
+    chunks of 22-23 points, and beyond 64 fields the 64 x 64 tiles (the
+    overlap from the upper tiles and their mirror, diagh's <phi|H phi> from
+    all tiles); eigenvalues of the overlap as the oracle's, orthonormal
+    output spanning the oracle's subspace, diagh's eigenvalues as the
+    oracle's"""
     spec = synth.hpsi_system(n=(24, 24, 20), d=0.5, natom=4, seed=8)
     st = state_from_spec(spec, nsys=2)
     _random_potentials(st, spec, 2)
@@ -251,6 +254,9 @@ def test_orthog_many_orbitals_matches_oracle(nb):
     assert np.max(np.abs(G - np.eye(nb))) < 1e-12
     M = (ref.conj() @ out.T) * spec.dv
     assert np.max(np.abs(M.conj().T @ M - np.eye(nb))) < 1e-10
+    ev = g.diagh(0, 1)
+    _, ev_ref, _ = O.diagh(st, out, 1, 0)
+    assert np.max(np.abs(ev - ev_ref)) < 1e-10 * max(1.0, np.max(np.abs(ev_ref)))
     g.close()
 
 
