@@ -10,6 +10,7 @@ module opaw_hip_mod
   private
 
   type(c_ptr), save, public :: opaw_ctx = c_null_ptr
+  real(c_double), save :: opaw_dt = 0d0     ! tddft_mod dt, as handed over by gpu_set_dt
 
   interface
     integer(c_int) function opaw_create(ctx, nx, ny, nz, dx, dy, dz, periodic, precision, nsys, nb) &
@@ -197,6 +198,14 @@ module opaw_hip_mod
       integer(c_int), value :: sys
       complex(c_double_complex) :: host(*)
     end function
+    integer(c_int) function opaw_write_dipoles(d_dip_path, dip_path, append, it0, nt, dt, d_dip, dip, nsys) &
+        bind(C, name="opaw_write_dipoles")
+      import :: c_int, c_double, c_char
+      character(kind=c_char) :: d_dip_path(*), dip_path(*)
+      integer(c_int), value :: append, it0, nt, nsys
+      real(c_double), value :: dt
+      real(c_double) :: d_dip(*), dip(*)
+    end function
     integer(c_int) function opaw_set_dt(ctx, dt) bind(C, name="opaw_set_dt")
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx
@@ -221,7 +230,8 @@ module opaw_hip_mod
             opaw_excite_psi_bar_pert, opaw_write_restart, opaw_read_restart, opaw_set_nocc, opaw_set_diis, &
             opaw_ncpaw_make_hamiltonian, opaw_hminmax, opaw_chebyshev_filter, opaw_orthog_phi, opaw_diagh, &
             opaw_h_phi_host, opaw_shs_host, opaw_sn_phi_host, opaw_proj_host, opaw_write_orbitals_c16, &
-            opaw_read_orbitals_c16, opaw_set_dt, opaw_rk4_prop_shs_host, opaw_check, opaw_require_ik
+            opaw_read_orbitals_c16, opaw_set_dt, opaw_rk4_prop_shs_host, opaw_write_dipoles, opaw_check, &
+            opaw_require_ik
   public :: gpu_set_dt, gpu_scatter_phi_bar, gpu_gather_phi_bar, gpu_ncpaw_make_hamiltonian_tddft, gpu_plot_dip, &
             gpu_write_to_restart, gpu_read_phi_bar, gpu_ncpaw_make_hamiltonian, gpu_get_hminmax, gpu_chebyr_m, &
             gpu_orthog_phi, gpu_diagh
@@ -260,6 +270,7 @@ contains
   subroutine gpu_set_dt(dt)
     real(c_double), intent(in) :: dt
     call opaw_check(opaw_set_dt(opaw_ctx, dt), 'set_dt')
+    opaw_dt = dt
   end subroutine
 
   ! scatter_phi_bar (src/tddft_scatter.f90): phi_bar and phi_bar_pert
@@ -285,11 +296,27 @@ contains
     call opaw_check(opaw_make_hamiltonian_tddft(opaw_ctx, -1_c_int), 'make_hamiltonian_tddft')
   end subroutine
 
-  ! drop-in for the dipole part of plot_dip (src/tddft_plot.f90:1-57)
-  subroutine gpu_plot_dip(sm, d_dip, dip, dip_pert)
+  ! drop-in for plot_dip (src/tddft_plot.f90:1-105): the dipoles of both
+  ! systems, and with the time step it given, the lines plot_dip writes at
+  ! (it-1)*dt: d_dip.dat (' ',4e18.8) and dip.dat (list-directed t, dip,
+  ! dip_pert); the files start afresh at it = 1, as the reference rewinds
+  ! them there, and are appended to after (dt from gpu_set_dt)
+  subroutine gpu_plot_dip(sm, d_dip, dip, dip_pert, it)
     real(c_double), intent(in) :: sm
     real(c_double) :: d_dip(3), dip(3), dip_pert(3)
+    integer, intent(in), optional :: it
+    real(c_double) :: both(6)
+    integer(c_int) :: app
     call opaw_check(opaw_plot_dip(opaw_ctx, sm, d_dip, dip, dip_pert), 'plot_dip')
+    if (present(it)) then
+      both(1:3) = dip
+      both(4:6) = dip_pert
+      app = 1
+      if (it == 1) app = 0
+      call opaw_check(opaw_write_dipoles('d_dip.dat'//c_null_char, 'dip.dat'//c_null_char, app, &
+                                         int(it - 1, c_int), 1_c_int, opaw_dt, d_dip, both, 2_c_int), &
+                      'plot_dip files')
+    end if
   end subroutine
 
   ! drop-in for write_to_restart (src/tddft_write_to_restart.f90:1): the
