@@ -2,9 +2,9 @@
 oracle on the reference's naphthalene input (cnt.ini geometry, the JTH C
 and H datasets, grid 56 x 52 x 32 at 0.5 bohr), a periodic box whose
 spheres and boxes wrap, and one TDDFT step of naphthalene built from the
-datasets, GPU against the CPU oracle.  Local potential, core density and
-on-site D_ij are stand-ins (libpaw's part of the set-up; see
-opaw_amd.paw.system_from_datasets)."""
+datasets, GPU against the CPU oracle.  libpaw's part of the set-up (core
+and starting densities, dij0, e_ijkl, the on-site XC of D_ij) is checked in
+tests/test_gpu_onsite.py and tests/test_libpaw_cpu.py."""
 import os
 
 import numpy as np
