@@ -18,6 +18,7 @@
 #include "opaw_linalg.h"
 #include "opaw_onsite.h"
 #include "opaw_paw.h"
+#include "opaw_rr.h"
 
 using namespace opaw;
 
@@ -287,6 +288,8 @@ struct opaw_ctx {
   void* pin = nullptr;   // pinned host staging of the block solver's small transfers
   size_t pin_bytes = 0;
   DevBuf gsTab;       // preconditioner multipliers per (band, shell)
+  DevBuf gsRR;        // the device Rayleigh-Ritz work (matrices, eigenvalues, flags)
+  void* blas = nullptr;   // rocBLAS handle of the device Rayleigh-Ritz, created on first use
   int nsh = 0;
 
   size_t esize() const { return prec == OPAW_PREC_F64 ? sizeof(c64) : sizeof(c32); }
@@ -479,6 +482,7 @@ int opaw_destroy(opaw_ctx* c) {
   for (void* p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
   if (c->pin) (void)hipHostFree(c->pin);
+  rr_release(c->blas);
   delete c;
   return OPAW_OK;
 }
@@ -2568,17 +2572,6 @@ static int apply_h(opaw_ctx* c, int sys, int h_type, const c64* in, c64* out, in
   return flow_h_phi<double>(c, sys, in, out, nf, nf);
 }
 
-// generalised eigenproblem H C = S C w on an m-dimensional basis (row-major
-// host matrices): Cholesky when the unit-scaled overlap is safely positive
-// definite, else directions with overlap below 1e-12 of the largest are
-// dropped; returns the lowest nout eigenpairs, C column-major m x nout
-static int subspace_eig(int m, const std::vector<std::complex<double>>& S0, const std::vector<std::complex<double>>& H0,
-                        int nout, std::vector<std::complex<double>>& Ccol, std::vector<double>& w) {
-  const int r = pencil_lowest(m, S0, H0, nout, 1e-12, Ccol, w);
-  if (r < nout) return fail(OPAW_ERR_ARG, "subspace rank %d below the %d bands", r, nout);
-  return OPAW_OK;
-}
-
 // gram of m fields (row-major m x m on the host), with the solver's buffers
 static int gram_m(opaw_ctx* c, const c64* a, const c64* b, int m, std::vector<std::complex<double>>& out) {
   RCHK(ensure(c, c->gsG, gram_partials(m) * sizeof(c64)));
@@ -2853,20 +2846,21 @@ int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, in
   double* dy = dt + nb;
   RCHK(ensure_shells(c));
   RCHK(ensure(c, c->gsTab, (size_t)nb * c->nsh * sizeof(double)));
-  // pinned staging, one area per transfer: the partial sums (down), the
-  // Gram matrices S and H (down), e then t, y (up), the coefficients C and
-  // Cp (up).  Each area is rewritten only after a stream synchronisation
-  // that follows its previous transfer.
-  const size_t mmax = (size_t)3 * nb;
+  // pinned staging, one area per transfer: the partial sums (down), e (down)
+  // then t and y (up), the Rayleigh-Ritz flags (down).  Each area is
+  // rewritten only after a stream synchronisation that follows its previous
+  // transfer.
   const size_t pin_h = (size_t)kGsChunks * nb * 3 * sizeof(double);
-  const size_t pin_sh = 2 * mmax * mmax * sizeof(c64);
-  const size_t pin_ety = (3 * (size_t)nb * sizeof(double) + 15) / 16 * 16;   // keeps pc 16-byte aligned
-  const size_t pin_c = (mmax + (mmax - nb)) * nb * sizeof(c64);
-  RCHK(ensure_pinned(c, pin_h + pin_sh + pin_ety + pin_c));
+  const size_t pin_ety = (3 * (size_t)nb * sizeof(double) + 15) / 16 * 16;
+  RCHK(ensure_pinned(c, pin_h + pin_ety + 16));
   double* ph = (double*)c->pin;
-  c64* psh = (c64*)((char*)c->pin + pin_h);
-  double* pety = (double*)((char*)psh + pin_sh);
-  c64* pc = (c64*)((char*)pety + pin_ety);
+  double* pety = (double*)((char*)c->pin + pin_h);
+  int* pflags = (int*)((char*)pety + pin_ety);
+  // the subspace work and the coefficients C (m x nb) and Cp (its rows
+  // nb..m, (m - nb) x nb), for the largest m = 3 nb
+  const int mmax = 3 * nb;
+  RCHK(ensure(c, c->gsRR, rr_work_bytes(mmax)));
+  RCHK(ensure(c, c->gsSmallC, (size_t)(mmax + (mmax - nb)) * nb * sizeof(c64)));
   HCHK(hipMemcpyAsync(Y, phi, fb, hipMemcpyDeviceToDevice, c->st));
   RCHK(apply_h(c, sys, h_type, Y, HY, nb));
   std::vector<double> e(nb), t(nb), y(nb);
@@ -2875,17 +2869,37 @@ int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, in
     std::vector<std::complex<double>> G;
     RCHK(gram_m(c, Y, HY, nb, G));
     for (int i = 0; i < nb; ++i) e[i] = G[(size_t)i * nb + i].real();
+    std::copy(e.begin(), e.end(), pety);
+    HCHK(hipMemcpyAsync(deig, pety, nb * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HCHK(hipStreamSynchronize(c->st));
   }
   bool have_p = false;
+  int m = 0;   // dimension of the last Rayleigh-Ritz, 0 once its flags are checked
   int rc;
   // one partial-sum buffer for the residual norms and the kinetic sums
   double* part = gs_part(c, (size_t)kGsChunks * nb * 3, &rc);
   RCHK(rc);
   const double* h = ph;
+  // the Ritz values and the flags of the last Rayleigh-Ritz, read after a
+  // synchronisation
+  auto check_rr = [&]() -> int {
+    if (m == 0) return OPAW_OK;
+    std::copy(pety, pety + nb, e.begin());
+    const int rank = pflags[0], info_s = pflags[1], info_h = pflags[2];
+    if (info_s != 0 || info_h != 0)
+      return fail(OPAW_ERR_HIP, "subspace eigensolver did not converge (info %d, %d)", info_s, info_h);
+    if (rank < nb) return fail(OPAW_ERR_ARG, "subspace rank %d below the %d bands", rank, nb);
+    m = 0;
+    return OPAW_OK;
+  };
+  auto fetch_rr = [&]() -> int {
+    if (m == 0) return OPAW_OK;
+    HCHK(hipMemcpyAsync(pety, deig, nb * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HCHK(hipMemcpyAsync(pflags, rr_flags(c->gsRR.p, m), 3 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+    return OPAW_OK;
+  };
   int it = 0;
   for (; it < maxit; ++it) {
-    std::copy(e.begin(), e.end(), pety);
-    HCHK(hipMemcpyAsync(deig, pety, nb * sizeof(double), hipMemcpyHostToDevice, c->st));
     c64* W = Y + off;
     HCHK(residual_vectors(c->st, HY, Y, deig, N, nb, W));
     HCHK(residual_partials<double>(c->st, HY, Y, deig, N, nb, part));
@@ -2893,7 +2907,9 @@ int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, in
     HCHK(fft3d<double>(c->g, c->st, -1, Y, Kb, nb, twx, twy, twz));
     HCHK(kinetic_partials(c->st, Kb, c->ek, N, nb, part + (size_t)kGsChunks * nb));
     HCHK(hipMemcpyAsync(ph, part, pin_h, hipMemcpyDeviceToHost, c->st));
+    RCHK(fetch_rr());
     HCHK(hipStreamSynchronize(c->st));
+    RCHK(check_rr());
     double worst = 0.0;
     for (int i = 0; i < nconv; ++i) {
       double sm = 0.0;
@@ -2933,36 +2949,22 @@ int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, in
                               shape, (double*)c->gsTab.p));
     HCHK(fft3d<double>(c->g, c->st, 1, Kb, W, nb, twx, twy, twz));
     RCHK(apply_h(c, sys, h_type, W, HY + off, nb));
-    // Rayleigh-Ritz on [phi, w, p] (LOBPCG; no p in the first iteration);
-    // both Gram matrices in one download
-    const int m = have_p ? 3 * nb : 2 * nb;
-    std::vector<std::complex<double>> S((size_t)m * m), H((size_t)m * m), Ccol;
+    // Rayleigh-Ritz on [phi, w, p] (LOBPCG; no p in the first iteration),
+    // on the device: the Gram matrices, the pencil's lowest nb eigenpairs
+    // (Ritz values into deig for the next residuals) and the rotations
+    m = have_p ? 3 * nb : 2 * nb;
     RCHK(ensure(c, c->gsG, gram_partials(m) * sizeof(c64)));
     RCHK(ensure(c, c->gsM, (size_t)2 * m * m * sizeof(c64)));
-    HCHK(gram<double>(c->st, Y, Y, N, m, dvol(c), (c64*)c->gsG.p, (c64*)c->gsM.p));
-    HCHK(gram<double>(c->st, Y, HY, N, m, dvol(c), (c64*)c->gsG.p, (c64*)c->gsM.p + (size_t)m * m));
-    HCHK(hipMemcpyAsync(psh, c->gsM.p, 2 * (size_t)m * m * sizeof(c64), hipMemcpyDeviceToHost, c->st));
-    HCHK(hipStreamSynchronize(c->st));
-    std::memcpy((void*)S.data(), psh, (size_t)m * m * sizeof(c64));
-    std::memcpy((void*)H.data(), psh + (size_t)m * m, (size_t)m * m * sizeof(c64));
-    for (int a = 0; a < m; ++a)
-      for (int b = a + 1; b < m; ++b) {
-        const std::complex<double> hs = 0.5 * (S[(size_t)a * m + b] + std::conj(S[(size_t)b * m + a]));
-        S[(size_t)a * m + b] = hs;
-        S[(size_t)b * m + a] = std::conj(hs);
-      }
-    std::vector<double> w;
-    RCHK(subspace_eig(m, S, H, nb, Ccol, w));
-    // C (m x nb) and its lower block C[nb:m] ((m - nb) x nb), column-major
-    std::vector<std::complex<double>> Cp((size_t)(m - nb) * nb);
-    for (int j = 0; j < nb; ++j)
-      for (int i = nb; i < m; ++i) Cp[(i - nb) + (size_t)(m - nb) * j] = Ccol[i + (size_t)m * j];
-    RCHK(ensure(c, c->gsSmallC, (size_t)(m + (m - nb)) * nb * sizeof(c64)));
+    c64* dS = (c64*)c->gsM.p;
+    c64* dH = dS + (size_t)m * m;
+    HCHK(gram<double>(c->st, Y, Y, N, m, dvol(c), (c64*)c->gsG.p, dS));
+    HCHK(gram<double>(c->st, Y, HY, N, m, dvol(c), (c64*)c->gsG.p, dH));
     c64* dC = (c64*)c->gsSmallC.p;
     c64* dCp = dC + (size_t)m * nb;
-    std::memcpy((void*)pc, Ccol.data(), Ccol.size() * sizeof(c64));
-    std::memcpy((void*)(pc + Ccol.size()), Cp.data(), Cp.size() * sizeof(c64));
-    HCHK(hipMemcpyAsync(dC, pc, (Ccol.size() + Cp.size()) * sizeof(c64), hipMemcpyHostToDevice, c->st));   // dCp follows dC
+    const int rr = rr_lowest(&c->blas, c->st, m, nb, 1e-12, dS, dH, c->gsRR.p, dC, deig);
+    if (rr != 0) return fail(OPAW_ERR_HIP, "device Rayleigh-Ritz (m %d): %s %d", m, rr > 0 ? "rocBLAS status" : "HIP error", rr > 0 ? rr : -rr);
+    HCHK(hipMemcpy2DAsync(dCp, (size_t)(m - nb) * sizeof(c64), dC + nb, (size_t)m * sizeof(c64),
+                          (size_t)(m - nb) * sizeof(c64), nb, hipMemcpyDeviceToDevice, c->st));
     HCHK(rotate_rect(c->st, Y, Yn, N, m, nb, dC));
     HCHK(rotate_rect(c->st, HY, HYn, N, m, nb, dC));
     HCHK(rotate_rect(c->st, Y + off, Yn + 2 * off, N, m - nb, nb, dCp));
@@ -2970,8 +2972,11 @@ int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, in
     std::swap(Y, Yn);
     std::swap(HY, HYn);
     have_p = true;
-    e = w;
   }
+  // a last Rayleigh-Ritz not followed by a residual pass
+  RCHK(fetch_rr());
+  HCHK(hipStreamSynchronize(c->st));
+  RCHK(check_rr());
   if (iters) *iters = it;
   HCHK(hipMemcpyAsync(phi, Y, fb, hipMemcpyDeviceToDevice, c->st));
   HCHK(hipStreamSynchronize(c->st));

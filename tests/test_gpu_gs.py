@@ -114,6 +114,26 @@ def test_block_solver_converges_and_oracle_agrees():
     g.close()
 
 
+def _update_basis(spec, st, phi, P, nh):
+    """The [phi, w] block of one LOBPCG update formed on the CPU (the
+    oracle's shs and FFTs, the multiplier at every k point) and its H image."""
+    N, dv, grid = spec.n, spec.dv, st.grid
+    ek = O.ek_prep(grid, spec.ekcut)                 # carries 1/N
+    H = O.shs(st, phi, 0)
+    e = np.real(np.sum(phi.conj() * H, axis=1)) * dv
+    r = H - e[:, None] * phi
+    F = O.fft3d_forward(grid, phi)
+    t = np.array([np.sum(ek * np.abs(f) ** 2) / np.sum(np.abs(f) ** 2) * N for f in F])
+    t = np.maximum(t, 1e-6)
+    y = e / t
+    Rk = O.fft3d_forward(grid, r)
+    for i in range(len(e)):
+        m, _ = GO.multiplier(P, nh, ek * N / t[i], y[i])
+        Rk[i] *= m / N
+    W = O.fft3d_backward(grid, Rk)
+    return np.concatenate([phi, W]), np.concatenate([H, O.shs(st, W, 0)])
+
+
 @pytest.mark.parametrize("learned", [False, True])
 def test_one_preconditioned_update_matches_oracle(learned):
     """One LOBPCG update on the GPU (residuals r = H phi - e phi, band
@@ -137,23 +157,8 @@ def test_one_preconditioned_update_matches_oracle(learned):
     assert its == 1
     out = g.read_orbitals(0)
     g.close()
-    # the same update on the CPU
-    N, dv, grid = spec.n, spec.dv, st.grid
-    ek = O.ek_prep(grid, spec.ekcut)                 # carries 1/N
-    H = O.shs(st, phi, 0)
-    e = np.real(np.sum(phi.conj() * H, axis=1)) * dv
-    r = H - e[:, None] * phi
-    F = O.fft3d_forward(grid, phi)
-    t = np.array([np.sum(ek * np.abs(f) ** 2) / np.sum(np.abs(f) ** 2) * N for f in F])
-    t = np.maximum(t, 1e-6)
-    y = e / t
-    Rk = O.fft3d_forward(grid, r)
-    for i in range(len(e)):
-        m, _ = GO.multiplier(P, nh, ek * N / t[i], y[i])
-        Rk[i] *= m / N
-    W = O.fft3d_backward(grid, Rk)
-    Yb = np.concatenate([phi, W])
-    HYb = np.concatenate([H, O.shs(st, W, 0)])
+    Yb, HYb = _update_basis(spec, st, phi, P, nh)
+    dv = spec.dv
     S = (Yb.conj() @ Yb.T) * dv
     Hm = (Yb.conj() @ HYb.T) * dv
     w, V = scipy.linalg.eigh(0.5 * (Hm + Hm.conj().T), 0.5 * (S + S.conj().T))
@@ -165,6 +170,64 @@ def test_one_preconditioned_update_matches_oracle(learned):
     resid = out.T - Q @ (Q.conj().T @ out.T)
     assert np.linalg.norm(resid) < 1e-9 * np.linalg.norm(out)
 
+
+
+def _range_ritz(S, Hm, nb, floor=1e-12):
+    """The lowest nb Ritz pairs of the pencil (Hm, S) on the range of S:
+    unit-diagonal scaling, directions of the scaled overlap with eigenvalue
+    <= floor * largest dropped.  Any basis of that range gives the same
+    Ritz values and span."""
+    d = 1.0 / np.sqrt(np.real(np.diag(S)))
+    lam, U = np.linalg.eigh(d[:, None] * S * d[None, :])
+    keep = lam > floor * lam[-1]
+    X = d[:, None] * U[:, keep] / np.sqrt(lam[keep])
+    w, V = np.linalg.eigh(X.conj().T @ Hm @ X)
+    return int(keep.sum()), w[:nb], X @ V[:, :nb]
+
+
+def test_device_rayleigh_ritz_drops_dependent_directions():
+    """The device Rayleigh-Ritz's rank-revealing route: with two equal
+    orbitals the [phi, w] block has two exactly dependent columns; the
+    update must return the Ritz values of the pencil on the range of the
+    overlap and orthonormal orbitals in that range.  Keeping a dropped
+    direction gives a zero (or, padded, a spurious) orbital and fails the
+    orthonormality; not dropping it leaves a singular overlap."""
+    spec, st, phi = _system(nb=4)
+    phi = phi.copy()
+    phi[3] = phi[2]
+    g = _gpu(spec, st, phi.shape[0])
+    g.write_orbitals(0, phi)
+    eig, _, its = g.block_solve(1, 0.0)
+    assert its == 1
+    out = g.read_orbitals(0)
+    g.close()
+    Yb, HYb = _update_basis(spec, st, phi, None, 8)
+    dv = spec.dv
+    S = (Yb.conj() @ Yb.T) * dv
+    Hm = (Yb.conj() @ HYb.T) * dv
+    rank, w, C = _range_ritz(0.5 * (S + S.conj().T), 0.5 * (Hm + Hm.conj().T), 4)
+    assert rank == 6
+    assert np.max(np.abs(eig - w)) < 1e-10 * max(1.0, np.max(np.abs(w)))
+    G = (out.conj() @ out.T) * dv
+    assert np.max(np.abs(G - np.eye(4))) < 1e-10
+    Q = np.linalg.qr((C.T @ Yb).T)[0]
+    resid = out.T - Q @ (Q.conj().T @ out.T)
+    assert np.linalg.norm(resid) < 1e-9 * np.linalg.norm(out)
+
+
+def test_device_rayleigh_ritz_refuses_rank_below_bands():
+    """Four equal orbitals: the [phi, w] block spans two directions, fewer
+    than the four bands; the solver fails with the rank and leaves the
+    resident orbitals as they were."""
+    from opaw_amd.lib import OpawError
+    spec, st, phi = _system(nb=4)
+    same = np.repeat(phi[:1], 4, axis=0)
+    g = _gpu(spec, st, 4)
+    g.write_orbitals(0, same)
+    with pytest.raises(OpawError, match="subspace rank 2 below the 4 bands"):
+        g.block_solve(1, 0.0)
+    assert np.array_equal(g.read_orbitals(0), same)
+    g.close()
 
 def test_preconditioner_loss_and_gradient_match_oracle():
     spec, st, phi = _system(nb=4)
