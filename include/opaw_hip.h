@@ -409,6 +409,14 @@ int opaw_residuals(opaw_ctx* ctx, int sys, int h_type, const double* eig, double
  * their H images, one k-space block). */
 int opaw_block_solve(opaw_ctx* ctx, int sys, int h_type, int maxit, double tol, int nconv, const double* mlp, int nh,
                      int record, double* eig, double* hist, int* iters);
+/* Where opaw_block_solve solves its Rayleigh-Ritz pencils (dimension m = 2 nb
+ * or 3 nb): on the device (rocSOLVER's zheevd, rocBLAS) for m >= min_m, on
+ * the host below; no counterpart in the reference.  The device route keeps
+ * the Gram matrices in HBM and always drops overlap directions below 1e-12
+ * of the largest; the host route takes Cholesky when the scaled overlap is
+ * safely positive definite.  Default 144, where the two cost the same on
+ * an MI355X host; 0: always the device. */
+int opaw_set_subspace_device(opaw_ctx* ctx, int min_m);
 /* MLP of the preconditioner: 2 -> nh -> nh -> 1 with tanh hidden layers;
  * parameters W1 (nh x 2), b1, W2 (nh x nh), b2, w3 (nh), b3. */
 int opaw_mlp_param_count(int nh);

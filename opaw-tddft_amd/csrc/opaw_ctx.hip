@@ -290,6 +290,11 @@ struct opaw_ctx {
   DevBuf gsTab;       // preconditioner multipliers per (band, shell)
   DevBuf gsRR;        // the device Rayleigh-Ritz work (matrices, eigenvalues, flags)
   void* blas = nullptr;   // rocBLAS handle of the device Rayleigh-Ritz, created on first use
+  // the block solver's pencils of dimension >= rr_dev_min are solved on the
+  // device (rocSOLVER), smaller ones on the host, where they are cheaper:
+  // zheevd costs about 1 ms even at m = 48, the host 0.25 ms; they cross near
+  // m = 144 (2 x 3.8 ms on the device against about 9 ms on the host at 160)
+  int rr_dev_min = 144;
   int nsh = 0;
 
   size_t esize() const { return prec == OPAW_PREC_F64 ? sizeof(c64) : sizeof(c32); }
@@ -2572,6 +2577,17 @@ static int apply_h(opaw_ctx* c, int sys, int h_type, const c64* in, c64* out, in
   return flow_h_phi<double>(c, sys, in, out, nf, nf);
 }
 
+// generalised eigenproblem H C = S C w on an m-dimensional basis (row-major
+// host matrices): Cholesky when the unit-scaled overlap is safely positive
+// definite, else directions with overlap below 1e-12 of the largest are
+// dropped; returns the lowest nout eigenpairs, C column-major m x nout
+static int subspace_eig(int m, const std::vector<std::complex<double>>& S0, const std::vector<std::complex<double>>& H0,
+                        int nout, std::vector<std::complex<double>>& Ccol, std::vector<double>& w) {
+  const int r = pencil_lowest(m, S0, H0, nout, 1e-12, Ccol, w);
+  if (r < nout) return fail(OPAW_ERR_ARG, "subspace rank %d below the %d bands", r, nout);
+  return OPAW_OK;
+}
+
 // gram of m fields (row-major m x m on the host), with the solver's buffers
 static int gram_m(opaw_ctx* c, const c64* a, const c64* b, int m, std::vector<std::complex<double>>& out) {
   RCHK(ensure(c, c->gsG, gram_partials(m) * sizeof(c64)));
@@ -2631,6 +2647,13 @@ int opaw_set_diis(opaw_ctx* c, int ndi, double mix) {
   }
   c->diis_L = ndi;
   c->diis_mix = mix;
+  return OPAW_OK;
+}
+
+int opaw_set_subspace_device(opaw_ctx* c, int min_m) {
+  CTX_CHECK(c);
+  if (min_m < 0) return fail(OPAW_ERR_ARG, "min_m must be >= 0");
+  c->rr_dev_min = min_m;
   return OPAW_OK;
 }
 
@@ -2846,20 +2869,28 @@ int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, in
   double* dy = dt + nb;
   RCHK(ensure_shells(c));
   RCHK(ensure(c, c->gsTab, (size_t)nb * c->nsh * sizeof(double)));
-  // pinned staging, one area per transfer: the partial sums (down), e (down)
-  // then t and y (up), the Rayleigh-Ritz flags (down).  Each area is
-  // rewritten only after a stream synchronisation that follows its previous
-  // transfer.
+  // pinned staging, one area per transfer: the partial sums (down), e (down
+  // or up) then t and y (up), the device pencil's flags (down), the host
+  // pencil's Gram matrices S and H (down) and coefficients C and Cp (up).
+  // Each area is rewritten only after a stream synchronisation that follows
+  // its previous transfer.
+  const int mmax = 3 * nb;
+  const bool any_host = 2 * nb < c->rr_dev_min;
+  const bool any_dev = mmax >= c->rr_dev_min;
   const size_t pin_h = (size_t)kGsChunks * nb * 3 * sizeof(double);
-  const size_t pin_ety = (3 * (size_t)nb * sizeof(double) + 15) / 16 * 16;
-  RCHK(ensure_pinned(c, pin_h + pin_ety + 16));
+  const size_t pin_ety = (3 * (size_t)nb * sizeof(double) + 15) / 16 * 16;   // keeps what follows 16-byte aligned
+  const size_t pin_fl = 16;
+  const size_t pin_sh = any_host ? 2 * (size_t)mmax * mmax * sizeof(c64) : 0;
+  const size_t pin_c = any_host ? (size_t)(mmax + (mmax - nb)) * nb * sizeof(c64) : 0;
+  RCHK(ensure_pinned(c, pin_h + pin_ety + pin_fl + pin_sh + pin_c));
   double* ph = (double*)c->pin;
   double* pety = (double*)((char*)c->pin + pin_h);
   int* pflags = (int*)((char*)pety + pin_ety);
-  // the subspace work and the coefficients C (m x nb) and Cp (its rows
-  // nb..m, (m - nb) x nb), for the largest m = 3 nb
-  const int mmax = 3 * nb;
-  RCHK(ensure(c, c->gsRR, rr_work_bytes(mmax)));
+  c64* psh = (c64*)((char*)pflags + pin_fl);
+  c64* pc = psh + (any_host ? 2 * (size_t)mmax * mmax : 0);
+  // the device pencil's work and the coefficients C (m x nb) and Cp (its
+  // rows nb..m, (m - nb) x nb), for the largest m = 3 nb
+  if (any_dev) RCHK(ensure(c, c->gsRR, rr_work_bytes(mmax)));
   RCHK(ensure(c, c->gsSmallC, (size_t)(mmax + (mmax - nb)) * nb * sizeof(c64)));
   HCHK(hipMemcpyAsync(Y, phi, fb, hipMemcpyDeviceToDevice, c->st));
   RCHK(apply_h(c, sys, h_type, Y, HY, nb));
@@ -2874,7 +2905,7 @@ int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, in
     HCHK(hipStreamSynchronize(c->st));
   }
   bool have_p = false;
-  int m = 0;   // dimension of the last Rayleigh-Ritz, 0 once its flags are checked
+  int m = 0;   // dimension of the last device Rayleigh-Ritz, 0 once its flags are checked
   int rc;
   // one partial-sum buffer for the residual norms and the kinetic sums
   double* part = gs_part(c, (size_t)kGsChunks * nb * 3, &rc);
@@ -2949,26 +2980,57 @@ int opaw_block_solve(opaw_ctx* c, int sys, int h_type, int maxit, double tol, in
                               shape, (double*)c->gsTab.p));
     HCHK(fft3d<double>(c->g, c->st, 1, Kb, W, nb, twx, twy, twz));
     RCHK(apply_h(c, sys, h_type, W, HY + off, nb));
-    // Rayleigh-Ritz on [phi, w, p] (LOBPCG; no p in the first iteration),
-    // on the device: the Gram matrices, the pencil's lowest nb eigenpairs
-    // (Ritz values into deig for the next residuals) and the rotations
-    m = have_p ? 3 * nb : 2 * nb;
-    RCHK(ensure(c, c->gsG, gram_partials(m) * sizeof(c64)));
-    RCHK(ensure(c, c->gsM, (size_t)2 * m * m * sizeof(c64)));
+    // Rayleigh-Ritz on [phi, w, p] (LOBPCG; no p in the first iteration):
+    // the Gram matrices, the pencil's lowest nb eigenpairs (Ritz values into
+    // deig for the next residuals) and the rotations
+    const int mi = have_p ? 3 * nb : 2 * nb;
+    RCHK(ensure(c, c->gsG, gram_partials(mi) * sizeof(c64)));
+    RCHK(ensure(c, c->gsM, (size_t)2 * mi * mi * sizeof(c64)));
     c64* dS = (c64*)c->gsM.p;
-    c64* dH = dS + (size_t)m * m;
-    HCHK(gram<double>(c->st, Y, Y, N, m, dvol(c), (c64*)c->gsG.p, dS));
-    HCHK(gram<double>(c->st, Y, HY, N, m, dvol(c), (c64*)c->gsG.p, dH));
+    c64* dH = dS + (size_t)mi * mi;
+    HCHK(gram<double>(c->st, Y, Y, N, mi, dvol(c), (c64*)c->gsG.p, dS));
+    HCHK(gram<double>(c->st, Y, HY, N, mi, dvol(c), (c64*)c->gsG.p, dH));
     c64* dC = (c64*)c->gsSmallC.p;
-    c64* dCp = dC + (size_t)m * nb;
-    const int rr = rr_lowest(&c->blas, c->st, m, nb, 1e-12, dS, dH, c->gsRR.p, dC, deig);
-    if (rr != 0) return fail(OPAW_ERR_HIP, "device Rayleigh-Ritz (m %d): %s %d", m, rr > 0 ? "rocBLAS status" : "HIP error", rr > 0 ? rr : -rr);
-    HCHK(hipMemcpy2DAsync(dCp, (size_t)(m - nb) * sizeof(c64), dC + nb, (size_t)m * sizeof(c64),
-                          (size_t)(m - nb) * sizeof(c64), nb, hipMemcpyDeviceToDevice, c->st));
-    HCHK(rotate_rect(c->st, Y, Yn, N, m, nb, dC));
-    HCHK(rotate_rect(c->st, HY, HYn, N, m, nb, dC));
-    HCHK(rotate_rect(c->st, Y + off, Yn + 2 * off, N, m - nb, nb, dCp));
-    HCHK(rotate_rect(c->st, HY + off, HYn + 2 * off, N, m - nb, nb, dCp));
+    c64* dCp = dC + (size_t)mi * nb;
+    if (mi >= c->rr_dev_min) {
+      // on the device: its rank and info are read at the next synchronisation
+      const int rr = rr_lowest(&c->blas, c->st, mi, nb, 1e-12, dS, dH, c->gsRR.p, dC, deig);
+      if (rr != 0)
+        return fail(OPAW_ERR_HIP, "device Rayleigh-Ritz (m %d): %s %d", mi, rr > 0 ? "rocBLAS status" : "HIP error",
+                    rr > 0 ? rr : -rr);
+      HCHK(hipMemcpy2DAsync(dCp, (size_t)(mi - nb) * sizeof(c64), dC + nb, (size_t)mi * sizeof(c64),
+                            (size_t)(mi - nb) * sizeof(c64), nb, hipMemcpyDeviceToDevice, c->st));
+      m = mi;
+    } else {
+      // on the host: both Gram matrices in one download, the coefficients
+      // and the Ritz values in two uploads
+      std::vector<std::complex<double>> S((size_t)mi * mi), H((size_t)mi * mi), Ccol;
+      HCHK(hipMemcpyAsync(psh, dS, 2 * (size_t)mi * mi * sizeof(c64), hipMemcpyDeviceToHost, c->st));
+      HCHK(hipStreamSynchronize(c->st));
+      std::memcpy((void*)S.data(), psh, (size_t)mi * mi * sizeof(c64));
+      std::memcpy((void*)H.data(), psh + (size_t)mi * mi, (size_t)mi * mi * sizeof(c64));
+      for (int a = 0; a < mi; ++a)
+        for (int b = a + 1; b < mi; ++b) {
+          const std::complex<double> hs = 0.5 * (S[(size_t)a * mi + b] + std::conj(S[(size_t)b * mi + a]));
+          S[(size_t)a * mi + b] = hs;
+          S[(size_t)b * mi + a] = std::conj(hs);
+        }
+      std::vector<double> w;
+      RCHK(subspace_eig(mi, S, H, nb, Ccol, w));
+      std::vector<std::complex<double>> Cp((size_t)(mi - nb) * nb);   // rows nb..m of C
+      for (int j = 0; j < nb; ++j)
+        for (int i = nb; i < mi; ++i) Cp[(i - nb) + (size_t)(mi - nb) * j] = Ccol[i + (size_t)mi * j];
+      std::memcpy((void*)pc, Ccol.data(), Ccol.size() * sizeof(c64));
+      std::memcpy((void*)(pc + Ccol.size()), Cp.data(), Cp.size() * sizeof(c64));
+      HCHK(hipMemcpyAsync(dC, pc, ((size_t)mi + (mi - nb)) * nb * sizeof(c64), hipMemcpyHostToDevice, c->st));   // dCp follows dC
+      e = w;
+      std::copy(e.begin(), e.end(), pety);
+      HCHK(hipMemcpyAsync(deig, pety, nb * sizeof(double), hipMemcpyHostToDevice, c->st));
+    }
+    HCHK(rotate_rect(c->st, Y, Yn, N, mi, nb, dC));
+    HCHK(rotate_rect(c->st, HY, HYn, N, mi, nb, dC));
+    HCHK(rotate_rect(c->st, Y + off, Yn + 2 * off, N, mi - nb, nb, dCp));
+    HCHK(rotate_rect(c->st, HY + off, HYn + 2 * off, N, mi - nb, nb, dCp));
     std::swap(Y, Yn);
     std::swap(HY, HYn);
     have_p = true;

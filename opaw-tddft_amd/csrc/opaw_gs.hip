@@ -622,6 +622,7 @@ hipError_t sub_real(hipStream_t st, const double* a, const double* b, long long 
 }
 
 hipError_t rotate_rect(hipStream_t st, const c64* in, c64* out, long long N, int m, int nout, const c64* T) {
+  if (m < 1 || nout < 1 || N < 1) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_rotate_mfma, dim3((unsigned)((N + kRmP - 1) / kRmP), (nout + kRmJ - 1) / kRmJ), dim3(256), 0,
                      st, in, out, N, m, nout, T);
   return hipGetLastError();

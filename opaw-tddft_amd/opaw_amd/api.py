@@ -352,6 +352,12 @@ class OpawGPU:
     def set_diis(self, ndi: int = 4, mix: float = 0.15):
         lib.call("opaw_set_diis", self.ctx, int(ndi), float(mix))
 
+    def set_subspace_device(self, min_m: int = 144):
+        """block_solve's Rayleigh-Ritz pencils of dimension >= min_m run on
+        the device (rocSOLVER), smaller ones on the host; 0: always the
+        device."""
+        lib.call("opaw_set_subspace_device", self.ctx, int(min_m))
+
     def ncpaw_make_hamiltonian(self, sys: int = 0, diis: bool = True):
         lib.call("opaw_ncpaw_make_hamiltonian", self.ctx, sys, int(diis))
 

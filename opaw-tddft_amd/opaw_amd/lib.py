@@ -101,6 +101,7 @@ SIGNATURES = {
     "opaw_paw_sphere": (I, [I, I, I, D, D, D, DP, D, D, I, DP, I, I, DP, I, IP, IP, DP]),
     "opaw_set_nocc": (I, [P, I]),
     "opaw_set_diis": (I, [P, I, D]),
+    "opaw_set_subspace_device": (I, [P, I]),
     "opaw_ncpaw_make_hamiltonian": (I, [P, I, I]),
     "opaw_ncpaw_make_hamiltonian_density": (I, [P, I, DP, I]),
     "opaw_hminmax": (I, [P, I, I, I, D, C.c_ulonglong, DP, DP]),

@@ -134,15 +134,17 @@ def _update_basis(spec, st, phi, P, nh):
     return np.concatenate([phi, W]), np.concatenate([H, O.shs(st, W, 0)])
 
 
+@pytest.mark.parametrize("route", ["host", "device"])
 @pytest.mark.parametrize("learned", [False, True])
-def test_one_preconditioned_update_matches_oracle(learned):
+def test_one_preconditioned_update_matches_oracle(learned, route):
     """One LOBPCG update on the GPU (residuals r = H phi - e phi, band
     kinetic energies t, w = M r in k space with M evaluated per (band, k
     shell) and looked up per k point, Rayleigh-Ritz on [phi, w]) against the
     same update formed on the CPU with the oracle's shs, FFTs and the
     multiplier evaluated at every k point.  A k point given the wrong shell,
     or a wrong multiplier, changes w and moves the Ritz values and the
-    updated span at first order."""
+    updated span at first order.  The pencil is solved on the host and on
+    the device (rocSOLVER)."""
     import scipy.linalg
     spec, st, phi = _system(nb=4)
     nh = 8
@@ -152,6 +154,7 @@ def test_one_preconditioned_update_matches_oracle(learned):
         P = mlp_init(nh, seed=3, scale=0.4)
         P[4 * nh + nh * nh:] = np.random.default_rng(4).normal(0, 0.3, nh + 1)
     g = _gpu(spec, st, phi.shape[0])
+    g.set_subspace_device(0 if route == "device" else 10**6)
     g.write_orbitals(0, phi)
     eig, _, its = g.block_solve(1, 0.0, mlp=P, nh=nh)
     assert its == 1
@@ -185,8 +188,10 @@ def _range_ritz(S, Hm, nb, floor=1e-12):
     return int(keep.sum()), w[:nb], X @ V[:, :nb]
 
 
-def test_device_rayleigh_ritz_drops_dependent_directions():
-    """The device Rayleigh-Ritz's rank-revealing route: with two equal
+@pytest.mark.parametrize("route", ["host", "device"])
+def test_rayleigh_ritz_drops_dependent_directions(route):
+    """The Rayleigh-Ritz's rank-revealing route, on the host and on the
+    device: with two equal
     orbitals the [phi, w] block has two exactly dependent columns; the
     update must return the Ritz values of the pencil on the range of the
     overlap and orthonormal orbitals in that range.  Keeping a dropped
@@ -196,6 +201,7 @@ def test_device_rayleigh_ritz_drops_dependent_directions():
     phi = phi.copy()
     phi[3] = phi[2]
     g = _gpu(spec, st, phi.shape[0])
+    g.set_subspace_device(0 if route == "device" else 10**6)
     g.write_orbitals(0, phi)
     eig, _, its = g.block_solve(1, 0.0)
     assert its == 1
@@ -215,14 +221,16 @@ def test_device_rayleigh_ritz_drops_dependent_directions():
     assert np.linalg.norm(resid) < 1e-9 * np.linalg.norm(out)
 
 
-def test_device_rayleigh_ritz_refuses_rank_below_bands():
+@pytest.mark.parametrize("route", ["host", "device"])
+def test_rayleigh_ritz_refuses_rank_below_bands(route):
     """Four equal orbitals: the [phi, w] block spans two directions, fewer
-    than the four bands; the solver fails with the rank and leaves the
-    resident orbitals as they were."""
+    than the four bands; the solver (host or device pencil) fails with the
+    rank and leaves the resident orbitals as they were."""
     from opaw_amd.lib import OpawError
     spec, st, phi = _system(nb=4)
     same = np.repeat(phi[:1], 4, axis=0)
     g = _gpu(spec, st, 4)
+    g.set_subspace_device(0 if route == "device" else 10**6)
     g.write_orbitals(0, same)
     with pytest.raises(OpawError, match="subspace rank 2 below the 4 bands"):
         g.block_solve(1, 0.0)
