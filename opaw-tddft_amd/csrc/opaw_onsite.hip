@@ -12,7 +12,7 @@
 // with n_s = (n_00 / sqrt(4pi) + core) / 2 the spin-up spherical density
 // (clamped at 1e-14), v = xcpbe option 1 (ABINIT's ixc = 7 PW92,
 // src/libpaw/0_xc_m_xcpbe.F90:881-945) and v', v'' by finite differences at
-// n_s (1 +- 1e-4); pawdijxcm (src/libpaw/4_m_pawdij.F90:1959-2178)
+// n_s (1 +- 1e-4), evaluated from a Taylor jet of v; pawdijxcm (src/libpaw/4_m_pawdij.F90:1959-2178)
 // integrates sum_lm G(lm, ij) int (v_lm phi_i phi_j - vt_lm tphi_i tphi_j)
 // with simp_gen's weights.
 //
@@ -50,25 +50,112 @@ constexpr double kSqRsfac = 0.7876233178997433;
 constexpr double kSqRsfacInv = 1.269642451250142;
 constexpr double kA = 0.031091, kA1 = 0.21370, kB1 = 7.5957, kB2 = 3.5876, kB3 = 1.6382, kB4 = 0.49294;
 
-// xcpbe option 1, nspden = 1: potential of the spin-up density rho_up
-__device__ __forceinline__ double lda_v(double rho_up) {
+// pawxcm's differences of v at rho (1 +- delta) from a Taylor jet of v in s,
+// v(rho (1 + s)) = sum_k c_k s^k:
+//   d1 = (v(rho(1+d)) - v(rho(1-d))) / (4 d rho)             = (c1 + c3 d^2) / (2 rho)
+//   d2 = (v(rho(1+d)) + v(rho(1-d)) - 2 v(rho)) / (4 d^2 rho^2) = (c2 + c4 d^2) / (2 rho^2)
+// up to the c5 d^4 and c6 d^4 terms (below 1e-15 of the sums); the
+// difference quotients themselves would multiply a one-ulp difference in v
+// by 1/d^2 = 1e8.  Every power of rho in xcpbe is a monomial in (1 + s),
+// whose coefficients are binomial; the logarithm and the quotients use the
+// recurrences of Taylor-mode differentiation.
+constexpr int kJK = 5;
+struct Jet {
+  double c[kJK];
+};
+
+// a (1 + s)^p
+__device__ __forceinline__ Jet jmono(double a, double p) {
+  Jet r;
+#pragma unroll
+  for (int k = 0; k < kJK; ++k) {
+    r.c[k] = a;
+    a = a * (p - k) / (k + 1);
+  }
+  return r;
+}
+__device__ __forceinline__ void jaxpy(Jet& y, double a, const Jet& x) {
+#pragma unroll
+  for (int k = 0; k < kJK; ++k) y.c[k] += a * x.c[k];
+}
+__device__ __forceinline__ Jet jmul(const Jet& a, const Jet& b) {
+  Jet r;
+#pragma unroll
+  for (int k = 0; k < kJK; ++k) {
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i <= k; ++i) acc += a.c[i] * b.c[k - i];
+    r.c[k] = acc;
+  }
+  return r;
+}
+// a / b
+__device__ __forceinline__ Jet jdiv(const Jet& a, const Jet& b) {
+  Jet r;
+  const double i0 = 1.0 / b.c[0];
+#pragma unroll
+  for (int k = 0; k < kJK; ++k) {
+    double acc = a.c[k];
+#pragma unroll
+    for (int i = 1; i <= k; ++i) acc -= b.c[i] * r.c[k - i];
+    r.c[k] = acc * i0;
+  }
+  return r;
+}
+// log a for k >= 1 (c[0] is set by the caller)
+__device__ __forceinline__ Jet jlog_tail(const Jet& a) {
+  Jet r;
+  r.c[0] = 0.0;
+  const double i0 = 1.0 / a.c[0];
+#pragma unroll
+  for (int k = 1; k < kJK; ++k) {
+    double acc = a.c[k];
+#pragma unroll
+    for (int i = 1; i < k; ++i) acc -= ((double)i / k) * r.c[i] * a.c[k - i];
+    r.c[k] = acc * i0;
+  }
+  return r;
+}
+
+// xcpbe option 1, nspden = 1, as a jet in s: the potential of the spin-up
+// density rho_up (1 + s).  With t = (2 rho_up)^-1/3 and x = t^1/2:
+// rs ~ t (1+s)^-1/3, sqrt(rs) ~ x (1+s)^-1/6, rs^-1/2 ~ 2 rho_up t^2 x
+// (1+s)^1/6 and the exchange ~ (1+s)^1/3.  The value c[0] follows xcpbe's
+// own operations.
+__device__ __forceinline__ Jet lda_v_jet(double rho_up) {
   const double rum13 = 1.0 / cbrt(rho_up);
-  const double rhotmot = kTwoM13 * rum13;
-  const double rhotot = 2.0 * rho_up;
-  const double rhotmo6 = sqrt(rhotmot);
-  const double rhoto6 = rhotot * rhotmot * rhotmot * rhotmo6;
-  const double ex = -kThreeFourthDivPi * kSixPi2_13 * rum13 * rum13 * rho_up;
-  const double rs = kRsfac * rhotmot;
-  const double sqr_rs = kSqRsfac * rhotmo6;
-  const double rsm1_2 = kSqRsfacInv * rhoto6;
-  const double q0 = -2.0 * kA * (1.0 + kA1 * rs);
-  const double q1 = 2.0 * kA * (kB1 * sqr_rs + kB2 * rs + kB3 * rs * sqr_rs + kB4 * rs * rs);
-  const double q1p = kA * (kB1 * rsm1_2 + 2.0 * kB2 + 3.0 * kB3 * sqr_rs + 4.0 * kB4 * rs);
-  const double den = 1.0 / (q1 * q1 + q1);
-  const double lg = -log(q1 * q1 * den);
-  const double ecrs = q0 * lg;
-  const double decrs = -2.0 * kA * kA1 * lg - q0 * q1p * den;
-  return (4.0 / 3.0) * ex + ecrs - rs * (1.0 / 3.0) * decrs;
+  const double t = kTwoM13 * rum13;
+  const double x = sqrt(t);
+  const double rhoto6 = 2.0 * rho_up * t * t * x;
+  const double rs = kRsfac * t, sqr_rs = kSqRsfac * x, rsm1_2 = kSqRsfacInv * rhoto6;
+  // q0 = -2A (1 + A1 rs)
+  Jet q0 = jmono(-2.0 * kA * kA1 * rs, -1.0 / 3.0);
+  q0.c[0] += -2.0 * kA;
+  // q1 = 2A (B1 sqrt(rs) + B2 rs + B3 rs^3/2 + B4 rs^2)
+  Jet q1 = jmono(2.0 * kA * kB1 * sqr_rs, -1.0 / 6.0);
+  jaxpy(q1, 1.0, jmono(2.0 * kA * kB2 * rs, -1.0 / 3.0));
+  jaxpy(q1, 1.0, jmono(2.0 * kA * kB3 * rs * sqr_rs, -0.5));
+  jaxpy(q1, 1.0, jmono(2.0 * kA * kB4 * rs * rs, -2.0 / 3.0));
+  // q1p = A (B1 rs^-1/2 + 2 B2 + 3 B3 sqrt(rs) + 4 B4 rs)
+  Jet q1p = jmono(kA * kB1 * rsm1_2, 1.0 / 6.0);
+  q1p.c[0] += 2.0 * kA * kB2;
+  jaxpy(q1p, 1.0, jmono(3.0 * kA * kB3 * sqr_rs, -1.0 / 6.0));
+  jaxpy(q1p, 1.0, jmono(4.0 * kA * kB4 * rs, -1.0 / 3.0));
+  // den = 1 / (q1^2 + q1), lg = -log(q1^2 den) = -log(q1 / (q1 + 1))
+  Jet q1p1 = q1;
+  q1p1.c[0] += 1.0;
+  const Jet den = jdiv(jmono(1.0, 0.0), jmul(q1, q1p1));
+  const double q10 = q1.c[0];
+  Jet lg = jlog_tail(q1p1);
+  jaxpy(lg, -1.0, jlog_tail(q1));
+  lg.c[0] = -log(q10 * q10 * (1.0 / (q10 * q10 + q10)));
+  // v = 4/3 ex + q0 lg - rs/3 (-2 A A1 lg - q0 q1p den)
+  Jet v = jmono((4.0 / 3.0) * (-kThreeFourthDivPi * kSixPi2_13 * rum13 * rum13 * rho_up), 1.0 / 3.0);
+  jaxpy(v, 1.0, jmul(q0, lg));
+  Jet decrs = jmul(jmul(q0, q1p), den);
+  jaxpy(decrs, 2.0 * kA * kA1, lg);
+  jaxpy(v, 1.0 / 3.0, jmul(jmono(rs, -1.0 / 3.0), decrs));
+  return v;
 }
 
 // pawxcm potential moments at one point from the density moments n (lm < L)
@@ -78,12 +165,12 @@ __device__ __forceinline__ double xc_moments(const double (&n)[kOnsiteMaxLm], in
   const double r0 = n[0] + kSqfpi * core;
   double rs = r0 * (0.5 / kSqfpi);
   if (rs < kDenpos) rs = kDenpos;                      // mkdenpos
-  const double vi = lda_v(rs);
-  const double v1 = lda_v((1.0 + kDelta) * rs);
-  const double v2 = lda_v((1.0 - kDelta) * rs);
-  const double rinv = rs > kRhoMin ? (0.5 / kDelta) / rs : 0.0;
-  const double d1 = (v1 - v2) * 0.5 * rinv;
-  const double d2 = (v1 + v2 - 2.0 * vi) * rinv * rinv;
+  const Jet c = lda_v_jet(rs);
+  const double vi = c.c[0];
+  const double dd = kDelta * kDelta;
+  const bool on = rs > kRhoMin;
+  const double d1 = on ? (c.c[1] + c.c[3] * dd) / (2.0 * rs) : 0.0;
+  const double d2 = on ? (c.c[2] + c.c[4] * dd) / (2.0 * rs * rs) : 0.0;
   double v1sum = 0.0;
 #pragma unroll
   for (int lm = 1; lm < kOnsiteMaxLm; ++lm)

@@ -18,7 +18,9 @@ Restated here, each from the routine named:
   pawdensities     rho1, trho1 lm moments (m_paw_denpot.F90:1216-1574), opt_dens = 1
   invcb, xcpbe     rho^-1/3 by Newton iteration and the option-1 LSD
                    exchange + PW92 correlation (xc_invcb.F90, 0_xc_m_xcpbe.F90:881-945)
-  pawxcm           vxc moments, pawxcdev = 1 (2_m_pawxc.F90:3763-4513)
+  pawxcm           vxc moments, pawxcdev = 1 (2_m_pawxc.F90:3763-4513); its
+                   differences of v at rho (1 +- 1e-4) from a Taylor jet of v
+                   (pawxcm_differences)
   pawdijxcm        (4_m_pawdij.F90:1959-2178)
   pawdijhartree    (4_m_pawdij.F90:1228-1313)
   pawdijhat        (4_m_pawdij.F90:2227-2514) with libpaw's own g_l S_lm and q_ij^L
@@ -147,8 +149,11 @@ def xcpbe_lda(rho_up: np.ndarray, exact_invcb: bool = False) -> np.ndarray:
     density 2 rho_up).  exact_invcb: the cube roots by invcb's sequential
     iteration (what the reference runs); otherwise 1/cbrt, which agrees
     with it to ~1e-13 relative at every point that invcb iterates
-    (tests/test_libpaw_cpu.py pins the two against each other)."""
-    rho_up = np.asarray(rho_up, dtype=np.float64)
+    (tests/test_libpaw_cpu.py pins the two against each other).  A
+    long-double input is evaluated in long double (with cbrt)."""
+    rho_up = np.asarray(rho_up)
+    if rho_up.dtype != np.longdouble:
+        rho_up = rho_up.astype(np.float64)
     rum13 = invcb(rho_up) if exact_invcb else 1.0 / np.cbrt(rho_up)    # rho_updnm1_3
     rhom1_3 = TWOM1_3 * rum13                # (2 rho_up)^-1/3
     rhotot = 2.0 * rho_up
@@ -168,6 +173,114 @@ def xcpbe_lda(rho_up: np.ndarray, exact_invcb: bool = False) -> np.ndarray:
     ecrs = q0 * lg
     decrs_drs = -2.0 * EC0_AA * EC0_A1 * lg - q0 * q1p * den
     return v + ecrs - rs * (1.0 / 3.0) * decrs_drs
+
+
+# ---------------------------------------------------------------------------
+# pawxcm's differences of the potential, evaluated from a Taylor jet
+#
+# pawxcm takes v' and v'' of the spin-up potential v(rho) by differences at
+# rho (1 +- delta), delta = 1e-4 (2_m_pawxc.F90:3945-3979):
+#   d1 = (v(rho(1+d)) - v(rho(1-d))) / (4 d rho),
+#   d2 = (v(rho(1+d)) + v(rho(1-d)) - 2 v(rho)) / (4 d^2 rho^2).
+# With v(rho (1+s)) = sum_k c_k s^k these are exactly
+#   d1 = (c1 + c3 d^2 + c5 d^4 + ...) / (2 rho),
+#   d2 = (c2 + c4 d^2 + c6 d^4 + ...) / (2 rho^2),
+# so the jet below (c_0..c_4; the first terms left out, c5 d^4 and c6 d^4,
+# are below 1e-15 of the sums) gives the reference's differences without
+# their cancellation, which multiplies a one-ulp difference in v by
+# 1/d^2 = 1e8.  Two implementations whose cube roots or logarithms differ in
+# the last bit then agree to rounding in d1 and d2.  Jets are arrays
+# (K, npoints) of Taylor coefficients in s.
+
+JET_K = 5
+
+
+def _jmul(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    r = np.zeros_like(a)
+    for k in range(a.shape[0]):
+        for i in range(k + 1):
+            r[k] += a[i] * b[k - i]
+    return r
+
+
+def _jdiv(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    r = np.zeros_like(a)
+    for k in range(a.shape[0]):
+        acc = a[k].copy()
+        for i in range(1, k + 1):
+            acc -= b[i] * r[k - i]
+        r[k] = acc / b[0]
+    return r
+
+
+def _jlog(a: np.ndarray) -> np.ndarray:
+    r = np.zeros_like(a)
+    r[0] = np.log(a[0])
+    for k in range(1, a.shape[0]):
+        acc = a[k].copy()
+        for i in range(1, k):
+            acc -= (i / k) * r[i] * a[k - i]
+        r[k] = acc / a[0]
+    return r
+
+
+def _jpow1p(p: float, base: np.ndarray, K: int) -> np.ndarray:
+    """base (1 + s)^p: binomial coefficients times base"""
+    r = np.zeros((K,) + np.shape(base))
+    c = 1.0
+    for k in range(K):
+        r[k] = c * base
+        c = c * (p - k) / (k + 1)
+    return r
+
+
+def _jconst(x, K: int, n: int) -> np.ndarray:
+    r = np.zeros((K, n))
+    r[0] = x
+    return r
+
+
+def xcpbe_lda_jet(rho_up: np.ndarray, exact_invcb: bool = False) -> np.ndarray:
+    """Taylor coefficients c_k (JET_K, n) of xcpbe_lda(rho_up (1 + s)) in s,
+    the same operations as xcpbe_lda carried through the jet (the cube root
+    of rho_up once, its powers of (1 + s) by the binomial series)."""
+    rho_up = np.asarray(rho_up, dtype=np.float64)
+    K, n = JET_K, rho_up.shape[0]
+    rum13_0 = invcb(rho_up) if exact_invcb else 1.0 / np.cbrt(rho_up)
+    rum13 = _jpow1p(-1.0 / 3.0, rum13_0, K)                 # rho_up^-1/3
+    rhotmot = TWOM1_3 * rum13                               # (2 rho_up)^-1/3
+    rho = _jpow1p(1.0, rho_up, K)                           # rho_up (1 + s)
+    rhotot = 2.0 * rho
+    rhotmo6 = _jpow1p(-1.0 / 6.0, np.sqrt(TWOM1_3 * rum13_0), K)
+    rhoto6 = _jmul(_jmul(_jmul(rhotot, rhotmot), rhotmot), rhotmo6)
+    ex_lsd = -THREEFOURTH_DIVPI * SIXPI2_1_3 * _jmul(_jmul(rum13, rum13), rho)
+    v = (4.0 / 3.0) * ex_lsd
+    rs = RSFAC * rhotmot
+    sqr_rs = SQ_RSFAC * rhotmo6
+    rsm1_2 = SQ_RSFAC_INV * rhoto6
+    one = _jconst(1.0, K, n)
+    q0 = -2.0 * EC0_AA * (one + EC0_A1 * rs)
+    rs_sq = _jmul(rs, sqr_rs)
+    q1 = 2.0 * EC0_AA * (EC0_B1 * sqr_rs + EC0_B2 * rs + EC0_B3 * rs_sq + EC0_B4 * _jmul(rs, rs))
+    q1p = EC0_AA * (EC0_B1 * rsm1_2 + _jconst(2.0 * EC0_B2, K, n) + 3.0 * EC0_B3 * sqr_rs + 4.0 * EC0_B4 * rs)
+    q1q1 = _jmul(q1, q1)
+    den = _jdiv(one, q1q1 + q1)
+    lg = -_jlog(_jmul(q1q1, den))
+    ecrs = _jmul(q0, lg)
+    decrs_drs = -2.0 * EC0_AA * EC0_A1 * lg - _jmul(_jmul(q0, q1p), den)
+    return v + ecrs - (1.0 / 3.0) * _jmul(rs, decrs_drs)
+
+
+def pawxcm_differences(rhosph: np.ndarray, exact_invcb: bool = False) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """(v, d1, d2) of pawxcm at the spin-up spherical densities rhosph (after
+    mkdenpos): v(rho) and the two differences, zero where rho <= rho_min."""
+    c = xcpbe_lda_jet(rhosph, exact_invcb)
+    d2_ = DELTA * DELTA
+    on = rhosph > RHO_MIN
+    safe = np.where(on, rhosph, 1.0)
+    d1 = np.where(on, (c[1] + c[3] * d2_) / (2.0 * safe), 0.0)
+    d2 = np.where(on, (c[2] + c[4] * d2_) / (2.0 * safe * safe), 0.0)
+    return c[0], d1, d2
 
 
 # ---------------------------------------------------------------------------
@@ -225,13 +338,7 @@ def pawxcm_vxc(rhor: np.ndarray, corexc: np.ndarray, usecore: int, tab: Dict, ex
     invsqfpi2 = 0.5 / SQFPI
     rhosph = rho_updn[:, 0] * invsqfpi2
     rhosph = np.where(rhosph < XC_DENPOS, XC_DENPOS, rhosph)          # mkdenpos
-    vxci = xcpbe_lda(rhosph, exact_invcb)
-    vxc1 = xcpbe_lda((1.0 + DELTA) * rhosph, exact_invcb)
-    vxc2 = xcpbe_lda((1.0 - DELTA) * rhosph, exact_invcb)
-    fact = 0.5 / DELTA
-    rhoinv = np.where(rhosph > RHO_MIN, fact / rhosph, 0.0)
-    d1vxc = (vxc1 - vxc2) * 0.5 * rhoinv
-    d2vxc = (vxc1 + vxc2 - 2.0 * vxci) * rhoinv ** 2
+    vxci, d1vxc, d2vxc = pawxcm_differences(rhosph, exact_invcb)
     v1sum = np.zeros(nr)
     for lm in range(1, lmsz):
         v1sum += rho_updn[:, lm] ** 2

@@ -10,10 +10,10 @@ state (64^3), and a 100-step dipole trace on the smallest grid at 0.5 bohr
 that holds the molecule's projector boxes (40 x 32 x 24).  And the mixed
 path against FP64 over the reference's naphthalene run length (2000 steps
 at dt = 0.05 from the 64^3 ground state): the dipole traces and the
-absorption peaks.  The tolerances
-of the steps carry the on-site XC's conditioning (v'' by a second
-difference with delta = 1e-4 in pawxcm: D_ij reproducible to ~1e-8, see
-tests/test_gpu_onsite.py)."""
+absorption peaks.  pawxcm's differences of the LDA potential are
+evaluated from a Taylor jet on both sides (tests/test_gpu_onsite.py), so
+the step agrees with the oracle to rounding: measured 7e-16 on the
+orbitals, 7e-12 on the kick response, 2e-13 over the 100-step trace."""
 import os
 
 import numpy as np
@@ -108,9 +108,12 @@ def test_handoff_and_one_step(c1, tmp_path):
     phis = [phi0, O.excite_psi_bar_pert(st, phi0, spec.sm, spec.ipol)]
     print("oracle step at 64^3", flush=True)
     new, d_dip, _, _ = O.tddft_step(st, phis, spec.sm)
+    errs = [rel(out[s], new[s]) for s in (0, 1)]
+    derr = np.max(np.abs(d_gpu[0] - d_dip)) / np.max(np.abs(d_dip))
+    print(f"one step vs the oracle: orbitals {errs[0]:.2e} / {errs[1]:.2e}, d_dip {derr:.2e}", flush=True)
     for s in (0, 1):
-        assert rel(out[s], new[s]) < 1e-9, s
-    assert np.max(np.abs(d_gpu[0] - d_dip)) <= 1e-7 * np.max(np.abs(d_dip))
+        assert errs[s] < 1e-12, s
+    assert derr <= 1e-10
 
 
 def test_dipole_trace_100_steps(c1_small):
@@ -130,7 +133,7 @@ def test_dipole_trace_100_steps(c1_small):
     ref = np.array(ref)
     err = np.linalg.norm(d_gpu - ref) / np.linalg.norm(ref)
     print(f"100-step d_dip relative L2 error {err:.2e}", flush=True)
-    assert err <= 1e-8
+    assert err <= 1e-11
 
 
 def test_2000_steps_mixed_vs_fp64(c1):

@@ -2,12 +2,14 @@
 the reference's naphthalene input (cnt.ini geometry, JTH C/H datasets,
 56 x 52 x 32 at 0.5 bohr), against the CPU oracle.
 
-Tolerance of D_ij: 1e-7 of max |D|.  pawxcm takes v''(n) by a second
-difference with delta = 1e-4, which turns a last-bit difference of the
-cube root or the logarithm (HIP's ocml against the host libm, or invcb's
-iteration against cbrt) into ~1e-8 of D^xc; tests/test_libpaw_cpu.py shows
-the same spread between the reference's invcb and cbrt on the CPU.  Every
-other part of D (dij0, dijhartree, dijhat) and rho_ij is held to 1e-11."""
+Tolerance of D_ij: 1e-12 of max |D| (measured 2e-15).  pawxcm takes v'(n) and v''(n) by
+differences at n (1 +- 1e-4); evaluated directly, they would turn a
+last-bit difference of the cube root or the logarithm (HIP's device math
+against the host libm, or invcb's iteration against cbrt) into ~1e-8 of
+D^xc.  Both sides evaluate the same differences from a Taylor jet of v
+instead (tests/test_libpaw_cpu.py pins the jet against the three-point
+formulas in long double), so D^xc agrees to rounding.  Every other part of
+D (dij0, dijhartree, dijhat) and rho_ij is held to 1e-11."""
 import os
 
 import numpy as np
@@ -19,7 +21,7 @@ from opaw_amd import paw, synth
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "jth")
-DIJ_TOL = 1e-7
+DIJ_TOL = 1e-12
 
 
 def _torch():
@@ -64,7 +66,7 @@ def test_rebuild_dij_matches_oracle(naph):
     g.write_orbitals(1, phis[1])
     print("rebuild of both systems", flush=True)
     g.ncpaw_make_hamiltonian_tddft(-1)
-    dmax = 0.0
+    dmax = worst = 0.0
     for s in (0, 1):
         O.rebuild(st, phis[s], s)
         S = st.systems[s]
@@ -74,7 +76,10 @@ def test_rebuild_dij_matches_oracle(naph):
             d_gpu = g.read_dij(s, ia)
             d_ref = S.dij[ia - 1]
             dmax = max(dmax, float(np.max(np.abs(d_ref))))
-            assert np.max(np.abs(d_gpu - d_ref)) < DIJ_TOL * max(1.0, np.max(np.abs(d_ref))), (s, ia)
+            err = np.max(np.abs(d_gpu - d_ref)) / max(1.0, np.max(np.abs(d_ref)))
+            worst = max(worst, err)
+            assert err < DIJ_TOL, (s, ia, err)
+    print(f"D_ij GPU vs oracle: {worst:.2e} of max|D|", flush=True)
     assert dmax > 0.1
     g.close()
 

@@ -10,6 +10,8 @@ restate it:
   - xcpbe (ABINIT's ixc = 7): the libxc-form PW92 LDA of the grid oracle,
     written independently, and v = d(n eps)/dn;
   - invcb: numpy's cbrt;
+  - pawxcm's differences of v at rho (1 +- 1e-4) from a Taylor jet: the
+    same three-point formulas in long double;
   - pawxcm's first-order moment expansion: the exact angular projection of
     v_xc, error second order in the anisotropy;
   - Dij^xc: central differences of the exact (angular-quadrature) on-site
@@ -136,16 +138,40 @@ def test_invcb_matches_cbrt():
     assert np.max(np.abs(got[1:] - want[1:]) / want[1:]) < 1e-12
     # the first point uses the single-precision exponent
     assert abs(got[0] - r[0] ** np.float32(-1.0 / 3.0)) < 1e-15 * abs(got[0])
-    # D_ij^xc with the exact iteration equals the cbrt form to the noise
-    # floor of pawxcm's second difference (delta = 1e-4 amplifies the
-    # ~3e-13 convergence of invcb by 1/delta^2)
+    # D_ij^xc with the exact iteration equals the cbrt form: pawxcm's
+    # differences come from the Taylor jet, so the cube roots' last-bit
+    # differences are not amplified by 1/delta^2
     for sym in ("C", "H"):
         t = L.load_type(os.path.join(GOLD, f"{sym}.LDA_PW-JTH.xml"))
         tab = L.onsite_spec(t)
         rho = _rho_psd(t.lmn_size)
         d_exact = LO.onsite_xc_dij(dict(tab), rho, exact_invcb=True)
         d_fast = LO.onsite_xc_dij(dict(tab), rho)
-        assert np.max(np.abs(d_exact - d_fast)) < 1e-7 * max(1.0, np.max(np.abs(d_fast)))
+        assert np.max(np.abs(d_exact - d_fast)) < 1e-11 * max(1.0, np.max(np.abs(d_fast)))
+
+
+def test_pawxcm_jet_equals_the_reference_differences():
+    """pawxcm's v, v' and v'' differences at rho (1 +- 1e-4) from the
+    Taylor jet against the same three-point formulas evaluated in long
+    double (80-bit; its own rounding, amplified by 1/delta^2, stays near
+    1e-10).  Leaving out the c3 d^2 or c4 d^2 term of the jet moves v' or v''
+    by 2e-9 / 4e-9; in double the three-point v'' is off by up to 2e-2."""
+    rho = np.geomspace(1e-5, 50.0, 400)
+    v, d1, d2 = LO.pawxcm_differences(rho)
+    r = rho.astype(np.longdouble)
+    dl = np.longdouble(LO.DELTA)
+    vi = LO.xcpbe_lda(r)
+    v1 = LO.xcpbe_lda((1 + dl) * r)
+    v2 = LO.xcpbe_lda((1 - dl) * r)
+    rinv = np.longdouble(0.5) / dl / r
+    d1_ref = ((v1 - v2) * np.longdouble(0.5) * rinv).astype(np.float64)
+    d2_ref = ((v1 + v2 - 2 * vi) * rinv * rinv).astype(np.float64)
+    assert np.max(np.abs(v / vi.astype(np.float64) - 1)) < 1e-14
+    assert np.max(np.abs(d1 / d1_ref - 1)) < 1e-12
+    assert np.max(np.abs(d2 / d2_ref - 1)) < 1.5e-9
+    # zero at and below rho_min, as pawxcm's rhoinv
+    _, z1, z2 = LO.pawxcm_differences(np.array([LO.RHO_MIN, 0.5 * LO.RHO_MIN]))
+    assert np.all(z1 == 0.0) and np.all(z2 == 0.0)
 
 
 def _rho_psd(ms, seed=0):
