@@ -56,33 +56,38 @@ constexpr double kA = 0.031091, kA1 = 0.21370, kB1 = 7.5957, kB2 = 3.5876, kB3 =
 //   d2 = (v(rho(1+d)) + v(rho(1-d)) - 2 v(rho)) / (4 d^2 rho^2) = (c2 + c4 d^2) / (2 rho^2)
 // up to the c5 d^4 and c6 d^4 terms (below 1e-15 of the sums); the
 // difference quotients themselves would multiply a one-ulp difference in v
-// by 1/d^2 = 1e8.  Every power of rho in xcpbe is a monomial in (1 + s),
-// whose coefficients are binomial; the logarithm and the quotients use the
+// by 1/d^2 = 1e8.  c3 d^2 and c4 d^2 are ~1e-8 of the sums, so c3 and c4 come
+// from an FP32 jet (their 1e-7 rounding lands at 1e-15) and c0..c2 from an
+// FP64 one.  Every power of rho in xcpbe is a monomial in (1 + s), whose
+// coefficients are binomial; the logarithm and the quotients use the
 // recurrences of Taylor-mode differentiation.
-constexpr int kJK = 5;
+template <typename T, int K>
 struct Jet {
-  double c[kJK];
+  T c[K];
 };
 
 // a (1 + s)^p
-__device__ __forceinline__ Jet jmono(double a, double p) {
-  Jet r;
+template <typename T, int K>
+__device__ __forceinline__ Jet<T, K> jmono(T a, T p) {
+  Jet<T, K> r;
 #pragma unroll
-  for (int k = 0; k < kJK; ++k) {
+  for (int k = 0; k < K; ++k) {
     r.c[k] = a;
-    a = a * (p - k) / (k + 1);
+    a = a * (p - (T)k) / (T)(k + 1);
   }
   return r;
 }
-__device__ __forceinline__ void jaxpy(Jet& y, double a, const Jet& x) {
+template <typename T, int K>
+__device__ __forceinline__ void jaxpy(Jet<T, K>& y, T a, const Jet<T, K>& x) {
 #pragma unroll
-  for (int k = 0; k < kJK; ++k) y.c[k] += a * x.c[k];
+  for (int k = 0; k < K; ++k) y.c[k] += a * x.c[k];
 }
-__device__ __forceinline__ Jet jmul(const Jet& a, const Jet& b) {
-  Jet r;
+template <typename T, int K>
+__device__ __forceinline__ Jet<T, K> jmul(const Jet<T, K>& a, const Jet<T, K>& b) {
+  Jet<T, K> r;
 #pragma unroll
-  for (int k = 0; k < kJK; ++k) {
-    double acc = 0.0;
+  for (int k = 0; k < K; ++k) {
+    T acc = 0;
 #pragma unroll
     for (int i = 0; i <= k; ++i) acc += a.c[i] * b.c[k - i];
     r.c[k] = acc;
@@ -90,12 +95,13 @@ __device__ __forceinline__ Jet jmul(const Jet& a, const Jet& b) {
   return r;
 }
 // a / b
-__device__ __forceinline__ Jet jdiv(const Jet& a, const Jet& b) {
-  Jet r;
-  const double i0 = 1.0 / b.c[0];
+template <typename T, int K>
+__device__ __forceinline__ Jet<T, K> jdiv(const Jet<T, K>& a, const Jet<T, K>& b) {
+  Jet<T, K> r;
+  const T i0 = (T)1 / b.c[0];
 #pragma unroll
-  for (int k = 0; k < kJK; ++k) {
-    double acc = a.c[k];
+  for (int k = 0; k < K; ++k) {
+    T acc = a.c[k];
 #pragma unroll
     for (int i = 1; i <= k; ++i) acc -= b.c[i] * r.c[k - i];
     r.c[k] = acc * i0;
@@ -103,58 +109,71 @@ __device__ __forceinline__ Jet jdiv(const Jet& a, const Jet& b) {
   return r;
 }
 // log a for k >= 1 (c[0] is set by the caller)
-__device__ __forceinline__ Jet jlog_tail(const Jet& a) {
-  Jet r;
-  r.c[0] = 0.0;
-  const double i0 = 1.0 / a.c[0];
+template <typename T, int K>
+__device__ __forceinline__ Jet<T, K> jlog_tail(const Jet<T, K>& a) {
+  Jet<T, K> r;
+  r.c[0] = 0;
+  const T i0 = (T)1 / a.c[0];
 #pragma unroll
-  for (int k = 1; k < kJK; ++k) {
-    double acc = a.c[k];
+  for (int k = 1; k < K; ++k) {
+    T acc = a.c[k];
 #pragma unroll
-    for (int i = 1; i < k; ++i) acc -= ((double)i / k) * r.c[i] * a.c[k - i];
+    for (int i = 1; i < k; ++i) acc -= ((T)i / (T)k) * r.c[i] * a.c[k - i];
     r.c[k] = acc * i0;
   }
   return r;
 }
 
+__device__ __forceinline__ double dev_cbrt(double x) { return cbrt(x); }
+__device__ __forceinline__ float dev_cbrt(float x) { return cbrtf(x); }
+__device__ __forceinline__ double dev_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float dev_sqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double dev_log(double x) { return log(x); }
+__device__ __forceinline__ float dev_log(float x) { return logf(x); }
+
 // xcpbe option 1, nspden = 1, as a jet in s: the potential of the spin-up
 // density rho_up (1 + s).  With t = (2 rho_up)^-1/3 and x = t^1/2:
 // rs ~ t (1+s)^-1/3, sqrt(rs) ~ x (1+s)^-1/6, rs^-1/2 ~ 2 rho_up t^2 x
-// (1+s)^1/6 and the exchange ~ (1+s)^1/3.  The value c[0] follows xcpbe's
-// own operations.
-__device__ __forceinline__ Jet lda_v_jet(double rho_up) {
-  const double rum13 = 1.0 / cbrt(rho_up);
-  const double t = kTwoM13 * rum13;
-  const double x = sqrt(t);
-  const double rhoto6 = 2.0 * rho_up * t * t * x;
-  const double rs = kRsfac * t, sqr_rs = kSqRsfac * x, rsm1_2 = kSqRsfacInv * rhoto6;
+// (1+s)^1/6 and the exchange ~ (1+s)^1/3.  lg = -log(q1^2 / (q1^2 + q1)) =
+// log(1 + 1/q1); its value follows xcpbe's own operations.
+template <typename T, int K>
+__device__ __forceinline__ Jet<T, K> lda_v_jet(T rho_up) {
+  using J = Jet<T, K>;
+  const T rum13 = (T)1 / dev_cbrt(rho_up);
+  const T t = (T)kTwoM13 * rum13;
+  const T x = dev_sqrt(t);
+  const T rhoto6 = (T)2 * rho_up * t * t * x;
+  const T rs = (T)kRsfac * t, sqr_rs = (T)kSqRsfac * x, rsm1_2 = (T)kSqRsfacInv * rhoto6;
+  const T A = (T)kA;
   // q0 = -2A (1 + A1 rs)
-  Jet q0 = jmono(-2.0 * kA * kA1 * rs, -1.0 / 3.0);
-  q0.c[0] += -2.0 * kA;
+  J q0 = jmono<T, K>((T)-2 * A * (T)kA1 * rs, (T)(-1.0 / 3.0));
+  q0.c[0] += (T)-2 * A;
   // q1 = 2A (B1 sqrt(rs) + B2 rs + B3 rs^3/2 + B4 rs^2)
-  Jet q1 = jmono(2.0 * kA * kB1 * sqr_rs, -1.0 / 6.0);
-  jaxpy(q1, 1.0, jmono(2.0 * kA * kB2 * rs, -1.0 / 3.0));
-  jaxpy(q1, 1.0, jmono(2.0 * kA * kB3 * rs * sqr_rs, -0.5));
-  jaxpy(q1, 1.0, jmono(2.0 * kA * kB4 * rs * rs, -2.0 / 3.0));
+  J q1 = jmono<T, K>((T)2 * A * (T)kB1 * sqr_rs, (T)(-1.0 / 6.0));
+  jaxpy(q1, (T)1, jmono<T, K>((T)2 * A * (T)kB2 * rs, (T)(-1.0 / 3.0)));
+  jaxpy(q1, (T)1, jmono<T, K>((T)2 * A * (T)kB3 * rs * sqr_rs, (T)-0.5));
+  jaxpy(q1, (T)1, jmono<T, K>((T)2 * A * (T)kB4 * rs * rs, (T)(-2.0 / 3.0)));
   // q1p = A (B1 rs^-1/2 + 2 B2 + 3 B3 sqrt(rs) + 4 B4 rs)
-  Jet q1p = jmono(kA * kB1 * rsm1_2, 1.0 / 6.0);
-  q1p.c[0] += 2.0 * kA * kB2;
-  jaxpy(q1p, 1.0, jmono(3.0 * kA * kB3 * sqr_rs, -1.0 / 6.0));
-  jaxpy(q1p, 1.0, jmono(4.0 * kA * kB4 * rs, -1.0 / 3.0));
-  // den = 1 / (q1^2 + q1), lg = -log(q1^2 den) = -log(q1 / (q1 + 1))
-  Jet q1p1 = q1;
-  q1p1.c[0] += 1.0;
-  const Jet den = jdiv(jmono(1.0, 0.0), jmul(q1, q1p1));
-  const double q10 = q1.c[0];
-  Jet lg = jlog_tail(q1p1);
-  jaxpy(lg, -1.0, jlog_tail(q1));
-  lg.c[0] = -log(q10 * q10 * (1.0 / (q10 * q10 + q10)));
+  J q1p = jmono<T, K>(A * (T)kB1 * rsm1_2, (T)(1.0 / 6.0));
+  q1p.c[0] += (T)2 * A * (T)kB2;
+  jaxpy(q1p, (T)1, jmono<T, K>((T)3 * A * (T)kB3 * sqr_rs, (T)(-1.0 / 6.0)));
+  jaxpy(q1p, (T)1, jmono<T, K>((T)4 * A * (T)kB4 * rs, (T)(-1.0 / 3.0)));
+  // 1/q1, w = 1 + 1/q1 = (q1 + 1)/q1, den = 1/(q1^2 + q1) = (1/q1)^2 / w
+  const J one = jmono<T, K>((T)1, (T)0);
+  const J iq1 = jdiv(one, q1);
+  J w = iq1;
+  w.c[0] += (T)1;
+  const J den = jdiv(jmul(iq1, iq1), w);
+  const T q10 = q1.c[0];
+  J lg = jlog_tail(w);
+  lg.c[0] = -dev_log(q10 * q10 * ((T)1 / (q10 * q10 + q10)));
   // v = 4/3 ex + q0 lg - rs/3 (-2 A A1 lg - q0 q1p den)
-  Jet v = jmono((4.0 / 3.0) * (-kThreeFourthDivPi * kSixPi2_13 * rum13 * rum13 * rho_up), 1.0 / 3.0);
-  jaxpy(v, 1.0, jmul(q0, lg));
-  Jet decrs = jmul(jmul(q0, q1p), den);
-  jaxpy(decrs, 2.0 * kA * kA1, lg);
-  jaxpy(v, 1.0 / 3.0, jmul(jmono(rs, -1.0 / 3.0), decrs));
+  J v = jmono<T, K>((T)(4.0 / 3.0) * ((T)(-kThreeFourthDivPi * kSixPi2_13) * rum13 * rum13 * rho_up),
+                    (T)(1.0 / 3.0));
+  jaxpy(v, (T)1, jmul(q0, lg));
+  J decrs = jmul(jmul(q0, q1p), den);
+  jaxpy(decrs, (T)2 * A * (T)kA1, lg);
+  jaxpy(v, (T)(1.0 / 3.0), jmul(jmono<T, K>(rs, (T)(-1.0 / 3.0)), decrs));
   return v;
 }
 
@@ -165,12 +184,13 @@ __device__ __forceinline__ double xc_moments(const double (&n)[kOnsiteMaxLm], in
   const double r0 = n[0] + kSqfpi * core;
   double rs = r0 * (0.5 / kSqfpi);
   if (rs < kDenpos) rs = kDenpos;                      // mkdenpos
-  const Jet c = lda_v_jet(rs);
+  const Jet<double, 3> c = lda_v_jet<double, 3>(rs);
+  const Jet<float, 5> ct = lda_v_jet<float, 5>((float)rs);
   const double vi = c.c[0];
   const double dd = kDelta * kDelta;
   const bool on = rs > kRhoMin;
-  const double d1 = on ? (c.c[1] + c.c[3] * dd) / (2.0 * rs) : 0.0;
-  const double d2 = on ? (c.c[2] + c.c[4] * dd) / (2.0 * rs * rs) : 0.0;
+  const double d1 = on ? (c.c[1] + (double)ct.c[3] * dd) / (2.0 * rs) : 0.0;
+  const double d2 = on ? (c.c[2] + (double)ct.c[4] * dd) / (2.0 * rs * rs) : 0.0;
   double v1sum = 0.0;
 #pragma unroll
   for (int lm = 1; lm < kOnsiteMaxLm; ++lm)
