@@ -643,6 +643,86 @@ hipError_t fft_s_pass(int cls, int n, int axis, int dir, int epi, hipStream_t st
 }
 
 // ---------------------------------------------------------------------------
+// The kinetic z pass of 64-point lines held in registers: out = IFFT_z(ek *
+// FFT_z(in)) along z for XT x-columns of one (field, y) per workgroup,
+// 64 = 8 x 8.  Thread (l, j) loads z = j + 8m (m < 8) of column l; a radix-8
+// DFT over m, the twiddle w64^(j k1) and one LDS exchange give thread
+// (l, k1) the values X[k1 + 8 k2] of its column after a second radix-8 DFT;
+// it multiplies them by ek, and the mirror sequence (radix-8 inverse over
+// k2, twiddle w64^-(a k1), exchange, radix-8 inverse over k1) leaves thread
+// (l, a) holding z = a + 8b, stored as it was loaded.  Two LDS exchanges
+// per transform in place of the Stockham tile's stage round trips; the LDS
+// is column-fastest, so every exchange is conflict-free.  In place is
+// allowed (a workgroup reads its columns before it writes them).
+template <typename R, int XT>
+__global__ __launch_bounds__(XT * 8) void k_zpass64(const cx<R>* in, cx<R>* out, int nx, int ny, long long N,
+                                                   const R* __restrict__ ek, const cx<R>* __restrict__ twg) {
+  __shared__ cx<R> x[64 * XT];
+  const int l = threadIdx.x % XT, j = threadIdx.x / XT;
+  const int col = blockIdx.x * XT + l, y = blockIdx.y, f = blockIdx.z;
+  const long long sz = (long long)nx * ny;
+  const long long base = (long long)f * N + col + (long long)nx * y;
+  cx<R> v[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) v[m] = in[base + (j + 8 * m) * sz];
+  fft::dft<R, 8, -1>(v);                                  // over m: index k1
+#pragma unroll
+  for (int k = 1; k < 8; ++k) v[k] = cmul(v[k], twg[(j * k) & 63]);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) x[(j * 8 + k) * XT + l] = v[k];
+  __syncthreads();
+  const int k1 = j;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) v[i] = x[(i * 8 + k1) * XT + l];
+  fft::dft<R, 8, -1>(v);                                  // over j: v[k2] = X[k1 + 8 k2]
+  const R* e = ek + col + (long long)nx * y;
+#pragma unroll
+  for (int k2 = 0; k2 < 8; ++k2) {
+    const R t = e[(k1 + 8 * k2) * sz];
+    v[k2] = mk(v[k2].x * t, v[k2].y * t);
+  }
+  fft::dft<R, 8, 1>(v);                                   // over k2: index a
+#pragma unroll
+  for (int a = 1; a < 8; ++a) {
+    const cx<R> w = twg[(a * k1) & 63];
+    v[a] = cmul(v[a], mk(w.x, -w.y));
+  }
+  __syncthreads();                                        // the exchange reads above are done
+#pragma unroll
+  for (int a = 0; a < 8; ++a) x[(k1 * 8 + a) * XT + l] = v[a];
+  __syncthreads();
+  const int a = j;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) v[i] = x[(i * 8 + a) * XT + l];
+  fft::dft<R, 8, 1>(v);                                   // over k1: v[b] = out(z = a + 8 b)
+#pragma unroll
+  for (int b = 0; b < 8; ++b) out[base + (a + 8 * b) * sz] = v[b];
+}
+
+// columns per workgroup of k_zpass64 for an nx-wide grid (0: not covered)
+static int zpass64_xt(const GridGeom& g) {
+  if (g.nz != 64) return 0;
+  if (g.nx % 32 == 0) return 32;
+  if (g.nx % 16 == 0) return 16;
+  return 0;
+}
+
+template <typename R>
+static hipError_t zpass64(const GridGeom& g, hipStream_t st, const cx<R>* in, cx<R>* out, int nf, const R* ek,
+                          const cx<R>* twz) {
+  const int xt = zpass64_xt(g);
+  const long long N = (long long)g.nx * g.ny * g.nz;
+  const dim3 grid((unsigned)(g.nx / xt), (unsigned)g.ny, (unsigned)nf);
+  prof_begin(st, KC_FFT_Z);
+  if (xt == 32)
+    hipLaunchKernelGGL((k_zpass64<R, 32>), grid, dim3(256), 0, st, in, out, g.nx, g.ny, N, ek, twz);
+  else
+    hipLaunchKernelGGL((k_zpass64<R, 16>), grid, dim3(128), 0, st, in, out, g.nx, g.ny, N, ek, twz);
+  prof_end(st, KC_FFT_Z);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Kinetic + local potential (+ gathered projector terms):
 //   out_f = IFFT(ek * FFT(u_f)) + vks(sys(f)) * u_f [+ cnl_f],
 //   u_f = in_f [+ cpre_f]   (compact corrections on the union of boxes)
@@ -678,7 +758,11 @@ hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* in, cx
                                       upt, uzr, cpre, nullptr, U);
     prof_end(st, KC_FFT_X);
     if (e) return e;
-    if ((e = fft_s_pass<R>(KC_FFT_Z, g.nz, 2, -1, EPI_MULT, st, tmp, v, tmp, v, Lz, twz, ek, vt))) return e;
+    if (zpass64_xt(g) > 0) {
+      if ((e = zpass64<R>(g, st, tmp, tmp, nf, ek, twz))) return e;
+    } else if ((e = fft_s_pass<R>(KC_FFT_Z, g.nz, 2, -1, EPI_MULT, st, tmp, v, tmp, v, Lz, twz, ek, vt))) {
+      return e;
+    }
     if ((e = hand_over())) return e;
     if (before_inv && (e = hipStreamWaitEvent(si, before_inv, 0))) return e;
     prof_begin(si, KC_FFT_XH);
