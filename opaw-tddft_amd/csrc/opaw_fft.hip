@@ -699,6 +699,133 @@ __global__ __launch_bounds__(XT * 8) void k_zpass64(const cx<R>* in, cx<R>* out,
   for (int b = 0; b < 8; ++b) out[base + (a + 8 * b) * sz] = v[b];
 }
 
+// 16-point DFT in registers with sign S (natural order in and out):
+// n = 4 n1 + n2, k = k1 + 4 k2, four DFT4s over n1, the twiddles
+// w16^(n2 k1), four DFT4s over n2
+template <int S>
+__device__ __forceinline__ void dft16(c64 (&v)[16]) {
+  constexpr double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173, h = 0.70710678118654752440;
+  const c64 w[4][4] = {{mk(1.0, 0.0), mk(1.0, 0.0), mk(1.0, 0.0), mk(1.0, 0.0)},
+                       {mk(1.0, 0.0), mk(c1, S * s1), mk(h, S * h), mk(s1, S * c1)},
+                       {mk(1.0, 0.0), mk(h, S * h), mk(0.0, S * 1.0), mk(-h, S * h)},
+                       {mk(1.0, 0.0), mk(s1, S * c1), mk(-h, S * h), mk(-c1, -S * s1)}};
+  c64 t[4][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    c64 a[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+    fft::dft<double, 4, S>(a);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) t[n2][k1] = cmul(a[k1], w[n2][k1]);
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    c64 b[4] = {t[0][k1], t[1][k1], t[2][k1], t[3][k1]};
+    fft::dft<double, 4, S>(b);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = b[k2];
+  }
+}
+
+// FP64 passes of 128-point lines in registers along y (AXIS 1) or z
+// (AXIS 2), XT x-columns of one (other coordinate o) per workgroup: 128 =
+// 16 x 8.  Thread (l, j) loads z = j + 8m (m < 16; zero outside the input
+// window [la, lb)), takes a DFT16 over m, the twiddle w128^(S j k1) and one
+// LDS exchange; a DFT8 over j for k1 = j and j + 8 leaves it X[k1 + 16 k2].
+//   MODE 0: one transform of sign S; the outputs inside [sa, sb) are stored.
+//   MODE 1: S = -1, then X * tab (read through vtab) and the inverse by the
+//           mirror sequence (inverse DFT8 over k2, twiddle w128^-(jj k1),
+//           exchange, inverse DFT16 over k1), which returns z = j + 8m to
+//           thread (l, j); the values inside [sa, sb) are stored.
+// In place is allowed: every line is read before the first barrier.
+template <int AXIS, int MODE, int S, int XT>
+__global__ __launch_bounds__(XT * 8) void k_lines128(const c64* in, c64* out, View v, SLines L,
+                                                    const c64* __restrict__ twg, const double* __restrict__ tab,
+                                                    View vtab) {
+  __shared__ c64 x[128 * XT];
+  const int l = threadIdx.x % XT, j = threadIdx.x / XT;
+  const int col = blockIdx.x * XT + l;
+  const int o = L.o0 + blockIdx.y, f = blockIdx.z;
+  const bool on = col < L.nx;
+  auto addr = [&](int z) -> long long { return AXIS == 1 ? vaddr(v, f, col, z, o) : vaddr(v, f, col, o, z); };
+  c64 u[16];
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const int z = j + 8 * m;
+    u[m] = (on && z >= L.la && z < L.lb) ? in[addr(z)] : mk(0.0, 0.0);
+  }
+  dft16<S>(u);                                            // over m: index k1
+#pragma unroll
+  for (int k1 = 1; k1 < 16; ++k1) {
+    const c64 w = twg[(j * k1) & 127];
+    u[k1] = cmul(u[k1], mk(w.x, S * -w.y));               // w128^(S j k1); twg = e^(-2 pi i m/128)
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < 16; ++k1) x[(j * 16 + k1) * XT + l] = u[k1];
+  __syncthreads();
+  c64 r[2][8];                                            // r[h][k2] = X[k1 + 16 k2], k1 = j + 8 h
+#pragma unroll
+  for (int hh = 0; hh < 2; ++hh) {
+    const int k1 = j + 8 * hh;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) r[hh][i] = x[(i * 16 + k1) * XT + l];
+    fft::dft<double, 8, S>(r[hh]);
+  }
+  if constexpr (MODE == 0) {
+#pragma unroll
+    for (int hh = 0; hh < 2; ++hh)
+#pragma unroll
+      for (int k2 = 0; k2 < 8; ++k2) {
+        const int k = j + 8 * hh + 16 * k2;
+        if (on && k >= L.sa && k < L.sb) out[addr(k)] = r[hh][k2];
+      }
+  } else {
+#pragma unroll
+    for (int hh = 0; hh < 2; ++hh) {
+      const int k1 = j + 8 * hh;
+#pragma unroll
+      for (int k2 = 0; k2 < 8; ++k2) {
+        const int k = k1 + 16 * k2;
+        const double t = on ? tab[AXIS == 1 ? taddr(vtab, col, k, o) : taddr(vtab, col, o, k)] : 0.0;
+        r[hh][k2] = mk(r[hh][k2].x * t, r[hh][k2].y * t);
+      }
+      fft::dft<double, 8, -S>(r[hh]);                     // over k2: index jj
+#pragma unroll
+      for (int jj = 1; jj < 8; ++jj) {
+        const c64 w = twg[(jj * k1) & 127];
+        r[hh][jj] = cmul(r[hh][jj], mk(w.x, S * w.y));    // w128^(-S jj k1)
+      }
+    }
+    __syncthreads();                                      // the exchange reads above are done
+#pragma unroll
+    for (int hh = 0; hh < 2; ++hh) {
+      const int k1 = j + 8 * hh;
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) x[(k1 * 8 + jj) * XT + l] = r[hh][jj];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) u[k1] = x[(k1 * 8 + j) * XT + l];
+    dft16<-S>(u);                                         // over k1: u[m] = out(z = j + 8 m)
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const int z = j + 8 * m;
+      if (on && z >= L.sa && z < L.sb) out[addr(z)] = u[m];
+    }
+  }
+}
+
+// a 128-point line pass of k_lines128 over the line set L (nf fields)
+template <int AXIS, int MODE, int S>
+static hipError_t lines128(int cls, hipStream_t st, const c64* in, c64* out, View v, SLines L, const c64* tw,
+                           const double* tab, View vtab) {
+  constexpr int XT = 16;
+  const dim3 grid((unsigned)((L.nx + XT - 1) / XT), (unsigned)L.lo, (unsigned)L.nf);
+  prof_begin(st, cls);
+  hipLaunchKernelGGL((k_lines128<AXIS, MODE, S, XT>), grid, dim3(XT * 8), 0, st, in, out, v, L, tw, tab, vtab);
+  prof_end(st, cls);
+  return hipGetLastError();
+}
+
 // columns per workgroup of k_zpass64 for an nx-wide grid (0: not covered)
 static int zpass64_xt(const GridGeom& g) {
   if (g.nz != 64) return 0;
@@ -987,9 +1114,15 @@ hipError_t hartree_fused(const GridGeom& g, hipStream_t st, const HartreeRebuild
   View big{0, 0, 0, 0, NX, NY};
   SLines Lz{NX, 0, NY, 1, oz, oz + nz, oz, oz + nz};
   const View voct = vk_octant_view(NX, NY, NZ);
-  if ((e = fft_s_pass<double>(KC_HARTREE, NZ, 2, -1, EPI_MULT, st, work, big, work, big, Lz, twz, vk, voct))) return e;
   SLines Ly2{NX, oz, nz, 1, 0, NY, ny / 2, ny / 2 + ny};
-  if ((e = fft_s_pass<double>(KC_HARTREE, NY, 1, 1, EPI_NONE, st, work, big, work, big, Ly2, twy, nullptr, big))) return e;
+  if (NZ == 128 && NY == 128) {
+    if ((e = lines128<2, 1, -1>(KC_HARTREE, st, work, work, big, Lz, twz, vk, voct))) return e;
+    if ((e = lines128<1, 0, 1>(KC_HARTREE, st, work, work, big, Ly2, twy, nullptr, big))) return e;
+  } else {
+    if ((e = fft_s_pass<double>(KC_HARTREE, NZ, 2, -1, EPI_MULT, st, work, big, work, big, Lz, twz, vk, voct))) return e;
+    if ((e = fft_s_pass<double>(KC_HARTREE, NY, 1, 1, EPI_NONE, st, work, big, work, big, Ly2, twy, nullptr, big)))
+      return e;
+  }
   XLines Lx2{ny / 2, ny, oz, nz, 1, 0, NX, nx / 2, nx / 2 + nx};
   return fft_x_pass<double>(KC_HARTREE, NX, 1, EPI_NONE, st, work, big, vh, small, Lx2, twx, nullptr, nullptr, 1);
 }
