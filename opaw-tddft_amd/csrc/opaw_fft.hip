@@ -826,6 +826,158 @@ static hipError_t lines128(int cls, hipStream_t st, const c64* in, c64* out, Vie
   return hipGetLastError();
 }
 
+// Roots of unity exp(2 pi i m / N), m < N, evaluated at compile time (Taylor
+// series after reduction to [-pi, pi]; 1e-16 accurate), for the composite
+// register DFTs below.
+constexpr double ct_sin_cos(double x, bool want_sin) {
+  const double pi = 3.14159265358979323846;
+  while (x > pi) x -= 2.0 * pi;
+  while (x < -pi) x += 2.0 * pi;
+  double sum = want_sin ? x : 1.0, term = sum;
+  const double x2 = x * x;
+  for (int k = 1; k < 26; ++k) {
+    const double a = want_sin ? (2.0 * k) * (2.0 * k + 1.0) : (2.0 * k - 1.0) * (2.0 * k);
+    term = -term * x2 / a;
+    sum += term;
+  }
+  return sum;
+}
+template <int N>
+struct Roots {
+  double c[N], s[N];
+  constexpr Roots() : c(), s() {
+    for (int m = 0; m < N; ++m) {
+      const double x = 2.0 * 3.14159265358979323846 * m / N;
+      c[m] = ct_sin_cos(x, false);
+      s[m] = ct_sin_cos(x, true);
+    }
+  }
+};
+
+// A*B-point DFT in registers with sign S (natural order in and out):
+// n = B n1 + n2, k = k1 + A k2: B DFT_A over n1, the twiddles
+// w_AB^(S n2 k1), A DFT_B over n2
+template <typename R, int A, int B, int S>
+__device__ __forceinline__ void dft_ab(cx<R>* v) {
+  constexpr Roots<A * B> W{};
+  cx<R> t[B][A];
+#pragma unroll
+  for (int n2 = 0; n2 < B; ++n2) {
+    cx<R> a[A];
+#pragma unroll
+    for (int n1 = 0; n1 < A; ++n1) a[n1] = v[B * n1 + n2];
+    fft::dft<R, A, S>(a);
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) {
+      const int m = (n2 * k1) % (A * B);
+      t[n2][k1] = m == 0 ? a[k1] : cmul(a[k1], mk((R)W.c[m], (R)(S * W.s[m])));
+    }
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < A; ++k1) {
+    cx<R> b[B];
+#pragma unroll
+    for (int n2 = 0; n2 < B; ++n2) b[n2] = t[n2][k1];
+    fft::dft<R, B, S>(b);
+#pragma unroll
+    for (int k2 = 0; k2 < B; ++k2) v[k1 + A * k2] = b[k2];
+  }
+}
+// the P-point register DFT of the generic z pass (P = N / 8)
+template <typename R, int P, int S>
+__device__ __forceinline__ void dft_p(cx<R>* v) {
+  if constexpr (P == 6) dft_ab<R, 3, 2, S>(v);
+  else dft_ab<R, 4, 3, S>(v);                              // P == 12
+}
+
+// The kinetic z pass of N-point lines in registers for N = 8 P (P = 6, 12:
+// nz = 48, 96), the scheme of k_zpass64 with the P-point DFT composite:
+// thread (l, j) loads z = j + 8m (m < P), DFT_P over m, twiddle w_N^(j k1),
+// exchange, DFT8 over j for k1 = j, j + 8 (< P), ek, and the mirror sequence.
+template <typename R, int N, int XT>
+__global__ __launch_bounds__(XT * 8) void k_zpass_gen(const cx<R>* in, cx<R>* out, int nx, int ny, long long NF,
+                                                      const R* __restrict__ ek, const cx<R>* __restrict__ twg) {
+  constexpr int P = N / 8, H = (P + 7) / 8;
+  __shared__ cx<R> x[N * XT];
+  const int l = threadIdx.x % XT, j = threadIdx.x / XT;
+  const int col = blockIdx.x * XT + l, y = blockIdx.y, f = blockIdx.z;
+  const long long sz = (long long)nx * ny;
+  const long long base = (long long)f * NF + col + (long long)nx * y;
+  cx<R> u[P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) u[m] = in[base + (j + 8 * m) * sz];
+  dft_p<R, P, -1>(u);                                      // over m: index k1
+#pragma unroll
+  for (int k = 1; k < P; ++k) u[k] = cmul(u[k], twg[(j * k) % N]);
+#pragma unroll
+  for (int k = 0; k < P; ++k) x[(j * P + k) * XT + l] = u[k];
+  __syncthreads();
+  cx<R> r[H][8];
+  const R* e = ek + col + (long long)nx * y;
+#pragma unroll
+  for (int hh = 0; hh < H; ++hh) {
+    const int k1 = j + 8 * hh;
+    if (k1 < P) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) r[hh][i] = x[(i * P + k1) * XT + l];
+      fft::dft<R, 8, -1>(r[hh]);                           // over j: r[k2] = X[k1 + P k2]
+#pragma unroll
+      for (int k2 = 0; k2 < 8; ++k2) {
+        const R t = e[(k1 + P * k2) * sz];
+        r[hh][k2] = mk(r[hh][k2].x * t, r[hh][k2].y * t);
+      }
+      fft::dft<R, 8, 1>(r[hh]);                            // over k2: index jj
+#pragma unroll
+      for (int jj = 1; jj < 8; ++jj) r[hh][jj] = cmulc(r[hh][jj], twg[(jj * k1) % N]);
+    }
+  }
+  __syncthreads();                                         // the exchange reads above are done
+#pragma unroll
+  for (int hh = 0; hh < H; ++hh) {
+    const int k1 = j + 8 * hh;
+    if (k1 < P) {
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) x[(k1 * 8 + jj) * XT + l] = r[hh][jj];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k1 = 0; k1 < P; ++k1) u[k1] = x[(k1 * 8 + j) * XT + l];
+  dft_p<R, P, 1>(u);                                       // over k1: u[m] = out(z = j + 8 m)
+#pragma unroll
+  for (int m = 0; m < P; ++m) out[base + (j + 8 * m) * sz] = u[m];
+}
+
+// columns per workgroup of k_zpass_gen (0: not covered)
+static int zpass_gen_xt(const GridGeom& g) {
+  if (g.nz != 48 && g.nz != 96) return 0;
+  if (g.nx % 32 == 0) return 32;
+  if (g.nx % 16 == 0) return 16;
+  return 0;
+}
+
+template <typename R>
+static hipError_t zpass_gen(const GridGeom& g, hipStream_t st, const cx<R>* in, cx<R>* out, int nf, const R* ek,
+                            const cx<R>* twz) {
+  const int xt = zpass_gen_xt(g);
+  const long long NF = (long long)g.nx * g.ny * g.nz;
+  const dim3 grid((unsigned)(g.nx / xt), (unsigned)g.ny, (unsigned)nf);
+  prof_begin(st, KC_FFT_Z);
+  if (g.nz == 48) {
+    if (xt == 32)
+      hipLaunchKernelGGL((k_zpass_gen<R, 48, 32>), grid, dim3(256), 0, st, in, out, g.nx, g.ny, NF, ek, twz);
+    else
+      hipLaunchKernelGGL((k_zpass_gen<R, 48, 16>), grid, dim3(128), 0, st, in, out, g.nx, g.ny, NF, ek, twz);
+  } else {
+    if (xt == 32)
+      hipLaunchKernelGGL((k_zpass_gen<R, 96, 32>), grid, dim3(256), 0, st, in, out, g.nx, g.ny, NF, ek, twz);
+    else
+      hipLaunchKernelGGL((k_zpass_gen<R, 96, 16>), grid, dim3(128), 0, st, in, out, g.nx, g.ny, NF, ek, twz);
+  }
+  prof_end(st, KC_FFT_Z);
+  return hipGetLastError();
+}
+
 // columns per workgroup of k_zpass64 for an nx-wide grid (0: not covered)
 static int zpass64_xt(const GridGeom& g) {
   if (g.nz != 64) return 0;
@@ -887,6 +1039,8 @@ hipError_t apply_t_plus_v(const GridGeom& g, hipStream_t st, const cx<R>* in, cx
     if (e) return e;
     if (zpass64_xt(g) > 0) {
       if ((e = zpass64<R>(g, st, tmp, tmp, nf, ek, twz))) return e;
+    } else if (zpass_gen_xt(g) > 0) {
+      if ((e = zpass_gen<R>(g, st, tmp, tmp, nf, ek, twz))) return e;
     } else if ((e = fft_s_pass<R>(KC_FFT_Z, g.nz, 2, -1, EPI_MULT, st, tmp, v, tmp, v, Lz, twz, ek, vt))) {
       return e;
     }
