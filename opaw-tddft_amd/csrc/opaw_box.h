@@ -32,6 +32,7 @@ struct BoxTables {
   const int* ovoff;          // natom: offset of atom a's overlap values
   const int* oc;             // overlap column -> coefficient index
   const double* ov;          // overlap values O(i, l), row-major per atom
+  const double* ov1;         // the same columns for the p1-p1 overlaps O11(i, l)
   const int2* ublk;          // union point blocks (first point, count) for the LDS kernel
   int nublk;                 // 0: no blocks (global kernel)
   const int* ucptr;          // nublk+1: coefficient list of block b is ucol[ucptr[b] .. ucptr[b+1])
@@ -72,8 +73,26 @@ hipError_t union_rk4_apply(const GridGeom& g, const BoxTables& bt, hipStream_t s
 // P1 coefficients and sn weights of fields stored only on the union of boxes
 hipError_t box_coeffs_compact(const GridGeom& g, const BoxTables& bt, hipStream_t st, const c64* vals, int nf,
                               double dv, double* part, c64* ca, const double* sn, c64* w);
-// cB += O wA: projections of S^-1/2 u from those of u (fixed box overlaps)
-hipError_t box_bfix(const BoxTables& bt, hipStream_t st, int nf, const c64* wA, c64* cB);
+// cB += O wA: projections of S^-1/2 u from those of u (fixed box overlaps);
+// with wC, cB += O (wA + a1 (-i) wC)
+hipError_t box_bfix(const BoxTables& bt, hipStream_t st, int nf, const c64* wA, c64* cB, const c64* wC = nullptr,
+                    double a1 = 0.0);
+// cA += a1 (-i) O11 wC, wA = sn cA (P1 coefficients after an RK4 stage's
+// union add-back, from those before it)
+hipError_t box_afix(const BoxTables& bt, hipStream_t st, int nf, const c64* wC, double a1, const double* sn, c64* cA,
+                    c64* wA);
+// cA = dv P1^T y, cB = dv P^T y and cC = dv P1^T h (h compact, on the union
+// of boxes) with wC = sn cC, in one projection launch and one finish launch;
+// part holds three partial sets
+template <typename R>
+hipError_t box_coeffs_triple(const GridGeom& g, const BoxTables& bt, hipStream_t st, const cx<R>* y, const c64* hU,
+                             int nf, double dv, double* part, c64* cA, c64* cB, c64* cC, const double* sn, c64* wC);
+// RK4 stage add-back and the next stage's compact correction in one pass over
+// the union of boxes: with kc = -i P1 wC, y += a1 kc and acc += a2 kc, and
+// corrA = P1 wA
+template <typename R>
+hipError_t union_rk4_both(const GridGeom& g, const BoxTables& bt, hipStream_t st, int nf, const c64* wC,
+                          const c64* wA, c64* corrC, c64* corrA, cx<R>* y, cx<R>* acc, double a1, double a2);
 // union_corr added straight into the grid fields out (corr is scratch for the
 // fallback path); the per-field order of additions is that of union_apply
 template <typename R>

@@ -26,7 +26,9 @@ constexpr int kPointLanes = kBoxThreads / kFieldsPerWG;   // 16
 
 // SRC: 0 = the grid fields u, 1 = u + cpre (compact correction),
 // 2 = the compact values cpre alone (fields stored only on the union of
-// boxes, cpre[k*nf + f]; u is not read)
+// boxes, cpre[k*nf + f]; u is not read), 4 = by blockIdx.z: z = 0, 1 the
+// grid fields u with tabs0 / tabs1, z = 2 the compact values cpre with tabs0
+// into part2 (three projections in one launch)
 template <typename R, int MSP, int SRC>
 __global__ __launch_bounds__(kBoxThreads) void k_proj(const AtomDesc* __restrict__ atoms,
                                                       const int2* __restrict__ chunks, const int* __restrict__ bpt,
@@ -34,10 +36,15 @@ __global__ __launch_bounds__(kBoxThreads) void k_proj(const AtomDesc* __restrict
                                                       const double* __restrict__ tabs1,
                                                       const cx<R>* __restrict__ u, long long N, int nf,
                                                       const c64* __restrict__ cpre, int /*table sets, = gridDim.z*/,
-                                                      double* __restrict__ part0, double* __restrict__ part1) {
-  // blockIdx.z selects the table set (two projections of the same u in one launch)
-  const double* __restrict__ tabs = blockIdx.z ? tabs1 : tabs0;
-  double* __restrict__ part = blockIdx.z ? part1 : part0;
+                                                      double* __restrict__ part0, double* __restrict__ part1,
+                                                      double* __restrict__ part2) {
+  // blockIdx.z selects the table set (two projections of the same u in one
+  // launch; SRC 4: a third one of the compact values)
+  const double* __restrict__ tabs = blockIdx.z == 1 ? tabs1 : tabs0;
+  double* __restrict__ part = blockIdx.z == 0 ? part0 : (blockIdx.z == 1 ? part1 : part2);
+  // where the values come from (uniform per workgroup)
+  const bool gsrc = SRC == 4 ? blockIdx.z < 2 : SRC < 2;
+  const bool csrc = SRC == 4 ? blockIdx.z == 2 : SRC != 0;
   const int ch = blockIdx.x;
   const int kl = threadIdx.x % kPointLanes;
   const int f = blockIdx.y * kFieldsPerWG + threadIdx.x / kPointLanes;
@@ -53,7 +60,7 @@ __global__ __launch_bounds__(kBoxThreads) void k_proj(const AtomDesc* __restrict
   // compiler issues them together; masked points contribute zero and
   // projectors i >= ms and fields f >= nf are never written.
   const int fc = min(f, nf - 1);
-  const cx<R>* uf = (SRC != 2) ? u + (long long)fc * N : nullptr;
+  const cx<R>* uf = gsrc ? u + (long long)fc * N : nullptr;
   int kk[PPT];
   bool ok[PPT];
 #pragma unroll
@@ -64,7 +71,7 @@ __global__ __launch_bounds__(kBoxThreads) void k_proj(const AtomDesc* __restrict
   }
   int pt[PPT];
 #pragma unroll
-  for (int q = 0; q < PPT; ++q) pt[q] = (SRC != 2) ? bpt[A.boff + kk[q]] : 0;
+  for (int q = 0; q < PPT; ++q) pt[q] = gsrc ? bpt[A.boff + kk[q]] : 0;
   double tv[PPT][MSP];
 #pragma unroll
   for (int q = 0; q < PPT; ++q) {
@@ -73,7 +80,7 @@ __global__ __launch_bounds__(kBoxThreads) void k_proj(const AtomDesc* __restrict
     for (int i = 0; i < MSP; ++i) tv[q][i] = t[(long long)(i < A.ms ? i : 0) * A.npt];
   }
   double vr[PPT], vi[PPT];
-  if constexpr (SRC != 2) {
+  if (gsrc) {
     cx<R> v[PPT];
 #pragma unroll
     for (int q = 0; q < PPT; ++q) v[q] = uf[pt[q]];
@@ -86,7 +93,7 @@ __global__ __launch_bounds__(kBoxThreads) void k_proj(const AtomDesc* __restrict
 #pragma unroll
     for (int q = 0; q < PPT; ++q) vr[q] = vi[q] = 0.0;
   }
-  if constexpr (SRC != 0) {
+  if (csrc) {
     int bq[PPT];
 #pragma unroll
     for (int q = 0; q < PPT; ++q) bq[q] = bk[A.boff + kk[q]];   // every box point is a union point
@@ -139,12 +146,16 @@ __global__ __launch_bounds__(256) void k_proj_finish(const AtomDesc* __restrict_
                                                      const double* __restrict__ part0, int nf, int ctot, double dv,
                                                      c64* __restrict__ ca0, int mode0, const double* __restrict__ sn,
                                                      const double* __restrict__ dij, int nf_per_sys, int mtot,
-                                                     c64* __restrict__ w, const double* __restrict__ part1,
-                                                     c64* __restrict__ ca1) {
-  // blockIdx.z = 1: the second projection of a dual launch (coefficients only)
-  const double* __restrict__ part = blockIdx.z ? part1 : part0;
-  c64* __restrict__ ca = blockIdx.z ? ca1 : ca0;
-  const int mode = blockIdx.z ? 0 : mode0;
+                                                     c64* __restrict__ w0, const double* __restrict__ part1,
+                                                     c64* __restrict__ ca1, const double* __restrict__ part2,
+                                                     c64* __restrict__ ca2, c64* __restrict__ w2) {
+  // blockIdx.z = 1: the second projection of a dual launch (coefficients
+  // only); z = 2: a third one with the sn weights (mode 1) into w2
+  const int z = blockIdx.z;
+  const double* __restrict__ part = z == 0 ? part0 : (z == 1 ? part1 : part2);
+  c64* __restrict__ ca = z == 0 ? ca0 : (z == 1 ? ca1 : ca2);
+  const int mode = z == 0 ? mode0 : (z == 1 ? 0 : 1);
+  c64* __restrict__ w = z == 2 ? w2 : w0;
   const int a = blockIdx.x, f = blockIdx.y;
   const AtomDesc A = atoms[a];
   const int2 cr = achunk[a];                     // (first chunk, chunk count)
@@ -279,18 +290,21 @@ __global__ __launch_bounds__(kBoxThreads) void k_union_apply(const int* __restri
 template <typename R, int MSP>
 static void launch_proj(const GridGeom& g, const BoxTables& bt, hipStream_t st, const cx<R>* u, int nf,
                         const double* tab, const c64* cpre, double* part, const double* tab1 = nullptr,
-                        double* part1 = nullptr) {
+                        double* part1 = nullptr, double* part2 = nullptr) {
   const long long N = (long long)g.nx * g.ny * g.nz;
-  dim3 grid(bt.nchunks, (nf + kFieldsPerWG - 1) / kFieldsPerWG, tab1 ? 2 : 1);
-  if (cpre && !u)
+  dim3 grid(bt.nchunks, (nf + kFieldsPerWG - 1) / kFieldsPerWG, part2 ? 3 : (tab1 ? 2 : 1));
+  if (part2)
+    hipLaunchKernelGGL((k_proj<R, MSP, 4>), grid, dim3(kBoxThreads), 0, st, bt.atoms, bt.chunks, bt.bpt, bt.bk,
+                       tab, tab1, u, N, nf, cpre, bt.U.U, part, part1, part2);
+  else if (cpre && !u)
     hipLaunchKernelGGL((k_proj<R, MSP, 2>), grid, dim3(kBoxThreads), 0, st, bt.atoms, bt.chunks, bt.bpt, bt.bk,
-                       tab, tab1, u, N, nf, cpre, bt.U.U, part, part1);
+                       tab, tab1, u, N, nf, cpre, bt.U.U, part, part1, nullptr);
   else if (cpre)
     hipLaunchKernelGGL((k_proj<R, MSP, 1>), grid, dim3(kBoxThreads), 0, st, bt.atoms, bt.chunks, bt.bpt, bt.bk,
-                       tab, tab1, u, N, nf, cpre, bt.U.U, part, part1);
+                       tab, tab1, u, N, nf, cpre, bt.U.U, part, part1, nullptr);
   else
     hipLaunchKernelGGL((k_proj<R, MSP, 0>), grid, dim3(kBoxThreads), 0, st, bt.atoms, bt.chunks, bt.bpt, bt.bk,
-                       tab, tab1, u, N, nf, cpre, bt.U.U, part, part1);
+                       tab, tab1, u, N, nf, cpre, bt.U.U, part, part1, nullptr);
 }
 
 template <typename R>
@@ -311,10 +325,10 @@ hipError_t box_coeffs(const GridGeom& g, const BoxTables& bt, hipStream_t st, co
   const dim3 fg(bt.natom, nf);
   if (bt.msp == 8)
     hipLaunchKernelGGL(k_proj_finish<8>, fg, dim3(256), 0, st, bt.atoms, bt.achunk, part, nf, bt.ctot, dv, ca, mode,
-                       sn, dij, nf_per_sys, mtot, w, nullptr, nullptr);
+                       sn, dij, nf_per_sys, mtot, w, nullptr, nullptr, nullptr, nullptr, nullptr);
   else
     hipLaunchKernelGGL(k_proj_finish<16>, fg, dim3(256), 0, st, bt.atoms, bt.achunk, part, nf, bt.ctot, dv, ca, mode,
-                       sn, dij, nf_per_sys, mtot, w, nullptr, nullptr);
+                       sn, dij, nf_per_sys, mtot, w, nullptr, nullptr, nullptr, nullptr, nullptr);
   prof_end(st, KC_BOX_ADD);
   return hipGetLastError();
 }
@@ -339,10 +353,10 @@ hipError_t box_coeffs_dual(const GridGeom& g, const BoxTables& bt, hipStream_t s
   const dim3 fg(bt.natom, nf, 2);
   if (bt.msp == 8)
     hipLaunchKernelGGL(k_proj_finish<8>, fg, dim3(256), 0, st, bt.atoms, bt.achunk, part, nf, bt.ctot, dv, cA, 1, sn,
-                       nullptr, nf, 0, wA, part1, cB);
+                       nullptr, nf, 0, wA, part1, cB, nullptr, nullptr, nullptr);
   else
     hipLaunchKernelGGL(k_proj_finish<16>, fg, dim3(256), 0, st, bt.atoms, bt.achunk, part, nf, bt.ctot, dv, cA, 1, sn,
-                       nullptr, nf, 0, wA, part1, cB);
+                       nullptr, nf, 0, wA, part1, cB, nullptr, nullptr, nullptr);
   prof_end(st, KC_BOX_ADD);
   return hipGetLastError();
 }
@@ -354,11 +368,15 @@ hipError_t box_coeffs_dual(const GridGeom& g, const BoxTables& bt, hipStream_t s
 // columns are flattened: column l is coefficient oc[optr[a] + l] and
 // O(i, l) = ov[ovoff[a] + i * L_a + l].  One wave per (atom, field): lanes
 // over columns, a fixed shuffle tree over the lanes (deterministic).
+//
+// With wC (the RK4 stage's outer-S^-1/2 weights) the column weights are
+// wA + a1 (-i) wC: cB of y + a1 (-i) P1 wC from cB of y, as the stage's
+// union add-back changes y by a1 (-i) P1 wC.
 template <int MSP>
 __global__ __launch_bounds__(64) void k_bfix(const AtomDesc* __restrict__ atoms, const int* __restrict__ optr,
                                              const int* __restrict__ ovoff, const int* __restrict__ oc,
                                              const double* __restrict__ ov, const c64* __restrict__ wA, int nf,
-                                             int ctot, c64* __restrict__ cB) {
+                                             int ctot, c64* __restrict__ cB, const c64* __restrict__ wC, double a1) {
   const int a = blockIdx.x, f = blockIdx.y, lane = threadIdx.x;
   const AtomDesc A = atoms[a];
   const int l0 = optr[a], La = optr[a + 1] - l0;
@@ -373,7 +391,11 @@ __global__ __launch_bounds__(64) void k_bfix(const AtomDesc* __restrict__ atoms,
     double o[MSP];
 #pragma unroll
     for (int i = 0; i < MSP; ++i) o[i] = O[(long long)(i < A.ms ? i : 0) * La + l];
-    const c64 w = wA[(long long)c * nf + f];
+    c64 w = wA[(long long)c * nf + f];
+    if (wC) {
+      const c64 v = wC[(long long)c * nf + f];
+      w = mk(w.x + a1 * v.y, w.y - a1 * v.x);
+    }
     const double wr = on ? w.x : 0.0, wi = on ? w.y : 0.0;
 #pragma unroll
     for (int i = 0; i < MSP; ++i) {
@@ -398,18 +420,117 @@ __global__ __launch_bounds__(64) void k_bfix(const AtomDesc* __restrict__ atoms,
   }
 }
 
-hipError_t box_bfix(const BoxTables& bt, hipStream_t st, int nf, const c64* wA, c64* cB) {
+hipError_t box_bfix(const BoxTables& bt, hipStream_t st, int nf, const c64* wA, c64* cB, const c64* wC, double a1) {
   if (bt.natom == 0 || nf == 0) return hipSuccess;
   prof_begin(st, KC_BOX_ADD);
   if (bt.msp == 8)
     hipLaunchKernelGGL(k_bfix<8>, dim3(bt.natom, nf), dim3(64), 0, st, bt.atoms, bt.optr, bt.ovoff, bt.oc, bt.ov, wA,
-                       nf, bt.ctot, cB);
+                       nf, bt.ctot, cB, wC, a1);
   else
     hipLaunchKernelGGL(k_bfix<16>, dim3(bt.natom, nf), dim3(64), 0, st, bt.atoms, bt.optr, bt.ovoff, bt.oc, bt.ov, wA,
-                       nf, bt.ctot, cB);
+                       nf, bt.ctot, cB, wC, a1);
   prof_end(st, KC_BOX_ADD);
   return hipGetLastError();
 }
+
+// cA += a1 (-i) O11 wC, then wA = sn cA, for every (atom a, field f): the
+// P1 coefficients of y + a1 (-i) P1 wC from those of y, with the fixed p1-p1
+// box overlaps O11_ab(i, j) = dv sum_r p1_ai(r) p1_bj(r) (same columns as
+// k_bfix's O).  One wave per (atom, field), fixed shuffle tree.
+template <int MSP>
+__global__ __launch_bounds__(64) void k_afix(const AtomDesc* __restrict__ atoms, const int* __restrict__ optr,
+                                             const int* __restrict__ ovoff, const int* __restrict__ oc,
+                                             const double* __restrict__ ov1, const c64* __restrict__ wC, int nf,
+                                             int ctot, double a1, const double* __restrict__ sn,
+                                             c64* __restrict__ cA, c64* __restrict__ wA) {
+  const int a = blockIdx.x, f = blockIdx.y, lane = threadIdx.x;
+  const AtomDesc A = atoms[a];
+  const int l0 = optr[a], La = optr[a + 1] - l0;
+  const double* O = ov1 + ovoff[a];
+  double xr[MSP], xi[MSP];
+#pragma unroll
+  for (int i = 0; i < MSP; ++i) xr[i] = xi[i] = 0.0;
+  for (int lb = 0; lb < La; lb += 64) {
+    const int l = min(lb + lane, La - 1);
+    const bool on = lb + lane < La;
+    const int c = oc[l0 + l];
+    double o[MSP];
+#pragma unroll
+    for (int i = 0; i < MSP; ++i) o[i] = O[(long long)(i < A.ms ? i : 0) * La + l];
+    const c64 w = wC[(long long)c * nf + f];
+    const double wr = on ? w.x : 0.0, wi = on ? w.y : 0.0;
+#pragma unroll
+    for (int i = 0; i < MSP; ++i) {
+      xr[i] += o[i] * wr;
+      xi[i] += o[i] * wi;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < MSP; ++i) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      xr[i] += __shfl_xor(xr[i], off, 64);
+      xi[i] += __shfl_xor(xi[i], off, 64);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < MSP; ++i) {
+    if (lane == i && i < A.ms) {
+      const int c = A.coff + i;
+      c64& v = cA[(long long)f * ctot + c];
+      v = mk(v.x + a1 * xi[i], v.y - a1 * xr[i]);
+      const double s = sn[c];
+      wA[(long long)c * nf + f] = mk(s * v.x, s * v.y);
+    }
+  }
+}
+
+hipError_t box_afix(const BoxTables& bt, hipStream_t st, int nf, const c64* wC, double a1, const double* sn, c64* cA,
+                    c64* wA) {
+  if (bt.natom == 0 || nf == 0) return hipSuccess;
+  prof_begin(st, KC_BOX_ADD);
+  if (bt.msp == 8)
+    hipLaunchKernelGGL(k_afix<8>, dim3(bt.natom, nf), dim3(64), 0, st, bt.atoms, bt.optr, bt.ovoff, bt.oc, bt.ov1, wC,
+                       nf, bt.ctot, a1, sn, cA, wA);
+  else
+    hipLaunchKernelGGL(k_afix<16>, dim3(bt.natom, nf), dim3(64), 0, st, bt.atoms, bt.optr, bt.ovoff, bt.oc, bt.ov1, wC,
+                       nf, bt.ctot, a1, sn, cA, wA);
+  prof_end(st, KC_BOX_ADD);
+  return hipGetLastError();
+}
+
+// The three projections after an RK4 stage's inverse kinetic pass in one
+// launch: A = P1^T y and B = P^T y of the grid fields y, and C = P1^T h of
+// the compact h (hU), then their coefficients and the weights wC = sn C.
+template <typename R>
+hipError_t box_coeffs_triple(const GridGeom& g, const BoxTables& bt, hipStream_t st, const cx<R>* y, const c64* hU,
+                             int nf, double dv, double* part, c64* cA, c64* cB, c64* cC, const double* sn, c64* wC) {
+  if (bt.natom == 0 || nf == 0) return hipSuccess;
+  double* part1 = part + (size_t)bt.nchunks * nf * bt.msp * 2;
+  double* part2 = part1 + (size_t)bt.nchunks * nf * bt.msp * 2;
+  prof_begin(st, KC_BOX_PROJ);
+  if (bt.msp == 8)
+    launch_proj<R, 8>(g, bt, st, y, nf, bt.p1, hU, part, bt.p, part1, part2);
+  else
+    launch_proj<R, 16>(g, bt, st, y, nf, bt.p1, hU, part, bt.p, part1, part2);
+  prof_end(st, KC_BOX_PROJ);
+  hipError_t e = hipGetLastError();
+  if (e) return e;
+  prof_begin(st, KC_BOX_ADD);
+  const dim3 fg(bt.natom, nf, 3);
+  if (bt.msp == 8)
+    hipLaunchKernelGGL(k_proj_finish<8>, fg, dim3(256), 0, st, bt.atoms, bt.achunk, part, nf, bt.ctot, dv, cA, 0, sn,
+                       nullptr, nf, 0, nullptr, part1, cB, part2, cC, wC);
+  else
+    hipLaunchKernelGGL(k_proj_finish<16>, fg, dim3(256), 0, st, bt.atoms, bt.achunk, part, nf, bt.ctot, dv, cA, 0, sn,
+                       nullptr, nf, 0, nullptr, part1, cB, part2, cC, wC);
+  prof_end(st, KC_BOX_ADD);
+  return hipGetLastError();
+}
+template hipError_t box_coeffs_triple<double>(const GridGeom&, const BoxTables&, hipStream_t, const c64*, const c64*,
+                                              int, double, double*, c64*, c64*, c64*, const double*, c64*);
+template hipError_t box_coeffs_triple<float>(const GridGeom&, const BoxTables&, hipStream_t, const c32*, const c64*,
+                                             int, double, double*, c64*, c64*, c64*, const double*, c64*);
 
 hipError_t box_weights(const BoxTables& bt, hipStream_t st, const c64* ca, int nf, int mode, const double* sn,
                        const double* dij, int nf_per_sys, int mtot, c64* w) {
@@ -444,17 +565,21 @@ __global__ __launch_bounds__(kBoxThreads) void k_union_corr_lds(const int2* __re
                                                                 const c64* __restrict__ w, int nf, int ctot,
                                                                 c64* __restrict__ corr, const int* __restrict__ upt,
                                                                 cx<R>* __restrict__ out, long long N,
-                                                                cx<R>* __restrict__ out2, double a1, double a2) {
+                                                                cx<R>* __restrict__ out2, double a1, double a2,
+                                                                const c64* __restrict__ w2) {
   // MODE 0: corr(k, f) = sum; 1: out_f(upt[k]) += sum; 2 (RK4 stage
   // add-back of the outer S^-1/2, linear in h): with kc = -i sum,
-  // out_f += a1 kc and out2_f += a2 kc at upt[k]
+  // out_f += a1 kc and out2_f += a2 kc at upt[k]; 3: MODE 2 and, from the
+  // second weight set w2 staged beside w, corr(k, f) = its sum
   constexpr bool APPLY = MODE != 0;
+  constexpr int NSET = MODE == 3 ? 2 : 1;
   extern __shared__ __align__(16) double smem[];
   c64* wl = reinterpret_cast<c64*>(smem);                 // [ctot][FG]
-  double* tl = smem + 2 * (long long)ctot * FG;           // pair values
+  c64* wl2 = wl + (long long)ctot * FG;                   // [ctot][FG], MODE 3
+  double* tl = smem + 2 * NSET * (long long)ctot * FG;    // pair values
   int* cl = reinterpret_cast<int*>(tl + kUnionBlockPairs);  // pair coefficients
   int* rl = cl + kUnionBlockPairs;                        // row starts (block-relative)
-  c64* res = reinterpret_cast<c64*>(smem + 2 * (long long)ctot * FG + (kUnionBlockPairs * 12 + (kUnionBlockRows + 1) * 4 + 15) / 16 * 2);
+  c64* res = reinterpret_cast<c64*>(smem + 2 * NSET * (long long)ctot * FG + (kUnionBlockPairs * 12 + (kUnionBlockRows + 1) * 4 + 15) / 16 * 2);
   const int2 blk = ublk[blockIdx.x];                      // (first point, point count)
   const int f0 = blockIdx.y * FG;
   const int q0 = pptr[blk.x];
@@ -476,6 +601,18 @@ __global__ __launch_bounds__(kBoxThreads) void k_union_corr_lds(const int2* __re
     for (int s = 0; s < 8; ++s) {
       const int q = b0 + s * kBoxThreads + (int)threadIdx.x;
       if (q < nw) wl[q] = (f0 + q % FG < nf) ? t[s] : mk(0.0, 0.0);
+    }
+    if constexpr (NSET == 2) {
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        const int q = min(b0 + s * kBoxThreads + (int)threadIdx.x, nw - 1);
+        t[s] = w2[(long long)ucol[c0 + q / FG] * nf + min(f0 + q % FG, nf - 1)];
+      }
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        const int q = b0 + s * kBoxThreads + (int)threadIdx.x;
+        if (q < nw) wl2[q] = (f0 + q % FG < nf) ? t[s] : mk(0.0, 0.0);
+      }
     }
   }
   {
@@ -504,7 +641,7 @@ __global__ __launch_bounds__(kBoxThreads) void k_union_corr_lds(const int2* __re
   const int f = f0 + j;
   for (int r = threadIdx.x / FG; r < blk.y; r += PL) {
     const int a = rl[r], e = rl[r + 1];
-    double xr = 0.0, xi = 0.0;
+    double xr = 0.0, xi = 0.0, yr = 0.0, yi = 0.0;
     int p = a;
     for (; p + 4 <= e; p += 4) {
       int cc[4];
@@ -522,17 +659,33 @@ __global__ __launch_bounds__(kBoxThreads) void k_union_corr_lds(const int2* __re
         xr += tt[s] * wv[s].x;
         xi += tt[s] * wv[s].y;
       }
+      if constexpr (NSET == 2) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) wv[s] = wl2[cc[s] * FG + j];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          yr += tt[s] * wv[s].x;
+          yi += tt[s] * wv[s].y;
+        }
+      }
     }
     for (; p < e; ++p) {
       const c64 wv = wl[cl[p] * FG + j];
       const double t = tl[p];
       xr += t * wv.x;
       xi += t * wv.y;
+      if constexpr (NSET == 2) {
+        const c64 wv2 = wl2[cl[p] * FG + j];
+        yr += t * wv2.x;
+        yi += t * wv2.y;
+      }
     }
     if constexpr (APPLY)
       res[r * FG + j] = mk(xr, xi);
     else if (f < nf)
       corr[(long long)(blk.x + r) * nf + f] = mk(xr, xi);
+    if constexpr (NSET == 2)
+      if (f < nf) corr[(long long)(blk.x + r) * nf + f] = mk(yr, yi);
   }
   if constexpr (APPLY) {
     __syncthreads();
@@ -556,36 +709,38 @@ __global__ __launch_bounds__(kBoxThreads) void k_union_corr_lds(const int2* __re
   }
 }
 
-// LDS of the kernel for coefficient lists of at most ctot rows
-static size_t union_lds_bytes(int ctot, int fg, bool apply) {
-  size_t s = (size_t)ctot * fg * 16 + ((size_t)kUnionBlockPairs * 12 + (kUnionBlockRows + 1) * 4 + 15) / 16 * 16;
+// LDS of the kernel for coefficient lists of at most ctot rows and nset
+// weight sets
+static size_t union_lds_bytes(int ctot, int fg, bool apply, int nset = 1) {
+  size_t s = (size_t)nset * ctot * fg * 16 +
+             ((size_t)kUnionBlockPairs * 12 + (kUnionBlockRows + 1) * 4 + 15) / 16 * 16;
   if (apply) s += (size_t)kUnionBlockRows * fg * 16;
   return s;
 }
 
 // fields per workgroup of the LDS kernel for ctot coefficients, within 64 KiB
 // of LDS (0: use the global kernel)
-static int union_lds_fields(int ctot, bool apply) {
+static int union_lds_fields(int ctot, bool apply, int nset = 1) {
   for (int fg : {16, 8, 4})
-    if (union_lds_bytes(ctot, fg, apply) <= 65536) return fg;
+    if (union_lds_bytes(ctot, fg, apply, nset) <= 65536) return fg;
   return 0;
 }
 
 template <typename R, int MODE>
 static void launch_union_lds(const BoxTables& bt, hipStream_t st, int fg, int nf, const double* pv, const c64* w,
                              c64* corr, cx<R>* out, long long N, cx<R>* out2 = nullptr, double a1 = 0.0,
-                             double a2 = 0.0) {
-  const size_t lds = union_lds_bytes(bt.ucmax, fg, MODE != 0);
+                             double a2 = 0.0, const c64* w2 = nullptr) {
+  const size_t lds = union_lds_bytes(bt.ucmax, fg, MODE != 0, MODE == 3 ? 2 : 1);
   const dim3 grid(bt.nublk, (nf + fg - 1) / fg);
   if (fg == 16)
     hipLaunchKernelGGL((k_union_corr_lds<16, R, MODE>), grid, dim3(kBoxThreads), lds, st, bt.ublk, bt.pptr, bt.pcl,
-                       bt.ucptr, bt.ucol, pv, w, nf, bt.ucmax, corr, bt.U.upt, out, N, out2, a1, a2);
+                       bt.ucptr, bt.ucol, pv, w, nf, bt.ucmax, corr, bt.U.upt, out, N, out2, a1, a2, w2);
   else if (fg == 8)
     hipLaunchKernelGGL((k_union_corr_lds<8, R, MODE>), grid, dim3(kBoxThreads), lds, st, bt.ublk, bt.pptr, bt.pcl,
-                       bt.ucptr, bt.ucol, pv, w, nf, bt.ucmax, corr, bt.U.upt, out, N, out2, a1, a2);
+                       bt.ucptr, bt.ucol, pv, w, nf, bt.ucmax, corr, bt.U.upt, out, N, out2, a1, a2, w2);
   else
     hipLaunchKernelGGL((k_union_corr_lds<4, R, MODE>), grid, dim3(kBoxThreads), lds, st, bt.ublk, bt.pptr, bt.pcl,
-                       bt.ucptr, bt.ucol, pv, w, nf, bt.ucmax, corr, bt.U.upt, out, N, out2, a1, a2);
+                       bt.ucptr, bt.ucol, pv, w, nf, bt.ucmax, corr, bt.U.upt, out, N, out2, a1, a2, w2);
 }
 
 hipError_t union_corr(const BoxTables& bt, hipStream_t st, int nf, int which, const c64* w, c64* corr) {
@@ -660,6 +815,25 @@ hipError_t union_rk4_apply(const GridGeom& g, const BoxTables& bt, hipStream_t s
   return hipGetLastError();
 }
 
+// RK4 stage add-back of wC and corrA = P1 wA in one LDS pass (fallback
+// without a fitting LDS layout: the two separate passes)
+template <typename R>
+hipError_t union_rk4_both(const GridGeom& g, const BoxTables& bt, hipStream_t st, int nf, const c64* wC,
+                          const c64* wA, c64* corrC, c64* corrA, cx<R>* y, cx<R>* acc, double a1, double a2) {
+  if (bt.natom == 0 || nf == 0 || bt.U.U == 0) return hipSuccess;
+  const long long N = (long long)g.nx * g.ny * g.nz;
+  const int fg = bt.nublk > 0 ? union_lds_fields(bt.ucmax, true, 2) : 0;
+  if (fg == 0) {
+    hipError_t e = union_rk4_apply<R>(g, bt, st, nf, wC, corrC, y, acc, a1, a2);
+    if (e) return e;
+    return union_corr(bt, st, nf, 1, wA, corrA);
+  }
+  prof_begin(st, KC_BOX_ADD);
+  launch_union_lds<R, 3>(bt, st, fg, nf, bt.pv1, wC, corrA, y, N, acc, a1, a2, wA);
+  prof_end(st, KC_BOX_ADD);
+  return hipGetLastError();
+}
+
 // coefficients (P1, weights sn) of fields stored only on the union of boxes
 // (vals[k*nf + f])
 hipError_t box_coeffs_compact(const GridGeom& g, const BoxTables& bt, hipStream_t st, const c64* vals, int nf,
@@ -677,10 +851,10 @@ hipError_t box_coeffs_compact(const GridGeom& g, const BoxTables& bt, hipStream_
   const dim3 fg(bt.natom, nf);
   if (bt.msp == 8)
     hipLaunchKernelGGL(k_proj_finish<8>, fg, dim3(256), 0, st, bt.atoms, bt.achunk, part, nf, bt.ctot, dv, ca, 1, sn,
-                       nullptr, nf, 0, w, nullptr, nullptr);
+                       nullptr, nf, 0, w, nullptr, nullptr, nullptr, nullptr, nullptr);
   else
     hipLaunchKernelGGL(k_proj_finish<16>, fg, dim3(256), 0, st, bt.atoms, bt.achunk, part, nf, bt.ctot, dv, ca, 1, sn,
-                       nullptr, nf, 0, w, nullptr, nullptr);
+                       nullptr, nf, 0, w, nullptr, nullptr, nullptr, nullptr, nullptr);
   prof_end(st, KC_BOX_ADD);
   return hipGetLastError();
 }
@@ -712,6 +886,10 @@ template hipError_t union_rk4_apply<double>(const GridGeom&, const BoxTables&, h
                                             c64*, c64*, double, double);
 template hipError_t union_rk4_apply<float>(const GridGeom&, const BoxTables&, hipStream_t, int, const c64*, c64*,
                                            c32*, c32*, double, double);
+template hipError_t union_rk4_both<double>(const GridGeom&, const BoxTables&, hipStream_t, int, const c64*,
+                                            const c64*, c64*, c64*, c64*, c64*, double, double);
+template hipError_t union_rk4_both<float>(const GridGeom&, const BoxTables&, hipStream_t, int, const c64*,
+                                           const c64*, c64*, c64*, c32*, c32*, double, double);
 template hipError_t union_corr_apply<double>(const GridGeom&, const BoxTables&, hipStream_t, int, int, const c64*,
                                              c64*, c64*);
 template hipError_t union_corr_apply<float>(const GridGeom&, const BoxTables&, hipStream_t, int, int, const c64*,

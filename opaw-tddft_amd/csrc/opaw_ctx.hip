@@ -180,6 +180,7 @@ struct opaw_ctx {
   int ucmax = 0;
   int *optr = nullptr, *ovoff = nullptr, *oc = nullptr;   // box overlap tables (k_bfix)
   double* ov = nullptr;
+  double* ov1 = nullptr;   // p1-p1 overlaps on the same columns
   // rebuild: nhat kept on the sphere union (nhs, nsys x S, before the rnel
   // factor); the full nhat field of system s is stale while nhat_lazy[s]
   double* nhs = nullptr;
@@ -192,6 +193,9 @@ struct opaw_ctx {
   int2* sent = nullptr;
   double* wl = nullptr;
   bool prep_valid = false;   // cA/wA/cB hold the coefficients of the resident orbitals
+  // an RK4 stage left cA/wA/cB (B not yet fixed: st2) and corrA of these
+  // fields, for the pipelined rebuild that follows it
+  const void* coef_ready = nullptr;
   double *P = nullptr, *P1 = nullptr;
   double* sn[4] = {nullptr, nullptr, nullptr, nullptr};  // n = -1, -1/2, +1/2, +1
   double* sn_custom = nullptr;
@@ -820,7 +824,7 @@ int opaw_finalize_tables(opaw_ctx* c) {
                   (void*)c->uent, (void*)c->utp, (void*)c->utp1, (void*)c->smap, (void*)c->sptr, (void*)c->sphn,
                   (void*)c->sent, (void*)c->wl, (void*)c->bpt, (void*)c->bk,
                   (void*)c->pptr, (void*)c->pc, (void*)c->pv, (void*)c->pv1, (void*)c->ublk,
-                  (void*)c->uzr, (void*)c->optr, (void*)c->ovoff, (void*)c->oc, (void*)c->ov,
+                  (void*)c->uzr, (void*)c->optr, (void*)c->ovoff, (void*)c->oc, (void*)c->ov, (void*)c->ov1,
                   (void*)c->nhs, (void*)c->rpart, (void*)c->otypes, (void*)c->otab, (void*)c->oint,
                   (void*)c->dh_idx, (void*)c->dh_g, (void*)c->dxc, (void*)c->oscr})
     dfree(c, p);
@@ -1027,7 +1031,8 @@ int opaw_finalize_tables(opaw_ctx* c) {
     std::vector<double> tp, tp1;
     // box overlaps O_ab(i, j) = sum over common points of p_ai p1_bj (times
     // dv below), accumulated in grid order
-    std::vector<std::vector<double>> Oab((size_t)natom * natom);
+    // and O11_ab(i, j) of p1_ai p1_bj on the same atom pairs
+    std::vector<std::vector<double>> Oab((size_t)natom * natom), Oab1((size_t)natom * natom);
     for (int k : order) {
       const int pt = upt[k];
       for (auto& ea : at[k])
@@ -1035,10 +1040,17 @@ int opaw_finalize_tables(opaw_ctx* c) {
           const AtomDesc& Aa = c->hatoms[ea.first];
           const AtomDesc& Ab = c->hatoms[eb.first];
           std::vector<double>& O = Oab[(size_t)ea.first * natom + eb.first];
+          std::vector<double>& O1 = Oab1[(size_t)ea.first * natom + eb.first];
           if (O.empty()) O.assign((size_t)Aa.ms * Ab.ms, 0.0);
+          if (O1.empty()) O1.assign((size_t)Aa.ms * Ab.ms, 0.0);
           for (int i = 0; i < Aa.ms; ++i) {
             const double pa = c->hp[ea.first][(size_t)i * Aa.npt + ea.second];
-            for (int j = 0; j < Ab.ms; ++j) O[(size_t)i * Ab.ms + j] += pa * c->hp1[eb.first][(size_t)j * Ab.npt + eb.second];
+            const double qa = c->hp1[ea.first][(size_t)i * Aa.npt + ea.second];
+            for (int j = 0; j < Ab.ms; ++j) {
+              const double qb = c->hp1[eb.first][(size_t)j * Ab.npt + eb.second];
+              O[(size_t)i * Ab.ms + j] += pa * qb;
+              O1[(size_t)i * Ab.ms + j] += qa * qb;
+            }
           }
         }
       umap[pt] = (int)upt2.size();
@@ -1057,7 +1069,7 @@ int opaw_finalize_tables(opaw_ctx* c) {
     {
       const double dv = c->dx * c->dy * c->dz;
       std::vector<int> optr(1, 0), ovoff, oc;
-      std::vector<double> ov;
+      std::vector<double> ov, ov1;
       for (int a = 0; a < natom; ++a) {
         const AtomDesc& Aa = c->hatoms[a];
         std::vector<std::pair<int, int>> cols;   // (b, j)
@@ -1072,12 +1084,14 @@ int opaw_finalize_tables(opaw_ctx* c) {
           for (int l = 0; l < La; ++l) {
             const int b2 = cols[l].first, j = cols[l].second;
             ov.push_back(dv * Oab[(size_t)a * natom + b2][(size_t)i * c->hatoms[b2].ms + j]);
+            ov1.push_back(dv * Oab1[(size_t)a * natom + b2][(size_t)i * c->hatoms[b2].ms + j]);
           }
       }
       RCHK(upload(c, &c->optr, optr));
       RCHK(upload(c, &c->ovoff, ovoff));
       RCHK(upload(c, &c->oc, oc));
       RCHK(upload(c, &c->ov, ov));
+      RCHK(upload(c, &c->ov1, ov1));
     }
     // the same entries flattened to (coefficient, value) pairs per union point
     {
@@ -1399,6 +1413,7 @@ static BoxTables box_tables(opaw_ctx* c) {
   b.ovoff = c->ovoff;
   b.oc = c->oc;
   b.ov = c->ov;
+  b.ov1 = c->ov1;
   b.U = UnionTabs{c->umap, c->uptr, c->uent, c->upt, c->utp, c->utp1, c->U};
   return b;
 }
@@ -1431,7 +1446,7 @@ static int ensure_coef(opaw_ctx* c, int nf) {
   if (nf <= c->coef_cap) return OPAW_OK;
   const size_t cb = (size_t)nf * std::max(1, c->ctot) * sizeof(c64);
   for (DevBuf* b : {&c->cA, &c->wA, &c->cB, &c->wB, &c->cC, &c->wC}) RCHK(ensure(c, *b, cb));
-  RCHK(ensure(c, c->ppart, (size_t)2 * std::max(1, c->nchunks) * nf * c->msp * 2 * sizeof(double)));
+  RCHK(ensure(c, c->ppart, (size_t)3 * std::max(1, c->nchunks) * nf * c->msp * 2 * sizeof(double)));
   const size_t ub = (size_t)nf * std::max(1, c->U) * sizeof(c64);
   for (DevBuf* b : {&c->corrA, &c->corrNL, &c->corrC, &c->hU}) RCHK(ensure(c, *b, ub));
   c->coef_cap = nf;
@@ -1596,6 +1611,14 @@ static int flow_hs(opaw_ctx* c, int sys0, const cx<R>* in, cx<R>* h, int nf, int
 // the A/B coefficients of yin.  The kinetic pass writes h only on the union
 // of boxes (hU); its C coefficients give the outer S^-1/2 add-back, which is
 // linear in h and is applied to y and acc afterwards.
+//
+// The next rebuild's coefficients of y come from the same launch: with y0 the
+// pass's output and y = y0 + c (-i) P1 wC the stage's,
+//   P1^T y dv = P1^T y0 dv + c (-i) O11 wC        (box_afix)
+//   P^T S^-1/2 y dv = P^T y0 dv + O (wA + c (-i) wC)   (box_bfix)
+// so A, B (of y0) and C (of h) are one projection, and the add-back of wC
+// and the compact correction corrA = P1 wA one union pass.  The rebuild
+// that follows consumes them (coef_ready) and starts at the density.
 template <typename R>
 static int flow_hs_rk4(opaw_ctx* c, const cx<R>* yin, cx<R>* y, cx<R>* acc, const cx<R>* p, int nf, bool first,
                        double cst) {
@@ -1607,10 +1630,23 @@ static int flow_hs_rk4(opaw_ctx* c, const cx<R>* yin, cx<R>* y, cx<R>* acc, cons
   rk.first = first ? 1 : 0;
   RCHK(kinetic_stage<R>(c, 0, yin, y, nf, c->nb, &rk));
   const BoxTables bt = box_tables(c);
+  const double a2 = first ? 1.0 : 2.0;
+  if (c->pot_ready && c->vk_ready) {
+    const c64* wC = (const c64*)c->wC.p;
+    HCHK(box_coeffs_triple<R>(c->g, bt, c->st, y, (const c64*)c->hU.p, nf, dvol(c), (double*)c->ppart.p,
+                              (c64*)c->cA.p, (c64*)c->cB.p, (c64*)c->cC.p, c->sn[1], (c64*)c->wC.p));
+    HCHK(box_afix(bt, c->st, nf, wC, cst, c->sn[1], (c64*)c->cA.p, (c64*)c->wA.p));
+    RCHK(fork_side(c));
+    HCHK(box_bfix(bt, c->st2, nf, (const c64*)c->wA.p, (c64*)c->cB.p, wC, cst));
+    HCHK(union_rk4_both<R>(c->g, bt, c->st, nf, wC, (const c64*)c->wA.p, (c64*)c->corrC.p, (c64*)c->corrA.p, y,
+                           acc, cst, a2));
+    HCHK(hipEventRecord(c->ev_corr, c->st));
+    c->coef_ready = y;
+    return OPAW_OK;
+  }
   HCHK(box_coeffs_compact(c->g, bt, c->st, (const c64*)c->hU.p, nf, dvol(c), (double*)c->ppart.p, (c64*)c->cC.p,
                           c->sn[1], (c64*)c->wC.p));
-  HCHK(union_rk4_apply<R>(c->g, bt, c->st, nf, (const c64*)c->wC.p, (c64*)c->corrC.p, y, acc, cst,
-                          first ? 1.0 : 2.0));
+  HCHK(union_rk4_apply<R>(c->g, bt, c->st, nf, (const c64*)c->wC.p, (c64*)c->corrC.p, y, acc, cst, a2));
   return OPAW_OK;
 }
 
@@ -1644,7 +1680,13 @@ static int flow_rebuild(opaw_ctx* c, int s0, int ns, const cx<R>* phi, bool pipe
   const long long N = c->N;
   const double dv = dvol(c);
   hipStream_t sd, sr;   // streams of the density pass and of rho_ij + nhat
-  if (pipelined) {
+  const bool ready = pipelined && c->coef_ready == (const void*)phi;
+  c->coef_ready = nullptr;
+  if (ready) {
+    // the RK4 stage before formed A, B (fixing on st2) and corrA (ev_corr)
+    sd = c->st;
+    sr = c->st2;
+  } else if (pipelined) {
     RCHK(ensure_coef(c, nf));
     const BoxTables bt = box_tables(c);
     HCHK(box_coeffs_dual<R>(c->g, bt, c->st, phi, nf, dv, (double*)c->ppart.p, (c64*)c->cA.p, c->sn[1],
