@@ -72,22 +72,12 @@ __global__ __launch_bounds__(kBoxThreads) void k_proj(const AtomDesc* __restrict
   int pt[PPT];
 #pragma unroll
   for (int q = 0; q < PPT; ++q) pt[q] = gsrc ? bpt[A.boff + kk[q]] : 0;
-  // the chunk's table rows, shared by the workgroup's 16 fields, staged in
-  // LDS once (coalesced: 64 consecutive points per projector row; rows
-  // i >= ms and points past the atom's box are clamped copies, never used)
-  __shared__ double tl[MSP * kProjChunk];
-  {
-    constexpr int TPT = MSP * kProjChunk / kBoxThreads;
-    const double* t0 = tabs + A.tab;
-    double t[TPT];
+  double tv[PPT][MSP];
 #pragma unroll
-    for (int u = 0; u < TPT; ++u) {
-      const int e = u * kBoxThreads + (int)threadIdx.x;
-      const int i = min(e / kProjChunk, A.ms - 1), k = min(cd.y + e % kProjChunk, A.npt - 1);
-      t[u] = t0[(long long)i * A.npt + k];
-    }
+  for (int q = 0; q < PPT; ++q) {
+    const double* t = tabs + A.tab + kk[q];
 #pragma unroll
-    for (int u = 0; u < TPT; ++u) tl[u * kBoxThreads + threadIdx.x] = t[u];
+    for (int i = 0; i < MSP; ++i) tv[q][i] = t[(long long)(i < A.ms ? i : 0) * A.npt];
   }
   double vr[PPT], vi[PPT];
   if (gsrc) {
@@ -118,14 +108,12 @@ __global__ __launch_bounds__(kBoxThreads) void k_proj(const AtomDesc* __restrict
       }
     }
   }
-  __syncthreads();
 #pragma unroll
   for (int q = 0; q < PPT; ++q) {
 #pragma unroll
     for (int i = 0; i < MSP; ++i) {
-      const double t = tl[i * kProjChunk + kl + q * kPointLanes];
-      accr[i] += vr[q] * t;
-      acci[i] += vi[q] * t;
+      accr[i] += vr[q] * tv[q][i];
+      acci[i] += vi[q] * tv[q][i];
     }
   }
 #pragma unroll
