@@ -394,14 +394,28 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fft_plane(const cx<R>* in, cx
     }
     if constexpr (EPI == EPI_HPSI) {
       if constexpr (!EARLY) {
+        // the operands of a batch are requested together: out may alias
+        // uin, so the compiler cannot move a load above an earlier store,
+        // but every element is read and written by one thread, so reading a
+        // batch before writing it is safe in place
+        constexpr int HB = PER <= 24 ? PER : 8;
+        for (int q0 = 0; q0 < PER; q0 += HB) {
+          cx<R> ub[HB];
+          R vb[HB];
 #pragma unroll
-        for (int q = 0; q < PER; ++q) {
-          uu[0] = us[lidx(q)];
-          vv[0] = vk[lidx(q)];
-          if (EXACT || slot(q) < NP) {
-            const cx<R> t = tile[tix(slot(q))];
-            const double w = (double)vv[0];
-            dst[slot(q)] = mk<R>((R)((double)t.x + w * (double)uu[0].x), (R)((double)t.y + w * (double)uu[0].y));
+          for (int s = 0; s < HB; ++s) {
+            const int q = min(q0 + s, PER - 1);
+            ub[s] = us[lidx(q)];
+            vb[s] = vk[lidx(q)];
+          }
+#pragma unroll
+          for (int s = 0; s < HB; ++s) {
+            const int q = q0 + s;
+            if (q < PER && (EXACT || slot(q) < NP)) {
+              const cx<R> t = tile[tix(slot(q))];
+              const double w = (double)vb[s];
+              dst[slot(q)] = mk<R>((R)((double)t.x + w * (double)ub[s].x), (R)((double)t.y + w * (double)ub[s].y));
+            }
           }
         }
       } else {
